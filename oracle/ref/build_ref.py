@@ -1,0 +1,170 @@
+#!/usr/bin/env python3
+"""Build recipe for the reference GP-MOOD Fortran, used as a parity checker.
+
+Test infrastructure only: nothing in the product imports or runs this.
+
+The reference solver is compiled from its own sources where they lie under
+``/root/reference/gp-mood-code`` (nothing is copied into this repository).
+Outputs go only to ``oracle/_ref/<config>/`` (git-ignored).  What differs from
+the reference's own Makefile (gp-mood-code/Makefile:8-58):
+
+* compiler: ``amdflang`` (gfortran is not in this image) with the Makefile's
+  ``-fdefault-real-8`` plus ``-O3 -ffp-contract=off``; ``-mcmodel=medium`` for
+  grids of 2048 and more (the solution arrays are static module arrays);
+* ``parameters.f90``: the reference takes its compile-time configuration from a
+  ``module parameters`` copied out of ``problems/*.f90`` (README.md:10-12).  This
+  script writes that module from the config table below, with the schema of
+  ``problems/2DRP3_GPMOOD.f90:1-48``;
+* ``main.f90``, ``output.f90`` and ``reader.f90`` (HDF5 I/O) are not built; the
+  harness driver ``oracle/ref/ref_driver.f90`` replaces ``main.f90``;
+* two compile errors under flang are resolved with preprocessor definitions,
+  leaving the sources untouched:
+  - ``mod_subroutine_MOOD.f90:197`` declares locals ``pR``/``pL`` that clash,
+    case-insensitively, with the kind parameter ``PR``.  ``-DpR=pRgt -DpL=pLft``
+    renames them (case-sensitive macro, so ``PR`` itself is untouched);
+  - ``mod_NN.f90:41,44`` index ``values_NN`` with the NN input-size parameter
+    ``L`` (the reference's undeclared loop variable ``l`` resolves to it), a
+    compile-time bounds error when ``L > lf+2``.  The NN arrays are never used
+    with ``method = GP_MOOD``, so ``parameters.f90`` sets ``L = 32``;
+  - ``global_variables.f90:15`` evaluates ``szs_sphere(Mord+2)``, out of bounds
+    for ``Mord=7`` (the value only feeds the reference NN stencil size).  The
+    macro ``szs_sphere(x)=szs_sphere(min(x,7))`` clamps the index; flang's
+    preprocessor only expands this function-like macro in ``-E`` mode, so this
+    one file is preprocessed to a temporary directory outside the repository,
+    compiled from there, and the temporary file is deleted.
+
+Usage: python oracle/ref/build_ref.py [config ...]   (default: all configs)
+"""
+import os
+import shutil
+import subprocess
+import sys
+import tempfile
+from concurrent.futures import ThreadPoolExecutor
+
+REF_SRC = "/root/reference/gp-mood-code"
+HERE = os.path.dirname(os.path.abspath(__file__))
+OUT_ROOT = os.path.join(os.path.dirname(HERE), "_ref")
+FC = shutil.which("amdflang") or "/opt/rocm/bin/amdflang"
+
+# Module build order of gp-mood-code/Makefile:27-58 without output/reader/main.
+MODULES = [
+    "constants", "parameters", "linalg", "global_variables", "GP_init",
+    "mesh_init", "physics", "IC", "mod_append_NN_dataset", "mod_NN",
+    "mod_HLLC", "mod_HLL", "mod_LLF", "mod_unsplit", "mod_GP_MOOD",
+    "mod_NN_GP_MOOD_CC", "mod_GP_SE", "mod_FOG", "mod_pol_MOOD",
+    "mod_Unlim_Pol", "mod_Reconstruction", "BC", "mod_subroutine_NN_MOOD",
+    "mod_subroutine_MOOD", "mod_FE", "mod_SSP_RK2", "mod_SSP_RK3",
+    "mod_SSP_RK4", "time_stepper",
+]
+
+BC_NAMES = {"neumann": "Neumann", "periodic": "Periodic", "reflective": "Reflective"}
+
+# name: (N, Mord, ngp, bc, DMP, U2, U2_tol)
+CONFIGS = {
+    "r1_n32_neu": (32, 3, 2, "neumann", True, True, True),
+    "r2_n32_neu": (32, 5, 3, "neumann", True, True, True),
+    "r2_n64_neu": (64, 5, 3, "neumann", True, True, True),
+    "r2_n64_per": (64, 5, 3, "periodic", True, True, True),
+    "r2_n64_ref": (64, 5, 3, "reflective", True, True, True),
+    "r2_n64_nou2": (64, 5, 3, "neumann", True, False, False),
+    "r3_n32_ref": (32, 7, 4, "reflective", True, True, True),
+    "r3_n64_ref": (64, 7, 4, "reflective", True, True, True),
+    "r3_n64_per": (64, 7, 4, "periodic", True, True, True),
+    "r2_n256_neu": (256, 5, 3, "neumann", True, True, True),
+    "r3_n2048_ref": (2048, 7, 4, "reflective", True, True, True),
+}
+
+PARAMS_TEMPLATE = """module parameters
+   use constants
+   implicit none
+   real(PR), parameter :: CFL  =  0.8
+   integer , parameter :: integrator    = SSP_RK3
+   logical , parameter :: restart = .false.
+   character(100) :: restart_filename='.'
+   integer, parameter :: method   = GP_MOOD
+   real(16), parameter :: ell_o_dx = 12.0
+   integer, parameter :: Mord= {mord}
+   integer, parameter :: ngp = {ngp}
+   logical , parameter :: DMP        = {dmp}
+   logical , parameter :: U2         = {u2}
+   logical , parameter :: U2_tol     = {u2tol}
+   logical :: write_NN_dataset=.false.
+   integer, parameter :: dataset_size = 1
+   integer, parameter :: L=32
+   integer, parameter :: length=60
+   character(100) :: NN_filename=''
+   integer, parameter :: numFlux = HLLC
+   integer, parameter :: IO_freqStep = -100
+   real(PR), parameter:: IO_freqTime = -1.e-3
+   integer,  parameter :: lf = {n}
+   integer,  parameter :: nf = {n}
+   integer, parameter  :: problem = RP_2D_3
+   real(PR), parameter :: tmax = 1.e10
+   integer, parameter  :: nmax = 100000
+   real(16), parameter :: Lx_16 = 1.
+   real(16), parameter :: Ly_16 = 1.
+   integer, parameter  :: BC_type = {bc}
+end module
+"""
+
+
+def fbool(b):
+    return ".true." if b else ".false."
+
+
+def build(name):
+    n, mord, ngp, bc, dmp, u2, u2tol = CONFIGS[name]
+    out = os.path.join(OUT_ROOT, name)
+    os.makedirs(out, exist_ok=True)
+    params = os.path.join(out, "parameters.f90")
+    with open(params, "w") as f:
+        f.write(PARAMS_TEMPLATE.format(mord=mord, ngp=ngp, dmp=fbool(dmp), u2=fbool(u2),
+                                       u2tol=fbool(u2tol), n=n, bc=BC_NAMES[bc]))
+    flags = ["-fdefault-real-8", "-O3", "-ffp-contract=off", "-J", out, "-I", out]
+    if n >= 1024:
+        flags.append("-mcmodel=medium")
+    objs = []
+    tmpdir = tempfile.mkdtemp(prefix="gpm_ref_pp_")
+    try:
+        for mod in MODULES:
+            src = params if mod == "parameters" else os.path.join(REF_SRC, mod + ".f90")
+            obj = os.path.join(out, mod + ".o")
+            extra = []
+            if mod == "mod_subroutine_MOOD":
+                extra = ["-cpp", "-DpR=pRgt", "-DpL=pLft"]
+            if mod == "global_variables" and mord == 7:
+                pp = os.path.join(tmpdir, "global_variables_pp.f90")
+                with open(pp, "w") as f:
+                    subprocess.run([FC, "-cpp", "-E", "-P", "-Dszs_sphere(x)=szs_sphere(min(x,7))", src],
+                                   check=True, stdout=f)
+                src = pp
+            subprocess.run([FC] + flags + extra + ["-c", src, "-o", obj], check=True, cwd=out)
+            objs.append(obj)
+        drv = os.path.join(HERE, "ref_driver.f90")
+        dobj = os.path.join(out, "ref_driver.o")
+        subprocess.run([FC] + flags + ["-c", drv, "-o", dobj], check=True, cwd=out)
+        exe = os.path.join(out, "gpm_ref.exe")
+        link = [FC] + [f for f in flags if f not in ("-J", "-I", out)] + objs + [dobj, "-o", exe]
+        subprocess.run(link, check=True, cwd=out)
+    finally:
+        shutil.rmtree(tmpdir, ignore_errors=True)
+    for f in os.listdir(out):
+        if f.endswith(".o") or f.endswith(".mod"):
+            os.remove(os.path.join(out, f))
+    return exe
+
+
+def main(argv):
+    if not os.path.isdir(REF_SRC):
+        print("reference sources not present; nothing to build", file=sys.stderr)
+        return 0
+    names = argv or list(CONFIGS)
+    with ThreadPoolExecutor(max_workers=min(8, len(names))) as ex:
+        for exe in ex.map(build, names):
+            print("built", exe)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main(sys.argv[1:]))
