@@ -1,0 +1,170 @@
+"""gpmood_amd: MI355X-native GP-MOOD 2D Euler time step.
+
+Python face of libgpmood.so.  The names mirror the reference's interface for
+this path (slug-cfd/gp-mood, gp-mood-code/): ``time_stepping`` (the drop-in of
+time_stepper.f90:12-23), ``setdt`` (physics.f90:78-129), the compile-time
+``module parameters`` as :class:`Config`, and the state array
+``U(4, lb:le, nb:ne)`` as a numpy array of shape ``(nf+8, lf+8, 4)``.
+
+The numerics run only in the HIP library; when it is missing or no GPU is
+present the calls raise.
+"""
+import ctypes
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import capi, ic  # noqa: F401
+from .capi import gp_tables  # noqa: F401
+
+NGC = 4
+
+
+@dataclass
+class Config:
+    """Runtime form of the reference's ``module parameters``
+    (problems/2DRP3_GPMOOD.f90:8-46)."""
+
+    lf: int
+    nf: int = None
+    Mord: int = 5
+    ngp: int = 3
+    BC_type: str = "neumann"
+    CFL: float = 0.8
+    DMP: bool = True
+    U2: bool = True
+    U2_tol: bool = True
+    cascade: str = "reference"
+    Lx: float = 1.0
+    Ly: float = 1.0
+    ell_o_dx: float = 12.0
+    method: int = capi.GP_MOOD
+
+    def to_c(self):
+        nf = self.lf if self.nf is None else self.nf
+        return capi.GpmConfig(
+            lf=self.lf, nf=nf, mord=self.Mord, ngp=self.ngp, method=self.method, integrator=capi.SSP_RK3,
+            num_flux=capi.HLLC, bc_type=capi.BC_IDS[self.BC_type], dmp=int(self.DMP), u2=int(self.U2),
+            u2_tol=int(self.U2_tol), cascade=capi.CASCADE_IDS[self.cascade], cfl=self.CFL, lx=self.Lx,
+            ly=self.Ly, ell_o_dx=self.ell_o_dx)
+
+
+class GpmError(RuntimeError):
+    pass
+
+
+class Solver:
+    """One solver context on the current HIP device."""
+
+    def __init__(self, config, tables=None):
+        self.config = config
+        self.lib = capi.load()
+        self.n = config.lf
+        self._cfg = config.to_c()
+        h = ctypes.c_void_p()
+        rc = self.lib.gpm_create(ctypes.byref(self._cfg), ctypes.byref(h))
+        if rc != 0:
+            raise GpmError(f"gpm_create failed ({rc})")
+        self._h = h
+        if tables is not None:
+            self.set_gp_tables(tables)
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self.lib.gpm_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+    def _check(self, rc, what):
+        if rc != 0:
+            msg = self.lib.gpm_last_error(self._h)
+            raise GpmError(f"{what} failed ({rc}): {msg.decode() if msg else ''}")
+
+    # -- tables -----------------------------------------------------------
+    def set_gp_tables(self, t):
+        arr = {k: np.ascontiguousarray(t[k], dtype=np.int32 if k == "ixiy" else np.float64) for k in
+               ("zT", "d2x", "d2y", "gw", "ixiy")}
+        self._check(self.lib.gpm_set_gp_tables(self._h, *(capi.ptr(arr[k]) for k in ("zT", "d2x", "d2y", "gw", "ixiy"))),
+                    "gpm_set_gp_tables")
+
+    # -- state ------------------------------------------------------------
+    def full_shape(self):
+        return (self.n + 2 * NGC, self.n + 2 * NGC, 4)
+
+    def set_state(self, U):
+        """U: full array (nf+8, lf+8, 4) or interior (nf, lf, 4), float64."""
+        U = np.asarray(U, dtype=np.float64)
+        if U.shape == (self.n, self.n, 4):
+            full = np.zeros(self.full_shape())
+            full[NGC:-NGC, NGC:-NGC] = U
+            U = full
+        U = np.ascontiguousarray(U)
+        if U.shape != self.full_shape():
+            raise ValueError(f"state shape {U.shape} != {self.full_shape()}")
+        self._check(self.lib.gpm_set_state(self._h, capi.ptr(U)), "gpm_set_state")
+
+    def get_state(self, order=False):
+        U = np.zeros(self.full_shape())
+        g = np.zeros(self.full_shape()[:2], dtype=np.int32) if order else None
+        self._check(self.lib.gpm_get_state(self._h, capi.ptr(U), capi.ptr(g)), "gpm_get_state")
+        return (U, g) if order else U
+
+    def set_state_device(self, u_dev_ptr):
+        self._check(self.lib.gpm_set_state_device(self._h, ctypes.c_void_p(u_dev_ptr)), "gpm_set_state_device")
+
+    def get_state_device(self, u_dev_ptr, order_dev_ptr=None):
+        self._check(self.lib.gpm_get_state_device(self._h, ctypes.c_void_p(u_dev_ptr) if u_dev_ptr else None,
+                                                  ctypes.c_void_p(order_dev_ptr) if order_dev_ptr else None),
+                    "gpm_get_state_device")
+
+    # -- the reference's step interface ----------------------------------
+    def setdt(self, dtfinal=1e30):
+        """Setdt (physics.f90:78-129) on the resident state."""
+        dt = ctypes.c_double()
+        self._check(self.lib.gpm_compute_dt(self._h, dtfinal, ctypes.byref(dt)), "gpm_compute_dt")
+        return dt.value
+
+    def time_stepping(self, Uin, t, dt):
+        """Drop-in of time_stepping(Uin, Uout) (time_stepper.f90:12-23).
+
+        Uin (full array) has its ghost cells refilled in place, as the
+        reference's Boundary_C(Uin) does.  Returns (Uout, CellGPO, stats)."""
+        Uin = np.ascontiguousarray(Uin, dtype=np.float64)
+        Uout = np.zeros(self.full_shape())
+        g = np.zeros(self.full_shape()[:2], dtype=np.int32)
+        st = capi.GpmStats()
+        self._check(self.lib.gpm_time_stepping(self._h, capi.ptr(Uin), capi.ptr(Uout), t, dt, capi.ptr(g),
+                                               ctypes.byref(st)), "gpm_time_stepping")
+        return Uout, g, st.as_dict()
+
+    def run(self, nsteps, tmax=1e10, return_dts=False):
+        """Device-resident driver loop of main.f90:145-162 for nsteps steps."""
+        dts = np.zeros(nsteps) if return_dts else None
+        self._check(self.lib.gpm_run(self._h, nsteps, tmax, capi.ptr(dts)), "gpm_run")
+        return dts
+
+    def set_time(self, t):
+        self._check(self.lib.gpm_set_time(self._h, t), "gpm_set_time")
+
+    def stats(self):
+        st = capi.GpmStats()
+        self._check(self.lib.gpm_get_stats(self._h, ctypes.byref(st)), "gpm_get_stats")
+        return st.as_dict()
+
+    def reset_stats(self):
+        self._check(self.lib.gpm_reset_stats(self._h), "gpm_reset_stats")
+
+    def set_timing(self, on=True):
+        self._check(self.lib.gpm_set_timing(self._h, int(on)), "gpm_set_timing")
+
+    def kernel_time(self):
+        a, b = ctypes.c_double(), ctypes.c_double()
+        na, nb = ctypes.c_int64(), ctypes.c_int64()
+        self._check(self.lib.gpm_kernel_time(self._h, ctypes.byref(a), ctypes.byref(na), ctypes.byref(b),
+                                             ctypes.byref(nb)), "gpm_kernel_time")
+        return {"stage_ms": a.value, "stage_launches": na.value, "mood_ms": b.value, "mood_launches": nb.value}
+
+    def stream(self):
+        return self.lib.gpm_stream(self._h)

@@ -1,0 +1,121 @@
+"""ctypes binding of libgpmood.so (the C ABI in include/gpmood.h).
+
+torch is imported before the library is loaded so that the process holds one
+HIP runtime: the library then binds to the runtime torch already loaded, and
+device pointers of torch tensors are valid inside the library's kernels.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+try:  # one HIP runtime per process (see module docstring)
+    import torch  # noqa: F401
+except Exception:  # pragma: no cover - torch is part of the image
+    torch = None
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libgpmood.so")
+
+GP_MOOD = 2
+NN_GP_MOOD = 203
+SSP_RK3 = 10
+HLLC = 100
+NEUMANN, PERIODIC, REFLECTIVE = 17, 18, 20
+BC_IDS = {"neumann": NEUMANN, "periodic": PERIODIC, "reflective": REFLECTIVE}
+CASCADE_IDS = {"reference": 0, "full": 1}
+
+# Every symbol include/gpmood.h declares.
+EXPORTS = [
+    "gpm_create", "gpm_destroy", "gpm_last_error", "gpm_set_gp_tables", "gpm_gp_tables",
+    "gpm_set_state", "gpm_get_state", "gpm_set_state_device", "gpm_get_state_device",
+    "gpm_compute_dt", "gpm_time_stepping", "gpm_run", "gpm_get_stats", "gpm_reset_stats",
+    "gpm_set_time", "gpm_stream", "gpm_set_timing", "gpm_kernel_time",
+]
+
+
+class GpmConfig(ctypes.Structure):
+    _fields_ = [
+        ("lf", ctypes.c_int32), ("nf", ctypes.c_int32), ("mord", ctypes.c_int32), ("ngp", ctypes.c_int32),
+        ("method", ctypes.c_int32), ("integrator", ctypes.c_int32), ("num_flux", ctypes.c_int32),
+        ("bc_type", ctypes.c_int32), ("dmp", ctypes.c_int32), ("u2", ctypes.c_int32), ("u2_tol", ctypes.c_int32),
+        ("cascade", ctypes.c_int32), ("cfl", ctypes.c_double), ("lx", ctypes.c_double), ("ly", ctypes.c_double),
+        ("ell_o_dx", ctypes.c_double),
+    ]
+
+
+class GpmStats(ctypes.Structure):
+    _fields_ = [
+        ("steps", ctypes.c_int64), ("count_detected_cell", ctypes.c_int64), ("detected_total", ctypes.c_int64),
+        ("passes", ctypes.c_int32 * 3), ("max_passes", ctypes.c_int32), ("second_pass_cells", ctypes.c_int64),
+        ("a_priori_lowered", ctypes.c_int64), ("t", ctypes.c_double), ("dt", ctypes.c_double),
+    ]
+
+    def as_dict(self):
+        return {
+            "steps": self.steps, "count_detected_cell": self.count_detected_cell,
+            "detected_total": self.detected_total, "passes": list(self.passes), "max_passes": self.max_passes,
+            "second_pass_cells": self.second_pass_cells, "a_priori_lowered": self.a_priori_lowered,
+            "t": self.t, "dt": self.dt,
+        }
+
+
+_lib = None
+
+
+def load():
+    """Load libgpmood.so; raises if it has not been built (there is no fallback)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(f"{LIB_PATH} is missing: build it with `python gp-mood_amd/build.py` "
+                           "(or __graft_entry__.build())")
+    L = ctypes.CDLL(LIB_PATH)
+    P, I, D = ctypes.c_void_p, ctypes.c_int32, ctypes.c_double
+    sig = {
+        "gpm_create": ([ctypes.POINTER(GpmConfig), ctypes.POINTER(P)], I),
+        "gpm_destroy": ([P], I),
+        "gpm_last_error": ([P], ctypes.c_char_p),
+        "gpm_set_gp_tables": ([P, P, P, P, P, P], I),
+        "gpm_gp_tables": ([I, I, D, D, D, I, I, P, P, P, P, P], I),
+        "gpm_set_state": ([P, P], I),
+        "gpm_get_state": ([P, P, P], I),
+        "gpm_set_state_device": ([P, P], I),
+        "gpm_get_state_device": ([P, P, P], I),
+        "gpm_compute_dt": ([P, D, ctypes.POINTER(D)], I),
+        "gpm_time_stepping": ([P, P, P, D, D, P, ctypes.POINTER(GpmStats)], I),
+        "gpm_run": ([P, I, D, P], I),
+        "gpm_get_stats": ([P, ctypes.POINTER(GpmStats)], I),
+        "gpm_reset_stats": ([P], I),
+        "gpm_set_time": ([P, D], I),
+        "gpm_stream": ([P], P),
+        "gpm_set_timing": ([P, I], I),
+        "gpm_kernel_time": ([P, ctypes.POINTER(D), ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(D),
+                             ctypes.POINTER(ctypes.c_int64)], I),
+    }
+    for name, (args, res) in sig.items():
+        fn = getattr(L, name)
+        fn.argtypes = args
+        fn.restype = res
+    _lib = L
+    return L
+
+
+def ptr(a):
+    return ctypes.c_void_p(a.ctypes.data) if a is not None else None
+
+
+def gp_tables(mord, ngp, lf, nf=None, ell_o_dx=12.0, lx=1.0, ly=1.0):
+    """GP tables of GP_presim_sphere (GP_init.f90:26-357) in the reference layouts."""
+    nf = lf if nf is None else nf
+    sz = {1: 1, 3: 5, 5: 13, 7: 25}[mord]
+    zT = np.zeros(mord * sz * 4 * ngp)
+    d2x = np.zeros(mord * sz)
+    d2y = np.zeros(mord * sz)
+    gw = np.zeros(25)
+    ixiy = np.zeros(7 * 25 * 2, dtype=np.int32)
+    rc = load().gpm_gp_tables(mord, ngp, ell_o_dx, lx, ly, lf, nf, ptr(zT), ptr(d2x), ptr(d2y), ptr(gw), ptr(ixiy))
+    if rc != 0:
+        raise ValueError(f"gpm_gp_tables failed ({rc})")
+    return {"zT": zT, "d2x": d2x, "d2y": d2y, "gw": gw, "ixiy": ixiy}

@@ -1,0 +1,581 @@
+// Host side of libgpmood: the C ABI declared in include/gpmood.h.
+//
+// Owns the device buffers of one solver context and enqueues the step on the
+// context's HIP stream: k_dt, then per RK stage k_stage (first MOOD pass) and
+// the cooperative k_mood (remaining passes).  Host-array entry points convert
+// between the reference's ghosted layout U(4, lb:le, nb:ne) and the device's
+// interior-only layout.
+#include <hip/hip_runtime.h>
+
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/gpmood.h"
+#include "gpm_internal.h"
+#include "gpm_kernels.h"
+
+using namespace gpm;
+
+struct gpm_ctx {
+    gpm_config cfg{};
+    int N = 0, mord = 0, ngp = 0, bc = 0, device = 0;
+    double dx = 0, dy = 0;
+    hipStream_t stream = nullptr;
+    DevTables hT{};
+    DevTables *dT = nullptr;
+    double4 *dU = nullptr, *dA = nullptr, *dB = nullptr;
+    int8_t *gpo = nullptr;
+    int *flistA = nullptr, *flistB = nullptr, *slist = nullptr, *mark = nullptr;
+    int *counters = nullptr, *stamp = nullptr, *err = nullptr;
+    DevStats *stats = nullptr;
+    double *tstate = nullptr, *block_min = nullptr, *dt_log = nullptr;
+    unsigned *done = nullptr;
+    int dt_log_cap = 0;
+    int tiles_x = 0, ntiles = 0, dt_blocks = 0, mood_blocks[3] = {0, 0, 0};
+    bool has_state = false;
+    bool timing = false;
+    double stage_ms = 0, mood_ms = 0;
+    long long stage_launches = 0, mood_launches = 0;
+    std::vector<hipEvent_t> ev_pool;
+    std::vector<std::pair<int, int>> ev_pairs;  // (start index, kind 0 stage / 1 mood)
+    size_t ev_used = 0;
+    std::string msg;
+};
+
+namespace {
+
+int fail(gpm_ctx *c, int code, const std::string &m)
+{
+    if (c) c->msg = m;
+    return code;
+}
+
+int hipfail(gpm_ctx *c, hipError_t e, const char *what)
+{
+    return fail(c, GPM_EHIP, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+#define HIPCHK(ctx, expr)                                          \
+    do {                                                           \
+        hipError_t e_ = (expr);                                    \
+        if (e_ != hipSuccess) return hipfail((ctx), e_, #expr);    \
+    } while (0)
+
+template <class T>
+hipError_t dalloc(T **p, size_t n)
+{
+    return hipMalloc(reinterpret_cast<void **>(p), n * sizeof(T) > 0 ? n * sizeof(T) : 1);
+}
+
+// Host form of the ghost-cell rule (Boundary_C, BC.f90:18-135): the interior
+// cell a ghost index copies and whether the normal momentum flips sign.
+int host_bc_map(int i, int N, int bc, bool &flip)
+{
+    flip = false;
+    if (i < 0) {
+        if (bc == GPM_NEUMANN) return 0;
+        if (bc == GPM_PERIODIC) return i + N;
+        flip = true;
+        return -1 - i;
+    }
+    if (i >= N) {
+        if (bc == GPM_NEUMANN) return N - 1;
+        if (bc == GPM_PERIODIC) return i - N;
+        flip = true;
+        return 2 * N - 1 - i;
+    }
+    return i;
+}
+
+// interior [N][N][4] -> ghosted [N+8][N+8][4] with the boundary rule
+void fill_ghosted(const double *in, double *out, int N, int bc)
+{
+    const int W = N + 8;
+    for (int jj = 0; jj < W; ++jj) {
+        bool fy;
+        const int sj = host_bc_map(jj - 4, N, bc, fy);
+        for (int ii = 0; ii < W; ++ii) {
+            bool fx;
+            const int si = host_bc_map(ii - 4, N, bc, fx);
+            const double *s = in + ((size_t)sj * N + si) * 4;
+            double *d = out + ((size_t)jj * W + ii) * 4;
+            d[0] = s[0];
+            d[1] = fx ? -s[1] : s[1];
+            d[2] = fy ? -s[2] : s[2];
+            d[3] = s[3];
+        }
+    }
+}
+
+void extract_interior(const double *full, double *in, int N)
+{
+    const int W = N + 8;
+    for (int j = 0; j < N; ++j)
+        std::memcpy(in + (size_t)j * N * 4, full + ((size_t)(j + 4) * W + 4) * 4, sizeof(double) * N * 4);
+}
+
+int upload_tables(gpm_ctx *c, const double *zT, const double *d2x, const double *d2y, const double *gw,
+                  const int32_t *ixiy)
+{
+    const int mord = c->mord, ngp = c->ngp, szm = stencil_size(mord);
+    // The stencil gather order is compiled into the kernels; tables must use it.
+    for (int ord = 1; ord <= mord; ord += 2)
+        for (int s = 0; s < stencil_size(ord); ++s)
+            if (ixiy[(ord - 1) + 7 * (s + 25 * 0)] != kStencilX[s] || ixiy[(ord - 1) + 7 * (s + 25 * 1)] != kStencilY[s])
+                return fail(c, GPM_EINVAL, "ixiy_sp does not follow the sphere stencil order of GP_init.f90:97-176");
+    DevTables &h = c->hT;
+    std::memset(&h, 0, sizeof(h));
+    for (int ord = 1; ord <= mord; ord += 2)
+        for (int i = 0; i < 4; ++i)
+            for (int s = 0; s < stencil_size(ord); ++s)
+                for (int j = 0; j < ngp; ++j)
+                    h.zT[(ord - 1) / 2][i][s][j] = zT[(ord - 1) + mord * (s + szm * (i + 4 * j))];
+    if (mord >= 3)
+        for (int k = 0; k < 5; ++k) {
+            h.d2x[k] = d2x[2 + mord * k];
+            h.d2y[k] = d2y[2 + mord * k];
+        }
+    for (int j = 0; j < ngp; ++j) h.gw[j] = gw[(ngp - 1) + 5 * j];
+    HIPCHK(c, hipMemcpyAsync(c->dT, &h, sizeof(h), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return GPM_OK;
+}
+
+hipEvent_t next_event(gpm_ctx *c)
+{
+    if (c->ev_used == c->ev_pool.size()) {
+        hipEvent_t e;
+        if (hipEventCreate(&e) != hipSuccess) return nullptr;
+        c->ev_pool.push_back(e);
+    }
+    return c->ev_pool[c->ev_used++];
+}
+
+int collect_timing(gpm_ctx *c)
+{
+    if (!c->timing || c->ev_pairs.empty()) return GPM_OK;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    for (auto &pr : c->ev_pairs) {
+        float ms = 0;
+        HIPCHK(c, hipEventElapsedTime(&ms, c->ev_pool[pr.first], c->ev_pool[pr.first + 1]));
+        if (pr.second == 0) { c->stage_ms += ms; ++c->stage_launches; }
+        else { c->mood_ms += ms; ++c->mood_launches; }
+    }
+    c->ev_pairs.clear();
+    c->ev_used = 0;
+    return GPM_OK;
+}
+
+int launch_dt_kernel(gpm_ctx *c, int mode, int use_tmax, double tmax, double dtfinal, double dt_given,
+                     int advance_t, int step_idx, bool log_dt, int new_step)
+{
+    DtArgs a{};
+    a.U = c->dU;
+    a.block_min = c->block_min;
+    a.done = c->done;
+    a.tstate = c->tstate;
+    a.dt_log = log_dt ? c->dt_log : nullptr;
+    a.counters = c->counters;
+    a.ncounters = 3 * kCountersPerStage;
+    a.stats = c->stats;
+    a.N = c->N;
+    a.mode = mode;
+    a.use_tmax = use_tmax;
+    a.advance_t = advance_t;
+    a.step_idx = step_idx;
+    a.new_step = new_step;
+    a.dx = c->dx;
+    a.dy = c->dy;
+    a.cfl = c->cfg.cfl;
+    a.tmax = tmax;
+    a.dtfinal = dtfinal;
+    a.dt_given = dt_given;
+    HIPCHK(c, launch_dt(a, mode == kDtCompute ? c->dt_blocks : 1, c->stream));
+    return GPM_OK;
+}
+
+// One SSP-RK3 step (mod_SSP_RK3.f90:12-39) after k_dt has set dt.
+int enqueue_stages(gpm_ctx *c)
+{
+    for (int stage = 1; stage <= 3; ++stage) {
+        const double4 *in = stage == 1 ? c->dU : (stage == 2 ? c->dA : c->dB);
+        double4 *out = stage == 1 ? c->dA : (stage == 2 ? c->dB : c->dA);
+        int *cnt = c->counters + (stage - 1) * kCountersPerStage;
+        StageArgs s{};
+        s.Uin = in; s.Un = c->dU; s.Uout = out; s.gpo = c->gpo;
+        s.flist = c->flistA; s.cnt_f = cnt; s.T = c->dT; s.tstate = c->tstate;
+        s.N = c->N; s.bc = c->bc; s.dmp = c->cfg.dmp; s.u2 = c->cfg.u2; s.u2tol = c->cfg.u2_tol;
+        s.dx = c->dx; s.dy = c->dy; s.tiles_x = c->tiles_x; s.ntiles = c->ntiles;
+        hipEvent_t e0 = nullptr, e1 = nullptr;
+        if (c->timing) {
+            size_t i0 = c->ev_used;
+            e0 = next_event(c); e1 = next_event(c);
+            if (!e0 || !e1) return fail(c, GPM_EHIP, "hipEventCreate failed");
+            c->ev_pairs.push_back({(int)i0, 0});
+            HIPCHK(c, hipEventRecord(e0, c->stream));
+        }
+        // a multiple of 8 workgroups, so the XCD-aware remap in k_stage is a bijection
+        HIPCHK(c, launch_stage(c->mord, c->ngp, stage, s, (c->ntiles + 7) / 8 * 8, c->stream));
+        if (c->timing) HIPCHK(c, hipEventRecord(e1, c->stream));
+
+        MoodArgs m{};
+        m.Uin = in; m.Un = c->dU; m.Uout = out; m.gpo = c->gpo;
+        m.flist_a = c->flistA; m.flist_b = c->flistB; m.slist = c->slist;
+        m.cnt_f = cnt; m.cnt_s = cnt + kMaxPasses + 1; m.mark = c->mark; m.stamp = c->stamp; m.err = c->err;
+        m.stats = c->stats; m.T = c->dT; m.tstate = c->tstate;
+        m.N = c->N; m.bc = c->bc; m.dmp = c->cfg.dmp; m.u2 = c->cfg.u2; m.u2tol = c->cfg.u2_tol;
+        m.cascade = c->cfg.cascade; m.pmax = kMaxPasses; m.stage_idx = stage - 1;
+        m.dx = c->dx; m.dy = c->dy;
+        void *args[] = {&m};
+        if (c->timing) {
+            size_t i0 = c->ev_used;
+            e0 = next_event(c); e1 = next_event(c);
+            if (!e0 || !e1) return fail(c, GPM_EHIP, "hipEventCreate failed");
+            c->ev_pairs.push_back({(int)i0, 1});
+            HIPCHK(c, hipEventRecord(e0, c->stream));
+        }
+        HIPCHK(c, hipLaunchCooperativeKernel(mood_kernel(c->mord, c->ngp, stage), dim3(c->mood_blocks[stage - 1]),
+                                             dim3(kMoodThreads), args, 0, c->stream));
+        if (c->timing) HIPCHK(c, hipEventRecord(e1, c->stream));
+    }
+    std::swap(c->dU, c->dA);
+    return GPM_OK;
+}
+
+int check_device_error(gpm_ctx *c)
+{
+    int e = 0;
+    HIPCHK(c, hipMemcpyAsync(&e, c->err, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    if (e) return fail(c, GPM_EINVAL, "MOOD iteration exceeded the pass limit");
+    return GPM_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int gpm_create(const gpm_config *cfg, gpm_ctx **out)
+{
+    if (!cfg || !out) return GPM_EINVAL;
+    *out = nullptr;
+    gpm_ctx *c = new gpm_ctx();
+    c->cfg = *cfg;
+    c->N = cfg->lf;
+    c->mord = cfg->mord;
+    c->ngp = cfg->ngp;
+    c->bc = cfg->bc_type;
+    auto bad = [&](int code, const char *m) {
+        std::fprintf(stderr, "gpm_create: %s\n", m);
+        delete c;
+        return code;
+    };
+    if (cfg->lf != cfg->nf || cfg->lf < 8) return bad(GPM_EINVAL, "square grids with lf = nf >= 8 only");
+    if (cfg->lx != cfg->ly) return bad(GPM_EINVAL, "square cells only (Lx = Ly)");
+    if (cfg->integrator != GPM_SSP_RK3) return bad(GPM_EUNSUP, "integrator must be SSP_RK3");
+    if (cfg->num_flux != GPM_HLLC) return bad(GPM_EUNSUP, "numFlux must be HLLC");
+    if (cfg->method != GPM_GP_MOOD) return bad(GPM_EUNSUP, "method must be GP_MOOD");
+    if (cfg->bc_type != GPM_NEUMANN && cfg->bc_type != GPM_PERIODIC && cfg->bc_type != GPM_REFLECTIVE)
+        return bad(GPM_EUNSUP, "BC_type must be Neumann, Periodic or Reflective");
+    if (cfg->cascade != GPM_CASCADE_REFERENCE && cfg->cascade != GPM_CASCADE_FULL)
+        return bad(GPM_EINVAL, "unknown cascade");
+    if (!supported(cfg->mord, cfg->ngp)) return bad(GPM_EUNSUP, "(Mord, ngp) combination not compiled");
+    if ((long long)cfg->lf * cfg->lf > (1ll << 31) - 1) return bad(GPM_EUNSUP, "grid too large for int32 cell ids");
+    // dx = real(Lx_16/lf, 8) (global_variables.f90:19-21)
+    c->dx = quad_ratio(cfg->lx, cfg->lf);
+    c->dy = quad_ratio(cfg->ly, cfg->nf);
+    const size_t n = (size_t)c->N * c->N;
+    hipError_t e;
+    if ((e = hipGetDevice(&c->device)) != hipSuccess) return bad(GPM_EHIP, hipGetErrorString(e));
+    if ((e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking)) != hipSuccess)
+        return bad(GPM_EHIP, hipGetErrorString(e));
+    bool ok = dalloc(&c->dT, 1) == hipSuccess && dalloc(&c->dU, n) == hipSuccess && dalloc(&c->dA, n) == hipSuccess &&
+              dalloc(&c->dB, n) == hipSuccess && dalloc(&c->gpo, n) == hipSuccess &&
+              dalloc(&c->flistA, n) == hipSuccess && dalloc(&c->flistB, n) == hipSuccess &&
+              dalloc(&c->slist, n) == hipSuccess && dalloc(&c->mark, n) == hipSuccess &&
+              dalloc(&c->counters, 3 * kCountersPerStage) == hipSuccess && dalloc(&c->stamp, 1) == hipSuccess &&
+              dalloc(&c->err, 1) == hipSuccess && dalloc(&c->stats, 1) == hipSuccess &&
+              dalloc(&c->tstate, 4) == hipSuccess && dalloc(&c->done, 1) == hipSuccess;
+    if (!ok) {
+        gpm_destroy(c);
+        std::fprintf(stderr, "gpm_create: device allocation failed\n");
+        return GPM_ENOMEM;
+    }
+    int ncu = 0;
+    hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, c->device);
+    if (ncu <= 0) ncu = 256;
+    c->dt_blocks = ncu * 4;
+    if (dalloc(&c->block_min, c->dt_blocks) != hipSuccess) {
+        gpm_destroy(c);
+        return GPM_ENOMEM;
+    }
+    c->tiles_x = (c->N + kTX - 1) / kTX;
+    c->ntiles = c->tiles_x * ((c->N + kTY - 1) / kTY);
+    for (int st = 0; st < 3; ++st) {
+        int per = 0;
+        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, mood_kernel(c->mord, c->ngp, st + 1), kMoodThreads, 0);
+        if (e != hipSuccess || per <= 0) per = 1;
+        c->mood_blocks[st] = per * ncu;
+    }
+    hipMemsetAsync(c->mark, 0, n * sizeof(int), c->stream);
+    hipMemsetAsync(c->gpo, c->mord, n, c->stream);
+    hipMemsetAsync(c->counters, 0, 3 * kCountersPerStage * sizeof(int), c->stream);
+    hipMemsetAsync(c->stats, 0, sizeof(DevStats), c->stream);
+    hipMemsetAsync(c->tstate, 0, 4 * sizeof(double), c->stream);
+    hipMemsetAsync(c->done, 0, sizeof(unsigned), c->stream);
+    hipMemsetAsync(c->err, 0, sizeof(int), c->stream);
+    int one = 1;
+    hipMemcpyAsync(c->stamp, &one, sizeof(int), hipMemcpyHostToDevice, c->stream);
+    if ((e = hipStreamSynchronize(c->stream)) != hipSuccess) {
+        gpm_destroy(c);
+        std::fprintf(stderr, "gpm_create: %s\n", hipGetErrorString(e));
+        return GPM_EHIP;
+    }
+    // GP tables from the library's own precompute (GP_presim_sphere).
+    std::vector<double> zT((size_t)c->mord * stencil_size(c->mord) * 4 * c->ngp), d2x(c->mord * stencil_size(c->mord)),
+        d2y(d2x.size()), gw(25);
+    std::vector<int32_t> ixiy(7 * 25 * 2);
+    if (gp_tables(c->mord, c->ngp, cfg->ell_o_dx, cfg->lx, cfg->ly, cfg->lf, cfg->nf, zT.data(), d2x.data(),
+                  d2y.data(), gw.data(), ixiy.data()) != 0) {
+        gpm_destroy(c);
+        return GPM_EINVAL;
+    }
+    int rc = upload_tables(c, zT.data(), d2x.data(), d2y.data(), gw.data(), ixiy.data());
+    if (rc) {
+        std::fprintf(stderr, "gpm_create: %s\n", c->msg.c_str());
+        gpm_destroy(c);
+        return rc;
+    }
+    *out = c;
+    return GPM_OK;
+}
+
+int gpm_destroy(gpm_ctx *c)
+{
+    if (!c) return GPM_OK;
+    if (c->stream) hipStreamSynchronize(c->stream);
+    void *bufs[] = {c->dT, c->dU, c->dA, c->dB, c->gpo, c->flistA, c->flistB, c->slist, c->mark, c->counters,
+                    c->stamp, c->err, c->stats, c->tstate, c->block_min, c->dt_log, c->done};
+    for (void *b : bufs)
+        if (b) hipFree(b);
+    for (hipEvent_t e : c->ev_pool) hipEventDestroy(e);
+    if (c->stream) hipStreamDestroy(c->stream);
+    delete c;
+    return GPM_OK;
+}
+
+const char *gpm_last_error(const gpm_ctx *c) { return c ? c->msg.c_str() : "null context"; }
+
+int gpm_set_gp_tables(gpm_ctx *c, const double *zT, const double *d2x, const double *d2y, const double *gw,
+                      const int32_t *ixiy)
+{
+    if (!c || !zT || !d2x || !d2y || !gw || !ixiy) return fail(c, GPM_EINVAL, "null argument");
+    return upload_tables(c, zT, d2x, d2y, gw, ixiy);
+}
+
+int gpm_gp_tables(int32_t mord, int32_t ngp, double ell_o_dx, double lx, double ly, int32_t lf, int32_t nf,
+                  double *zT, double *d2x, double *d2y, double *gw, int32_t *ixiy)
+{
+    if (!zT || !d2x || !d2y || !gw || !ixiy) return GPM_EINVAL;
+    return gp_tables(mord, ngp, ell_o_dx, lx, ly, lf, nf, zT, d2x, d2y, gw, ixiy) == 0 ? GPM_OK : GPM_EINVAL;
+}
+
+int gpm_set_state(gpm_ctx *c, const double *U)
+{
+    if (!c || !U) return fail(c, GPM_EINVAL, "null argument");
+    std::vector<double> in((size_t)c->N * c->N * 4);
+    extract_interior(U, in.data(), c->N);
+    HIPCHK(c, hipMemcpyAsync(c->dU, in.data(), in.size() * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipMemsetAsync(c->gpo, c->mord, (size_t)c->N * c->N, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    c->has_state = true;
+    return GPM_OK;
+}
+
+int gpm_get_state(gpm_ctx *c, double *U, int32_t *cellgpo)
+{
+    if (!c || !U) return fail(c, GPM_EINVAL, "null argument");
+    if (!c->has_state) return fail(c, GPM_ESTATE, "no state set");
+    const size_t n = (size_t)c->N * c->N;
+    std::vector<double> in(n * 4);
+    std::vector<int8_t> g(n);
+    HIPCHK(c, hipMemcpyAsync(in.data(), c->dU, n * 4 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipMemcpyAsync(g.data(), c->gpo, n, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    fill_ghosted(in.data(), U, c->N, c->bc);
+    if (cellgpo) {
+        const int W = c->N + 8;
+        for (int jj = 0; jj < W; ++jj)
+            for (int ii = 0; ii < W; ++ii) {
+                const int i = ii - 4, j = jj - 4;
+                cellgpo[(size_t)jj * W + ii] =
+                    (i < 0 || i >= c->N || j < 0 || j >= c->N) ? c->mord : (int32_t)g[(size_t)j * c->N + i];
+            }
+    }
+    return GPM_OK;
+}
+
+int gpm_set_state_device(gpm_ctx *c, const double *u_dev)
+{
+    if (!c || !u_dev) return fail(c, GPM_EINVAL, "null argument");
+    const size_t n = (size_t)c->N * c->N;
+    HIPCHK(c, hipMemcpyAsync(c->dU, u_dev, n * 4 * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+    HIPCHK(c, hipMemsetAsync(c->gpo, c->mord, n, c->stream));
+    c->has_state = true;
+    return GPM_OK;
+}
+
+int gpm_get_state_device(gpm_ctx *c, double *u_dev, int8_t *order_dev)
+{
+    if (!c) return GPM_EINVAL;
+    if (!c->has_state) return fail(c, GPM_ESTATE, "no state set");
+    const size_t n = (size_t)c->N * c->N;
+    if (u_dev) HIPCHK(c, hipMemcpyAsync(u_dev, c->dU, n * 4 * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+    if (order_dev) HIPCHK(c, hipMemcpyAsync(order_dev, c->gpo, n, hipMemcpyDeviceToDevice, c->stream));
+    return GPM_OK;
+}
+
+int gpm_compute_dt(gpm_ctx *c, double dtfinal, double *dt)
+{
+    if (!c || !dt) return fail(c, GPM_EINVAL, "null argument");
+    if (!c->has_state) return fail(c, GPM_ESTATE, "no state set");
+    int rc = launch_dt_kernel(c, kDtCompute, 0, 0.0, dtfinal, 0.0, 0, 0, false, 0);
+    if (rc) return rc;
+    double ts[4];
+    HIPCHK(c, hipMemcpyAsync(ts, c->tstate, sizeof(ts), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    *dt = ts[1];
+    return GPM_OK;
+}
+
+int gpm_set_time(gpm_ctx *c, double t)
+{
+    if (!c) return GPM_EINVAL;
+    double ts[4] = {t, 0, 0, 0};
+    HIPCHK(c, hipMemcpyAsync(c->tstate, ts, sizeof(ts), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return GPM_OK;
+}
+
+int gpm_time_stepping(gpm_ctx *c, double *Uin, double *Uout, double t, double dt, int32_t *cellgpo,
+                      gpm_stats *stats)
+{
+    if (!c || !Uin || !Uout) return fail(c, GPM_EINVAL, "null argument");
+    int rc = gpm_set_state(c, Uin);
+    if (rc) return rc;
+    // Boundary_C(Uin) of Forward_Euler (mod_FE.f90:64) refills Uin's ghosts.
+    {
+        std::vector<double> in((size_t)c->N * c->N * 4);
+        extract_interior(Uin, in.data(), c->N);
+        fill_ghosted(in.data(), Uin, c->N, c->bc);
+    }
+    rc = gpm_set_time(c, t);
+    if (rc) return rc;
+    rc = launch_dt_kernel(c, kDtGiven, 0, 0.0, 0.0, dt, 0, 0, false, 1);
+    if (rc) return rc;
+    rc = enqueue_stages(c);
+    if (rc) return rc;
+    rc = check_device_error(c);
+    if (rc) return rc;
+    rc = collect_timing(c);
+    if (rc) return rc;
+    rc = gpm_get_state(c, Uout, cellgpo);
+    if (rc) return rc;
+    if (stats) {
+        rc = gpm_get_stats(c, stats);
+        if (rc) return rc;
+        stats->t = t;
+        stats->dt = dt;
+    }
+    return GPM_OK;
+}
+
+int gpm_run(gpm_ctx *c, int32_t nsteps, double tmax, double *dts)
+{
+    if (!c || nsteps < 0) return fail(c, GPM_EINVAL, "invalid argument");
+    if (!c->has_state) return fail(c, GPM_ESTATE, "no state set");
+    if (dts && nsteps > c->dt_log_cap) {
+        if (c->dt_log) hipFree(c->dt_log);
+        c->dt_log = nullptr;
+        HIPCHK(c, dalloc(&c->dt_log, nsteps));
+        c->dt_log_cap = nsteps;
+    }
+    for (int s = 0; s < nsteps; ++s) {
+        int rc = launch_dt_kernel(c, kDtCompute, 1, tmax, 0.0, 0.0, 1, s, dts != nullptr, 1);
+        if (rc) return rc;
+        rc = enqueue_stages(c);
+        if (rc) return rc;
+        if (c->timing && c->ev_pairs.size() > 512) {
+            rc = collect_timing(c);
+            if (rc) return rc;
+        }
+    }
+    int rc = check_device_error(c);
+    if (rc) return rc;
+    rc = collect_timing(c);
+    if (rc) return rc;
+    if (dts && nsteps > 0) {
+        HIPCHK(c, hipMemcpyAsync(dts, c->dt_log, nsteps * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+    }
+    return GPM_OK;
+}
+
+int gpm_get_stats(gpm_ctx *c, gpm_stats *st)
+{
+    if (!c || !st) return fail(c, GPM_EINVAL, "null argument");
+    DevStats d{};
+    double ts[4];
+    HIPCHK(c, hipMemcpyAsync(&d, c->stats, sizeof(d), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipMemcpyAsync(ts, c->tstate, sizeof(ts), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    st->steps = d.steps;
+    st->count_detected_cell = d.detected_step;
+    st->detected_total = d.detected_total;
+    for (int k = 0; k < 3; ++k) st->passes[k] = d.passes[k];
+    st->max_passes = d.max_passes;
+    st->second_pass_cells = d.second_pass_cells;
+    st->a_priori_lowered = d.a_priori_lowered;
+    st->t = ts[0];
+    st->dt = ts[1];
+    return GPM_OK;
+}
+
+int gpm_reset_stats(gpm_ctx *c)
+{
+    if (!c) return GPM_EINVAL;
+    HIPCHK(c, hipMemsetAsync(c->stats, 0, sizeof(DevStats), c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    c->stage_ms = c->mood_ms = 0;
+    c->stage_launches = c->mood_launches = 0;
+    c->ev_pairs.clear();
+    c->ev_used = 0;
+    return GPM_OK;
+}
+
+void *gpm_stream(gpm_ctx *c) { return c ? (void *)c->stream : nullptr; }
+
+int gpm_set_timing(gpm_ctx *c, int32_t enable)
+{
+    if (!c) return GPM_EINVAL;
+    int rc = collect_timing(c);
+    c->timing = enable != 0;
+    return rc;
+}
+
+int gpm_kernel_time(gpm_ctx *c, double *avg_stage_ms, int64_t *stage_launches, double *avg_mood_ms,
+                    int64_t *mood_launches)
+{
+    if (!c) return GPM_EINVAL;
+    int rc = collect_timing(c);
+    if (rc) return rc;
+    if (avg_stage_ms) *avg_stage_ms = c->stage_launches ? c->stage_ms / c->stage_launches : 0.0;
+    if (stage_launches) *stage_launches = c->stage_launches;
+    if (avg_mood_ms) *avg_mood_ms = c->mood_launches ? c->mood_ms / c->mood_launches : 0.0;
+    if (mood_launches) *mood_launches = c->mood_launches;
+    return GPM_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,317 @@
+// Device-side numerics of the GP-MOOD step for gfx950.
+//
+// Every FP64 expression keeps the operation order of the reference Fortran
+// (built with -fdefault-real-8, no FMA) so that results are bit-identical:
+// the library is compiled with -ffp-contract=off, IEEE division and sqrt.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace gpm {
+
+constexpr double kGamma = 1.4;  // constants.f90:9 (y = 1.4)
+
+// Boundary-condition ids (constants.f90:25).
+enum { kNeumann = 17, kPeriodic = 18, kReflective = 20 };
+// Face ids (constants.f90:37): iL, iT, iR, iB.
+enum { kL = 0, kT = 1, kR = 2, kB = 3 };
+
+// Sphere stencil offsets in the reference's order (GP_init.f90:97-176).
+__device__ constexpr int kSX[25] = {0, -1, 0, 1, 0, -1, -1, 1, 1, 0, -2, 0, 2,
+                                    -3, -2, -1, 0, 1, 2, 3, 2, 1, 0, -1, -2};
+__device__ constexpr int kSY[25] = {0, 0, 1, 0, -1, -1, 1, 1, -1, -2, 0, 2, 0,
+                                    0, 1, 2, 3, 2, 1, 0, -1, -2, -3, -2, -1};
+
+__host__ __device__ constexpr int stencil_sz(int ord) { return ord == 1 ? 1 : ord == 3 ? 5 : ord == 5 ? 13 : 25; }
+
+// GP tables on the device, laid out for scalar loads in the unrolled loops:
+// zT[(ord-1)/2][face][s][gp].
+struct DevTables {
+    double zT[4][4][25][5];
+    double d2x[5], d2y[5];  // GP_d2x_sp(3, 1:5), GP_d2y_sp(3, 1:5)
+    double gw[5];           // gauss_weight(ngp, 1:ngp)
+};
+
+// Fortran MIN/MAX of the reference build: the second operand unless the first
+// compares strictly smaller (larger).
+__device__ __forceinline__ double fmin2(double a, double b) { return (a < b) ? a : b; }
+__device__ __forceinline__ double fmax2(double a, double b) { return (a > b) ? a : b; }
+
+// pressure, physics.f90:132-143
+__device__ __forceinline__ double pressure(double r, double mx, double my, double E)
+{
+    return (kGamma - 1.0) * (E - (0.5 * (mx * mx + my * my)) / r);
+}
+
+// Conserved state of a cell as read from memory (rho, rho*u, rho*v, E).
+struct Cell {
+    double r, mx, my, E;
+};
+
+__device__ __forceinline__ Cell to_cell(const double4 &v) { return Cell{v.x, v.y, v.z, v.w}; }
+
+// Ghost-cell rule of Boundary_C (BC.f90:18-135) as a map from a cell index in
+// [-4, N+3] to the interior cell it copies and whether the normal momentum is
+// negated.  Indices further out (only reached by partial tiles, whose values are
+// never used) are clamped into the grid.
+__device__ __forceinline__ int bc_map(int i, int N, int bc, bool &flip)
+{
+    flip = false;
+    if (i < 0) {
+        if (bc == kNeumann) i = 0;
+        else if (bc == kPeriodic) i = i + N;
+        else { i = -1 - i; flip = true; }
+    } else if (i >= N) {
+        if (bc == kNeumann) i = N - 1;
+        else if (bc == kPeriodic) i = i - N;
+        else { i = 2 * N - 1 - i; flip = true; }
+    }
+    i = i < 0 ? 0 : (i > N - 1 ? N - 1 : i);
+    return i;
+}
+
+__device__ __forceinline__ double4 load_bc(const double4 *__restrict__ U, int N, int bc, int i, int j)
+{
+    bool fx, fy;
+    int si = bc_map(i, N, bc, fx);
+    int sj = bc_map(j, N, bc, fy);
+    double4 v = U[(size_t)sj * N + si];
+    if (fx) v.y = -v.y;
+    if (fy) v.z = -v.z;
+    return v;
+}
+
+// HLLC flux (mod_HLLC.f90:11-169) with c2p_local and Flux (physics.f90:9-75).
+// DIR 0: x-face (normal momentum mx), DIR 1: y-face.
+template <int DIR>
+__device__ __forceinline__ void hllc(const double ul[4], const double ur[4], double F[4])
+{
+    const double vxl = ul[1] / ul[0], vyl = ul[2] / ul[0];
+    const double pl = pressure(ul[0], ul[1], ul[2], ul[3]);
+    const double cl = sqrt(kGamma * pl / ul[0]);
+    const double vxr = ur[1] / ur[0], vyr = ur[2] / ur[0];
+    const double pr = pressure(ur[0], ur[1], ur[2], ur[3]);
+    const double cr = sqrt(kGamma * pr / ur[0]);
+    const bool adm = !(ul[0] <= 0. || pl <= 0. || isnan(ul[0]) || isnan(pl) ||
+                       ur[0] <= 0. || pr <= 0. || isnan(ur[0]) || isnan(pr));
+    if (!adm) {
+        // mod_HLLC.f90:166: F = 1/(vx-vx), vx from the right state's c2p
+        const double z = vxr - vxr;
+        const double f = 1.0 / z;
+        F[0] = f; F[1] = f; F[2] = f; F[3] = f;
+        return;
+    }
+    constexpr int md = DIR == 0 ? 1 : 2, om = DIR == 0 ? 2 : 1;
+    const double vnl = DIR == 0 ? vxl : vyl, vtl = DIR == 0 ? vyl : vxl;
+    const double vnr = DIR == 0 ? vxr : vyr, vtr = DIR == 0 ? vyr : vxr;
+    double Fl[4], Fr[4];
+    if (DIR == 0) {
+        Fl[0] = ul[1]; Fl[1] = ul[0] * (vxl * vxl) + pl; Fl[2] = ul[2] * vxl; Fl[3] = vxl * (ul[3] + pl);
+        Fr[0] = ur[1]; Fr[1] = ur[0] * (vxr * vxr) + pr; Fr[2] = ur[2] * vxr; Fr[3] = vxr * (ur[3] + pr);
+    } else {
+        Fl[0] = ul[2]; Fl[1] = ul[1] * vyl; Fl[2] = ul[0] * (vyl * vyl) + pl; Fl[3] = vyl * (ul[3] + pl);
+        Fr[0] = ur[2]; Fr[1] = ur[1] * vyr; Fr[2] = ur[0] * (vyr * vyr) + pr; Fr[3] = vyr * (ur[3] + pr);
+    }
+    const double SL = fmin2(vnl - cl, vnr - cr);
+    const double SR = fmax2(vnl + cl, vnr + cr);
+    const double numerL = SL - vnl, numerR = SR - vnr;
+    double uStar = ul[0] * vnl * numerL - ur[0] * vnr * numerR;
+    uStar = (pr - pl) + uStar;
+    uStar = uStar / (ul[0] * numerL - ur[0] * numerR);
+    if (SL >= 0.) {
+        F[0] = Fl[0]; F[1] = Fl[1]; F[2] = Fl[2]; F[3] = Fl[3];
+    } else if (SL < 0. && uStar >= 0.) {
+        const double dStarL = ul[0] * numerL / (SL - uStar);
+        double Us[4];
+        Us[0] = 1.0; Us[md] = uStar; Us[om] = vtl;
+        Us[3] = ul[3] / ul[0] + (uStar - vnl) * (uStar + pl / (ul[0] * numerL));
+#pragma unroll
+        for (int k = 0; k < 4; ++k) F[k] = Fl[k] + SL * (Us[k] * dStarL - ul[k]);
+    } else if (uStar < 0. && SR >= 0.) {
+        const double dStarR = ur[0] * numerR / (SR - uStar);
+        double Us[4];
+        Us[0] = 1.0; Us[md] = uStar; Us[om] = vtr;
+        Us[3] = ur[3] / ur[0] + (uStar - vnr) * (uStar + pr / (ur[0] * numerR));
+#pragma unroll
+        for (int k = 0; k < 4; ++k) F[k] = Fr[k] + SR * (Us[k] * dStarR - ur[k]);
+    } else {
+        F[0] = Fr[0]; F[1] = Fr[1]; F[2] = Fr[2]; F[3] = Fr[3];
+    }
+}
+
+// GP reconstruction of both sides of a face at all NGP Gauss points
+// (mod_GP_MOOD.f90:46-65): ua[j][k] = sum_s zT(ORD, s, FA, j) * q_A(k, s) and
+// likewise ub for cell B, each summed from s = 1 in stencil order starting at 0
+// (dot_product).  The stencil loop stays rolled (UNR points per trip) so the
+// weights are fetched with scalar loads per trip instead of being held in
+// registers across the whole stencil.
+template <int ORD, int NGP, int FA, int FB, int UNR, class Acc>
+__device__ __forceinline__ void recon2(const DevTables *__restrict__ T, const Acc &acc, int ai, int aj, int bi,
+                                       int bj, double (&ua)[NGP][4], double (&ub)[NGP][4])
+{
+    constexpr int SZ = stencil_sz(ORD);
+    constexpr int OI = (ORD - 1) / 2;
+#pragma unroll
+    for (int j = 0; j < NGP; ++j)
+#pragma unroll
+        for (int k = 0; k < 4; ++k) { ua[j][k] = 0.0; ub[j][k] = 0.0; }
+#pragma unroll UNR
+    for (int s = 0; s < SZ; ++s) {
+        const int dx = kSX[s], dy = kSY[s];
+        const double4 qa = acc(ai + dx, aj + dy);
+        const double4 qb = acc(bi + dx, bj + dy);
+#pragma unroll
+        for (int j = 0; j < NGP; ++j) {
+            const double za = T->zT[OI][FA][s][j];
+            const double zb = T->zT[OI][FB][s][j];
+            ua[j][0] = ua[j][0] + za * qa.x;
+            ua[j][1] = ua[j][1] + za * qa.y;
+            ua[j][2] = ua[j][2] + za * qa.z;
+            ua[j][3] = ua[j][3] + za * qa.w;
+            ub[j][0] = ub[j][0] + zb * qb.x;
+            ub[j][1] = ub[j][1] + zb * qb.y;
+            ub[j][2] = ub[j][2] + zb * qb.z;
+            ub[j][3] = ub[j][3] + zb * qb.w;
+        }
+    }
+}
+
+#ifndef GPM_RECON_UNROLL
+#define GPM_RECON_UNROLL 1
+#endif
+
+// Face flux: Riemann solve at each Gauss point of the face between cell A
+// (left/bottom) and B (right/top), then L_Flux = sum_j F_j * gauss_weight(ngp,j)
+// from 0 (mod_FE.f90:80-98 and 112-130).
+template <int ORD, int NGP, int DIR, class Acc>
+__device__ __forceinline__ void face_flux(const DevTables *__restrict__ T, const Acc &acc, int ai, int aj,
+                                          int bi, int bj, double L[4])
+{
+    double ua[NGP][4], ub[NGP][4];
+    recon2<ORD, NGP, DIR == 0 ? kR : kT, DIR == 0 ? kL : kB, GPM_RECON_UNROLL>(T, acc, ai, aj, bi, bj, ua, ub);
+    L[0] = 0.0; L[1] = 0.0; L[2] = 0.0; L[3] = 0.0;
+    if (ORD == 1) {
+        // First order: every Gauss point carries the same two states, so one
+        // Riemann solve serves all of them; the quadrature sum is unchanged.
+        double F[4];
+        hllc<DIR>(ua[0], ub[0], F);
+#pragma unroll
+        for (int j = 0; j < NGP; ++j) {
+            const double w = T->gw[j];
+#pragma unroll
+            for (int k = 0; k < 4; ++k) L[k] = L[k] + F[k] * w;
+        }
+    } else {
+#pragma unroll
+        for (int j = 0; j < NGP; ++j) {
+            double F[4];
+            hllc<DIR>(ua[j], ub[j], F);
+            const double w = T->gw[j];
+#pragma unroll
+            for (int k = 0; k < 4; ++k) L[k] = L[k] + F[k] * w;
+        }
+    }
+}
+
+// Face flux for a face whose order is only known at run time (ord <= MORD).
+template <int MORD, int NGP, int DIR, class Acc>
+__device__ __noinline__ void face_flux_dyn(const DevTables *__restrict__ T, const Acc &acc, int ord, int ai,
+                                              int aj, int bi, int bj, double L[4])
+{
+    if (MORD >= 7 && ord == 7) face_flux<(MORD >= 7 ? 7 : 1), NGP, DIR>(T, acc, ai, aj, bi, bj, L);
+    else if (MORD >= 5 && ord == 5) face_flux<(MORD >= 5 ? 5 : 1), NGP, DIR>(T, acc, ai, aj, bi, bj, L);
+    else if (MORD >= 3 && ord == 3) face_flux<(MORD >= 3 ? 3 : 1), NGP, DIR>(T, acc, ai, aj, bi, bj, L);
+    else face_flux<1, NGP, DIR>(T, acc, ai, aj, bi, bj, L);
+}
+
+struct MoodParams {
+    int dmp, u2, u2tol;
+    double dx, dy;
+};
+
+// shock_detector, mod_subroutine_MOOD.f90:191-238
+template <class Acc>
+__device__ __forceinline__ bool shock_detector(const MoodParams &P, const Acc &acc, int i, int j)
+{
+    const Cell c = to_cell(acc(i, j));
+    const Cell e = to_cell(acc(i + 1, j)), w = to_cell(acc(i - 1, j));
+    const Cell nn = to_cell(acc(i, j + 1)), s = to_cell(acc(i, j - 1));
+    const double dx = P.dx, dy = P.dy;
+    double divV = (e.mx - w.mx) / dx + (nn.my - s.my) / dy;
+    divV = 0.5 * divV / c.r;
+    divV = divV - 0.5 * (c.mx * (e.r - w.r) / dx + c.my * (nn.r - s.r) / dy) / (c.r * c.r);
+    const double Ca = 1.4 * pressure(c.r, c.mx, c.my, c.E) / c.r;
+    double Mach = (c.mx * c.mx + c.my * c.my) / (c.r * c.r);
+    Mach = sqrt(Mach / Ca);
+    const double pL = pressure(w.r, w.mx, w.my, w.E);
+    const double pR = pressure(e.r, e.mx, e.my, e.E);
+    const double pB = pressure(s.r, s.mx, s.my, s.E);
+    const double pT = pressure(nn.r, nn.mx, nn.my, nn.E);
+    const double gradP = 0.5 * (fabs(pR - pL) / fmin2(pL, pR) / dx + fabs(pT - pB) / fmin2(pB, pT) / dy);
+    const double m = fmax2(dx, dy);
+    return divV < -(m * m) && Mach > 0.2 && gradP > 5.0;
+}
+
+// DETECTION loop-1 decision of one cell (mod_subroutine_MOOD.f90:41-131) for a
+// cell whose order is above 1: PAD, then the shock-gated DMP with plateau test
+// and U2 smooth-extrema relaxation.  `cand` is the cell's candidate Uout.
+template <class Acc>
+__device__ __noinline__ bool u2_relax(const MoodParams &P, const DevTables *__restrict__ T, const Acc &acc, int i,
+                                      int j)
+{
+    double D2x[5], D2y[5];  // (-1,0), (0,0), (1,0), (0,-1), (0,1)
+    const int oi[5] = {-1, 0, 1, 0, 0}, oj[5] = {0, 0, 0, -1, 1};
+#pragma unroll
+    for (int t = 0; t < 5; ++t) {
+        double q[5];
+#pragma unroll
+        for (int k = 0; k < 5; ++k) q[k] = acc(i + kSX[k] + oi[t], j + kSY[k] + oj[t]).x;
+        double ax = 0.0, ay = 0.0;
+#pragma unroll
+        for (int k = 0; k < 5; ++k) ax = ax + T->d2x[k] * q[k];
+#pragma unroll
+        for (int k = 0; k < 5; ++k) ay = ay + T->d2y[k] * q[k];
+        D2x[t] = ax;
+        D2y[t] = ay;
+    }
+    const double Xmin = fmin2(fmin2(fmin2(fmin2(D2x[0], D2x[1]), D2x[2]), D2x[3]), D2x[4]);
+    const double Xmax = fmax2(fmax2(fmax2(fmax2(D2x[0], D2x[1]), D2x[2]), D2x[3]), D2x[4]);
+    const double Ymin = fmin2(fmin2(fmin2(fmin2(D2y[0], D2y[1]), D2y[2]), D2y[3]), D2y[4]);
+    const double Ymax = fmax2(fmax2(fmax2(fmax2(D2y[0], D2y[1]), D2y[2]), D2y[3]), D2y[4]);
+    double delta = 0.0;
+    if (P.u2tol) delta = fmax2(P.dx, P.dy);
+    if ((Xmin * Xmax > -delta) && ((fmax2(fabs(Xmin), fabs(Xmax)) < delta) || (fabs(Xmin / Xmax) >= 0.5)))
+        return true;
+    if ((Ymin * Ymax > -delta) && ((fmax2(fabs(Ymin), fabs(Ymax)) < delta) || (fabs(Ymin / Ymax) >= 0.5)))
+        return true;
+    return false;
+}
+
+template <class Acc>
+__device__ __forceinline__ bool decide(const MoodParams &P, const DevTables *__restrict__ T, const Acc &acc, int i,
+                                       int j, const double cand[4])
+{
+    if (cand[0] <= 0. || isnan(cand[0])) return true;
+    const double p = pressure(cand[0], cand[1], cand[2], cand[3]);
+    if (p <= 0. || isnan(p)) return true;
+    if (!P.dmp) return false;
+    if (!shock_detector(P, acc, i, j)) return false;
+    const double UB = acc(i, j - 1).x, UL = acc(i - 1, j).x, UM = acc(i, j).x;
+    const double UR = acc(i + 1, j).x, UT = acc(i, j + 1).x;
+    const double mn = fmin2(fmin2(fmin2(fmin2(UM, UL), UB), UT), UR);
+    const double mx = fmax2(fmax2(fmax2(fmax2(UM, UL), UB), UT), UR);
+    const double Mm = mx - mn;
+    if (!(Mm > P.dx * P.dx * P.dx)) return false;
+    if (!(cand[0] > mx || cand[0] < mn)) return false;
+    if (P.u2 && u2_relax(P, T, acc, i, j)) return false;
+    return true;
+}
+
+// Cascade of a failing cell (mod_subroutine_MOOD.f90:159-163 or the full one).
+__device__ __forceinline__ int lower_order(int ord, int cascade)
+{
+    if (cascade == 0) return ord == 3 ? 1 : (ord > 3 ? 3 : ord);
+    return ord > 1 ? ord - 2 : ord;
+}
+
+}  // namespace gpm

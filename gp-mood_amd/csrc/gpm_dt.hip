@@ -1,0 +1,83 @@
+// Setdt reduction kernel and per-step bookkeeping.
+#include "gpm_kernels_impl.h"
+
+namespace gpm {
+
+// Setdt: per-cell 1/(1/(dx/(|vx|+c)) + 1/(dy/(|vy|+c))), grid minimum, times
+// CFL, clipped to dtfinal (physics.f90:98-124).  The last workgroup to finish
+// reduces the per-workgroup minima, advances t and zeroes the step counters.
+__global__ __launch_bounds__(256) void k_dt(DtArgs a)
+{
+    __shared__ double red[256];
+    __shared__ bool last;
+    const int tid = threadIdx.x;
+    double m = 1.0e30;
+    if (a.mode == kDtCompute) {
+        const size_t ncell = (size_t)a.N * a.N;
+        for (size_t c = (size_t)blockIdx.x * blockDim.x + tid; c < ncell; c += (size_t)gridDim.x * blockDim.x) {
+            const double4 u = a.U[c];
+            const double vx = u.y / u.x, vy = u.z / u.x;
+            const double pres = pressure(u.x, u.y, u.z, u.w);
+            const double cs = sqrt(kGamma * pres / u.x);
+            const double v = 1.0 / (1.0 / (a.dx / (fabs(vx) + cs)) + 1.0 / (a.dy / (fabs(vy) + cs)));
+            m = fmin2(m, v);
+        }
+    }
+    red[tid] = m;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if (tid < s) red[tid] = fmin2(red[tid], red[tid + s]);
+        __syncthreads();
+    }
+    if (tid == 0) {
+        a.block_min[blockIdx.x] = red[0];
+        __threadfence();
+        const unsigned prev = atomicAdd(a.done, 1u);
+        last = (prev == gridDim.x - 1);
+    }
+    __syncthreads();
+    if (!last) return;
+    __threadfence();
+    double mm = 1.0e30;
+    for (int b = tid; b < (int)gridDim.x; b += 256) mm = fmin2(mm, __hip_atomic_load(&a.block_min[b], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+    red[tid] = mm;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if (tid < s) red[tid] = fmin2(red[tid], red[tid + s]);
+        __syncthreads();
+    }
+    // a new step zeroes the MOOD pass counters of its three stages
+    if (a.new_step)
+        for (int k = tid; k < a.ncounters; k += 256) a.counters[k] = 0;
+    if (tid == 0) {
+        double *ts = a.tstate;
+        const double t = ts[0];
+        double dt;
+        if (a.mode == kDtCompute) {
+            dt = red[0] * a.cfl;
+            const double dtfinal = a.use_tmax ? a.tmax - t : a.dtfinal;
+            if (dt > dtfinal) dt = dtfinal;
+        } else {
+            dt = a.dt_given;
+        }
+        ts[1] = dt;
+        ts[2] = dt / a.dx;
+        ts[3] = dt / a.dy;
+        if (a.advance_t) ts[0] = t + dt;
+        if (a.dt_log) a.dt_log[a.step_idx] = dt;
+        if (a.new_step) {
+            a.stats->detected_step = 0;
+            a.stats->steps += 1;
+        }
+        *a.done = 0u;
+    }
+}
+
+
+hipError_t launch_dt(const DtArgs &a, int nblocks, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_dt, dim3(nblocks), dim3(256), 0, s, a);
+    return hipGetLastError();
+}
+
+}  // namespace gpm

@@ -1,0 +1,92 @@
+// Kernel argument blocks and launch entry points shared by the HIP kernels and
+// the host side of libgpmood.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "gpm_device.h"
+
+namespace gpm {
+
+// First-pass tile: kTX x kTY cells per workgroup of kStageThreads threads.
+constexpr int kTX = 32;
+constexpr int kTY = 16;
+constexpr int kStageThreads = 256;
+constexpr int kMoodThreads = 256;
+// Halo width: the stencil radius R = (Mord-1)/2 plus one cell for the face
+// neighbour, and at least 2 for the detection (U2 reads a cell's 5-point
+// neighbours' 5-point stencils).
+template <int MORD>
+constexpr int kHalo = ((MORD - 1) / 2 + 1) > 2 ? ((MORD - 1) / 2 + 1) : 2;
+
+constexpr int kMaxPasses = 4096;  // MOOD passes per stage before the run is declared failed
+constexpr int kCountersPerStage = 2 * kMaxPasses + 2;
+
+struct DevStats {
+    long long steps;
+    long long detected_step;
+    long long detected_total;
+    long long second_pass_cells;
+    long long a_priori_lowered;
+    int passes[3];
+    int max_passes;
+};
+
+struct StageArgs {
+    const double4 *__restrict__ Uin;  // stage input (interior, N*N)
+    const double4 *__restrict__ Un;   // step-start state for the RK combination
+    double4 *__restrict__ Uout;       // combined stage output
+    int8_t *__restrict__ gpo;         // CellGPO (interior)
+    int *__restrict__ flist;          // troubled cells of the first pass
+    int *cnt_f;                       // their count
+    const DevTables *__restrict__ T;
+    const double *__restrict__ tstate; // t, dt, dt/dx, dt/dy
+    int N, bc, dmp, u2, u2tol;
+    double dx, dy;
+    int tiles_x, ntiles;
+};
+
+struct MoodArgs {
+    const double4 *__restrict__ Uin;
+    const double4 *__restrict__ Un;
+    double4 *__restrict__ Uout;
+    int8_t *gpo;
+    int *flist_a, *flist_b, *slist;
+    int *cnt_f;  // [kMaxPasses+1] troubled-cell counts per pass
+    int *cnt_s;  // [kMaxPasses]   recomputed-cell counts per pass
+    int *mark;   // per-cell pass stamp (dedup of neighbourhoods)
+    int *stamp;  // running stamp
+    int *err;
+    DevStats *stats;
+    const DevTables *__restrict__ T;
+    const double *__restrict__ tstate;
+    int N, bc, dmp, u2, u2tol, cascade, pmax, stage_idx;
+    double dx, dy;
+};
+
+enum { kDtCompute = 0, kDtGiven = 1 };
+
+struct DtArgs {
+    const double4 *__restrict__ U;
+    double *block_min;
+    unsigned *done;
+    double *tstate;
+    double *dt_log;
+    int *counters;
+    int ncounters;
+    DevStats *stats;
+    int N, mode, use_tmax, advance_t, step_idx, new_step;
+    double dx, dy, cfl, tmax, dtfinal, dt_given;
+};
+
+template <int MORD, int NGP>
+hipError_t launch_stage_mg(int stage, const StageArgs &a, int nblocks, hipStream_t s);
+template <int MORD, int NGP>
+const void *mood_kernel_mg(int stage);
+
+hipError_t launch_stage(int mord, int ngp, int stage, const StageArgs &a, int nblocks, hipStream_t s);
+const void *mood_kernel(int mord, int ngp, int stage);
+bool supported(int mord, int ngp);
+hipError_t launch_dt(const DtArgs &a, int nblocks, hipStream_t s);
+
+}  // namespace gpm

@@ -1,0 +1,154 @@
+/*
+ * gpmood.h -- C ABI of the MI355X (gfx950) GP-MOOD 2D Euler step.
+ *
+ * The library replaces the reference's time-stepping layer (slug-cfd/gp-mood,
+ * gp-mood-code/) behind plain C entry points: plain pointers, sizes and ints,
+ * no C++ or torch types.  A Fortran program binds them with iso_c_binding
+ * (gp-mood_amd/fortran/gpm_capi.f90); Python binds them with ctypes.
+ *
+ * State layout (host arrays): the reference's U(4, lb:le, nb:ne) with
+ * lb = nb = 1-ngc, le = lf+ngc, ne = nf+ngc, ngc = 4 (global_variables.f90:8-26),
+ * i.e. a C array double U[nf+8][lf+8][4], variable fastest (rho, rho*u, rho*v, E).
+ * Order masks are int32 CellGPO(lb:le, nb:ne) (global_variables.f90:48).
+ *
+ * Error convention: every entry point returns 0 on success and a negative
+ * GPM_E* code on failure; gpm_last_error() gives the message.  A numerically
+ * inadmissible state is not an error: it propagates as in the reference and
+ * MOOD lowers the order of the cells it reaches (mod_HLLC.f90:166,
+ * mod_subroutine_MOOD.f90:43-51).
+ *
+ * Threading: one context per host thread.  Host-array calls are synchronous;
+ * device-resident calls are ordered on the context's HIP stream.
+ */
+#ifndef GPMOOD_H
+#define GPMOOD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Identifiers shared with the reference's constants.f90:16-27. */
+#define GPM_GP_MOOD 2
+#define GPM_NN_GP_MOOD 203
+#define GPM_SSP_RK3 10
+#define GPM_HLLC 100
+#define GPM_NEUMANN 17
+#define GPM_PERIODIC 18
+#define GPM_REFLECTIVE 20
+
+/* Order cascade applied to a cell that fails detection. */
+#define GPM_CASCADE_REFERENCE 0 /* 7->3, 5->3, 3->1 (mod_subroutine_MOOD.f90:159-163) */
+#define GPM_CASCADE_FULL 1      /* 7->5->3->1 */
+
+#define GPM_OK 0
+#define GPM_EINVAL -1   /* invalid configuration or argument */
+#define GPM_EHIP -2     /* HIP runtime error */
+#define GPM_ENOMEM -3   /* device allocation failed */
+#define GPM_EUNSUP -4   /* configuration not supported by this build */
+#define GPM_ESTATE -5   /* call out of order (e.g. no state set) */
+
+/* Runtime form of the reference's compile-time `module parameters`
+ * (problems/2DRP3_GPMOOD.f90:8-46).  Field names follow that module. */
+typedef struct gpm_config {
+    int32_t lf, nf;       /* interior cells in x and y (square grids: lf == nf) */
+    int32_t mord;         /* Mord: 1, 3, 5 or 7 */
+    int32_t ngp;          /* Gauss points per face: 1..5 */
+    int32_t method;       /* GPM_GP_MOOD or GPM_NN_GP_MOOD */
+    int32_t integrator;   /* GPM_SSP_RK3 */
+    int32_t num_flux;     /* GPM_HLLC */
+    int32_t bc_type;      /* GPM_NEUMANN, GPM_PERIODIC or GPM_REFLECTIVE */
+    int32_t dmp, u2, u2_tol; /* MOOD switches DMP, U2, U2_tol (logical: 0/1) */
+    int32_t cascade;      /* GPM_CASCADE_* */
+    double cfl;           /* CFL */
+    double lx, ly;        /* domain lengths Lx, Ly */
+    double ell_o_dx;      /* GP length scale over dx */
+} gpm_config;
+
+/* Per-step statistics (the reference's counters in global_variables.f90:62-64
+ * and the MOOD timers of mod_FE.f90:52-55,164-173, as counts). */
+typedef struct gpm_stats {
+    int64_t steps;                 /* steps taken since the last reset */
+    int64_t count_detected_cell;   /* decreases of the last step, all stages and passes */
+    int64_t detected_total;        /* decreases summed over all steps since reset */
+    int32_t passes[3];             /* MOOD passes per RK stage of the last step */
+    int32_t max_passes;            /* largest pass count of any stage since reset */
+    int64_t second_pass_cells;     /* cells recomputed after the first pass, since reset */
+    int64_t a_priori_lowered;      /* cells started below Mord by the learned predictor */
+    double t;                      /* simulation time */
+    double dt;                     /* dt of the last step */
+} gpm_stats;
+
+typedef struct gpm_ctx gpm_ctx;
+
+/* Create a context on the current HIP device.  Replaces the set-up part of
+ * main.f90:112-125 that the step needs (GP_presim_sphere, GP_init.f90:26-357,
+ * runs on the host inside this call with the library's own quad-precision
+ * precompute unless gpm_set_gp_tables() is called afterwards). */
+int gpm_create(const gpm_config *cfg, gpm_ctx **out);
+int gpm_destroy(gpm_ctx *ctx);
+const char *gpm_last_error(const gpm_ctx *ctx);
+
+/* GP tables in the reference's layouts (global_variables.f90:37-38,53,61):
+ *   zT_sp(Mord, sz_sphere, 4, ngp), GP_d2x_sp(Mord, sz_sphere),
+ *   GP_d2y_sp(Mord, sz_sphere), gauss_weight(5,5), ixiy_sp(7,25,2).
+ * gpm_set_gp_tables loads tables computed by the caller (e.g. the Fortran
+ * GP_presim_sphere), so that a drop-in run uses bit-identical weights.
+ * gpm_gp_tables computes them on the host (GP_init.f90:26-357) for a grid of
+ * lf x nf cells on an lx x ly domain (dx = lx/lf evaluated in quad precision). */
+int gpm_set_gp_tables(gpm_ctx *ctx, const double *zT_sp, const double *gp_d2x_sp,
+                      const double *gp_d2y_sp, const double *gauss_weight,
+                      const int32_t *ixiy_sp);
+int gpm_gp_tables(int32_t mord, int32_t ngp, double ell_o_dx, double lx, double ly,
+                  int32_t lf, int32_t nf, double *zT_sp, double *gp_d2x_sp,
+                  double *gp_d2y_sp, double *gauss_weight, int32_t *ixiy_sp);
+
+/* Host-array state transfer in the reference layout.  gpm_get_state fills the
+ * ghost cells with the boundary condition (Boundary_C, BC.f90:18-135) and, if
+ * cellgpo is non-NULL, writes the order mask of the last RK stage. */
+int gpm_set_state(gpm_ctx *ctx, const double *U);
+int gpm_get_state(gpm_ctx *ctx, double *U, int32_t *cellgpo);
+
+/* Device-resident state transfer: `u_dev` points to nf*lf*4 doubles of device
+ * memory holding the interior cells [n][l][k] (no ghosts). */
+int gpm_set_state_device(gpm_ctx *ctx, const double *u_dev);
+int gpm_get_state_device(gpm_ctx *ctx, double *u_dev, int8_t *order_dev);
+
+/* Setdt (physics.f90:78-129) on the resident state: dt = CFL*min(...),
+ * clipped to dtfinal. */
+int gpm_compute_dt(gpm_ctx *ctx, double dtfinal, double *dt);
+
+/* Drop-in for `subroutine time_stepping(Uin, Uout)` (time_stepper.f90:12-23,
+ * SSP_RK3_ of mod_SSP_RK3.f90:12-39) with the reference's implicit globals made
+ * explicit: t and dt are the module variables set by main.f90 and Setdt; the
+ * order mask of the last stage is written to cellgpo (may be NULL) and the
+ * counters to stats (may be NULL).  Uin's ghost cells are refilled as the
+ * reference's Boundary_C(Uin) does; Uout's ghost cells are filled. */
+int gpm_time_stepping(gpm_ctx *ctx, double *Uin, double *Uout, double t, double dt,
+                      int32_t *cellgpo, gpm_stats *stats);
+
+/* Device-resident driver loop of main.f90:145-162 for nsteps steps:
+ * dtfinal = tmax - t; Setdt; SSP-RK3 step; t = t + dt.  No host round trip per
+ * step.  If dts is non-NULL it receives the nsteps time steps (host memory). */
+int gpm_run(gpm_ctx *ctx, int32_t nsteps, double tmax, double *dts);
+
+int gpm_get_stats(gpm_ctx *ctx, gpm_stats *stats);
+int gpm_reset_stats(gpm_ctx *ctx);
+int gpm_set_time(gpm_ctx *ctx, double t);
+
+/* The HIP stream the context enqueues on (a hipStream_t, as void*). */
+void *gpm_stream(gpm_ctx *ctx);
+
+/* Launch shape and timing of the dominant kernel, for roofline reporting:
+ * average duration in ms of the first-pass stage kernel over the launches
+ * since the last gpm_reset_stats (0 if timing is off), and launch count. */
+int gpm_set_timing(gpm_ctx *ctx, int32_t enable);
+int gpm_kernel_time(gpm_ctx *ctx, double *avg_stage_ms, int64_t *stage_launches,
+                    double *avg_mood_ms, int64_t *mood_launches);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* GPMOOD_H */

@@ -1,0 +1,64 @@
+"""Shared test helpers: loaders for the package (directory gp-mood_amd/), the
+CPU oracle (oracle/, test infrastructure) and the golden fixtures."""
+import importlib.util
+import os
+import sys
+
+import numpy as np
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+GOLDEN = os.path.join(ROOT, "tests", "golden")
+
+
+def pytest_configure(config):
+    config.addinivalue_line("markers", "gpu: needs an MI355X (run with -m gpu)")
+
+
+def load_package():
+    if "gpmood_amd" in sys.modules:
+        return sys.modules["gpmood_amd"]
+    pkg_dir = os.path.join(ROOT, "gp-mood_amd")
+    spec = importlib.util.spec_from_file_location("gpmood_amd", os.path.join(pkg_dir, "__init__.py"),
+                                                  submodule_search_locations=[pkg_dir])
+    mod = importlib.util.module_from_spec(spec)
+    sys.modules["gpmood_amd"] = mod
+    spec.loader.exec_module(mod)
+    return mod
+
+
+def load_oracle():
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    try:
+        import oracle as O
+    finally:
+        sys.path.pop(0)
+    return O
+
+
+def golden(name):
+    return np.load(os.path.join(GOLDEN, name), allow_pickle=False)
+
+
+def tables_from_npz(z):
+    return {k: z[k] for k in ("zT", "d2x", "d2y", "gw", "ixiy")}
+
+
+@pytest.fixture(scope="session")
+def gpm():
+    return load_package()
+
+
+@pytest.fixture(scope="session")
+def orc():
+    O = load_oracle()
+    O.lib()  # raises if not built
+    return O
+
+
+def has_gpu():
+    try:
+        import torch
+        return torch.cuda.is_available()
+    except Exception:
+        return False
