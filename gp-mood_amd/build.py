@@ -35,7 +35,7 @@ def run(cmd):
         raise RuntimeError("build step failed: " + os.path.basename(cmd[-3] if "-o" in cmd else cmd[-1]))
 
 
-def build(verbose=True, jobs=None):
+def build(verbose=True, jobs=None, out=None, defines=(), combos=None):
     objdir = os.path.join(HERE, "_build")
     os.makedirs(objdir, exist_ok=True)
     cmds, objs = [], []
@@ -43,17 +43,22 @@ def build(verbose=True, jobs=None):
         obj = os.path.join(objdir, src + ".o")
         cmds.append(["g++", "-O2", "-fPIC", "-std=c++17", "-ffp-contract=off", "-c", os.path.join(CSRC, src), "-o", obj])
         objs.append(obj)
+    dflags = [f"-D{d}" for d in defines]
+    tag = "_".join(d.replace("=", "") for d in defines)
     for src in HIP_SOURCES:
-        obj = os.path.join(objdir, src + ".o")
-        cmds.append([HIPCC] + HIP_FLAGS + ["-x", "hip", "-c", os.path.join(CSRC, src), "-o", obj])
+        if combos is not None and src.startswith("gpm_k_m") and src not in combos:
+            continue
+        obj = os.path.join(objdir, src + (("." + tag) if tag else "") + ".o")
+        cmds.append([HIPCC] + HIP_FLAGS + dflags + ["-x", "hip", "-c", os.path.join(CSRC, src), "-o", obj])
         objs.append(obj)
     jobs = jobs or min(8, os.cpu_count() or 1)
     with ThreadPoolExecutor(max_workers=jobs) as ex:
         list(ex.map(run, cmds))
-    tmp = LIB + ".tmp"
+    lib = out or LIB
+    tmp = lib + ".tmp"
     run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp] + objs + ["-lquadmath"])
-    os.replace(tmp, LIB)
-    return LIB
+    os.replace(tmp, lib)
+    return lib
 
 
 if __name__ == "__main__":

@@ -15,7 +15,7 @@ except Exception:  # pragma: no cover - torch is part of the image
     torch = None
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "libgpmood.so")
+LIB_PATH = os.environ.get("GPM_LIB_PATH") or os.path.join(HERE, "libgpmood.so")
 
 GP_MOOD = 2
 NN_GP_MOOD = 203

@@ -8,6 +8,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -26,8 +27,8 @@ struct gpm_ctx {
     DevTables hT{};
     DevTables *dT = nullptr;
     double4 *dU = nullptr, *dA = nullptr, *dB = nullptr;
-    int8_t *gpo = nullptr;
-    int *flistA = nullptr, *flistB = nullptr, *slist = nullptr, *mark = nullptr;
+    order_t *gpo = nullptr;
+    int *slistA = nullptr, *slistB = nullptr, *mark = nullptr;
     int *counters = nullptr, *stamp = nullptr, *err = nullptr;
     DevStats *stats = nullptr;
     double *tstate = nullptr, *block_min = nullptr, *dt_log = nullptr;
@@ -205,8 +206,10 @@ int enqueue_stages(gpm_ctx *c)
         int *cnt = c->counters + (stage - 1) * kCountersPerStage;
         StageArgs s{};
         s.Uin = in; s.Un = c->dU; s.Uout = out; s.gpo = c->gpo;
-        s.flist = c->flistA; s.cnt_f = cnt; s.T = c->dT; s.tstate = c->tstate;
+        s.slist1 = c->slistB; s.cnt_s1 = cnt + kMaxPasses + 1 + 1; s.cnt_f0 = cnt;
+        s.mark = c->mark; s.stamp = c->stamp; s.T = c->dT; s.tstate = c->tstate;
         s.N = c->N; s.bc = c->bc; s.dmp = c->cfg.dmp; s.u2 = c->cfg.u2; s.u2tol = c->cfg.u2_tol;
+        s.cascade = c->cfg.cascade;
         s.dx = c->dx; s.dy = c->dy; s.tiles_x = c->tiles_x; s.ntiles = c->ntiles;
         hipEvent_t e0 = nullptr, e1 = nullptr;
         if (c->timing) {
@@ -222,7 +225,7 @@ int enqueue_stages(gpm_ctx *c)
 
         MoodArgs m{};
         m.Uin = in; m.Un = c->dU; m.Uout = out; m.gpo = c->gpo;
-        m.flist_a = c->flistA; m.flist_b = c->flistB; m.slist = c->slist;
+        m.slist[0] = c->slistA; m.slist[1] = c->slistB;
         m.cnt_f = cnt; m.cnt_s = cnt + kMaxPasses + 1; m.mark = c->mark; m.stamp = c->stamp; m.err = c->err;
         m.stats = c->stats; m.T = c->dT; m.tstate = c->tstate;
         m.N = c->N; m.bc = c->bc; m.dmp = c->cfg.dmp; m.u2 = c->cfg.u2; m.u2tol = c->cfg.u2_tol;
@@ -293,8 +296,8 @@ int gpm_create(const gpm_config *cfg, gpm_ctx **out)
         return bad(GPM_EHIP, hipGetErrorString(e));
     bool ok = dalloc(&c->dT, 1) == hipSuccess && dalloc(&c->dU, n) == hipSuccess && dalloc(&c->dA, n) == hipSuccess &&
               dalloc(&c->dB, n) == hipSuccess && dalloc(&c->gpo, n) == hipSuccess &&
-              dalloc(&c->flistA, n) == hipSuccess && dalloc(&c->flistB, n) == hipSuccess &&
-              dalloc(&c->slist, n) == hipSuccess && dalloc(&c->mark, n) == hipSuccess &&
+              dalloc(&c->slistA, n) == hipSuccess && dalloc(&c->slistB, n) == hipSuccess &&
+              dalloc(&c->mark, n) == hipSuccess &&
               dalloc(&c->counters, 3 * kCountersPerStage) == hipSuccess && dalloc(&c->stamp, 1) == hipSuccess &&
               dalloc(&c->err, 1) == hipSuccess && dalloc(&c->stats, 1) == hipSuccess &&
               dalloc(&c->tstate, 4) == hipSuccess && dalloc(&c->done, 1) == hipSuccess;
@@ -306,21 +309,26 @@ int gpm_create(const gpm_config *cfg, gpm_ctx **out)
     int ncu = 0;
     hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, c->device);
     if (ncu <= 0) ncu = 256;
-    c->dt_blocks = ncu * 4;
+    c->dt_blocks = ncu * 8;
     if (dalloc(&c->block_min, c->dt_blocks) != hipSuccess) {
         gpm_destroy(c);
         return GPM_ENOMEM;
     }
     c->tiles_x = (c->N + kTX - 1) / kTX;
     c->ntiles = c->tiles_x * ((c->N + kTY - 1) / kTY);
+    // Cooperative grid of the MOOD kernel: all its workgroups must be resident.
+    // The correction work is sparse and the per-pass grid barrier gets cheaper
+    // with fewer workgroups: by default one per four CUs.  GPM_MOOD_BLOCKS overrides the count.
+    int want = ncu / 4 > 0 ? ncu / 4 : 1;
+    if (const char *env = std::getenv("GPM_MOOD_BLOCKS")) want = std::atoi(env) > 0 ? std::atoi(env) : ncu;
     for (int st = 0; st < 3; ++st) {
         int per = 0;
         e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, mood_kernel(c->mord, c->ngp, st + 1), kMoodThreads, 0);
         if (e != hipSuccess || per <= 0) per = 1;
-        c->mood_blocks[st] = per * ncu;
+        c->mood_blocks[st] = want < per * ncu ? want : per * ncu;
     }
     hipMemsetAsync(c->mark, 0, n * sizeof(int), c->stream);
-    hipMemsetAsync(c->gpo, c->mord, n, c->stream);
+    hipMemsetD16Async(c->gpo, (unsigned short)c->mord, n, c->stream);
     hipMemsetAsync(c->counters, 0, 3 * kCountersPerStage * sizeof(int), c->stream);
     hipMemsetAsync(c->stats, 0, sizeof(DevStats), c->stream);
     hipMemsetAsync(c->tstate, 0, 4 * sizeof(double), c->stream);
@@ -356,7 +364,7 @@ int gpm_destroy(gpm_ctx *c)
 {
     if (!c) return GPM_OK;
     if (c->stream) hipStreamSynchronize(c->stream);
-    void *bufs[] = {c->dT, c->dU, c->dA, c->dB, c->gpo, c->flistA, c->flistB, c->slist, c->mark, c->counters,
+    void *bufs[] = {c->dT, c->dU, c->dA, c->dB, c->gpo, c->slistA, c->slistB, c->mark, c->counters,
                     c->stamp, c->err, c->stats, c->tstate, c->block_min, c->dt_log, c->done};
     for (void *b : bufs)
         if (b) hipFree(b);
@@ -388,7 +396,7 @@ int gpm_set_state(gpm_ctx *c, const double *U)
     std::vector<double> in((size_t)c->N * c->N * 4);
     extract_interior(U, in.data(), c->N);
     HIPCHK(c, hipMemcpyAsync(c->dU, in.data(), in.size() * sizeof(double), hipMemcpyHostToDevice, c->stream));
-    HIPCHK(c, hipMemsetAsync(c->gpo, c->mord, (size_t)c->N * c->N, c->stream));
+    HIPCHK(c, hipMemsetD16Async(c->gpo, (unsigned short)c->mord, (size_t)c->N * c->N, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
     c->has_state = true;
     return GPM_OK;
@@ -400,9 +408,9 @@ int gpm_get_state(gpm_ctx *c, double *U, int32_t *cellgpo)
     if (!c->has_state) return fail(c, GPM_ESTATE, "no state set");
     const size_t n = (size_t)c->N * c->N;
     std::vector<double> in(n * 4);
-    std::vector<int8_t> g(n);
+    std::vector<order_t> g(n);
     HIPCHK(c, hipMemcpyAsync(in.data(), c->dU, n * 4 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
-    HIPCHK(c, hipMemcpyAsync(g.data(), c->gpo, n, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipMemcpyAsync(g.data(), c->gpo, n * sizeof(order_t), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
     fill_ghosted(in.data(), U, c->N, c->bc);
     if (cellgpo) {
@@ -411,7 +419,7 @@ int gpm_get_state(gpm_ctx *c, double *U, int32_t *cellgpo)
             for (int ii = 0; ii < W; ++ii) {
                 const int i = ii - 4, j = jj - 4;
                 cellgpo[(size_t)jj * W + ii] =
-                    (i < 0 || i >= c->N || j < 0 || j >= c->N) ? c->mord : (int32_t)g[(size_t)j * c->N + i];
+                    (i < 0 || i >= c->N || j < 0 || j >= c->N) ? c->mord : (int32_t)(g[(size_t)j * c->N + i] & 0xF);
             }
     }
     return GPM_OK;
@@ -422,7 +430,7 @@ int gpm_set_state_device(gpm_ctx *c, const double *u_dev)
     if (!c || !u_dev) return fail(c, GPM_EINVAL, "null argument");
     const size_t n = (size_t)c->N * c->N;
     HIPCHK(c, hipMemcpyAsync(c->dU, u_dev, n * 4 * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
-    HIPCHK(c, hipMemsetAsync(c->gpo, c->mord, n, c->stream));
+    HIPCHK(c, hipMemsetD16Async(c->gpo, (unsigned short)c->mord, n, c->stream));
     c->has_state = true;
     return GPM_OK;
 }
@@ -433,7 +441,7 @@ int gpm_get_state_device(gpm_ctx *c, double *u_dev, int8_t *order_dev)
     if (!c->has_state) return fail(c, GPM_ESTATE, "no state set");
     const size_t n = (size_t)c->N * c->N;
     if (u_dev) HIPCHK(c, hipMemcpyAsync(u_dev, c->dU, n * 4 * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
-    if (order_dev) HIPCHK(c, hipMemcpyAsync(order_dev, c->gpo, n, hipMemcpyDeviceToDevice, c->stream));
+    if (order_dev) HIPCHK(c, launch_unpack_orders(c->gpo, order_dev, n, c->stream));
     return GPM_OK;
 }
 
@@ -507,7 +515,7 @@ int gpm_run(gpm_ctx *c, int32_t nsteps, double tmax, double *dts)
         if (rc) return rc;
         rc = enqueue_stages(c);
         if (rc) return rc;
-        if (c->timing && c->ev_pairs.size() > 512) {
+        if (c->timing && c->ev_pairs.size() > 8192) {
             rc = collect_timing(c);
             if (rc) return rc;
         }

@@ -83,8 +83,10 @@ __device__ __forceinline__ double4 load_bc(const double4 *__restrict__ U, int N,
 
 // HLLC flux (mod_HLLC.f90:11-169) with c2p_local and Flux (physics.f90:9-75).
 // DIR 0: x-face (normal momentum mx), DIR 1: y-face.
+// hllc_adm returns false, leaving F untouched, when either state is
+// inadmissible (rho <= 0, p <= 0 or NaN: c2p_local, physics.f90:72).
 template <int DIR>
-__device__ __forceinline__ void hllc(const double ul[4], const double ur[4], double F[4])
+__device__ __forceinline__ bool hllc_adm(const double ul[4], const double ur[4], double F[4])
 {
     const double vxl = ul[1] / ul[0], vyl = ul[2] / ul[0];
     const double pl = pressure(ul[0], ul[1], ul[2], ul[3]);
@@ -94,13 +96,7 @@ __device__ __forceinline__ void hllc(const double ul[4], const double ur[4], dou
     const double cr = sqrt(kGamma * pr / ur[0]);
     const bool adm = !(ul[0] <= 0. || pl <= 0. || isnan(ul[0]) || isnan(pl) ||
                        ur[0] <= 0. || pr <= 0. || isnan(ur[0]) || isnan(pr));
-    if (!adm) {
-        // mod_HLLC.f90:166: F = 1/(vx-vx), vx from the right state's c2p
-        const double z = vxr - vxr;
-        const double f = 1.0 / z;
-        F[0] = f; F[1] = f; F[2] = f; F[3] = f;
-        return;
-    }
+    if (!adm) return false;
     constexpr int md = DIR == 0 ? 1 : 2, om = DIR == 0 ? 2 : 1;
     const double vnl = DIR == 0 ? vxl : vyl, vtl = DIR == 0 ? vyl : vxl;
     const double vnr = DIR == 0 ? vxr : vyr, vtr = DIR == 0 ? vyr : vxr;
@@ -136,6 +132,42 @@ __device__ __forceinline__ void hllc(const double ul[4], const double ur[4], dou
         for (int k = 0; k < 4; ++k) F[k] = Fr[k] + SR * (Us[k] * dStarR - ur[k]);
     } else {
         F[0] = Fr[0]; F[1] = Fr[1]; F[2] = Fr[2]; F[3] = Fr[3];
+    }
+    return true;
+}
+
+// Flux of an inadmissible face: mod_HLLC.f90:166, F = 1/(vx-vx) with vx the
+// x-velocity of the right state's c2p (+Inf for finite vx, NaN otherwise).
+__device__ __forceinline__ void hllc_inadmissible(const double ur[4], double F[4])
+{
+    const double vx = ur[1] / ur[0];
+    const double z = vx - vx;
+    const double f = 1.0 / z;
+    F[0] = f; F[1] = f; F[2] = f; F[3] = f;
+}
+
+template <int DIR>
+__device__ __forceinline__ void hllc(const double ul[4], const double ur[4], double F[4])
+{
+    if (!hllc_adm<DIR>(ul, ur, F)) hllc_inadmissible(ur, F);
+}
+
+// y-face flux through the x-direction solver: HLLC in y equals HLLC in x on
+// states with the two momenta swapped, with the flux's momentum components
+// swapped back (pressure, speeds and star states are symmetric under the swap,
+// operation for operation), so one code path serves both directions.
+__device__ __forceinline__ void hllc_dir(int dir, const double ul[4], const double ur[4], double F[4])
+{
+    if (dir == 0) {
+        if (!hllc_adm<0>(ul, ur, F)) hllc_inadmissible(ur, F);
+        return;
+    }
+    const double sl[4] = {ul[0], ul[2], ul[1], ul[3]}, sr[4] = {ur[0], ur[2], ur[1], ur[3]};
+    double G[4];
+    if (hllc_adm<0>(sl, sr, G)) {
+        F[0] = G[0]; F[1] = G[2]; F[2] = G[1]; F[3] = G[3];
+    } else {
+        hllc_inadmissible(ur, F);
     }
 }
 
@@ -213,15 +245,37 @@ __device__ __forceinline__ void face_flux(const DevTables *__restrict__ T, const
     }
 }
 
-// Face flux for a face whose order is only known at run time (ord <= MORD).
-template <int MORD, int NGP, int DIR, class Acc>
-__device__ __noinline__ void face_flux_dyn(const DevTables *__restrict__ T, const Acc &acc, int ord, int ai,
-                                              int aj, int bi, int bj, double L[4])
+// Flux of one face at one Gauss point j (run-time face ids and direction):
+// the two GP reconstructions (same summation order as recon2) and one Riemann
+// solve.  Used by the sparse MOOD correction, one lane per (face, point).
+template <int ORD, class Acc>
+__device__ __forceinline__ void face_gp_flux(const DevTables *__restrict__ T, const Acc &acc, int fa, int fb,
+                                             int dir, int j, int ai, int aj, int bi, int bj, double F[4])
 {
-    if (MORD >= 7 && ord == 7) face_flux<(MORD >= 7 ? 7 : 1), NGP, DIR>(T, acc, ai, aj, bi, bj, L);
-    else if (MORD >= 5 && ord == 5) face_flux<(MORD >= 5 ? 5 : 1), NGP, DIR>(T, acc, ai, aj, bi, bj, L);
-    else if (MORD >= 3 && ord == 3) face_flux<(MORD >= 3 ? 3 : 1), NGP, DIR>(T, acc, ai, aj, bi, bj, L);
-    else face_flux<1, NGP, DIR>(T, acc, ai, aj, bi, bj, L);
+    constexpr int SZ = stencil_sz(ORD);
+    constexpr int OI = (ORD - 1) / 2;
+    double ua[4] = {0.0, 0.0, 0.0, 0.0}, ub[4] = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll 5
+    for (int s = 0; s < SZ; ++s) {
+        const int dx = kSX[s], dy = kSY[s];
+        const double4 qa = acc(ai + dx, aj + dy);
+        const double4 qb = acc(bi + dx, bj + dy);
+        const double za = T->zT[OI][fa][s][j];
+        const double zb = T->zT[OI][fb][s][j];
+        ua[0] = ua[0] + za * qa.x; ua[1] = ua[1] + za * qa.y; ua[2] = ua[2] + za * qa.z; ua[3] = ua[3] + za * qa.w;
+        ub[0] = ub[0] + zb * qb.x; ub[1] = ub[1] + zb * qb.y; ub[2] = ub[2] + zb * qb.z; ub[3] = ub[3] + zb * qb.w;
+    }
+    hllc_dir(dir, ua, ub, F);
+}
+
+template <int MORD, class Acc>
+__device__ __forceinline__ void face_gp_flux_dyn(const DevTables *__restrict__ T, const Acc &acc, int ord, int fa,
+                                                 int fb, int dir, int j, int ai, int aj, int bi, int bj, double F[4])
+{
+    if (MORD >= 7 && ord == 7) face_gp_flux<(MORD >= 7 ? 7 : 1)>(T, acc, fa, fb, dir, j, ai, aj, bi, bj, F);
+    else if (MORD >= 5 && ord == 5) face_gp_flux<(MORD >= 5 ? 5 : 1)>(T, acc, fa, fb, dir, j, ai, aj, bi, bj, F);
+    else if (MORD >= 3 && ord == 3) face_gp_flux<(MORD >= 3 ? 3 : 1)>(T, acc, fa, fb, dir, j, ai, aj, bi, bj, F);
+    else face_gp_flux<1>(T, acc, fa, fb, dir, j, ai, aj, bi, bj, F);
 }
 
 struct MoodParams {
@@ -312,6 +366,26 @@ __device__ __forceinline__ int lower_order(int ord, int cascade)
 {
     if (cascade == 0) return ord == 3 ? 1 : (ord > 3 ? 3 : ord);
     return ord > 1 ? ord - 2 : ord;
+}
+
+// The order a cell had before the cascade lowered it to `ord`.
+__device__ __forceinline__ int raise_order(int ord, int mord, int cascade)
+{
+    if (cascade == 0) return ord == 1 ? 3 : mord;
+    return ord + 2;
+}
+
+// Per-cell order word: bits 0-3 the order (CellGPO), bits 4-15 the MOOD pass
+// in which the cascade last lowered it (0: the first pass or a priori).  A
+// correction pass reads the orders its cells had when the pass began: a cell
+// lowered during the same pass p (version p) still counts with the order it
+// had before, so decisions and lowering can share one grid barrier per pass.
+typedef uint16_t order_t;
+__device__ __forceinline__ order_t pack_order(int ord, int ver) { return (order_t)((ver << 4) | ord); }
+__device__ __forceinline__ int order_at_pass(order_t v, int p, int mord, int cascade)
+{
+    const int o = v & 0xF;
+    return ((int)(v >> 4) == p) ? raise_order(o, mord, cascade) : o;
 }
 
 }  // namespace gpm

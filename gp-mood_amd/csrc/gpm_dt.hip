@@ -13,14 +13,22 @@ __global__ __launch_bounds__(256) void k_dt(DtArgs a)
     const int tid = threadIdx.x;
     double m = 1.0e30;
     if (a.mode == kDtCompute) {
+        // each thread takes 4 cells per trip so that their loads and division
+        // chains overlap
         const size_t ncell = (size_t)a.N * a.N;
-        for (size_t c = (size_t)blockIdx.x * blockDim.x + tid; c < ncell; c += (size_t)gridDim.x * blockDim.x) {
-            const double4 u = a.U[c];
-            const double vx = u.y / u.x, vy = u.z / u.x;
-            const double pres = pressure(u.x, u.y, u.z, u.w);
-            const double cs = sqrt(kGamma * pres / u.x);
-            const double v = 1.0 / (1.0 / (a.dx / (fabs(vx) + cs)) + 1.0 / (a.dy / (fabs(vy) + cs)));
-            m = fmin2(m, v);
+        const size_t stride = (size_t)gridDim.x * blockDim.x * 4;
+        for (size_t c0 = ((size_t)blockIdx.x * blockDim.x + tid) * 4; c0 < ncell; c0 += stride) {
+            double4 u[4];
+#pragma unroll
+            for (int q = 0; q < 4; ++q) u[q] = (c0 + q < ncell) ? a.U[c0 + q] : make_double4(1.0, 0.0, 0.0, 1.0);
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const double vx = u[q].y / u[q].x, vy = u[q].z / u[q].x;
+                const double pres = pressure(u[q].x, u[q].y, u[q].z, u[q].w);
+                const double cs = sqrt(kGamma * pres / u[q].x);
+                const double v = 1.0 / (1.0 / (a.dx / (fabs(vx) + cs)) + 1.0 / (a.dy / (fabs(vy) + cs)));
+                if (c0 + q < ncell) m = fmin2(m, v);
+            }
         }
     }
     red[tid] = m;
@@ -73,6 +81,19 @@ __global__ __launch_bounds__(256) void k_dt(DtArgs a)
     }
 }
 
+
+// Order words -> plain CellGPO bytes.
+__global__ void k_unpack_orders(const order_t *__restrict__ g, int8_t *__restrict__ out, size_t n)
+{
+    for (size_t c = (size_t)blockIdx.x * blockDim.x + threadIdx.x; c < n; c += (size_t)gridDim.x * blockDim.x)
+        out[c] = (int8_t)(g[c] & 0xF);
+}
+
+hipError_t launch_unpack_orders(const order_t *g, int8_t *out, size_t n, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_unpack_orders, dim3(1024), dim3(256), 0, s, g, out, n);
+    return hipGetLastError();
+}
 
 hipError_t launch_dt(const DtArgs &a, int nblocks, hipStream_t s)
 {

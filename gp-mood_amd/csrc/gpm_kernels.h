@@ -9,8 +9,17 @@
 namespace gpm {
 
 // First-pass tile: kTX x kTY cells per workgroup of kStageThreads threads.
-constexpr int kTX = 32;
-constexpr int kTY = 16;
+#ifndef GPM_TX
+#define GPM_TX 32
+#endif
+#ifndef GPM_TY
+#define GPM_TY 11
+#endif
+#ifndef GPM_STAGE_WAVES_PER_EU
+#define GPM_STAGE_WAVES_PER_EU 3
+#endif
+constexpr int kTX = GPM_TX;
+constexpr int kTY = GPM_TY;
 constexpr int kStageThreads = 256;
 constexpr int kMoodThreads = 256;
 // Halo width: the stencil radius R = (Mord-1)/2 plus one cell for the face
@@ -36,12 +45,15 @@ struct StageArgs {
     const double4 *__restrict__ Uin;  // stage input (interior, N*N)
     const double4 *__restrict__ Un;   // step-start state for the RK combination
     double4 *__restrict__ Uout;       // combined stage output
-    int8_t *__restrict__ gpo;         // CellGPO (interior)
-    int *__restrict__ flist;          // troubled cells of the first pass
-    int *cnt_f;                       // their count
+    order_t *__restrict__ gpo;        // packed CellGPO (interior)
+    int *__restrict__ slist1;         // cells to recompute in the first correction pass
+    int *cnt_s1;                      // their count
+    int *cnt_f0;                      // troubled cells of the first pass
+    int *mark;                        // per-cell pass stamp (dedup of neighbourhoods)
+    const int *stamp;                 // running stamp base
     const DevTables *__restrict__ T;
     const double *__restrict__ tstate; // t, dt, dt/dx, dt/dy
-    int N, bc, dmp, u2, u2tol;
+    int N, bc, dmp, u2, u2tol, cascade;
     double dx, dy;
     int tiles_x, ntiles;
 };
@@ -50,12 +62,12 @@ struct MoodArgs {
     const double4 *__restrict__ Uin;
     const double4 *__restrict__ Un;
     double4 *__restrict__ Uout;
-    int8_t *gpo;
-    int *flist_a, *flist_b, *slist;
-    int *cnt_f;  // [kMaxPasses+1] troubled-cell counts per pass
-    int *cnt_s;  // [kMaxPasses]   recomputed-cell counts per pass
-    int *mark;   // per-cell pass stamp (dedup of neighbourhoods)
-    int *stamp;  // running stamp
+    order_t *gpo;
+    int *slist[2];  // recompute lists, pass p reads slist[p & 1]
+    int *cnt_f;     // [kMaxPasses+1] troubled-cell counts per pass (pass 0 = k_stage)
+    int *cnt_s;     // [kMaxPasses+1] recomputed-cell counts per pass
+    int *mark;
+    int *stamp;
     int *err;
     DevStats *stats;
     const DevTables *__restrict__ T;
@@ -88,5 +100,6 @@ hipError_t launch_stage(int mord, int ngp, int stage, const StageArgs &a, int nb
 const void *mood_kernel(int mord, int ngp, int stage);
 bool supported(int mord, int ngp);
 hipError_t launch_dt(const DtArgs &a, int nblocks, hipStream_t s);
+hipError_t launch_unpack_orders(const order_t *g, int8_t *out, size_t n, hipStream_t s);
 
 }  // namespace gpm

@@ -105,8 +105,22 @@ __device__ __forceinline__ int xcd_tile(int b, int nb)
     return t < nb ? t : -1;
 }
 
+// Flag the 5-point neighbourhood of a failing cell for recomputation in the
+// next pass (DetCell, mod_subroutine_MOOD.f90:165-169): each cell enters the
+// next pass's list once, deduplicated by the pass stamp.
+__device__ __forceinline__ void mark_neighbourhood(int i, int j, int N, int stamp, int *mark, int *list, int *cnt)
+{
+    const int ni[5] = {i, i - 1, i, i + 1, i}, nj[5] = {j - 1, j, j, j, j + 1};
+#pragma unroll
+    for (int k = 0; k < 5; ++k) {
+        if (ni[k] < 0 || ni[k] >= N || nj[k] < 0 || nj[k] >= N) continue;
+        const int d = nj[k] * N + ni[k];
+        if (atomicExch(&mark[d], stamp) != stamp) list[atomicAdd(cnt, 1)] = d;
+    }
+}
+
 template <int MORD, int NGP, int STAGE>
-__global__ __launch_bounds__(kStageThreads) void k_stage(StageArgs a)
+__global__ __launch_bounds__(kStageThreads, GPM_STAGE_WAVES_PER_EU) void k_stage(StageArgs a)
 {
     constexpr int H = kHalo<MORD>;
     constexpr int PW = kTX + 2 * H, PH = kTY + 2 * H;
@@ -146,6 +160,7 @@ __global__ __launch_bounds__(kStageThreads) void k_stage(StageArgs a)
 
     const double dtdx = a.tstate[2], dtdy = a.tstate[3];
     const MoodParams P{a.dmp, a.u2, a.u2tol, a.dx, a.dy};
+    const int stamp1 = *a.stamp + 1;
     for (int c = tid; c < kTX * kTY; c += kStageThreads) {
         const int cx = c % kTX, cy = c / kTX;
         const int gi = x0 + cx, gj = y0 + cy;
@@ -162,50 +177,13 @@ __global__ __launch_bounds__(kStageThreads) void k_stage(StageArgs a)
             if (MORD > 1) dec = decide(P, T, acc, cx + H, cy + H, cand);
             const size_t gc = (size_t)gj * N + gi;
             a.Uout[gc] = rk_combine<STAGE>(a.Un, gc, cand);
-            a.gpo[gc] = (int8_t)MORD;
+            // CellGPO := Mord (mod_FE.f90:48); a failing cell takes the cascade at once
+            a.gpo[gc] = pack_order(dec ? lower_order(MORD, a.cascade) : MORD, 0);
+            if (dec) mark_neighbourhood(gi, gj, N, stamp1, a.mark, a.slist1, a.cnt_s1);
         }
-        wave_append(dec, gj * N + gi, a.flist, a.cnt_f);
+        const unsigned long long m = __ballot(dec);
+        if (__lane_id() == 0 && m) atomicAdd(a.cnt_f0, (int)__popcll(m));
     }
-}
-
-// Recompute one cell with the current orders (all four faces) and decide.
-template <int MORD, int NGP, int STAGE>
-__device__ __forceinline__ bool recompute_cell(const MoodArgs &a, int c)
-{
-    const int N = a.N;
-    const int i = c % N, j = c / N;
-    const GlobAcc acc{a.Uin, N, a.bc};
-    const int8_t *__restrict__ g = a.gpo;
-    const int me = g[c];
-    auto ord_of = [&](int ii, int jj) -> int {
-        return (ii < 0 || ii >= N || jj < 0 || jj >= N) ? MORD : (int)g[(size_t)jj * N + ii];
-    };
-    double fyb[4], fxl[4], fxr[4], fyt[4];
-    {
-        const int o = min(ord_of(i, j - 1), me);
-        face_flux_dyn<MORD, NGP, 1>(a.T, acc, o, i, j - 1, i, j, fyb);
-    }
-    {
-        const int o = min(ord_of(i - 1, j), me);
-        face_flux_dyn<MORD, NGP, 0>(a.T, acc, o, i - 1, j, i, j, fxl);
-    }
-    {
-        const int o = min(ord_of(i + 1, j), me);
-        face_flux_dyn<MORD, NGP, 0>(a.T, acc, o, i, j, i + 1, j, fxr);
-    }
-    {
-        const int o = min(ord_of(i, j + 1), me);
-        face_flux_dyn<MORD, NGP, 1>(a.T, acc, o, i, j, i, j + 1, fyt);
-    }
-    double cand[4];
-    apply_fluxes(a.Uin[c], fyb, fxl, fxr, fyt, a.tstate[2], a.tstate[3], cand);
-    bool dec = false;
-    if (me > 1) {
-        const MoodParams P{a.dmp, a.u2, a.u2tol, a.dx, a.dy};
-        dec = decide(P, a.T, acc, i, j, cand);
-    }
-    a.Uout[c] = rk_combine<STAGE>(a.Un, c, cand);
-    return dec;
 }
 
 __device__ __forceinline__ int load_relaxed(const int *p)
@@ -213,66 +191,126 @@ __device__ __forceinline__ int load_relaxed(const int *p)
     return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
+__device__ __forceinline__ order_t load_order(const order_t *p)
+{
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// MOOD correction passes 1, 2, ... of one RK stage (mod_FE.f90:69-174), as a
+// cooperative persistent kernel with one grid barrier per pass.  Pass p
+// recomputes the cells of its list from scratch with the orders they had when
+// the pass began; a cell that fails takes the cascade at once (version p) and
+// flags its neighbourhood for pass p+1.  Each cell is handled by G lanes, one
+// per (face, Gauss point): the lanes reconstruct and solve their Riemann
+// problem, then cross-lane shuffles form the quadrature sum per face and the
+// four-face update in the reference's order on the cell's first lane.
 template <int MORD, int NGP, int STAGE>
 __global__ __launch_bounds__(kMoodThreads) void k_mood(MoodArgs a)
 {
+    constexpr int NGPP = NGP <= 1 ? 1 : NGP <= 2 ? 2 : NGP <= 4 ? 4 : 8;
+    constexpr int G = 4 * NGPP;
+    static_assert(64 % G == 0, "lanes per cell must divide the wavefront");
     cg::grid_group grid = cg::this_grid();
     const int N = a.N;
     const int gtid = blockIdx.x * blockDim.x + threadIdx.x;
     const int gsz = gridDim.x * blockDim.x;
-    const int stamp0 = *a.stamp;
-    int p = 0;
+    const int lane = __lane_id();
+    const int gl = lane % G, gbase = lane - gl;
+    const int f = gl / NGPP, jg = gl % NGPP;
+    const int cells_per_iter = gsz / G;
+    const int stamp_base = *a.stamp;
+    const DevTables *__restrict__ T = a.T;
+    const MoodParams P{a.dmp, a.u2, a.u2tol, a.dx, a.dy};
+    const double dtdx = a.tstate[2], dtdy = a.tstate[3];
+    // face f of a cell (i, j): 0 bottom y-face, 1 left x-face, 2 right x-face, 3 top y-face
+    const int fdir = (f == 1 || f == 2) ? 0 : 1;
+    const int fa = fdir == 0 ? kR : kT, fb = fdir == 0 ? kL : kB;
+    const int oai = f == 1 ? -1 : 0, oaj = f == 0 ? -1 : 0;
+    const int obi = f == 2 ? 1 : 0, obj = f == 3 ? 1 : 0;
+    auto order_of = [&](int ii, int jj, int p) -> int {
+        if (ii < 0 || ii >= N || jj < 0 || jj >= N) return MORD;  // ghost cells keep Mord
+        return order_at_pass(load_order(&a.gpo[(size_t)jj * N + ii]), p, MORD, a.cascade);
+    };
+    // Each cell's (2H+1)^2 neighbourhood of the stage input is staged in LDS by
+    // its G lanes (all loads in flight at once); reconstruction and detection
+    // then read LDS.  H covers the stencils of the cell and its four
+    // neighbours and the detection's 2-cell reach.
+    constexpr int HB = kHalo<MORD>;
+    constexpr int BOX = 2 * HB + 1;
+    constexpr int CPB = kMoodThreads / G;  // cells per workgroup per trip
+    __shared__ double4 nb[CPB][BOX * BOX];
+    const int cs = threadIdx.x / G;
+    const LdsAcc lacc{&nb[cs][0], BOX};
+    int p = 1;
     long long second = 0;
     for (;;) {
-        const int nF = load_relaxed(&a.cnt_f[p]);
-        if (nF == 0) break;
+        const int nS = load_relaxed(&a.cnt_s[p]);
+        if (nS == 0) break;
         if (p + 1 >= a.pmax) {
             if (gtid == 0) atomicOr(a.err, 1);
             break;
         }
-        const int *__restrict__ F = (p & 1) ? a.flist_b : a.flist_a;
-        int *__restrict__ Fn = (p & 1) ? a.flist_a : a.flist_b;
-        const int stamp = stamp0 + p;
-        // phase A: cascade and neighbourhood marking (mod_subroutine_MOOD.f90:148-185)
-        for (int t = gtid; t < nF; t += gsz) {
-            const int c = F[t];
-            const int i = c % N, j = c / N;
-            a.gpo[c] = (int8_t)lower_order(a.gpo[c], a.cascade);
-            const int ni[5] = {i, i - 1, i, i + 1, i}, nj[5] = {j - 1, j, j, j, j + 1};
+        second += nS;
+        const int *__restrict__ S = a.slist[p & 1];
+        int *Sn = a.slist[(p + 1) & 1];
+        const int stamp_next = stamp_base + p + 1;
+        const int niter = (nS + cells_per_iter - 1) / cells_per_iter;
+        for (int it = 0; it < niter; ++it) {
+            const int t = it * cells_per_iter + gtid / G;
+            const bool valid = t < nS;
+            const int d = valid ? S[t] : 0;
+            const int i = d % N, j = d / N;
+            __syncthreads();  // the previous trip's readers are done with nb
+            for (int q = gl; q < BOX * BOX; q += G)
+                nb[cs][q] = load_bc(a.Uin, N, a.bc, i - HB + q % BOX, j - HB + q / BOX);
+            __syncthreads();
+            double F[4] = {0.0, 0.0, 0.0, 0.0};
+            if (valid && jg < NGP) {
+                const int ord = min(order_of(i + oai, j + oaj, p), order_of(i + obi, j + obj, p));
+                face_gp_flux_dyn<MORD>(T, lacc, ord, fa, fb, fdir, jg, HB + oai, HB + oaj, HB + obi, HB + obj, F);
+            }
+            // quadrature over the face's Gauss points, j = 1..ngp from 0
+            double L[4] = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
-            for (int k = 0; k < 5; ++k) {
-                if (ni[k] < 0 || ni[k] >= N || nj[k] < 0 || nj[k] >= N) continue;
-                const int d = nj[k] * N + ni[k];
-                if (atomicExch(&a.mark[d], stamp) != stamp) {
-                    const int slot = atomicAdd(&a.cnt_s[p], 1);
-                    a.slist[slot] = d;
+            for (int jj = 0; jj < NGP; ++jj) {
+                const int src = gbase + f * NGPP + jj;
+                const double w = T->gw[jj];
+#pragma unroll
+                for (int k = 0; k < 4; ++k) L[k] = L[k] + __shfl(F[k], src) * w;
+            }
+            double fyb[4], fxl[4], fxr[4], fyt[4];
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                fyb[k] = __shfl(L[k], gbase);
+                fxl[k] = __shfl(L[k], gbase + NGPP);
+                fxr[k] = __shfl(L[k], gbase + 2 * NGPP);
+                fyt[k] = __shfl(L[k], gbase + 3 * NGPP);
+            }
+            if (valid && gl == 0) {
+                double cand[4];
+                apply_fluxes(lacc(HB, HB), fyb, fxl, fxr, fyt, dtdx, dtdy, cand);
+                const int me = order_at_pass(load_order(&a.gpo[d]), p, MORD, a.cascade);
+                bool dec = false;
+                if (me > 1) dec = decide(P, T, lacc, HB, HB, cand);
+                a.Uout[d] = rk_combine<STAGE>(a.Un, d, cand);
+                if (dec) {
+                    __hip_atomic_store(&a.gpo[d], pack_order(lower_order(me, a.cascade), p), __ATOMIC_RELAXED,
+                                       __HIP_MEMORY_SCOPE_AGENT);
+                    atomicAdd(&a.cnt_f[p], 1);
+                    mark_neighbourhood(i, j, N, stamp_next, a.mark, Sn, &a.cnt_s[p + 1]);
                 }
             }
-        }
-        grid.sync();
-        // phase B: recompute the marked cells and detect
-        const int nS = load_relaxed(&a.cnt_s[p]);
-        second += nS;
-        const int nloop = (nS + gsz - 1) / gsz * gsz;
-        for (int t = gtid; t < nloop; t += gsz) {
-            bool dec = false;
-            int d = 0;
-            if (t < nS) {
-                d = a.slist[t];
-                dec = recompute_cell<MORD, NGP, STAGE>(a, d);
-            }
-            wave_append(dec, d, Fn, &a.cnt_f[p + 1]);
         }
         grid.sync();
         ++p;
     }
     if (gtid == 0) {
-        *a.stamp = stamp0 + p + 1;
+        *a.stamp = stamp_base + p + 2;
         long long det = 0;
-        for (int q = 0; q <= p; ++q) det += a.cnt_f[q];
+        for (int q = 0; q < p; ++q) det += a.cnt_f[q];
         DevStats *st = a.stats;
-        st->passes[a.stage_idx] = p + 1;
-        if (p + 1 > st->max_passes) st->max_passes = p + 1;
+        st->passes[a.stage_idx] = p;
+        if (p > st->max_passes) st->max_passes = p;
         st->detected_step += det;
         st->detected_total += det;
         st->second_pass_cells += second;

@@ -117,5 +117,44 @@ def main(argv):
         print("wrote", name)
 
 
+
+
+def long_case(n=2048, steps=500, seed=0, pin_steps=2, threads=8):
+    """Config-1 workload (2048^2, R3 = Mord 7/ngp 4, reflective, blast IC, 500
+    steps).  The reference needs about 3 h for it on one core, so: (1) the
+    reference and the C restatement (oracle/gpm_oracle.c) are run for
+    `pin_steps` steps and must agree bit for bit; (2) the C restatement runs
+    the 500 steps; its dt sequence, counts and checksums are stored."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+    os.environ.setdefault("OMP_NUM_THREADS", str(threads))
+    cfg = "r3_n2048_ref"
+    U0 = ic.make("blast", n, seed)
+    r = run_ref(cfg, U0, pin_steps, 0)
+    t = r["tables"]
+    orc = O.Oracle(n, 7, 4, "reflective", {k: t[k] for k in ("zT", "d2x", "d2y", "gw", "ixiy")})
+    U = orc.full(U0)
+    ro = orc.run(U, pin_steps, masks=True)
+    assert np.array_equal(ro["dt"], r["steps"]["dt"]), "oracle dt differs from the reference"
+    assert np.array_equal(ro["masks"], r["masks"]), "oracle masks differ from the reference"
+    assert np.array_equal(U, r["U"]), "oracle state differs from the reference"
+    print(f"oracle == reference for {pin_steps} steps at {n}^2", flush=True)
+    ro2 = orc.run(U, steps - pin_steps, t=ro["t"], masks=False)
+    dts = np.concatenate([ro["dt"], ro2["dt"]])
+    counts = np.concatenate([ro["counts"], ro2["counts"]])
+    Ui = np.ascontiguousarray(U[4:-4, 4:-4])
+    g = np.zeros((n, n), dtype=np.int8)
+    np.savez_compressed(
+        os.path.join(OUT, f"hash_r3_n{n}_ref_bl{seed}_{steps}.npz"), n=n, mord=7, ngp=4, bc="reflective", dmp=True,
+        u2=True, u2tol=True, kind="blast", seed=seed, nsteps=steps, pin_steps=pin_steps, ic_sha=sha(U0), dt=dts,
+        counts=counts, U_sha=sha(Ui), U_sum=Ui.sum(axis=(0, 1)), U_abs_sum=np.abs(Ui).sum(axis=(0, 1)),
+        U_pin_sha=sha(np.ascontiguousarray(r["U"][4:-4, 4:-4])), mask_pin=r["masks"][-1], dt_pin=r["steps"]["dt"],
+        **{k: v for k, v in t.items()})
+    print("wrote long case", flush=True)
+
+
 if __name__ == "__main__":
-    main(sys.argv[1:])
+    if sys.argv[1:2] == ["--long"]:
+        long_case()
+    else:
+        main(sys.argv[1:])
