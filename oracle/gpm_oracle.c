@@ -547,3 +547,6 @@ int orc_step_stages(orc_t *o, double *U, double dt, double *stage_out /* 3*(N+8)
 
 long orc_count_detected(const orc_t *o) { return o->count_detected; }
 void orc_passes(const orc_t *o, int *p) { for (int k = 0; k < 3; ++k) p[k] = o->passes[k]; }
+
+/* The Riemann solver alone, for unit checks (dir 0 = x, 1 = y). */
+void orc_hllc(const double *ul, const double *ur, int dir, double *F) { hllc(ul, ur, dir, F); }

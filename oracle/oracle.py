@@ -34,6 +34,7 @@ def lib():
         L.orc_count_detected.restype = ctypes.c_long
         L.orc_count_detected.argtypes = [P]
         L.orc_passes.argtypes = [P, P]
+        L.orc_hllc.argtypes = [P, P, ctypes.c_int, P]
         _lib = L
     return _lib
 
@@ -94,3 +95,12 @@ class Oracle:
         p = np.zeros(3, dtype=np.int32)
         lib().orc_passes(self._h, _ptr(p))
         return outs, masks, p
+
+
+def hllc(ul, ur, direction):
+    """HLLC flux of the oracle (mod_HLLC.f90:11-169) for one face."""
+    ul = np.ascontiguousarray(ul, dtype=np.float64)
+    ur = np.ascontiguousarray(ur, dtype=np.float64)
+    F = np.zeros(4)
+    lib().orc_hllc(_ptr(ul), _ptr(ur), int(direction), _ptr(F))
+    return F
