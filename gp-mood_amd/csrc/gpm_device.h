@@ -309,9 +309,11 @@ __device__ __forceinline__ bool shock_detector(const MoodParams &P, const Acc &a
 // DETECTION loop-1 decision of one cell (mod_subroutine_MOOD.f90:41-131) for a
 // cell whose order is above 1: PAD, then the shock-gated DMP with plateau test
 // and U2 smooth-extrema relaxation.  `cand` is the cell's candidate Uout.
+// Kept out of line (it runs only for DMP failures); its arguments are passed by
+// value so that the call does not make every thread build a stack frame.
 template <class Acc>
-__device__ __noinline__ bool u2_relax(const MoodParams &P, const DevTables *__restrict__ T, const Acc &acc, int i,
-                                      int j)
+__device__ __noinline__ bool u2_relax(int u2tol, double dx, double dy, const DevTables *__restrict__ T,
+                                      const Acc acc, int i, int j)
 {
     double D2x[5], D2y[5];  // (-1,0), (0,0), (1,0), (0,-1), (0,1)
     const int oi[5] = {-1, 0, 1, 0, 0}, oj[5] = {0, 0, 0, -1, 1};
@@ -333,7 +335,7 @@ __device__ __noinline__ bool u2_relax(const MoodParams &P, const DevTables *__re
     const double Ymin = fmin2(fmin2(fmin2(fmin2(D2y[0], D2y[1]), D2y[2]), D2y[3]), D2y[4]);
     const double Ymax = fmax2(fmax2(fmax2(fmax2(D2y[0], D2y[1]), D2y[2]), D2y[3]), D2y[4]);
     double delta = 0.0;
-    if (P.u2tol) delta = fmax2(P.dx, P.dy);
+    if (u2tol) delta = fmax2(dx, dy);
     if ((Xmin * Xmax > -delta) && ((fmax2(fabs(Xmin), fabs(Xmax)) < delta) || (fabs(Xmin / Xmax) >= 0.5)))
         return true;
     if ((Ymin * Ymax > -delta) && ((fmax2(fabs(Ymin), fabs(Ymax)) < delta) || (fabs(Ymin / Ymax) >= 0.5)))
@@ -357,7 +359,7 @@ __device__ __forceinline__ bool decide(const MoodParams &P, const DevTables *__r
     const double Mm = mx - mn;
     if (!(Mm > P.dx * P.dx * P.dx)) return false;
     if (!(cand[0] > mx || cand[0] < mn)) return false;
-    if (P.u2 && u2_relax(P, T, acc, i, j)) return false;
+    if (P.u2 && u2_relax(P.u2tol, P.dx, P.dy, T, acc, i, j)) return false;
     return true;
 }
 

@@ -13,21 +13,24 @@ __global__ __launch_bounds__(256) void k_dt(DtArgs a)
     const int tid = threadIdx.x;
     double m = 1.0e30;
     if (a.mode == kDtCompute) {
-        // each thread takes 4 cells per trip so that their loads and division
-        // chains overlap
+        // four coalesced cells per thread and trip, so that loads and division
+        // chains of independent cells overlap
         const size_t ncell = (size_t)a.N * a.N;
-        const size_t stride = (size_t)gridDim.x * blockDim.x * 4;
-        for (size_t c0 = ((size_t)blockIdx.x * blockDim.x + tid) * 4; c0 < ncell; c0 += stride) {
+        const size_t gs = (size_t)gridDim.x * blockDim.x;
+        for (size_t c0 = (size_t)blockIdx.x * blockDim.x + tid; c0 < ncell; c0 += 4 * gs) {
             double4 u[4];
 #pragma unroll
-            for (int q = 0; q < 4; ++q) u[q] = (c0 + q < ncell) ? a.U[c0 + q] : make_double4(1.0, 0.0, 0.0, 1.0);
+            for (int q = 0; q < 4; ++q) {
+                const size_t c = c0 + q * gs;
+                u[q] = c < ncell ? a.U[c] : make_double4(1.0, 0.0, 0.0, 1.0);
+            }
 #pragma unroll
             for (int q = 0; q < 4; ++q) {
                 const double vx = u[q].y / u[q].x, vy = u[q].z / u[q].x;
                 const double pres = pressure(u[q].x, u[q].y, u[q].z, u[q].w);
                 const double cs = sqrt(kGamma * pres / u[q].x);
                 const double v = 1.0 / (1.0 / (a.dx / (fabs(vx) + cs)) + 1.0 / (a.dy / (fabs(vy) + cs)));
-                if (c0 + q < ncell) m = fmin2(m, v);
+                if (c0 + q * gs < ncell) m = fmin2(m, v);
             }
         }
     }
