@@ -1,0 +1,64 @@
+! Drop-in replacement of the reference's time_stepper module
+! (gp-mood-code/time_stepper.f90:1-25): same module and subroutine names and
+! the same signature time_stepping(Uin, Uout), so main.f90, mod_*.f90 and the
+! parameters.f90 / problems interface compile against it unchanged.  The SSP-RK3
+! step (mod_SSP_RK3.f90:12-39) runs on the GPU through libgpmood; the GP tables
+! are the ones GP_presim_sphere left in global_variables, so both codes use the
+! same weights.  Link with -lgpmood.
+module time_stepper
+   use iso_c_binding
+   use parameters
+   use constants
+   use global_variables
+   use gpm_capi
+   implicit none
+
+   type(c_ptr), save :: gpm_handle = c_null_ptr
+   integer(c_int32_t), allocatable, save :: gpm_order(:,:)
+
+contains
+
+   subroutine gpm_open()
+      type(gpm_config) :: cfg
+      integer(c_int) :: rc
+      integer(c_int32_t), allocatable :: ixiy32(:,:,:)
+      cfg%lf = lf; cfg%nf = nf; cfg%mord = Mord; cfg%ngp = ngp
+      cfg%method = method; cfg%integrator = integrator; cfg%num_flux = numFlux; cfg%bc_type = BC_type
+      cfg%dmp = merge(1, 0, DMP); cfg%u2 = merge(1, 0, U2); cfg%u2_tol = merge(1, 0, U2_tol)
+      cfg%cascade = GPM_CASCADE_REFERENCE
+      cfg%cfl = CFL; cfg%lx = Lx; cfg%ly = Ly; cfg%ell_o_dx = real(ell_o_dx, c_double)
+      rc = gpm_create(cfg, gpm_handle)
+      if (rc /= 0) then
+         print *, 'gpm_create failed: ', rc
+         stop 1
+      end if
+      allocate(ixiy32(7, 25, 2))
+      ixiy32 = int(ixiy_sp, c_int32_t)
+      rc = gpm_set_gp_tables(gpm_handle, zT_sp, GP_d2x_sp, GP_d2y_sp, gauss_weight, ixiy32)
+      if (rc /= 0) then
+         print *, 'gpm_set_gp_tables failed: ', trim(gpm_error_message(gpm_handle))
+         stop 1
+      end if
+      allocate(gpm_order(lb:le, nb:ne))
+   end subroutine gpm_open
+
+   subroutine time_stepping(Uin, Uout)
+      real(PR), intent(inout), dimension(4,lb:le, nb:ne) :: Uin
+      real(PR), intent(out  ), dimension(4,lb:le, nb:ne) :: Uout
+      type(gpm_stats) :: st
+      integer(c_int) :: rc
+
+      if (.not. c_associated(gpm_handle)) call gpm_open()
+      t_rk = t
+      rc = gpm_time_stepping(gpm_handle, Uin, Uout, t, dt, gpm_order, st)
+      if (rc /= 0) then
+         print *, 'gpm_time_stepping failed: ', trim(gpm_error_message(gpm_handle))
+         stop 1
+      end if
+      CellGPO = gpm_order
+      count_detected_cell = int(st%count_detected_cell)
+      count_detected_cell_a_posteriori = int(st%count_detected_cell)
+      count_detected_cell_a_priori = 0
+   end subroutine time_stepping
+
+end module time_stepper

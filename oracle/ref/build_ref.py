@@ -75,6 +75,10 @@ CONFIGS = {
     "r3_n2048_ref": (2048, 7, 4, "reflective", True, True, True),
 }
 
+# configs for which the Fortran drop-in (reference modules + product
+# time_stepper over libgpmood) is also built
+DROPIN_CONFIGS = ["r2_n64_neu", "r3_n64_ref"]
+
 PARAMS_TEMPLATE = """module parameters
    use constants
    implicit none
@@ -113,9 +117,13 @@ def fbool(b):
     return ".true." if b else ".false."
 
 
-def build(name):
+def build(name, dropin=False):
+    """Build the reference solver for config `name`.  With dropin=True the
+    reference's time_stepper.f90 is replaced by the product's Fortran drop-in
+    (gp-mood_amd/fortran/time_stepper.f90 over gpm_capi.f90), which runs the
+    step through libgpmood; everything else is the reference's own code."""
     n, mord, ngp, bc, dmp, u2, u2tol = CONFIGS[name]
-    out = os.path.join(OUT_ROOT, name)
+    out = os.path.join(OUT_ROOT, name + ("_dropin" if dropin else ""))
     os.makedirs(out, exist_ok=True)
     params = os.path.join(out, "parameters.f90")
     with open(params, "w") as f:
@@ -127,7 +135,16 @@ def build(name):
     objs = []
     tmpdir = tempfile.mkdtemp(prefix="gpm_ref_pp_")
     try:
-        for mod in MODULES:
+        shim = os.path.join(os.path.dirname(OUT_ROOT), "..", "gp-mood_amd", "fortran")
+        for mod in MODULES + (["__shim__"] if dropin else []):
+            if dropin and mod == "time_stepper":
+                continue
+            if mod == "__shim__":
+                for f in ("gpm_capi", "time_stepper"):
+                    obj = os.path.join(out, f + "_shim.o")
+                    subprocess.run([FC] + flags + ["-c", os.path.join(shim, f + ".f90"), "-o", obj], check=True, cwd=out)
+                    objs.append(obj)
+                continue
             src = params if mod == "parameters" else os.path.join(REF_SRC, mod + ".f90")
             obj = os.path.join(out, mod + ".o")
             extra = []
@@ -144,8 +161,11 @@ def build(name):
         drv = os.path.join(HERE, "ref_driver.f90")
         dobj = os.path.join(out, "ref_driver.o")
         subprocess.run([FC] + flags + ["-c", drv, "-o", dobj], check=True, cwd=out)
-        exe = os.path.join(out, "gpm_ref.exe")
+        exe = os.path.join(out, "gpm_dropin.exe" if dropin else "gpm_ref.exe")
         link = [FC] + [f for f in flags if f not in ("-J", "-I", out)] + objs + [dobj, "-o", exe]
+        if dropin:
+            libdir = os.path.abspath(os.path.join(shim, ".."))
+            link += ["-L" + libdir, "-lgpmood", "-Wl,-rpath,$ORIGIN/../../../gp-mood_amd"]
         subprocess.run(link, check=True, cwd=out)
     finally:
         shutil.rmtree(tmpdir, ignore_errors=True)
@@ -159,9 +179,12 @@ def main(argv):
     if not os.path.isdir(REF_SRC):
         print("reference sources not present; nothing to build", file=sys.stderr)
         return 0
-    names = argv or list(CONFIGS)
-    with ThreadPoolExecutor(max_workers=min(8, len(names))) as ex:
-        for exe in ex.map(build, names):
+    names = [a for a in argv if not a.startswith("--")] or list(CONFIGS)
+    jobs = [(nm, False) for nm in names]
+    if "--no-dropin" not in argv:
+        jobs += [(nm, True) for nm in DROPIN_CONFIGS if nm in names]
+    with ThreadPoolExecutor(max_workers=min(8, len(jobs))) as ex:
+        for exe in ex.map(lambda j: build(*j), jobs):
             print("built", exe)
     return 0
 
