@@ -328,7 +328,7 @@ int gpm_create(const gpm_config *cfg, gpm_ctx **out)
         c->mood_blocks[st] = want < per * ncu ? want : per * ncu;
     }
     hipMemsetAsync(c->mark, 0, n * sizeof(int), c->stream);
-    hipMemsetD16Async(c->gpo, (unsigned short)c->mord, n, c->stream);
+    hipMemsetD16Async(c->gpo, pack_order(c->mord, c->mord, 0), n, c->stream);
     hipMemsetAsync(c->counters, 0, 3 * kCountersPerStage * sizeof(int), c->stream);
     hipMemsetAsync(c->stats, 0, sizeof(DevStats), c->stream);
     hipMemsetAsync(c->tstate, 0, 4 * sizeof(double), c->stream);
@@ -396,7 +396,7 @@ int gpm_set_state(gpm_ctx *c, const double *U)
     std::vector<double> in((size_t)c->N * c->N * 4);
     extract_interior(U, in.data(), c->N);
     HIPCHK(c, hipMemcpyAsync(c->dU, in.data(), in.size() * sizeof(double), hipMemcpyHostToDevice, c->stream));
-    HIPCHK(c, hipMemsetD16Async(c->gpo, (unsigned short)c->mord, (size_t)c->N * c->N, c->stream));
+    HIPCHK(c, hipMemsetD16Async(c->gpo, pack_order(c->mord, c->mord, 0), (size_t)c->N * c->N, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
     c->has_state = true;
     return GPM_OK;
@@ -419,7 +419,7 @@ int gpm_get_state(gpm_ctx *c, double *U, int32_t *cellgpo)
             for (int ii = 0; ii < W; ++ii) {
                 const int i = ii - 4, j = jj - 4;
                 cellgpo[(size_t)jj * W + ii] =
-                    (i < 0 || i >= c->N || j < 0 || j >= c->N) ? c->mord : (int32_t)(g[(size_t)j * c->N + i] & 0xF);
+                    (i < 0 || i >= c->N || j < 0 || j >= c->N) ? c->mord : (int32_t)order_of_word(g[(size_t)j * c->N + i]);
             }
     }
     return GPM_OK;
@@ -430,7 +430,7 @@ int gpm_set_state_device(gpm_ctx *c, const double *u_dev)
     if (!c || !u_dev) return fail(c, GPM_EINVAL, "null argument");
     const size_t n = (size_t)c->N * c->N;
     HIPCHK(c, hipMemcpyAsync(c->dU, u_dev, n * 4 * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
-    HIPCHK(c, hipMemsetD16Async(c->gpo, (unsigned short)c->mord, n, c->stream));
+    HIPCHK(c, hipMemsetD16Async(c->gpo, pack_order(c->mord, c->mord, 0), n, c->stream));
     c->has_state = true;
     return GPM_OK;
 }

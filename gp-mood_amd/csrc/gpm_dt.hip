@@ -89,7 +89,7 @@ __global__ __launch_bounds__(256) void k_dt(DtArgs a)
 __global__ void k_unpack_orders(const order_t *__restrict__ g, int8_t *__restrict__ out, size_t n)
 {
     for (size_t c = (size_t)blockIdx.x * blockDim.x + threadIdx.x; c < n; c += (size_t)gridDim.x * blockDim.x)
-        out[c] = (int8_t)(g[c] & 0xF);
+        out[c] = (int8_t)order_of_word(g[c]);
 }
 
 hipError_t launch_unpack_orders(const order_t *g, int8_t *out, size_t n, hipStream_t s)

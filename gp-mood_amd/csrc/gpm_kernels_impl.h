@@ -178,7 +178,7 @@ __global__ __launch_bounds__(kStageThreads, GPM_STAGE_WAVES_PER_EU) void k_stage
             const size_t gc = (size_t)gj * N + gi;
             a.Uout[gc] = rk_combine<STAGE>(a.Un, gc, cand);
             // CellGPO := Mord (mod_FE.f90:48); a failing cell takes the cascade at once
-            a.gpo[gc] = pack_order(dec ? lower_order(MORD, a.cascade) : MORD, 0);
+            a.gpo[gc] = pack_order(dec ? lower_order(MORD, a.cascade) : MORD, MORD, 0);
             if (dec) mark_neighbourhood(gi, gj, N, stamp1, a.mark, a.slist1, a.cnt_s1);
         }
         const unsigned long long m = __ballot(dec);
@@ -229,7 +229,7 @@ __global__ __launch_bounds__(kMoodThreads) void k_mood(MoodArgs a)
     const int obi = f == 2 ? 1 : 0, obj = f == 3 ? 1 : 0;
     auto order_of = [&](int ii, int jj, int p) -> int {
         if (ii < 0 || ii >= N || jj < 0 || jj >= N) return MORD;  // ghost cells keep Mord
-        return order_at_pass(load_order(&a.gpo[(size_t)jj * N + ii]), p, MORD, a.cascade);
+        return order_at_pass(load_order(&a.gpo[(size_t)jj * N + ii]), p);
     };
     // Each cell's (2H+1)^2 neighbourhood of the stage input is staged in LDS by
     // its G lanes (all loads in flight at once); reconstruction and detection
@@ -289,12 +289,12 @@ __global__ __launch_bounds__(kMoodThreads) void k_mood(MoodArgs a)
             if (valid && gl == 0) {
                 double cand[4];
                 apply_fluxes(lacc(HB, HB), fyb, fxl, fxr, fyt, dtdx, dtdy, cand);
-                const int me = order_at_pass(load_order(&a.gpo[d]), p, MORD, a.cascade);
+                const int me = order_at_pass(load_order(&a.gpo[d]), p);
                 bool dec = false;
                 if (me > 1) dec = decide(P, T, lacc, HB, HB, cand);
                 a.Uout[d] = rk_combine<STAGE>(a.Un, d, cand);
                 if (dec) {
-                    __hip_atomic_store(&a.gpo[d], pack_order(lower_order(me, a.cascade), p), __ATOMIC_RELAXED,
+                    __hip_atomic_store(&a.gpo[d], pack_order(lower_order(me, a.cascade), me, p), __ATOMIC_RELAXED,
                                        __HIP_MEMORY_SCOPE_AGENT);
                     atomicAdd(&a.cnt_f[p], 1);
                     mark_neighbourhood(i, j, N, stamp_next, a.mark, Sn, &a.cnt_s[p + 1]);
