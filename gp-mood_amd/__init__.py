@@ -15,6 +15,7 @@ from dataclasses import dataclass
 import numpy as np
 
 from . import capi, ic  # noqa: F401
+from .nn import MLP  # noqa: F401
 from .capi import gp_tables  # noqa: F401
 
 NGC = 4
@@ -168,3 +169,28 @@ class Solver:
 
     def stream(self):
         return self.lib.gpm_stream(self._h)
+
+    # -- learned a priori predictor (NN-GP-MOOD) ----------------------------
+    def set_predictor(self, mlp, gate=True):
+        """Attach an :class:`MLP` (None detaches): every RK stage starts from
+        its predicted orders; the a posteriori check still runs."""
+        self._predictor = mlp
+        self._check(self.lib.gpm_set_predictor(self._h, mlp._h if mlp is not None else None, int(gate)),
+                    "gpm_set_predictor")
+
+    def set_apriori_orders(self, orders):
+        """Fixed a priori CellGPO (int8 [nf][lf]) for every stage, or None."""
+        arr = None if orders is None else np.ascontiguousarray(orders, dtype=np.int8)
+        self._check(self.lib.gpm_set_apriori_orders(self._h, capi.ptr(arr)), "gpm_set_apriori_orders")
+
+    def set_dump(self, x_ptr, y_ptr, capacity, count_ptr, keep_mord=0.05, seed=0, gated_only=True, stage_mask=1):
+        """Training-data dump into device buffers (see include/gpmood.h)."""
+        self._check(self.lib.gpm_set_dump(self._h, ctypes.c_void_p(x_ptr) if x_ptr else None,
+                                          ctypes.c_void_p(y_ptr) if y_ptr else None, int(capacity),
+                                          ctypes.c_void_p(count_ptr) if count_ptr else None, float(keep_mord),
+                                          int(seed) & ((1 << 64) - 1), int(gated_only), int(stage_mask)),
+                    "gpm_set_dump")
+
+    def features(self, cells_ptr, n, x_ptr):
+        self._check(self.lib.gpm_features(self._h, ctypes.c_void_p(cells_ptr), int(n), ctypes.c_void_p(x_ptr)),
+                    "gpm_features")

@@ -31,6 +31,10 @@ EXPORTS = [
     "gpm_set_state", "gpm_get_state", "gpm_set_state_device", "gpm_get_state_device",
     "gpm_compute_dt", "gpm_time_stepping", "gpm_run", "gpm_get_stats", "gpm_reset_stats",
     "gpm_set_time", "gpm_stream", "gpm_set_timing", "gpm_kernel_time",
+    "gpm_mlp_create", "gpm_mlp_destroy", "gpm_mlp_last_error", "gpm_mlp_num_params", "gpm_mlp_set_params",
+    "gpm_mlp_get_params", "gpm_mlp_get_grads", "gpm_mlp_forward_backward", "gpm_mlp_adamw_step",
+    "gpm_mlp_train_step", "gpm_mlp_logits", "gpm_set_predictor", "gpm_set_dump", "gpm_features",
+    "gpm_set_apriori_orders", "gpm_get_apriori_orders",
 ]
 
 
@@ -93,6 +97,22 @@ def load():
         "gpm_set_timing": ([P, I], I),
         "gpm_kernel_time": ([P, ctypes.POINTER(D), ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(D),
                              ctypes.POINTER(ctypes.c_int64)], I),
+        "gpm_mlp_create": ([I, I, I, ctypes.POINTER(P)], I),
+        "gpm_mlp_destroy": ([P], I),
+        "gpm_mlp_last_error": ([P], ctypes.c_char_p),
+        "gpm_mlp_num_params": ([P], ctypes.c_int64),
+        "gpm_mlp_set_params": ([P, P], I),
+        "gpm_mlp_get_params": ([P, P], I),
+        "gpm_mlp_get_grads": ([P, P], I),
+        "gpm_mlp_forward_backward": ([P, P, P, I, ctypes.POINTER(D), ctypes.POINTER(D)], I),
+        "gpm_mlp_adamw_step": ([P, D, D, D, D, D], I),
+        "gpm_mlp_train_step": ([P, P, P, I, D, D, ctypes.POINTER(D), ctypes.POINTER(D)], I),
+        "gpm_mlp_logits": ([P, P, I, P], I),
+        "gpm_set_predictor": ([P, P, I], I),
+        "gpm_set_dump": ([P, P, P, ctypes.c_int64, P, D, ctypes.c_uint64, I, I], I),
+        "gpm_features": ([P, P, I, P], I),
+        "gpm_set_apriori_orders": ([P, P], I),
+        "gpm_get_apriori_orders": ([P, P], I),
     }
     for name, (args, res) in sig.items():
         fn = getattr(L, name)

@@ -16,6 +16,7 @@
 #include "../../include/gpmood.h"
 #include "gpm_internal.h"
 #include "gpm_kernels.h"
+#include "gpm_mlp.h"
 
 using namespace gpm;
 
@@ -43,6 +44,21 @@ struct gpm_ctx {
     std::vector<std::pair<int, int>> ev_pairs;  // (start index, kind 0 stage / 1 mood)
     size_t ev_used = 0;
     std::string msg;
+    // learned a priori predictor (not owned)
+    gpm_mlp *nn = nullptr;
+    int nn_gate = 1;
+    bool fixed_orders = false;  // a priori orders given by the caller (pred holds them)
+    int8_t *pred = nullptr;
+    int *nnlist = nullptr, *nncount = nullptr;
+    // training-data dump
+    float *dump_x = nullptr;
+    int32_t *dump_y = nullptr;
+    long long dump_cap = 0;
+    unsigned long long *dump_count = nullptr;
+    double dump_keep = 0.0;
+    unsigned long long dump_seed = 0;
+    int dump_gated = 1, dump_stages = 0;
+    long long step_counter = 0;
 };
 
 namespace {
@@ -204,7 +220,19 @@ int enqueue_stages(gpm_ctx *c)
         const double4 *in = stage == 1 ? c->dU : (stage == 2 ? c->dA : c->dB);
         double4 *out = stage == 1 ? c->dA : (stage == 2 ? c->dB : c->dA);
         int *cnt = c->counters + (stage - 1) * kCountersPerStage;
+        if (c->nn) {
+            // a priori orders of this stage's input (compute_CellGPO_with_NN, mod_NN.f90:14-58)
+            HIPCHK(c, hipMemsetAsync(c->nncount, 0, sizeof(int), c->stream));
+            HIPCHK(c, mlp_launch_gate(in, c->N, c->bc, c->dx, c->dy, c->nn_gate, c->mord, c->gpo, c->pred, c->nnlist,
+                                      c->nncount, c->stream));
+            HIPCHK(c, mlp_launch_predict(in, c->N, c->bc, c->nnlist, c->nncount, mlp_device_view(c->nn),
+                                         mlp_stencil(c->nn), mlp_classes(c->nn), c->mord, c->gpo, c->pred,
+                                         &c->stats->a_priori_lowered, c->stream));
+        }
+        if (c->fixed_orders && !c->nn) HIPCHK(c, launch_orders_from_pred(c->pred, c->gpo, (size_t)c->N * c->N, c->stream));
         StageArgs s{};
+        s.apriori = c->nn != nullptr || c->fixed_orders;
+        s.pred = c->pred;
         s.Uin = in; s.Un = c->dU; s.Uout = out; s.gpo = c->gpo;
         s.slist1 = c->slistB; s.cnt_s1 = cnt + kMaxPasses + 1 + 1; s.cnt_f0 = cnt;
         s.mark = c->mark; s.stamp = c->stamp; s.T = c->dT; s.tstate = c->tstate;
@@ -242,8 +270,16 @@ int enqueue_stages(gpm_ctx *c)
         HIPCHK(c, hipLaunchCooperativeKernel(mood_kernel(c->mord, c->ngp, stage), dim3(c->mood_blocks[stage - 1]),
                                              dim3(kMoodThreads), args, 0, c->stream));
         if (c->timing) HIPCHK(c, hipEventRecord(e1, c->stream));
+        if (c->dump_x && (c->dump_stages & (1 << (stage - 1)))) {
+            // (stencil of the stage input, final order of the stage) pairs
+            const int S = 2 * ((c->mord - 1) / 2) + 1;
+            HIPCHK(c, mlp_launch_dump(in, c->N, c->bc, c->dx, c->dy, c->gpo, c->mord, S, (float)c->dump_keep,
+                                      c->dump_seed ^ (0x632BE59BD9B4E019ull * (unsigned long long)(c->step_counter * 3 + stage)),
+                                      c->dump_gated, c->dump_x, c->dump_y, c->dump_cap, c->dump_count, c->stream));
+        }
     }
     std::swap(c->dU, c->dA);
+    ++c->step_counter;
     return GPM_OK;
 }
 
@@ -279,7 +315,8 @@ int gpm_create(const gpm_config *cfg, gpm_ctx **out)
     if (cfg->lx != cfg->ly) return bad(GPM_EINVAL, "square cells only (Lx = Ly)");
     if (cfg->integrator != GPM_SSP_RK3) return bad(GPM_EUNSUP, "integrator must be SSP_RK3");
     if (cfg->num_flux != GPM_HLLC) return bad(GPM_EUNSUP, "numFlux must be HLLC");
-    if (cfg->method != GPM_GP_MOOD) return bad(GPM_EUNSUP, "method must be GP_MOOD");
+    if (cfg->method != GPM_GP_MOOD && cfg->method != GPM_NN_GP_MOOD)
+        return bad(GPM_EUNSUP, "method must be GP_MOOD or NN_GP_MOOD");
     if (cfg->bc_type != GPM_NEUMANN && cfg->bc_type != GPM_PERIODIC && cfg->bc_type != GPM_REFLECTIVE)
         return bad(GPM_EUNSUP, "BC_type must be Neumann, Periodic or Reflective");
     if (cfg->cascade != GPM_CASCADE_REFERENCE && cfg->cascade != GPM_CASCADE_FULL)
@@ -300,7 +337,9 @@ int gpm_create(const gpm_config *cfg, gpm_ctx **out)
               dalloc(&c->mark, n) == hipSuccess &&
               dalloc(&c->counters, 3 * kCountersPerStage) == hipSuccess && dalloc(&c->stamp, 1) == hipSuccess &&
               dalloc(&c->err, 1) == hipSuccess && dalloc(&c->stats, 1) == hipSuccess &&
-              dalloc(&c->tstate, 4) == hipSuccess && dalloc(&c->done, 1) == hipSuccess;
+              dalloc(&c->tstate, 4) == hipSuccess && dalloc(&c->done, 1) == hipSuccess &&
+              dalloc(&c->pred, n) == hipSuccess && dalloc(&c->nnlist, n) == hipSuccess &&
+              dalloc(&c->nncount, 1) == hipSuccess;
     if (!ok) {
         gpm_destroy(c);
         std::fprintf(stderr, "gpm_create: device allocation failed\n");
@@ -365,7 +404,8 @@ int gpm_destroy(gpm_ctx *c)
     if (!c) return GPM_OK;
     if (c->stream) hipStreamSynchronize(c->stream);
     void *bufs[] = {c->dT, c->dU, c->dA, c->dB, c->gpo, c->slistA, c->slistB, c->mark, c->counters,
-                    c->stamp, c->err, c->stats, c->tstate, c->block_min, c->dt_log, c->done};
+                    c->stamp, c->err, c->stats, c->tstate, c->block_min, c->dt_log, c->done, c->pred, c->nnlist,
+                    c->nncount};
     for (void *b : bufs)
         if (b) hipFree(b);
     for (hipEvent_t e : c->ev_pool) hipEventDestroy(e);
@@ -583,6 +623,71 @@ int gpm_kernel_time(gpm_ctx *c, double *avg_stage_ms, int64_t *stage_launches, d
     if (stage_launches) *stage_launches = c->stage_launches;
     if (avg_mood_ms) *avg_mood_ms = c->mood_launches ? c->mood_ms / c->mood_launches : 0.0;
     if (mood_launches) *mood_launches = c->mood_launches;
+    return GPM_OK;
+}
+
+int gpm_set_predictor(gpm_ctx *c, gpm_mlp *m, int32_t gate)
+{
+    if (!c) return GPM_EINVAL;
+    if (m) {
+        const int R = (c->mord - 1) / 2;
+        if (mlp_stencil(m) != 2 * R + 1 || mlp_classes(m) != (c->mord + 1) / 2)
+            return fail(c, GPM_EINVAL, "predictor stencil/classes do not match Mord");
+    }
+    c->nn = m;
+    c->nn_gate = gate;
+    return GPM_OK;
+}
+
+int gpm_set_dump(gpm_ctx *c, float *x_dev, int32_t *y_dev, int64_t capacity, uint64_t *count_dev, double keep_mord,
+                 uint64_t seed, int32_t gated_only, int32_t stage_mask)
+{
+    if (!c) return GPM_EINVAL;
+    if (x_dev && (!y_dev || !count_dev || capacity < 0)) return fail(c, GPM_EINVAL, "dump buffers incomplete");
+    const int R = (c->mord - 1) / 2;
+    if (x_dev && R != 2 && R != 3) return fail(c, GPM_EUNSUP, "dump needs Mord 5 or 7");
+    c->dump_x = x_dev;
+    c->dump_y = y_dev;
+    c->dump_cap = capacity;
+    c->dump_count = reinterpret_cast<unsigned long long *>(count_dev);
+    c->dump_keep = keep_mord;
+    c->dump_seed = seed;
+    c->dump_gated = gated_only;
+    c->dump_stages = stage_mask;
+    return GPM_OK;
+}
+
+int gpm_features(gpm_ctx *c, const int32_t *cells_dev, int32_t n, float *x_dev)
+{
+    if (!c || !cells_dev || !x_dev || n < 0) return GPM_EINVAL;
+    if (!c->has_state) return fail(c, GPM_ESTATE, "no state set");
+    const int R = (c->mord - 1) / 2;
+    HIPCHK(c, mlp_launch_features(c->dU, c->N, c->bc, 2 * R + 1, cells_dev, n, x_dev, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return GPM_OK;
+}
+
+int gpm_get_apriori_orders(gpm_ctx *c, int8_t *pred_dev)
+{
+    if (!c || !pred_dev) return GPM_EINVAL;
+    HIPCHK(c, hipMemcpyAsync(pred_dev, c->pred, (size_t)c->N * c->N, hipMemcpyDeviceToDevice, c->stream));
+    return GPM_OK;
+}
+
+int gpm_set_apriori_orders(gpm_ctx *c, const int8_t *orders)
+{
+    if (!c) return GPM_EINVAL;
+    if (!orders) {
+        c->fixed_orders = false;
+        return GPM_OK;
+    }
+    const size_t n = (size_t)c->N * c->N;
+    for (size_t i = 0; i < n; ++i)
+        if (orders[i] < 1 || orders[i] > c->mord || !(orders[i] & 1))
+            return fail(c, GPM_EINVAL, "a priori orders must be odd and in [1, Mord]");
+    HIPCHK(c, hipMemcpyAsync(c->pred, orders, n, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    c->fixed_orders = true;
     return GPM_OK;
 }
 

@@ -245,6 +245,17 @@ __device__ __forceinline__ void face_flux(const DevTables *__restrict__ T, const
     }
 }
 
+// Face flux for a face whose order is only known at run time (ord <= MORD).
+template <int MORD, int NGP, int DIR, class Acc>
+__device__ __forceinline__ void face_flux_dyn(const DevTables *__restrict__ T, const Acc acc, int ord, int ai,
+                                              int aj, int bi, int bj, double L[4])
+{
+    if (MORD >= 7 && ord == 7) face_flux<(MORD >= 7 ? 7 : 1), NGP, DIR>(T, acc, ai, aj, bi, bj, L);
+    else if (MORD >= 5 && ord == 5) face_flux<(MORD >= 5 ? 5 : 1), NGP, DIR>(T, acc, ai, aj, bi, bj, L);
+    else if (MORD >= 3 && ord == 3) face_flux<(MORD >= 3 ? 3 : 1), NGP, DIR>(T, acc, ai, aj, bi, bj, L);
+    else face_flux<1, NGP, DIR>(T, acc, ai, aj, bi, bj, L);
+}
+
 // Flux of one face at one Gauss point j (run-time face ids and direction):
 // the two GP reconstructions (same summation order as recon2) and one Riemann
 // solve.  Used by the sparse MOOD correction, one lane per (face, point).

@@ -92,6 +92,19 @@ __global__ void k_unpack_orders(const order_t *__restrict__ g, int8_t *__restric
         out[c] = (int8_t)order_of_word(g[c]);
 }
 
+// Order words from a priori orders (one per cell, version 0).
+__global__ void k_orders_from_pred(const int8_t *__restrict__ pred, order_t *__restrict__ g, size_t n)
+{
+    for (size_t c = (size_t)blockIdx.x * blockDim.x + threadIdx.x; c < n; c += (size_t)gridDim.x * blockDim.x)
+        g[c] = pack_order(pred[c], pred[c], 0);
+}
+
+hipError_t launch_orders_from_pred(const int8_t *pred, order_t *g, size_t n, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_orders_from_pred, dim3(1024), dim3(256), 0, s, pred, g, n);
+    return hipGetLastError();
+}
+
 hipError_t launch_unpack_orders(const order_t *g, int8_t *out, size_t n, hipStream_t s)
 {
     hipLaunchKernelGGL(k_unpack_orders, dim3(1024), dim3(256), 0, s, g, out, n);

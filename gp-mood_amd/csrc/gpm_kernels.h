@@ -54,6 +54,8 @@ struct StageArgs {
     const DevTables *__restrict__ T;
     const double *__restrict__ tstate; // t, dt, dt/dx, dt/dy
     int N, bc, dmp, u2, u2tol, cascade;
+    int apriori;                      // start from the a priori orders in pred (learned predictor)
+    const int8_t *__restrict__ pred;  // a priori orders (read only here; gpo is rewritten)
     double dx, dy;
     int tiles_x, ntiles;
 };
@@ -101,5 +103,6 @@ const void *mood_kernel(int mord, int ngp, int stage);
 bool supported(int mord, int ngp);
 hipError_t launch_dt(const DtArgs &a, int nblocks, hipStream_t s);
 hipError_t launch_unpack_orders(const order_t *g, int8_t *out, size_t n, hipStream_t s);
+hipError_t launch_orders_from_pred(const int8_t *pred, order_t *g, size_t n, hipStream_t s);
 
 }  // namespace gpm

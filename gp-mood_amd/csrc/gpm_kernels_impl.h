@@ -138,6 +138,15 @@ __global__ __launch_bounds__(kStageThreads, GPM_STAGE_WAVES_PER_EU) void k_stage
         const int ti = idx % PW, tj = idx / PW;
         sU[idx] = load_bc(a.Uin, N, a.bc, x0 - H + ti, y0 - H + tj);
     }
+    // a priori orders of the tile and its one-cell ring (ghost cells: Mord)
+    constexpr int OW = kTX + 2;
+    __shared__ int8_t sO[(kTY + 2) * OW];
+    if (a.apriori) {
+        for (int idx = tid; idx < (kTY + 2) * OW; idx += kStageThreads) {
+            const int gi = x0 - 1 + idx % OW, gj = y0 - 1 + idx / OW;
+            sO[idx] = (gi < 0 || gi >= N || gj < 0 || gj >= N) ? (int8_t)MORD : a.pred[(size_t)gj * N + gi];
+        }
+    }
     __syncthreads();
 
     const DevTables *__restrict__ T = a.T;
@@ -147,12 +156,22 @@ __global__ __launch_bounds__(kStageThreads, GPM_STAGE_WAVES_PER_EU) void k_stage
         double L[4];
         if (f < NFX) {
             const int fy = f / (kTX + 1), fx = f % (kTX + 1);
-            face_flux<MORD, NGP, 0>(T, acc, fx - 1 + H, fy + H, fx + H, fy + H, L);
+            if (!a.apriori) {
+                face_flux<MORD, NGP, 0>(T, acc, fx - 1 + H, fy + H, fx + H, fy + H, L);
+            } else {
+                const int o = min(sO[(fy + 1) * OW + fx], sO[(fy + 1) * OW + fx + 1]);
+                face_flux_dyn<MORD, NGP, 0>(T, acc, o, fx - 1 + H, fy + H, fx + H, fy + H, L);
+            }
             sFx[f] = make_double4(L[0], L[1], L[2], L[3]);
         } else {
             const int g = f - NFX;
             const int gy = g / kTX, gx = g % kTX;
-            face_flux<MORD, NGP, 1>(T, acc, gx + H, gy - 1 + H, gx + H, gy + H, L);
+            if (!a.apriori) {
+                face_flux<MORD, NGP, 1>(T, acc, gx + H, gy - 1 + H, gx + H, gy + H, L);
+            } else {
+                const int o = min(sO[gy * OW + gx + 1], sO[(gy + 1) * OW + gx + 1]);
+                face_flux_dyn<MORD, NGP, 1>(T, acc, o, gx + H, gy - 1 + H, gx + H, gy + H, L);
+            }
             sFy[g] = make_double4(L[0], L[1], L[2], L[3]);
         }
     }
@@ -174,11 +193,13 @@ __global__ __launch_bounds__(kStageThreads, GPM_STAGE_WAVES_PER_EU) void k_stage
             const double fxr[4] = {r.x, r.y, r.z, r.w}, fyt[4] = {t.x, t.y, t.z, t.w};
             double cand[4];
             apply_fluxes(u, fyb, fxl, fxr, fyt, dtdx, dtdy, cand);
-            if (MORD > 1) dec = decide(P, T, acc, cx + H, cy + H, cand);
+            // CellGPO := Mord (mod_FE.f90:48), or the a priori order
+            const int me = a.apriori ? (int)sO[(cy + 1) * OW + cx + 1] : MORD;
+            if (me > 1) dec = decide(P, T, acc, cx + H, cy + H, cand);
             const size_t gc = (size_t)gj * N + gi;
             a.Uout[gc] = rk_combine<STAGE>(a.Un, gc, cand);
-            // CellGPO := Mord (mod_FE.f90:48); a failing cell takes the cascade at once
-            a.gpo[gc] = pack_order(dec ? lower_order(MORD, a.cascade) : MORD, MORD, 0);
+            // a failing cell takes the cascade at once
+            a.gpo[gc] = pack_order(dec ? lower_order(me, a.cascade) : me, me, 0);
             if (dec) mark_neighbourhood(gi, gj, N, stamp1, a.mark, a.slist1, a.cnt_s1);
         }
         const unsigned long long m = __ballot(dec);

@@ -147,6 +147,64 @@ int gpm_set_timing(gpm_ctx *ctx, int32_t enable);
 int gpm_kernel_time(gpm_ctx *ctx, double *avg_stage_ms, int64_t *stage_launches,
                     double *avg_mood_ms, int64_t *mood_launches);
 
+/* ------------------------------------------------------------------------
+ * Learned a priori order predictor (NN-GP-MOOD).
+ *
+ * An MLP  in -> 128 -> 128 -> 64 -> C  (ReLU; BF16 weights and activations,
+ * FP32 accumulation) on each cell's (2R+1)^2 x 4 conserved stencil, normalised
+ * per variable by the stencil max |value|; class c means order 2c+1
+ * (C = (Mord+1)/2).  It replaces the reference's 52-input sigmoid network
+ * (mod_NN.f90:14-58,168-183); the reference's own role -- CellGPO chosen a
+ * priori, MOOD a posteriori check kept -- is unchanged, except that the full
+ * PAD + DMP check runs after the prediction (the reference kept PAD only,
+ * mod_subroutine_NN_MOOD.f90:13-101).  Parameters are exchanged in PyTorch
+ * Linear order (W0 [128][in], b0, W1, b1, W2, b2, W3 [C][64], b3), as the
+ * reference's exporter writes them (neural_network/save_NN_for_fortran.py:12-25).
+ * Training (forward, backward, AdamW) runs on the device; x / y buffers below
+ * are device pointers (float [n][in], int32 [n]).
+ * ---------------------------------------------------------------------- */
+typedef struct gpm_mlp gpm_mlp;
+
+int gpm_mlp_create(int32_t in_features, int32_t n_classes, int32_t max_batch, gpm_mlp **out);
+int gpm_mlp_destroy(gpm_mlp *m);
+const char *gpm_mlp_last_error(const gpm_mlp *m);
+int64_t gpm_mlp_num_params(const gpm_mlp *m);
+int gpm_mlp_set_params(gpm_mlp *m, const float *params);
+int gpm_mlp_get_params(gpm_mlp *m, float *params);
+int gpm_mlp_get_grads(gpm_mlp *m, float *grads);
+/* forward + mean cross-entropy + backward; gradients stay on the device */
+int gpm_mlp_forward_backward(gpm_mlp *m, const float *x, const int32_t *y, int32_t batch, double *loss,
+                             double *accuracy);
+/* AdamW on the FP32 master weights (torch.optim.AdamW semantics) */
+int gpm_mlp_adamw_step(gpm_mlp *m, double lr, double beta1, double beta2, double eps, double weight_decay);
+int gpm_mlp_train_step(gpm_mlp *m, const float *x, const int32_t *y, int32_t batch, double lr,
+                       double weight_decay, double *loss, double *accuracy);
+/* class logits [n][C] for n feature rows */
+int gpm_mlp_logits(gpm_mlp *m, const float *x, int32_t n, float *logits);
+
+/* Attach a predictor to a solver (NULL detaches).  Each RK stage then starts
+ * from the predicted orders; gate = 1 sends only cells where the a posteriori
+ * check can fail (the shock detector fires, or near vacuum) to the network and
+ * starts every other cell at Mord; gate = 0 predicts every cell. */
+int gpm_set_predictor(gpm_ctx *ctx, gpm_mlp *m, int32_t gate);
+/* Training-data dump (append_to_NN_datatset, mod_append_NN_dataset.f90:11-79):
+ * after each RK stage in stage_mask (bit s-1 for stage s), for sampled cells,
+ * the features of the stage input and the stage's final order class.  Cells
+ * below Mord are always kept, cells at Mord with probability keep_mord;
+ * count_dev (uint64) is incremented past capacity when the buffer is full.
+ * x_dev = NULL switches the dump off. */
+int gpm_set_dump(gpm_ctx *ctx, float *x_dev, int32_t *y_dev, int64_t capacity, uint64_t *count_dev,
+                 double keep_mord, uint64_t seed, int32_t gated_only, int32_t stage_mask);
+/* features of the listed cells of the resident state (device buffers) */
+int gpm_features(gpm_ctx *ctx, const int32_t *cells_dev, int32_t n, float *x_dev);
+/* Fixed a priori orders (int8 [nf][lf], odd, 1..Mord; host memory) used as the
+ * starting CellGPO of every RK stage in place of Mord -- the hook the
+ * reference fills from its network (mod_FE.f90:51-56).  NULL restores Mord.
+ * A predictor attached with gpm_set_predictor takes precedence. */
+int gpm_set_apriori_orders(gpm_ctx *ctx, const int8_t *orders);
+/* a priori orders of the last stage (int8, device) */
+int gpm_get_apriori_orders(gpm_ctx *ctx, int8_t *pred_dev);
+
 #ifdef __cplusplus
 }
 #endif

@@ -49,6 +49,7 @@ typedef struct {
     uint8_t *decrease;         /* decrease(1:N, 1:N) */
     double *U1, *U2;           /* RK stage states */
     long count_detected;       /* count_detected_cell (global_variables.f90:63) */
+    int8_t *init_orders;       /* a priori CellGPO of the interior [N][N] (NULL: Mord) */
     int passes[3];             /* MOOD passes per stage of the last step */
 } orc_t;
 
@@ -115,7 +116,7 @@ void orc_destroy(orc_t *o)
     if (!o) return;
     free(o->zT); free(o->d2x); free(o->d2y); free(o->Uh); free(o->Fx); free(o->Fy);
     free(o->gpo); free(o->detcell); free(o->detfx); free(o->detfy); free(o->decrease);
-    free(o->U1); free(o->U2); free(o);
+    free(o->U1); free(o->U2); free(o->init_orders); free(o);
 }
 
 /* Boundary_C, BC.f90:18-59 (Neumann, Periodic) and :89-135 (Reflective):
@@ -434,6 +435,10 @@ void orc_fe(orc_t *o, double *Uin, double *Uout, double dt, int *npass)
     const int N = o->N;
     const size_t ncell = (size_t)o->W * o->W;
     for (size_t c = 0; c < ncell; ++c) { o->gpo[c] = (int8_t)o->mord; o->detcell[c] = 1; }
+    /* a priori orders (compute_CellGPO_with_NN, mod_FE.f90:51-56): interior only */
+    if (o->init_orders)
+        for (int n = 1; n <= N; ++n)
+            for (int l = 1; l <= N; ++l) o->gpo[CELL(o, l, n)] = o->init_orders[(size_t)(n - 1) * N + (l - 1)];
     memset(o->detfx, 1, (size_t)(N + 3) * (N + 2));
     memset(o->detfy, 1, (size_t)(N + 2) * (N + 3));
     orc_bc(o, Uin);
@@ -550,3 +555,13 @@ void orc_passes(const orc_t *o, int *p) { for (int k = 0; k < 3; ++k) p[k] = o->
 
 /* The Riemann solver alone, for unit checks (dir 0 = x, 1 = y). */
 void orc_hllc(const double *ul, const double *ur, int dir, double *F) { hllc(ul, ur, dir, F); }
+
+/* Fixed a priori orders for every stage (NULL restores CellGPO = Mord). */
+void orc_set_initial_orders(orc_t *o, const int8_t *orders)
+{
+    free(o->init_orders);
+    o->init_orders = NULL;
+    if (!orders) return;
+    o->init_orders = (int8_t *)malloc((size_t)o->N * o->N);
+    memcpy(o->init_orders, orders, (size_t)o->N * o->N);
+}

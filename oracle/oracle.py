@@ -35,6 +35,7 @@ def lib():
         L.orc_count_detected.argtypes = [P]
         L.orc_passes.argtypes = [P, P]
         L.orc_hllc.argtypes = [P, P, ctypes.c_int, P]
+        L.orc_set_initial_orders.argtypes = [P, P]
         _lib = L
     return _lib
 
@@ -83,6 +84,11 @@ class Oracle:
         lib().orc_run(self._h, _ptr(U), nsteps, ctypes.byref(tt), tmax, _ptr(dts), _ptr(counts),
                       _ptr(m), _ptr(passes))
         return dict(dt=dts, counts=counts, masks=m, passes=passes, t=tt.value)
+
+    def set_initial_orders(self, orders):
+        """Fixed a priori CellGPO (int8 [N][N]) for every stage, or None."""
+        self._orders = None if orders is None else np.ascontiguousarray(orders, dtype=np.int8)
+        lib().orc_set_initial_orders(self._h, _ptr(self._orders))
 
     def setdt(self, U, dtfinal=1e10):
         return lib().orc_setdt(self._h, _ptr(U), dtfinal)

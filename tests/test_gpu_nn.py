@@ -1,0 +1,228 @@
+"""The learned predictor on the GPU: matrix-core forward/backward/AdamW
+against the CPU restatement (oracle/mlp_oracle.py) and PyTorch autograd, the
+a priori order path of the step against the oracle, and the dump."""
+import os
+import sys
+
+import numpy as np
+import pytest
+
+from conftest import ROOT, has_gpu, load_oracle
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module", autouse=True)
+def _need_gpu():
+    if not has_gpu():
+        pytest.skip("no GPU")
+
+
+def mlp_oracle():
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    try:
+        import mlp_oracle as M
+    finally:
+        sys.path.pop(0)
+    return M
+
+
+def rel(a, b):
+    return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-30))
+
+
+@pytest.mark.parametrize("in_f,C", [(100, 3), (196, 4)])
+def test_logits_vs_oracle(gpm, in_f, C):
+    import torch
+    M = mlp_oracle()
+    rng = np.random.default_rng(1)
+    m = gpm.MLP(in_f, C, max_batch=1024, seed=3)
+    x = rng.uniform(-1, 1, size=(777, in_f)).astype(np.float32)
+    z_gpu = m.logits(torch.from_numpy(x).cuda()).cpu().numpy()
+    z_ref, _ = M.forward(m.get_params(), in_f, C, x)
+    assert rel(z_gpu, z_ref) < 2e-3
+    assert np.mean(np.argmax(z_gpu, 1) == np.argmax(z_ref, 1)) > 0.99
+
+
+@pytest.mark.parametrize("in_f,C,B", [(100, 3, 8192), (196, 4, 1000)])
+def test_gradients_vs_oracle_and_autograd(gpm, in_f, C, B):
+    import torch
+    M = mlp_oracle()
+    rng = np.random.default_rng(2)
+    m = gpm.MLP(in_f, C, max_batch=B, seed=5)
+    x = rng.uniform(-1, 1, size=(B, in_f)).astype(np.float32)
+    y = rng.integers(0, C, size=B).astype(np.int32)
+    loss, acc = m.forward_backward(torch.from_numpy(x).cuda(), torch.from_numpy(y).cuda())
+    g = m.get_grads()
+    p = m.get_params()
+    loss_r, acc_r, g_r = M.forward_backward(p, in_f, C, x, y)
+    assert abs(loss - loss_r) < 1e-4 * max(1.0, abs(loss_r))
+    off = 0
+    for shape in gpm.nn.param_shapes(in_f, C):
+        n = int(np.prod(shape))
+        assert rel(g[off:off + n], g_r[off:off + n]) < 2e-2, shape
+        off += n
+    # against PyTorch FP32 autograd (no BF16 rounding): same direction
+    layers, off = [], 0
+    dims = (in_f, 128, 128, 64, C)
+    for a, b in zip(dims[:-1], dims[1:]):
+        lin = torch.nn.Linear(a, b)
+        with torch.no_grad():
+            lin.weight.copy_(torch.from_numpy(p[off:off + a * b].reshape(b, a)))
+            off += a * b
+            lin.bias.copy_(torch.from_numpy(p[off:off + b]))
+            off += b
+        layers += [lin, torch.nn.ReLU()]
+    net = torch.nn.Sequential(*layers[:-1])
+    lt = torch.nn.functional.cross_entropy(net(torch.from_numpy(x)), torch.from_numpy(y).long())
+    lt.backward()
+    gt = np.concatenate([t.grad.numpy().ravel() for t in net.parameters()])
+    cos = float(np.dot(g, gt) / (np.linalg.norm(g) * np.linalg.norm(gt)))
+    assert cos > 0.995 and abs(loss - lt.item()) < 2e-2
+
+
+def test_adamw_vs_torch(gpm):
+    import torch
+    rng = np.random.default_rng(3)
+    B = 512
+    m = gpm.MLP(100, 3, max_batch=B, seed=7)
+    p0 = m.get_params()
+    pt = torch.nn.Parameter(torch.from_numpy(p0.copy()))
+    opt = torch.optim.AdamW([pt], lr=1e-3, betas=(0.9, 0.999), eps=1e-8, weight_decay=1e-2)
+    for it in range(3):
+        x = torch.from_numpy(rng.uniform(-1, 1, size=(B, 100)).astype(np.float32)).cuda()
+        y = torch.from_numpy(rng.integers(0, 3, size=B).astype(np.int32)).cuda()
+        m.forward_backward(x, y)
+        g = m.get_grads()
+        m.adamw_step(lr=1e-3, weight_decay=1e-2)
+        pt.grad = torch.from_numpy(g)
+        opt.step()
+        assert np.allclose(m.get_params(), pt.detach().numpy(), rtol=1e-5, atol=1e-7), f"step {it}"
+
+
+def test_mlp_learns_a_separable_rule(gpm):
+    import torch
+    rng = np.random.default_rng(4)
+    n, B = 32768, 1024
+    A = rng.normal(size=(100, 3)).astype(np.float32)
+    x = rng.uniform(-1, 1, size=(n, 100)).astype(np.float32)
+    y = np.argmax(x @ A, axis=1).astype(np.int32)
+    xd, yd = torch.from_numpy(x).cuda(), torch.from_numpy(y).cuda()
+    m = gpm.MLP(100, 3, max_batch=B, seed=1)
+    for it in range(400):
+        i = (it * B) % n
+        loss, acc = m.train_step(xd[i:i + B], yd[i:i + B], lr=3e-3, weight_decay=1e-4)
+    pred = np.argmax(m.logits(xd).cpu().numpy(), 1)
+    assert np.mean(pred == y) > 0.9
+
+
+def _orders_field(n, mord, seed):
+    rng = np.random.default_rng(seed)
+    levels = np.arange(1, mord + 1, 2)
+    coarse = rng.integers(0, len(levels), size=(n // 8 + 1, n // 8 + 1))
+    f = np.kron(coarse, np.ones((8, 8), dtype=int))[:n, :n]
+    noise = rng.random((n, n)) < 0.05
+    f = np.where(noise, rng.integers(0, len(levels), size=(n, n)), f)
+    return levels[f].astype(np.int8)
+
+
+@pytest.mark.parametrize("n,mord,ngp,bc,kind,cascade", [
+    (64, 5, 3, "neumann", "four_quadrant", "reference"),
+    (48, 7, 4, "reflective", "blast", "reference"),
+    (64, 7, 4, "periodic", "four_quadrant", "full")])
+def test_apriori_orders_vs_oracle(gpm, n, mord, ngp, bc, kind, cascade):
+    """Fixed a priori orders (random piecewise field): the step that starts
+    each stage from them equals the oracle's, bit for bit."""
+    O = load_oracle()
+    orders = _orders_field(n, mord, n + mord)
+    U0 = gpm.ic.make(kind, n, 21)
+    t = gpm.gp_tables(mord, ngp, n)
+    s = gpm.Solver(gpm.Config(lf=n, Mord=mord, ngp=ngp, BC_type=bc, cascade=cascade), t)
+    s.set_state(U0)
+    s.set_apriori_orders(orders)
+    dts = s.run(8, return_dts=True)
+    U, g = s.get_state(order=True)
+    o = O.Oracle(n, mord, ngp, bc, t, cascade=cascade)
+    o.set_initial_orders(orders)
+    Uo = o.full(U0)
+    r = o.run(Uo, 8, masks=True)
+    assert np.array_equal(dts, r["dt"])
+    assert np.array_equal(g[4:-4, 4:-4], r["masks"][-1])
+    assert np.array_equal(U, Uo)
+    assert s.stats()["detected_total"] == int(r["counts"].sum())
+
+
+@pytest.mark.parametrize("gate", [0, 1])
+def test_predictor_of_mord_equals_plain(gpm, gate):
+    """A network that predicts Mord everywhere leaves the step unchanged."""
+    n = 96
+    U0 = gpm.ic.four_quadrant(n, 9)
+    cfg = gpm.Config(lf=n, Mord=5, ngp=3)
+    a = gpm.Solver(cfg)
+    a.set_state(U0)
+    a.run(6)
+    m = gpm.MLP(100, 3, seed=0)
+    p = np.zeros(m.num_params, dtype=np.float32)
+    p[-3:] = [0.0, 0.0, 5.0]  # b3: class 2 = order 5
+    m.set_params(p)
+    b = gpm.Solver(cfg)
+    b.set_predictor(m, gate=gate)
+    b.set_state(U0)
+    b.run(6)
+    assert np.array_equal(a.get_state(), b.get_state())
+    assert b.stats()["a_priori_lowered"] == 0
+
+
+def test_predictor_of_order_one(gpm):
+    """A network that predicts order 1 everywhere gives the step with CellGPO = 1
+    a priori (first order, no cell checked), i.e. the oracle with that field."""
+    O = load_oracle()
+    n = 64
+    U0 = gpm.ic.four_quadrant(n, 10)
+    t = gpm.gp_tables(5, 3, n)
+    m = gpm.MLP(100, 3, seed=0)
+    p = np.zeros(m.num_params, dtype=np.float32)
+    p[-3:] = [5.0, 0.0, 0.0]
+    m.set_params(p)
+    s = gpm.Solver(gpm.Config(lf=n, Mord=5, ngp=3), t)
+    s.set_predictor(m, gate=0)
+    s.set_state(U0)
+    s.run(5)
+    o = O.Oracle(n, 5, 3, "neumann", t)
+    o.set_initial_orders(np.ones((n, n), dtype=np.int8))
+    Uo = o.full(U0)
+    o.run(Uo, 5)
+    assert np.array_equal(s.get_state(), Uo)
+    assert s.stats()["a_priori_lowered"] == 3 * 5 * n * n
+
+
+def test_dump_and_features(gpm):
+    import torch
+    n = 128
+    s = gpm.Solver(gpm.Config(lf=n, Mord=5, ngp=3))
+    s.set_state(gpm.ic.four_quadrant(n, 2))
+    cap = 200000
+    X = torch.zeros((cap, 100), dtype=torch.float32, device="cuda")
+    Y = torch.zeros(cap, dtype=torch.int32, device="cuda")
+    cnt = torch.zeros(1, dtype=torch.int64, device="cuda")
+    s.set_dump(X.data_ptr(), Y.data_ptr(), cap, cnt.data_ptr(), keep_mord=0.02, seed=1, gated_only=False)
+    s.run(3)
+    s.set_dump(0, 0, 0, 0)
+    k = int(cnt.item())
+    assert 0 < k <= cap
+    y = Y[:k].cpu().numpy()
+    x = X[:k].cpu().numpy()
+    assert set(np.unique(y)) <= {0, 1, 2} and (y < 2).any()
+    mx = np.abs(x).reshape(k, 25, 4).max(axis=1)
+    assert np.abs(x).max() <= 1.0 + 1e-6 and np.all((mx == 0) | (np.abs(mx - 1.0) < 1e-6))
+    # the features of a cell (library) match a direct restatement
+    cells = torch.tensor([0, 5 * n + 7, n * n - 1], dtype=torch.int32, device="cuda")
+    F = torch.zeros((3, 100), dtype=torch.float32, device="cuda")
+    s.features(cells.data_ptr(), 3, F.data_ptr())
+    U = s.get_state()
+    F = F.cpu().numpy()
+    for r, c in enumerate([0, 5 * n + 7, n * n - 1]):
+        i, j = c % n, c // n
+        st = U[4 + j - 2:4 + j + 3, 4 + i - 2:4 + i + 3].astype(np.float32).reshape(25, 4)
+        mx = np.abs(st).max(axis=0)
+        assert np.allclose(F[r], (st / mx).ravel(), rtol=1e-6, atol=1e-7)
