@@ -30,12 +30,12 @@ struct gpm_ctx {
     double4 *dU = nullptr, *dA = nullptr, *dB = nullptr;
     order_t *gpo = nullptr;
     int *slistA = nullptr, *slistB = nullptr, *mark = nullptr;
-    int *counters = nullptr, *stamp = nullptr, *err = nullptr;
+    int *counters = nullptr, *stamp = nullptr, *err = nullptr, *solo_p = nullptr;
     DevStats *stats = nullptr;
     double *tstate = nullptr, *block_min = nullptr, *dt_log = nullptr;
     unsigned *done = nullptr;
     int dt_log_cap = 0;
-    int tiles_x = 0, ntiles = 0, dt_blocks = 0, mood_blocks[3] = {0, 0, 0};
+    int tiles_x = 0, ntiles = 0, dt_blocks = 0, mood_blocks[3] = {0, 0, 0}, mood_solo = 0;
     bool has_state = false;
     bool timing = false;
     double stage_ms = 0, mood_ms = 0;
@@ -255,6 +255,7 @@ int enqueue_stages(gpm_ctx *c)
         m.Uin = in; m.Un = c->dU; m.Uout = out; m.gpo = c->gpo;
         m.slist[0] = c->slistA; m.slist[1] = c->slistB;
         m.cnt_f = cnt; m.cnt_s = cnt + kMaxPasses + 1; m.mark = c->mark; m.stamp = c->stamp; m.err = c->err;
+        m.solo_p = c->solo_p; m.solo_max = c->mood_solo;
         m.stats = c->stats; m.T = c->dT; m.tstate = c->tstate;
         m.N = c->N; m.bc = c->bc; m.dmp = c->cfg.dmp; m.u2 = c->cfg.u2; m.u2tol = c->cfg.u2_tol;
         m.cascade = c->cfg.cascade; m.pmax = kMaxPasses; m.stage_idx = stage - 1;
@@ -336,7 +337,7 @@ int gpm_create(const gpm_config *cfg, gpm_ctx **out)
               dalloc(&c->slistA, n) == hipSuccess && dalloc(&c->slistB, n) == hipSuccess &&
               dalloc(&c->mark, n) == hipSuccess &&
               dalloc(&c->counters, 3 * kCountersPerStage) == hipSuccess && dalloc(&c->stamp, 1) == hipSuccess &&
-              dalloc(&c->err, 1) == hipSuccess && dalloc(&c->stats, 1) == hipSuccess &&
+              dalloc(&c->err, 1) == hipSuccess && dalloc(&c->solo_p, 1) == hipSuccess && dalloc(&c->stats, 1) == hipSuccess &&
               dalloc(&c->tstate, 4) == hipSuccess && dalloc(&c->done, 1) == hipSuccess &&
               dalloc(&c->pred, n) == hipSuccess && dalloc(&c->nnlist, n) == hipSuccess &&
               dalloc(&c->nncount, 1) == hipSuccess;
@@ -366,6 +367,9 @@ int gpm_create(const gpm_config *cfg, gpm_ctx **out)
         if (e != hipSuccess || per <= 0) per = 1;
         c->mood_blocks[st] = want < per * ncu ? want : per * ncu;
     }
+    // Passes over at most this many cells run on one workgroup (GPM_MOOD_SOLO overrides; 0 disables).
+    c->mood_solo = 16;
+    if (const char *env = std::getenv("GPM_MOOD_SOLO")) c->mood_solo = std::atoi(env) > 0 ? std::atoi(env) : 0;
     hipMemsetAsync(c->mark, 0, n * sizeof(int), c->stream);
     hipMemsetD16Async(c->gpo, pack_order(c->mord, c->mord, 0), n, c->stream);
     hipMemsetAsync(c->counters, 0, 3 * kCountersPerStage * sizeof(int), c->stream);
@@ -404,7 +408,7 @@ int gpm_destroy(gpm_ctx *c)
     if (!c) return GPM_OK;
     if (c->stream) hipStreamSynchronize(c->stream);
     void *bufs[] = {c->dT, c->dU, c->dA, c->dB, c->gpo, c->slistA, c->slistB, c->mark, c->counters,
-                    c->stamp, c->err, c->stats, c->tstate, c->block_min, c->dt_log, c->done, c->pred, c->nnlist,
+                    c->stamp, c->err, c->solo_p, c->stats, c->tstate, c->block_min, c->dt_log, c->done, c->pred, c->nnlist,
                     c->nncount};
     for (void *b : bufs)
         if (b) hipFree(b);

@@ -71,10 +71,12 @@ struct MoodArgs {
     int *mark;
     int *stamp;
     int *err;
+    int *solo_p;    // first pass after a run of workgroup-0-only passes
     DevStats *stats;
     const DevTables *__restrict__ T;
     const double *__restrict__ tstate;
     int N, bc, dmp, u2, u2tol, cascade, pmax, stage_idx;
+    int solo_max;   // lists of at most this many cells run on workgroup 0 alone (0: never)
     double dx, dy;
 };
 

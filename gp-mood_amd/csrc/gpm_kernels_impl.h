@@ -238,7 +238,6 @@ __global__ __launch_bounds__(kMoodThreads) void k_mood(MoodArgs a)
     const int lane = __lane_id();
     const int gl = lane % G, gbase = lane - gl;
     const int f = gl / NGPP, jg = gl % NGPP;
-    const int cells_per_iter = gsz / G;
     const int stamp_base = *a.stamp;
     const DevTables *__restrict__ T = a.T;
     const MoodParams P{a.dmp, a.u2, a.u2tol, a.dx, a.dy};
@@ -262,22 +261,16 @@ __global__ __launch_bounds__(kMoodThreads) void k_mood(MoodArgs a)
     __shared__ double4 nb[CPB][BOX * BOX];
     const int cs = threadIdx.x / G;
     const LdsAcc lacc{&nb[cs][0], BOX};
-    int p = 1;
-    long long second = 0;
-    for (;;) {
-        const int nS = load_relaxed(&a.cnt_s[p]);
-        if (nS == 0) break;
-        if (p + 1 >= a.pmax) {
-            if (gtid == 0) atomicOr(a.err, 1);
-            break;
-        }
-        second += nS;
+    // One pass over the nS cells of list slist[p & 1], spread over the lanes
+    // [t0, t0 + nthr) of the grid (all workgroups) or of workgroup 0 alone.
+    auto run_pass = [&](int p, int nS, int t0, int nthr) {
         const int *__restrict__ S = a.slist[p & 1];
         int *Sn = a.slist[(p + 1) & 1];
         const int stamp_next = stamp_base + p + 1;
-        const int niter = (nS + cells_per_iter - 1) / cells_per_iter;
+        const int per_iter = nthr / G;
+        const int niter = (nS + per_iter - 1) / per_iter;
         for (int it = 0; it < niter; ++it) {
-            const int t = it * cells_per_iter + gtid / G;
+            const int t = it * per_iter + t0 / G;
             const bool valid = t < nS;
             const int d = valid ? S[t] : 0;
             const int i = d % N, j = d / N;
@@ -322,6 +315,40 @@ __global__ __launch_bounds__(kMoodThreads) void k_mood(MoodArgs a)
                 }
             }
         }
+    };
+    int p = 1;
+    long long second = 0;
+    for (;;) {
+        const int nS = load_relaxed(&a.cnt_s[p]);
+        if (nS == 0) break;
+        if (p + 1 >= a.pmax) {
+            if (gtid == 0) atomicOr(a.err, 1);
+            break;
+        }
+        if (nS <= a.solo_max) {
+            // Short lists (the long tail of a failure chain): workgroup 0 runs
+            // the passes alone with workgroup barriers, which cost far less
+            // than grid barriers; the other workgroups wait at the next grid
+            // barrier.  Same passes, same order of effects: the results are
+            // those of the grid-wide path.
+            if (blockIdx.x == 0) {
+                int q = p;
+                for (;;) {
+                    const int m = load_relaxed(&a.cnt_s[q]);
+                    if (m == 0 || m > a.solo_max || q + 1 >= a.pmax) break;
+                    second += m;
+                    run_pass(q, m, threadIdx.x, blockDim.x);
+                    __syncthreads();
+                    ++q;
+                }
+                if (threadIdx.x == 0) __hip_atomic_store(a.solo_p, q, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            }
+            grid.sync();
+            p = load_relaxed(a.solo_p);
+            continue;
+        }
+        second += nS;
+        run_pass(p, nS, gtid, gsz);
         grid.sync();
         ++p;
     }

@@ -15,6 +15,16 @@ def pytest_configure(config):
     config.addinivalue_line("markers", "gpu: needs an MI355X (run with -m gpu)")
 
 
+def pytest_collection_modifyitems(session, config, items):
+    """Run the Fortran drop-in cases first.  They start the drop-in program
+    as a child process, whose MOOD kernel is a cooperative launch; the GPU
+    grants the global-wave-sync resource of cooperative launches to one
+    process at a time, and a test process that has made a cooperative launch
+    itself keeps it until it exits (the child's launch then fails with
+    hipErrorIllegalState)."""
+    items.sort(key=lambda it: 0 if "test_gpu_dropin" in it.nodeid else 1)
+
+
 def load_package():
     if "gpmood_amd" in sys.modules:
         return sys.modules["gpmood_amd"]
