@@ -194,3 +194,33 @@ class Solver:
     def features(self, cells_ptr, n, x_ptr):
         self._check(self.lib.gpm_features(self._h, ctypes.c_void_p(cells_ptr), int(n), ctypes.c_void_p(x_ptr)),
                     "gpm_features")
+
+    # -- differentiable form (frozen masks and dt) ---------------------------
+    def record_masks(self, masks_ptr, max_steps):
+        """Record the final CellGPO of each RK stage of the next max_steps steps
+        into int8 device memory [max_steps][3][nf][lf] (0 stops recording)."""
+        self._check(self.lib.gpm_record_masks(self._h, ctypes.c_void_p(masks_ptr) if masks_ptr else None,
+                                              int(max_steps)), "gpm_record_masks")
+
+    def recorded_steps(self):
+        k = ctypes.c_int32()
+        self._check(self.lib.gpm_recorded_steps(self._h, ctypes.byref(k)), "gpm_recorded_steps")
+        return k.value
+
+    def frozen_run(self, masks_ptr, dts, stage_states_ptr=None):
+        """Advance the resident state len(dts) steps with the given per-stage
+        orders and time steps (no detection); optionally store stage inputs."""
+        dts = np.ascontiguousarray(dts, dtype=np.float64)
+        self._check(self.lib.gpm_frozen_run(self._h, ctypes.c_void_p(masks_ptr), capi.ptr(dts), int(dts.size),
+                                            ctypes.c_void_p(stage_states_ptr) if stage_states_ptr else None),
+                    "gpm_frozen_run")
+
+    def adjoint(self, stage_states_ptr, masks_ptr, dts, ubar_ptr):
+        """In place on ubar (device, interior layout): dJ/dU after len(dts)
+        frozen steps -> dJ/dU before them."""
+        dts = np.ascontiguousarray(dts, dtype=np.float64)
+        self._check(self.lib.gpm_adjoint(self._h, ctypes.c_void_p(stage_states_ptr), ctypes.c_void_p(masks_ptr),
+                                         capi.ptr(dts), int(dts.size), ctypes.c_void_p(ubar_ptr)), "gpm_adjoint")
+
+
+from . import adjoint  # noqa: E402,F401

@@ -35,6 +35,7 @@ EXPORTS = [
     "gpm_mlp_get_params", "gpm_mlp_get_grads", "gpm_mlp_forward_backward", "gpm_mlp_adamw_step",
     "gpm_mlp_train_step", "gpm_mlp_logits", "gpm_set_predictor", "gpm_set_dump", "gpm_features",
     "gpm_set_apriori_orders", "gpm_get_apriori_orders",
+    "gpm_record_masks", "gpm_recorded_steps", "gpm_frozen_run", "gpm_adjoint", "gpm_hllc_vjp",
 ]
 
 
@@ -113,6 +114,11 @@ def load():
         "gpm_features": ([P, P, I, P], I),
         "gpm_set_apriori_orders": ([P, P], I),
         "gpm_get_apriori_orders": ([P, P], I),
+        "gpm_record_masks": ([P, P, I], I),
+        "gpm_recorded_steps": ([P, P], I),
+        "gpm_frozen_run": ([P, P, P, I, P], I),
+        "gpm_adjoint": ([P, P, P, P, I, P], I),
+        "gpm_hllc_vjp": ([I, I, P, P, P, P, P], I),
     }
     for name, (args, res) in sig.items():
         fn = getattr(L, name)

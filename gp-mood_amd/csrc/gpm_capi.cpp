@@ -16,6 +16,7 @@
 #include "../../include/gpmood.h"
 #include "gpm_internal.h"
 #include "gpm_kernels.h"
+#include "gpm_adjoint.h"
 #include "gpm_mlp.h"
 
 using namespace gpm;
@@ -59,6 +60,12 @@ struct gpm_ctx {
     unsigned long long dump_seed = 0;
     int dump_gated = 1, dump_stages = 0;
     long long step_counter = 0;
+    // order masks of every RK stage, recorded for the frozen-mask adjoint
+    int8_t *rec_masks = nullptr;
+    int rec_cap = 0, rec_count = 0;
+    // adjoint work buffers (allocated on first use)
+    double4 *adj_ub = nullptr, *adj_G = nullptr, *adj_A = nullptr, *adj_tmp = nullptr;
+    int8_t *adj_ordx = nullptr, *adj_ordy = nullptr;
 };
 
 namespace {
@@ -213,6 +220,20 @@ int launch_dt_kernel(gpm_ctx *c, int mode, int use_tmax, double tmax, double dtf
     return GPM_OK;
 }
 
+// Arguments of k_stage for RK stage `stage` (1..3) from `in` to `out`.
+StageArgs stage_args(gpm_ctx *c, int stage, const double4 *in, double4 *out)
+{
+    int *cnt = c->counters + (stage - 1) * kCountersPerStage;
+    StageArgs s{};
+    s.Uin = in; s.Un = c->dU; s.Uout = out; s.gpo = c->gpo;
+    s.slist1 = c->slistB; s.cnt_s1 = cnt + kMaxPasses + 1 + 1; s.cnt_f0 = cnt;
+    s.mark = c->mark; s.stamp = c->stamp; s.T = c->dT; s.tstate = c->tstate;
+    s.N = c->N; s.bc = c->bc; s.dmp = c->cfg.dmp; s.u2 = c->cfg.u2; s.u2tol = c->cfg.u2_tol;
+    s.cascade = c->cfg.cascade;
+    s.dx = c->dx; s.dy = c->dy; s.tiles_x = c->tiles_x; s.ntiles = c->ntiles;
+    return s;
+}
+
 // One SSP-RK3 step (mod_SSP_RK3.f90:12-39) after k_dt has set dt.
 int enqueue_stages(gpm_ctx *c)
 {
@@ -230,15 +251,9 @@ int enqueue_stages(gpm_ctx *c)
                                          &c->stats->a_priori_lowered, c->stream));
         }
         if (c->fixed_orders && !c->nn) HIPCHK(c, launch_orders_from_pred(c->pred, c->gpo, (size_t)c->N * c->N, c->stream));
-        StageArgs s{};
+        StageArgs s = stage_args(c, stage, in, out);
         s.apriori = c->nn != nullptr || c->fixed_orders;
         s.pred = c->pred;
-        s.Uin = in; s.Un = c->dU; s.Uout = out; s.gpo = c->gpo;
-        s.slist1 = c->slistB; s.cnt_s1 = cnt + kMaxPasses + 1 + 1; s.cnt_f0 = cnt;
-        s.mark = c->mark; s.stamp = c->stamp; s.T = c->dT; s.tstate = c->tstate;
-        s.N = c->N; s.bc = c->bc; s.dmp = c->cfg.dmp; s.u2 = c->cfg.u2; s.u2tol = c->cfg.u2_tol;
-        s.cascade = c->cfg.cascade;
-        s.dx = c->dx; s.dy = c->dy; s.tiles_x = c->tiles_x; s.ntiles = c->ntiles;
         hipEvent_t e0 = nullptr, e1 = nullptr;
         if (c->timing) {
             size_t i0 = c->ev_used;
@@ -271,6 +286,11 @@ int enqueue_stages(gpm_ctx *c)
         HIPCHK(c, hipLaunchCooperativeKernel(mood_kernel(c->mord, c->ngp, stage), dim3(c->mood_blocks[stage - 1]),
                                              dim3(kMoodThreads), args, 0, c->stream));
         if (c->timing) HIPCHK(c, hipEventRecord(e1, c->stream));
+        if (c->rec_masks && c->rec_count < c->rec_cap) {
+            const size_t n = (size_t)c->N * c->N;
+            HIPCHK(c, launch_unpack_orders(c->gpo, c->rec_masks + ((size_t)c->rec_count * 3 + (stage - 1)) * n, n,
+                                           c->stream));
+        }
         if (c->dump_x && (c->dump_stages & (1 << (stage - 1)))) {
             // (stencil of the stage input, final order of the stage) pairs
             const int S = 2 * ((c->mord - 1) / 2) + 1;
@@ -281,6 +301,7 @@ int enqueue_stages(gpm_ctx *c)
     }
     std::swap(c->dU, c->dA);
     ++c->step_counter;
+    if (c->rec_masks && c->rec_count < c->rec_cap) ++c->rec_count;
     return GPM_OK;
 }
 
@@ -409,7 +430,7 @@ int gpm_destroy(gpm_ctx *c)
     if (c->stream) hipStreamSynchronize(c->stream);
     void *bufs[] = {c->dT, c->dU, c->dA, c->dB, c->gpo, c->slistA, c->slistB, c->mark, c->counters,
                     c->stamp, c->err, c->solo_p, c->stats, c->tstate, c->block_min, c->dt_log, c->done, c->pred, c->nnlist,
-                    c->nncount};
+                    c->nncount, c->adj_ub, c->adj_G, c->adj_A, c->adj_tmp, c->adj_ordx, c->adj_ordy};
     for (void *b : bufs)
         if (b) hipFree(b);
     for (hipEvent_t e : c->ev_pool) hipEventDestroy(e);
@@ -693,6 +714,119 @@ int gpm_set_apriori_orders(gpm_ctx *c, const int8_t *orders)
     HIPCHK(c, hipStreamSynchronize(c->stream));
     c->fixed_orders = true;
     return GPM_OK;
+}
+
+// ---- differentiable form: frozen-mask step and its adjoint ------------------
+
+int gpm_record_masks(gpm_ctx *c, int8_t *masks_dev, int32_t max_steps)
+{
+    if (!c || max_steps < 0) return GPM_EINVAL;
+    if (!masks_dev || max_steps == 0) {  // stop recording; the count stays readable
+        c->rec_masks = nullptr;
+        c->rec_cap = 0;
+        return GPM_OK;
+    }
+    c->rec_masks = masks_dev;
+    c->rec_cap = max_steps;
+    c->rec_count = 0;
+    return GPM_OK;
+}
+
+int gpm_recorded_steps(gpm_ctx *c, int32_t *nsteps)
+{
+    if (!c || !nsteps) return GPM_EINVAL;
+    *nsteps = c->rec_count;
+    return GPM_OK;
+}
+
+int gpm_frozen_run(gpm_ctx *c, const int8_t *masks_dev, const double *dts, int32_t nsteps, double *stage_states_dev)
+{
+    if (!c || !masks_dev || !dts || nsteps < 0) return fail(c, GPM_EINVAL, "invalid argument");
+    if (!c->has_state) return fail(c, GPM_ESTATE, "no state set");
+    const size_t n = (size_t)c->N * c->N;
+    for (int st = 0; st < nsteps; ++st) {
+        int rc = launch_dt_kernel(c, kDtGiven, 0, 0.0, 0.0, dts[st], 1, 0, false, 1);
+        if (rc) return rc;
+        for (int stage = 1; stage <= 3; ++stage) {
+            const double4 *in = stage == 1 ? c->dU : (stage == 2 ? c->dA : c->dB);
+            double4 *out = stage == 1 ? c->dA : (stage == 2 ? c->dB : c->dA);
+            if (stage_states_dev)
+                HIPCHK(c, hipMemcpyAsync(stage_states_dev + ((size_t)st * 3 + (stage - 1)) * n * 4, in,
+                                         n * sizeof(double4), hipMemcpyDeviceToDevice, c->stream));
+            StageArgs s = stage_args(c, stage, in, out);
+            s.apriori = 1;
+            s.frozen = 1;
+            s.pred = masks_dev + ((size_t)st * 3 + (stage - 1)) * n;
+            HIPCHK(c, launch_stage(c->mord, c->ngp, stage, s, (c->ntiles + 7) / 8 * 8, c->stream));
+        }
+        std::swap(c->dU, c->dA);
+    }
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return GPM_OK;
+}
+
+int gpm_adjoint(gpm_ctx *c, const double *stage_states_dev, const int8_t *masks_dev, const double *dts, int32_t nsteps,
+                double *ubar_dev)
+{
+    if (!c || !stage_states_dev || !masks_dev || !dts || !ubar_dev || nsteps < 0)
+        return fail(c, GPM_EINVAL, "invalid argument");
+    if (!adj_supported(c->mord, c->ngp)) return fail(c, GPM_EUNSUP, "adjoint needs (Mord, ngp) in {(3,2), (5,3), (7,4)}");
+    const int N = c->N, R = (c->mord - 1) / 2;
+    const size_t n = (size_t)N * N, W = (size_t)(N + 2), E = (size_t)(N + 2 * R + 2);
+    if (!c->adj_ub) {
+        const bool ok = dalloc(&c->adj_ub, 4 * (size_t)c->ngp * W * W) == hipSuccess &&
+                        dalloc(&c->adj_G, E * E) == hipSuccess && dalloc(&c->adj_A, n) == hipSuccess &&
+                        dalloc(&c->adj_tmp, n) == hipSuccess && dalloc(&c->adj_ordx, (size_t)N * (N + 1)) == hipSuccess &&
+                        dalloc(&c->adj_ordy, (size_t)N * (N + 1)) == hipSuccess;
+        if (!ok) return fail(c, GPM_ENOMEM, "adjoint work buffers");
+    }
+    const double4 *S = reinterpret_cast<const double4 *>(stage_states_dev);
+    double4 *cur = reinterpret_cast<double4 *>(ubar_dev), *nxt = c->adj_tmp;
+    AdjFaceArgs fa{};
+    fa.T = c->dT; fa.ub = c->adj_ub; fa.ordx = c->adj_ordx; fa.ordy = c->adj_ordy; fa.N = N; fa.bc = c->bc;
+    AdjGatherArgs ga{};
+    ga.ub = c->adj_ub; ga.ordx = c->adj_ordx; ga.ordy = c->adj_ordy; ga.T = c->dT; ga.G = c->adj_G; ga.N = N;
+    AdjFoldArgs fo{};
+    fo.G = c->adj_G; fo.N = N; fo.bc = c->bc;
+    // SSP-RK3 (mod_SSP_RK3.f90:12-39): U1 = E1(U0); U2 = 3/4 U0 + 1/4 E2(U1); U3 = 1/3 U0 + 2/3 E3(U2),
+    // E_s(V) = V + dt L_s(V).  Reverse, from the adjoint of U3 in `cur`:
+    //   U2b = 2/3 E3'^T U3b,   A  = 1/3 U3b
+    //   U1b = 1/4 E2'^T U2b,   A += 3/4 U2b
+    //   U0b = E1'^T U1b + A
+    const double third = 1.0 / 3, two_thirds = 2.0 / 3;
+    for (int st = nsteps - 1; st >= 0; --st) {
+        fa.dtdx = dts[st] / c->dx;
+        fa.dtdy = dts[st] / c->dy;
+        const int8_t *m = masks_dev + (size_t)st * 3 * n;
+        // stage 3: input U2
+        fa.V = S + ((size_t)st * 3 + 2) * n; fa.mask = m + 2 * n; fa.Wb = cur; fa.alpha = two_thirds;
+        fo.Wn = cur; fo.alpha = two_thirds; fo.B = nullptr; fo.gamma = 0.0; fo.out = nxt;
+        fo.A = c->adj_A; fo.aA = 0.0; fo.aU = third;
+        HIPCHK(c, launch_adj_stage(c->mord, c->ngp, fa, ga, fo, c->stream));
+        std::swap(cur, nxt);
+        // stage 2: input U1
+        fa.V = S + ((size_t)st * 3 + 1) * n; fa.mask = m + n; fa.Wb = cur; fa.alpha = 0.25;
+        fo.Wn = cur; fo.alpha = 0.25; fo.out = nxt; fo.aA = 1.0; fo.aU = 0.75;
+        HIPCHK(c, launch_adj_stage(c->mord, c->ngp, fa, ga, fo, c->stream));
+        std::swap(cur, nxt);
+        // stage 1: input U0
+        fa.V = S + (size_t)st * 3 * n; fa.mask = m; fa.Wb = cur; fa.alpha = 1.0;
+        fo.Wn = cur; fo.alpha = 1.0; fo.out = nxt; fo.A = nullptr; fo.B = c->adj_A; fo.gamma = 1.0;
+        HIPCHK(c, launch_adj_stage(c->mord, c->ngp, fa, ga, fo, c->stream));
+        std::swap(cur, nxt);
+    }
+    if (cur != reinterpret_cast<double4 *>(ubar_dev))
+        HIPCHK(c, hipMemcpyAsync(ubar_dev, cur, n * sizeof(double4), hipMemcpyDeviceToDevice, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return GPM_OK;
+}
+
+int gpm_hllc_vjp(int32_t dir, int32_t n, const double *ul, const double *ur, const double *fbar, double *ulbar,
+                 double *urbar)
+{
+    if (n < 0 || (dir != 0 && dir != 1) || (n > 0 && (!ul || !ur || !fbar || !ulbar || !urbar))) return GPM_EINVAL;
+    if (launch_hllc_vjp(dir, n, ul, ur, fbar, ulbar, urbar, nullptr) != hipSuccess) return GPM_EHIP;
+    return hipDeviceSynchronize() == hipSuccess ? GPM_OK : GPM_EHIP;
 }
 
 }  // extern "C"

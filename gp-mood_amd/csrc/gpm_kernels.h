@@ -55,6 +55,7 @@ struct StageArgs {
     const double *__restrict__ tstate; // t, dt, dt/dx, dt/dy
     int N, bc, dmp, u2, u2tol, cascade;
     int apriori;                      // start from the a priori orders in pred (learned predictor)
+    int frozen;                       // orders fixed to pred, no detection (frozen-mask step)
     const int8_t *__restrict__ pred;  // a priori orders (read only here; gpo is rewritten)
     double dx, dy;
     int tiles_x, ntiles;

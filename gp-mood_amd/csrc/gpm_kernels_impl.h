@@ -195,7 +195,7 @@ __global__ __launch_bounds__(kStageThreads, GPM_STAGE_WAVES_PER_EU) void k_stage
             apply_fluxes(u, fyb, fxl, fxr, fyt, dtdx, dtdy, cand);
             // CellGPO := Mord (mod_FE.f90:48), or the a priori order
             const int me = a.apriori ? (int)sO[(cy + 1) * OW + cx + 1] : MORD;
-            if (me > 1) dec = decide(P, T, acc, cx + H, cy + H, cand);
+            if (me > 1 && !a.frozen) dec = decide(P, T, acc, cx + H, cy + H, cand);
             const size_t gc = (size_t)gj * N + gi;
             a.Uout[gc] = rk_combine<STAGE>(a.Un, gc, cand);
             // a failing cell takes the cascade at once

@@ -205,6 +205,41 @@ int gpm_set_apriori_orders(gpm_ctx *ctx, const int8_t *orders);
 /* a priori orders of the last stage (int8, device) */
 int gpm_get_apriori_orders(gpm_ctx *ctx, int8_t *pred_dev);
 
+/* ------------------------------------------------------------------------
+ * Differentiable form.  No counterpart in the reference: the adjoint of K
+ * SSP-RK3 steps of GP reconstruction + HLLC + update with the MOOD order mask
+ * and dt of every RK stage frozen from a forward run, for sensitivities of an
+ * output functional with respect to the initial state.
+ *
+ * gpm_record_masks: the next max_steps steps taken by gpm_run or
+ * gpm_time_stepping write the final CellGPO of each RK stage (int8 orders,
+ * interior, [step][stage][nf][lf]) to masks_dev, device memory of
+ * max_steps*3*lf*nf bytes.  NULL stops recording.  gpm_recorded_steps reports
+ * how many steps were recorded since recording last started.
+ *
+ * gpm_frozen_run: advances the resident state nsteps steps with the given
+ * per-stage orders (no a posteriori detection) and the given dt per step (host
+ * array).  With masks recorded from a run, it reproduces that run's states bit
+ * for bit.  If stage_states_dev is non-NULL it receives the input of every RK
+ * stage, [step][stage][nf][lf][4] doubles.
+ *
+ * gpm_adjoint: ubar_dev (interior layout, nf*lf*4 doubles, device) holds dJ/dU
+ * after nsteps steps on entry and dJ/dU before them on return, for the steps
+ * whose stage inputs, masks and dts are given.  Deterministic (no atomics).
+ * Supports (Mord, ngp) = (3,2), (5,3), (7,4).
+ *
+ * gpm_hllc_vjp: reverse mode of the HLLC flux (mod_HLLC.f90:11-169) on n state
+ * pairs in device memory ([n][4] each): ulbar = (dF/dul)^T fbar, urbar =
+ * (dF/dur)^T fbar, for x-faces (dir 0) or y-faces (dir 1).  Synchronous. */
+int gpm_record_masks(gpm_ctx *ctx, int8_t *masks_dev, int32_t max_steps);
+int gpm_recorded_steps(gpm_ctx *ctx, int32_t *nsteps);
+int gpm_frozen_run(gpm_ctx *ctx, const int8_t *masks_dev, const double *dts, int32_t nsteps,
+                   double *stage_states_dev);
+int gpm_adjoint(gpm_ctx *ctx, const double *stage_states_dev, const int8_t *masks_dev, const double *dts,
+                int32_t nsteps, double *ubar_dev);
+int gpm_hllc_vjp(int32_t dir, int32_t n, const double *ul, const double *ur, const double *fbar,
+                 double *ulbar, double *urbar);
+
 #ifdef __cplusplus
 }
 #endif
