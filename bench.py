@@ -11,8 +11,10 @@ three RK stages, each a first-pass kernel plus the on-device MOOD passes.
 
 Prints one JSON line: metric, value (cell-updates/s per RK stage), the
 roofline of the dominant kernel (the first-pass stage kernel, timed with HIP
-events on its own stream) and a CPU baseline (the reference Fortran, serial,
-timed on a one-step sample of the same workload on this host).
+events on its own stream), a CPU baseline (the reference Fortran, serial,
+timed on a one-step sample of the same workload on this host), the plain vs
+NN-GP-MOOD comparison at 2048^2 GP-R2 with the committed predictor weights,
+and the host-array drop-in rate (PCIe round trip per step included).
 """
 import argparse
 import importlib.util
@@ -102,6 +104,31 @@ def cpu_baseline_port(U0, n, steps=1):
         return None
 
 
+def nn_comparison(gpm, n, steps):
+    """Plain GP-MOOD vs NN-GP-MOOD (committed predictor weights) on the
+    2048^2 GP-R2 four-quadrant workload: throughput, second-pass cells and the
+    density L1 distance after `steps` steps; held-out precision/recall of the
+    predictor from its training report."""
+    wdir = os.path.join(ROOT, "gp-mood_amd", "weights")
+    wfile = os.path.join(wdir, "nn_r2.npz")
+    if not os.path.exists(wfile):
+        return None
+    from gpmood_amd import nn_train
+    m = gpm.MLP.load(wfile)
+    out = nn_train.in_step(m, n=n, steps=steps, kind="four_quadrant", seed=0, bc="neumann",
+                           log=lambda msg: print(msg, file=sys.stderr, flush=True))
+    rep = os.path.join(wdir, "nn_r2_report.json")
+    if os.path.exists(rep):
+        r = json.load(open(rep))
+        ev = r.get("eval_test", {})
+        out["held_out"] = {"troubled_precision": ev.get("troubled", {}).get("precision"),
+                           "troubled_recall": ev.get("troubled", {}).get("recall"),
+                           "accuracy": ev.get("accuracy"), "samples": ev.get("samples")}
+    out["weights"] = "gp-mood_amd/weights/nn_r2.npz"
+    m.close()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -111,6 +138,8 @@ def main():
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-steps", type=int, default=1)
+    ap.add_argument("--nn-steps", type=int, default=100, help="steps of the plain vs NN-MOOD comparison (0: skip)")
+    ap.add_argument("--dropin-steps", type=int, default=3, help="host-array drop-in steps timed (0: skip)")
     ap.add_argument("--calib-copy-mib", type=int, default=0,
                     help="after the run, copy a buffer of this size with a plain device copy kernel "
                          "(reference point for scaling hardware byte counters)")
@@ -176,6 +205,28 @@ def main():
         "note": "FP64-bound (AI %.1f FLOP/B > ridge 9.8); runs on the FP64 VALU, whose peak equals the "
                 "FP64 matrix peak" % (fl / BYTES_PER_CELL_STAGE),
     }
+    dropin = None
+    if args.dropin_steps > 0:
+        # gpm_time_stepping with host arrays in the reference layout: the
+        # per-step drop-in of time_stepping(Uin, Uout), PCIe copies included
+        U = s.get_state()
+        t_host = 0.0
+        el_d = 0.0
+        for _ in range(args.dropin_steps):
+            dt = s.setdt()
+            t0 = time.perf_counter()
+            U, _, _ = s.time_stepping(U, t_host, dt)
+            el_d += time.perf_counter() - t0
+            t_host += dt
+        dropin = {"value": 3.0 * n * n * args.dropin_steps / el_d, "unit": "cell-updates/s per RK stage",
+                  "ms_per_step": 1e3 * el_d / args.dropin_steps,
+                  "note": "gpm_time_stepping on host arrays (nf+8)x(lf+8)x4 FP64: one host-to-device and one "
+                          "device-to-host copy of the state per step, plus ghost filling on the host"}
+    s.close()
+    nn = None
+    if args.nn_steps > 0:
+        print(f"[bench] plain vs NN-GP-MOOD, {args.nn_steps} steps at {n}^2 GP-R2", file=sys.stderr, flush=True)
+        nn = nn_comparison(gpm, n, args.nn_steps)
     cpu = None
     if not args.no_cpu_baseline:
         print("[bench] CPU baseline", file=sys.stderr, flush=True)
@@ -191,7 +242,7 @@ def main():
         "mood": {"max_passes": st["max_passes"], "detected_per_step": st["detected_total"] / max(1, args.steps),
                  "second_pass_cells_per_step": st["second_pass_cells"] / max(1, args.steps),
                  "mood_kernel_avg_ms": kt["mood_ms"]},
-        "roofline": roofline, "cpu_baseline": cpu,
+        "roofline": roofline, "cpu_baseline": cpu, "nn_mood": nn, "host_array_drop_in": dropin,
     }
     print(json.dumps(out))
 

@@ -370,7 +370,8 @@ int gpm_create(const gpm_config *cfg, gpm_ctx **out)
     int ncu = 0;
     hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, c->device);
     if (ncu <= 0) ncu = 256;
-    c->dt_blocks = ncu * 8;
+    c->dt_blocks = ncu;
+    if (const char *env = std::getenv("GPM_DT_BLOCKS")) c->dt_blocks = std::atoi(env) > 0 ? std::atoi(env) : c->dt_blocks;
     if (dalloc(&c->block_min, c->dt_blocks) != hipSuccess) {
         gpm_destroy(c);
         return GPM_ENOMEM;

@@ -3,12 +3,33 @@
 
 namespace gpm {
 
+constexpr int kDtThreads = 1024;
+
+// Minimum over the workgroup (fmin2 at every level): wavefront shuffles, then
+// one value per wavefront through LDS.  Valid on every lane of wavefront 0.
+__device__ __forceinline__ double block_min(double m, double *red)
+{
+    for (int o = 32; o > 0; o >>= 1) m = fmin2(m, __shfl_xor(m, o));
+    const int w = threadIdx.x / 64, nw = blockDim.x / 64;
+    __syncthreads();  // red is reused across calls
+    if (__lane_id() == 0) red[w] = m;
+    __syncthreads();
+    m = 1.0e30;
+    if (w == 0) {
+        if (__lane_id() < nw) m = red[__lane_id()];
+        for (int o = 32; o > 0; o >>= 1) m = fmin2(m, __shfl_xor(m, o));
+    }
+    return m;
+}
+
 // Setdt: per-cell 1/(1/(dx/(|vx|+c)) + 1/(dy/(|vy|+c))), grid minimum, times
 // CFL, clipped to dtfinal (physics.f90:98-124).  The last workgroup to finish
 // reduces the per-workgroup minima, advances t and zeroes the step counters.
-__global__ __launch_bounds__(256) void k_dt(DtArgs a)
+// Few large workgroups: every workgroup ends with a fence and an atomic on one
+// counter, which serialise across the device.
+__global__ __launch_bounds__(kDtThreads) void k_dt(DtArgs a)
 {
-    __shared__ double red[256];
+    __shared__ double red[kDtThreads / 64];
     __shared__ bool last;
     const int tid = threadIdx.x;
     double m = 1.0e30;
@@ -34,14 +55,9 @@ __global__ __launch_bounds__(256) void k_dt(DtArgs a)
             }
         }
     }
-    red[tid] = m;
-    __syncthreads();
-    for (int s = 128; s > 0; s >>= 1) {
-        if (tid < s) red[tid] = fmin2(red[tid], red[tid + s]);
-        __syncthreads();
-    }
+    m = block_min(m, red);
     if (tid == 0) {
-        a.block_min[blockIdx.x] = red[0];
+        a.block_min[blockIdx.x] = m;
         __threadfence();
         const unsigned prev = atomicAdd(a.done, 1u);
         last = (prev == gridDim.x - 1);
@@ -50,22 +66,18 @@ __global__ __launch_bounds__(256) void k_dt(DtArgs a)
     if (!last) return;
     __threadfence();
     double mm = 1.0e30;
-    for (int b = tid; b < (int)gridDim.x; b += 256) mm = fmin2(mm, __hip_atomic_load(&a.block_min[b], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
-    red[tid] = mm;
-    __syncthreads();
-    for (int s = 128; s > 0; s >>= 1) {
-        if (tid < s) red[tid] = fmin2(red[tid], red[tid + s]);
-        __syncthreads();
-    }
+    for (int b = tid; b < (int)gridDim.x; b += blockDim.x)
+        mm = fmin2(mm, __hip_atomic_load(&a.block_min[b], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+    mm = block_min(mm, red);
     // a new step zeroes the MOOD pass counters of its three stages
     if (a.new_step)
-        for (int k = tid; k < a.ncounters; k += 256) a.counters[k] = 0;
+        for (int k = tid; k < a.ncounters; k += blockDim.x) a.counters[k] = 0;
     if (tid == 0) {
         double *ts = a.tstate;
         const double t = ts[0];
         double dt;
         if (a.mode == kDtCompute) {
-            dt = red[0] * a.cfl;
+            dt = mm * a.cfl;
             const double dtfinal = a.use_tmax ? a.tmax - t : a.dtfinal;
             if (dt > dtfinal) dt = dtfinal;
         } else {
@@ -113,7 +125,7 @@ hipError_t launch_unpack_orders(const order_t *g, int8_t *out, size_t n, hipStre
 
 hipError_t launch_dt(const DtArgs &a, int nblocks, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_dt, dim3(nblocks), dim3(256), 0, s, a);
+    hipLaunchKernelGGL(k_dt, dim3(nblocks), dim3(kDtThreads), 0, s, a);
     return hipGetLastError();
 }
 
