@@ -67,7 +67,7 @@ def cpu_baseline_reference(U0, n, steps=1):
     exe = os.path.join(ROOT, "oracle", "_ref", f"r3_n{n}_ref", "gpm_ref.exe")
     if not os.path.exists(exe):
         return None
-    with tempfile.TemporaryDirectory() as d:
+    with tempfile.TemporaryDirectory(dir=os.path.dirname(exe)) as d:
         U0.tofile(os.path.join(d, "ic.bin"))
         env = dict(os.environ, OMP_NUM_THREADS="1")
         r = subprocess.run([exe, os.path.join(d, "ic.bin"), str(steps), d, "0"], capture_output=True, text=True,

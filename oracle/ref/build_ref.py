@@ -30,8 +30,8 @@ the reference's own Makefile (gp-mood-code/Makefile:8-58):
     for ``Mord=7`` (the value only feeds the reference NN stencil size).  The
     macro ``szs_sphere(x)=szs_sphere(min(x,7))`` clamps the index; flang's
     preprocessor only expands this function-like macro in ``-E`` mode, so this
-    one file is preprocessed to a temporary directory outside the repository,
-    compiled from there, and the temporary file is deleted.
+    one file is preprocessed to a temporary directory inside the (git-ignored)
+    output directory, compiled from there, and the temporary file is deleted.
 
 Usage: python oracle/ref/build_ref.py [config ...]   (default: all configs)
 """
@@ -133,7 +133,7 @@ def build(name, dropin=False):
     if n >= 1024:
         flags.append("-mcmodel=medium")
     objs = []
-    tmpdir = tempfile.mkdtemp(prefix="gpm_ref_pp_")
+    tmpdir = tempfile.mkdtemp(prefix="pp_", dir=out)
     try:
         shim = os.path.join(os.path.dirname(OUT_ROOT), "..", "gp-mood_amd", "fortran")
         for mod in MODULES + (["__shim__"] if dropin else []):

@@ -50,7 +50,7 @@ def run_ref(cfg, U0, nsteps, mode):
     exe = os.path.join(REF, cfg, "gpm_ref.exe")
     if not os.path.exists(exe):
         build_ref.build(cfg)
-    with tempfile.TemporaryDirectory() as d:
+    with tempfile.TemporaryDirectory(dir=os.path.join(REF, cfg)) as d:
         U0.tofile(os.path.join(d, "ic.bin"))
         subprocess.run(f"ulimit -s unlimited; {exe} {d}/ic.bin {nsteps} {d} {mode}", shell=True, check=True,
                        stdout=subprocess.DEVNULL)
