@@ -392,6 +392,7 @@ int gpm_create(const gpm_config *cfg, gpm_ctx **out)
     // Passes over at most this many cells run on one workgroup (GPM_MOOD_SOLO overrides; 0 disables).
     c->mood_solo = 16;
     if (const char *env = std::getenv("GPM_MOOD_SOLO")) c->mood_solo = std::atoi(env) > 0 ? std::atoi(env) : 0;
+    if (c->mood_solo > kSoloMax) c->mood_solo = kSoloMax;
     hipMemsetAsync(c->mark, 0, n * sizeof(int), c->stream);
     hipMemsetD16Async(c->gpo, pack_order(c->mord, c->mord, 0), n, c->stream);
     hipMemsetAsync(c->counters, 0, 3 * kCountersPerStage * sizeof(int), c->stream);

@@ -279,14 +279,31 @@ __device__ __forceinline__ void face_gp_flux(const DevTables *__restrict__ T, co
     hllc_dir(dir, ua, ub, F);
 }
 
+// Flux of one face at one Gauss point with the face order known only at run
+// time (ord <= MORD): one reconstruction loop whose trip count follows the
+// order (the same operations, in the same order, as face_gp_flux<ord>) and one
+// Riemann solve, so lanes of a wavefront that hold different orders run a
+// single code path instead of one inlined solver per order.
 template <int MORD, class Acc>
 __device__ __forceinline__ void face_gp_flux_dyn(const DevTables *__restrict__ T, const Acc &acc, int ord, int fa,
                                                  int fb, int dir, int j, int ai, int aj, int bi, int bj, double F[4])
 {
-    if (MORD >= 7 && ord == 7) face_gp_flux<(MORD >= 7 ? 7 : 1)>(T, acc, fa, fb, dir, j, ai, aj, bi, bj, F);
-    else if (MORD >= 5 && ord == 5) face_gp_flux<(MORD >= 5 ? 5 : 1)>(T, acc, fa, fb, dir, j, ai, aj, bi, bj, F);
-    else if (MORD >= 3 && ord == 3) face_gp_flux<(MORD >= 3 ? 3 : 1)>(T, acc, fa, fb, dir, j, ai, aj, bi, bj, F);
-    else face_gp_flux<1>(T, acc, fa, fb, dir, j, ai, aj, bi, bj, F);
+    constexpr int SZM = stencil_sz(MORD);
+    const int oi = (ord - 1) / 2, sz = stencil_sz(ord);
+    const double *__restrict__ za_row = &T->zT[oi][fa][0][j];
+    const double *__restrict__ zb_row = &T->zT[oi][fb][0][j];
+    double ua[4] = {0.0, 0.0, 0.0, 0.0}, ub[4] = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll 5
+    for (int s = 0; s < SZM; ++s) {
+        if (s >= sz) break;
+        const int dx = kSX[s], dy = kSY[s];
+        const double4 qa = acc(ai + dx, aj + dy);
+        const double4 qb = acc(bi + dx, bj + dy);
+        const double za = za_row[s * 5], zb = zb_row[s * 5];
+        ua[0] = ua[0] + za * qa.x; ua[1] = ua[1] + za * qa.y; ua[2] = ua[2] + za * qa.z; ua[3] = ua[3] + za * qa.w;
+        ub[0] = ub[0] + zb * qb.x; ub[1] = ub[1] + zb * qb.y; ub[2] = ub[2] + zb * qb.z; ub[3] = ub[3] + zb * qb.w;
+    }
+    hllc_dir(dir, ua, ub, F);
 }
 
 struct MoodParams {

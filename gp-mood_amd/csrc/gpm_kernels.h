@@ -22,6 +22,7 @@ constexpr int kTX = GPM_TX;
 constexpr int kTY = GPM_TY;
 constexpr int kStageThreads = 256;
 constexpr int kMoodThreads = 256;
+constexpr int kSoloMax = 64;  // largest list a single workgroup takes over
 // Halo width: the stencil radius R = (Mord-1)/2 plus one cell for the face
 // neighbour, and at least 2 for the detection (U2 reads a cell's 5-point
 // neighbours' 5-point stencils).

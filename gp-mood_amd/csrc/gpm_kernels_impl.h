@@ -24,6 +24,7 @@
 // yields the same value.
 #pragma once
 #include <hip/hip_cooperative_groups.h>
+#include <type_traits>
 #include <hip/hip_runtime.h>
 
 #include "gpm_device.h"
@@ -212,9 +213,49 @@ __device__ __forceinline__ int load_relaxed(const int *p)
     return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-__device__ __forceinline__ order_t load_order(const order_t *p)
+// Memory operations of the MOOD passes at a chosen coherence scope: device
+// scope while all workgroups take part in a pass, workgroup scope while
+// workgroup 0 runs passes alone (its own writes then stay in its XCD's L2).
+template <int SC>
+struct Scoped {
+    static __device__ __forceinline__ int load(const int *p) { return __hip_atomic_load(p, __ATOMIC_RELAXED, SC); }
+    static __device__ __forceinline__ order_t load(const order_t *p)
+    {
+        return __hip_atomic_load(p, __ATOMIC_RELAXED, SC);
+    }
+    static __device__ __forceinline__ void store(order_t *p, order_t v) { __hip_atomic_store(p, v, __ATOMIC_RELAXED, SC); }
+    static __device__ __forceinline__ int add(int *p, int v) { return __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, SC); }
+    static __device__ __forceinline__ int exch(int *p, int v) { return __hip_atomic_exchange(p, v, __ATOMIC_RELAXED, SC); }
+};
+
+// mark_neighbourhood at scope SC.
+template <int SC>
+__device__ __forceinline__ void mark_neighbourhood_sc(int i, int j, int N, int stamp, int *mark, int *list, int *cnt)
 {
-    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    const int ni[5] = {i, i - 1, i, i + 1, i}, nj[5] = {j - 1, j, j, j, j + 1};
+    int old[5];
+#pragma unroll
+    for (int k = 0; k < 5; ++k) {
+        const bool in = !(ni[k] < 0 || ni[k] >= N || nj[k] < 0 || nj[k] >= N);
+        old[k] = in ? Scoped<SC>::exch(&mark[nj[k] * N + ni[k]], stamp) : stamp;
+    }
+#pragma unroll
+    for (int k = 0; k < 5; ++k)
+        if (old[k] != stamp) list[Scoped<SC>::add(cnt, 1)] = nj[k] * N + ni[k];
+}
+
+// RK combination with the step-start state already loaded.
+template <int STAGE>
+__device__ __forceinline__ double4 rk_combine_u(const double4 &u, const double cand[4])
+{
+    if (STAGE == 1) return make_double4(cand[0], cand[1], cand[2], cand[3]);
+    if (STAGE == 2) {  // mod_SSP_RK3.f90:28: U2 = 0.75*Uin + 0.25*U2
+        return make_double4(0.75 * u.x + 0.25 * cand[0], 0.75 * u.y + 0.25 * cand[1], 0.75 * u.z + 0.25 * cand[2],
+                            0.75 * u.w + 0.25 * cand[3]);
+    }
+    // mod_SSP_RK3.f90:33: Uout = (1./3)*Uin + (2./3)*Uout
+    const double a = 1.0 / 3, b = 2.0 / 3;
+    return make_double4(a * u.x + b * cand[0], a * u.y + b * cand[1], a * u.z + b * cand[2], a * u.w + b * cand[3]);
 }
 
 // MOOD correction passes 1, 2, ... of one RK stage (mod_FE.f90:69-174), as a
@@ -225,14 +266,21 @@ __device__ __forceinline__ order_t load_order(const order_t *p)
 // per (face, Gauss point): the lanes reconstruct and solve their Riemann
 // problem, then cross-lane shuffles form the quadrature sum per face and the
 // four-face update in the reference's order on the cell's first lane.
+//
+// A pass is a chain of dependent memory round trips, so the loads that do not
+// depend on the staged neighbourhood (the face's two order words, the cell's
+// own order word and step-start state, the next trip's list entry) are
+// issued together with the neighbourhood loads.
 template <int MORD, int NGP, int STAGE>
 __global__ __launch_bounds__(kMoodThreads) void k_mood(MoodArgs a)
 {
     constexpr int NGPP = NGP <= 1 ? 1 : NGP <= 2 ? 2 : NGP <= 4 ? 4 : 8;
     constexpr int G = 4 * NGPP;
     static_assert(64 % G == 0, "lanes per cell must divide the wavefront");
+    constexpr int DEV = __HIP_MEMORY_SCOPE_AGENT, WG = __HIP_MEMORY_SCOPE_WORKGROUP;
     cg::grid_group grid = cg::this_grid();
     const int N = a.N;
+    const int ncell = N * N;
     const int gtid = blockIdx.x * blockDim.x + threadIdx.x;
     const int gsz = gridDim.x * blockDim.x;
     const int lane = __lane_id();
@@ -247,10 +295,6 @@ __global__ __launch_bounds__(kMoodThreads) void k_mood(MoodArgs a)
     const int fa = fdir == 0 ? kR : kT, fb = fdir == 0 ? kL : kB;
     const int oai = f == 1 ? -1 : 0, oaj = f == 0 ? -1 : 0;
     const int obi = f == 2 ? 1 : 0, obj = f == 3 ? 1 : 0;
-    auto order_of = [&](int ii, int jj, int p) -> int {
-        if (ii < 0 || ii >= N || jj < 0 || jj >= N) return MORD;  // ghost cells keep Mord
-        return order_at_pass(load_order(&a.gpo[(size_t)jj * N + ii]), p);
-    };
     // Each cell's (2H+1)^2 neighbourhood of the stage input is staged in LDS by
     // its G lanes (all loads in flight at once); reconstruction and detection
     // then read LDS.  H covers the stencils of the cell and its four
@@ -261,28 +305,51 @@ __global__ __launch_bounds__(kMoodThreads) void k_mood(MoodArgs a)
     __shared__ double4 nb[CPB][BOX * BOX];
     const int cs = threadIdx.x / G;
     const LdsAcc lacc{&nb[cs][0], BOX};
-    // One pass over the nS cells of list slist[p & 1], spread over the lanes
-    // [t0, t0 + nthr) of the grid (all workgroups) or of workgroup 0 alone.
-    auto run_pass = [&](int p, int nS, int t0, int nthr) {
-        const int *__restrict__ S = a.slist[p & 1];
-        int *Sn = a.slist[(p + 1) & 1];
+    // lists of the passes workgroup 0 runs alone (a failing cell adds at most
+    // five cells, so a list of at most kSoloMax cells yields at most 5x)
+    __shared__ int s_list[2][5 * kSoloMax];
+    __shared__ int s_cnt[2];
+    __shared__ int s_fail;
+    // One pass over the nS cells of list S, spread over the lanes
+    // [t0, t0 + nthr) of the grid (all workgroups) or of workgroup 0 alone;
+    // d0 is this lane's entry of the first trip (loaded with the count).
+    // Sn and cnt_next receive the next pass's list, cnt_fail counts the cells
+    // that fail.
+    auto run_pass = [&](auto sc, int p, int nS, int d0, int t0, int nthr, const int *S, int *Sn, int *cnt_next,
+                        int *cnt_fail) {
+        constexpr int SC = decltype(sc)::value;
+        using M = Scoped<SC>;
+        auto order_of = [&](int ii, int jj) -> int {
+            if (ii < 0 || ii >= N || jj < 0 || jj >= N) return MORD;  // ghost cells keep Mord
+            return order_at_pass(M::load(&a.gpo[(size_t)jj * N + ii]), p);
+        };
         const int stamp_next = stamp_base + p + 1;
         const int per_iter = nthr / G;
         const int niter = (nS + per_iter - 1) / per_iter;
+        int dnext = d0;
         for (int it = 0; it < niter; ++it) {
             const int t = it * per_iter + t0 / G;
             const bool valid = t < nS;
-            const int d = valid ? S[t] : 0;
+            const int d = valid ? dnext : 0;
             const int i = d % N, j = d / N;
+            // independent of the neighbourhood: issue before staging it
+            int ord = 1;
+            if (valid && jg < NGP) ord = min(order_of(i + oai, j + oaj), order_of(i + obi, j + obj));
+            int me = 1;
+            double4 un = make_double4(0.0, 0.0, 0.0, 0.0);
+            if (valid && gl == 0) {
+                me = order_at_pass(M::load(&a.gpo[d]), p);
+                if (STAGE > 1) un = a.Un[d];
+            }
+            const int tn = t + per_iter;
+            if (it + 1 < niter && tn < nS) dnext = S[tn];
             __syncthreads();  // the previous trip's readers are done with nb
             for (int q = gl; q < BOX * BOX; q += G)
                 nb[cs][q] = load_bc(a.Uin, N, a.bc, i - HB + q % BOX, j - HB + q / BOX);
             __syncthreads();
             double F[4] = {0.0, 0.0, 0.0, 0.0};
-            if (valid && jg < NGP) {
-                const int ord = min(order_of(i + oai, j + oaj, p), order_of(i + obi, j + obj, p));
+            if (valid && jg < NGP)
                 face_gp_flux_dyn<MORD>(T, lacc, ord, fa, fb, fdir, jg, HB + oai, HB + oaj, HB + obi, HB + obj, F);
-            }
             // quadrature over the face's Gauss points, j = 1..ngp from 0
             double L[4] = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
@@ -303,23 +370,25 @@ __global__ __launch_bounds__(kMoodThreads) void k_mood(MoodArgs a)
             if (valid && gl == 0) {
                 double cand[4];
                 apply_fluxes(lacc(HB, HB), fyb, fxl, fxr, fyt, dtdx, dtdy, cand);
-                const int me = order_at_pass(load_order(&a.gpo[d]), p);
                 bool dec = false;
                 if (me > 1) dec = decide(P, T, lacc, HB, HB, cand);
-                a.Uout[d] = rk_combine<STAGE>(a.Un, d, cand);
+                a.Uout[d] = rk_combine_u<STAGE>(un, cand);
                 if (dec) {
-                    __hip_atomic_store(&a.gpo[d], pack_order(lower_order(me, a.cascade), me, p), __ATOMIC_RELAXED,
-                                       __HIP_MEMORY_SCOPE_AGENT);
-                    atomicAdd(&a.cnt_f[p], 1);
-                    mark_neighbourhood(i, j, N, stamp_next, a.mark, Sn, &a.cnt_s[p + 1]);
+                    M::store(&a.gpo[d], pack_order(lower_order(me, a.cascade), me, p));
+                    M::add(cnt_fail, 1);
+                    mark_neighbourhood_sc<SC>(i, j, N, stamp_next, a.mark, Sn, cnt_next);
                 }
             }
         }
     };
+    using Dev = std::integral_constant<int, DEV>;
+    using Wg = std::integral_constant<int, WG>;
     int p = 1;
     long long second = 0;
     for (;;) {
+        // the pass's count and this lane's first list entry, in one round trip
         const int nS = load_relaxed(&a.cnt_s[p]);
+        const int d0 = gtid / G < ncell ? a.slist[p & 1][gtid / G] : 0;
         if (nS == 0) break;
         if (p + 1 >= a.pmax) {
             if (gtid == 0) atomicOr(a.err, 1);
@@ -327,28 +396,44 @@ __global__ __launch_bounds__(kMoodThreads) void k_mood(MoodArgs a)
         }
         if (nS <= a.solo_max) {
             // Short lists (the long tail of a failure chain): workgroup 0 runs
-            // the passes alone with workgroup barriers, which cost far less
-            // than grid barriers; the other workgroups wait at the next grid
-            // barrier.  Same passes, same order of effects: the results are
-            // those of the grid-wide path.
+            // the passes alone, with workgroup barriers, the lists and counts
+            // in LDS and workgroup-scope memory operations, which cost far
+            // less than grid barriers and device-scope round trips; the other
+            // workgroups wait at the next grid barrier.  Same passes, same
+            // order of effects: the results are those of the grid-wide path.
             if (blockIdx.x == 0) {
+                for (int k = threadIdx.x; k < nS; k += blockDim.x) s_list[p & 1][k] = a.slist[p & 1][k];
+                if (threadIdx.x == 0) s_cnt[p & 1] = nS;
                 int q = p;
                 for (;;) {
-                    const int m = load_relaxed(&a.cnt_s[q]);
-                    if (m == 0 || m > a.solo_max || q + 1 >= a.pmax) break;
-                    second += m;
-                    run_pass(q, m, threadIdx.x, blockDim.x);
                     __syncthreads();
+                    const int m = s_cnt[q & 1];
+                    if (m == 0 || m > a.solo_max || q + 1 >= a.pmax) break;
+                    __syncthreads();  // every lane has read the count
+                    if (threadIdx.x == 0) { s_cnt[(q + 1) & 1] = 0; s_fail = 0; }
+                    __syncthreads();
+                    second += m;
+                    const int e0 = (int)threadIdx.x / G < m ? s_list[q & 1][threadIdx.x / G] : 0;
+                    run_pass(Wg{}, q, m, e0, threadIdx.x, blockDim.x, s_list[q & 1], s_list[(q + 1) & 1],
+                             &s_cnt[(q + 1) & 1], &s_fail);
+                    __syncthreads();
+                    if (threadIdx.x == 0) a.cnt_f[q] = s_fail;
                     ++q;
                 }
-                if (threadIdx.x == 0) __hip_atomic_store(a.solo_p, q, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                // hand the current list back to the grid-wide path
+                const int m = s_cnt[q & 1];
+                for (int k = threadIdx.x; k < m; k += blockDim.x) a.slist[q & 1][k] = s_list[q & 1][k];
+                if (threadIdx.x == 0) {
+                    a.cnt_s[q] = m;
+                    __hip_atomic_store(a.solo_p, q, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                }
             }
             grid.sync();
             p = load_relaxed(a.solo_p);
             continue;
         }
         second += nS;
-        run_pass(p, nS, gtid, gsz);
+        run_pass(Dev{}, p, nS, d0, gtid, gsz, a.slist[p & 1], a.slist[(p + 1) & 1], &a.cnt_s[p + 1], &a.cnt_f[p]);
         grid.sync();
         ++p;
     }
