@@ -44,7 +44,12 @@ __device__ __forceinline__ void hllc_x_vjp(const double ul[4], const double ur[4
     const double num = (pr - pl) + (rL * vxl * nL - rR * vxr * nR);
     const double den = rL * nL - rR * nR;
     const double uS = num / den;
-
+    // Above: the forward quantities, in the forward's arithmetic, so that the
+    // branches taken are the forward's.  Below: the reverse sweep, which needs
+    // no bit-level agreement with anything and may contract to fused
+    // multiply-adds.
+  {
+#pragma clang fp contract(fast)
     double aL[4] = {0.0, 0.0, 0.0, 0.0}, aR[4] = {0.0, 0.0, 0.0, 0.0};  // direct adjoints of ul, ur
     double Flb[4] = {0.0, 0.0, 0.0, 0.0}, Frb[4] = {0.0, 0.0, 0.0, 0.0};
     double vxlb = 0.0, vylb = 0.0, plb = 0.0, clb = 0.0;
@@ -173,6 +178,7 @@ __device__ __forceinline__ void hllc_x_vjp(const double ul[4], const double ur[4
         ulb[k] = aL[k];
         urb[k] = aR[k];
     }
+  }
 }
 
 // Reverse mode of hllc_dir: a y-face solve is the x-solve on momentum-swapped
@@ -202,107 +208,162 @@ __device__ __forceinline__ double4 cell_or_zero(const double4 *__restrict__ W, i
     return (i < 0 || i >= N || j < 0 || j >= N) ? make_double4(0.0, 0.0, 0.0, 0.0) : W[(size_t)j * N + i];
 }
 
-// Reconstruction of cell (ci, cj) at face `face`, Gauss point g, order ord
-// (dot_product order of mod_GP_MOOD.f90:46-65).
-__device__ __forceinline__ void recon_point(const DevTables *__restrict__ T, const double4 *__restrict__ V, int N,
-                                            int bc, int ord, int face, int g, int ci, int cj, double u[4])
+constexpr int kAdjTX = 32, kAdjTY = 8;  // 256 threads, one item per thread and trip
+
+// Reconstructions of both sides of a face at Gauss point g with the face order
+// known at run time (dot_product order of mod_GP_MOOD.f90:46-65).
+template <int MORD, class Acc>
+__device__ __forceinline__ void recon_pair(const DevTables *__restrict__ T, const Acc &acc, int ord, int fa, int fb,
+                                           int g, int ai, int aj, int bi, int bj, double ua[4], double ub[4])
 {
     const int oi = (ord - 1) / 2, sz = stencil_sz(ord);
-    u[0] = 0.0; u[1] = 0.0; u[2] = 0.0; u[3] = 0.0;
-    for (int s = 0; s < sz; ++s) {
-        const double4 q = load_bc(V, N, bc, ci + kSX[s], cj + kSY[s]);
-        const double z = T->zT[oi][face][s][g];
-        u[0] = u[0] + z * q.x;
-        u[1] = u[1] + z * q.y;
-        u[2] = u[2] + z * q.z;
-        u[3] = u[3] + z * q.w;
+    const double *__restrict__ za_row = &T->zT[oi][fa][0][g];
+    const double *__restrict__ zb_row = &T->zT[oi][fb][0][g];
+    for (int k = 0; k < 4; ++k) { ua[k] = 0.0; ub[k] = 0.0; }
+#pragma unroll 5
+    for (int s = 0; s < stencil_sz(MORD); ++s) {
+        if (s >= sz) break;
+        const double4 qa = acc(ai + kSX[s], aj + kSY[s]);
+        const double4 qb = acc(bi + kSX[s], bj + kSY[s]);
+        const double za = za_row[s * 5], zb = zb_row[s * 5];
+        ua[0] = ua[0] + za * qa.x; ua[1] = ua[1] + za * qa.y; ua[2] = ua[2] + za * qa.z; ua[3] = ua[3] + za * qa.w;
+        ub[0] = ub[0] + zb * qb.x; ub[1] = ub[1] + zb * qb.y; ub[2] = ub[2] + zb * qb.z; ub[3] = ub[3] + zb * qb.w;
     }
 }
 
+struct AdjLds {
+    const double4 *s;
+    int pitch;
+    __device__ __forceinline__ double4 operator()(int i, int j) const { return s[j * pitch + i]; }
+};
+
+// Faces are owned by the cell on their high side: the x-face left of cell
+// (cx, cy), cx in [0, N], and the y-face below it, cy in [0, N].  A workgroup
+// takes a 32x8 tile of owner cells, stages the stage input of the tile with a
+// halo of R+1 cells and the output adjoint and orders of the tile and its
+// low-side ring in LDS, then runs one (face, Gauss point) item per thread and
+// trip.
 template <int MORD, int NGP>
 __global__ __launch_bounds__(256) void k_adj_face(AdjFaceArgs a)
 {
-    const int N = a.N;
-    const long long nfx = (long long)(N + 1) * N, nf = 2 * nfx;
-    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= nf * NGP) return;
-    const int g = (int)(t / nf);
-    const long long f = t % nf;
-    const int dir = f < nfx ? 0 : 1;
-    int ai, aj, bi, bj;
-    if (dir == 0) {
-        ai = (int)(f % (N + 1)) - 1; aj = (int)(f / (N + 1));
-        bi = ai + 1; bj = aj;
-    } else {
-        const long long h = f - nfx;
-        ai = (int)(h % N); aj = (int)(h / N) - 1;
-        bi = ai; bj = aj + 1;
+    constexpr int R = (MORD - 1) / 2, TX = kAdjTX, TY = kAdjTY;
+    constexpr int PW = TX + 2 * R + 1, PH = TY + 2 * R + 1;  // cells x0-1-R .. x0+TX-1+R
+    constexpr int WW = TX + 1, WH = TY + 1;                  // cells x0-1 .. x0+TX-1
+    __shared__ double4 sV[PW * PH];
+    __shared__ double4 sW[WW * WH];
+    __shared__ int8_t sM[WW * WH];
+    const int N = a.N, tid = threadIdx.x;
+    const int tiles_x = (N + TX) / TX;  // owner cells [0, N]
+    const int x0 = (blockIdx.x % tiles_x) * TX, y0 = (blockIdx.x / tiles_x) * TY;
+    for (int idx = tid; idx < PW * PH; idx += 256)
+        sV[idx] = load_bc(a.V, N, a.bc, x0 - 1 - R + idx % PW, y0 - 1 - R + idx / PW);
+    for (int idx = tid; idx < WW * WH; idx += 256) {
+        const int ci = x0 - 1 + idx % WW, cj = y0 - 1 + idx / WW;
+        sW[idx] = cell_or_zero(a.Wb, N, ci, cj);
+        sM[idx] = (int8_t)mask_at(a.mask, N, ci, cj, MORD);
     }
-    const int o = min(mask_at(a.mask, N, ai, aj, MORD), mask_at(a.mask, N, bi, bj, MORD));
-    if (g == 0) (dir == 0 ? a.ordx : a.ordy)[f - (dir == 0 ? 0 : nfx)] = (int8_t)o;
-    const int fa = dir == 0 ? kR : kT, fb = dir == 0 ? kL : kB;
-    double ua[4], ub[4];
-    recon_point(a.T, a.V, N, a.bc, o, fa, g, ai, aj, ua);
-    recon_point(a.T, a.V, N, a.bc, o, fb, g, bi, bj, ub);
-    const double4 wa = cell_or_zero(a.Wb, N, ai, aj), wb = cell_or_zero(a.Wb, N, bi, bj);
-    const double c = a.alpha * (dir == 0 ? a.dtdx : a.dtdy) * a.T->gw[g];
-    const double Fb[4] = {c * (wb.x - wa.x), c * (wb.y - wa.y), c * (wb.z - wa.z), c * (wb.w - wa.w)};
-    double uab[4], ubb[4];
-    hllc_vjp(dir, ua, ub, Fb, uab, ubb);
+    __syncthreads();
+    const AdjLds acc{sV, PW};
     const int W = N + 2;
     const size_t plane = (size_t)W * W;
-    a.ub[((size_t)fa * NGP + g) * plane + (size_t)(aj + 1) * W + (ai + 1)] = make_double4(uab[0], uab[1], uab[2], uab[3]);
-    a.ub[((size_t)fb * NGP + g) * plane + (size_t)(bj + 1) * W + (bi + 1)] = make_double4(ubb[0], ubb[1], ubb[2], ubb[3]);
+    for (int it = tid; it < 2 * NGP * TX * TY; it += 256) {
+        const int lx = it % TX, ly = (it / TX) % TY, g = (it / (TX * TY)) % NGP, dir = it / (TX * TY * NGP);
+        const int bx = x0 + lx, by = y0 + ly;  // owner cell B
+        if (dir == 0 ? (bx > N || by > N - 1) : (bx > N - 1 || by > N)) continue;
+        const int ax = dir == 0 ? bx - 1 : bx, ay = dir == 0 ? by : by - 1;
+        const int wa = (ay - y0 + 1) * WW + (ax - x0 + 1), wb = (by - y0 + 1) * WW + (bx - x0 + 1);
+        const int o = min((int)sM[wa], (int)sM[wb]);
+        const int fa = dir == 0 ? kR : kT, fb = dir == 0 ? kL : kB;
+        double ua[4], ub[4];
+        recon_pair<MORD>(a.T, acc, o, fa, fb, g, ax - x0 + 1 + R, ay - y0 + 1 + R, bx - x0 + 1 + R,
+                         by - y0 + 1 + R, ua, ub);
+        const double4 wA = sW[wa], wB = sW[wb];
+        const double c = a.alpha * (dir == 0 ? a.dtdx : a.dtdy) * a.T->gw[g];
+        const double Fb[4] = {c * (wB.x - wA.x), c * (wB.y - wA.y), c * (wB.z - wA.z), c * (wB.w - wA.w)};
+        double uab[4], ubb[4];
+        hllc_vjp(dir, ua, ub, Fb, uab, ubb);
+        a.ub[((size_t)fa * NGP + g) * plane + (size_t)(ay + 1) * W + (ax + 1)] = make_double4(uab[0], uab[1], uab[2], uab[3]);
+        a.ub[((size_t)fb * NGP + g) * plane + (size_t)(by + 1) * W + (bx + 1)] = make_double4(ubb[0], ubb[1], ubb[2], ubb[3]);
+        if (g == 0) {
+            if (dir == 0) a.ordx[(size_t)ay * (N + 1) + (ax + 1)] = (int8_t)o;
+            else a.ordy[(size_t)(ay + 1) * N + ax] = (int8_t)o;
+        }
+    }
 }
 
+// Order of the face on side `side` of cell (ci, cj), 0 if that face does not
+// exist (faces lie between cells of [-1, N]^2 with at least one interior side).
+__device__ __forceinline__ int side_order(const AdjGatherArgs &a, int side, int ci, int cj)
+{
+    const int N = a.N;
+    if (side == kR) return (ci >= -1 && ci <= N - 1 && cj >= 0 && cj <= N - 1) ? a.ordx[(size_t)cj * (N + 1) + ci + 1] : 0;
+    if (side == kL) return (ci >= 0 && ci <= N && cj >= 0 && cj <= N - 1) ? a.ordx[(size_t)cj * (N + 1) + ci] : 0;
+    if (side == kT) return (cj >= -1 && cj <= N - 1 && ci >= 0 && ci <= N - 1) ? a.ordy[(size_t)(cj + 1) * N + ci] : 0;
+    return (cj >= 0 && cj <= N && ci >= 0 && ci <= N - 1) ? a.ordy[(size_t)cj * N + ci] : 0;
+}
+
+// Transpose of the reconstruction over the ghost-extended grid [-(R+1), N+R]^2:
+// G(p) = sum over sides, stencil points s and Gauss points g of
+// zT(o, side, s, g) * ub[side][g][p - off(s)].  A workgroup takes a 32x8 tile
+// of positions and, side by side, stages the adjoints and orders of the cells
+// within R of the tile in LDS.
 template <int MORD, int NGP>
 __global__ __launch_bounds__(256) void k_adj_gather(AdjGatherArgs a)
 {
-    constexpr int R = (MORD - 1) / 2;
-    const int N = a.N, E = N + 2 * R + 2;
-    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= (long long)E * E) return;
-    const int px = (int)(t % E) - (R + 1), py = (int)(t / E) - (R + 1);
+    constexpr int R = (MORD - 1) / 2, TX = kAdjTX, TY = kAdjTY;
+    constexpr int QW = TX + 2 * R, QH = TY + 2 * R;
+    constexpr int NO = (MORD + 1) / 2, SZ = stencil_sz(MORD);
+    __shared__ double4 sU[NGP][QW * QH];
+    __shared__ int8_t sO[QW * QH];
+    __shared__ double sZ[NO][SZ][NGP];  // zT(o, side, s, g) of the current side
+    const int N = a.N, E = N + 2 * R + 2, tid = threadIdx.x;
+    const int tiles_x = (E + TX - 1) / TX;
+    const int px0 = -(R + 1) + (blockIdx.x % tiles_x) * TX, py0 = -(R + 1) + (blockIdx.x / tiles_x) * TY;
+    const int lx = tid % TX, ly = tid / TX;
+    const int px = px0 + lx, py = py0 + ly;
+    const bool valid = px <= N + R && py <= N + R;
     const int W = N + 2;
     const size_t plane = (size_t)W * W;
+    // the adjoint is checked against autograd and finite differences, not bit
+    // for bit, so its sums use fused multiply-adds
     double acc[4] = {0.0, 0.0, 0.0, 0.0};
-    for (int s = 0; s < stencil_sz(MORD); ++s) {
-        const int ci = px - kSX[s], cj = py - kSY[s];
-        if (ci < -1 || ci > N || cj < -1 || cj > N) continue;
+    for (int side = 0; side < 4; ++side) {
+        __syncthreads();
+        for (int idx = tid; idx < QW * QH; idx += 256) {
+            const int ci = px0 - R + idx % QW, cj = py0 - R + idx / QW;
+            const int o = side_order(a, side, ci, cj);
+            sO[idx] = (int8_t)o;
+            const size_t base = (size_t)side * NGP * plane + (size_t)(cj + 1) * W + (ci + 1);
 #pragma unroll
-        for (int side = 0; side < 4; ++side) {
-            int o;
-            if (side == kR) {
-                if (ci > N - 1 || cj < 0 || cj > N - 1) continue;
-                o = a.ordx[(size_t)cj * (N + 1) + (ci + 1)];
-            } else if (side == kL) {
-                if (ci < 0 || cj < 0 || cj > N - 1) continue;
-                o = a.ordx[(size_t)cj * (N + 1) + ci];
-            } else if (side == kT) {
-                if (cj > N - 1 || ci < 0 || ci > N - 1) continue;
-                o = a.ordy[(size_t)(cj + 1) * N + ci];
-            } else {
-                if (cj < 0 || ci < 0 || ci > N - 1) continue;
-                o = a.ordy[(size_t)cj * N + ci];
-            }
-            if (s >= stencil_sz(o)) continue;
+            for (int g = 0; g < NGP; ++g) sU[g][idx] = o ? a.ub[base + (size_t)g * plane] : make_double4(0.0, 0.0, 0.0, 0.0);
+        }
+        for (int idx = tid; idx < NO * SZ * NGP; idx += 256) {
+            const int g = idx % NGP, sidx = (idx / NGP) % SZ, oi = idx / (NGP * SZ);
+            sZ[oi][sidx][g] = a.T->zT[oi][side][sidx][g];
+        }
+        __syncthreads();
+        if (!valid) continue;
+        for (int s = 0; s < SZ; ++s) {
+            const int q = (ly + R - kSY[s]) * QW + (lx + R - kSX[s]);
+            const int o = sO[q];
+            if (o == 0 || s >= stencil_sz(o)) continue;
             const int oi = (o - 1) / 2;
-            const double4 *__restrict__ u = a.ub + (size_t)side * NGP * plane + (size_t)(cj + 1) * W + (ci + 1);
 #pragma unroll
             for (int g = 0; g < NGP; ++g) {
-                const double z = a.T->zT[oi][side][s][g];
-                const double4 v = u[(size_t)g * plane];
-                acc[0] += z * v.x;
-                acc[1] += z * v.y;
-                acc[2] += z * v.z;
-                acc[3] += z * v.w;
+                const double z = sZ[oi][s][g];
+                const double4 v = sU[g][q];
+                acc[0] = fma(z, v.x, acc[0]);
+                acc[1] = fma(z, v.y, acc[1]);
+                acc[2] = fma(z, v.z, acc[2]);
+                acc[3] = fma(z, v.w, acc[3]);
             }
         }
     }
-    a.G[t] = make_double4(acc[0], acc[1], acc[2], acc[3]);
+    if (valid) a.G[(size_t)(py + R + 1) * E + (px + R + 1)] = make_double4(acc[0], acc[1], acc[2], acc[3]);
 }
 
-// out = alpha*Wn + fold(G) + gamma*B;  A = aA*A + aU*Wn (if A is given)
+// out = alpha*Wn + fold(G) + gamma*B;  A = aA*A + aU*Wn (if A is given).
+// Cells further than R+1 from every edge are not copied by any ghost cell.
 template <int R>
 __global__ __launch_bounds__(256) void k_adj_fold(AdjFoldArgs a)
 {
@@ -311,31 +372,31 @@ __global__ __launch_bounds__(256) void k_adj_fold(AdjFoldArgs a)
     const size_t c = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= n) return;
     const int cx = (int)(c % N), cy = (int)(c / N);
-    // preimages of cx (cy) under the ghost rule within [-(R+1), N+R]
-    int px[2 * R + 3], py[2 * R + 3];
-    bool fx[2 * R + 3], fy[2 * R + 3];
-    int nx = 0, ny = 0;
-    px[nx] = cx; fx[nx++] = false;
-    py[ny] = cy; fy[ny++] = false;
-    for (int k = 1; k <= R + 1; ++k) {
-        const int cand[2] = {-k, N - 1 + k};
-#pragma unroll
-        for (int e = 0; e < 2; ++e) {
-            bool fl;
-            if (bc_map(cand[e], N, a.bc, fl) == cx) { px[nx] = cand[e]; fx[nx++] = fl; }
-            if (bc_map(cand[e], N, a.bc, fl) == cy) { py[ny] = cand[e]; fy[ny++] = fl; }
-        }
-    }
     const double4 w = a.Wn[c];
     double v[4] = {a.alpha * w.x, a.alpha * w.y, a.alpha * w.z, a.alpha * w.w};
-    for (int jy = 0; jy < ny; ++jy)
-        for (int jx = 0; jx < nx; ++jx) {
-            const double4 gq = a.G[(size_t)(py[jy] + R + 1) * E + (px[jx] + R + 1)];
-            v[0] += gq.x;
-            v[1] += fx[jx] ? -gq.y : gq.y;
-            v[2] += fy[jy] ? -gq.z : gq.z;
-            v[3] += gq.w;
+    const bool near_x = cx <= R || cx >= N - 1 - R, near_y = cy <= R || cy >= N - 1 - R;
+    if (!near_x && !near_y) {
+        const double4 gq = a.G[(size_t)(cy + R + 1) * E + (cx + R + 1)];
+        v[0] += gq.x; v[1] += gq.y; v[2] += gq.z; v[3] += gq.w;
+    } else {
+        // positions of [-(R+1), N+R] whose ghost rule copies cx (cy): the cell
+        // itself (e = 0) and the ghost candidates -k, N-1+k, k = 1..R+1
+        for (int ey = 0; ey <= 2 * (R + 1); ++ey) {
+            const int qy = ey == 0 ? cy : (ey <= R + 1 ? -ey : N - 1 + ey - (R + 1));
+            bool fy = false;
+            if (ey > 0 && bc_map(qy, N, a.bc, fy) != cy) continue;
+            for (int ex = 0; ex <= 2 * (R + 1); ++ex) {
+                const int qx = ex == 0 ? cx : (ex <= R + 1 ? -ex : N - 1 + ex - (R + 1));
+                bool fx = false;
+                if (ex > 0 && bc_map(qx, N, a.bc, fx) != cx) continue;
+                const double4 gq = a.G[(size_t)(qy + R + 1) * E + (qx + R + 1)];
+                v[0] += gq.x;
+                v[1] += fx ? -gq.y : gq.y;
+                v[2] += fy ? -gq.z : gq.z;
+                v[3] += gq.w;
+            }
         }
+    }
     if (a.B) {
         const double4 b = a.B[c];
         v[0] += a.gamma * b.x; v[1] += a.gamma * b.y; v[2] += a.gamma * b.z; v[3] += a.gamma * b.w;
@@ -364,12 +425,13 @@ template <int MORD, int NGP>
 static hipError_t adj_stage_mg(const AdjFaceArgs &fa, const AdjGatherArgs &ga, const AdjFoldArgs &fo, hipStream_t s)
 {
     constexpr int R = (MORD - 1) / 2;
-    const long long N = fa.N;
-    const long long nface = 2 * (N + 1) * N * NGP;
-    k_adj_face<MORD, NGP><<<(unsigned)((nface + 255) / 256), 256, 0, s>>>(fa);
-    const long long E = N + 2 * R + 2;
-    k_adj_gather<MORD, NGP><<<(unsigned)((E * E + 255) / 256), 256, 0, s>>>(ga);
-    k_adj_fold<R><<<(unsigned)((N * N + 255) / 256), 256, 0, s>>>(fo);
+    const int N = fa.N;
+    const int ftiles = ((N + kAdjTX) / kAdjTX) * ((N + kAdjTY) / kAdjTY);
+    k_adj_face<MORD, NGP><<<ftiles, 256, 0, s>>>(fa);
+    const int E = N + 2 * R + 2;
+    const int gtiles = ((E + kAdjTX - 1) / kAdjTX) * ((E + kAdjTY - 1) / kAdjTY);
+    k_adj_gather<MORD, NGP><<<gtiles, 256, 0, s>>>(ga);
+    k_adj_fold<R><<<(unsigned)(((long long)N * N + 255) / 256), 256, 0, s>>>(fo);
     return hipGetLastError();
 }
 
