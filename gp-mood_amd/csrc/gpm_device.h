@@ -311,27 +311,45 @@ struct MoodParams {
     double dx, dy;
 };
 
-// shock_detector, mod_subroutine_MOOD.f90:191-238
+// shock_detector, mod_subroutine_MOOD.f90:191-238: compression (divV <
+// -max(dx,dy)^2), Mach > 0.2 and a pressure jump (gradP > 5).  The three tests
+// are evaluated one after the other and the first that fails ends the
+// detector (same result as evaluating all three; most cells fail the Mach or
+// compression test, so the divisions of the others are skipped).
+__device__ __forceinline__ bool shock_mach(const Cell &c, double pc)
+{
+    const double Ca = 1.4 * pc / c.r;
+    double Mach = (c.mx * c.mx + c.my * c.my) / (c.r * c.r);
+    Mach = sqrt(Mach / Ca);
+    return Mach > 0.2;
+}
+
+__device__ __forceinline__ bool shock_compression(const Cell &c, const Cell &e, const Cell &w, const Cell &nn,
+                                                  const Cell &s, double dx, double dy)
+{
+    double divV = (e.mx - w.mx) / dx + (nn.my - s.my) / dy;
+    divV = 0.5 * divV / c.r;
+    divV = divV - 0.5 * (c.mx * (e.r - w.r) / dx + c.my * (nn.r - s.r) / dy) / (c.r * c.r);
+    const double m = fmax2(dx, dy);
+    return divV < -(m * m);
+}
+
+__device__ __forceinline__ bool shock_pressure_jump(double pL, double pR, double pB, double pT, double dx, double dy)
+{
+    const double gradP = 0.5 * (fabs(pR - pL) / fmin2(pL, pR) / dx + fabs(pT - pB) / fmin2(pB, pT) / dy);
+    return gradP > 5.0;
+}
+
 template <class Acc>
 __device__ __forceinline__ bool shock_detector(const MoodParams &P, const Acc &acc, int i, int j)
 {
     const Cell c = to_cell(acc(i, j));
     const Cell e = to_cell(acc(i + 1, j)), w = to_cell(acc(i - 1, j));
     const Cell nn = to_cell(acc(i, j + 1)), s = to_cell(acc(i, j - 1));
-    const double dx = P.dx, dy = P.dy;
-    double divV = (e.mx - w.mx) / dx + (nn.my - s.my) / dy;
-    divV = 0.5 * divV / c.r;
-    divV = divV - 0.5 * (c.mx * (e.r - w.r) / dx + c.my * (nn.r - s.r) / dy) / (c.r * c.r);
-    const double Ca = 1.4 * pressure(c.r, c.mx, c.my, c.E) / c.r;
-    double Mach = (c.mx * c.mx + c.my * c.my) / (c.r * c.r);
-    Mach = sqrt(Mach / Ca);
-    const double pL = pressure(w.r, w.mx, w.my, w.E);
-    const double pR = pressure(e.r, e.mx, e.my, e.E);
-    const double pB = pressure(s.r, s.mx, s.my, s.E);
-    const double pT = pressure(nn.r, nn.mx, nn.my, nn.E);
-    const double gradP = 0.5 * (fabs(pR - pL) / fmin2(pL, pR) / dx + fabs(pT - pB) / fmin2(pB, pT) / dy);
-    const double m = fmax2(dx, dy);
-    return divV < -(m * m) && Mach > 0.2 && gradP > 5.0;
+    if (!shock_mach(c, pressure(c.r, c.mx, c.my, c.E))) return false;
+    if (!shock_compression(c, e, w, nn, s, P.dx, P.dy)) return false;
+    return shock_pressure_jump(pressure(w.r, w.mx, w.my, w.E), pressure(e.r, e.mx, e.my, e.E),
+                               pressure(s.r, s.mx, s.my, s.E), pressure(nn.r, nn.mx, nn.my, nn.E), P.dx, P.dy);
 }
 
 // DETECTION loop-1 decision of one cell (mod_subroutine_MOOD.f90:41-131) for a

@@ -338,10 +338,29 @@ __global__ void k_to_bf16(const float *__restrict__ p, int no, int kp, uint16_t 
 // ---------------------------------------------------------------------------
 // inference inside the step
 
+// LDS written by the lanes of one wavefront, read by other lanes of it.
+__device__ __forceinline__ void wave_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+template <int NE>
+__device__ __forceinline__ void stage_lds(uint16_t *dst, const uint16_t *__restrict__ src)
+{
+    static_assert(NE % 8 == 0, "16-byte chunks");
+    const uint4 *s4 = reinterpret_cast<const uint4 *>(src);
+    uint4 *d4 = reinterpret_cast<uint4 *>(dst);
+    for (int e = threadIdx.x; e < NE / 8; e += blockDim.x) d4[e] = s4[e];
+}
+
 // Predicted order per listed cell: features from the stage input, the four
-// layers on the matrix cores, argmax -> order 2c+1.  Each wavefront takes 16
-// cells per trip, four lanes per cell for the features (max |value| combined
-// across the four lanes); the cell list and its count live on the device.
+// layers on the matrix cores, argmax -> order 2c+1.  The workgroup copies all
+// weights to LDS once; each wavefront then takes 16 cells per trip on its own
+// activation buffers (four lanes per cell for the features, max |value|
+// combined across the four lanes).  The cell list and its count live on the
+// device.
 template <int S, int KP>
 __global__ __launch_bounds__(256) void k_mlp_predict(const double4 *__restrict__ U, int N, int bc, const int *__restrict__ list,
                                                      const int *__restrict__ count, const MlpDev w, int C, int mord,
@@ -349,14 +368,26 @@ __global__ __launch_bounds__(256) void k_mlp_predict(const double4 *__restrict__
                                                      long long *__restrict__ lowered)
 {
     constexpr int IN = 4 * S * S;
-    __shared__ uint16_t xa[4][16 * (KP > 128 ? KP : 128)];
-    __shared__ uint16_t xb[4][16 * 128];
+    constexpr int KX = KP > 128 ? KP : 128;
+    __shared__ __align__(16) uint16_t sW0[128 * KP];
+    __shared__ __align__(16) uint16_t sW1[128 * 128];
+    __shared__ __align__(16) uint16_t sW2[64 * 128];
+    __shared__ __align__(16) uint16_t sW3[16 * 64];
+    __shared__ __align__(16) uint16_t xa[4][16 * KX];
+    __shared__ __align__(16) uint16_t xb[4][16 * 128];
     __shared__ float lg[4][16 * 16];
+    const int n = *count;
+    if (blockIdx.x * 64 >= n) return;  // the same for the whole workgroup
+    stage_lds<128 * KP>(sW0, w.W[0]);
+    stage_lds<128 * 128>(sW1, w.W[1]);
+    stage_lds<64 * 128>(sW2, w.W[2]);
+    stage_lds<16 * 64>(sW3, w.W[3]);
+    __syncthreads();
     const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int m = lane & 15, q = lane >> 4;
-    const int n = *count;
     for (int bb = blockIdx.x * 64; bb < n; bb += gridDim.x * 64) {
         const int base = bb + wv * 16;
+        if (base >= n) break;  // per wavefront: no workgroup barrier below
         const int t = base + m;
         const int cell = t < n ? list[t] : 0;
         const int ci = cell % N, cj = cell / N;
@@ -374,15 +405,15 @@ __global__ __launch_bounds__(256) void k_mlp_predict(const double4 *__restrict__
             for (int e = q; e < IN; e += 4) row[e] = 0;
         }
         for (int e = IN + q; e < KP; e += 4) row[e] = 0;
-        __syncthreads();
-        dense16<KP, 128, true, false>(xa[wv], KP, w.W[0], w.b[0], xb[wv], 128, lane);
-        __syncthreads();
-        dense16<128, 128, true, false>(xb[wv], 128, w.W[1], w.b[1], xa[wv], 128, lane);
-        __syncthreads();
-        dense16<128, 64, true, false>(xa[wv], 128, w.W[2], w.b[2], xb[wv], 64, lane);
-        __syncthreads();
-        dense16<64, 16, false, true>(xb[wv], 64, w.W[3], w.b[3], lg[wv], 16, lane);
-        __syncthreads();
+        wave_sync();
+        dense16<KP, 128, true, false>(xa[wv], KP, sW0, w.b[0], xb[wv], 128, lane);
+        wave_sync();
+        dense16<128, 128, true, false>(xb[wv], 128, sW1, w.b[1], xa[wv], 128, lane);
+        wave_sync();
+        dense16<128, 64, true, false>(xa[wv], 128, sW2, w.b[2], xb[wv], 64, lane);
+        wave_sync();
+        dense16<64, 16, false, true>(xb[wv], 64, sW3, w.b[3], lg[wv], 16, lane);
+        wave_sync();
         if (q == 0 && t < n) {
             const float *z = &lg[wv][m * 16];
             int am = 0;
@@ -394,7 +425,7 @@ __global__ __launch_bounds__(256) void k_mlp_predict(const double4 *__restrict__
             pred[cell] = (int8_t)ord;
             if (ord < mord) atomicAdd((unsigned long long *)lowered, 1ull);
         }
-        __syncthreads();
+        wave_sync();  // the next trip rewrites xa
     }
 }
 
@@ -407,42 +438,54 @@ __global__ __launch_bounds__(256) void k_mlp_gate(const double4 *__restrict__ U,
                                                   int gate, int mord, order_t *__restrict__ gpo, int8_t *__restrict__ pred,
                                                   int *__restrict__ list, int *__restrict__ count)
 {
-    const size_t ncell = (size_t)N * N;
-    for (size_t c0 = (size_t)blockIdx.x * blockDim.x; c0 < ncell; c0 += (size_t)gridDim.x * blockDim.x) {
-        const size_t c = c0 + threadIdx.x;
-        bool sel = false;
-        if (c < ncell) {
-            const int i = (int)(c % N), j = (int)(c / N);
-            if (!gate) {
-                sel = true;
-            } else {
-                struct GA {
-                    const double4 *U;
-                    int N, bc;
-                    __device__ double4 operator()(int a, int b) const { return load_bc(U, N, bc, a, b); }
-                } acc{U, N, bc};
-                const MoodParams P{1, 1, 1, dx, dy};
-                sel = shock_detector(P, acc, i, j);
-                if (!sel) {
-                    const double4 u = U[c];
-                    const double p = pressure(u.x, u.y, u.z, u.w);
-                    sel = !(u.x > 1e-6) || !(p > 1e-9);
-                }
-            }
-            if (!sel) {
-                gpo[c] = pack_order(mord, mord, 0);
-                pred[c] = (int8_t)mord;
-            }
+    // 32x8 cells per workgroup; the tile and a one-cell ring of the stage
+    // input, and each of their pressures, are staged in LDS once
+    constexpr int TX = 32, TY = 8, PW = TX + 2, PH = TY + 2;
+    __shared__ double4 sU[PW * PH];
+    __shared__ double sP[PW * PH];
+    const int tiles_x = (N + TX - 1) / TX;
+    const int x0 = (blockIdx.x % tiles_x) * TX, y0 = (blockIdx.x / tiles_x) * TY;
+    const int tid = threadIdx.x;
+    if (gate) {
+        for (int idx = tid; idx < PW * PH; idx += 256) {
+            const double4 u = load_bc(U, N, bc, x0 - 1 + idx % PW, y0 - 1 + idx / PW);
+            sU[idx] = u;
+            sP[idx] = pressure(u.x, u.y, u.z, u.w);
         }
-        const unsigned long long m = __ballot(sel);
-        if (m) {
-            const int lane = __lane_id();
-            const int leader = __ffsll((long long)m) - 1;
-            int base = 0;
-            if (lane == leader) base = atomicAdd(count, __popcll(m));
-            base = __shfl(base, leader);
-            if (sel) list[base + __popcll(m & ((1ull << lane) - 1ull))] = (int)c;
+        __syncthreads();
+    }
+    const int lx = tid % TX, ly = tid / TX;
+    const int i = x0 + lx, j = y0 + ly;
+    bool sel = false;
+    if (i < N && j < N) {
+        const size_t c = (size_t)j * N + i;
+        if (!gate) {
+            sel = true;
+        } else {
+            // shock_detector with the pressures from LDS
+            const int q = (ly + 1) * PW + lx + 1;
+            const Cell cc = to_cell(sU[q]);
+            const double pc = sP[q];
+            sel = shock_mach(cc, pc) &&
+                  shock_compression(cc, to_cell(sU[q + 1]), to_cell(sU[q - 1]), to_cell(sU[q + PW]),
+                                    to_cell(sU[q - PW]), dx, dy) &&
+                  shock_pressure_jump(sP[q - 1], sP[q + 1], sP[q - PW], sP[q + PW], dx, dy);
+            // near-vacuum cells (positivity)
+            if (!sel) sel = !(cc.r > 1e-6) || !(pc > 1e-9);
         }
+        if (!sel) {
+            gpo[c] = pack_order(mord, mord, 0);
+            pred[c] = (int8_t)mord;
+        }
+    }
+    const unsigned long long msk = __ballot(sel);
+    if (msk) {
+        const int lane = __lane_id();
+        const int leader = __ffsll((long long)msk) - 1;
+        int base = 0;
+        if (lane == leader) base = atomicAdd(count, __popcll(msk));
+        base = __shfl(base, leader);
+        if (sel) list[base + __popcll(msk & ((1ull << lane) - 1ull))] = j * N + i;
     }
 }
 
@@ -508,19 +551,23 @@ __global__ void k_features(const double4 *__restrict__ U, int N, int bc, const i
 hipError_t mlp_launch_gate(const double4 *U, int N, int bc, double dx, double dy, int gate, int mord, order_t *gpo,
                            int8_t *pred, int *list, int *count, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_mlp_gate, dim3(2048), dim3(256), 0, s, U, N, bc, dx, dy, gate, mord, gpo, pred, list, count);
+    const int tiles = ((N + 31) / 32) * ((N + 7) / 8);
+    hipLaunchKernelGGL(k_mlp_gate, dim3(tiles), dim3(256), 0, s, U, N, bc, dx, dy, gate, mord, gpo, pred, list, count);
     return hipGetLastError();
 }
+
+// one workgroup per CU: each holds all weights in LDS
+constexpr int kPredictBlocks = 256;
 
 hipError_t mlp_launch_predict(const double4 *U, int N, int bc, const int *list, const int *count, const MlpDev &w,
                               int stencil, int C, int mord, order_t *gpo, int8_t *pred, long long *lowered,
                               hipStream_t s)
 {
     if (stencil == 5)
-        hipLaunchKernelGGL((k_mlp_predict<5, 112>), dim3(1024), dim3(256), 0, s, U, N, bc, list, count, w, C, mord, gpo,
+        hipLaunchKernelGGL((k_mlp_predict<5, 112>), dim3(kPredictBlocks), dim3(256), 0, s, U, N, bc, list, count, w, C, mord, gpo,
                            pred, lowered);
     else if (stencil == 7)
-        hipLaunchKernelGGL((k_mlp_predict<7, 208>), dim3(1024), dim3(256), 0, s, U, N, bc, list, count, w, C, mord, gpo,
+        hipLaunchKernelGGL((k_mlp_predict<7, 208>), dim3(kPredictBlocks), dim3(256), 0, s, U, N, bc, list, count, w, C, mord, gpo,
                            pred, lowered);
     else
         return hipErrorInvalidValue;

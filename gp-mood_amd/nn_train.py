@@ -14,6 +14,13 @@ Training: MLP 100 -> 128 -> 128 -> 64 -> 3 (ReLU), BF16 weights with FP32
 accumulation, AdamW lr 1e-3, batch 8192, cross-entropy; the last ICs of each
 family are held out.
 
+Decision threshold: the order is the argmax of the logits after subtracting
+`mord_logit_shift` from the Mord class's logit (folded into that class's
+bias), so a cell keeps Mord a priori only when the network prefers it by that
+margin.  Missing a troubled cell costs MOOD passes, while predicting a lower
+order than needed only adds a little diffusion.  The held-out evaluation uses
+the shifted rule.
+
 Usage: python gp-mood_amd/nn_train.py --out DIR [--n-ics 128 --n 512 --steps 400]
 """
 import argparse
@@ -122,6 +129,14 @@ def train(Xtr, Ytr, epochs=4, batch=8192, lr=1e-3, weight_decay=1e-2, seed=0, lo
     return m, hist
 
 
+def shift_mord_logit(m, shift):
+    """Fold the decision threshold into the Mord class's output bias."""
+    if shift:
+        p = m.get_params()
+        p[-1] -= shift  # last layer bias, class C-1 = order Mord
+        m.set_params(p)
+
+
 def evaluate(m, X, Y):
     """Confusion matrix over the order classes and the troubled-cell view
     (predicted below Mord vs. final order below Mord)."""
@@ -193,6 +208,7 @@ def main():
     ap.add_argument("--epochs", type=int, default=4)
     ap.add_argument("--batch", type=int, default=8192)
     ap.add_argument("--lr", type=float, default=1e-3)
+    ap.add_argument("--mord-logit-shift", type=float, default=1.0)
     ap.add_argument("--eval-n", type=int, default=2048)
     ap.add_argument("--eval-steps", type=int, default=100)
     args = ap.parse_args()
@@ -211,6 +227,7 @@ def main():
         f"{time.time() - t0:.1f}s")
     del X, Y
     m, hist = train(Xtr, Ytr, args.epochs, args.batch, args.lr, log=log)
+    shift_mord_logit(m, args.mord_logit_shift)
     m.save(os.path.join(args.out, "nn_r2.npz"))
     ev_train = evaluate(m, Xtr[:2_000_000], Ytr[:2_000_000])
     ev_test = evaluate(m, Xte, Yte)
@@ -218,7 +235,8 @@ def main():
     report = {"dataset": {"n": args.n, "n_ics": args.n_ics, "steps": args.steps, "dump_every": args.dump_every,
                           "keep_mord": args.keep_mord, "samples": int(Xtr.shape[0] + Xte.shape[0]),
                           "train": int(Xtr.shape[0]), "test": int(Xte.shape[0]), "class_counts": counts},
-              "training": {"epochs": args.epochs, "batch": args.batch, "lr": args.lr, "history": hist},
+              "training": {"epochs": args.epochs, "batch": args.batch, "lr": args.lr, "history": hist,
+                           "mord_logit_shift": args.mord_logit_shift},
               "eval_train": ev_train, "eval_test": ev_test}
     del Xtr, Ytr, Xte, Yte
     report["in_step"] = in_step(m, n=args.eval_n, steps=args.eval_steps, log=log)
