@@ -226,3 +226,46 @@ def test_dump_and_features(gpm):
         st = U[4 + j - 2:4 + j + 3, 4 + i - 2:4 + i + 3].astype(np.float32).reshape(25, 4)
         mx = np.abs(st).max(axis=0)
         assert np.allclose(F[r], (st / mx).ravel(), rtol=1e-6, atol=1e-7)
+
+
+def test_trained_predictor_gains_over_plain(gpm):
+    """The committed GP-R2 predictor (gp-mood_amd/weights/nn_r2.npz) on a
+    held-out four-quadrant problem (seed 0 at 1024^2; training used 512^2):
+    far fewer cells go to a second MOOD pass and fewer cells fail the
+    a posteriori check, the density stays within 1e-3 (relative L1) of plain
+    GP-MOOD, and the step is faster."""
+    import time
+
+    import torch
+
+    wfile = os.path.join(ROOT, "gp-mood_amd", "weights", "nn_r2.npz")
+    m = gpm.MLP.load(wfile)
+    n, steps = 1024, 40
+    U0 = gpm.ic.four_quadrant(n, 0)
+    res = {}
+    for label, use_nn in (("plain", False), ("nn", True)):
+        s = gpm.Solver(gpm.Config(lf=n, Mord=5, ngp=3, method=gpm.capi.NN_GP_MOOD if use_nn else gpm.capi.GP_MOOD))
+        if use_nn:
+            s.set_predictor(m, gate=True)
+        s.set_state(U0)
+        s.run(2)
+        s.set_state(U0)
+        s.reset_stats()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        s.run(steps)
+        torch.cuda.synchronize()
+        el = time.perf_counter() - t0
+        res[label] = dict(s.stats(), U=s.get_state()[4:-4, 4:-4, 0], ms=1e3 * el / steps)
+        s.close()
+    p, q = res["plain"], res["nn"]
+    l1 = float(np.abs(q["U"] - p["U"]).sum() / np.abs(p["U"]).sum())
+    print(f"  plain {p['ms']:.2f} ms/step, NN {q['ms']:.2f} ms/step ({p['ms'] / q['ms']:.2f}x); second-pass cells "
+          f"{p['second_pass_cells']} -> {q['second_pass_cells']}; detections {p['detected_total']} -> "
+          f"{q['detected_total']}; density L1 {l1:.2e}", flush=True)
+    assert q["a_priori_lowered"] > 0
+    assert q["second_pass_cells"] < 0.5 * p["second_pass_cells"]
+    assert q["detected_total"] < 0.5 * p["detected_total"]
+    assert l1 < 1e-3
+    assert q["ms"] < p["ms"]
+    m.close()
