@@ -14,7 +14,8 @@ roofline of the dominant kernel (the first-pass stage kernel, timed with HIP
 events on its own stream), a CPU baseline (the reference Fortran, serial,
 timed on a one-step sample of the same workload on this host), the plain vs
 NN-GP-MOOD comparison at 2048^2 GP-R2 with the committed predictor weights,
-and the host-array drop-in rate (PCIe round trip per step included).
+the host-array drop-in rate (PCIe round trip per step included) and the cost
+of the 512^2 initial-state sensitivity run (frozen-mask adjoint).
 """
 import argparse
 import importlib.util
@@ -104,6 +105,31 @@ def cpu_baseline_port(U0, n, steps=1):
         return None
 
 
+def adjoint_measure(gpm, K, n=512, m=20, w=64):
+    """Initial-state sensitivity of J = sum over a 64x64 subregion of
+    rho (u^2 + v^2) after K steps of 512^2 GP-R2 (four-quadrant IC, outflow),
+    masks and dt frozen, a checkpoint every m steps."""
+    import torch
+    s = gpm.Solver(gpm.Config(lf=n, Mord=5, ngp=3, BC_type="neumann"))
+    U0 = torch.from_numpy(gpm.ic.make("four_quadrant", n, 3)).cuda().requires_grad_(True)
+    gpm.adjoint.steps(s, U0, min(K, 2), checkpoint_every=m)  # warm-up
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    UK = gpm.adjoint.steps(s, U0, K, checkpoint_every=m)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    J = gpm.adjoint.kinetic_energy(UK, 224, 224, w)
+    J.backward()
+    torch.cuda.synchronize()
+    t2 = time.perf_counter()
+    s.close()
+    return {"workload": f"{n}x{n} GP-R2 four-quadrant seed 3, outflow, K={K} SSP-RK3 steps, checkpoint every {m}, "
+                        f"J over a {w}x{w} subregion", "forward_s": t1 - t0, "backward_s": t2 - t1,
+            "stage_adjoints_per_s": 3.0 * n * n * K / (t2 - t1),
+            "note": "backward = frozen-mask recompute of each checkpoint segment + adjoint kernels; "
+                    "unit: cell-stages per second"}
+
+
 def nn_comparison(gpm, n, steps):
     """Plain GP-MOOD vs NN-GP-MOOD (committed predictor weights) on the
     2048^2 GP-R2 four-quadrant workload: throughput, second-pass cells and the
@@ -140,6 +166,8 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=1)
     ap.add_argument("--nn-steps", type=int, default=100, help="steps of the plain vs NN-MOOD comparison (0: skip)")
     ap.add_argument("--dropin-steps", type=int, default=3, help="host-array drop-in steps timed (0: skip)")
+    ap.add_argument("--adjoint-steps", type=int, default=200,
+                    help="K of the 512^2 GP-R2 initial-state sensitivity run (0: skip)")
     ap.add_argument("--calib-copy-mib", type=int, default=0,
                     help="after the run, copy a buffer of this size with a plain device copy kernel "
                          "(reference point for scaling hardware byte counters)")
@@ -227,6 +255,10 @@ def main():
     if args.nn_steps > 0:
         print(f"[bench] plain vs NN-GP-MOOD, {args.nn_steps} steps at {n}^2 GP-R2", file=sys.stderr, flush=True)
         nn = nn_comparison(gpm, n, args.nn_steps)
+    adj = None
+    if args.adjoint_steps > 0:
+        print(f"[bench] adjoint, K={args.adjoint_steps} at 512^2 GP-R2", file=sys.stderr, flush=True)
+        adj = adjoint_measure(gpm, args.adjoint_steps)
     cpu = None
     if not args.no_cpu_baseline:
         print("[bench] CPU baseline", file=sys.stderr, flush=True)
@@ -242,7 +274,7 @@ def main():
         "mood": {"max_passes": st["max_passes"], "detected_per_step": st["detected_total"] / max(1, args.steps),
                  "second_pass_cells_per_step": st["second_pass_cells"] / max(1, args.steps),
                  "mood_kernel_avg_ms": kt["mood_ms"]},
-        "roofline": roofline, "cpu_baseline": cpu, "nn_mood": nn, "host_array_drop_in": dropin,
+        "roofline": roofline, "cpu_baseline": cpu, "nn_mood": nn, "host_array_drop_in": dropin, "adjoint": adj,
     }
     print(json.dumps(out))
 
