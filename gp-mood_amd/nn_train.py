@@ -21,7 +21,11 @@ margin.  Missing a troubled cell costs MOOD passes, while predicting a lower
 order than needed only adds a little diffusion.  The held-out evaluation uses
 the shifted rule.
 
-Usage: python gp-mood_amd/nn_train.py --out DIR [--n-ics 128 --n 512 --steps 400]
+GP-R3 predictor (--mord 7): the same pipeline with Mord 7, 4 Gauss points,
+the 7x7x4 stencil (196 inputs) and 4 classes {1, 3, 5, 7}, run with the full
+cascade 7 -> 5 -> 3 -> 1 so that every class occurs.
+
+Usage: python gp-mood_amd/nn_train.py --out DIR [--mord 5|7 --n-ics 128 --n 512 --steps 400]
 """
 import argparse
 import importlib.util
@@ -46,7 +50,25 @@ def _package():
     return mod
 
 
-MORD, NGP, IN, C = 5, 3, 100, 3
+class Family:
+    """Order family of a predictor: Mord, Gauss points, inputs, classes, cascade."""
+
+    def __init__(self, mord=5):
+        if mord not in (5, 7):
+            raise ValueError("predictors exist for Mord 5 and 7")
+        self.mord = mord
+        self.ngp = {5: 3, 7: 4}[mord]
+        self.stencil = mord  # box stencil of side 2R+1 = Mord
+        self.inputs = 4 * self.stencil * self.stencil
+        self.classes = (mord + 1) // 2
+        self.cascade = "reference" if mord == 5 else "full"
+        self.tag = {5: "r2", 7: "r3"}[mord]
+
+    def config(self, gpm, n, bc, **kw):
+        return gpm.Config(lf=n, Mord=self.mord, ngp=self.ngp, BC_type=bc, cascade=self.cascade, **kw)
+
+
+R2 = Family(5)
 
 
 def ic_list(n_ics):
@@ -56,16 +78,16 @@ def ic_list(n_ics):
            [("shear_layer", s, "periodic") for s in range(n_ics - half)]
 
 
-def generate(n=512, n_ics=128, steps=400, dump_every=8, keep_mord=0.3, cap=20_000_000, log=print):
+def generate(n=512, n_ics=128, steps=400, dump_every=8, keep_mord=0.3, cap=20_000_000, log=print, fam=R2):
     import torch
     gpm = _package()
-    X = torch.empty((cap, IN), dtype=torch.float32, device="cuda")
+    X = torch.empty((cap, fam.inputs), dtype=torch.float32, device="cuda")
     Y = torch.empty(cap, dtype=torch.int32, device="cuda")
     cnt = torch.zeros(1, dtype=torch.int64, device="cuda")
     owner = []  # (ic index, first sample, end)
     t0 = time.time()
     for k, (kind, seed, bc) in enumerate(ic_list(n_ics)):
-        s = gpm.Solver(gpm.Config(lf=n, Mord=MORD, ngp=NGP, BC_type=bc))
+        s = gpm.Solver(fam.config(gpm, n, bc))
         s.set_state(gpm.ic.make(kind, n, seed))
         first = int(cnt.item())
         done = 0
@@ -107,10 +129,10 @@ def gather(X, Y, ranges):
     return X[idx].contiguous(), Y[idx].contiguous()
 
 
-def train(Xtr, Ytr, epochs=4, batch=8192, lr=1e-3, weight_decay=1e-2, seed=0, log=print):
+def train(Xtr, Ytr, epochs=4, batch=8192, lr=1e-3, weight_decay=1e-2, seed=0, log=print, fam=R2):
     import torch
     gpm = _package()
-    m = gpm.MLP(IN, C, max_batch=batch, seed=seed)
+    m = gpm.MLP(fam.inputs, fam.classes, max_batch=batch, seed=seed)
     g = torch.Generator(device="cuda")
     g.manual_seed(seed)
     n = Xtr.shape[0]
@@ -137,10 +159,11 @@ def shift_mord_logit(m, shift):
         m.set_params(p)
 
 
-def evaluate(m, X, Y):
+def evaluate(m, X, Y, fam=R2):
     """Confusion matrix over the order classes and the troubled-cell view
     (predicted below Mord vs. final order below Mord)."""
-    pred = np.argmax(m.logits(X).cpu().numpy(), axis=1)
+    C = fam.classes
+    pred = np.argmax(m.logits(X).cpu().numpy()[:, :C], axis=1)
     y = Y.cpu().numpy()
     cm = np.zeros((C, C), dtype=np.int64)
     np.add.at(cm, (y, pred), 1)
@@ -158,7 +181,7 @@ def evaluate(m, X, Y):
                          "support": int(t_true.sum())}, "samples": int(y.size)}
 
 
-def in_step(m, n=2048, steps=100, kind="four_quadrant", seed=0, bc="neumann", gate=True, log=print):
+def in_step(m, n=2048, steps=100, kind="four_quadrant", seed=0, bc="neumann", gate=True, log=print, fam=R2):
     """Plain vs NN-GP-MOOD on the same initial state: throughput, second-pass
     cells, a posteriori detections, and the density L1 distance after `steps`."""
     import torch
@@ -167,8 +190,7 @@ def in_step(m, n=2048, steps=100, kind="four_quadrant", seed=0, bc="neumann", ga
     out = {}
     states = {}
     for label, use_nn in (("plain", False), ("nn", True)):
-        s = gpm.Solver(gpm.Config(lf=n, Mord=MORD, ngp=NGP, BC_type=bc,
-                                  method=gpm.capi.NN_GP_MOOD if use_nn else gpm.capi.GP_MOOD))
+        s = gpm.Solver(fam.config(gpm, n, bc, method=gpm.capi.NN_GP_MOOD if use_nn else gpm.capi.GP_MOOD))
         if use_nn:
             s.set_predictor(m, gate=gate)
         s.set_state(U0)
@@ -192,13 +214,15 @@ def in_step(m, n=2048, steps=100, kind="four_quadrant", seed=0, bc="neumann", ga
     rp, rn = states["plain"][..., 0], states["nn"][..., 0]
     out["density_l1_distance"] = float(np.abs(rn - rp).sum() / np.abs(rp).sum())
     out["speedup"] = out["nn"]["cell_updates_per_s"] / out["plain"]["cell_updates_per_s"]
-    out["workload"] = f"{n}x{n} GP-R2 (Mord 5, ngp 3) {kind} seed {seed} {bc}, {steps} steps, gate {int(gate)}"
+    out["workload"] = (f"{n}x{n} GP-R{(fam.mord - 1) // 2} (Mord {fam.mord}, ngp {fam.ngp}, cascade {fam.cascade}) "
+                       f"{kind} seed {seed} {bc}, {steps} steps, gate {int(gate)}")
     return out
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--out", required=True)
+    ap.add_argument("--mord", type=int, default=5, choices=(5, 7))
     ap.add_argument("--n", type=int, default=512)
     ap.add_argument("--n-ics", type=int, default=128)
     ap.add_argument("--steps", type=int, default=400)
@@ -213,36 +237,43 @@ def main():
     ap.add_argument("--eval-steps", type=int, default=100)
     args = ap.parse_args()
     os.makedirs(args.out, exist_ok=True)
+    fam = Family(args.mord)
 
     def log(msg):
         print(msg, flush=True)
 
     t0 = time.time()
-    X, Y, owner = generate(args.n, args.n_ics, args.steps, args.dump_every, args.keep_mord, args.cap, log)
+    X, Y, owner = generate(args.n, args.n_ics, args.steps, args.dump_every, args.keep_mord, args.cap, log, fam=fam)
     tr, te = split(owner, args.n_ics, holdout_per_family=max(1, args.n_ics // 16))
     Xtr, Ytr = gather(X, Y, tr)
     Xte, Yte = gather(X, Y, te)
-    counts = np.bincount(Y.cpu().numpy(), minlength=C).tolist()
+    counts = np.bincount(Y.cpu().numpy(), minlength=fam.classes).tolist()
     log(f"[data] {X.shape[0]} samples (train {Xtr.shape[0]}, test {Xte.shape[0]}), class counts {counts}, "
         f"{time.time() - t0:.1f}s")
     del X, Y
-    m, hist = train(Xtr, Ytr, args.epochs, args.batch, args.lr, log=log)
+    m, hist = train(Xtr, Ytr, args.epochs, args.batch, args.lr, log=log, fam=fam)
     shift_mord_logit(m, args.mord_logit_shift)
-    m.save(os.path.join(args.out, "nn_r2.npz"))
-    ev_train = evaluate(m, Xtr[:2_000_000], Ytr[:2_000_000])
-    ev_test = evaluate(m, Xte, Yte)
+    m.save(os.path.join(args.out, f"nn_{fam.tag}.npz"))
+    ev_train = evaluate(m, Xtr[:2_000_000], Ytr[:2_000_000], fam=fam)
+    ev_test = evaluate(m, Xte, Yte, fam=fam)
     log(f"[eval] test: {ev_test}")
-    report = {"dataset": {"n": args.n, "n_ics": args.n_ics, "steps": args.steps, "dump_every": args.dump_every,
+    report = {"family": {"mord": fam.mord, "ngp": fam.ngp, "inputs": fam.inputs, "classes": fam.classes,
+                         "cascade": fam.cascade},
+              "dataset": {"n": args.n, "n_ics": args.n_ics, "steps": args.steps, "dump_every": args.dump_every,
                           "keep_mord": args.keep_mord, "samples": int(Xtr.shape[0] + Xte.shape[0]),
                           "train": int(Xtr.shape[0]), "test": int(Xte.shape[0]), "class_counts": counts},
               "training": {"epochs": args.epochs, "batch": args.batch, "lr": args.lr, "history": hist,
                            "mord_logit_shift": args.mord_logit_shift},
               "eval_train": ev_train, "eval_test": ev_test}
     del Xtr, Ytr, Xte, Yte
-    report["in_step"] = in_step(m, n=args.eval_n, steps=args.eval_steps, log=log)
-    report["in_step_blast"] = in_step(m, n=args.eval_n, steps=args.eval_steps, kind="blast", bc="reflective", log=log)
+    if fam.mord == 5:
+        evals = (("in_step", "four_quadrant", "neumann"), ("in_step_blast", "blast", "reflective"))
+    else:
+        evals = (("in_step", "shear_layer", "periodic"), ("in_step_four_quadrant", "four_quadrant", "neumann"))
+    for key, kind, bc in evals:
+        report[key] = in_step(m, n=args.eval_n, steps=args.eval_steps, kind=kind, bc=bc, log=log, fam=fam)
     report["seconds"] = time.time() - t0
-    json.dump(report, open(os.path.join(args.out, "nn_report.json"), "w"), indent=1)
+    json.dump(report, open(os.path.join(args.out, f"nn_{fam.tag}_report.json"), "w"), indent=1)
     log(f"[done] {report['seconds']:.1f}s")
 
 
