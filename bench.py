@@ -130,6 +130,65 @@ def adjoint_measure(gpm, K, n=512, m=20, w=64):
                     "unit: cell-stages per second"}
 
 
+def large_run(gpm, steps, n=8192, seed=0):
+    """The large NN-GP-MOOD run: 8192^2 FP64, GP-R3 with the full cascade
+    7 -> 5 -> 3 -> 1, HLLC, SSP-RK3, periodic, shear-layer IC; plain GP-MOOD and
+    NN-GP-MOOD with the committed GP-R3 predictor from the same state.  Reports
+    throughput, MOOD activity, the density L1 distance between the two and the
+    drift of total mass and energy (periodic: conserved to round-off)."""
+    import torch
+    wfile = os.path.join(ROOT, "gp-mood_amd", "weights", "nn_r3.npz")
+    U0 = torch.from_numpy(gpm.ic.make("shear_layer", n, seed)).cuda()
+    tot0 = U0.sum(dim=(0, 1))
+    out, final = {}, {}
+    m = gpm.MLP.load(wfile) if os.path.exists(wfile) else None
+    for label in ("plain", "nn"):
+        if label == "nn" and m is None:
+            continue
+        cfg = gpm.Config(lf=n, Mord=7, ngp=4, BC_type="periodic", cascade="full",
+                         method=gpm.capi.NN_GP_MOOD if label == "nn" else gpm.capi.GP_MOOD)
+        s = gpm.Solver(cfg)
+        if label == "nn":
+            s.set_predictor(m, gate=True)
+        torch.cuda.synchronize()
+        s.set_state_device(U0.data_ptr())
+        s.set_time(0.0)
+        s.reset_stats()
+        print(f"[bench] large run {label}: {steps} steps at {n}^2", file=sys.stderr, flush=True)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        done = 0
+        while done < steps:
+            k = min(100, steps - done)
+            s.run(k)
+            done += k
+            torch.cuda.synchronize()
+            print(f"[bench]   {label} {done}/{steps} steps, {time.perf_counter() - t0:.1f}s", file=sys.stderr, flush=True)
+        el = time.perf_counter() - t0
+        st = s.stats()
+        U = torch.empty_like(U0)
+        s.get_state_device(U.data_ptr())
+        torch.cuda.synchronize()
+        tot = U.sum(dim=(0, 1))
+        final[label] = U[..., 0].clone()
+        out[label] = {"cell_updates_per_s": 3.0 * n * n * steps / el, "ms_per_step": 1e3 * el / steps,
+                      "second_pass_fraction": st["second_pass_cells"] / (3.0 * n * n * steps),
+                      "max_passes": st["max_passes"], "a_priori_lowered_fraction":
+                          st["a_priori_lowered"] / (3.0 * n * n * steps),
+                      "mass_drift": float(abs(tot[0] - tot0[0]) / abs(tot0[0])),
+                      "energy_drift": float(abs(tot[3] - tot0[3]) / abs(tot0[3]))}
+        del U
+        s.close()
+    if "nn" in final:
+        out["speedup"] = out["nn"]["cell_updates_per_s"] / out["plain"]["cell_updates_per_s"]
+        out["density_l1_distance"] = float((final["nn"] - final["plain"]).abs().sum() / final["plain"].abs().sum())
+    out["workload"] = (f"{n}x{n} FP64 GP-R3 (Mord 7, ngp 4), full cascade, HLLC, SSP-RK3, CFL 0.8, periodic, "
+                       f"shear layer seed {seed}, {steps} steps; NN: gp-mood_amd/weights/nn_r3.npz")
+    if m is not None:
+        m.close()
+    return out
+
+
 def nn_comparison(gpm, n, steps):
     """Plain GP-MOOD vs NN-GP-MOOD (committed predictor weights) on the
     2048^2 GP-R2 four-quadrant workload: throughput, second-pass cells and the
@@ -166,6 +225,8 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=1)
     ap.add_argument("--nn-steps", type=int, default=100, help="steps of the plain vs NN-MOOD comparison (0: skip)")
     ap.add_argument("--dropin-steps", type=int, default=3, help="host-array drop-in steps timed (0: skip)")
+    ap.add_argument("--large-steps", type=int, default=0,
+                    help="steps of the 8192^2 GP-R3 periodic shear-layer run, plain and NN-GP-MOOD (0: skip)")
     ap.add_argument("--adjoint-steps", type=int, default=200,
                     help="K of the 512^2 GP-R2 initial-state sensitivity run (0: skip)")
     ap.add_argument("--calib-copy-mib", type=int, default=0,
@@ -255,6 +316,9 @@ def main():
     if args.nn_steps > 0:
         print(f"[bench] plain vs NN-GP-MOOD, {args.nn_steps} steps at {n}^2 GP-R2", file=sys.stderr, flush=True)
         nn = nn_comparison(gpm, n, args.nn_steps)
+    large = None
+    if args.large_steps > 0:
+        large = large_run(gpm, args.large_steps)
     adj = None
     if args.adjoint_steps > 0:
         print(f"[bench] adjoint, K={args.adjoint_steps} at 512^2 GP-R2", file=sys.stderr, flush=True)
@@ -275,6 +339,7 @@ def main():
                  "second_pass_cells_per_step": st["second_pass_cells"] / max(1, args.steps),
                  "mood_kernel_avg_ms": kt["mood_ms"]},
         "roofline": roofline, "cpu_baseline": cpu, "nn_mood": nn, "host_array_drop_in": dropin, "adjoint": adj,
+        "large_run": large,
     }
     print(json.dumps(out))
 

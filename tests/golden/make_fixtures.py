@@ -78,6 +78,10 @@ STEP_CASES = {
     "r3_n64_ref_bl6": ("r3_n64_ref", "blast", 6, 20),
     "r3_n64_per_sh7": ("r3_n64_per", "shear_layer", 7, 20),
     "r2_n64_neu_fq8": ("r2_n64_neu", "four_quadrant", 8, 25),
+    # periodic with quadrant boundaries on the seam x = 0 (= 1): MOOD lowers
+    # cells next to the seam, where ghost cells keep order Mord
+    "r2_n64_per_fq3": ("r2_n64_per", "four_quadrant", 3, 10),
+    "r3_n64_per_fq2": ("r3_n64_per", "four_quadrant", 2, 10),
 }
 # long runs stored as checksums: name: (config, IC kind, seed, steps)
 HASH_CASES = {

@@ -1,0 +1,59 @@
+"""The large configuration at its full size: 8192^2 FP64, GP-R3 with the full
+cascade, periodic shear layer (2.1 GB per state), plain GP-MOOD and
+NN-GP-MOOD with the committed GP-R3 predictor.  Size-independent properties:
+the totals of mass, momentum and energy stay constant up to the reference's
+seam effect (ghost cells keep order Mord, so the two evaluations of a seam
+face differ once MOOD lowers a cell next to the seam; see
+test_oracle_invariants.py) -- about 1e-12 here, against 1e-14 for round-off
+alone; the state stays finite with positive density and pressure; the
+learned a priori orders move the density by less than 1e-3 (relative L1)."""
+import os
+
+import numpy as np
+import pytest
+
+from conftest import ROOT, has_gpu
+
+pytestmark = pytest.mark.gpu
+
+
+def test_8192_shear_layer_plain_and_nn(gpm):
+    if not has_gpu():
+        pytest.skip("no GPU")
+    import torch
+
+    n, steps = 8192, 10
+    print(f"[case 8192^2 GP-R3 periodic shear layer, {steps} steps]", flush=True)
+    U0 = torch.from_numpy(gpm.ic.make("shear_layer", n, 0)).cuda()
+    tot0 = U0.sum(dim=(0, 1)).cpu().numpy()
+    m = gpm.MLP.load(os.path.join(ROOT, "gp-mood_amd", "weights", "nn_r3.npz"))
+    rho = {}
+    for label in ("plain", "nn"):
+        s = gpm.Solver(gpm.Config(lf=n, Mord=7, ngp=4, BC_type="periodic", cascade="full",
+                                  method=gpm.capi.NN_GP_MOOD if label == "nn" else gpm.capi.GP_MOOD))
+        if label == "nn":
+            s.set_predictor(m, gate=True)
+        torch.cuda.synchronize()
+        s.set_state_device(U0.data_ptr())
+        s.run(steps)
+        U = torch.empty_like(U0)
+        s.get_state_device(U.data_ptr())
+        torch.cuda.synchronize()
+        tot = U.sum(dim=(0, 1)).cpu().numpy()
+        scale = np.array([abs(tot0[0]), abs(tot0[0]), abs(tot0[0]), abs(tot0[3])])
+        drift = np.abs(tot - tot0) / scale
+        print(f"  {label}: conservation drift {drift}, lowered a priori {s.stats()['a_priori_lowered']}", flush=True)
+        assert np.all(drift < 1e-10)
+        assert bool(torch.isfinite(U).all())
+        r = U[..., 0]
+        p = 0.4 * (U[..., 3] - 0.5 * (U[..., 1] ** 2 + U[..., 2] ** 2) / r)
+        assert float(r.min()) > 0 and float(p.min()) > 0
+        if label == "nn":
+            assert s.stats()["a_priori_lowered"] > 0
+        rho[label] = r.clone()
+        del U
+        s.close()
+    l1 = float((rho["nn"] - rho["plain"]).abs().sum() / rho["plain"].abs().sum())
+    print(f"  density L1 NN vs plain {l1:.2e}", flush=True)
+    assert l1 < 1e-3
+    m.close()

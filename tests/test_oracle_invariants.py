@@ -4,7 +4,7 @@ preservation, conservation, and the positivity cascade."""
 import numpy as np
 import pytest
 
-from conftest import load_package
+from conftest import golden, load_package
 
 GAMMA = 1.4
 
@@ -140,3 +140,22 @@ def test_positivity_cascade(orc):
     rho = U[4:-4, 4:-4, 0]
     p = (GAMMA - 1) * (U[4:-4, 4:-4, 3] - 0.5 * (U[4:-4, 4:-4, 1] ** 2 + U[4:-4, 4:-4, 2] ** 2) / rho)
     assert np.all(np.isfinite(U)) and rho.min() > 0 and p.min() > 0
+
+
+def test_periodic_seam_is_not_conservative_in_the_reference():
+    """The reference gives ghost cells order Mord in every MOOD pass
+    (mod_FE.f90:48 resets CellGPO over the whole array; DETECTION,
+    mod_subroutine_MOOD.f90:146-185, lowers interior cells only).  With periodic
+    walls the face on the seam is evaluated twice -- from the ghost side at x=0
+    with min(Mord, CellGPO(1)) and from the ghost side at x=1 with
+    min(CellGPO(lf), Mord) -- so once MOOD lowers a cell next to the seam the
+    two copies differ and total mass is no longer conserved.  The reference's
+    own run (golden fixture, quadrant boundaries on the seam) shows it; the
+    restatement reproduces that run bit for bit (test_oracle.py)."""
+    z = golden("step_r2_n64_per_fq3.npz")
+    n = int(z["n"])
+    m = z["masks"]
+    assert ((m[:, :, 0] < 5) | (m[:, :, n - 1] < 5)).any()  # lowered cells next to the seam
+    U0, U = z["U0"], z["U"][4:-4, 4:-4]
+    drift = abs(U[..., 0].sum() - U0[..., 0].sum()) / U0[..., 0].sum()
+    assert drift > 1e-6
