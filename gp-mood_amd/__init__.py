@@ -40,12 +40,13 @@ class Config:
     Ly: float = 1.0
     ell_o_dx: float = 12.0
     method: int = capi.GP_MOOD
+    numFlux: str = "hllc"
 
     def to_c(self):
         nf = self.lf if self.nf is None else self.nf
         return capi.GpmConfig(
             lf=self.lf, nf=nf, mord=self.Mord, ngp=self.ngp, method=self.method, integrator=capi.SSP_RK3,
-            num_flux=capi.HLLC, bc_type=capi.BC_IDS[self.BC_type], dmp=int(self.DMP), u2=int(self.U2),
+            num_flux=capi.FLUX_IDS[self.numFlux], bc_type=capi.BC_IDS[self.BC_type], dmp=int(self.DMP), u2=int(self.U2),
             u2_tol=int(self.U2_tol), cascade=capi.CASCADE_IDS[self.cascade], cfl=self.CFL, lx=self.Lx,
             ly=self.Ly, ell_o_dx=self.ell_o_dx)
 

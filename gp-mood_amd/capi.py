@@ -20,9 +20,10 @@ LIB_PATH = os.environ.get("GPM_LIB_PATH") or os.path.join(HERE, "libgpmood.so")
 GP_MOOD = 2
 NN_GP_MOOD = 203
 SSP_RK3 = 10
-HLLC = 100
-NEUMANN, PERIODIC, REFLECTIVE = 17, 18, 20
-BC_IDS = {"neumann": NEUMANN, "periodic": PERIODIC, "reflective": REFLECTIVE}
+HLLC, HLL, UNSPLIT = 100, 102, 103
+FLUX_IDS = {"hllc": HLLC, "hll": HLL, "unsplit": UNSPLIT}
+NEUMANN, PERIODIC, REFLECTIVE, KH_BC = 17, 18, 20, 403
+BC_IDS = {"neumann": NEUMANN, "periodic": PERIODIC, "reflective": REFLECTIVE, "kh": KH_BC}
 CASCADE_IDS = {"reference": 0, "full": 1}
 
 # Every symbol include/gpmood.h declares.

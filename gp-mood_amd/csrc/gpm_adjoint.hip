@@ -181,7 +181,7 @@ __device__ __forceinline__ void hllc_x_vjp(const double ul[4], const double ur[4
   }
 }
 
-// Reverse mode of hllc_dir: a y-face solve is the x-solve on momentum-swapped
+// Reverse mode of riemann_dir<kHLLC>: a y-face solve is the x-solve on momentum-swapped
 // states with the flux's momenta swapped back.
 __device__ __forceinline__ void hllc_vjp(int dir, const double ul[4], const double ur[4], const double Fb[4],
                                          double ulb[4], double urb[4])
@@ -384,11 +384,11 @@ __global__ __launch_bounds__(256) void k_adj_fold(AdjFoldArgs a)
         for (int ey = 0; ey <= 2 * (R + 1); ++ey) {
             const int qy = ey == 0 ? cy : (ey <= R + 1 ? -ey : N - 1 + ey - (R + 1));
             bool fy = false;
-            if (ey > 0 && bc_map(qy, N, a.bc, fy) != cy) continue;
+            if (ey > 0 && bc_map(qy, N, bc_y(a.bc), fy) != cy) continue;
             for (int ex = 0; ex <= 2 * (R + 1); ++ex) {
                 const int qx = ex == 0 ? cx : (ex <= R + 1 ? -ex : N - 1 + ex - (R + 1));
                 bool fx = false;
-                if (ex > 0 && bc_map(qx, N, a.bc, fx) != cx) continue;
+                if (ex > 0 && bc_map(qx, N, bc_x(a.bc), fx) != cx) continue;
                 const double4 gq = a.G[(size_t)(qy + R + 1) * E + (qx + R + 1)];
                 v[0] += gq.x;
                 v[1] += fx ? -gq.y : gq.y;

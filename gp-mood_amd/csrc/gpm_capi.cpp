@@ -93,8 +93,9 @@ hipError_t dalloc(T **p, size_t n)
     return hipMalloc(reinterpret_cast<void **>(p), n * sizeof(T) > 0 ? n * sizeof(T) : 1);
 }
 
-// Host form of the ghost-cell rule (Boundary_C, BC.f90:18-135): the interior
-// cell a ghost index copies and whether the normal momentum flips sign.
+// Host form of the ghost-cell rule (Boundary_C, BC.f90:18-135, 383-406): the
+// interior cell a ghost index copies and whether the normal momentum flips
+// sign, for the rule of one axis (KH_BC: periodic in x, Neumann in y).
 int host_bc_map(int i, int N, int bc, bool &flip)
 {
     flip = false;
@@ -113,16 +114,19 @@ int host_bc_map(int i, int N, int bc, bool &flip)
     return i;
 }
 
+int host_bc_x(int bc) { return bc == GPM_KH_BC ? GPM_PERIODIC : bc; }
+int host_bc_y(int bc) { return bc == GPM_KH_BC ? GPM_NEUMANN : bc; }
+
 // interior [N][N][4] -> ghosted [N+8][N+8][4] with the boundary rule
 void fill_ghosted(const double *in, double *out, int N, int bc)
 {
     const int W = N + 8;
     for (int jj = 0; jj < W; ++jj) {
         bool fy;
-        const int sj = host_bc_map(jj - 4, N, bc, fy);
+        const int sj = host_bc_map(jj - 4, N, host_bc_y(bc), fy);
         for (int ii = 0; ii < W; ++ii) {
             bool fx;
-            const int si = host_bc_map(ii - 4, N, bc, fx);
+            const int si = host_bc_map(ii - 4, N, host_bc_x(bc), fx);
             const double *s = in + ((size_t)sj * N + si) * 4;
             double *d = out + ((size_t)jj * W + ii) * 4;
             d[0] = s[0];
@@ -263,7 +267,7 @@ int enqueue_stages(gpm_ctx *c)
             HIPCHK(c, hipEventRecord(e0, c->stream));
         }
         // a multiple of 8 workgroups, so the XCD-aware remap in k_stage is a bijection
-        HIPCHK(c, launch_stage(c->mord, c->ngp, stage, s, (c->ntiles + 7) / 8 * 8, c->stream));
+        HIPCHK(c, launch_stage(c->mord, c->ngp, c->cfg.num_flux, stage, s, (c->ntiles + 7) / 8 * 8, c->stream));
         if (c->timing) HIPCHK(c, hipEventRecord(e1, c->stream));
 
         MoodArgs m{};
@@ -283,7 +287,7 @@ int enqueue_stages(gpm_ctx *c)
             c->ev_pairs.push_back({(int)i0, 1});
             HIPCHK(c, hipEventRecord(e0, c->stream));
         }
-        HIPCHK(c, hipLaunchCooperativeKernel(mood_kernel(c->mord, c->ngp, stage), dim3(c->mood_blocks[stage - 1]),
+        HIPCHK(c, hipLaunchCooperativeKernel(mood_kernel(c->mord, c->ngp, c->cfg.num_flux, stage), dim3(c->mood_blocks[stage - 1]),
                                              dim3(kMoodThreads), args, 0, c->stream));
         if (c->timing) HIPCHK(c, hipEventRecord(e1, c->stream));
         if (c->rec_masks && c->rec_count < c->rec_cap) {
@@ -336,11 +340,13 @@ int gpm_create(const gpm_config *cfg, gpm_ctx **out)
     if (cfg->lf != cfg->nf || cfg->lf < 8) return bad(GPM_EINVAL, "square grids with lf = nf >= 8 only");
     if (cfg->lx != cfg->ly) return bad(GPM_EINVAL, "square cells only (Lx = Ly)");
     if (cfg->integrator != GPM_SSP_RK3) return bad(GPM_EUNSUP, "integrator must be SSP_RK3");
-    if (cfg->num_flux != GPM_HLLC) return bad(GPM_EUNSUP, "numFlux must be HLLC");
+    if (cfg->num_flux != GPM_HLLC && cfg->num_flux != GPM_HLL && cfg->num_flux != GPM_UNSPLIT)
+        return bad(GPM_EUNSUP, "numFlux must be HLLC, HLL or unsplit");
     if (cfg->method != GPM_GP_MOOD && cfg->method != GPM_NN_GP_MOOD)
         return bad(GPM_EUNSUP, "method must be GP_MOOD or NN_GP_MOOD");
-    if (cfg->bc_type != GPM_NEUMANN && cfg->bc_type != GPM_PERIODIC && cfg->bc_type != GPM_REFLECTIVE)
-        return bad(GPM_EUNSUP, "BC_type must be Neumann, Periodic or Reflective");
+    if (cfg->bc_type != GPM_NEUMANN && cfg->bc_type != GPM_PERIODIC && cfg->bc_type != GPM_REFLECTIVE &&
+        cfg->bc_type != GPM_KH_BC)
+        return bad(GPM_EUNSUP, "BC_type must be Neumann, Periodic, Reflective or KH_BC");
     if (cfg->cascade != GPM_CASCADE_REFERENCE && cfg->cascade != GPM_CASCADE_FULL)
         return bad(GPM_EINVAL, "unknown cascade");
     if (!supported(cfg->mord, cfg->ngp)) return bad(GPM_EUNSUP, "(Mord, ngp) combination not compiled");
@@ -385,7 +391,7 @@ int gpm_create(const gpm_config *cfg, gpm_ctx **out)
     if (const char *env = std::getenv("GPM_MOOD_BLOCKS")) want = std::atoi(env) > 0 ? std::atoi(env) : ncu;
     for (int st = 0; st < 3; ++st) {
         int per = 0;
-        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, mood_kernel(c->mord, c->ngp, st + 1), kMoodThreads, 0);
+        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, mood_kernel(c->mord, c->ngp, c->cfg.num_flux, st + 1), kMoodThreads, 0);
         if (e != hipSuccess || per <= 0) per = 1;
         c->mood_blocks[st] = want < per * ncu ? want : per * ncu;
     }
@@ -759,7 +765,7 @@ int gpm_frozen_run(gpm_ctx *c, const int8_t *masks_dev, const double *dts, int32
             s.apriori = 1;
             s.frozen = 1;
             s.pred = masks_dev + ((size_t)st * 3 + (stage - 1)) * n;
-            HIPCHK(c, launch_stage(c->mord, c->ngp, stage, s, (c->ntiles + 7) / 8 * 8, c->stream));
+            HIPCHK(c, launch_stage(c->mord, c->ngp, c->cfg.num_flux, stage, s, (c->ntiles + 7) / 8 * 8, c->stream));
         }
         std::swap(c->dU, c->dA);
     }
@@ -773,6 +779,7 @@ int gpm_adjoint(gpm_ctx *c, const double *stage_states_dev, const int8_t *masks_
     if (!c || !stage_states_dev || !masks_dev || !dts || !ubar_dev || nsteps < 0)
         return fail(c, GPM_EINVAL, "invalid argument");
     if (!adj_supported(c->mord, c->ngp)) return fail(c, GPM_EUNSUP, "adjoint needs (Mord, ngp) in {(3,2), (5,3), (7,4)}");
+    if (c->cfg.num_flux != GPM_HLLC) return fail(c, GPM_EUNSUP, "adjoint needs numFlux = HLLC");
     const int N = c->N, R = (c->mord - 1) / 2;
     const size_t n = (size_t)N * N, W = (size_t)(N + 2), E = (size_t)(N + 2 * R + 2);
     if (!c->adj_ub) {

@@ -11,8 +11,10 @@ namespace gpm {
 
 constexpr double kGamma = 1.4;  // constants.f90:9 (y = 1.4)
 
-// Boundary-condition ids (constants.f90:25).
-enum { kNeumann = 17, kPeriodic = 18, kReflective = 20 };
+// Boundary-condition ids (constants.f90:25-26).
+enum { kNeumann = 17, kPeriodic = 18, kReflective = 20, kKH = 403 };
+// Flux ids (constants.f90:21) and the template tags of the Riemann solvers.
+enum { kHLLC = 100, kHLL = 102, kUnsplit = 103 };
 // Face ids (constants.f90:37): iL, iT, iR, iB.
 enum { kL = 0, kT = 1, kR = 2, kB = 3 };
 
@@ -70,11 +72,16 @@ __device__ __forceinline__ int bc_map(int i, int N, int bc, bool &flip)
     return i;
 }
 
+// The rule of each axis: KH_BC (BC.f90:383-406) is periodic in x and Neumann
+// in y; the other boundary types use one rule on both axes.
+__device__ __forceinline__ int bc_x(int bc) { return bc == kKH ? kPeriodic : bc; }
+__device__ __forceinline__ int bc_y(int bc) { return bc == kKH ? kNeumann : bc; }
+
 __device__ __forceinline__ double4 load_bc(const double4 *__restrict__ U, int N, int bc, int i, int j)
 {
     bool fx, fy;
-    int si = bc_map(i, N, bc, fx);
-    int sj = bc_map(j, N, bc, fy);
+    int si = bc_map(i, N, bc_x(bc), fx);
+    int sj = bc_map(j, N, bc_y(bc), fy);
     double4 v = U[(size_t)sj * N + si];
     if (fx) v.y = -v.y;
     if (fy) v.z = -v.z;
@@ -152,20 +159,118 @@ __device__ __forceinline__ void hllc(const double ul[4], const double ur[4], dou
     if (!hllc_adm<DIR>(ul, ur, F)) hllc_inadmissible(ur, F);
 }
 
-// y-face flux through the x-direction solver: HLLC in y equals HLLC in x on
-// states with the two momenta swapped, with the flux's momentum components
-// swapped back (pressure, speeds and star states are symmetric under the swap,
-// operation for operation), so one code path serves both directions.
-__device__ __forceinline__ void hllc_dir(int dir, const double ul[4], const double ur[4], double F[4])
+// HLL flux (mod_HLL.f90:11-166); false, leaving F untouched, when either
+// state is inadmissible (the flux is then that of hllc_inadmissible).
+template <int DIR>
+__device__ __forceinline__ bool hll_adm(const double ul[4], const double ur[4], double F[4])
+{
+    const double vxl = ul[1] / ul[0], vyl = ul[2] / ul[0];
+    const double pl = pressure(ul[0], ul[1], ul[2], ul[3]);
+    const double cl = sqrt(kGamma * pl / ul[0]);
+    const double vxr = ur[1] / ur[0], vyr = ur[2] / ur[0];
+    const double pr = pressure(ur[0], ur[1], ur[2], ur[3]);
+    const double cr = sqrt(kGamma * pr / ur[0]);
+    const bool adm = !(ul[0] <= 0. || pl <= 0. || isnan(ul[0]) || isnan(pl) ||
+                       ur[0] <= 0. || pr <= 0. || isnan(ur[0]) || isnan(pr));
+    if (!adm) return false;
+    const double vnl = DIR == 0 ? vxl : vyl, vnr = DIR == 0 ? vxr : vyr;
+    double Fl[4], Fr[4];
+    if (DIR == 0) {
+        Fl[0] = ul[1]; Fl[1] = ul[0] * (vxl * vxl) + pl; Fl[2] = ul[2] * vxl; Fl[3] = vxl * (ul[3] + pl);
+        Fr[0] = ur[1]; Fr[1] = ur[0] * (vxr * vxr) + pr; Fr[2] = ur[2] * vxr; Fr[3] = vxr * (ur[3] + pr);
+    } else {
+        Fl[0] = ul[2]; Fl[1] = ul[1] * vyl; Fl[2] = ul[0] * (vyl * vyl) + pl; Fl[3] = vyl * (ul[3] + pl);
+        Fr[0] = ur[2]; Fr[1] = ur[1] * vyr; Fr[2] = ur[0] * (vyr * vyr) + pr; Fr[3] = vyr * (ur[3] + pr);
+    }
+    const double SL = fmin2(vnl - cl, vnr - cr);
+    const double SR = fmax2(vnl + cl, vnr + cr);
+    if (SL > 0.) {
+#pragma unroll
+        for (int k = 0; k < 4; ++k) F[k] = Fl[k];
+    } else if (SR < 0.) {
+#pragma unroll
+        for (int k = 0; k < 4; ++k) F[k] = Fr[k];
+    } else {
+#pragma unroll
+        for (int k = 0; k < 4; ++k) F[k] = (SR * Fl[k] - SL * Fr[k] + SR * SL * (ur[k] - ul[k])) / (SR - SL);
+    }
+    return true;
+}
+
+// Pressure-velocity ("unsplit") flux (mod_unsplit.f90:11-71); false when
+// either state is inadmissible.
+template <int DIR>
+__device__ __forceinline__ bool unsplit_adm(const double ul[4], const double ur[4], double F[4])
+{
+    const double vxl = ul[1] / ul[0], vyl = ul[2] / ul[0];
+    const double pl = pressure(ul[0], ul[1], ul[2], ul[3]);
+    const double cl = sqrt(kGamma * pl / ul[0]);
+    const double vxr = ur[1] / ur[0], vyr = ur[2] / ur[0];
+    const double pr = pressure(ur[0], ur[1], ur[2], ur[3]);
+    const double cr = sqrt(kGamma * pr / ur[0]);
+    const bool adm = !(ul[0] <= 0. || pl <= 0. || isnan(ul[0]) || isnan(pl) ||
+                       ur[0] <= 0. || pr <= 0. || isnan(ur[0]) || isnan(pr));
+    if (!adm) return false;
+    const double a = 1.1 * fmax2(ul[0] * cl, ur[0] * cr);
+    const double vnl = DIR == 0 ? vxl : vyl, vnr = DIR == 0 ? vxr : vyr;
+    const double vtl = DIR == 0 ? vyl : vxl, vtr = DIR == 0 ? vyr : vxr;
+    const double pistar = 0.5 * (pl + pr) - 0.5 * a * (vnr - vnl);
+    const double ustar = 0.5 * (vnl + vnr) - 0.5 * (pr - pl) / a;
+    const double trans = -(0.5 * a * (vtr - vtl));
+    const bool left = ustar > 0.;
+    constexpr int md = DIR == 0 ? 1 : 2, om = DIR == 0 ? 2 : 1;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) F[k] = (left ? ul[k] : ur[k]) * ustar;
+    F[md] = F[md] + pistar;
+    F[om] = F[om] + trans;
+    F[3] = F[3] + pistar * ustar;
+    return true;
+}
+
+// Flux of an inadmissible face in mod_unsplit.f90:68: F = 1/(no-no), where the
+// reference's c2p_local calls write vx2 and vy2 into the one variable `no`,
+// which ends up holding the right state's vy**2.
+__device__ __forceinline__ void unsplit_inadmissible(const double ur[4], double F[4])
+{
+    const double vy = ur[2] / ur[0];
+    const double no = vy * vy;
+    const double z = no - no;
+    const double f = 1.0 / z;
+    F[0] = f; F[1] = f; F[2] = f; F[3] = f;
+}
+
+// The Riemann solver of numFlux FLUX on a DIR face.
+template <int FLUX, int DIR>
+__device__ __forceinline__ void riemann(const double ul[4], const double ur[4], double F[4])
+{
+    if (FLUX == kHLL) {
+        if (!hll_adm<DIR>(ul, ur, F)) hllc_inadmissible(ur, F);
+    } else if (FLUX == kUnsplit) {
+        if (!unsplit_adm<DIR>(ul, ur, F)) unsplit_inadmissible(ur, F);
+    } else {
+        if (!hllc_adm<DIR>(ul, ur, F)) hllc_inadmissible(ur, F);
+    }
+}
+
+// Run-time direction: a y-face solve is the x-solve on momentum-swapped states
+// with the flux's momenta swapped back (exact for all three solvers: their
+// y-direction formulas are the x-direction ones with the roles of the
+// momenta exchanged, operation for operation).
+template <int FLUX>
+__device__ __forceinline__ void riemann_dir(int dir, const double ul[4], const double ur[4], double F[4])
 {
     if (dir == 0) {
-        if (!hllc_adm<0>(ul, ur, F)) hllc_inadmissible(ur, F);
+        riemann<FLUX, 0>(ul, ur, F);
         return;
     }
     const double sl[4] = {ul[0], ul[2], ul[1], ul[3]}, sr[4] = {ur[0], ur[2], ur[1], ur[3]};
     double G[4];
-    if (hllc_adm<0>(sl, sr, G)) {
+    const bool ok = FLUX == kHLL ? hll_adm<0>(sl, sr, G) : FLUX == kUnsplit ? unsplit_adm<0>(sl, sr, G)
+                                                                          : hllc_adm<0>(sl, sr, G);
+    if (ok) {
         F[0] = G[0]; F[1] = G[2]; F[2] = G[1]; F[3] = G[3];
+    } else if (FLUX == kUnsplit) {
+        unsplit_inadmissible(ur, F);
     } else {
         hllc_inadmissible(ur, F);
     }
@@ -215,7 +320,7 @@ __device__ __forceinline__ void recon2(const DevTables *__restrict__ T, const Ac
 // Face flux: Riemann solve at each Gauss point of the face between cell A
 // (left/bottom) and B (right/top), then L_Flux = sum_j F_j * gauss_weight(ngp,j)
 // from 0 (mod_FE.f90:80-98 and 112-130).
-template <int ORD, int NGP, int DIR, class Acc>
+template <int ORD, int NGP, int DIR, int FLUX, class Acc>
 __device__ __forceinline__ void face_flux(const DevTables *__restrict__ T, const Acc &acc, int ai, int aj,
                                           int bi, int bj, double L[4])
 {
@@ -226,7 +331,7 @@ __device__ __forceinline__ void face_flux(const DevTables *__restrict__ T, const
         // First order: every Gauss point carries the same two states, so one
         // Riemann solve serves all of them; the quadrature sum is unchanged.
         double F[4];
-        hllc<DIR>(ua[0], ub[0], F);
+        riemann<FLUX, DIR>(ua[0], ub[0], F);
 #pragma unroll
         for (int j = 0; j < NGP; ++j) {
             const double w = T->gw[j];
@@ -237,7 +342,7 @@ __device__ __forceinline__ void face_flux(const DevTables *__restrict__ T, const
 #pragma unroll
         for (int j = 0; j < NGP; ++j) {
             double F[4];
-            hllc<DIR>(ua[j], ub[j], F);
+            riemann<FLUX, DIR>(ua[j], ub[j], F);
             const double w = T->gw[j];
 #pragma unroll
             for (int k = 0; k < 4; ++k) L[k] = L[k] + F[k] * w;
@@ -246,45 +351,22 @@ __device__ __forceinline__ void face_flux(const DevTables *__restrict__ T, const
 }
 
 // Face flux for a face whose order is only known at run time (ord <= MORD).
-template <int MORD, int NGP, int DIR, class Acc>
+template <int MORD, int NGP, int DIR, int FLUX, class Acc>
 __device__ __forceinline__ void face_flux_dyn(const DevTables *__restrict__ T, const Acc acc, int ord, int ai,
                                               int aj, int bi, int bj, double L[4])
 {
-    if (MORD >= 7 && ord == 7) face_flux<(MORD >= 7 ? 7 : 1), NGP, DIR>(T, acc, ai, aj, bi, bj, L);
-    else if (MORD >= 5 && ord == 5) face_flux<(MORD >= 5 ? 5 : 1), NGP, DIR>(T, acc, ai, aj, bi, bj, L);
-    else if (MORD >= 3 && ord == 3) face_flux<(MORD >= 3 ? 3 : 1), NGP, DIR>(T, acc, ai, aj, bi, bj, L);
-    else face_flux<1, NGP, DIR>(T, acc, ai, aj, bi, bj, L);
-}
-
-// Flux of one face at one Gauss point j (run-time face ids and direction):
-// the two GP reconstructions (same summation order as recon2) and one Riemann
-// solve.  Used by the sparse MOOD correction, one lane per (face, point).
-template <int ORD, class Acc>
-__device__ __forceinline__ void face_gp_flux(const DevTables *__restrict__ T, const Acc &acc, int fa, int fb,
-                                             int dir, int j, int ai, int aj, int bi, int bj, double F[4])
-{
-    constexpr int SZ = stencil_sz(ORD);
-    constexpr int OI = (ORD - 1) / 2;
-    double ua[4] = {0.0, 0.0, 0.0, 0.0}, ub[4] = {0.0, 0.0, 0.0, 0.0};
-#pragma unroll 5
-    for (int s = 0; s < SZ; ++s) {
-        const int dx = kSX[s], dy = kSY[s];
-        const double4 qa = acc(ai + dx, aj + dy);
-        const double4 qb = acc(bi + dx, bj + dy);
-        const double za = T->zT[OI][fa][s][j];
-        const double zb = T->zT[OI][fb][s][j];
-        ua[0] = ua[0] + za * qa.x; ua[1] = ua[1] + za * qa.y; ua[2] = ua[2] + za * qa.z; ua[3] = ua[3] + za * qa.w;
-        ub[0] = ub[0] + zb * qb.x; ub[1] = ub[1] + zb * qb.y; ub[2] = ub[2] + zb * qb.z; ub[3] = ub[3] + zb * qb.w;
-    }
-    hllc_dir(dir, ua, ub, F);
+    if (MORD >= 7 && ord == 7) face_flux<(MORD >= 7 ? 7 : 1), NGP, DIR, FLUX>(T, acc, ai, aj, bi, bj, L);
+    else if (MORD >= 5 && ord == 5) face_flux<(MORD >= 5 ? 5 : 1), NGP, DIR, FLUX>(T, acc, ai, aj, bi, bj, L);
+    else if (MORD >= 3 && ord == 3) face_flux<(MORD >= 3 ? 3 : 1), NGP, DIR, FLUX>(T, acc, ai, aj, bi, bj, L);
+    else face_flux<1, NGP, DIR, FLUX>(T, acc, ai, aj, bi, bj, L);
 }
 
 // Flux of one face at one Gauss point with the face order known only at run
 // time (ord <= MORD): one reconstruction loop whose trip count follows the
-// order (the same operations, in the same order, as face_gp_flux<ord>) and one
+// order (the same operations, in the same order, as recon2 for that order) and one
 // Riemann solve, so lanes of a wavefront that hold different orders run a
 // single code path instead of one inlined solver per order.
-template <int MORD, class Acc>
+template <int MORD, int FLUX, class Acc>
 __device__ __forceinline__ void face_gp_flux_dyn(const DevTables *__restrict__ T, const Acc &acc, int ord, int fa,
                                                  int fb, int dir, int j, int ai, int aj, int bi, int bj, double F[4])
 {
@@ -303,7 +385,7 @@ __device__ __forceinline__ void face_gp_flux_dyn(const DevTables *__restrict__ T
         ua[0] = ua[0] + za * qa.x; ua[1] = ua[1] + za * qa.y; ua[2] = ua[2] + za * qa.z; ua[3] = ua[3] + za * qa.w;
         ub[0] = ub[0] + zb * qb.x; ub[1] = ub[1] + zb * qb.y; ub[2] = ub[2] + zb * qb.z; ub[3] = ub[3] + zb * qb.w;
     }
-    hllc_dir(dir, ua, ub, F);
+    riemann_dir<FLUX>(dir, ua, ub, F);
 }
 
 struct MoodParams {

@@ -1,25 +1,8 @@
-// Kernel instantiations for Mord = 3, ngp = 2.
+// Kernel instantiations for Mord = 3, ngp = 2 (all numFlux options).
 #include "gpm_kernels_impl.h"
 
 namespace gpm {
 
-template <>
-hipError_t launch_stage_mg<3, 2>(int stage, const StageArgs &a, int nblocks, hipStream_t s)
-{
-    switch (stage) {
-    case 1: hipLaunchKernelGGL((k_stage<3, 2, 1>), dim3(nblocks), dim3(kStageThreads), 0, s, a); break;
-    case 2: hipLaunchKernelGGL((k_stage<3, 2, 2>), dim3(nblocks), dim3(kStageThreads), 0, s, a); break;
-    default: hipLaunchKernelGGL((k_stage<3, 2, 3>), dim3(nblocks), dim3(kStageThreads), 0, s, a); break;
-    }
-    return hipGetLastError();
-}
-
-template <>
-const void *mood_kernel_mg<3, 2>(int stage)
-{
-    if (stage == 1) return reinterpret_cast<const void *>(&k_mood<3, 2, 1>);
-    if (stage == 2) return reinterpret_cast<const void *>(&k_mood<3, 2, 2>);
-    return reinterpret_cast<const void *>(&k_mood<3, 2, 3>);
-}
+GPM_INSTANTIATE_COMBO(3, 2)
 
 }  // namespace gpm

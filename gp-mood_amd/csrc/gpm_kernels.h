@@ -98,12 +98,13 @@ struct DtArgs {
 };
 
 template <int MORD, int NGP>
-hipError_t launch_stage_mg(int stage, const StageArgs &a, int nblocks, hipStream_t s);
+hipError_t launch_stage_mg(int flux, int stage, const StageArgs &a, int nblocks, hipStream_t s);
 template <int MORD, int NGP>
-const void *mood_kernel_mg(int stage);
+const void *mood_kernel_mg(int flux, int stage);
 
-hipError_t launch_stage(int mord, int ngp, int stage, const StageArgs &a, int nblocks, hipStream_t s);
-const void *mood_kernel(int mord, int ngp, int stage);
+// flux: numFlux id (kHLLC, kHLL, kUnsplit)
+hipError_t launch_stage(int mord, int ngp, int flux, int stage, const StageArgs &a, int nblocks, hipStream_t s);
+const void *mood_kernel(int mord, int ngp, int flux, int stage);
 bool supported(int mord, int ngp);
 hipError_t launch_dt(const DtArgs &a, int nblocks, hipStream_t s);
 hipError_t launch_unpack_orders(const order_t *g, int8_t *out, size_t n, hipStream_t s);

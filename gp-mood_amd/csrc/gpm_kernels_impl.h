@@ -120,7 +120,7 @@ __device__ __forceinline__ void mark_neighbourhood(int i, int j, int N, int stam
     }
 }
 
-template <int MORD, int NGP, int STAGE>
+template <int MORD, int NGP, int STAGE, int FLUX>
 __global__ __launch_bounds__(kStageThreads, GPM_STAGE_WAVES_PER_EU) void k_stage(StageArgs a)
 {
     constexpr int H = kHalo<MORD>;
@@ -158,20 +158,20 @@ __global__ __launch_bounds__(kStageThreads, GPM_STAGE_WAVES_PER_EU) void k_stage
         if (f < NFX) {
             const int fy = f / (kTX + 1), fx = f % (kTX + 1);
             if (!a.apriori) {
-                face_flux<MORD, NGP, 0>(T, acc, fx - 1 + H, fy + H, fx + H, fy + H, L);
+                face_flux<MORD, NGP, 0, FLUX>(T, acc, fx - 1 + H, fy + H, fx + H, fy + H, L);
             } else {
                 const int o = min(sO[(fy + 1) * OW + fx], sO[(fy + 1) * OW + fx + 1]);
-                face_flux_dyn<MORD, NGP, 0>(T, acc, o, fx - 1 + H, fy + H, fx + H, fy + H, L);
+                face_flux_dyn<MORD, NGP, 0, FLUX>(T, acc, o, fx - 1 + H, fy + H, fx + H, fy + H, L);
             }
             sFx[f] = make_double4(L[0], L[1], L[2], L[3]);
         } else {
             const int g = f - NFX;
             const int gy = g / kTX, gx = g % kTX;
             if (!a.apriori) {
-                face_flux<MORD, NGP, 1>(T, acc, gx + H, gy - 1 + H, gx + H, gy + H, L);
+                face_flux<MORD, NGP, 1, FLUX>(T, acc, gx + H, gy - 1 + H, gx + H, gy + H, L);
             } else {
                 const int o = min(sO[gy * OW + gx + 1], sO[(gy + 1) * OW + gx + 1]);
-                face_flux_dyn<MORD, NGP, 1>(T, acc, o, gx + H, gy - 1 + H, gx + H, gy + H, L);
+                face_flux_dyn<MORD, NGP, 1, FLUX>(T, acc, o, gx + H, gy - 1 + H, gx + H, gy + H, L);
             }
             sFy[g] = make_double4(L[0], L[1], L[2], L[3]);
         }
@@ -271,7 +271,7 @@ __device__ __forceinline__ double4 rk_combine_u(const double4 &u, const double c
 // depend on the staged neighbourhood (the face's two order words, the cell's
 // own order word and step-start state, the next trip's list entry) are
 // issued together with the neighbourhood loads.
-template <int MORD, int NGP, int STAGE>
+template <int MORD, int NGP, int STAGE, int FLUX>
 __global__ __launch_bounds__(kMoodThreads) void k_mood(MoodArgs a)
 {
     constexpr int NGPP = NGP <= 1 ? 1 : NGP <= 2 ? 2 : NGP <= 4 ? 4 : 8;
@@ -349,7 +349,7 @@ __global__ __launch_bounds__(kMoodThreads) void k_mood(MoodArgs a)
             __syncthreads();
             double F[4] = {0.0, 0.0, 0.0, 0.0};
             if (valid && jg < NGP)
-                face_gp_flux_dyn<MORD>(T, lacc, ord, fa, fb, fdir, jg, HB + oai, HB + oaj, HB + obi, HB + obj, F);
+                face_gp_flux_dyn<MORD, FLUX>(T, lacc, ord, fa, fb, fdir, jg, HB + oai, HB + oaj, HB + obi, HB + obj, F);
             // quadrature over the face's Gauss points, j = 1..ngp from 0
             double L[4] = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
@@ -449,5 +449,38 @@ __global__ __launch_bounds__(kMoodThreads) void k_mood(MoodArgs a)
         st->second_pass_cells += second;
     }
 }
+
+// Launch entry points of one (Mord, ngp) combination, for every numFlux; each
+// combination is compiled in a translation unit of its own.
+#define GPM_INSTANTIATE_COMBO(M, G)                                                                          \
+    template <int F>                                                                                          \
+    static void launch_stage_f##M##G(int stage, const StageArgs &a, int nblocks, hipStream_t s)              \
+    {                                                                                                         \
+        if (stage == 1) hipLaunchKernelGGL((k_stage<M, G, 1, F>), dim3(nblocks), dim3(kStageThreads), 0, s, a); \
+        else if (stage == 2) hipLaunchKernelGGL((k_stage<M, G, 2, F>), dim3(nblocks), dim3(kStageThreads), 0, s, a); \
+        else hipLaunchKernelGGL((k_stage<M, G, 3, F>), dim3(nblocks), dim3(kStageThreads), 0, s, a);       \
+    }                                                                                                         \
+    template <int F>                                                                                          \
+    static const void *mood_kernel_f##M##G(int stage)                                                         \
+    {                                                                                                         \
+        if (stage == 1) return reinterpret_cast<const void *>(&k_mood<M, G, 1, F>);                          \
+        if (stage == 2) return reinterpret_cast<const void *>(&k_mood<M, G, 2, F>);                          \
+        return reinterpret_cast<const void *>(&k_mood<M, G, 3, F>);                                          \
+    }                                                                                                         \
+    template <>                                                                                               \
+    hipError_t launch_stage_mg<M, G>(int flux, int stage, const StageArgs &a, int nblocks, hipStream_t s)    \
+    {                                                                                                         \
+        if (flux == kHLL) launch_stage_f##M##G<kHLL>(stage, a, nblocks, s);                                  \
+        else if (flux == kUnsplit) launch_stage_f##M##G<kUnsplit>(stage, a, nblocks, s);                     \
+        else launch_stage_f##M##G<kHLLC>(stage, a, nblocks, s);                                              \
+        return hipGetLastError();                                                                             \
+    }                                                                                                         \
+    template <>                                                                                               \
+    const void *mood_kernel_mg<M, G>(int flux, int stage)                                                    \
+    {                                                                                                         \
+        if (flux == kHLL) return mood_kernel_f##M##G<kHLL>(stage);                                           \
+        if (flux == kUnsplit) return mood_kernel_f##M##G<kUnsplit>(stage);                                   \
+        return mood_kernel_f##M##G<kHLLC>(stage);                                                            \
+    }
 
 }  // namespace gpm

@@ -34,9 +34,12 @@ extern "C" {
 #define GPM_NN_GP_MOOD 203
 #define GPM_SSP_RK3 10
 #define GPM_HLLC 100
+#define GPM_HLL 102
+#define GPM_UNSPLIT 103
 #define GPM_NEUMANN 17
 #define GPM_PERIODIC 18
 #define GPM_REFLECTIVE 20
+#define GPM_KH_BC 403
 
 /* Order cascade applied to a cell that fails detection. */
 #define GPM_CASCADE_REFERENCE 0 /* 7->3, 5->3, 3->1 (mod_subroutine_MOOD.f90:159-163) */
@@ -57,8 +60,8 @@ typedef struct gpm_config {
     int32_t ngp;          /* Gauss points per face: 1..5 */
     int32_t method;       /* GPM_GP_MOOD or GPM_NN_GP_MOOD */
     int32_t integrator;   /* GPM_SSP_RK3 */
-    int32_t num_flux;     /* GPM_HLLC */
-    int32_t bc_type;      /* GPM_NEUMANN, GPM_PERIODIC or GPM_REFLECTIVE */
+    int32_t num_flux;     /* GPM_HLLC (mod_HLLC.f90), GPM_HLL (mod_HLL.f90) or GPM_UNSPLIT (mod_unsplit.f90) */
+    int32_t bc_type;      /* GPM_NEUMANN, GPM_PERIODIC, GPM_REFLECTIVE or GPM_KH_BC (BC.f90:18-135, 383-406) */
     int32_t dmp, u2, u2_tol; /* MOOD switches DMP, U2, U2_tol (logical: 0/1) */
     int32_t cascade;      /* GPM_CASCADE_* */
     double cfl;           /* CFL */
@@ -226,7 +229,7 @@ int gpm_get_apriori_orders(gpm_ctx *ctx, int8_t *pred_dev);
  * gpm_adjoint: ubar_dev (interior layout, nf*lf*4 doubles, device) holds dJ/dU
  * after nsteps steps on entry and dJ/dU before them on return, for the steps
  * whose stage inputs, masks and dts are given.  Deterministic (no atomics).
- * Supports (Mord, ngp) = (3,2), (5,3), (7,4).
+ * Supports (Mord, ngp) = (3,2), (5,3), (7,4) with numFlux = HLLC.
  *
  * gpm_hllc_vjp: reverse mode of the HLLC flux (mod_HLLC.f90:11-169) on n state
  * pairs in device memory ([n][4] each): ulbar = (dF/dul)^T fbar, urbar =

@@ -7,8 +7,9 @@
  *
  * It restates, operation for operation, the FP64 arithmetic of the reference
  * Fortran (slug-cfd/gp-mood, gp-mood-code/) for the configuration
- *   method = GP_MOOD, numFlux = HLLC, integrator = SSP_RK3,
- *   BC_type in {Neumann, Periodic, Reflective}
+ *   method = GP_MOOD, integrator = SSP_RK3,
+ *   numFlux in {HLLC, HLL, unsplit},
+ *   BC_type in {Neumann, Periodic, Reflective, KH_BC}
  * and keeps the reference's MOOD bookkeeping (CellGPO, DetCell, DetFace_x/y,
  * stored face fluxes L_Flux_x/y) literally, so that every decision is taken on
  * the same data as in the reference.  Build with -ffp-contract=off (no FMA) so
@@ -29,7 +30,9 @@
 #define NGC 4
 
 typedef struct {
-    int N, mord, ngp, bc;      /* bc: 0 Neumann, 1 Periodic, 2 Reflective */
+    int N, mord, ngp, bc;      /* bc: 0 Neumann, 1 Periodic, 2 Reflective, 3 KH_BC */
+    int bcx, bcy;              /* rule per axis: 0 Neumann, 1 Periodic, 2 Reflective */
+    int flux;                  /* numFlux: 0 HLLC, 1 HLL, 2 unsplit */
     int dmp, u2, u2tol;        /* MOOD switches (problems/<name>.f90: DMP, U2, U2_tol) */
     int cascade;               /* 0: reference (>3 -> 3, 3 -> 1); 1: full (7->5->3->1) */
     double cfl, dx, dy;
@@ -85,6 +88,10 @@ orc_t *orc_create(int N, int mord, int ngp, int bc, int dmp, int u2, int u2tol, 
 {
     orc_t *o = (orc_t *)calloc(1, sizeof(orc_t));
     o->N = N; o->mord = mord; o->ngp = ngp; o->bc = bc;
+    /* KH_BC (BC.f90:383-406): periodic in x, Neumann in y */
+    o->bcx = bc == 3 ? 1 : bc;
+    o->bcy = bc == 3 ? 0 : bc;
+    o->flux = 0;
     o->dmp = dmp; o->u2 = u2; o->u2tol = u2tol; o->cascade = cascade;
     o->cfl = cfl; o->dx = dx; o->dy = dy;
     o->sz = stcl_sz(mord);
@@ -119,8 +126,9 @@ void orc_destroy(orc_t *o)
     free(o->U1); free(o->U2); free(o->init_orders); free(o);
 }
 
-/* Boundary_C, BC.f90:18-59 (Neumann, Periodic) and :89-135 (Reflective):
-   x-ghosts for every row first, then y-ghosts for every column. */
+/* Boundary_C, BC.f90:18-59 (Neumann, Periodic), :89-135 (Reflective) and
+   :383-406 (KH_BC): x-ghosts for every row first, then y-ghosts for every
+   column. */
 void orc_bc(const orc_t *o, double *U)
 {
     const int N = o->N;
@@ -128,8 +136,8 @@ void orc_bc(const orc_t *o, double *U)
         for (int g = 1; g <= NGC; ++g) {
             int lr = N + g, ll = 1 - g, sr, sl;
             double sgn = 1.0;
-            if (o->bc == 0) { sr = N; sl = 1; }
-            else if (o->bc == 1) { sr = g; sl = N - g + 1; }
+            if (o->bcx == 0) { sr = N; sl = 1; }
+            else if (o->bcx == 1) { sr = g; sl = N - g + 1; }
             else { sr = N - g + 1; sl = g; sgn = -1.0; }
             for (int k = 0; k < 4; ++k) {
                 U[CELL(o, lr, n) * 4 + k] = U[CELL(o, sr, n) * 4 + k];
@@ -145,8 +153,8 @@ void orc_bc(const orc_t *o, double *U)
         for (int g = 1; g <= NGC; ++g) {
             int nt = N + g, nb = 1 - g, st, sb;
             double sgn = 1.0;
-            if (o->bc == 0) { st = N; sb = 1; }
-            else if (o->bc == 1) { st = g; sb = N - g + 1; }
+            if (o->bcy == 0) { st = N; sb = 1; }
+            else if (o->bcy == 1) { st = g; sb = N - g + 1; }
             else { st = N - g + 1; sb = g; sgn = -1.0; }
             for (int k = 0; k < 4; ++k) {
                 U[CELL(o, l, nt) * 4 + k] = U[CELL(o, l, st) * 4 + k];
@@ -246,6 +254,85 @@ static void hllc(const double *ul, const double *ur, int dir, double *F)
     } else {
         for (int k = 0; k < 4; ++k) F[k] = Fr[k];
     }
+}
+
+/* HLL_Flux, mod_HLL.f90:11-166, with c2p_local and Flux (physics.f90:9-75). */
+static void hll(const double *ul, const double *ur, int dir, double *F)
+{
+    int adm = 1;
+    double vxl = ul[1] / ul[0], vyl = ul[2] / ul[0];
+    double pl = pressure(ul);
+    double cl = sqrt(GAMMA * pl / ul[0]);
+    if (ul[0] <= 0. || pl <= 0. || isnan(ul[0]) || isnan(pl)) adm = 0;
+    double vxr = ur[1] / ur[0], vyr = ur[2] / ur[0];
+    double pr = pressure(ur);
+    double cr = sqrt(GAMMA * pr / ur[0]);
+    if (ur[0] <= 0. || pr <= 0. || isnan(ur[0]) || isnan(pr)) adm = 0;
+    if (!adm) {
+        /* mod_HLL.f90:162: F = 1/(vx-vx), vx of the right state's c2p */
+        double z = vxr - vxr;
+        for (int k = 0; k < 4; ++k) F[k] = 1.0 / z;
+        return;
+    }
+    const double vnl = dir == 0 ? vxl : vyl, vnr = dir == 0 ? vxr : vyr;
+    double Fl[4], Fr[4];
+    if (dir == 0) {
+        Fl[0] = ul[1]; Fl[1] = ul[0] * (vxl * vxl) + pl; Fl[2] = ul[2] * vxl; Fl[3] = vxl * (ul[3] + pl);
+        Fr[0] = ur[1]; Fr[1] = ur[0] * (vxr * vxr) + pr; Fr[2] = ur[2] * vxr; Fr[3] = vxr * (ur[3] + pr);
+    } else {
+        Fl[0] = ul[2]; Fl[1] = ul[1] * vyl; Fl[2] = ul[0] * (vyl * vyl) + pl; Fl[3] = vyl * (ul[3] + pl);
+        Fr[0] = ur[2]; Fr[1] = ur[1] * vyr; Fr[2] = ur[0] * (vyr * vyr) + pr; Fr[3] = vyr * (ur[3] + pr);
+    }
+    const double SL = fmin2(vnl - cl, vnr - cr);
+    const double SR = fmax2(vnl + cl, vnr + cr);
+    if (SL > 0.) {
+        for (int k = 0; k < 4; ++k) F[k] = Fl[k];
+    } else if (SR < 0.) {
+        for (int k = 0; k < 4; ++k) F[k] = Fr[k];
+    } else {
+        for (int k = 0; k < 4; ++k) F[k] = (SR * Fl[k] - SL * Fr[k] + SR * SL * (ur[k] - ul[k])) / (SR - SL);
+    }
+}
+
+/* unsplit_flux, mod_unsplit.f90:11-71, with c2p_local (physics.f90:51-75).
+   The reference passes one variable for both vx2 and vy2 of c2p_local; it
+   ends up holding the right state's vy**2, which the inadmissible branch uses. */
+static void unsplit(const double *ul, const double *ur, int dir, double *F)
+{
+    int adm = 1;
+    double vxl = ul[1] / ul[0], vyl = ul[2] / ul[0];
+    double pl = pressure(ul);
+    double cl = sqrt(GAMMA * pl / ul[0]);
+    if (ul[0] <= 0. || pl <= 0. || isnan(ul[0]) || isnan(pl)) adm = 0;
+    double vxr = ur[1] / ur[0], vyr = ur[2] / ur[0];
+    double no = vyr * vyr;
+    double pr = pressure(ur);
+    double cr = sqrt(GAMMA * pr / ur[0]);
+    if (ur[0] <= 0. || pr <= 0. || isnan(ur[0]) || isnan(pr)) adm = 0;
+    const double a = 1.1 * fmax2(ul[0] * cl, ur[0] * cr);
+    const double vnl = dir == 0 ? vxl : vyl, vnr = dir == 0 ? vxr : vyr;
+    const double vtl = dir == 0 ? vyl : vxl, vtr = dir == 0 ? vyr : vxr;
+    const double pistar = 0.5 * (pl + pr) - 0.5 * a * (vnr - vnl);
+    const double ustar = 0.5 * (vnl + vnr) - 0.5 * (pr - pl) / a;
+    const double trans = -(0.5 * a * (vtr - vtl));
+    if (!adm) {
+        double z = no - no;
+        for (int k = 0; k < 4; ++k) F[k] = 1.0 / z;
+        return;
+    }
+    const double *up = ustar > 0 ? ul : ur;
+    for (int k = 0; k < 4; ++k) F[k] = up[k] * ustar;
+    const int md = dir == 0 ? 1 : 2, om = dir == 0 ? 2 : 1;
+    F[md] = F[md] + pistar;
+    F[om] = F[om] + trans;
+    F[3] = F[3] + pistar * ustar;
+}
+
+static void riemann(int flux, const double *ul, const double *ur, int dir, double *F)
+{
+    if (flux == 1) hll(ul, ur, dir, F);
+    else if (flux == 2) unsplit(ul, ur, dir, F);
+    else hllc(ul, ur, dir, F);
 }
 
 /* GP_MOOD_, mod_GP_MOOD.f90:11-76: face states at Gauss points for cells
@@ -412,7 +499,7 @@ static void face_flux(const orc_t *o, int dir, int l, int n, double *Lf)
             if (dir == 0) { ul[k] = o->Uh[UHI(o, k, iR, j, l, n)]; ur[k] = o->Uh[UHI(o, k, iL, j, l + 1, n)]; }
             else { ul[k] = o->Uh[UHI(o, k, iT, j, l, n)]; ur[k] = o->Uh[UHI(o, k, iB, j, l, n + 1)]; }
         }
-        hllc(ul, ur, dir, Fg[j]);
+        riemann(o->flux, ul, ur, dir, Fg[j]);
     }
     for (int k = 0; k < 4; ++k) {
         double acc = 0.0;
@@ -553,8 +640,13 @@ int orc_step_stages(orc_t *o, double *U, double dt, double *stage_out /* 3*(N+8)
 long orc_count_detected(const orc_t *o) { return o->count_detected; }
 void orc_passes(const orc_t *o, int *p) { for (int k = 0; k < 3; ++k) p[k] = o->passes[k]; }
 
-/* The Riemann solver alone, for unit checks (dir 0 = x, 1 = y). */
+/* The Riemann solvers alone, for unit checks (dir 0 = x, 1 = y; flux 0 HLLC,
+   1 HLL, 2 unsplit). */
 void orc_hllc(const double *ul, const double *ur, int dir, double *F) { hllc(ul, ur, dir, F); }
+void orc_riemann(int flux, const double *ul, const double *ur, int dir, double *F) { riemann(flux, ul, ur, dir, F); }
+
+/* numFlux of the step (0 HLLC, 1 HLL, 2 unsplit). */
+void orc_set_flux(orc_t *o, int flux) { o->flux = flux; }
 
 /* Fixed a priori orders for every stage (NULL restores CellGPO = Mord). */
 void orc_set_initial_orders(orc_t *o, const int8_t *orders)

@@ -10,7 +10,8 @@ import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "liboracle.so")
-BC_CODES = {"neumann": 0, "periodic": 1, "reflective": 2}
+BC_CODES = {"neumann": 0, "periodic": 1, "reflective": 2, "kh": 3}
+FLUX_CODES = {"hllc": 0, "hll": 1, "unsplit": 2}
 CASCADES = {"reference": 0, "full": 1}
 
 _lib = None
@@ -26,6 +27,8 @@ def lib():
         L.orc_create.restype = P
         L.orc_create.argtypes = [ctypes.c_int] * 8 + [ctypes.c_double] * 3 + [P] * 5
         L.orc_destroy.argtypes = [P]
+        L.orc_set_flux.argtypes = [P, ctypes.c_int]
+        L.orc_riemann.argtypes = [ctypes.c_int, P, P, ctypes.c_int, P]
         L.orc_bc.argtypes = [P, P]
         L.orc_setdt.restype = ctypes.c_double
         L.orc_setdt.argtypes = [P, P, ctypes.c_double]
@@ -49,7 +52,7 @@ class Oracle:
     gw, ixiy in the reference's Fortran layouts (flattened column-major)."""
 
     def __init__(self, n, mord, ngp, bc, tables, dmp=True, u2=True, u2tol=True,
-                 cascade="reference", cfl=0.8):
+                 cascade="reference", cfl=0.8, flux="hllc"):
         self.n = n
         self.tables = {k: np.ascontiguousarray(v) for k, v in tables.items()}
         t = self.tables
@@ -60,6 +63,7 @@ class Oracle:
                                    _ptr(t["d2y"].astype(np.float64)), _ptr(t["gw"].astype(np.float64)),
                                    _ptr(t["ixiy"].astype(np.int32)))
         self._keep = [t[k] for k in t]
+        lib().orc_set_flux(self._h, FLUX_CODES[flux])
 
     def __del__(self):
         if getattr(self, "_h", None):
@@ -101,6 +105,15 @@ class Oracle:
         p = np.zeros(3, dtype=np.int32)
         lib().orc_passes(self._h, _ptr(p))
         return outs, masks, p
+
+
+def riemann(flux, ul, ur, direction):
+    """Riemann solver of the oracle (HLLC, HLL or unsplit) for one face."""
+    ul = np.ascontiguousarray(ul, dtype=np.float64)
+    ur = np.ascontiguousarray(ur, dtype=np.float64)
+    F = np.zeros(4)
+    lib().orc_riemann(FLUX_CODES[flux], _ptr(ul), _ptr(ur), int(direction), _ptr(F))
+    return F
 
 
 def hllc(ul, ur, direction):

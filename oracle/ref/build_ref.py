@@ -58,9 +58,10 @@ MODULES = [
     "mod_SSP_RK4", "time_stepper",
 ]
 
-BC_NAMES = {"neumann": "Neumann", "periodic": "Periodic", "reflective": "Reflective"}
+BC_NAMES = {"neumann": "Neumann", "periodic": "Periodic", "reflective": "Reflective", "kh": "KH_BC"}
+FLUX_NAMES = {"hllc": "HLLC", "hll": "HLL", "unsplit": "unsplit"}
 
-# name: (N, Mord, ngp, bc, DMP, U2, U2_tol)
+# name: (N, Mord, ngp, bc, DMP, U2, U2_tol[, numFlux]); numFlux defaults to HLLC
 CONFIGS = {
     "r1_n32_neu": (32, 3, 2, "neumann", True, True, True),
     "r2_n32_neu": (32, 5, 3, "neumann", True, True, True),
@@ -73,11 +74,26 @@ CONFIGS = {
     "r3_n64_per": (64, 7, 4, "periodic", True, True, True),
     "r2_n256_neu": (256, 5, 3, "neumann", True, True, True),
     "r3_n2048_ref": (2048, 7, 4, "reflective", True, True, True),
+    # the other Riemann solvers and the Kelvin-Helmholtz boundary of the
+    # reference's problem files (problems/2DRP3_GPMOOD.f90, KH_GPMOOD.f90,
+    # shu_osher_GPMOOD.f90, Mach800_GPMOOD.f90)
+    "r2_n64_neu_uns": (64, 5, 3, "neumann", True, True, True, "unsplit"),
+    "r2_n64_neu_hll": (64, 5, 3, "neumann", True, True, True, "hll"),
+    "r1_n32_ref_uns": (32, 3, 2, "reflective", True, True, True, "unsplit"),
+    "r3_n64_per_hll": (64, 7, 4, "periodic", True, True, True, "hll"),
+    "r3_n64_kh_uns": (64, 7, 4, "kh", True, True, True, "unsplit"),
+    "r2_n64_kh_hllc": (64, 5, 3, "kh", True, True, True),
 }
+
+
+def config(name):
+    """(N, Mord, ngp, bc, DMP, U2, U2_tol, flux) of a configuration."""
+    c = CONFIGS[name]
+    return tuple(c[:7]) + ((c[7] if len(c) > 7 else "hllc"),)
 
 # configs for which the Fortran drop-in (reference modules + product
 # time_stepper over libgpmood) is also built
-DROPIN_CONFIGS = ["r2_n64_neu", "r3_n64_ref"]
+DROPIN_CONFIGS = ["r2_n64_neu", "r3_n64_ref", "r3_n64_kh_uns"]
 
 PARAMS_TEMPLATE = """module parameters
    use constants
@@ -98,7 +114,7 @@ PARAMS_TEMPLATE = """module parameters
    integer, parameter :: L=32
    integer, parameter :: length=60
    character(100) :: NN_filename=''
-   integer, parameter :: numFlux = HLLC
+   integer, parameter :: numFlux = {flux}
    integer, parameter :: IO_freqStep = -100
    real(PR), parameter:: IO_freqTime = -1.e-3
    integer,  parameter :: lf = {n}
@@ -122,13 +138,13 @@ def build(name, dropin=False):
     reference's time_stepper.f90 is replaced by the product's Fortran drop-in
     (gp-mood_amd/fortran/time_stepper.f90 over gpm_capi.f90), which runs the
     step through libgpmood; everything else is the reference's own code."""
-    n, mord, ngp, bc, dmp, u2, u2tol = CONFIGS[name]
+    n, mord, ngp, bc, dmp, u2, u2tol, flux = config(name)
     out = os.path.join(OUT_ROOT, name + ("_dropin" if dropin else ""))
     os.makedirs(out, exist_ok=True)
     params = os.path.join(out, "parameters.f90")
     with open(params, "w") as f:
         f.write(PARAMS_TEMPLATE.format(mord=mord, ngp=ngp, dmp=fbool(dmp), u2=fbool(u2),
-                                       u2tol=fbool(u2tol), n=n, bc=BC_NAMES[bc]))
+                                       u2tol=fbool(u2tol), n=n, bc=BC_NAMES[bc], flux=FLUX_NAMES[flux]))
     flags = ["-fdefault-real-8", "-O3", "-ffp-contract=off", "-J", out, "-I", out]
     if n >= 1024:
         flags.append("-mcmodel=medium")

@@ -82,6 +82,13 @@ STEP_CASES = {
     # cells next to the seam, where ghost cells keep order Mord
     "r2_n64_per_fq3": ("r2_n64_per", "four_quadrant", 3, 10),
     "r3_n64_per_fq2": ("r3_n64_per", "four_quadrant", 2, 10),
+    # the other Riemann solvers and the Kelvin-Helmholtz boundary
+    "r2_n64_neu_uns_fq11": ("r2_n64_neu_uns", "four_quadrant", 11, 20),
+    "r2_n64_neu_hll_fq12": ("r2_n64_neu_hll", "four_quadrant", 12, 20),
+    "r1_n32_ref_uns_bl13": ("r1_n32_ref_uns", "blast", 13, 20),
+    "r3_n64_per_hll_fq14": ("r3_n64_per_hll", "four_quadrant", 14, 15),
+    "r3_n64_kh_uns_fq15": ("r3_n64_kh_uns", "four_quadrant", 15, 15),
+    "r2_n64_kh_hllc_fq16": ("r2_n64_kh_hllc", "four_quadrant", 16, 15),
 }
 # long runs stored as checksums: name: (config, IC kind, seed, steps)
 HASH_CASES = {
@@ -98,9 +105,9 @@ def main(argv):
         U0 = ic.make(kind, n, seed)
         r = run_ref(cfg, U0, nsteps, 0)
         r1 = run_ref(cfg, U0, 1, 1)  # per-stage data of the first step
-        _, mord, ngp, bc, dmp, u2, u2tol = build_ref.CONFIGS[cfg]
+        _, mord, ngp, bc, dmp, u2, u2tol, flux = build_ref.config(cfg)
         np.savez_compressed(
-            os.path.join(OUT, f"step_{name}.npz"), n=n, mord=mord, ngp=ngp, bc=bc, dmp=dmp, u2=u2, u2tol=u2tol,
+            os.path.join(OUT, f"step_{name}.npz"), n=n, mord=mord, ngp=ngp, bc=bc, dmp=dmp, u2=u2, u2tol=u2tol, flux=flux,
             kind=kind, seed=seed, nsteps=nsteps, ic_sha=sha(U0), U0=U0,
             dt=r["steps"]["dt"], t=r["steps"]["t"], counts=r["steps"]["cnt"], masks=r["masks"], U=r["U"],
             stage_U=r1["U_stage"], stage_masks=r1["masks"], **{k: v for k, v in r["tables"].items()})
@@ -111,7 +118,7 @@ def main(argv):
         n = build_ref.CONFIGS[cfg][0]
         U0 = ic.make(kind, n, seed)
         r = run_ref(cfg, U0, nsteps, 0)
-        _, mord, ngp, bc, dmp, u2, u2tol = build_ref.CONFIGS[cfg]
+        _, mord, ngp, bc, dmp, u2, u2tol, _ = build_ref.config(cfg)
         Ui = r["U"][4:-4, 4:-4]
         np.savez_compressed(
             os.path.join(OUT, f"hash_{name}.npz"), n=n, mord=mord, ngp=ngp, bc=bc, dmp=dmp, u2=u2, u2tol=u2tol,

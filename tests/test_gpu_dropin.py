@@ -15,7 +15,8 @@ from conftest import ROOT, golden, has_gpu
 
 pytestmark = pytest.mark.gpu
 
-CASES = [("r2_n64_neu", "step_r2_n64_neu_fq8.npz"), ("r3_n64_ref", "step_r3_n64_ref_bl6.npz")]
+CASES = [("r2_n64_neu", "step_r2_n64_neu_fq8.npz"), ("r3_n64_ref", "step_r3_n64_ref_bl6.npz"),
+         ("r3_n64_kh_uns", "step_r3_n64_kh_uns_fq15.npz")]
 STEP_TYPE = np.dtype([("dt", "<f8"), ("t", "<f8"), ("cnt", "<i4")])
 
 

@@ -20,7 +20,8 @@ BC_NAME = {"neumann": "neumann", "periodic": "periodic", "reflective": "reflecti
 
 def make_solver(gpm, z, tables=True):
     cfg = gpm.Config(lf=int(z["n"]), Mord=int(z["mord"]), ngp=int(z["ngp"]), BC_type=str(z["bc"]),
-                     DMP=bool(z["dmp"]), U2=bool(z["u2"]), U2_tol=bool(z["u2tol"]))
+                     DMP=bool(z["dmp"]), U2=bool(z["u2"]), U2_tol=bool(z["u2tol"]),
+                     numFlux=str(z["flux"]) if "flux" in z else "hllc")
     return gpm.Solver(cfg, tables_from_npz(z) if tables else None)
 
 

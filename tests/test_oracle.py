@@ -19,7 +19,8 @@ STEP_CASES = sorted(os.path.basename(p)[5:-4] for p in glob.glob(os.path.join(GO
 
 def make_oracle(O, z):
     return O.Oracle(int(z["n"]), int(z["mord"]), int(z["ngp"]), str(z["bc"]), tables_from_npz(z),
-                    dmp=bool(z["dmp"]), u2=bool(z["u2"]), u2tol=bool(z["u2tol"]))
+                    dmp=bool(z["dmp"]), u2=bool(z["u2"]), u2tol=bool(z["u2tol"]),
+                    flux=str(z["flux"]) if "flux" in z else "hllc")
 
 
 @pytest.mark.parametrize("case", STEP_CASES)
