@@ -111,3 +111,76 @@ GENERATORS = {"four_quadrant": four_quadrant, "blast": blast, "shear_layer": she
 
 def make(kind, n, seed):
     return GENERATORS[kind](n, seed)
+
+
+# --- problem set-ups of the formula boundary conditions, on lf x nf grids ---
+# Each returns the interior (nf, lf, 4) on [0, lx] x [0, ly]; ``seed`` adds a
+# +-``noise`` relative density perturbation per cell (0 disables it).
+
+def centres(m, length):
+    d = length / m
+    return np.arange(1, m + 1, dtype=np.float64) * d - d / 2
+
+
+def _grid(lf, nf, lx, ly):
+    return np.meshgrid(centres(lf, lx), centres(nf, ly), indexing="xy")
+
+
+def _noisy(rho, seed, noise):
+    if not noise:
+        return rho
+    return rho * (1.0 + noise * np.random.default_rng(seed).uniform(-1.0, 1.0, size=rho.shape))
+
+
+def dmr(lf, nf, lx=4.0, ly=1.0, seed=0, noise=0.0):
+    """Double Mach reflection: the Mach-10 shock through (1/6, 0) at 60 degrees,
+    post-shock state (8, 7.1447096, -4.125, 116.5) above the line
+    y = (x - 1/6) sqrt(3), (1.4, 0, 0, 1) below (the states of IC.f90:336-355)."""
+    X, Y = _grid(lf, nf, lx, ly)
+    post = Y > (X - 1.0 / 6) * np.sqrt(3.0)
+    rho = _noisy(np.where(post, 8.0, 1.4), seed, noise)
+    return prim_to_cons(rho, np.where(post, 7.1447096, 0.0), np.where(post, -4.125, 0.0), np.where(post, 116.5, 1.0))
+
+
+def mach800(lf, nf, lx=1.5, ly=1.5, seed=0, noise=0.0):
+    """Mach-800 jet: ambient (14, 0, 0, 1), and the jet state (1.4, 0, 800, 1)
+    in the first row under the inflow window |x - lx/2| <= 0.05."""
+    X, Y = _grid(lf, nf, lx, ly)
+    dy = ly / nf
+    jet = (np.abs(X - 0.5 * lx) <= 0.05) & (Y < dy)
+    rho = np.where(jet, 1.4, _noisy(np.full(X.shape, 14.0), seed, noise))
+    return prim_to_cons(rho, 0.0, np.where(jet, 800.0, 0.0), 1.0)
+
+
+def rayleigh_taylor(lf, nf, lx=0.5, ly=1.5, seed=0, noise=0.0, g=0.1):
+    """Rayleigh-Taylor: density 1 below y = ly/2 and 2 above, p = 2.5 + g y,
+    v = 0.01 (1 + cos(2 pi (x - lx/2)/lx)) (1 + cos(2 pi (y - ly/2)/ly)) / 4."""
+    X, Y = _grid(lf, nf, lx, ly)
+    rho = _noisy(np.where(Y < ly / 2, 1.0, 2.0), seed, noise)
+    v = 0.01 * (1 + np.cos(2 * np.pi * (X - lx / 2) / lx)) * (1 + np.cos(2 * np.pi * (Y - ly / 2) / ly)) / 4
+    return prim_to_cons(rho, 0.0, v, 2.5 + g * Y)
+
+
+def shock_tube_x(lf, nf, lx=1.0, ly=1.0, seed=0, noise=0.0, left=(3.857143, 2.629369, 0.0, 10.33333),
+                 right=(1.0, 0.0, 0.0, 1.0), x0=0.25):
+    """A planar shock in x: primitive state ``left`` for x < x0 * lx, ``right``
+    beyond it, with a seeded density perturbation."""
+    X, _ = _grid(lf, nf, lx, ly)
+    L = X < x0 * lx
+    prim = [np.where(L, a, b) for a, b in zip(left, right)]
+    return prim_to_cons(_noisy(prim[0], seed, noise), prim[1], prim[2], prim[3])
+
+
+def quadrants(lf, nf, lx=1.0, ly=1.0, seed=0, noise=0.0):
+    """Four seeded constant states (four_quadrant_states) split at the domain centre."""
+    X, Y = _grid(lf, nf, lx, ly)
+    q = np.where(Y < 0.5 * ly, np.where(X < 0.5 * lx, 0, 1), np.where(X < 0.5 * lx, 2, 3))
+    prim = np.array(four_quadrant_states(seed), dtype=np.float64)[q]
+    return prim_to_cons(_noisy(prim[..., 0], seed, noise), prim[..., 1], prim[..., 2], prim[..., 3])
+
+
+PROBLEMS = {"quadrants": quadrants, "dmr": dmr, "mach800": mach800, "rayleigh_taylor": rayleigh_taylor, "shock_tube_x": shock_tube_x}
+
+
+def problem(kind, lf, nf, lx, ly, seed=0, noise=0.0):
+    return PROBLEMS[kind](lf, nf, lx, ly, seed=seed, noise=noise)

@@ -6,18 +6,25 @@
  * product path.  The product is the HIP library in gp-mood_amd/csrc.
  *
  * It restates, operation for operation, the FP64 arithmetic of the reference
- * Fortran (slug-cfd/gp-mood, gp-mood-code/) for the configuration
- *   method = GP_MOOD, integrator = SSP_RK3,
- *   numFlux in {HLLC, HLL, unsplit},
- *   BC_type in {Neumann, Periodic, Reflective, KH_BC}
- * and keeps the reference's MOOD bookkeeping (CellGPO, DetCell, DetFace_x/y,
- * stored face fluxes L_Flux_x/y) literally, so that every decision is taken on
- * the same data as in the reference.  Build with -ffp-contract=off (no FMA) so
- * that every product and sum is rounded as in the reference build.
+ * Fortran (slug-cfd/gp-mood, gp-mood-code/) for
+ *   method = GP_MOOD,
+ *   integrator in {FE, SSP_RK2, SSP_RK3, SSP_RK4},
+ *   numFlux in {HLLC, LLF, HLL, unsplit},
+ *   BC_type in {Neumann, Periodic, Dirichlet, Reflective, KH_BC, Mach800_BC,
+ *               DoubleMach800_BC, RMI_BC, DMR, RT_BC},
+ *   the gravity source of problem = RT,
+ * on an lf x nf grid, and keeps the reference's MOOD bookkeeping (CellGPO,
+ * DetCell, DetFace_x/y, stored face fluxes L_Flux_x/y) literally, so that every
+ * decision is taken on the same data as in the reference.  Whole-array
+ * statements (RK combinations, the gravity source, Boundary_C) act on the ghost
+ * cells too, as in the reference, so Setdt sees the same ghost values.  Build
+ * with -ffp-contract=off (no FMA) so that every product and sum is rounded as in
+ * the reference build.
  *
- * Layout: a state array holds (N+8)^2 cells of 4 doubles, cell (l,n) with
- * l,n in [-3, N+4] at index ((n+3)*(N+8) + (l+3))*4 + k, k = rho, mx, my, E:
- * byte-identical to the reference's U(4, lb:le, nb:ne) (global_variables.f90:21-26).
+ * Layout: a state array holds (lf+8)*(nf+8) cells of 4 doubles, cell (l,n) with
+ * l in [-3, lf+4], n in [-3, nf+4] at index ((n+3)*(lf+8) + (l+3))*4 + k,
+ * k = rho, mx, my, E: byte-identical to the reference's U(4, lb:le, nb:ne)
+ * (global_variables.f90:21-26).
  *
  * Pinned against: the reference binary built by oracle/ref/build_ref.py (golden
  * fixtures under tests/golden/), bit for bit.  See the test suite for the list.
@@ -29,13 +36,24 @@
 
 #define NGC 4
 
+/* Ids of the reference's constants.f90:17-34. */
+enum { FE = 8, SSP_RK2 = 9, SSP_RK3 = 10, SSP_RK4 = 12 };
+enum { HLLC = 100, LLF = 101, HLL = 102, UNSPLIT = 103 };
+enum {
+    NEUMANN = 17, PERIODIC = 18, DIRICHLET = 19, REFLECTIVE = 20, KH_BC = 403,
+    MACH800_BC = 201, DOUBLEMACH800_BC = 301, RMI_BC = 401, DMR_BC = 35, RT_BC = 402
+};
+enum { PROBLEM_RT = 51 };
+
 typedef struct {
-    int N, mord, ngp, bc;      /* bc: 0 Neumann, 1 Periodic, 2 Reflective, 3 KH_BC */
-    int bcx, bcy;              /* rule per axis: 0 Neumann, 1 Periodic, 2 Reflective */
-    int flux;                  /* numFlux: 0 HLLC, 1 HLL, 2 unsplit */
+    int lf, nf, mord, ngp, bc; /* bc: the reference's BC_type id */
+    int flux;                  /* numFlux id */
+    int integrator;            /* integrator id */
+    int gravity;               /* problem == RT: source term after MOOD (mod_FE.f90:183-186) */
     int dmp, u2, u2tol;        /* MOOD switches (problems/<name>.f90: DMP, U2, U2_tol) */
     int cascade;               /* 0: reference (>3 -> 3, 3 -> 1); 1: full (7->5->3->1) */
     double cfl, dx, dy;
+    double t_rk;               /* global_variables.f90:33, time seen by the DMR boundary */
     /* Tables in the reference's Fortran layouts (global_variables.f90:37-38,53,61):
        zT_sp(Mord, sz_sphere, 4, ngp), GP_d2x_sp(Mord, sz_sphere),
        GP_d2y_sp(Mord, sz_sphere), gauss_weight(5,5), ixiy_sp(7,25,2). */
@@ -43,20 +61,22 @@ typedef struct {
     int ixiy[7 * 25 * 2];
     int sz;                    /* sz_sphere = stencil size of Mord */
     /* work arrays */
-    int W;                     /* N + 8 */
-    double *Uh;                /* Uh(4, 4, ngp, 0:N+1, 0:N+1) */
-    double *Fx, *Fy;           /* L_Flux_x(4,-1:N+1,0:N+1), L_Flux_y(4,0:N+1,-1:N+1) */
+    int W, H;                  /* lf + 8, nf + 8 */
+    double *Uh;                /* Uh(4, 4, ngp, 0:lf+1, 0:nf+1) */
+    double *Fx, *Fy;           /* L_Flux_x(4,-1:lf+1,0:nf+1), L_Flux_y(4,0:lf+1,-1:nf+1) */
     int8_t *gpo;               /* CellGPO(lb:le, nb:ne) */
     uint8_t *detcell;          /* DetCell(lb:le, nb:ne) */
-    uint8_t *detfx, *detfy;    /* DetFace_x(-1:N+1, 0:N+1), DetFace_y(0:N+1, -1:N+1) */
-    uint8_t *decrease;         /* decrease(1:N, 1:N) */
-    double *U1, *U2;           /* RK stage states */
+    uint8_t *detfx, *detfy;    /* DetFace_x(-1:lf+1, 0:nf+1), DetFace_y(0:lf+1, -1:nf+1) */
+    uint8_t *decrease;         /* decrease(1:lf, 1:nf) */
+    double *S[7];              /* RK stage states (U1, U2, U3, U4, Fl, FU3, FU4) */
     long count_detected;       /* count_detected_cell (global_variables.f90:63) */
-    int8_t *init_orders;       /* a priori CellGPO of the interior [N][N] (NULL: Mord) */
-    int passes[3];             /* MOOD passes per stage of the last step */
+    int8_t *init_orders;       /* a priori CellGPO of the interior [nf][lf] (NULL: Mord) */
+    int passes[5];             /* MOOD passes per stage of the last step */
 } orc_t;
 
 static const double GAMMA = 1.4;
+/* g of global_variables.f90:129 is real(4); it enters FP64 expressions widened. */
+static const double GRAV = (double)0.1f;
 
 /* Fortran MIN/MAX as compiled by the reference build: the second operand is
    returned unless the first compares strictly smaller (larger). */
@@ -74,24 +94,22 @@ static inline int stcl_sz(int ord) /* GP_init.f90:14-23 */
 #define D2Y(o, ord, s) ((o)->d2y[((ord) - 1) + (o)->mord * (s)])
 #define IXIY(o, ord, s, d) ((o)->ixiy[((ord) - 1) + 7 * ((s) + 25 * (d))])
 #define GW(o, ngp, j) ((o)->gw[((ngp) - 1) + 5 * (j)])
-#define FXI(o, l, n) ((size_t)(((n) * ((o)->N + 3) + ((l) + 1))))
-#define FYI(o, l, n) ((size_t)((((n) + 1) * ((o)->N + 2) + (l))))
+#define FXI(o, l, n) ((size_t)(((n) * ((o)->lf + 3) + ((l) + 1))))
+#define FYI(o, l, n) ((size_t)((((n) + 1) * ((o)->lf + 2) + (l))))
 #define DFX(o, l, n) ((o)->detfx[FXI(o, l, n)])
 #define DFY(o, l, n) ((o)->detfy[FYI(o, l, n)])
-#define UHI(o, k, i, j, l, n) ((size_t)(((((size_t)(n) * ((o)->N + 2) + (l)) * 4 + (i)) * (o)->ngp + (j)) * 4 + (k)))
+#define UHI(o, k, i, j, l, n) ((size_t)(((((size_t)(n) * ((o)->lf + 2) + (l)) * 4 + (i)) * (o)->ngp + (j)) * 4 + (k)))
 
 enum { iL = 0, iT = 1, iR = 2, iB = 3 };
 
-orc_t *orc_create(int N, int mord, int ngp, int bc, int dmp, int u2, int u2tol, int cascade,
+orc_t *orc_create(int lf, int nf, int mord, int ngp, int bc, int dmp, int u2, int u2tol, int cascade,
                   double cfl, double dx, double dy, const double *zT, const double *d2x,
                   const double *d2y, const double *gw, const int *ixiy)
 {
     orc_t *o = (orc_t *)calloc(1, sizeof(orc_t));
-    o->N = N; o->mord = mord; o->ngp = ngp; o->bc = bc;
-    /* KH_BC (BC.f90:383-406): periodic in x, Neumann in y */
-    o->bcx = bc == 3 ? 1 : bc;
-    o->bcy = bc == 3 ? 0 : bc;
-    o->flux = 0;
+    o->lf = lf; o->nf = nf; o->mord = mord; o->ngp = ngp; o->bc = bc;
+    o->flux = HLLC;
+    o->integrator = SSP_RK3;
     o->dmp = dmp; o->u2 = u2; o->u2tol = u2tol; o->cascade = cascade;
     o->cfl = cfl; o->dx = dx; o->dy = dy;
     o->sz = stcl_sz(mord);
@@ -103,18 +121,18 @@ orc_t *orc_create(int N, int mord, int ngp, int bc, int dmp, int u2, int u2tol, 
     memcpy(o->d2y, d2y, sizeof(double) * mord * o->sz);
     memcpy(o->gw, gw, sizeof(double) * 25);
     memcpy(o->ixiy, ixiy, sizeof(int) * 7 * 25 * 2);
-    o->W = N + 2 * NGC;
-    size_t ncell = (size_t)o->W * o->W;
-    o->Uh = (double *)calloc((size_t)(N + 2) * (N + 2) * 16 * ngp, sizeof(double));
-    o->Fx = (double *)calloc((size_t)(N + 3) * (N + 2) * 4, sizeof(double));
-    o->Fy = (double *)calloc((size_t)(N + 2) * (N + 3) * 4, sizeof(double));
+    o->W = lf + 2 * NGC;
+    o->H = nf + 2 * NGC;
+    size_t ncell = (size_t)o->W * o->H;
+    o->Uh = (double *)calloc((size_t)(lf + 2) * (nf + 2) * 16 * ngp, sizeof(double));
+    o->Fx = (double *)calloc((size_t)(lf + 3) * (nf + 2) * 4, sizeof(double));
+    o->Fy = (double *)calloc((size_t)(lf + 2) * (nf + 3) * 4, sizeof(double));
     o->gpo = (int8_t *)malloc(ncell);
     o->detcell = (uint8_t *)malloc(ncell);
-    o->detfx = (uint8_t *)malloc((size_t)(N + 3) * (N + 2));
-    o->detfy = (uint8_t *)malloc((size_t)(N + 2) * (N + 3));
-    o->decrease = (uint8_t *)malloc((size_t)N * N);
-    o->U1 = (double *)calloc(ncell * 4, sizeof(double));
-    o->U2 = (double *)calloc(ncell * 4, sizeof(double));
+    o->detfx = (uint8_t *)malloc((size_t)(lf + 3) * (nf + 2));
+    o->detfy = (uint8_t *)malloc((size_t)(lf + 2) * (nf + 3));
+    o->decrease = (uint8_t *)malloc((size_t)lf * nf);
+    for (int k = 0; k < 7; ++k) o->S[k] = (double *)calloc(ncell * 4, sizeof(double));
     return o;
 }
 
@@ -123,49 +141,8 @@ void orc_destroy(orc_t *o)
     if (!o) return;
     free(o->zT); free(o->d2x); free(o->d2y); free(o->Uh); free(o->Fx); free(o->Fy);
     free(o->gpo); free(o->detcell); free(o->detfx); free(o->detfy); free(o->decrease);
-    free(o->U1); free(o->U2); free(o->init_orders); free(o);
-}
-
-/* Boundary_C, BC.f90:18-59 (Neumann, Periodic), :89-135 (Reflective) and
-   :383-406 (KH_BC): x-ghosts for every row first, then y-ghosts for every
-   column. */
-void orc_bc(const orc_t *o, double *U)
-{
-    const int N = o->N;
-    for (int n = 1 - NGC; n <= N + NGC; ++n) {
-        for (int g = 1; g <= NGC; ++g) {
-            int lr = N + g, ll = 1 - g, sr, sl;
-            double sgn = 1.0;
-            if (o->bcx == 0) { sr = N; sl = 1; }
-            else if (o->bcx == 1) { sr = g; sl = N - g + 1; }
-            else { sr = N - g + 1; sl = g; sgn = -1.0; }
-            for (int k = 0; k < 4; ++k) {
-                U[CELL(o, lr, n) * 4 + k] = U[CELL(o, sr, n) * 4 + k];
-                U[CELL(o, ll, n) * 4 + k] = U[CELL(o, sl, n) * 4 + k];
-            }
-            if (sgn < 0) {
-                U[CELL(o, lr, n) * 4 + 1] = -U[CELL(o, lr, n) * 4 + 1];
-                U[CELL(o, ll, n) * 4 + 1] = -U[CELL(o, ll, n) * 4 + 1];
-            }
-        }
-    }
-    for (int l = 1 - NGC; l <= N + NGC; ++l) {
-        for (int g = 1; g <= NGC; ++g) {
-            int nt = N + g, nb = 1 - g, st, sb;
-            double sgn = 1.0;
-            if (o->bcy == 0) { st = N; sb = 1; }
-            else if (o->bcy == 1) { st = g; sb = N - g + 1; }
-            else { st = N - g + 1; sb = g; sgn = -1.0; }
-            for (int k = 0; k < 4; ++k) {
-                U[CELL(o, l, nt) * 4 + k] = U[CELL(o, l, st) * 4 + k];
-                U[CELL(o, l, nb) * 4 + k] = U[CELL(o, l, sb) * 4 + k];
-            }
-            if (sgn < 0) {
-                U[CELL(o, l, nt) * 4 + 2] = -U[CELL(o, l, nt) * 4 + 2];
-                U[CELL(o, l, nb) * 4 + 2] = -U[CELL(o, l, nb) * 4 + 2];
-            }
-        }
-    }
+    for (int k = 0; k < 7; ++k) free(o->S[k]);
+    free(o->init_orders); free(o);
 }
 
 /* pressure, physics.f90:132-143 */
@@ -174,16 +151,135 @@ static inline double pressure(const double *u)
     return (GAMMA - 1.0) * (u[3] - (0.5 * (u[1] * u[1] + u[2] * u[2])) / u[0]);
 }
 
+/* primitive_to_conservative, physics.f90:146-160 */
+static void p2c(double r, double vx, double vy, double p, double *u)
+{
+    u[0] = r;
+    u[1] = r * vx;
+    u[2] = r * vy;
+    u[3] = (p / (GAMMA - 1.)) + 0.5 * r * (vx * vx + vy * vy);
+}
+
+/* conservative_to_primitive, physics.f90:162-175 */
+static void c2p(const double *u, double *q)
+{
+    q[0] = u[0];
+    q[1] = u[1] / u[0];
+    q[2] = u[2] / u[0];
+    q[3] = pressure(u);
+}
+
+/* mesh_init.f90:14-20 */
+static inline double mesh_x(const orc_t *o, int l) { return (double)l * o->dx - o->dx / 2; }
+static inline double mesh_y(const orc_t *o, int n) { return (double)n * o->dy - o->dy / 2; }
+
+/* The two states of the double Mach reflection (IC.f90:336-355, BC.f90:305-330). */
+static void dmr_post(double *u) { p2c(8., 7.1447096, -4.125, 116.5, u); }
+static void dmr_pre(double *u) { p2c(1.4, 0.0, 0.0, 1.0, u); }
+
+/* f_DMR, IC.f90:336-355 */
+static void f_dmr(double x, double y, double *u)
+{
+    const double xmin = 1. / 6;
+    const double ymin = (x - xmin) * sqrt(3.);
+    if (y > ymin) dmr_post(u);
+    else dmr_pre(u);
+}
+
+static inline void cp4(double *d, const double *s) { d[0] = s[0]; d[1] = s[1]; d[2] = s[2]; d[3] = s[3]; }
+
+/* Boundary_C, BC.f90:18-413: the x-ghost columns of every row (ghost rows
+   included) first, then the y-ghost rows of every column (ghost columns
+   included), each in the reference's loop order, so corners take the y rule
+   applied to the x-ghost values.  The DMR top boundary reads t_rk. */
+void orc_bc(const orc_t *o, double *U)
+{
+    const int lf = o->lf, nf = o->nf, bc = o->bc;
+#define UC(l, n) (U + CELL(o, l, n) * 4)
+    for (int n = 1 - NGC; n <= nf + NGC; ++n) {
+        for (int g = 1; g <= NGC; ++g) {
+            double *ur = UC(lf + g, n), *ul = UC(1 - g, n);
+            switch (bc) {
+            case PERIODIC: case KH_BC:
+                cp4(ur, UC(g, n)); cp4(ul, UC(lf - g + 1, n));
+                break;
+            case REFLECTIVE: case RT_BC:
+                cp4(ur, UC(lf - g + 1, n)); ur[1] = -ur[1];
+                cp4(ul, UC(g, n)); ul[1] = -ul[1];
+                break;
+            case DIRICHLET:
+                p2c(1. + 0.2 * sin(5 * (mesh_x(o, lf + g) - 4.5)), 0., 0., 1., ur);
+                p2c(3.857143, 2.629369, 0., 10.33333, ul);
+                break;
+            case DMR_BC:
+                f_dmr(mesh_x(o, lf + g), mesh_y(o, n), ur);
+                f_dmr(mesh_x(o, 1 - g), mesh_y(o, n), ul);
+                break;
+            case RMI_BC:
+                p2c(2.666666666666667, 2. * sqrt(1.4), 0., 4.5, ul);
+                cp4(ur, UC(lf, n));
+                break;
+            default: /* Neumann, Mach800_BC, DoubleMach800_BC: outflow */
+                cp4(ur, UC(lf, n)); cp4(ul, UC(1, n));
+            }
+        }
+    }
+    const double xmin_top = 1. / 6 + 10. * o->t_rk / (0.5 * sqrt(3.)) + 1. / sqrt(3.);
+    for (int l = 1 - NGC; l <= lf + NGC; ++l) {
+        const double x = mesh_x(o, l);
+        for (int g = 1; g <= NGC; ++g) {
+            const int k = g - 1;
+            double *ut = UC(l, nf + g), *ub = UC(l, 1 - g);
+            switch (bc) {
+            case PERIODIC:
+                cp4(ut, UC(l, g)); cp4(ub, UC(l, nf - g + 1));
+                break;
+            case REFLECTIVE: case RMI_BC:
+                cp4(ut, UC(l, nf - g + 1)); ut[2] = -ut[2];
+                cp4(ub, UC(l, g)); ub[2] = -ub[2];
+                break;
+            case MACH800_BC:
+                cp4(ut, UC(l, nf));
+                if (fabs(x - 0.75) <= 0.05) p2c(1.4, 0.0, 800.0, 1.0, ub);
+                else cp4(ub, UC(l, 1));
+                break;
+            case DOUBLEMACH800_BC:
+                if (fabs(x - 0.75) <= 0.05) p2c(1.4, 0.0, -800.0, 1.0, ut);
+                else cp4(ut, UC(l, nf));
+                if (fabs(x - 0.75) <= 0.05) p2c(1.4, 0.0, 800.0, 1.0, ub);
+                else cp4(ub, UC(l, 1));
+                break;
+            case DMR_BC:
+                if (x < xmin_top) dmr_post(ut);
+                else dmr_pre(ut);
+                if (x > 1. / 6) { cp4(ub, UC(l, g)); ub[2] = -ub[2]; }
+                else dmr_post(ub);
+                break;
+            case RT_BC: {
+                double q[4];
+                c2p(UC(l, nf - k), q);
+                p2c(q[0], q[1], -q[2], q[3] - (double)(2 * k) * o->dy * GRAV * q[0], ut);
+                c2p(UC(l, 1 + k), q);
+                p2c(q[0], q[1], -q[2], q[3] + (double)(2 * k) * o->dy * GRAV * q[0], ub);
+                break;
+            }
+            default: /* Neumann, Dirichlet, KH_BC */
+                cp4(ut, UC(l, nf)); cp4(ub, UC(l, 1));
+            }
+        }
+    }
+#undef UC
+}
+
 /* Setdt, physics.f90:78-129 (dt_reduction is .false.): min over every cell
    including ghosts, then *CFL, then clipped to dtfinal. */
 double orc_setdt(const orc_t *o, const double *U, double dtfinal)
 {
-    const int N = o->N;
     double dt = 1.0e30;
 #pragma omp parallel for reduction(min : dt) schedule(static)
-    for (int n = 1 - NGC; n <= N + NGC; ++n) {
+    for (int n = 1 - NGC; n <= o->nf + NGC; ++n) {
         double dtl = 1.0e30;
-        for (int l = 1 - NGC; l <= N + NGC; ++l) {
+        for (int l = 1 - NGC; l <= o->lf + NGC; ++l) {
             const double *u = U + CELL(o, l, n) * 4;
             double vx = u[1] / u[0];
             double vy = u[2] / u[0];
@@ -328,22 +424,55 @@ static void unsplit(const double *ul, const double *ur, int dir, double *F)
     F[3] = F[3] + pistar * ustar;
 }
 
+/* LLF_Flux, mod_LLF.f90:11-93, with c2p_local and Flux (physics.f90:9-75). */
+static void llf(const double *ul, const double *ur, int dir, double *F)
+{
+    int adm = 1;
+    double vxl = ul[1] / ul[0], vyl = ul[2] / ul[0];
+    double pl = pressure(ul);
+    double cl = sqrt(GAMMA * pl / ul[0]);
+    if (ul[0] <= 0. || pl <= 0. || isnan(ul[0]) || isnan(pl)) adm = 0;
+    double vxr = ur[1] / ur[0], vyr = ur[2] / ur[0];
+    double pr = pressure(ur);
+    double cr = sqrt(GAMMA * pr / ur[0]);
+    if (ur[0] <= 0. || pr <= 0. || isnan(ur[0]) || isnan(pr)) adm = 0;
+    if (!adm) {
+        /* mod_LLF.f90:89: F = 1/(vx-vx), vx of the right state's c2p */
+        double z = vxr - vxr;
+        for (int k = 0; k < 4; ++k) F[k] = 1.0 / z;
+        return;
+    }
+    const double vnl = dir == 0 ? vxl : vyl, vnr = dir == 0 ? vxr : vyr;
+    double Fl[4], Fr[4];
+    if (dir == 0) {
+        Fl[0] = ul[1]; Fl[1] = ul[0] * (vxl * vxl) + pl; Fl[2] = ul[2] * vxl; Fl[3] = vxl * (ul[3] + pl);
+        Fr[0] = ur[1]; Fr[1] = ur[0] * (vxr * vxr) + pr; Fr[2] = ur[2] * vxr; Fr[3] = vxr * (ur[3] + pr);
+    } else {
+        Fl[0] = ul[2]; Fl[1] = ul[1] * vyl; Fl[2] = ul[0] * (vyl * vyl) + pl; Fl[3] = vyl * (ul[3] + pl);
+        Fr[0] = ur[2]; Fr[1] = ur[1] * vyr; Fr[2] = ur[0] * (vyr * vyr) + pr; Fr[3] = vyr * (ur[3] + pr);
+    }
+    const double vmcl = vnl - cl, vpcl = vnl + cl, vmcr = vnr - cr, vpcr = vnr + cr;
+    const double smax = fmax2(fmax2(fmax2(fabs(vmcl), fabs(vmcr)), fabs(vpcl)), fabs(vpcr));
+    for (int k = 0; k < 4; ++k) F[k] = 0.5 * (Fr[k] + Fl[k]) - 0.5 * smax * (ur[k] - ul[k]);
+}
+
 static void riemann(int flux, const double *ul, const double *ur, int dir, double *F)
 {
-    if (flux == 1) hll(ul, ur, dir, F);
-    else if (flux == 2) unsplit(ul, ur, dir, F);
+    if (flux == HLL) hll(ul, ur, dir, F);
+    else if (flux == UNSPLIT) unsplit(ul, ur, dir, F);
+    else if (flux == LLF) llf(ul, ur, dir, F);
     else hllc(ul, ur, dir, F);
 }
 
 /* GP_MOOD_, mod_GP_MOOD.f90:11-76: face states at Gauss points for cells
-   l,n in [0, N+1] whose face flag is set. */
+   l in [0, lf+1], n in [0, nf+1] whose face flag is set. */
 static void gp_mood_recons(orc_t *o, const double *Uin)
 {
-    const int N = o->N, mord = o->mord, ngp = o->ngp, sz = o->sz;
+    const int lf = o->lf, nf = o->nf, mord = o->mord, ngp = o->ngp, sz = o->sz;
 #pragma omp parallel for schedule(static)
-    for (int n = 0; n <= N + 1; ++n) {
+    for (int n = 0; n <= nf + 1; ++n) {
         double q[4][25];
-        for (int l = 0; l <= N + 1; ++l) {
+        for (int l = 0; l <= lf + 1; ++l) {
             int done = 0;
             for (int i = iL; i <= iB; ++i) {
                 int rec = 0, ord = 0;
@@ -453,26 +582,26 @@ static inline int lower_order(int ord, int cascade) /* mod_subroutine_MOOD.f90:1
 /* DETECTION, mod_subroutine_MOOD.f90:13-189.  Returns 1 when MOOD is finished. */
 static int detection(orc_t *o, const double *Uin, double *Uout)
 {
-    const int N = o->N;
+    const int lf = o->lf, nf = o->nf;
     orc_bc(o, Uout);
-    memset(o->decrease, 0, (size_t)N * N);
+    memset(o->decrease, 0, (size_t)lf * nf);
 #pragma omp parallel for schedule(dynamic, 4)
-    for (int n = 1; n <= N; ++n)
-        for (int l = 1; l <= N; ++l)
+    for (int n = 1; n <= nf; ++n)
+        for (int l = 1; l <= lf; ++l)
             if (o->detcell[CELL(o, l, n)] && o->gpo[CELL(o, l, n)] > 1)
-                o->decrease[(size_t)(n - 1) * N + (l - 1)] = (uint8_t)decide(o, Uin, Uout, l, n);
+                o->decrease[(size_t)(n - 1) * lf + (l - 1)] = (uint8_t)decide(o, Uin, Uout, l, n);
     /* clearing of passing cells' flags (:133-141); idempotent, order free */
-    for (int n = 1; n <= N; ++n)
-        for (int l = 1; l <= N; ++l)
-            if (!o->decrease[(size_t)(n - 1) * N + (l - 1)]) {
+    for (int n = 1; n <= nf; ++n)
+        for (int l = 1; l <= lf; ++l)
+            if (!o->decrease[(size_t)(n - 1) * lf + (l - 1)]) {
                 o->detcell[CELL(o, l, n)] = 0;
                 DFY(o, l, n - 1) = 0; DFY(o, l, n) = 0;
                 DFX(o, l - 1, n) = 0; DFX(o, l, n) = 0;
             }
     int finished = 1;
-    for (int n = 1; n <= N; ++n)
-        for (int l = 1; l <= N; ++l) {
-            if (!o->decrease[(size_t)(n - 1) * N + (l - 1)]) continue;
+    for (int n = 1; n <= nf; ++n)
+        for (int l = 1; l <= lf; ++l) {
+            if (!o->decrease[(size_t)(n - 1) * lf + (l - 1)]) continue;
             o->count_detected++;
             finished = 0;
             int8_t *g = &o->gpo[CELL(o, l, n)];
@@ -508,7 +637,7 @@ static void face_flux(const orc_t *o, int dir, int l, int n, double *Lf)
     }
 }
 
-/* Forward_Euler, mod_FE.f90:20-190 (GP_MOOD, HLLC, no NN).  The face loop
+/* Forward_Euler, mod_FE.f90:20-190 (GP_MOOD, no NN).  The face loop
    (:75-145) is evaluated as: every flagged face flux first, then for every
    interior cell with DetCell set the four updates in the order in which the
    reference's n-outer/l-inner loop applies them:
@@ -516,18 +645,21 @@ static void face_flux(const orc_t *o, int dir, int l, int n, double *Lf)
    Interior cells without DetCell are not touched by a re-pass (the reference
    updates them only through flagged faces, and a flagged face always has both
    neighbours flagged).  Ghost cells of Uout are overwritten by Boundary_C at the
-   start of DETECTION, so their face-loop updates are not formed. */
-void orc_fe(orc_t *o, double *Uin, double *Uout, double dt, int *npass)
+   start of DETECTION, so their face-loop updates are not formed.  After MOOD:
+   the gravity source of problem RT (:183-186) and, for SSP_RK4, Uout - Uin
+   (:187), both over the whole array. */
+static void orc_fe(orc_t *o, double *Uin, double *Uout, double dt, int first, int *npass)
 {
-    const int N = o->N;
-    const size_t ncell = (size_t)o->W * o->W;
+    const int lf = o->lf, nf = o->nf;
+    const size_t ncell = (size_t)o->W * o->H;
+    if (first) o->count_detected = 0;
     for (size_t c = 0; c < ncell; ++c) { o->gpo[c] = (int8_t)o->mord; o->detcell[c] = 1; }
     /* a priori orders (compute_CellGPO_with_NN, mod_FE.f90:51-56): interior only */
     if (o->init_orders)
-        for (int n = 1; n <= N; ++n)
-            for (int l = 1; l <= N; ++l) o->gpo[CELL(o, l, n)] = o->init_orders[(size_t)(n - 1) * N + (l - 1)];
-    memset(o->detfx, 1, (size_t)(N + 3) * (N + 2));
-    memset(o->detfy, 1, (size_t)(N + 2) * (N + 3));
+        for (int n = 1; n <= nf; ++n)
+            for (int l = 1; l <= lf; ++l) o->gpo[CELL(o, l, n)] = o->init_orders[(size_t)(n - 1) * lf + (l - 1)];
+    memset(o->detfx, 1, (size_t)(lf + 3) * (nf + 2));
+    memset(o->detfy, 1, (size_t)(lf + 2) * (nf + 3));
     orc_bc(o, Uin);
     memcpy(Uout, Uin, ncell * 4 * sizeof(double));
     const double dtdx = dt / o->dx, dtdy = dt / o->dy;
@@ -536,14 +668,14 @@ void orc_fe(orc_t *o, double *Uin, double *Uout, double dt, int *npass)
         ++passes;
         gp_mood_recons(o, Uin);
 #pragma omp parallel for schedule(static)
-        for (int n = 0; n <= N; ++n)
-            for (int l = 0; l <= N; ++l) {
+        for (int n = 0; n <= nf; ++n)
+            for (int l = 0; l <= lf; ++l) {
                 if (n >= 1 && DFX(o, l, n)) face_flux(o, 0, l, n, o->Fx + FXI(o, l, n) * 4);
                 if (l >= 1 && DFY(o, l, n)) face_flux(o, 1, l, n, o->Fy + FYI(o, l, n) * 4);
             }
 #pragma omp parallel for schedule(static)
-        for (int n = 1; n <= N; ++n)
-            for (int l = 1; l <= N; ++l) {
+        for (int n = 1; n <= nf; ++n)
+            for (int l = 1; l <= lf; ++l) {
                 if (!o->detcell[CELL(o, l, n)]) continue;
                 double *u = Uout + CELL(o, l, n) * 4;
                 const double *fyb = o->Fy + FYI(o, l, n - 1) * 4, *fxl = o->Fx + FXI(o, l - 1, n) * 4;
@@ -559,37 +691,83 @@ void orc_fe(orc_t *o, double *Uin, double *Uout, double dt, int *npass)
             }
         if (detection(o, Uin, Uout)) break;
     }
+    if (o->gravity) {
+        for (size_t c = 0; c < ncell; ++c) {
+            double *u = Uout + c * 4;
+            u[3] = u[3] - dt * GRAV * u[2];
+            u[2] = u[2] - dt * GRAV * u[0];
+        }
+    }
+    if (o->integrator == SSP_RK4)
+        for (size_t i = 0; i < ncell * 4; ++i) Uout[i] = Uout[i] - Uin[i];
     if (npass) *npass = passes;
 }
 
-/* SSP_RK3_, mod_SSP_RK3.f90:12-39 */
-void orc_ssprk3(orc_t *o, double *U, double *Ur, double dt)
+/* Coefficients of SSP_RK4 (constants.f90:49-55). */
+static const double a20 = 0.444370493651235, a21 = 0.555629506348765, c1 = 0.391752226571890,
+                    c2 = 0.368410593050371, a30 = 0.620101851488403, a32 = 0.379898148511597,
+                    c3 = 0.251891774271694, a40 = 0.178079954393132, a43 = 0.821920045606868,
+                    c4 = 0.544974750228521, f2 = 0.517231671970585, f3 = 0.096059710526147,
+                    f4 = 0.386708617503269, ff3 = 0.063692468666290, ff4 = 0.226007483236906;
+
+/* time_stepping, time_stepper.f90:12-23, with the integrators of
+   mod_SSP_RK2.f90:12-22, mod_SSP_RK3.f90:12-39 and mod_SSP_RK4.f90:12-34.
+   Every combination runs over the whole array, ghost cells included. */
+void orc_step(orc_t *o, double *U, double *Ur, double dt, double t)
 {
-    const size_t n4 = (size_t)o->W * o->W * 4;
-    o->count_detected = 0;
-    orc_fe(o, U, o->U1, dt, &o->passes[0]);
-    orc_fe(o, o->U1, o->U2, dt, &o->passes[1]);
-    for (size_t i = 0; i < n4; ++i) o->U2[i] = 0.75 * U[i] + 0.25 * o->U2[i];
-    orc_fe(o, o->U2, Ur, dt, &o->passes[2]);
-    const double a = 1.0 / 3, b = 2.0 / 3;
-    for (size_t i = 0; i < n4; ++i) Ur[i] = a * U[i] + b * Ur[i];
+    const size_t n4 = (size_t)o->W * o->H * 4;
+    double *U1 = o->S[0], *U2 = o->S[1], *U3 = o->S[2], *U4 = o->S[3], *Fl = o->S[4], *FU3 = o->S[5],
+           *FU4 = o->S[6];
+    o->t_rk = t;
+    memset(o->passes, 0, sizeof(o->passes));
+    if (o->integrator == FE) {
+        orc_fe(o, U, Ur, dt, 1, &o->passes[0]);
+    } else if (o->integrator == SSP_RK2) {
+        orc_fe(o, U, U1, dt, 1, &o->passes[0]);
+        orc_fe(o, U1, Ur, dt, 0, &o->passes[1]);
+        for (size_t i = 0; i < n4; ++i) Ur[i] = 0.5 * (U[i] + Ur[i]);
+    } else if (o->integrator == SSP_RK4) {
+        orc_fe(o, U, Fl, dt, 1, &o->passes[0]);
+        for (size_t i = 0; i < n4; ++i) U1[i] = U[i] + c1 * Fl[i];
+        orc_fe(o, U1, Fl, dt, 0, &o->passes[1]);
+        for (size_t i = 0; i < n4; ++i) U2[i] = a20 * U[i] + a21 * U1[i] + c2 * Fl[i];
+        orc_fe(o, U2, Fl, dt, 0, &o->passes[2]);
+        for (size_t i = 0; i < n4; ++i) U3[i] = a30 * U[i] + a32 * U2[i] + c3 * Fl[i];
+        orc_fe(o, U3, FU3, dt, 0, &o->passes[3]);
+        for (size_t i = 0; i < n4; ++i) U4[i] = a40 * U[i] + a43 * U3[i] + c4 * FU3[i];
+        orc_fe(o, U4, FU4, dt, 0, &o->passes[4]);
+        for (size_t i = 0; i < n4; ++i) Ur[i] = f2 * U2[i] + f3 * U3[i] + ff3 * FU3[i] + f4 * U4[i] + ff4 * FU4[i];
+    } else {
+        orc_fe(o, U, U1, dt, 1, &o->passes[0]);
+        o->t_rk = t + dt;
+        orc_fe(o, U1, U2, dt, 0, &o->passes[1]);
+        for (size_t i = 0; i < n4; ++i) U2[i] = 0.75 * U[i] + 0.25 * U2[i];
+        o->t_rk = t + 0.5 * dt;
+        orc_fe(o, U2, Ur, dt, 0, &o->passes[2]);
+        const double a = 1.0 / 3, b = 2.0 / 3;
+        for (size_t i = 0; i < n4; ++i) Ur[i] = a * U[i] + b * Ur[i];
+    }
 }
 
+/* SSP_RK3 only, kept for callers of the former interface (t_rk = 0 at start). */
+void orc_ssprk3(orc_t *o, double *U, double *Ur, double dt) { orc_step(o, U, Ur, dt, 0.0); }
+
 /* The step loop of main.f90:145-162 for nsteps steps on the full array U
-   (ghosts refreshed by Boundary_C before the first step, as main.f90:125).
-   Optional outputs: dts[nsteps], counts[nsteps], masks[nsteps][N][N] (CellGPO
-   after the last stage of each step), passes[nsteps][3]. */
+   (ghosts refreshed by Boundary_C before the first step, as main.f90:125, at
+   the current t_rk).  Optional outputs: dts[nsteps], counts[nsteps],
+   masks[nsteps][nf][lf] (CellGPO after the last stage of each step),
+   passes[nsteps][3]. */
 int orc_run(orc_t *o, double *U, int nsteps, double *t_io, double tmax,
             double *dts, long *counts, int8_t *masks, int *passes)
 {
-    const int N = o->N;
-    const size_t n4 = (size_t)o->W * o->W * 4;
+    const int lf = o->lf, nf = o->nf;
+    const size_t n4 = (size_t)o->W * o->H * 4;
     double *Ur = (double *)calloc(n4, sizeof(double));
     double t = t_io ? *t_io : 0.0;
     orc_bc(o, U);
     for (int s = 0; s < nsteps; ++s) {
         double dt = orc_setdt(o, U, tmax - t);
-        orc_ssprk3(o, U, Ur, dt);
+        orc_step(o, U, Ur, dt, t);
         orc_bc(o, U);
         t = t + dt;
         memcpy(U, Ur, n4 * sizeof(double));
@@ -597,39 +775,42 @@ int orc_run(orc_t *o, double *U, int nsteps, double *t_io, double tmax,
         if (counts) counts[s] = o->count_detected;
         if (passes) for (int k = 0; k < 3; ++k) passes[s * 3 + k] = o->passes[k];
         if (masks)
-            for (int n = 1; n <= N; ++n)
-                for (int l = 1; l <= N; ++l)
-                    masks[(size_t)s * N * N + (size_t)(n - 1) * N + (l - 1)] = o->gpo[CELL(o, l, n)];
+            for (int n = 1; n <= nf; ++n)
+                for (int l = 1; l <= lf; ++l)
+                    masks[(size_t)s * lf * nf + (size_t)(n - 1) * lf + (l - 1)] = o->gpo[CELL(o, l, n)];
     }
     if (t_io) *t_io = t;
     free(Ur);
     return 0;
 }
 
-/* Per-stage variant for parity on intermediate data: runs one step and returns
-   the three stage outputs (after RK combination) and the three stage masks. */
-int orc_step_stages(orc_t *o, double *U, double dt, double *stage_out /* 3*(N+8)^2*4 */,
-                    int8_t *stage_masks /* 3*N*N */)
+/* Per-stage variant of SSP_RK3 for parity on intermediate data: runs one step
+   from time t and returns the three stage outputs (after RK combination) and
+   the three stage masks, as the statements of mod_SSP_RK3.f90:23-33. */
+int orc_step_stages(orc_t *o, double *U, double dt, double t, double *stage_out /* 3*W*H*4 */,
+                    int8_t *stage_masks /* 3*lf*nf */)
 {
-    const int N = o->N;
-    const size_t n4 = (size_t)o->W * o->W * 4;
+    const int lf = o->lf, nf = o->nf;
+    const size_t n4 = (size_t)o->W * o->H * 4;
     double *Ur = (double *)calloc(n4, sizeof(double));
+    double *U1 = o->S[0], *U2 = o->S[1];
     orc_bc(o, U);
-    o->count_detected = 0;
-    double *outs[3] = {o->U1, o->U2, Ur};
-    double *ins[3] = {U, o->U1, o->U2};
+    double *outs[3] = {U1, U2, Ur};
+    double *ins[3] = {U, U1, U2};
+    const double trk[3] = {t, t + dt, t + 0.5 * dt};
     for (int st = 0; st < 3; ++st) {
-        orc_fe(o, ins[st], outs[st], dt, &o->passes[st]);
-        if (st == 1) for (size_t i = 0; i < n4; ++i) o->U2[i] = 0.75 * U[i] + 0.25 * o->U2[i];
+        o->t_rk = trk[st];
+        orc_fe(o, ins[st], outs[st], dt, st == 0, &o->passes[st]);
+        if (st == 1) for (size_t i = 0; i < n4; ++i) U2[i] = 0.75 * U[i] + 0.25 * U2[i];
         if (st == 2) {
             const double a = 1.0 / 3, b = 2.0 / 3;
             for (size_t i = 0; i < n4; ++i) Ur[i] = a * U[i] + b * Ur[i];
         }
         if (stage_out) memcpy(stage_out + st * n4, outs[st], n4 * sizeof(double));
         if (stage_masks)
-            for (int n = 1; n <= N; ++n)
-                for (int l = 1; l <= N; ++l)
-                    stage_masks[(size_t)st * N * N + (size_t)(n - 1) * N + (l - 1)] = o->gpo[CELL(o, l, n)];
+            for (int n = 1; n <= nf; ++n)
+                for (int l = 1; l <= lf; ++l)
+                    stage_masks[(size_t)st * lf * nf + (size_t)(n - 1) * lf + (l - 1)] = o->gpo[CELL(o, l, n)];
     }
     orc_bc(o, U);
     memcpy(U, Ur, n4 * sizeof(double));
@@ -640,13 +821,16 @@ int orc_step_stages(orc_t *o, double *U, double dt, double *stage_out /* 3*(N+8)
 long orc_count_detected(const orc_t *o) { return o->count_detected; }
 void orc_passes(const orc_t *o, int *p) { for (int k = 0; k < 3; ++k) p[k] = o->passes[k]; }
 
-/* The Riemann solvers alone, for unit checks (dir 0 = x, 1 = y; flux 0 HLLC,
-   1 HLL, 2 unsplit). */
+/* The Riemann solvers alone, for unit checks (dir 0 = x, 1 = y; flux: the
+   reference's numFlux id). */
 void orc_hllc(const double *ul, const double *ur, int dir, double *F) { hllc(ul, ur, dir, F); }
 void orc_riemann(int flux, const double *ul, const double *ur, int dir, double *F) { riemann(flux, ul, ur, dir, F); }
 
-/* numFlux of the step (0 HLLC, 1 HLL, 2 unsplit). */
+/* Options of the step: numFlux, integrator and problem ids; t_rk. */
 void orc_set_flux(orc_t *o, int flux) { o->flux = flux; }
+void orc_set_integrator(orc_t *o, int integrator) { o->integrator = integrator; }
+void orc_set_problem(orc_t *o, int problem) { o->gravity = problem == PROBLEM_RT; }
+void orc_set_t_rk(orc_t *o, double t_rk) { o->t_rk = t_rk; }
 
 /* Fixed a priori orders for every stage (NULL restores CellGPO = Mord). */
 void orc_set_initial_orders(orc_t *o, const int8_t *orders)
@@ -654,6 +838,6 @@ void orc_set_initial_orders(orc_t *o, const int8_t *orders)
     free(o->init_orders);
     o->init_orders = NULL;
     if (!orders) return;
-    o->init_orders = (int8_t *)malloc((size_t)o->N * o->N);
-    memcpy(o->init_orders, orders, (size_t)o->N * o->N);
+    o->init_orders = (int8_t *)malloc((size_t)o->lf * o->nf);
+    memcpy(o->init_orders, orders, (size_t)o->lf * o->nf);
 }

@@ -10,8 +10,12 @@ import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "liboracle.so")
-BC_CODES = {"neumann": 0, "periodic": 1, "reflective": 2, "kh": 3}
-FLUX_CODES = {"hllc": 0, "hll": 1, "unsplit": 2}
+# the reference's ids (constants.f90:17-34)
+BC_CODES = {"neumann": 17, "periodic": 18, "dirichlet": 19, "reflective": 20, "kh": 403, "mach800": 201,
+            "doublemach800": 301, "rmi": 401, "dmr": 35, "rt": 402}
+FLUX_CODES = {"hllc": 100, "llf": 101, "hll": 102, "unsplit": 103}
+INTEGRATORS = {"fe": 8, "ssp_rk2": 9, "ssp_rk3": 10, "ssp_rk4": 12}
+PROBLEM_RT = 51
 CASCADES = {"reference": 0, "full": 1}
 
 _lib = None
@@ -23,21 +27,25 @@ def lib():
         if not os.path.exists(LIB_PATH):
             raise RuntimeError("oracle library not built: run `make -C oracle`")
         L = ctypes.CDLL(LIB_PATH)
-        P = ctypes.c_void_p
+        P, I, D = ctypes.c_void_p, ctypes.c_int, ctypes.c_double
         L.orc_create.restype = P
-        L.orc_create.argtypes = [ctypes.c_int] * 8 + [ctypes.c_double] * 3 + [P] * 5
+        L.orc_create.argtypes = [I] * 9 + [D] * 3 + [P] * 5
         L.orc_destroy.argtypes = [P]
-        L.orc_set_flux.argtypes = [P, ctypes.c_int]
-        L.orc_riemann.argtypes = [ctypes.c_int, P, P, ctypes.c_int, P]
+        L.orc_set_flux.argtypes = [P, I]
+        L.orc_set_integrator.argtypes = [P, I]
+        L.orc_set_problem.argtypes = [P, I]
+        L.orc_set_t_rk.argtypes = [P, D]
+        L.orc_riemann.argtypes = [I, P, P, I, P]
         L.orc_bc.argtypes = [P, P]
-        L.orc_setdt.restype = ctypes.c_double
-        L.orc_setdt.argtypes = [P, P, ctypes.c_double]
-        L.orc_run.argtypes = [P, P, ctypes.c_int, P, ctypes.c_double, P, P, P, P]
-        L.orc_step_stages.argtypes = [P, P, ctypes.c_double, P, P]
+        L.orc_setdt.restype = D
+        L.orc_setdt.argtypes = [P, P, D]
+        L.orc_step.argtypes = [P, P, P, D, D]
+        L.orc_run.argtypes = [P, P, I, P, D, P, P, P, P]
+        L.orc_step_stages.argtypes = [P, P, D, D, P, P]
         L.orc_count_detected.restype = ctypes.c_long
         L.orc_count_detected.argtypes = [P]
         L.orc_passes.argtypes = [P, P]
-        L.orc_hllc.argtypes = [P, P, ctypes.c_int, P]
+        L.orc_hllc.argtypes = [P, P, I, P]
         L.orc_set_initial_orders.argtypes = [P, P]
         _lib = L
     return _lib
@@ -49,21 +57,29 @@ def _ptr(a):
 
 class Oracle:
     """The reference step on CPU.  ``tables`` is a dict with keys zT, d2x, d2y,
-    gw, ixiy in the reference's Fortran layouts (flattened column-major)."""
+    gw, ixiy in the reference's Fortran layouts (flattened column-major).
+
+    The grid is ``n`` (lf) cells in x by ``nf`` (default ``n``) in y on
+    [0, lx] x [0, ly]; arrays are indexed [row (y), column (x), variable]."""
 
     def __init__(self, n, mord, ngp, bc, tables, dmp=True, u2=True, u2tol=True,
-                 cascade="reference", cfl=0.8, flux="hllc"):
-        self.n = n
+                 cascade="reference", cfl=0.8, flux="hllc", nf=None, lx=1.0, ly=1.0,
+                 integrator="ssp_rk3", problem=0, dx=None, dy=None):
+        self.n = self.lf = n
+        self.nf = n if nf is None else nf
         self.tables = {k: np.ascontiguousarray(v) for k, v in tables.items()}
         t = self.tables
-        dx = 1.0 / n
-        self._h = lib().orc_create(n, mord, ngp, BC_CODES[bc], int(dmp), int(u2), int(u2tol),
-                                   CASCADES[cascade], cfl, dx, dx,
+        dx = lx / n if dx is None else dx
+        dy = ly / self.nf if dy is None else dy
+        self._h = lib().orc_create(n, self.nf, mord, ngp, BC_CODES[bc], int(dmp), int(u2), int(u2tol),
+                                   CASCADES[cascade], cfl, dx, dy,
                                    _ptr(t["zT"].astype(np.float64)), _ptr(t["d2x"].astype(np.float64)),
                                    _ptr(t["d2y"].astype(np.float64)), _ptr(t["gw"].astype(np.float64)),
                                    _ptr(t["ixiy"].astype(np.int32)))
         self._keep = [t[k] for k in t]
         lib().orc_set_flux(self._h, FLUX_CODES[flux])
+        lib().orc_set_integrator(self._h, INTEGRATORS[integrator])
+        lib().orc_set_problem(self._h, int(problem))
 
     def __del__(self):
         if getattr(self, "_h", None):
@@ -71,11 +87,14 @@ class Oracle:
             self._h = None
 
     def full(self, interior):
-        n = self.n
-        U = np.zeros((n + 8, n + 8, 4), dtype=np.float64)
-        U[4:n + 4, 4:n + 4] = interior
+        """Ghosted array from the interior, ghosts by Boundary_C at the current t_rk."""
+        U = np.zeros((self.nf + 8, self.lf + 8, 4), dtype=np.float64)
+        U[4:self.nf + 4, 4:self.lf + 4] = interior
         lib().orc_bc(self._h, _ptr(U))
         return U
+
+    def set_t_rk(self, t_rk):
+        lib().orc_set_t_rk(self._h, float(t_rk))
 
     def run(self, U, nsteps, t=0.0, tmax=1e10, masks=False):
         """Advance full array U (modified in place) by nsteps steps."""
@@ -83,32 +102,38 @@ class Oracle:
         dts = np.zeros(nsteps)
         counts = np.zeros(nsteps, dtype=np.int64)
         passes = np.zeros((nsteps, 3), dtype=np.int32)
-        m = np.zeros((nsteps, self.n, self.n), dtype=np.int8) if masks else None
+        m = np.zeros((nsteps, self.nf, self.lf), dtype=np.int8) if masks else None
         tt = ctypes.c_double(t)
         lib().orc_run(self._h, _ptr(U), nsteps, ctypes.byref(tt), tmax, _ptr(dts), _ptr(counts),
                       _ptr(m), _ptr(passes))
         return dict(dt=dts, counts=counts, masks=m, passes=passes, t=tt.value)
 
+    def step(self, U, dt, t=0.0):
+        """One time_stepping call from full array U at time t; returns Ur (U's
+        ghosts are refilled as by the reference)."""
+        Ur = np.zeros_like(U)
+        lib().orc_step(self._h, _ptr(U), _ptr(Ur), dt, t)
+        return Ur
+
     def set_initial_orders(self, orders):
-        """Fixed a priori CellGPO (int8 [N][N]) for every stage, or None."""
+        """Fixed a priori CellGPO (int8 [nf][lf]) for every stage, or None."""
         self._orders = None if orders is None else np.ascontiguousarray(orders, dtype=np.int8)
         lib().orc_set_initial_orders(self._h, _ptr(self._orders))
 
     def setdt(self, U, dtfinal=1e10):
         return lib().orc_setdt(self._h, _ptr(U), dtfinal)
 
-    def step_stages(self, U, dt):
-        n = self.n
-        outs = np.zeros((3, n + 8, n + 8, 4))
-        masks = np.zeros((3, n, n), dtype=np.int8)
-        lib().orc_step_stages(self._h, _ptr(U), dt, _ptr(outs), _ptr(masks))
+    def step_stages(self, U, dt, t=0.0):
+        outs = np.zeros((3, self.nf + 8, self.lf + 8, 4))
+        masks = np.zeros((3, self.nf, self.lf), dtype=np.int8)
+        lib().orc_step_stages(self._h, _ptr(U), dt, t, _ptr(outs), _ptr(masks))
         p = np.zeros(3, dtype=np.int32)
         lib().orc_passes(self._h, _ptr(p))
         return outs, masks, p
 
 
 def riemann(flux, ul, ur, direction):
-    """Riemann solver of the oracle (HLLC, HLL or unsplit) for one face."""
+    """Riemann solver of the oracle (HLLC, LLF, HLL or unsplit) for one face."""
     ul = np.ascontiguousarray(ul, dtype=np.float64)
     ur = np.ascontiguousarray(ur, dtype=np.float64)
     F = np.zeros(4)

@@ -24,8 +24,9 @@ the reference's own Makefile (gp-mood-code/Makefile:8-58):
     renames them (case-sensitive macro, so ``PR`` itself is untouched);
   - ``mod_NN.f90:41,44`` index ``values_NN`` with the NN input-size parameter
     ``L`` (the reference's undeclared loop variable ``l`` resolves to it), a
-    compile-time bounds error when ``L > lf+2``.  The NN arrays are never used
-    with ``method = GP_MOOD``, so ``parameters.f90`` sets ``L = 32``;
+    compile-time bounds error when ``L`` exceeds the grid.  The NN arrays are
+    never used with ``method = GP_MOOD``, so ``parameters.f90`` sets
+    ``L = min(32, lf)``;
   - ``global_variables.f90:15`` evaluates ``szs_sphere(Mord+2)``, out of bounds
     for ``Mord=7`` (the value only feeds the reference NN stencil size).  The
     macro ``szs_sphere(x)=szs_sphere(min(x,7))`` clamps the index; flang's
@@ -58,8 +59,11 @@ MODULES = [
     "mod_SSP_RK4", "time_stepper",
 ]
 
-BC_NAMES = {"neumann": "Neumann", "periodic": "Periodic", "reflective": "Reflective", "kh": "KH_BC"}
-FLUX_NAMES = {"hllc": "HLLC", "hll": "HLL", "unsplit": "unsplit"}
+BC_NAMES = {"neumann": "Neumann", "periodic": "Periodic", "reflective": "Reflective", "kh": "KH_BC",
+            "dirichlet": "Dirichlet", "mach800": "Mach800_BC", "doublemach800": "DoubleMach800_BC", "rmi": "RMI_BC",
+            "dmr": "DMR", "rt": "RT_BC"}
+FLUX_NAMES = {"hllc": "HLLC", "llf": "LLF", "hll": "HLL", "unsplit": "unsplit"}
+INTEGRATOR_NAMES = {"fe": "FE", "ssp_rk2": "SSP_RK2", "ssp_rk3": "SSP_RK3", "ssp_rk4": "SSP_RK4"}
 
 # name: (N, Mord, ngp, bc, DMP, U2, U2_tol[, numFlux]); numFlux defaults to HLLC
 CONFIGS = {
@@ -83,7 +87,53 @@ CONFIGS = {
     "r3_n64_per_hll": (64, 7, 4, "periodic", True, True, True, "hll"),
     "r3_n64_kh_uns": (64, 7, 4, "kh", True, True, True, "unsplit"),
     "r2_n64_kh_hllc": (64, 5, 3, "kh", True, True, True),
+    # formula boundary conditions on rectangular grids (problems/dmr.f90,
+    # Mach800_GPMOOD.f90, RT.f90 at reduced resolution; BC.f90:60-88, 137-381)
+    "r1_dmr_64x16": (64, 3, 2, "dmr", True, True, True),
+    "r2_dmr_64x16": (64, 5, 3, "dmr", True, True, True),
+    "r1_m800_32": (32, 3, 2, "mach800", True, True, True, "hll"),
+    "r1_rt_16x48": (16, 3, 2, "rt", True, True, True, "unsplit"),
+    "r2_rt_16x48": (16, 5, 3, "rt", True, True, True, "unsplit"),
+    "r1_dm800_32": (32, 3, 2, "doublemach800", True, True, True, "hll"),
+    "r1_rmi_48x8": (48, 3, 2, "rmi", True, True, True),
+    "r1_dir_64x8": (64, 3, 2, "dirichlet", True, True, True, "llf"),
+    # the other integrators (time_stepper.f90:17-21) and the LLF flux
+    "r2_n32_neu_fe": (32, 5, 3, "neumann", True, True, True),
+    "r2_n32_neu_rk2": (32, 5, 3, "neumann", True, True, True),
+    "r2_n32_ref_rk4": (32, 5, 3, "reflective", True, True, True),
+    "r3_n32_per_rk4": (32, 7, 4, "periodic", True, True, True, "hll"),
+    "r2_n32_neu_llf": (32, 5, 3, "neumann", True, True, True, "llf"),
+    "r1_dmr_64x16_rk2": (64, 3, 2, "dmr", True, True, True),
+    "r1_m800_32_fe": (32, 3, 2, "mach800", True, True, True, "hll"),
+    "r1_rt_16x48_rk2": (16, 3, 2, "rt", True, True, True, "unsplit"),
 }
+
+# grid, domain, integrator and problem of the configurations that differ from
+# a square unit domain, SSP_RK3 and a problem without source term
+EXTRAS = {
+    "r1_dmr_64x16": dict(nf=16, lx="4.", ly="1."),
+    "r2_dmr_64x16": dict(nf=16, lx="4.", ly="1."),
+    "r1_m800_32": dict(lx="1.5", ly="1.5"),
+    "r1_rt_16x48": dict(nf=48, lx="0.5", ly="1.5", problem="RT"),
+    "r2_rt_16x48": dict(nf=48, lx="0.5", ly="1.5", problem="RT"),
+    "r1_dm800_32": dict(lx="1.5", ly="1.5"),
+    "r1_rmi_48x8": dict(nf=8, lx="6.", ly="1."),
+    "r1_dir_64x8": dict(nf=8, lx="9.", ly="1.125"),
+    "r2_n32_neu_fe": dict(integrator="fe"),
+    "r2_n32_neu_rk2": dict(integrator="ssp_rk2"),
+    "r2_n32_ref_rk4": dict(integrator="ssp_rk4"),
+    "r3_n32_per_rk4": dict(integrator="ssp_rk4"),
+    "r1_dmr_64x16_rk2": dict(nf=16, lx="4.", ly="1.", integrator="ssp_rk2"),
+    "r1_m800_32_fe": dict(lx="1.5", ly="1.5", integrator="fe"),
+    "r1_rt_16x48_rk2": dict(nf=48, lx="0.5", ly="1.5", problem="RT", integrator="ssp_rk2"),
+}
+
+
+def extras(name):
+    """nf, Lx, Ly (as Fortran literals), integrator and problem of a configuration."""
+    e = dict(nf=CONFIGS[name][0], lx="1.", ly="1.", integrator="ssp_rk3", problem="RP_2D_3")
+    e.update(EXTRAS.get(name, {}))
+    return e
 
 
 def config(name):
@@ -99,7 +149,7 @@ PARAMS_TEMPLATE = """module parameters
    use constants
    implicit none
    real(PR), parameter :: CFL  =  0.8
-   integer , parameter :: integrator    = SSP_RK3
+   integer , parameter :: integrator    = {integrator}
    logical , parameter :: restart = .false.
    character(100) :: restart_filename='.'
    integer, parameter :: method   = GP_MOOD
@@ -111,19 +161,19 @@ PARAMS_TEMPLATE = """module parameters
    logical , parameter :: U2_tol     = {u2tol}
    logical :: write_NN_dataset=.false.
    integer, parameter :: dataset_size = 1
-   integer, parameter :: L=32
+   integer, parameter :: L={nn_l}
    integer, parameter :: length=60
    character(100) :: NN_filename=''
    integer, parameter :: numFlux = {flux}
    integer, parameter :: IO_freqStep = -100
    real(PR), parameter:: IO_freqTime = -1.e-3
    integer,  parameter :: lf = {n}
-   integer,  parameter :: nf = {n}
-   integer, parameter  :: problem = RP_2D_3
+   integer,  parameter :: nf = {nf}
+   integer, parameter  :: problem = {problem}
    real(PR), parameter :: tmax = 1.e10
    integer, parameter  :: nmax = 100000
-   real(16), parameter :: Lx_16 = 1.
-   real(16), parameter :: Ly_16 = 1.
+   real(16), parameter :: Lx_16 = {lx}
+   real(16), parameter :: Ly_16 = {ly}
    integer, parameter  :: BC_type = {bc}
 end module
 """
@@ -139,12 +189,15 @@ def build(name, dropin=False):
     (gp-mood_amd/fortran/time_stepper.f90 over gpm_capi.f90), which runs the
     step through libgpmood; everything else is the reference's own code."""
     n, mord, ngp, bc, dmp, u2, u2tol, flux = config(name)
+    e = extras(name)
     out = os.path.join(OUT_ROOT, name + ("_dropin" if dropin else ""))
     os.makedirs(out, exist_ok=True)
     params = os.path.join(out, "parameters.f90")
     with open(params, "w") as f:
         f.write(PARAMS_TEMPLATE.format(mord=mord, ngp=ngp, dmp=fbool(dmp), u2=fbool(u2),
-                                       u2tol=fbool(u2tol), n=n, bc=BC_NAMES[bc], flux=FLUX_NAMES[flux]))
+                                       u2tol=fbool(u2tol), n=n, bc=BC_NAMES[bc], flux=FLUX_NAMES[flux],
+                                       nf=e["nf"], lx=e["lx"], ly=e["ly"], problem=e["problem"],
+                                       integrator=INTEGRATOR_NAMES[e["integrator"]], nn_l=min(32, n)))
     flags = ["-fdefault-real-8", "-O3", "-ffp-contract=off", "-J", out, "-I", out]
     if n >= 1024:
         flags.append("-mcmodel=medium")

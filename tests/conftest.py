@@ -54,6 +54,16 @@ def tables_from_npz(z):
     return {k: z[k] for k in ("zT", "d2x", "d2y", "gw", "ixiy")}
 
 
+def case_options(z):
+    """Grid, domain and step options of a golden step case (defaults: square
+    unit domain, SSP_RK3, HLLC, no source term)."""
+    n = int(z["n"])
+    return dict(nf=int(z["nf"]) if "nf" in z else n, lx=float(z["lx"]) if "lx" in z else 1.0,
+                ly=float(z["ly"]) if "ly" in z else 1.0, flux=str(z["flux"]) if "flux" in z else "hllc",
+                integrator=str(z["integrator"]) if "integrator" in z else "ssp_rk3",
+                problem=int(z["problem"]) if "problem" in z else 0)
+
+
 @pytest.fixture(scope="session")
 def gpm():
     return load_package()

@@ -47,6 +47,7 @@ def read_tables(path):
 
 def run_ref(cfg, U0, nsteps, mode):
     n = build_ref.CONFIGS[cfg][0]
+    nf = build_ref.extras(cfg)["nf"]
     exe = os.path.join(REF, cfg, "gpm_ref.exe")
     if not os.path.exists(exe):
         build_ref.build(cfg)
@@ -56,10 +57,10 @@ def run_ref(cfg, U0, nsteps, mode):
                        stdout=subprocess.DEVNULL)
         res = {"tables": read_tables(os.path.join(d, "tables.bin")),
                "steps": np.fromfile(os.path.join(d, "steps.bin"), dtype=STEP_TYPE),
-               "masks": np.fromfile(os.path.join(d, "masks.bin"), dtype=np.int8).reshape(-1, n, n),
-               "U": np.fromfile(os.path.join(d, "U_final.bin"), dtype=np.float64).reshape(n + 8, n + 8, 4)}
+               "masks": np.fromfile(os.path.join(d, "masks.bin"), dtype=np.int8).reshape(-1, nf, n),
+               "U": np.fromfile(os.path.join(d, "U_final.bin"), dtype=np.float64).reshape(nf + 8, n + 8, 4)}
         if mode == 1:
-            res["U_stage"] = np.fromfile(os.path.join(d, "U_stage.bin"), dtype=np.float64).reshape(3, n + 8, n + 8, 4)
+            res["U_stage"] = np.fromfile(os.path.join(d, "U_stage.bin"), dtype=np.float64).reshape(3, nf + 8, n + 8, 4)
     return res
 
 
@@ -89,6 +90,28 @@ STEP_CASES = {
     "r3_n64_per_hll_fq14": ("r3_n64_per_hll", "four_quadrant", 14, 15),
     "r3_n64_kh_uns_fq15": ("r3_n64_kh_uns", "four_quadrant", 15, 15),
     "r2_n64_kh_hllc_fq16": ("r2_n64_kh_hllc", "four_quadrant", 16, 15),
+    # formula boundary conditions, rectangular grids, gravity (problem set-ups
+    # of gp-mood_amd/ic.py: "<problem>:<noise>")
+    "r1_dmr_64x16": ("r1_dmr_64x16", "dmr:0", 0, 20),
+    "r2_dmr_64x16_n21": ("r2_dmr_64x16", "dmr:0.02", 21, 20),
+    "r1_m800_32_n22": ("r1_m800_32", "mach800:0.02", 22, 20),
+    "r1_rt_16x48_n23": ("r1_rt_16x48", "rayleigh_taylor:0.05", 23, 20),
+    "r2_rt_16x48_n24": ("r2_rt_16x48", "rayleigh_taylor:0.05", 24, 20),
+    "r1_dm800_32_n25": ("r1_dm800_32", "mach800:0.02", 25, 15),
+    "r1_rmi_48x8_n26": ("r1_rmi_48x8", "shock_tube_x:0.05", 26, 20),
+    "r1_dir_64x8_n27": ("r1_dir_64x8", "shock_tube_x:0.05", 27, 20),
+    # the other integrators and the LLF flux
+    "r2_n32_neu_fe_fq28": ("r2_n32_neu_fe", "four_quadrant", 28, 20),
+    "r2_n32_neu_rk2_fq29": ("r2_n32_neu_rk2", "four_quadrant", 29, 20),
+    "r2_n32_ref_rk4_bl30": ("r2_n32_ref_rk4", "blast", 30, 15),
+    "r3_n32_per_rk4_fq31": ("r3_n32_per_rk4", "four_quadrant", 31, 10),
+    "r2_n32_neu_llf_fq32": ("r2_n32_neu_llf", "four_quadrant", 32, 20),
+    "r1_dmr_64x16_rk2": ("r1_dmr_64x16_rk2", "dmr:0", 0, 20),
+    "r1_m800_32_fe_n33": ("r1_m800_32_fe", "mach800:0.02", 33, 20),
+    "r1_rt_16x48_rk2_n34": ("r1_rt_16x48_rk2", "rayleigh_taylor:0.05", 34, 20),
+    "r1_rt_16x48_q35": ("r1_rt_16x48", "quadrants:0", 35, 20),
+    "r2_rt_16x48_q36": ("r2_rt_16x48", "quadrants:0", 36, 20),
+    "r1_rt_16x48_rk2_q37": ("r1_rt_16x48_rk2", "quadrants:0", 37, 20),
 }
 # long runs stored as checksums: name: (config, IC kind, seed, steps)
 HASH_CASES = {
@@ -102,15 +125,25 @@ def main(argv):
         if want and name not in want:
             continue
         n = build_ref.CONFIGS[cfg][0]
-        U0 = ic.make(kind, n, seed)
+        e = build_ref.extras(cfg)
+        if ":" in kind:
+            pk, noise = kind.split(":")
+            U0 = ic.problem(pk, n, e["nf"], float(e["lx"]), float(e["ly"]), seed=seed, noise=float(noise))
+        else:
+            U0 = ic.make(kind, n, seed)
         r = run_ref(cfg, U0, nsteps, 0)
-        r1 = run_ref(cfg, U0, 1, 1)  # per-stage data of the first step
+        extra = {}
+        if e["integrator"] == "ssp_rk3":
+            r1 = run_ref(cfg, U0, 1, 1)  # per-stage data of the first step
+            extra = dict(stage_U=r1["U_stage"], stage_masks=r1["masks"])
         _, mord, ngp, bc, dmp, u2, u2tol, flux = build_ref.config(cfg)
         np.savez_compressed(
             os.path.join(OUT, f"step_{name}.npz"), n=n, mord=mord, ngp=ngp, bc=bc, dmp=dmp, u2=u2, u2tol=u2tol, flux=flux,
+            nf=e["nf"], lx=float(e["lx"]), ly=float(e["ly"]), integrator=e["integrator"],
+            problem=51 if e["problem"] == "RT" else 0,
             kind=kind, seed=seed, nsteps=nsteps, ic_sha=sha(U0), U0=U0,
             dt=r["steps"]["dt"], t=r["steps"]["t"], counts=r["steps"]["cnt"], masks=r["masks"], U=r["U"],
-            stage_U=r1["U_stage"], stage_masks=r1["masks"], **{k: v for k, v in r["tables"].items()})
+            **extra, **{k: v for k, v in r["tables"].items()})
         print("wrote", name, "lowered", int((r["masks"] < mord).sum()))
     for name, (cfg, kind, seed, nsteps) in HASH_CASES.items():
         if want and name not in want:

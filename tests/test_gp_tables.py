@@ -7,7 +7,7 @@ import os
 import numpy as np
 import pytest
 
-from conftest import GOLDEN, golden, load_package
+from conftest import GOLDEN, case_options, golden, load_package
 
 SZ = {1: 1, 3: 5, 5: 13, 7: 25}
 
@@ -24,7 +24,8 @@ def fixture_cases():
     seen = {}
     for p in sorted(glob.glob(os.path.join(GOLDEN, "step_*.npz"))):
         z = np.load(p, allow_pickle=False)
-        key = (int(z["mord"]), int(z["ngp"]), int(z["n"]))
+        o = case_options(z)
+        key = (int(z["mord"]), int(z["ngp"]), int(z["n"]), o["nf"], o["lx"], o["ly"])
         seen.setdefault(key, os.path.basename(p))
     return sorted(seen.items())
 
@@ -36,10 +37,10 @@ def zt_view(z, mord, ngp):
 
 @pytest.mark.parametrize("key,fname", fixture_cases())
 def test_tables_match_reference(key, fname):
-    mord, ngp, n = key
+    mord, ngp, n, nf, lx, ly = key
     gpm = load_package()
     ref = golden(fname)
-    t = gpm.gp_tables(mord, ngp, n)
+    t = gpm.gp_tables(mord, ngp, n, nf, lx=lx, ly=ly)
     assert np.array_equal(t["ixiy"], ref["ixiy"])
     assert np.array_equal(t["gw"], ref["gw"]), "Gauss weights are FP64 expressions and must match exactly"
     zr, zm = zt_view(ref["zT"], mord, ngp), zt_view(t["zT"], mord, ngp)

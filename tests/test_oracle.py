@@ -12,15 +12,18 @@ import os
 import numpy as np
 import pytest
 
-from conftest import GOLDEN, golden, tables_from_npz
+from conftest import GOLDEN, case_options, golden, tables_from_npz
 
 STEP_CASES = sorted(os.path.basename(p)[5:-4] for p in glob.glob(os.path.join(GOLDEN, "step_*.npz")))
+STAGE_CASES = [c for c in STEP_CASES if "stage_U" in np.load(os.path.join(GOLDEN, f"step_{c}.npz")).files]
 
 
 def make_oracle(O, z):
+    o = case_options(z)
+    dx, dy = (float(v) for v in z["dxdy"])
     return O.Oracle(int(z["n"]), int(z["mord"]), int(z["ngp"]), str(z["bc"]), tables_from_npz(z),
-                    dmp=bool(z["dmp"]), u2=bool(z["u2"]), u2tol=bool(z["u2tol"]),
-                    flux=str(z["flux"]) if "flux" in z else "hllc")
+                    dmp=bool(z["dmp"]), u2=bool(z["u2"]), u2tol=bool(z["u2tol"]), flux=o["flux"], nf=o["nf"],
+                    integrator=o["integrator"], problem=o["problem"], dx=dx, dy=dy)
 
 
 @pytest.mark.parametrize("case", STEP_CASES)
@@ -35,7 +38,7 @@ def test_oracle_matches_reference_run(orc, case):
     assert np.array_equal(U, z["U"])
 
 
-@pytest.mark.parametrize("case", STEP_CASES)
+@pytest.mark.parametrize("case", STAGE_CASES)
 def test_oracle_matches_reference_stages(orc, case):
     z = golden(f"step_{case}.npz")
     o = make_oracle(orc, z)
