@@ -41,18 +41,36 @@ class Config:
     ell_o_dx: float = 12.0
     method: int = capi.GP_MOOD
     numFlux: str = "hllc"
+    integrator: str = "ssp_rk3"
+    problem: int = 0  # the reference's problem id; capi.PROBLEM_RT adds the gravity source
 
     def to_c(self):
         nf = self.lf if self.nf is None else self.nf
         return capi.GpmConfig(
-            lf=self.lf, nf=nf, mord=self.Mord, ngp=self.ngp, method=self.method, integrator=capi.SSP_RK3,
-            num_flux=capi.FLUX_IDS[self.numFlux], bc_type=capi.BC_IDS[self.BC_type], dmp=int(self.DMP), u2=int(self.U2),
-            u2_tol=int(self.U2_tol), cascade=capi.CASCADE_IDS[self.cascade], cfl=self.CFL, lx=self.Lx,
+            lf=self.lf, nf=nf, mord=self.Mord, ngp=self.ngp, method=self.method,
+            integrator=capi.INTEGRATOR_IDS[self.integrator], num_flux=capi.FLUX_IDS[self.numFlux],
+            bc_type=capi.BC_IDS[self.BC_type], dmp=int(self.DMP), u2=int(self.U2), u2_tol=int(self.U2_tol),
+            cascade=capi.CASCADE_IDS[self.cascade], problem=int(self.problem), cfl=self.CFL, lx=self.Lx,
             ly=self.Ly, ell_o_dx=self.ell_o_dx)
 
 
 class GpmError(RuntimeError):
     pass
+
+
+def boundary_c(config, U, t_rk=0.0):
+    """Boundary_C (BC.f90:18-413) in place on a host array U of shape
+    (nf+8, lf+8, 4) for ``config``'s grid and BC_type at stage time t_rk.
+    Host code of the library (the rule its kernels apply); needs no GPU."""
+    if not (U.flags.c_contiguous and U.dtype == np.float64):
+        raise ValueError("U must be C-contiguous float64")
+    c = config.to_c()
+    if U.shape != (c.nf + 2 * NGC, c.lf + 2 * NGC, 4):
+        raise ValueError(f"state shape {U.shape} does not match the grid")
+    rc = capi.load().gpm_boundary_c(ctypes.byref(c), float(t_rk), capi.ptr(U))
+    if rc != 0:
+        raise GpmError(f"gpm_boundary_c failed ({rc})")
+    return U
 
 
 class Solver:
@@ -61,7 +79,8 @@ class Solver:
     def __init__(self, config, tables=None):
         self.config = config
         self.lib = capi.load()
-        self.n = config.lf
+        self.n = self.lf = config.lf
+        self.nf = config.lf if config.nf is None else config.nf
         self._cfg = config.to_c()
         h = ctypes.c_void_p()
         rc = self.lib.gpm_create(ctypes.byref(self._cfg), ctypes.byref(h))
@@ -93,15 +112,20 @@ class Solver:
 
     # -- state ------------------------------------------------------------
     def full_shape(self):
-        return (self.n + 2 * NGC, self.n + 2 * NGC, 4)
+        return (self.nf + 2 * NGC, self.lf + 2 * NGC, 4)
 
     def set_state(self, U):
-        """U: full array (nf+8, lf+8, 4) or interior (nf, lf, 4), float64."""
+        """U: full array (nf+8, lf+8, 4) or interior (nf, lf, 4), float64.
+        An interior array gets its ghost cells from the boundary rule at the
+        current time (for copy boundaries the library ignores ghost cells)."""
         U = np.asarray(U, dtype=np.float64)
-        if U.shape == (self.n, self.n, 4):
-            full = np.zeros(self.full_shape())
-            full[NGC:-NGC, NGC:-NGC] = U
-            U = full
+        if U.shape == (self.nf, self.lf, 4):
+            import torch
+            dev = torch.from_numpy(np.ascontiguousarray(U)).to("cuda")
+            torch.cuda.synchronize()  # the copy runs on torch's stream, the library reads on its own
+            self.set_state_device(dev.data_ptr())
+            torch.cuda.synchronize()
+            return
         U = np.ascontiguousarray(U)
         if U.shape != self.full_shape():
             raise ValueError(f"state shape {U.shape} != {self.full_shape()}")

@@ -19,16 +19,20 @@ LIB_PATH = os.environ.get("GPM_LIB_PATH") or os.path.join(HERE, "libgpmood.so")
 
 GP_MOOD = 2
 NN_GP_MOOD = 203
-SSP_RK3 = 10
-HLLC, HLL, UNSPLIT = 100, 102, 103
-FLUX_IDS = {"hllc": HLLC, "hll": HLL, "unsplit": UNSPLIT}
-NEUMANN, PERIODIC, REFLECTIVE, KH_BC = 17, 18, 20, 403
-BC_IDS = {"neumann": NEUMANN, "periodic": PERIODIC, "reflective": REFLECTIVE, "kh": KH_BC}
+FE, SSP_RK2, SSP_RK3, SSP_RK4 = 8, 9, 10, 12
+INTEGRATOR_IDS = {"fe": FE, "ssp_rk2": SSP_RK2, "ssp_rk3": SSP_RK3, "ssp_rk4": SSP_RK4}
+HLLC, LLF, HLL, UNSPLIT = 100, 101, 102, 103
+FLUX_IDS = {"hllc": HLLC, "llf": LLF, "hll": HLL, "unsplit": UNSPLIT}
+NEUMANN, PERIODIC, DIRICHLET, REFLECTIVE, KH_BC = 17, 18, 19, 20, 403
+MACH800_BC, DOUBLEMACH800_BC, RMI_BC, DMR_BC, RT_BC = 201, 301, 401, 35, 402
+BC_IDS = {"neumann": NEUMANN, "periodic": PERIODIC, "reflective": REFLECTIVE, "kh": KH_BC, "dirichlet": DIRICHLET,
+          "mach800": MACH800_BC, "doublemach800": DOUBLEMACH800_BC, "rmi": RMI_BC, "dmr": DMR_BC, "rt": RT_BC}
+PROBLEM_RT = 51
 CASCADE_IDS = {"reference": 0, "full": 1}
 
 # Every symbol include/gpmood.h declares.
 EXPORTS = [
-    "gpm_create", "gpm_destroy", "gpm_last_error", "gpm_set_gp_tables", "gpm_gp_tables",
+    "gpm_create", "gpm_destroy", "gpm_last_error", "gpm_set_gp_tables", "gpm_gp_tables", "gpm_boundary_c",
     "gpm_set_state", "gpm_get_state", "gpm_set_state_device", "gpm_get_state_device",
     "gpm_compute_dt", "gpm_time_stepping", "gpm_run", "gpm_get_stats", "gpm_reset_stats",
     "gpm_set_time", "gpm_stream", "gpm_set_timing", "gpm_kernel_time",
@@ -45,7 +49,8 @@ class GpmConfig(ctypes.Structure):
         ("lf", ctypes.c_int32), ("nf", ctypes.c_int32), ("mord", ctypes.c_int32), ("ngp", ctypes.c_int32),
         ("method", ctypes.c_int32), ("integrator", ctypes.c_int32), ("num_flux", ctypes.c_int32),
         ("bc_type", ctypes.c_int32), ("dmp", ctypes.c_int32), ("u2", ctypes.c_int32), ("u2_tol", ctypes.c_int32),
-        ("cascade", ctypes.c_int32), ("cfl", ctypes.c_double), ("lx", ctypes.c_double), ("ly", ctypes.c_double),
+        ("cascade", ctypes.c_int32), ("problem", ctypes.c_int32), ("cfl", ctypes.c_double), ("lx", ctypes.c_double),
+        ("ly", ctypes.c_double),
         ("ell_o_dx", ctypes.c_double),
     ]
 
@@ -85,6 +90,7 @@ def load():
         "gpm_last_error": ([P], ctypes.c_char_p),
         "gpm_set_gp_tables": ([P, P, P, P, P, P], I),
         "gpm_gp_tables": ([I, I, D, D, D, I, I, P, P, P, P, P], I),
+        "gpm_boundary_c": ([ctypes.POINTER(GpmConfig), D, P], I),
         "gpm_set_state": ([P, P], I),
         "gpm_get_state": ([P, P, P], I),
         "gpm_set_state_device": ([P, P], I),

@@ -2,11 +2,13 @@
 //
 // Owns the device buffers of one solver context and enqueues the step on the
 // context's HIP stream: k_dt, then per RK stage k_stage (first MOOD pass) and
-// the cooperative k_mood (remaining passes).  Host-array entry points convert
-// between the reference's ghosted layout U(4, lb:le, nb:ne) and the device's
-// interior-only layout.
+// the cooperative k_mood (remaining passes), then, for boundaries whose ghost
+// cells are not copies of interior cells, k_ring (the ghost values the next
+// Setdt sees).  Host-array entry points convert between the reference's
+// ghosted layout U(4, lb:le, nb:ne) and the device's interior-only layout.
 #include <hip/hip_runtime.h>
 
+#include <cmath>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -23,12 +25,18 @@ using namespace gpm;
 
 struct gpm_ctx {
     gpm_config cfg{};
-    int N = 0, mord = 0, ngp = 0, bc = 0, device = 0;
+    int NX = 0, NY = 0, mord = 0, ngp = 0, bc = 0, device = 0;
+    int N = 0;  // NX where only square grids are supported (learned predictor, adjoint)
+    int integrator = 0, nstages = 0, gravity = 0;
+    bool ring_mode = false;  // ghost cells are not all copies of interior cells: keep them in `ring`
+    Bnd B{};
     double dx = 0, dy = 0;
     hipStream_t stream = nullptr;
     DevTables hT{};
     DevTables *dT = nullptr;
     double4 *dU = nullptr, *dA = nullptr, *dB = nullptr;
+    double4 *dC = nullptr, *dD = nullptr, *dE = nullptr, *dF = nullptr;  // SSP_RK4 stage states
+    double4 *ring = nullptr, *band = nullptr;
     order_t *gpo = nullptr;
     int *slistA = nullptr, *slistB = nullptr, *mark = nullptr;
     int *counters = nullptr, *stamp = nullptr, *err = nullptr, *solo_p = nullptr;
@@ -36,7 +44,7 @@ struct gpm_ctx {
     double *tstate = nullptr, *block_min = nullptr, *dt_log = nullptr;
     unsigned *done = nullptr;
     int dt_log_cap = 0;
-    int tiles_x = 0, ntiles = 0, dt_blocks = 0, mood_blocks[3] = {0, 0, 0}, mood_solo = 0;
+    int tiles_x = 0, ntiles = 0, dt_blocks = 0, mood_blocks = 0, mood_solo = 0;
     bool has_state = false;
     bool timing = false;
     double stage_ms = 0, mood_ms = 0;
@@ -93,55 +101,91 @@ hipError_t dalloc(T **p, size_t n)
     return hipMalloc(reinterpret_cast<void **>(p), n * sizeof(T) > 0 ? n * sizeof(T) : 1);
 }
 
-// Host form of the ghost-cell rule (Boundary_C, BC.f90:18-135, 383-406): the
-// interior cell a ghost index copies and whether the normal momentum flips
-// sign, for the rule of one axis (KH_BC: periodic in x, Neumann in y).
-int host_bc_map(int i, int N, int bc, bool &flip)
-{
-    flip = false;
-    if (i < 0) {
-        if (bc == GPM_NEUMANN) return 0;
-        if (bc == GPM_PERIODIC) return i + N;
-        flip = true;
-        return -1 - i;
+// Interior cells of a host state [NY][NX][4].
+struct HostSrc {
+    const double *in;
+    int NX;
+    double4 operator()(int i, int j) const
+    {
+        const double *p = in + ((size_t)j * NX + i) * 4;
+        return make_double4(p[0], p[1], p[2], p[3]);
     }
-    if (i >= N) {
-        if (bc == GPM_NEUMANN) return N - 1;
-        if (bc == GPM_PERIODIC) return i - N;
-        flip = true;
-        return 2 * N - 1 - i;
-    }
-    return i;
-}
+};
 
-int host_bc_x(int bc) { return bc == GPM_KH_BC ? GPM_PERIODIC : bc; }
-int host_bc_y(int bc) { return bc == GPM_KH_BC ? GPM_NEUMANN : bc; }
-
-// interior [N][N][4] -> ghosted [N+8][N+8][4] with the boundary rule
-void fill_ghosted(const double *in, double *out, int N, int bc)
+// interior [NY][NX][4] -> ghosted [NY+8][NX+8][4] with Boundary_C at the time
+// whose DMR switch abscissa is xmin_top (the same rule the kernels apply)
+void fill_ghosted(const double *in, double *out, const Bnd &B, double xmin_top)
 {
-    const int W = N + 8;
-    for (int jj = 0; jj < W; ++jj) {
-        bool fy;
-        const int sj = host_bc_map(jj - 4, N, host_bc_y(bc), fy);
+    const int W = B.NX + 8, H = B.NY + 8;
+    const HostSrc src{in, B.NX};
+    for (int jj = 0; jj < H; ++jj)
         for (int ii = 0; ii < W; ++ii) {
-            bool fx;
-            const int si = host_bc_map(ii - 4, N, host_bc_x(bc), fx);
-            const double *s = in + ((size_t)sj * N + si) * 4;
+            const double4 v = bnd_value(B, src, ii - 4, jj - 4, xmin_top);
             double *d = out + ((size_t)jj * W + ii) * 4;
-            d[0] = s[0];
-            d[1] = fx ? -s[1] : s[1];
-            d[2] = fy ? -s[2] : s[2];
-            d[3] = s[3];
+            d[0] = v.x; d[1] = v.y; d[2] = v.z; d[3] = v.w;
         }
+}
+
+void extract_interior(const double *full, double *in, int NX, int NY)
+{
+    const int W = NX + 8;
+    for (int j = 0; j < NY; ++j)
+        std::memcpy(in + (size_t)j * NX * 4, full + ((size_t)(j + 4) * W + 4) * 4, sizeof(double) * NX * 4);
+}
+
+// ghosted [NY+8][NX+8][4] <-> ring (ring_cell order)
+void ghosts_to_ring(const double *full, double4 *ring, int NX, int NY)
+{
+    const int W = NX + 8;
+    for (int r = 0; r < ring_size(NX, NY); ++r) {
+        int i, j;
+        ring_cell(r, NX, NY, i, j);
+        const double *p = full + ((size_t)(j + 4) * W + (i + 4)) * 4;
+        ring[r] = make_double4(p[0], p[1], p[2], p[3]);
     }
 }
 
-void extract_interior(const double *full, double *in, int N)
+void ring_to_ghosts(const double4 *ring, double *full, int NX, int NY)
 {
-    const int W = N + 8;
-    for (int j = 0; j < N; ++j)
-        std::memcpy(in + (size_t)j * N * 4, full + ((size_t)(j + 4) * W + 4) * 4, sizeof(double) * N * 4);
+    const int W = NX + 8;
+    for (int r = 0; r < ring_size(NX, NY); ++r) {
+        int i, j;
+        ring_cell(r, NX, NY, i, j);
+        double *p = full + ((size_t)(j + 4) * W + (i + 4)) * 4;
+        p[0] = ring[r].x; p[1] = ring[r].y; p[2] = ring[r].z; p[3] = ring[r].w;
+    }
+}
+
+// The boundary descriptor of a configuration: fixed ghost states of the
+// formula rules by primitive_to_conservative (BC.f90:60-381, IC.f90:336-355).
+Bnd make_bnd(int NX, int NY, int bc, double dx, double dy)
+{
+    Bnd B{};
+    B.NX = NX; B.NY = NY; B.bc = bc; B.dx = dx; B.dy = dy;
+    B.sqrt3 = std::sqrt(3.);
+    switch (bc) {
+    case kMach800:
+        B.st[0] = p2c(1.4, 0.0, 800.0, 1.0);
+        break;
+    case kDoubleMach800:
+        B.st[0] = p2c(1.4, 0.0, 800.0, 1.0);
+        B.st[1] = p2c(1.4, 0.0, -800.0, 1.0);
+        break;
+    case kDMR:
+        B.st[0] = p2c(8., 7.1447096, -4.125, 116.5);
+        B.st[1] = p2c(1.4, 0.0, 0.0, 1.0);
+        break;
+    case kRMI:
+        B.st[0] = p2c(2.666666666666667, 2. * std::sqrt(1.4), 0., 4.5);
+        break;
+    case kDirichlet:
+        B.st[0] = p2c(3.857143, 2.629369, 0., 10.33333);
+        for (int g = 1; g <= 4; ++g) B.st[g] = p2c(1. + 0.2 * std::sin(5 * (mesh_c(NX - 1 + g, dx) - 4.5)), 0., 0., 1.);
+        break;
+    default:
+        break;
+    }
+    return B;
 }
 
 int upload_tables(gpm_ctx *c, const double *zT, const double *d2x, const double *d2y, const double *gw,
@@ -201,14 +245,16 @@ int launch_dt_kernel(gpm_ctx *c, int mode, int use_tmax, double tmax, double dtf
 {
     DtArgs a{};
     a.U = c->dU;
+    a.ring = c->ring_mode ? c->ring : nullptr;
     a.block_min = c->block_min;
     a.done = c->done;
     a.tstate = c->tstate;
     a.dt_log = log_dt ? c->dt_log : nullptr;
     a.counters = c->counters;
-    a.ncounters = 3 * kCountersPerStage;
+    a.ncounters = kMaxStages * kCountersPerStage;
     a.stats = c->stats;
-    a.N = c->N;
+    a.NX = c->NX;
+    a.NY = c->NY;
     a.mode = mode;
     a.use_tmax = use_tmax;
     a.advance_t = advance_t;
@@ -224,27 +270,98 @@ int launch_dt_kernel(gpm_ctx *c, int mode, int use_tmax, double tmax, double dtf
     return GPM_OK;
 }
 
-// Arguments of k_stage for RK stage `stage` (1..3) from `in` to `out`.
-StageArgs stage_args(gpm_ctx *c, int stage, const double4 *in, double4 *out)
+// One stage of the integrator: input, output and how the output is formed.
+struct StagePlan {
+    const double4 *in;
+    double4 *out;
+    StepCtl C;
+};
+
+// The stages of one step of the configured integrator (time_stepper.f90:17-21)
+// on the context's buffers; the step's result lands in dA (swapped with dU
+// afterwards).  Returns the stage count.
+int plan_stages(gpm_ctx *c, StagePlan *P)
 {
-    int *cnt = c->counters + (stage - 1) * kCountersPerStage;
+    auto st = [&](int k, const double4 *in, double4 *out, int kind, int trk) {
+        P[k] = StagePlan{in, out, StepCtl{}};
+        P[k].C.Un = c->dU;
+        P[k].C.kind = kind;
+        P[k].C.trk = trk;
+        P[k].C.gravity = c->gravity;
+    };
+    int n = 0;
+    switch (c->integrator) {
+    case GPM_FE:  // mod_FE.f90:20-190
+        st(n++, c->dU, c->dA, kCmbCopy, 0);
+        break;
+    case GPM_SSP_RK2:  // mod_SSP_RK2.f90:12-22
+        st(n++, c->dU, c->dB, kCmbCopy, 0);
+        st(n++, c->dB, c->dA, kCmbRK2b, 0);
+        break;
+    case GPM_SSP_RK4:  // mod_SSP_RK4.f90:12-34: U1 dB, U2 dC, U3 dD, U4 dE, FU3 dF
+        st(n++, c->dU, c->dB, kCmbRK4a, 0);
+        st(n++, c->dB, c->dC, kCmbRK4b, 0);
+        st(n++, c->dC, c->dD, kCmbRK4c, 0);
+        st(n++, c->dD, c->dE, kCmbRK4d, 0);
+        P[n - 1].C.W = c->dF;
+        st(n++, c->dE, c->dA, kCmbRK4e, 0);
+        P[n - 1].C.X1 = c->dC;
+        P[n - 1].C.X2 = c->dD;
+        P[n - 1].C.X3 = c->dF;
+        break;
+    default:  // SSP_RK3, mod_SSP_RK3.f90:12-39
+        st(n++, c->dU, c->dA, kCmbCopy, 0);
+        st(n++, c->dA, c->dB, kCmbRK3b, 1);
+        st(n++, c->dB, c->dA, kCmbRK3c, 2);
+    }
+    if (c->ring_mode) P[n - 1].C.band = c->band;
+    return n;
+}
+
+// Arguments of k_stage for stage `k` (0-based) of plan P.
+StageArgs stage_args(gpm_ctx *c, int k, const StagePlan &P)
+{
+    int *cnt = c->counters + k * kCountersPerStage;
     StageArgs s{};
-    s.Uin = in; s.Un = c->dU; s.Uout = out; s.gpo = c->gpo;
+    s.Uin = P.in; s.Uout = P.out; s.C = P.C; s.B = c->B; s.gpo = c->gpo;
     s.slist1 = c->slistB; s.cnt_s1 = cnt + kMaxPasses + 1 + 1; s.cnt_f0 = cnt;
     s.mark = c->mark; s.stamp = c->stamp; s.T = c->dT; s.tstate = c->tstate;
-    s.N = c->N; s.bc = c->bc; s.dmp = c->cfg.dmp; s.u2 = c->cfg.u2; s.u2tol = c->cfg.u2_tol;
+    s.dmp = c->cfg.dmp; s.u2 = c->cfg.u2; s.u2tol = c->cfg.u2_tol;
     s.cascade = c->cfg.cascade;
     s.dx = c->dx; s.dy = c->dy; s.tiles_x = c->tiles_x; s.ntiles = c->ntiles;
     return s;
 }
 
-// One SSP-RK3 step (mod_SSP_RK3.f90:12-39) after k_dt has set dt.
+// Ghost ring of the step's output (dA) from the step-start state (dU) and the
+// last stage's band values, as the integrator's whole-array statements leave it.
+int enqueue_ring(gpm_ctx *c)
+{
+    RingArgs r{};
+    r.A = c->dU; r.band = c->band; r.ring = c->ring; r.tstate = c->tstate; r.B = c->B; r.gravity = c->gravity;
+    r.mode = c->integrator == GPM_FE ? kRingFE : c->integrator == GPM_SSP_RK2 ? kRingRK2 : kRingRK3;
+    HIPCHK(c, launch_ring(r, c->stream));
+    return GPM_OK;
+}
+
+// Ghost ring of the current state dU by Boundary_C at t = tstate[0].
+int enqueue_ring_fill(gpm_ctx *c)
+{
+    if (!c->ring_mode) return GPM_OK;
+    RingArgs r{};
+    r.A = c->dU; r.ring = c->ring; r.tstate = c->tstate; r.B = c->B; r.mode = kRingFill;
+    HIPCHK(c, launch_ring(r, c->stream));
+    return GPM_OK;
+}
+
+// One step of the configured integrator after k_dt has set dt.
 int enqueue_stages(gpm_ctx *c)
 {
-    for (int stage = 1; stage <= 3; ++stage) {
-        const double4 *in = stage == 1 ? c->dU : (stage == 2 ? c->dA : c->dB);
-        double4 *out = stage == 1 ? c->dA : (stage == 2 ? c->dB : c->dA);
-        int *cnt = c->counters + (stage - 1) * kCountersPerStage;
+    StagePlan plan[kMaxStages];
+    const int nst = plan_stages(c, plan);
+    for (int k = 0; k < nst; ++k) {
+        const int stage = k + 1;
+        const double4 *in = plan[k].in;
+        int *cnt = c->counters + k * kCountersPerStage;
         if (c->nn) {
             // a priori orders of this stage's input (compute_CellGPO_with_NN, mod_NN.f90:14-58)
             HIPCHK(c, hipMemsetAsync(c->nncount, 0, sizeof(int), c->stream));
@@ -254,8 +371,9 @@ int enqueue_stages(gpm_ctx *c)
                                          mlp_stencil(c->nn), mlp_classes(c->nn), c->mord, c->gpo, c->pred,
                                          &c->stats->a_priori_lowered, c->stream));
         }
-        if (c->fixed_orders && !c->nn) HIPCHK(c, launch_orders_from_pred(c->pred, c->gpo, (size_t)c->N * c->N, c->stream));
-        StageArgs s = stage_args(c, stage, in, out);
+        if (c->fixed_orders && !c->nn)
+            HIPCHK(c, launch_orders_from_pred(c->pred, c->gpo, (size_t)c->NX * c->NY, c->stream));
+        StageArgs s = stage_args(c, k, plan[k]);
         s.apriori = c->nn != nullptr || c->fixed_orders;
         s.pred = c->pred;
         hipEvent_t e0 = nullptr, e1 = nullptr;
@@ -267,17 +385,17 @@ int enqueue_stages(gpm_ctx *c)
             HIPCHK(c, hipEventRecord(e0, c->stream));
         }
         // a multiple of 8 workgroups, so the XCD-aware remap in k_stage is a bijection
-        HIPCHK(c, launch_stage(c->mord, c->ngp, c->cfg.num_flux, stage, s, (c->ntiles + 7) / 8 * 8, c->stream));
+        HIPCHK(c, launch_stage(c->mord, c->ngp, c->cfg.num_flux, s, (c->ntiles + 7) / 8 * 8, c->stream));
         if (c->timing) HIPCHK(c, hipEventRecord(e1, c->stream));
 
         MoodArgs m{};
-        m.Uin = in; m.Un = c->dU; m.Uout = out; m.gpo = c->gpo;
+        m.Uin = in; m.Uout = plan[k].out; m.C = plan[k].C; m.B = c->B; m.gpo = c->gpo;
         m.slist[0] = c->slistA; m.slist[1] = c->slistB;
         m.cnt_f = cnt; m.cnt_s = cnt + kMaxPasses + 1; m.mark = c->mark; m.stamp = c->stamp; m.err = c->err;
         m.solo_p = c->solo_p; m.solo_max = c->mood_solo;
         m.stats = c->stats; m.T = c->dT; m.tstate = c->tstate;
-        m.N = c->N; m.bc = c->bc; m.dmp = c->cfg.dmp; m.u2 = c->cfg.u2; m.u2tol = c->cfg.u2_tol;
-        m.cascade = c->cfg.cascade; m.pmax = kMaxPasses; m.stage_idx = stage - 1;
+        m.dmp = c->cfg.dmp; m.u2 = c->cfg.u2; m.u2tol = c->cfg.u2_tol;
+        m.cascade = c->cfg.cascade; m.pmax = kMaxPasses; m.stage_idx = k;
         m.dx = c->dx; m.dy = c->dy;
         void *args[] = {&m};
         if (c->timing) {
@@ -287,15 +405,14 @@ int enqueue_stages(gpm_ctx *c)
             c->ev_pairs.push_back({(int)i0, 1});
             HIPCHK(c, hipEventRecord(e0, c->stream));
         }
-        HIPCHK(c, hipLaunchCooperativeKernel(mood_kernel(c->mord, c->ngp, c->cfg.num_flux, stage), dim3(c->mood_blocks[stage - 1]),
+        HIPCHK(c, hipLaunchCooperativeKernel(mood_kernel(c->mord, c->ngp, c->cfg.num_flux), dim3(c->mood_blocks),
                                              dim3(kMoodThreads), args, 0, c->stream));
         if (c->timing) HIPCHK(c, hipEventRecord(e1, c->stream));
         if (c->rec_masks && c->rec_count < c->rec_cap) {
-            const size_t n = (size_t)c->N * c->N;
-            HIPCHK(c, launch_unpack_orders(c->gpo, c->rec_masks + ((size_t)c->rec_count * 3 + (stage - 1)) * n, n,
-                                           c->stream));
+            const size_t n = (size_t)c->NX * c->NY;
+            HIPCHK(c, launch_unpack_orders(c->gpo, c->rec_masks + ((size_t)c->rec_count * 3 + k) * n, n, c->stream));
         }
-        if (c->dump_x && (c->dump_stages & (1 << (stage - 1)))) {
+        if (c->dump_x && (c->dump_stages & (1 << k))) {
             // (stencil of the stage input, final order of the stage) pairs
             const int S = 2 * ((c->mord - 1) / 2) + 1;
             HIPCHK(c, mlp_launch_dump(in, c->N, c->bc, c->dx, c->dy, c->gpo, c->mord, S, (float)c->dump_keep,
@@ -303,10 +420,21 @@ int enqueue_stages(gpm_ctx *c)
                                       c->dump_gated, c->dump_x, c->dump_y, c->dump_cap, c->dump_count, c->stream));
         }
     }
+    if (c->ring_mode) {
+        int rc = enqueue_ring(c);
+        if (rc) return rc;
+    }
     std::swap(c->dU, c->dA);
     ++c->step_counter;
     if (c->rec_masks && c->rec_count < c->rec_cap) ++c->rec_count;
     return GPM_OK;
+}
+
+// The learned predictor, the training-data dump and the differentiable form
+// are built for square grids with a copy boundary, SSP_RK3 and no source term.
+bool square_copy_rk3(const gpm_ctx *c)
+{
+    return c->NX == c->NY && index_map_bc(c->bc) && !c->gravity && c->integrator == GPM_SSP_RK3;
 }
 
 int check_device_error(gpm_ctx *c)
@@ -328,33 +456,50 @@ int gpm_create(const gpm_config *cfg, gpm_ctx **out)
     *out = nullptr;
     gpm_ctx *c = new gpm_ctx();
     c->cfg = *cfg;
+    c->NX = cfg->lf;
+    c->NY = cfg->nf;
     c->N = cfg->lf;
     c->mord = cfg->mord;
     c->ngp = cfg->ngp;
     c->bc = cfg->bc_type;
+    c->integrator = cfg->integrator;
+    c->gravity = cfg->problem == GPM_PROBLEM_RT;
     auto bad = [&](int code, const char *m) {
         std::fprintf(stderr, "gpm_create: %s\n", m);
         delete c;
         return code;
     };
-    if (cfg->lf != cfg->nf || cfg->lf < 8) return bad(GPM_EINVAL, "square grids with lf = nf >= 8 only");
-    if (cfg->lx != cfg->ly) return bad(GPM_EINVAL, "square cells only (Lx = Ly)");
-    if (cfg->integrator != GPM_SSP_RK3) return bad(GPM_EUNSUP, "integrator must be SSP_RK3");
-    if (cfg->num_flux != GPM_HLLC && cfg->num_flux != GPM_HLL && cfg->num_flux != GPM_UNSPLIT)
-        return bad(GPM_EUNSUP, "numFlux must be HLLC, HLL or unsplit");
+    const int bcs[] = {GPM_NEUMANN, GPM_PERIODIC, GPM_DIRICHLET, GPM_REFLECTIVE, GPM_KH_BC, GPM_MACH800_BC,
+                       GPM_DOUBLEMACH800_BC, GPM_RMI_BC, GPM_DMR_BC, GPM_RT_BC};
+    bool bc_ok = false;
+    for (int b : bcs) bc_ok = bc_ok || cfg->bc_type == b;
+    if (cfg->lf < 8 || cfg->nf < 8) return bad(GPM_EINVAL, "grids need lf, nf >= 8");
+    if (cfg->integrator != GPM_FE && cfg->integrator != GPM_SSP_RK2 && cfg->integrator != GPM_SSP_RK3 &&
+        cfg->integrator != GPM_SSP_RK4)
+        return bad(GPM_EUNSUP, "integrator must be FE, SSP_RK2, SSP_RK3 or SSP_RK4");
+    if (cfg->num_flux != GPM_HLLC && cfg->num_flux != GPM_LLF && cfg->num_flux != GPM_HLL &&
+        cfg->num_flux != GPM_UNSPLIT)
+        return bad(GPM_EUNSUP, "numFlux must be HLLC, LLF, HLL or unsplit");
     if (cfg->method != GPM_GP_MOOD && cfg->method != GPM_NN_GP_MOOD)
         return bad(GPM_EUNSUP, "method must be GP_MOOD or NN_GP_MOOD");
-    if (cfg->bc_type != GPM_NEUMANN && cfg->bc_type != GPM_PERIODIC && cfg->bc_type != GPM_REFLECTIVE &&
-        cfg->bc_type != GPM_KH_BC)
-        return bad(GPM_EUNSUP, "BC_type must be Neumann, Periodic, Reflective or KH_BC");
+    if (!bc_ok) return bad(GPM_EUNSUP, "unknown BC_type");
+    c->ring_mode = !index_map_bc(cfg->bc_type) || c->gravity;
+    if (c->ring_mode && cfg->integrator == GPM_SSP_RK4)
+        return bad(GPM_EUNSUP, "SSP_RK4 needs a copy boundary (Neumann, Periodic, Reflective, KH_BC) and no source term");
+    const bool square_copy = cfg->lf == cfg->nf && index_map_bc(cfg->bc_type) && !c->gravity &&
+                             cfg->integrator == GPM_SSP_RK3;
+    if (cfg->method == GPM_NN_GP_MOOD && !square_copy)
+        return bad(GPM_EUNSUP, "NN_GP_MOOD needs lf = nf, a copy boundary, SSP_RK3 and no source term");
     if (cfg->cascade != GPM_CASCADE_REFERENCE && cfg->cascade != GPM_CASCADE_FULL)
         return bad(GPM_EINVAL, "unknown cascade");
     if (!supported(cfg->mord, cfg->ngp)) return bad(GPM_EUNSUP, "(Mord, ngp) combination not compiled");
-    if ((long long)cfg->lf * cfg->lf > (1ll << 31) - 1) return bad(GPM_EUNSUP, "grid too large for int32 cell ids");
+    if ((long long)cfg->lf * cfg->nf > (1ll << 31) - 1) return bad(GPM_EUNSUP, "grid too large for int32 cell ids");
     // dx = real(Lx_16/lf, 8) (global_variables.f90:19-21)
     c->dx = quad_ratio(cfg->lx, cfg->lf);
     c->dy = quad_ratio(cfg->ly, cfg->nf);
-    const size_t n = (size_t)c->N * c->N;
+    c->B = make_bnd(c->NX, c->NY, c->bc, c->dx, c->dy);
+    c->nstages = cfg->integrator == GPM_FE ? 1 : cfg->integrator == GPM_SSP_RK2 ? 2 : cfg->integrator == GPM_SSP_RK4 ? 5 : 3;
+    const size_t n = (size_t)c->NX * c->NY;
     hipError_t e;
     if ((e = hipGetDevice(&c->device)) != hipSuccess) return bad(GPM_EHIP, hipGetErrorString(e));
     if ((e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking)) != hipSuccess)
@@ -363,11 +508,16 @@ int gpm_create(const gpm_config *cfg, gpm_ctx **out)
               dalloc(&c->dB, n) == hipSuccess && dalloc(&c->gpo, n) == hipSuccess &&
               dalloc(&c->slistA, n) == hipSuccess && dalloc(&c->slistB, n) == hipSuccess &&
               dalloc(&c->mark, n) == hipSuccess &&
-              dalloc(&c->counters, 3 * kCountersPerStage) == hipSuccess && dalloc(&c->stamp, 1) == hipSuccess &&
+              dalloc(&c->counters, kMaxStages * kCountersPerStage) == hipSuccess && dalloc(&c->stamp, 1) == hipSuccess &&
               dalloc(&c->err, 1) == hipSuccess && dalloc(&c->solo_p, 1) == hipSuccess && dalloc(&c->stats, 1) == hipSuccess &&
-              dalloc(&c->tstate, 4) == hipSuccess && dalloc(&c->done, 1) == hipSuccess &&
+              dalloc(&c->tstate, 8) == hipSuccess && dalloc(&c->done, 1) == hipSuccess &&
               dalloc(&c->pred, n) == hipSuccess && dalloc(&c->nnlist, n) == hipSuccess &&
               dalloc(&c->nncount, 1) == hipSuccess;
+    if (ok && c->integrator == GPM_SSP_RK4)
+        ok = dalloc(&c->dC, n) == hipSuccess && dalloc(&c->dD, n) == hipSuccess && dalloc(&c->dE, n) == hipSuccess &&
+             dalloc(&c->dF, n) == hipSuccess;
+    if (ok && c->ring_mode)
+        ok = dalloc(&c->ring, (size_t)ring_size(c->NX, c->NY)) == hipSuccess && dalloc(&c->band, n) == hipSuccess;
     if (!ok) {
         gpm_destroy(c);
         std::fprintf(stderr, "gpm_create: device allocation failed\n");
@@ -382,18 +532,18 @@ int gpm_create(const gpm_config *cfg, gpm_ctx **out)
         gpm_destroy(c);
         return GPM_ENOMEM;
     }
-    c->tiles_x = (c->N + kTX - 1) / kTX;
-    c->ntiles = c->tiles_x * ((c->N + kTY - 1) / kTY);
+    c->tiles_x = (c->NX + kTX - 1) / kTX;
+    c->ntiles = c->tiles_x * ((c->NY + kTY - 1) / kTY);
     // Cooperative grid of the MOOD kernel: all its workgroups must be resident.
     // The correction work is sparse and the per-pass grid barrier gets cheaper
     // with fewer workgroups: by default one per four CUs.  GPM_MOOD_BLOCKS overrides the count.
     int want = ncu / 4 > 0 ? ncu / 4 : 1;
     if (const char *env = std::getenv("GPM_MOOD_BLOCKS")) want = std::atoi(env) > 0 ? std::atoi(env) : ncu;
-    for (int st = 0; st < 3; ++st) {
+    {
         int per = 0;
-        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, mood_kernel(c->mord, c->ngp, c->cfg.num_flux, st + 1), kMoodThreads, 0);
+        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, mood_kernel(c->mord, c->ngp, c->cfg.num_flux), kMoodThreads, 0);
         if (e != hipSuccess || per <= 0) per = 1;
-        c->mood_blocks[st] = want < per * ncu ? want : per * ncu;
+        c->mood_blocks = want < per * ncu ? want : per * ncu;
     }
     // Passes over at most this many cells run on one workgroup (GPM_MOOD_SOLO overrides; 0 disables).
     c->mood_solo = 16;
@@ -401,9 +551,9 @@ int gpm_create(const gpm_config *cfg, gpm_ctx **out)
     if (c->mood_solo > kSoloMax) c->mood_solo = kSoloMax;
     hipMemsetAsync(c->mark, 0, n * sizeof(int), c->stream);
     hipMemsetD16Async(c->gpo, pack_order(c->mord, c->mord, 0), n, c->stream);
-    hipMemsetAsync(c->counters, 0, 3 * kCountersPerStage * sizeof(int), c->stream);
+    hipMemsetAsync(c->counters, 0, kMaxStages * kCountersPerStage * sizeof(int), c->stream);
     hipMemsetAsync(c->stats, 0, sizeof(DevStats), c->stream);
-    hipMemsetAsync(c->tstate, 0, 4 * sizeof(double), c->stream);
+    hipMemsetAsync(c->tstate, 0, 8 * sizeof(double), c->stream);
     hipMemsetAsync(c->done, 0, sizeof(unsigned), c->stream);
     hipMemsetAsync(c->err, 0, sizeof(int), c->stream);
     int one = 1;
@@ -436,7 +586,8 @@ int gpm_destroy(gpm_ctx *c)
 {
     if (!c) return GPM_OK;
     if (c->stream) hipStreamSynchronize(c->stream);
-    void *bufs[] = {c->dT, c->dU, c->dA, c->dB, c->gpo, c->slistA, c->slistB, c->mark, c->counters,
+    void *bufs[] = {c->dT, c->dU, c->dA, c->dB, c->dC, c->dD, c->dE, c->dF, c->ring, c->band,
+                    c->gpo, c->slistA, c->slistB, c->mark, c->counters,
                     c->stamp, c->err, c->solo_p, c->stats, c->tstate, c->block_min, c->dt_log, c->done, c->pred, c->nnlist,
                     c->nncount, c->adj_ub, c->adj_G, c->adj_A, c->adj_tmp, c->adj_ordx, c->adj_ordy};
     for (void *b : bufs)
@@ -463,13 +614,32 @@ int gpm_gp_tables(int32_t mord, int32_t ngp, double ell_o_dx, double lx, double 
     return gp_tables(mord, ngp, ell_o_dx, lx, ly, lf, nf, zT, d2x, d2y, gw, ixiy) == 0 ? GPM_OK : GPM_EINVAL;
 }
 
+int gpm_boundary_c(const gpm_config *cfg, double t_rk, double *U)
+{
+    if (!cfg || !U || cfg->lf < 8 || cfg->nf < 8) return GPM_EINVAL;
+    const double dx = quad_ratio(cfg->lx, cfg->lf), dy = quad_ratio(cfg->ly, cfg->nf);
+    const Bnd B = make_bnd(cfg->lf, cfg->nf, cfg->bc_type, dx, dy);
+    std::vector<double> in((size_t)cfg->lf * cfg->nf * 4);
+    extract_interior(U, in.data(), cfg->lf, cfg->nf);
+    fill_ghosted(in.data(), U, B, dmr_xmin_top(t_rk, B.sqrt3));
+    return GPM_OK;
+}
+
 int gpm_set_state(gpm_ctx *c, const double *U)
 {
     if (!c || !U) return fail(c, GPM_EINVAL, "null argument");
-    std::vector<double> in((size_t)c->N * c->N * 4);
-    extract_interior(U, in.data(), c->N);
+    const size_t n = (size_t)c->NX * c->NY;
+    std::vector<double> in(n * 4);
+    extract_interior(U, in.data(), c->NX, c->NY);
     HIPCHK(c, hipMemcpyAsync(c->dU, in.data(), in.size() * sizeof(double), hipMemcpyHostToDevice, c->stream));
-    HIPCHK(c, hipMemsetD16Async(c->gpo, pack_order(c->mord, c->mord, 0), (size_t)c->N * c->N, c->stream));
+    HIPCHK(c, hipMemsetD16Async(c->gpo, pack_order(c->mord, c->mord, 0), n, c->stream));
+    std::vector<double4> ring;
+    if (c->ring_mode) {
+        // the caller's ghost cells are what Setdt sees before the first step
+        ring.resize(ring_size(c->NX, c->NY));
+        ghosts_to_ring(U, ring.data(), c->NX, c->NY);
+        HIPCHK(c, hipMemcpyAsync(c->ring, ring.data(), ring.size() * sizeof(double4), hipMemcpyHostToDevice, c->stream));
+    }
     HIPCHK(c, hipStreamSynchronize(c->stream));
     c->has_state = true;
     return GPM_OK;
@@ -479,20 +649,30 @@ int gpm_get_state(gpm_ctx *c, double *U, int32_t *cellgpo)
 {
     if (!c || !U) return fail(c, GPM_EINVAL, "null argument");
     if (!c->has_state) return fail(c, GPM_ESTATE, "no state set");
-    const size_t n = (size_t)c->N * c->N;
+    const size_t n = (size_t)c->NX * c->NY;
     std::vector<double> in(n * 4);
     std::vector<order_t> g(n);
+    std::vector<double4> ring(c->ring_mode ? ring_size(c->NX, c->NY) : 0);
     HIPCHK(c, hipMemcpyAsync(in.data(), c->dU, n * 4 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipMemcpyAsync(g.data(), c->gpo, n * sizeof(order_t), hipMemcpyDeviceToHost, c->stream));
+    if (c->ring_mode)
+        HIPCHK(c, hipMemcpyAsync(ring.data(), c->ring, ring.size() * sizeof(double4), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
-    fill_ghosted(in.data(), U, c->N, c->bc);
+    const int W = c->NX + 8, H = c->NY + 8;
+    if (c->ring_mode) {
+        for (int j = 0; j < c->NY; ++j)
+            std::memcpy(U + ((size_t)(j + 4) * W + 4) * 4, in.data() + (size_t)j * c->NX * 4, sizeof(double) * c->NX * 4);
+        ring_to_ghosts(ring.data(), U, c->NX, c->NY);
+    } else {
+        fill_ghosted(in.data(), U, c->B, 0.0);
+    }
     if (cellgpo) {
-        const int W = c->N + 8;
-        for (int jj = 0; jj < W; ++jj)
+        for (int jj = 0; jj < H; ++jj)
             for (int ii = 0; ii < W; ++ii) {
                 const int i = ii - 4, j = jj - 4;
-                cellgpo[(size_t)jj * W + ii] =
-                    (i < 0 || i >= c->N || j < 0 || j >= c->N) ? c->mord : (int32_t)order_of_word(g[(size_t)j * c->N + i]);
+                cellgpo[(size_t)jj * W + ii] = (i < 0 || i >= c->NX || j < 0 || j >= c->NY)
+                                                   ? c->mord
+                                                   : (int32_t)order_of_word(g[(size_t)j * c->NX + i]);
             }
     }
     return GPM_OK;
@@ -501,9 +681,11 @@ int gpm_get_state(gpm_ctx *c, double *U, int32_t *cellgpo)
 int gpm_set_state_device(gpm_ctx *c, const double *u_dev)
 {
     if (!c || !u_dev) return fail(c, GPM_EINVAL, "null argument");
-    const size_t n = (size_t)c->N * c->N;
+    const size_t n = (size_t)c->NX * c->NY;
     HIPCHK(c, hipMemcpyAsync(c->dU, u_dev, n * 4 * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
     HIPCHK(c, hipMemsetD16Async(c->gpo, pack_order(c->mord, c->mord, 0), n, c->stream));
+    int rc = enqueue_ring_fill(c);  // ghost cells by Boundary_C at the current time
+    if (rc) return rc;
     c->has_state = true;
     return GPM_OK;
 }
@@ -512,7 +694,7 @@ int gpm_get_state_device(gpm_ctx *c, double *u_dev, int8_t *order_dev)
 {
     if (!c) return GPM_EINVAL;
     if (!c->has_state) return fail(c, GPM_ESTATE, "no state set");
-    const size_t n = (size_t)c->N * c->N;
+    const size_t n = (size_t)c->NX * c->NY;
     if (u_dev) HIPCHK(c, hipMemcpyAsync(u_dev, c->dU, n * 4 * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
     if (order_dev) HIPCHK(c, launch_unpack_orders(c->gpo, order_dev, n, c->stream));
     return GPM_OK;
@@ -534,7 +716,7 @@ int gpm_compute_dt(gpm_ctx *c, double dtfinal, double *dt)
 int gpm_set_time(gpm_ctx *c, double t)
 {
     if (!c) return GPM_EINVAL;
-    double ts[4] = {t, 0, 0, 0};
+    double ts[8] = {t, 0, 0, 0, t, 0, 0, 0};
     HIPCHK(c, hipMemcpyAsync(c->tstate, ts, sizeof(ts), hipMemcpyHostToDevice, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
     return GPM_OK;
@@ -546,11 +728,12 @@ int gpm_time_stepping(gpm_ctx *c, double *Uin, double *Uout, double t, double dt
     if (!c || !Uin || !Uout) return fail(c, GPM_EINVAL, "null argument");
     int rc = gpm_set_state(c, Uin);
     if (rc) return rc;
-    // Boundary_C(Uin) of Forward_Euler (mod_FE.f90:64) refills Uin's ghosts.
+    // Boundary_C(Uin) of the first Forward_Euler stage (mod_FE.f90:64, t_rk = t)
+    // refills Uin's ghosts.
     {
-        std::vector<double> in((size_t)c->N * c->N * 4);
-        extract_interior(Uin, in.data(), c->N);
-        fill_ghosted(in.data(), Uin, c->N, c->bc);
+        std::vector<double> in((size_t)c->NX * c->NY * 4);
+        extract_interior(Uin, in.data(), c->NX, c->NY);
+        fill_ghosted(in.data(), Uin, c->B, dmr_xmin_top(t, c->B.sqrt3));
     }
     rc = gpm_set_time(c, t);
     if (rc) return rc;
@@ -662,6 +845,8 @@ int gpm_kernel_time(gpm_ctx *c, double *avg_stage_ms, int64_t *stage_launches, d
 int gpm_set_predictor(gpm_ctx *c, gpm_mlp *m, int32_t gate)
 {
     if (!c) return GPM_EINVAL;
+    if (m && !square_copy_rk3(c))
+        return fail(c, GPM_EUNSUP, "the predictor needs lf = nf, a copy boundary, SSP_RK3 and no source term");
     if (m) {
         const int R = (c->mord - 1) / 2;
         if (mlp_stencil(m) != 2 * R + 1 || mlp_classes(m) != (c->mord + 1) / 2)
@@ -677,6 +862,8 @@ int gpm_set_dump(gpm_ctx *c, float *x_dev, int32_t *y_dev, int64_t capacity, uin
 {
     if (!c) return GPM_EINVAL;
     if (x_dev && (!y_dev || !count_dev || capacity < 0)) return fail(c, GPM_EINVAL, "dump buffers incomplete");
+    if (x_dev && !square_copy_rk3(c))
+        return fail(c, GPM_EUNSUP, "the dump needs lf = nf, a copy boundary, SSP_RK3 and no source term");
     const int R = (c->mord - 1) / 2;
     if (x_dev && R != 2 && R != 3) return fail(c, GPM_EUNSUP, "dump needs Mord 5 or 7");
     c->dump_x = x_dev;
@@ -694,6 +881,7 @@ int gpm_features(gpm_ctx *c, const int32_t *cells_dev, int32_t n, float *x_dev)
 {
     if (!c || !cells_dev || !x_dev || n < 0) return GPM_EINVAL;
     if (!c->has_state) return fail(c, GPM_ESTATE, "no state set");
+    if (!square_copy_rk3(c)) return fail(c, GPM_EUNSUP, "features need lf = nf and a copy boundary");
     const int R = (c->mord - 1) / 2;
     HIPCHK(c, mlp_launch_features(c->dU, c->N, c->bc, 2 * R + 1, cells_dev, n, x_dev, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
@@ -703,7 +891,7 @@ int gpm_features(gpm_ctx *c, const int32_t *cells_dev, int32_t n, float *x_dev)
 int gpm_get_apriori_orders(gpm_ctx *c, int8_t *pred_dev)
 {
     if (!c || !pred_dev) return GPM_EINVAL;
-    HIPCHK(c, hipMemcpyAsync(pred_dev, c->pred, (size_t)c->N * c->N, hipMemcpyDeviceToDevice, c->stream));
+    HIPCHK(c, hipMemcpyAsync(pred_dev, c->pred, (size_t)c->NX * c->NY, hipMemcpyDeviceToDevice, c->stream));
     return GPM_OK;
 }
 
@@ -714,7 +902,7 @@ int gpm_set_apriori_orders(gpm_ctx *c, const int8_t *orders)
         c->fixed_orders = false;
         return GPM_OK;
     }
-    const size_t n = (size_t)c->N * c->N;
+    const size_t n = (size_t)c->NX * c->NY;
     for (size_t i = 0; i < n; ++i)
         if (orders[i] < 1 || orders[i] > c->mord || !(orders[i] & 1))
             return fail(c, GPM_EINVAL, "a priori orders must be odd and in [1, Mord]");
@@ -734,6 +922,8 @@ int gpm_record_masks(gpm_ctx *c, int8_t *masks_dev, int32_t max_steps)
         c->rec_cap = 0;
         return GPM_OK;
     }
+    if (!square_copy_rk3(c))
+        return fail(c, GPM_EUNSUP, "mask recording needs lf = nf, a copy boundary, SSP_RK3 and no source term");
     c->rec_masks = masks_dev;
     c->rec_cap = max_steps;
     c->rec_count = 0;
@@ -751,21 +941,23 @@ int gpm_frozen_run(gpm_ctx *c, const int8_t *masks_dev, const double *dts, int32
 {
     if (!c || !masks_dev || !dts || nsteps < 0) return fail(c, GPM_EINVAL, "invalid argument");
     if (!c->has_state) return fail(c, GPM_ESTATE, "no state set");
-    const size_t n = (size_t)c->N * c->N;
+    if (!square_copy_rk3(c))
+        return fail(c, GPM_EUNSUP, "the frozen run needs lf = nf, a copy boundary, SSP_RK3 and no source term");
+    const size_t n = (size_t)c->NX * c->NY;
     for (int st = 0; st < nsteps; ++st) {
         int rc = launch_dt_kernel(c, kDtGiven, 0, 0.0, 0.0, dts[st], 1, 0, false, 1);
         if (rc) return rc;
-        for (int stage = 1; stage <= 3; ++stage) {
-            const double4 *in = stage == 1 ? c->dU : (stage == 2 ? c->dA : c->dB);
-            double4 *out = stage == 1 ? c->dA : (stage == 2 ? c->dB : c->dA);
+        StagePlan plan[kMaxStages];
+        const int nst = plan_stages(c, plan);
+        for (int k = 0; k < nst; ++k) {
             if (stage_states_dev)
-                HIPCHK(c, hipMemcpyAsync(stage_states_dev + ((size_t)st * 3 + (stage - 1)) * n * 4, in,
+                HIPCHK(c, hipMemcpyAsync(stage_states_dev + ((size_t)st * 3 + k) * n * 4, plan[k].in,
                                          n * sizeof(double4), hipMemcpyDeviceToDevice, c->stream));
-            StageArgs s = stage_args(c, stage, in, out);
+            StageArgs s = stage_args(c, k, plan[k]);
             s.apriori = 1;
             s.frozen = 1;
-            s.pred = masks_dev + ((size_t)st * 3 + (stage - 1)) * n;
-            HIPCHK(c, launch_stage(c->mord, c->ngp, c->cfg.num_flux, stage, s, (c->ntiles + 7) / 8 * 8, c->stream));
+            s.pred = masks_dev + ((size_t)st * 3 + k) * n;
+            HIPCHK(c, launch_stage(c->mord, c->ngp, c->cfg.num_flux, s, (c->ntiles + 7) / 8 * 8, c->stream));
         }
         std::swap(c->dU, c->dA);
     }
@@ -779,6 +971,8 @@ int gpm_adjoint(gpm_ctx *c, const double *stage_states_dev, const int8_t *masks_
     if (!c || !stage_states_dev || !masks_dev || !dts || !ubar_dev || nsteps < 0)
         return fail(c, GPM_EINVAL, "invalid argument");
     if (!adj_supported(c->mord, c->ngp)) return fail(c, GPM_EUNSUP, "adjoint needs (Mord, ngp) in {(3,2), (5,3), (7,4)}");
+    if (!square_copy_rk3(c))
+        return fail(c, GPM_EUNSUP, "the adjoint needs lf = nf, a copy boundary, SSP_RK3 and no source term");
     if (c->cfg.num_flux != GPM_HLLC) return fail(c, GPM_EUNSUP, "adjoint needs numFlux = HLLC");
     const int N = c->N, R = (c->mord - 1) / 2;
     const size_t n = (size_t)N * N, W = (size_t)(N + 2), E = (size_t)(N + 2 * R + 2);

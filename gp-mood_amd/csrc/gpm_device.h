@@ -11,10 +11,18 @@ namespace gpm {
 
 constexpr double kGamma = 1.4;  // constants.f90:9 (y = 1.4)
 
-// Boundary-condition ids (constants.f90:25-26).
+// Boundary-condition ids (constants.f90:24-25, 29; DMR is the problem id the
+// reference also uses as BC_type).
 enum { kNeumann = 17, kPeriodic = 18, kReflective = 20, kKH = 403 };
+enum { kDirichlet = 19, kMach800 = 201, kDoubleMach800 = 301, kRMI = 401, kDMR = 35, kRT = 402 };
 // Flux ids (constants.f90:21) and the template tags of the Riemann solvers.
-enum { kHLLC = 100, kHLL = 102, kUnsplit = 103 };
+enum { kHLLC = 100, kLLF = 101, kHLL = 102, kUnsplit = 103 };
+// Integrator ids (constants.f90:19) and the problem id with a source term.
+enum { kFE = 8, kSSPRK2 = 9, kSSPRK3 = 10, kSSPRK4 = 12 };
+enum { kProblemRT = 51 };
+
+// g of global_variables.f90:129 is real(4); it enters FP64 expressions widened.
+constexpr double kGravity = (double)0.1f;
 // Face ids (constants.f90:37): iL, iT, iR, iB.
 enum { kL = 0, kT = 1, kR = 2, kB = 3 };
 
@@ -40,7 +48,7 @@ __device__ __forceinline__ double fmin2(double a, double b) { return (a < b) ? a
 __device__ __forceinline__ double fmax2(double a, double b) { return (a > b) ? a : b; }
 
 // pressure, physics.f90:132-143
-__device__ __forceinline__ double pressure(double r, double mx, double my, double E)
+__host__ __device__ __forceinline__ double pressure(double r, double mx, double my, double E)
 {
     return (kGamma - 1.0) * (E - (0.5 * (mx * mx + my * my)) / r);
 }
@@ -86,6 +94,139 @@ __device__ __forceinline__ double4 load_bc(const double4 *__restrict__ U, int N,
     if (fx) v.y = -v.y;
     if (fy) v.z = -v.z;
     return v;
+}
+
+// primitive_to_conservative, physics.f90:146-160
+__host__ __device__ __forceinline__ double4 p2c(double r, double vx, double vy, double p)
+{
+    return make_double4(r, r * vx, r * vy, (p / (kGamma - 1.)) + 0.5 * r * (vx * vx + vy * vy));
+}
+
+// Cell centre of 0-based cell i: mesh_x(l) = l*dx - dx/2 with l = i+1 (mesh_init.f90:14-20).
+__host__ __device__ __forceinline__ double mesh_c(int i, double d) { return (double)(i + 1) * d - d / 2; }
+
+// The boundary of a stage input on an NX x NY grid (lf x nf): the BC_type and
+// the fixed ghost states of the formula rules, computed on the host with
+// primitive_to_conservative (BC.f90:60-381):
+//   Mach800_BC        st[0] jet (1.4, 0, 800, 1)
+//   DoubleMach800_BC  st[0] bottom jet (1.4, 0, 800, 1), st[1] top jet (1.4, 0, -800, 1)
+//   DMR               st[0] post-shock (8, 7.1447096, -4.125, 116.5), st[1] pre-shock (1.4, 0, 0, 1)
+//   RMI_BC            st[0] inflow (2.666666666666667, 2 sqrt(1.4), 0, 4.5)
+//   Dirichlet         st[0] left (3.857143, 2.629369, 0, 10.33333), st[g] right ghost column g = 1..4
+struct Bnd {
+    int NX, NY, bc;
+    double dx, dy, sqrt3;
+    double4 st[5];
+};
+
+__host__ __device__ __forceinline__ bool index_map_bc(int bc)
+{
+    return bc == kNeumann || bc == kPeriodic || bc == kReflective || bc == kKH;
+}
+
+// Abscissa where the DMR top boundary switches from the post- to the
+// pre-shock state at time t_rk (BC.f90:301).
+__host__ __device__ __forceinline__ double dmr_xmin_top(double t_rk, double sqrt3)
+{
+    return 1. / 6 + 10. * t_rk / (0.5 * sqrt3) + 1. / sqrt3;
+}
+
+// Ghost column i (i < 0 or i >= NX) of interior row j: the x part of
+// Boundary_C, which the reference applies to every row before the y part.
+// src(i, j) returns interior cell (i, j).
+template <class Src>
+__host__ __device__ __forceinline__ double4 ghost_col(const Bnd &B, const Src &src, int i, int j)
+{
+    const int NX = B.NX;
+    const bool right = i >= NX;
+    const int g = right ? i - NX + 1 : -i;  // ghost layer 1..4
+    double4 v;
+    switch (B.bc) {
+    case kPeriodic: case kKH:
+        return src(right ? g - 1 : NX - g, j);
+    case kReflective: case kRT:
+        v = src(right ? NX - g : g - 1, j);
+        v.y = -v.y;
+        return v;
+    case kDirichlet:
+        return right ? B.st[g] : B.st[0];
+    case kDMR: {  // f_DMR, IC.f90:336-355
+        const double x = mesh_c(i, B.dx), y = mesh_c(j, B.dy);
+        return y > (x - 1. / 6) * B.sqrt3 ? B.st[0] : B.st[1];
+    }
+    case kRMI:
+        return right ? src(NX - 1, j) : B.st[0];
+    default:  // Neumann, Mach800_BC, DoubleMach800_BC: outflow
+        return src(right ? NX - 1 : 0, j);
+    }
+}
+
+// Value of cell (i, j), i in [-4, NX+3], j in [-4, NY+3], of a stage input
+// after Boundary_C (BC.f90:18-413): interior cells as they are; ghost columns
+// by ghost_col; ghost rows (corners included) by the y rule applied to the
+// column's values, as the reference's y loop runs over the ghost columns too.
+// Indices further out (reached only by partial tiles, whose values are never
+// used) are clamped first.  xmin_top: dmr_xmin_top at the stage's t_rk.
+template <class Src>
+__host__ __device__ __forceinline__ double4 bnd_value(const Bnd &B, const Src &src, int i, int j, double xmin_top)
+{
+    const int NX = B.NX, NY = B.NY;
+    i = i < -4 ? -4 : (i > NX + 3 ? NX + 3 : i);
+    j = j < -4 ? -4 : (j > NY + 3 ? NY + 3 : j);
+    const bool inx = i >= 0 && i < NX;
+    if (j >= 0 && j < NY) return inx ? src(i, j) : ghost_col(B, src, i, j);
+    const bool top = j >= NY;
+    const int g = top ? j - NY + 1 : -j;  // ghost layer 1..4
+    auto col = [&](int jj) { return inx ? src(i, jj) : ghost_col(B, src, i, jj); };
+    double4 v;
+    switch (B.bc) {
+    case kPeriodic:
+        return col(top ? g - 1 : NY - g);
+    case kReflective: case kRMI:
+        v = col(top ? NY - g : g - 1);
+        v.z = -v.z;
+        return v;
+    case kMach800:
+        if (!top && fabs(mesh_c(i, B.dx) - 0.75) <= 0.05) return B.st[0];
+        return col(top ? NY - 1 : 0);
+    case kDoubleMach800:
+        if (fabs(mesh_c(i, B.dx) - 0.75) <= 0.05) return top ? B.st[1] : B.st[0];
+        return col(top ? NY - 1 : 0);
+    case kDMR: {
+        const double x = mesh_c(i, B.dx);
+        if (top) return x < xmin_top ? B.st[0] : B.st[1];
+        if (x > 1. / 6) {
+            v = col(g - 1);
+            v.z = -v.z;
+            return v;
+        }
+        return B.st[0];
+    }
+    case kRT: {  // hydrostatic reflection in primitive variables
+        const int k = g - 1;
+        const double4 u = col(top ? NY - 1 - k : k);
+        const double vx = u.y / u.x, vy = u.z / u.x, p = pressure(u.x, u.y, u.z, u.w);
+        const double h = (double)(2 * k) * B.dy * kGravity * u.x;
+        return p2c(u.x, vx, -vy, top ? p - h : p + h);
+    }
+    default:  // Neumann, Dirichlet, KH_BC: outflow
+        return col(top ? NY - 1 : 0);
+    }
+}
+
+// Interior cells of a device state (row-major, NX per row).
+struct DevSrc {
+    const double4 *__restrict__ U;
+    int NX;
+    __device__ __forceinline__ double4 operator()(int i, int j) const { return U[(size_t)j * NX + i]; }
+};
+
+// A stage input's cell (i, j) with the boundary rule applied.
+__device__ __forceinline__ double4 load_bnd(const double4 *__restrict__ U, const Bnd &B, int i, int j,
+                                           double xmin_top)
+{
+    if (i >= 0 && i < B.NX && j >= 0 && j < B.NY) return U[(size_t)j * B.NX + i];
+    return bnd_value(B, DevSrc{U, B.NX}, i, j, xmin_top);
 }
 
 // HLLC flux (mod_HLLC.f90:11-169) with c2p_local and Flux (physics.f90:9-75).
@@ -197,6 +338,36 @@ __device__ __forceinline__ bool hll_adm(const double ul[4], const double ur[4], 
     return true;
 }
 
+// Local Lax-Friedrichs flux (mod_LLF.f90:11-93); false when either state is
+// inadmissible (the flux is then that of hllc_inadmissible, mod_LLF.f90:89).
+template <int DIR>
+__device__ __forceinline__ bool llf_adm(const double ul[4], const double ur[4], double F[4])
+{
+    const double vxl = ul[1] / ul[0], vyl = ul[2] / ul[0];
+    const double pl = pressure(ul[0], ul[1], ul[2], ul[3]);
+    const double cl = sqrt(kGamma * pl / ul[0]);
+    const double vxr = ur[1] / ur[0], vyr = ur[2] / ur[0];
+    const double pr = pressure(ur[0], ur[1], ur[2], ur[3]);
+    const double cr = sqrt(kGamma * pr / ur[0]);
+    const bool adm = !(ul[0] <= 0. || pl <= 0. || isnan(ul[0]) || isnan(pl) ||
+                       ur[0] <= 0. || pr <= 0. || isnan(ur[0]) || isnan(pr));
+    if (!adm) return false;
+    const double vnl = DIR == 0 ? vxl : vyl, vnr = DIR == 0 ? vxr : vyr;
+    double Fl[4], Fr[4];
+    if (DIR == 0) {
+        Fl[0] = ul[1]; Fl[1] = ul[0] * (vxl * vxl) + pl; Fl[2] = ul[2] * vxl; Fl[3] = vxl * (ul[3] + pl);
+        Fr[0] = ur[1]; Fr[1] = ur[0] * (vxr * vxr) + pr; Fr[2] = ur[2] * vxr; Fr[3] = vxr * (ur[3] + pr);
+    } else {
+        Fl[0] = ul[2]; Fl[1] = ul[1] * vyl; Fl[2] = ul[0] * (vyl * vyl) + pl; Fl[3] = vyl * (ul[3] + pl);
+        Fr[0] = ur[2]; Fr[1] = ur[1] * vyr; Fr[2] = ur[0] * (vyr * vyr) + pr; Fr[3] = vyr * (ur[3] + pr);
+    }
+    const double vmcl = vnl - cl, vpcl = vnl + cl, vmcr = vnr - cr, vpcr = vnr + cr;
+    const double smax = fmax2(fmax2(fmax2(fabs(vmcl), fabs(vmcr)), fabs(vpcl)), fabs(vpcr));
+#pragma unroll
+    for (int k = 0; k < 4; ++k) F[k] = 0.5 * (Fr[k] + Fl[k]) - 0.5 * smax * (ur[k] - ul[k]);
+    return true;
+}
+
 // Pressure-velocity ("unsplit") flux (mod_unsplit.f90:11-71); false when
 // either state is inadmissible.
 template <int DIR>
@@ -245,6 +416,8 @@ __device__ __forceinline__ void riemann(const double ul[4], const double ur[4], 
 {
     if (FLUX == kHLL) {
         if (!hll_adm<DIR>(ul, ur, F)) hllc_inadmissible(ur, F);
+    } else if (FLUX == kLLF) {
+        if (!llf_adm<DIR>(ul, ur, F)) hllc_inadmissible(ur, F);
     } else if (FLUX == kUnsplit) {
         if (!unsplit_adm<DIR>(ul, ur, F)) unsplit_inadmissible(ur, F);
     } else {
@@ -253,7 +426,7 @@ __device__ __forceinline__ void riemann(const double ul[4], const double ur[4], 
 }
 
 // Run-time direction: a y-face solve is the x-solve on momentum-swapped states
-// with the flux's momenta swapped back (exact for all three solvers: their
+// with the flux's momenta swapped back (exact for all four solvers: their
 // y-direction formulas are the x-direction ones with the roles of the
 // momenta exchanged, operation for operation).
 template <int FLUX>
@@ -265,8 +438,10 @@ __device__ __forceinline__ void riemann_dir(int dir, const double ul[4], const d
     }
     const double sl[4] = {ul[0], ul[2], ul[1], ul[3]}, sr[4] = {ur[0], ur[2], ur[1], ur[3]};
     double G[4];
-    const bool ok = FLUX == kHLL ? hll_adm<0>(sl, sr, G) : FLUX == kUnsplit ? unsplit_adm<0>(sl, sr, G)
-                                                                          : hllc_adm<0>(sl, sr, G);
+    const bool ok = FLUX == kHLL       ? hll_adm<0>(sl, sr, G)
+                    : FLUX == kLLF     ? llf_adm<0>(sl, sr, G)
+                    : FLUX == kUnsplit ? unsplit_adm<0>(sl, sr, G)
+                                       : hllc_adm<0>(sl, sr, G);
     if (ok) {
         F[0] = G[0]; F[1] = G[2]; F[2] = G[1]; F[3] = G[3];
     } else if (FLUX == kUnsplit) {

@@ -22,11 +22,21 @@ __device__ __forceinline__ double block_min(double m, double *red)
     return m;
 }
 
-// Setdt: per-cell 1/(1/(dx/(|vx|+c)) + 1/(dy/(|vy|+c))), grid minimum, times
-// CFL, clipped to dtfinal (physics.f90:98-124).  The last workgroup to finish
-// reduces the per-workgroup minima, advances t and zeroes the step counters.
-// Few large workgroups: every workgroup ends with a fence and an atomic on one
-// counter, which serialise across the device.
+__device__ __forceinline__ double dt_of_cell(const double4 &u, double dx, double dy)
+{
+    const double vx = u.y / u.x, vy = u.z / u.x;
+    const double pres = pressure(u.x, u.y, u.z, u.w);
+    const double cs = sqrt(kGamma * pres / u.x);
+    return 1.0 / (1.0 / (dx / (fabs(vx) + cs)) + 1.0 / (dy / (fabs(vy) + cs)));
+}
+
+// Setdt: per-cell 1/(1/(dx/(|vx|+c)) + 1/(dy/(|vy|+c))), minimum over the
+// grid and its ghost cells, times CFL, clipped to dtfinal (physics.f90:98-124).
+// Ghost cells that copy interior cells (index-map boundaries) add nothing to
+// the minimum; otherwise the ring holds them.  The last workgroup to finish
+// reduces the per-workgroup minima, records the step-start time, advances t
+// and zeroes the step counters.  Few large workgroups: every workgroup ends
+// with a fence and an atomic on one counter, which serialise across the device.
 __global__ __launch_bounds__(kDtThreads) void k_dt(DtArgs a)
 {
     __shared__ double red[kDtThreads / 64];
@@ -36,7 +46,7 @@ __global__ __launch_bounds__(kDtThreads) void k_dt(DtArgs a)
     if (a.mode == kDtCompute) {
         // four coalesced cells per thread and trip, so that loads and division
         // chains of independent cells overlap
-        const size_t ncell = (size_t)a.N * a.N;
+        const size_t ncell = (size_t)a.NX * a.NY;
         const size_t gs = (size_t)gridDim.x * blockDim.x;
         for (size_t c0 = (size_t)blockIdx.x * blockDim.x + tid; c0 < ncell; c0 += 4 * gs) {
             double4 u[4];
@@ -47,12 +57,14 @@ __global__ __launch_bounds__(kDtThreads) void k_dt(DtArgs a)
             }
 #pragma unroll
             for (int q = 0; q < 4; ++q) {
-                const double vx = u[q].y / u[q].x, vy = u[q].z / u[q].x;
-                const double pres = pressure(u[q].x, u[q].y, u[q].z, u[q].w);
-                const double cs = sqrt(kGamma * pres / u[q].x);
-                const double v = 1.0 / (1.0 / (a.dx / (fabs(vx) + cs)) + 1.0 / (a.dy / (fabs(vy) + cs)));
+                const double v = dt_of_cell(u[q], a.dx, a.dy);
                 if (c0 + q * gs < ncell) m = fmin2(m, v);
             }
+        }
+        if (a.ring) {
+            const size_t nr = (size_t)ring_size(a.NX, a.NY);
+            for (size_t r = (size_t)blockIdx.x * blockDim.x + tid; r < nr; r += gs)
+                m = fmin2(m, dt_of_cell(a.ring[r], a.dx, a.dy));
         }
     }
     m = block_min(m, red);
@@ -86,6 +98,7 @@ __global__ __launch_bounds__(kDtThreads) void k_dt(DtArgs a)
         ts[1] = dt;
         ts[2] = dt / a.dx;
         ts[3] = dt / a.dy;
+        ts[4] = t;
         if (a.advance_t) ts[0] = t + dt;
         if (a.dt_log) a.dt_log[a.step_idx] = dt;
         if (a.new_step) {
@@ -120,6 +133,46 @@ hipError_t launch_orders_from_pred(const int8_t *pred, order_t *g, size_t n, hip
 hipError_t launch_unpack_orders(const order_t *g, int8_t *out, size_t n, hipStream_t s)
 {
     hipLaunchKernelGGL(k_unpack_orders, dim3(1024), dim3(256), 0, s, g, out, n);
+    return hipGetLastError();
+}
+
+// Ghost ring of a state (RingArgs): one thread per ghost cell.
+__global__ void k_ring(RingArgs a)
+{
+    const int NX = a.B.NX, NY = a.B.NY;
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= ring_size(NX, NY)) return;
+    int i, j;
+    ring_cell(r, NX, NY, i, j);
+    const double *ts = a.tstate;
+    const double sq = a.B.sqrt3;
+    if (a.mode == kRingFill) {
+        a.ring[r] = bnd_value(a.B, DevSrc{a.A, NX}, i, j, dmr_xmin_top(ts[0], sq));
+        return;
+    }
+    const double t0 = ts[4], dt = ts[1];
+    // the last stage's Forward_Euler output: Boundary_C at its t_rk, then the source
+    const double tb = a.mode == kRingRK3 ? t0 + 0.5 * dt : t0;
+    const double4 b = bnd_value(a.B, DevSrc{a.band, NX}, i, j, dmr_xmin_top(tb, sq));
+    double c[4] = {b.x, b.y, b.z, b.w};
+    if (a.gravity) gravity_source(c, dt * kGravity);
+    if (a.mode == kRingFE) {
+        a.ring[r] = make_double4(c[0], c[1], c[2], c[3]);
+        return;
+    }
+    // the step-start state's ghosts as Boundary_C left them in the first stage
+    const double4 u = bnd_value(a.B, DevSrc{a.A, NX}, i, j, dmr_xmin_top(t0, sq));
+    if (a.mode == kRingRK2)
+        a.ring[r] = make_double4(0.5 * (u.x + c[0]), 0.5 * (u.y + c[1]), 0.5 * (u.z + c[2]), 0.5 * (u.w + c[3]));
+    else
+        a.ring[r] = make_double4((1.0 / 3) * u.x + (2.0 / 3) * c[0], (1.0 / 3) * u.y + (2.0 / 3) * c[1],
+                                 (1.0 / 3) * u.z + (2.0 / 3) * c[2], (1.0 / 3) * u.w + (2.0 / 3) * c[3]);
+}
+
+hipError_t launch_ring(const RingArgs &a, hipStream_t s)
+{
+    const int n = ring_size(a.B.NX, a.B.NY);
+    hipLaunchKernelGGL(k_ring, dim3((n + 255) / 256), dim3(256), 0, s, a);
     return hipGetLastError();
 }
 

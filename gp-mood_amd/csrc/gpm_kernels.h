@@ -38,13 +38,41 @@ struct DevStats {
     long long detected_total;
     long long second_pass_cells;
     long long a_priori_lowered;
-    int passes[3];
+    int passes[5];  // MOOD passes of each stage of the last step (up to 5 stages: SSP_RK4)
     int max_passes;
 };
 
+constexpr int kMaxStages = 5;
+
+// How the Forward_Euler result `cand` of a stage (after the gravity source)
+// becomes the stage output, with Un the step-start state and Uin the stage
+// input (mod_SSP_RK2.f90:16-19, mod_SSP_RK3.f90:23-33, mod_SSP_RK4.f90:20-32;
+// SSP_RK4's Forward_Euler returns Uout - Uin, mod_FE.f90:187).
+enum {
+    kCmbCopy = 0,  // FE, and stage 1 of SSP_RK2 / SSP_RK3: cand
+    kCmbRK3b,      // 0.75*Un + 0.25*cand
+    kCmbRK3c,      // (1./3)*Un + (2./3)*cand
+    kCmbRK2b,      // 0.5*(Un + cand)
+    kCmbRK4a,      // Un + c1*(cand - Un)
+    kCmbRK4b,      // a20*Un + a21*Uin + c2*(cand - Uin)
+    kCmbRK4c,      // a30*Un + a32*Uin + c3*(cand - Uin)
+    kCmbRK4d,      // a40*Un + a43*Uin + c4*FU3, FU3 = cand - Uin (also written to W)
+    kCmbRK4e,      // f2*X1 + f3*X2 + ff3*X3 + f4*Uin + ff4*(cand - Uin), X = (U2, U3, FU3)
+};
+
+// Per-stage control of the step shared by k_stage and k_mood.
+struct StepCtl {
+    const double4 *__restrict__ Un;               // step-start state
+    const double4 *__restrict__ X1, *__restrict__ X2, *__restrict__ X3;  // kCmbRK4e operands
+    double4 *__restrict__ W;                      // kCmbRK4d: FU3
+    double4 *__restrict__ band;                   // if set: pre-source FE values of the cells within 4 of the boundary
+    int kind;                                     // kCmb*
+    int gravity;                                  // problem RT: source term after MOOD (mod_FE.f90:183-186)
+    int trk;                                      // t_rk of the stage: t (0), t+dt (1), t+0.5*dt (2)
+};
+
 struct StageArgs {
-    const double4 *__restrict__ Uin;  // stage input (interior, N*N)
-    const double4 *__restrict__ Un;   // step-start state for the RK combination
+    const double4 *__restrict__ Uin;  // stage input (interior, NX*NY)
     double4 *__restrict__ Uout;       // combined stage output
     order_t *__restrict__ gpo;        // packed CellGPO (interior)
     int *__restrict__ slist1;         // cells to recompute in the first correction pass
@@ -53,8 +81,10 @@ struct StageArgs {
     int *mark;                        // per-cell pass stamp (dedup of neighbourhoods)
     const int *stamp;                 // running stamp base
     const DevTables *__restrict__ T;
-    const double *__restrict__ tstate; // t, dt, dt/dx, dt/dy
-    int N, bc, dmp, u2, u2tol, cascade;
+    const double *__restrict__ tstate; // t, dt, dt/dx, dt/dy, t at the step start
+    Bnd B;
+    StepCtl C;
+    int dmp, u2, u2tol, cascade;
     int apriori;                      // start from the a priori orders in pred (learned predictor)
     int frozen;                       // orders fixed to pred, no detection (frozen-mask step)
     const int8_t *__restrict__ pred;  // a priori orders (read only here; gpo is rewritten)
@@ -64,7 +94,6 @@ struct StageArgs {
 
 struct MoodArgs {
     const double4 *__restrict__ Uin;
-    const double4 *__restrict__ Un;
     double4 *__restrict__ Uout;
     order_t *gpo;
     int *slist[2];  // recompute lists, pass p reads slist[p & 1]
@@ -77,7 +106,9 @@ struct MoodArgs {
     DevStats *stats;
     const DevTables *__restrict__ T;
     const double *__restrict__ tstate;
-    int N, bc, dmp, u2, u2tol, cascade, pmax, stage_idx;
+    Bnd B;
+    StepCtl C;
+    int dmp, u2, u2tol, cascade, pmax, stage_idx;
     int solo_max;   // lists of at most this many cells run on workgroup 0 alone (0: never)
     double dx, dy;
 };
@@ -86,6 +117,7 @@ enum { kDtCompute = 0, kDtGiven = 1 };
 
 struct DtArgs {
     const double4 *__restrict__ U;
+    const double4 *__restrict__ ring;  // ghost cells of U (ring layout), or null when they copy interior cells
     double *block_min;
     unsigned *done;
     double *tstate;
@@ -93,20 +125,66 @@ struct DtArgs {
     int *counters;
     int ncounters;
     DevStats *stats;
-    int N, mode, use_tmax, advance_t, step_idx, new_step;
+    int NX, NY, mode, use_tmax, advance_t, step_idx, new_step;
     double dx, dy, cfl, tmax, dtfinal, dt_given;
 };
 
-template <int MORD, int NGP>
-hipError_t launch_stage_mg(int flux, int stage, const StageArgs &a, int nblocks, hipStream_t s);
-template <int MORD, int NGP>
-const void *mood_kernel_mg(int flux, int stage);
+// Ghost cells of a state in a compact ring: rows -4..-1 and NY..NY+3 over
+// columns -4..NX+3, then columns -4..-1 and NX..NX+3 of rows 0..NY-1.
+__host__ __device__ __forceinline__ int ring_size(int NX, int NY) { return 8 * (NX + 8) + 8 * NY; }
+__host__ __device__ __forceinline__ void ring_cell(int r, int NX, int NY, int &i, int &j)
+{
+    const int W = NX + 8;
+    if (r < 8 * W) {
+        const int row = r / W;
+        i = r % W - 4;
+        j = row < 4 ? row - 4 : NY + row - 4;
+        return;
+    }
+    r -= 8 * W;
+    j = r / 8;
+    const int c = r % 8;
+    i = c < 4 ? c - 4 : NX + c - 4;
+}
+__host__ __device__ __forceinline__ int ring_index(int i, int j, int NX, int NY)
+{
+    const int W = NX + 8;
+    if (j < 0) return (j + 4) * W + i + 4;
+    if (j >= NY) return (4 + j - NY) * W + i + 4;
+    return 8 * W + j * 8 + (i < 0 ? i + 4 : 4 + i - NX);
+}
 
-// flux: numFlux id (kHLLC, kHLL, kUnsplit)
-hipError_t launch_stage(int mord, int ngp, int flux, int stage, const StageArgs &a, int nblocks, hipStream_t s);
-const void *mood_kernel(int mord, int ngp, int flux, int stage);
+// How the ring of a step's output is formed (the whole-array statements of the
+// integrators act on the ghost cells too, so the reference's Setdt sees these
+// values): G_t(A) is Boundary_C of A's interior at t_rk = t, S the gravity
+// source, B the pre-source Forward_Euler values of the last stage (band).
+enum {
+    kRingFill = 0,  // G_t(A), t = tstate[0]
+    kRingFE,        // S(G_t0(B))
+    kRingRK2,       // 0.5*(G_t0(Un) + S(G_t0(B)))
+    kRingRK3,       // (1./3)*G_t0(Un) + (2./3)*S(G_{t0+dt/2}(B))
+};
+
+struct RingArgs {
+    const double4 *__restrict__ A;     // kRingFill: the state; else the step-start state Un
+    const double4 *__restrict__ band;
+    double4 *__restrict__ ring;
+    const double *__restrict__ tstate;
+    Bnd B;
+    int mode, gravity;
+};
+
+template <int MORD, int NGP>
+hipError_t launch_stage_mg(int flux, const StageArgs &a, int nblocks, hipStream_t s);
+template <int MORD, int NGP>
+const void *mood_kernel_mg(int flux);
+
+// flux: numFlux id (kHLLC, kLLF, kHLL, kUnsplit)
+hipError_t launch_stage(int mord, int ngp, int flux, const StageArgs &a, int nblocks, hipStream_t s);
+const void *mood_kernel(int mord, int ngp, int flux);
 bool supported(int mord, int ngp);
 hipError_t launch_dt(const DtArgs &a, int nblocks, hipStream_t s);
+hipError_t launch_ring(const RingArgs &a, hipStream_t s);
 hipError_t launch_unpack_orders(const order_t *g, int8_t *out, size_t n, hipStream_t s);
 hipError_t launch_orders_from_pred(const int8_t *pred, order_t *g, size_t n, hipStream_t s);
 

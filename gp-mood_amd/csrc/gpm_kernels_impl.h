@@ -41,25 +41,87 @@ struct LdsAcc {
     __device__ __forceinline__ double4 operator()(int i, int j) const { return s[j * pitch + i]; }
 };
 
-// Accessor of a stage input in HBM with the boundary rule applied.
-struct GlobAcc {
-    const double4 *__restrict__ U;
-    int N, bc;
-    __device__ __forceinline__ double4 operator()(int i, int j) const { return load_bc(U, N, bc, i, j); }
-};
-
-template <int STAGE>
-__device__ __forceinline__ double4 rk_combine(const double4 *__restrict__ Un, size_t c, const double cand[4])
+// Time t_rk of a stage (time_stepper.f90:17, mod_SSP_RK3.f90:22-31) from the
+// step-start time tstate[4] and dt tstate[1].
+__device__ __forceinline__ double stage_time(const double *__restrict__ ts, int trk)
 {
-    if (STAGE == 1) return make_double4(cand[0], cand[1], cand[2], cand[3]);
-    const double4 u = Un[c];
-    if (STAGE == 2) {  // mod_SSP_RK3.f90:28: U2 = 0.75*Uin + 0.25*U2
-        return make_double4(0.75 * u.x + 0.25 * cand[0], 0.75 * u.y + 0.25 * cand[1], 0.75 * u.z + 0.25 * cand[2],
-                            0.75 * u.w + 0.25 * cand[3]);
+    const double t0 = ts[4];
+    if (trk == 1) return t0 + ts[1];
+    if (trk == 2) return t0 + 0.5 * ts[1];
+    return t0;
+}
+
+__device__ __forceinline__ double stage_xmin_top(const Bnd &B, const double *__restrict__ ts, int trk)
+{
+    return B.bc == kDMR ? dmr_xmin_top(stage_time(ts, trk), B.sqrt3) : 0.0;
+}
+
+// Cells whose values the ghost rules read: within 4 of the boundary.
+__device__ __forceinline__ bool in_band(int i, int j, int NX, int NY)
+{
+    return i < 4 || j < 4 || i >= NX - 4 || j >= NY - 4;
+}
+
+// Gravity source of problem RT after the MOOD loop (mod_FE.f90:183-186), with
+// dtg = dt*g: E first (with the old momentum), then the y momentum.
+__device__ __forceinline__ void gravity_source(double c[4], double dtg)
+{
+    c[3] = c[3] - dtg * c[2];
+    c[2] = c[2] - dtg * c[0];
+}
+
+// SSP_RK4 coefficients (constants.f90:49-55).
+constexpr double kRK4c1 = 0.391752226571890, kRK4a20 = 0.444370493651235, kRK4a21 = 0.555629506348765,
+                 kRK4c2 = 0.368410593050371, kRK4a30 = 0.620101851488403, kRK4a32 = 0.379898148511597,
+                 kRK4c3 = 0.251891774271694, kRK4a40 = 0.178079954393132, kRK4a43 = 0.821920045606868,
+                 kRK4c4 = 0.544974750228521, kRK4f2 = 0.517231671970585, kRK4f3 = 0.096059710526147,
+                 kRK4f4 = 0.386708617503269, kRK4ff3 = 0.063692468666290, kRK4ff4 = 0.226007483236906;
+
+// Stage output of one component (kCmb* of StepCtl) from the step-start value
+// un, the stage input uin and the Forward_Euler value c (source applied); x1..x3
+// are the kCmbRK4e operands; *w receives FU3 for kCmbRK4d.
+__device__ __forceinline__ double stage_out1(int kind, double un, double uin, double c, double x1, double x2,
+                                             double x3, double *w)
+{
+    switch (kind) {
+    case kCmbRK3b: return 0.75 * un + 0.25 * c;
+    case kCmbRK3c: return (1.0 / 3) * un + (2.0 / 3) * c;
+    case kCmbRK2b: return 0.5 * (un + c);
+    case kCmbRK4a: return un + kRK4c1 * (c - un);
+    case kCmbRK4b: return kRK4a20 * un + kRK4a21 * uin + kRK4c2 * (c - uin);
+    case kCmbRK4c: return kRK4a30 * un + kRK4a32 * uin + kRK4c3 * (c - uin);
+    case kCmbRK4d: {
+        const double f = c - uin;
+        *w = f;
+        return kRK4a40 * un + kRK4a43 * uin + kRK4c4 * f;
     }
-    // mod_SSP_RK3.f90:33: Uout = (1./3)*Uin + (2./3)*Uout
-    const double a = 1.0 / 3, b = 2.0 / 3;
-    return make_double4(a * u.x + b * cand[0], a * u.y + b * cand[1], a * u.z + b * cand[2], a * u.w + b * cand[3]);
+    case kCmbRK4e: return kRK4f2 * x1 + kRK4f3 * x2 + kRK4ff3 * x3 + kRK4f4 * uin + kRK4ff4 * (c - uin);
+    default: return c;
+    }
+}
+
+// Writes the output of interior cell gc: the band copy of the Forward_Euler
+// value (before the source term), the gravity source, the RK combination and,
+// for kCmbRK4d, FU3.  un: the step-start state of the cell (loaded by the
+// caller when C.kind needs it), uin: the stage input.
+__device__ __forceinline__ void write_stage_output(const StepCtl &C, double4 *__restrict__ Uout, size_t gc,
+                                                   bool band, const double4 &un, const double4 &uin,
+                                                   double cand[4], double dtg)
+{
+    if (band) C.band[gc] = make_double4(cand[0], cand[1], cand[2], cand[3]);
+    if (C.gravity) gravity_source(cand, dtg);
+    if (C.kind == kCmbCopy) {
+        Uout[gc] = make_double4(cand[0], cand[1], cand[2], cand[3]);
+        return;
+    }
+    double4 x1 = make_double4(0, 0, 0, 0), x2 = x1, x3 = x1;
+    if (C.kind == kCmbRK4e) { x1 = C.X1[gc]; x2 = C.X2[gc]; x3 = C.X3[gc]; }
+    double4 w;
+    Uout[gc] = make_double4(stage_out1(C.kind, un.x, uin.x, cand[0], x1.x, x2.x, x3.x, &w.x),
+                            stage_out1(C.kind, un.y, uin.y, cand[1], x1.y, x2.y, x3.y, &w.y),
+                            stage_out1(C.kind, un.z, uin.z, cand[2], x1.z, x2.z, x3.z, &w.z),
+                            stage_out1(C.kind, un.w, uin.w, cand[3], x1.w, x2.w, x3.w, &w.w));
+    if (C.kind == kCmbRK4d) C.W[gc] = w;
 }
 
 // Append the cells of the lanes with `flag` set to list/cnt: one atomic per
@@ -109,18 +171,19 @@ __device__ __forceinline__ int xcd_tile(int b, int nb)
 // Flag the 5-point neighbourhood of a failing cell for recomputation in the
 // next pass (DetCell, mod_subroutine_MOOD.f90:165-169): each cell enters the
 // next pass's list once, deduplicated by the pass stamp.
-__device__ __forceinline__ void mark_neighbourhood(int i, int j, int N, int stamp, int *mark, int *list, int *cnt)
+__device__ __forceinline__ void mark_neighbourhood(int i, int j, int NX, int NY, int stamp, int *mark, int *list,
+                                                   int *cnt)
 {
     const int ni[5] = {i, i - 1, i, i + 1, i}, nj[5] = {j - 1, j, j, j, j + 1};
 #pragma unroll
     for (int k = 0; k < 5; ++k) {
-        if (ni[k] < 0 || ni[k] >= N || nj[k] < 0 || nj[k] >= N) continue;
-        const int d = nj[k] * N + ni[k];
+        if (ni[k] < 0 || ni[k] >= NX || nj[k] < 0 || nj[k] >= NY) continue;
+        const int d = nj[k] * NX + ni[k];
         if (atomicExch(&mark[d], stamp) != stamp) list[atomicAdd(cnt, 1)] = d;
     }
 }
 
-template <int MORD, int NGP, int STAGE, int FLUX>
+template <int MORD, int NGP, int FLUX>
 __global__ __launch_bounds__(kStageThreads, GPM_STAGE_WAVES_PER_EU) void k_stage(StageArgs a)
 {
     constexpr int H = kHalo<MORD>;
@@ -129,15 +192,16 @@ __global__ __launch_bounds__(kStageThreads, GPM_STAGE_WAVES_PER_EU) void k_stage
     __shared__ double4 sFx[kTY * (kTX + 1)];
     __shared__ double4 sFy[(kTY + 1) * kTX];
 
-    const int N = a.N;
+    const int NX = a.B.NX, NY = a.B.NY;
     const int tile = xcd_tile(blockIdx.x, gridDim.x);
     if (tile < 0 || tile >= a.ntiles) return;
     const int x0 = (tile % a.tiles_x) * kTX, y0 = (tile / a.tiles_x) * kTY;
     const int tid = threadIdx.x;
+    const double xmin_top = stage_xmin_top(a.B, a.tstate, a.C.trk);
 
     for (int idx = tid; idx < PW * PH; idx += kStageThreads) {
         const int ti = idx % PW, tj = idx / PW;
-        sU[idx] = load_bc(a.Uin, N, a.bc, x0 - H + ti, y0 - H + tj);
+        sU[idx] = load_bnd(a.Uin, a.B, x0 - H + ti, y0 - H + tj, xmin_top);
     }
     // a priori orders of the tile and its one-cell ring (ghost cells: Mord)
     constexpr int OW = kTX + 2;
@@ -145,7 +209,7 @@ __global__ __launch_bounds__(kStageThreads, GPM_STAGE_WAVES_PER_EU) void k_stage
     if (a.apriori) {
         for (int idx = tid; idx < (kTY + 2) * OW; idx += kStageThreads) {
             const int gi = x0 - 1 + idx % OW, gj = y0 - 1 + idx / OW;
-            sO[idx] = (gi < 0 || gi >= N || gj < 0 || gj >= N) ? (int8_t)MORD : a.pred[(size_t)gj * N + gi];
+            sO[idx] = (gi < 0 || gi >= NX || gj < 0 || gj >= NY) ? (int8_t)MORD : a.pred[(size_t)gj * NX + gi];
         }
     }
     __syncthreads();
@@ -179,12 +243,13 @@ __global__ __launch_bounds__(kStageThreads, GPM_STAGE_WAVES_PER_EU) void k_stage
     __syncthreads();
 
     const double dtdx = a.tstate[2], dtdy = a.tstate[3];
+    const double dtg = a.tstate[1] * kGravity;
     const MoodParams P{a.dmp, a.u2, a.u2tol, a.dx, a.dy};
     const int stamp1 = *a.stamp + 1;
     for (int c = tid; c < kTX * kTY; c += kStageThreads) {
         const int cx = c % kTX, cy = c / kTX;
         const int gi = x0 + cx, gj = y0 + cy;
-        const bool inside = gi < N && gj < N;
+        const bool inside = gi < NX && gj < NY;
         bool dec = false;
         if (inside) {
             const double4 u = sU[(cy + H) * PW + cx + H];
@@ -197,11 +262,12 @@ __global__ __launch_bounds__(kStageThreads, GPM_STAGE_WAVES_PER_EU) void k_stage
             // CellGPO := Mord (mod_FE.f90:48), or the a priori order
             const int me = a.apriori ? (int)sO[(cy + 1) * OW + cx + 1] : MORD;
             if (me > 1 && !a.frozen) dec = decide(P, T, acc, cx + H, cy + H, cand);
-            const size_t gc = (size_t)gj * N + gi;
-            a.Uout[gc] = rk_combine<STAGE>(a.Un, gc, cand);
+            const size_t gc = (size_t)gj * NX + gi;
+            const double4 un = a.C.kind != kCmbCopy ? a.C.Un[gc] : u;
+            write_stage_output(a.C, a.Uout, gc, a.C.band && in_band(gi, gj, NX, NY), un, u, cand, dtg);
             // a failing cell takes the cascade at once
             a.gpo[gc] = pack_order(dec ? lower_order(me, a.cascade) : me, me, 0);
-            if (dec) mark_neighbourhood(gi, gj, N, stamp1, a.mark, a.slist1, a.cnt_s1);
+            if (dec) mark_neighbourhood(gi, gj, NX, NY, stamp1, a.mark, a.slist1, a.cnt_s1);
         }
         const unsigned long long m = __ballot(dec);
         if (__lane_id() == 0 && m) atomicAdd(a.cnt_f0, (int)__popcll(m));
@@ -230,32 +296,19 @@ struct Scoped {
 
 // mark_neighbourhood at scope SC.
 template <int SC>
-__device__ __forceinline__ void mark_neighbourhood_sc(int i, int j, int N, int stamp, int *mark, int *list, int *cnt)
+__device__ __forceinline__ void mark_neighbourhood_sc(int i, int j, int NX, int NY, int stamp, int *mark, int *list,
+                                                      int *cnt)
 {
     const int ni[5] = {i, i - 1, i, i + 1, i}, nj[5] = {j - 1, j, j, j, j + 1};
     int old[5];
 #pragma unroll
     for (int k = 0; k < 5; ++k) {
-        const bool in = !(ni[k] < 0 || ni[k] >= N || nj[k] < 0 || nj[k] >= N);
-        old[k] = in ? Scoped<SC>::exch(&mark[nj[k] * N + ni[k]], stamp) : stamp;
+        const bool in = !(ni[k] < 0 || ni[k] >= NX || nj[k] < 0 || nj[k] >= NY);
+        old[k] = in ? Scoped<SC>::exch(&mark[nj[k] * NX + ni[k]], stamp) : stamp;
     }
 #pragma unroll
     for (int k = 0; k < 5; ++k)
-        if (old[k] != stamp) list[Scoped<SC>::add(cnt, 1)] = nj[k] * N + ni[k];
-}
-
-// RK combination with the step-start state already loaded.
-template <int STAGE>
-__device__ __forceinline__ double4 rk_combine_u(const double4 &u, const double cand[4])
-{
-    if (STAGE == 1) return make_double4(cand[0], cand[1], cand[2], cand[3]);
-    if (STAGE == 2) {  // mod_SSP_RK3.f90:28: U2 = 0.75*Uin + 0.25*U2
-        return make_double4(0.75 * u.x + 0.25 * cand[0], 0.75 * u.y + 0.25 * cand[1], 0.75 * u.z + 0.25 * cand[2],
-                            0.75 * u.w + 0.25 * cand[3]);
-    }
-    // mod_SSP_RK3.f90:33: Uout = (1./3)*Uin + (2./3)*Uout
-    const double a = 1.0 / 3, b = 2.0 / 3;
-    return make_double4(a * u.x + b * cand[0], a * u.y + b * cand[1], a * u.z + b * cand[2], a * u.w + b * cand[3]);
+        if (old[k] != stamp) list[Scoped<SC>::add(cnt, 1)] = nj[k] * NX + ni[k];
 }
 
 // MOOD correction passes 1, 2, ... of one RK stage (mod_FE.f90:69-174), as a
@@ -271,7 +324,7 @@ __device__ __forceinline__ double4 rk_combine_u(const double4 &u, const double c
 // depend on the staged neighbourhood (the face's two order words, the cell's
 // own order word and step-start state, the next trip's list entry) are
 // issued together with the neighbourhood loads.
-template <int MORD, int NGP, int STAGE, int FLUX>
+template <int MORD, int NGP, int FLUX>
 __global__ __launch_bounds__(kMoodThreads) void k_mood(MoodArgs a)
 {
     constexpr int NGPP = NGP <= 1 ? 1 : NGP <= 2 ? 2 : NGP <= 4 ? 4 : 8;
@@ -279,8 +332,8 @@ __global__ __launch_bounds__(kMoodThreads) void k_mood(MoodArgs a)
     static_assert(64 % G == 0, "lanes per cell must divide the wavefront");
     constexpr int DEV = __HIP_MEMORY_SCOPE_AGENT, WG = __HIP_MEMORY_SCOPE_WORKGROUP;
     cg::grid_group grid = cg::this_grid();
-    const int N = a.N;
-    const int ncell = N * N;
+    const int NX = a.B.NX, NY = a.B.NY;
+    const int ncell = NX * NY;
     const int gtid = blockIdx.x * blockDim.x + threadIdx.x;
     const int gsz = gridDim.x * blockDim.x;
     const int lane = __lane_id();
@@ -290,6 +343,8 @@ __global__ __launch_bounds__(kMoodThreads) void k_mood(MoodArgs a)
     const DevTables *__restrict__ T = a.T;
     const MoodParams P{a.dmp, a.u2, a.u2tol, a.dx, a.dy};
     const double dtdx = a.tstate[2], dtdy = a.tstate[3];
+    const double dtg = a.tstate[1] * kGravity;
+    const double xmin_top = stage_xmin_top(a.B, a.tstate, a.C.trk);
     // face f of a cell (i, j): 0 bottom y-face, 1 left x-face, 2 right x-face, 3 top y-face
     const int fdir = (f == 1 || f == 2) ? 0 : 1;
     const int fa = fdir == 0 ? kR : kT, fb = fdir == 0 ? kL : kB;
@@ -320,8 +375,8 @@ __global__ __launch_bounds__(kMoodThreads) void k_mood(MoodArgs a)
         constexpr int SC = decltype(sc)::value;
         using M = Scoped<SC>;
         auto order_of = [&](int ii, int jj) -> int {
-            if (ii < 0 || ii >= N || jj < 0 || jj >= N) return MORD;  // ghost cells keep Mord
-            return order_at_pass(M::load(&a.gpo[(size_t)jj * N + ii]), p);
+            if (ii < 0 || ii >= NX || jj < 0 || jj >= NY) return MORD;  // ghost cells keep Mord
+            return order_at_pass(M::load(&a.gpo[(size_t)jj * NX + ii]), p);
         };
         const int stamp_next = stamp_base + p + 1;
         const int per_iter = nthr / G;
@@ -331,7 +386,7 @@ __global__ __launch_bounds__(kMoodThreads) void k_mood(MoodArgs a)
             const int t = it * per_iter + t0 / G;
             const bool valid = t < nS;
             const int d = valid ? dnext : 0;
-            const int i = d % N, j = d / N;
+            const int i = d % NX, j = d / NX;
             // independent of the neighbourhood: issue before staging it
             int ord = 1;
             if (valid && jg < NGP) ord = min(order_of(i + oai, j + oaj), order_of(i + obi, j + obj));
@@ -339,13 +394,13 @@ __global__ __launch_bounds__(kMoodThreads) void k_mood(MoodArgs a)
             double4 un = make_double4(0.0, 0.0, 0.0, 0.0);
             if (valid && gl == 0) {
                 me = order_at_pass(M::load(&a.gpo[d]), p);
-                if (STAGE > 1) un = a.Un[d];
+                if (a.C.kind != kCmbCopy) un = a.C.Un[d];
             }
             const int tn = t + per_iter;
             if (it + 1 < niter && tn < nS) dnext = S[tn];
             __syncthreads();  // the previous trip's readers are done with nb
             for (int q = gl; q < BOX * BOX; q += G)
-                nb[cs][q] = load_bc(a.Uin, N, a.bc, i - HB + q % BOX, j - HB + q / BOX);
+                nb[cs][q] = load_bnd(a.Uin, a.B, i - HB + q % BOX, j - HB + q / BOX, xmin_top);
             __syncthreads();
             double F[4] = {0.0, 0.0, 0.0, 0.0};
             if (valid && jg < NGP)
@@ -369,14 +424,15 @@ __global__ __launch_bounds__(kMoodThreads) void k_mood(MoodArgs a)
             }
             if (valid && gl == 0) {
                 double cand[4];
-                apply_fluxes(lacc(HB, HB), fyb, fxl, fxr, fyt, dtdx, dtdy, cand);
+                const double4 uin = lacc(HB, HB);
+                apply_fluxes(uin, fyb, fxl, fxr, fyt, dtdx, dtdy, cand);
                 bool dec = false;
                 if (me > 1) dec = decide(P, T, lacc, HB, HB, cand);
-                a.Uout[d] = rk_combine_u<STAGE>(un, cand);
+                write_stage_output(a.C, a.Uout, d, a.C.band && in_band(i, j, NX, NY), un, uin, cand, dtg);
                 if (dec) {
                     M::store(&a.gpo[d], pack_order(lower_order(me, a.cascade), me, p));
                     M::add(cnt_fail, 1);
-                    mark_neighbourhood_sc<SC>(i, j, N, stamp_next, a.mark, Sn, cnt_next);
+                    mark_neighbourhood_sc<SC>(i, j, NX, NY, stamp_next, a.mark, Sn, cnt_next);
                 }
             }
         }
@@ -453,34 +509,23 @@ __global__ __launch_bounds__(kMoodThreads) void k_mood(MoodArgs a)
 // Launch entry points of one (Mord, ngp) combination, for every numFlux; each
 // combination is compiled in a translation unit of its own.
 #define GPM_INSTANTIATE_COMBO(M, G)                                                                          \
-    template <int F>                                                                                          \
-    static void launch_stage_f##M##G(int stage, const StageArgs &a, int nblocks, hipStream_t s)              \
-    {                                                                                                         \
-        if (stage == 1) hipLaunchKernelGGL((k_stage<M, G, 1, F>), dim3(nblocks), dim3(kStageThreads), 0, s, a); \
-        else if (stage == 2) hipLaunchKernelGGL((k_stage<M, G, 2, F>), dim3(nblocks), dim3(kStageThreads), 0, s, a); \
-        else hipLaunchKernelGGL((k_stage<M, G, 3, F>), dim3(nblocks), dim3(kStageThreads), 0, s, a);       \
-    }                                                                                                         \
-    template <int F>                                                                                          \
-    static const void *mood_kernel_f##M##G(int stage)                                                         \
-    {                                                                                                         \
-        if (stage == 1) return reinterpret_cast<const void *>(&k_mood<M, G, 1, F>);                          \
-        if (stage == 2) return reinterpret_cast<const void *>(&k_mood<M, G, 2, F>);                          \
-        return reinterpret_cast<const void *>(&k_mood<M, G, 3, F>);                                          \
-    }                                                                                                         \
     template <>                                                                                               \
-    hipError_t launch_stage_mg<M, G>(int flux, int stage, const StageArgs &a, int nblocks, hipStream_t s)    \
+    hipError_t launch_stage_mg<M, G>(int flux, const StageArgs &a, int nblocks, hipStream_t s)               \
     {                                                                                                         \
-        if (flux == kHLL) launch_stage_f##M##G<kHLL>(stage, a, nblocks, s);                                  \
-        else if (flux == kUnsplit) launch_stage_f##M##G<kUnsplit>(stage, a, nblocks, s);                     \
-        else launch_stage_f##M##G<kHLLC>(stage, a, nblocks, s);                                              \
+        if (flux == kHLL) hipLaunchKernelGGL((k_stage<M, G, kHLL>), dim3(nblocks), dim3(kStageThreads), 0, s, a); \
+        else if (flux == kLLF) hipLaunchKernelGGL((k_stage<M, G, kLLF>), dim3(nblocks), dim3(kStageThreads), 0, s, a); \
+        else if (flux == kUnsplit)                                                                            \
+            hipLaunchKernelGGL((k_stage<M, G, kUnsplit>), dim3(nblocks), dim3(kStageThreads), 0, s, a);       \
+        else hipLaunchKernelGGL((k_stage<M, G, kHLLC>), dim3(nblocks), dim3(kStageThreads), 0, s, a);       \
         return hipGetLastError();                                                                             \
     }                                                                                                         \
     template <>                                                                                               \
-    const void *mood_kernel_mg<M, G>(int flux, int stage)                                                    \
+    const void *mood_kernel_mg<M, G>(int flux)                                                                \
     {                                                                                                         \
-        if (flux == kHLL) return mood_kernel_f##M##G<kHLL>(stage);                                           \
-        if (flux == kUnsplit) return mood_kernel_f##M##G<kUnsplit>(stage);                                   \
-        return mood_kernel_f##M##G<kHLLC>(stage);                                                            \
+        if (flux == kHLL) return reinterpret_cast<const void *>(&k_mood<M, G, kHLL>);                        \
+        if (flux == kLLF) return reinterpret_cast<const void *>(&k_mood<M, G, kLLF>);                        \
+        if (flux == kUnsplit) return reinterpret_cast<const void *>(&k_mood<M, G, kUnsplit>);                \
+        return reinterpret_cast<const void *>(&k_mood<M, G, kHLLC>);                                         \
     }
 
 }  // namespace gpm

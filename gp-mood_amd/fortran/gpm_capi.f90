@@ -7,7 +7,7 @@ module gpm_capi
 
    type, bind(C) :: gpm_config
       integer(c_int32_t) :: lf, nf, mord, ngp, method, integrator, num_flux, bc_type
-      integer(c_int32_t) :: dmp, u2, u2_tol, cascade
+      integer(c_int32_t) :: dmp, u2, u2_tol, cascade, problem
       real(c_double) :: cfl, lx, ly, ell_o_dx
    end type gpm_config
 

@@ -1,8 +1,9 @@
 ! Drop-in replacement of the reference's time_stepper module
 ! (gp-mood-code/time_stepper.f90:1-25): same module and subroutine names and
 ! the same signature time_stepping(Uin, Uout), so main.f90, mod_*.f90 and the
-! parameters.f90 / problems interface compile against it unchanged.  The SSP-RK3
-! step (mod_SSP_RK3.f90:12-39) runs on the GPU through libgpmood; the GP tables
+! parameters.f90 / problems interface compile against it unchanged.  The step of
+! the configured integrator (FE, SSP_RK2, SSP_RK3 or SSP_RK4, time_stepper.f90:17-21)
+! runs on the GPU through libgpmood; the GP tables
 ! are the ones GP_presim_sphere left in global_variables, so both codes use the
 ! same weights.  Link with -lgpmood.
 module time_stepper
@@ -26,6 +27,7 @@ contains
       cfg%method = method; cfg%integrator = integrator; cfg%num_flux = numFlux; cfg%bc_type = BC_type
       cfg%dmp = merge(1, 0, DMP); cfg%u2 = merge(1, 0, U2); cfg%u2_tol = merge(1, 0, U2_tol)
       cfg%cascade = GPM_CASCADE_REFERENCE
+      cfg%problem = problem
       cfg%cfl = CFL; cfg%lx = Lx; cfg%ly = Ly; cfg%ell_o_dx = real(ell_o_dx, c_double)
       rc = gpm_create(cfg, gpm_handle)
       if (rc /= 0) then
@@ -55,6 +57,8 @@ contains
          print *, 'gpm_time_stepping failed: ', trim(gpm_error_message(gpm_handle))
          stop 1
       end if
+      ! t_rk as the integrator leaves it (mod_SSP_RK3.f90:31 sets the last stage's)
+      if (integrator == SSP_RK3) t_rk = t + 0.5*dt
       CellGPO = gpm_order
       count_detected_cell = int(st%count_detected_cell)
       count_detected_cell_a_posteriori = int(st%count_detected_cell)

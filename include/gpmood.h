@@ -29,17 +29,28 @@
 extern "C" {
 #endif
 
-/* Identifiers shared with the reference's constants.f90:16-27. */
+/* Identifiers shared with the reference's constants.f90:16-34. */
 #define GPM_GP_MOOD 2
 #define GPM_NN_GP_MOOD 203
+#define GPM_FE 8
+#define GPM_SSP_RK2 9
 #define GPM_SSP_RK3 10
+#define GPM_SSP_RK4 12
 #define GPM_HLLC 100
+#define GPM_LLF 101
 #define GPM_HLL 102
 #define GPM_UNSPLIT 103
 #define GPM_NEUMANN 17
 #define GPM_PERIODIC 18
+#define GPM_DIRICHLET 19
 #define GPM_REFLECTIVE 20
 #define GPM_KH_BC 403
+#define GPM_MACH800_BC 201
+#define GPM_DOUBLEMACH800_BC 301
+#define GPM_RMI_BC 401
+#define GPM_RT_BC 402
+#define GPM_DMR_BC 35        /* the reference's BC_type = DMR (the problem id) */
+#define GPM_PROBLEM_RT 51    /* problem = RT: gravity source after MOOD (mod_FE.f90:183-186) */
 
 /* Order cascade applied to a cell that fails detection. */
 #define GPM_CASCADE_REFERENCE 0 /* 7->3, 5->3, 3->1 (mod_subroutine_MOOD.f90:159-163) */
@@ -55,15 +66,20 @@ extern "C" {
 /* Runtime form of the reference's compile-time `module parameters`
  * (problems/2DRP3_GPMOOD.f90:8-46).  Field names follow that module. */
 typedef struct gpm_config {
-    int32_t lf, nf;       /* interior cells in x and y (square grids: lf == nf) */
+    int32_t lf, nf;       /* interior cells in x and y (>= 8 each) */
     int32_t mord;         /* Mord: 1, 3, 5 or 7 */
     int32_t ngp;          /* Gauss points per face: 1..5 */
-    int32_t method;       /* GPM_GP_MOOD or GPM_NN_GP_MOOD */
-    int32_t integrator;   /* GPM_SSP_RK3 */
-    int32_t num_flux;     /* GPM_HLLC (mod_HLLC.f90), GPM_HLL (mod_HLL.f90) or GPM_UNSPLIT (mod_unsplit.f90) */
-    int32_t bc_type;      /* GPM_NEUMANN, GPM_PERIODIC, GPM_REFLECTIVE or GPM_KH_BC (BC.f90:18-135, 383-406) */
+    int32_t method;       /* GPM_GP_MOOD or GPM_NN_GP_MOOD (lf == nf, copy boundary, SSP_RK3) */
+    int32_t integrator;   /* GPM_FE, GPM_SSP_RK2, GPM_SSP_RK3 or GPM_SSP_RK4 (time_stepper.f90:17-21;
+                             SSP_RK4 only with a copy boundary and no source term) */
+    int32_t num_flux;     /* GPM_HLLC (mod_HLLC.f90), GPM_LLF (mod_LLF.f90), GPM_HLL (mod_HLL.f90)
+                             or GPM_UNSPLIT (mod_unsplit.f90) */
+    int32_t bc_type;      /* copy boundaries GPM_NEUMANN, GPM_PERIODIC, GPM_REFLECTIVE, GPM_KH_BC;
+                             formula boundaries GPM_DIRICHLET, GPM_MACH800_BC, GPM_DOUBLEMACH800_BC,
+                             GPM_RMI_BC, GPM_DMR_BC, GPM_RT_BC (BC.f90:18-413) */
     int32_t dmp, u2, u2_tol; /* MOOD switches DMP, U2, U2_tol (logical: 0/1) */
     int32_t cascade;      /* GPM_CASCADE_* */
+    int32_t problem;      /* the reference's problem id; only GPM_PROBLEM_RT changes the step */
     double cfl;           /* CFL */
     double lx, ly;        /* domain lengths Lx, Ly */
     double ell_o_dx;      /* GP length scale over dx */
@@ -75,7 +91,7 @@ typedef struct gpm_stats {
     int64_t steps;                 /* steps taken since the last reset */
     int64_t count_detected_cell;   /* decreases of the last step, all stages and passes */
     int64_t detected_total;        /* decreases summed over all steps since reset */
-    int32_t passes[3];             /* MOOD passes per RK stage of the last step */
+    int32_t passes[3];             /* MOOD passes of the first three RK stages of the last step */
     int32_t max_passes;            /* largest pass count of any stage since reset */
     int64_t second_pass_cells;     /* cells recomputed after the first pass, since reset */
     int64_t a_priori_lowered;      /* cells started below Mord by the learned predictor */
@@ -107,33 +123,49 @@ int gpm_gp_tables(int32_t mord, int32_t ngp, double ell_o_dx, double lx, double 
                   int32_t lf, int32_t nf, double *zT_sp, double *gp_d2x_sp,
                   double *gp_d2y_sp, double *gauss_weight, int32_t *ixiy_sp);
 
+/* Boundary_C (BC.f90:18-413) on a host array U[nf+8][lf+8][4] of the
+ * configuration's grid and BC_type, at stage time t_rk (read by the DMR
+ * boundary): fills the ghost cells from the interior in place.  Host code
+ * only; needs no device and no context.  The kernels apply the same rule. */
+int gpm_boundary_c(const gpm_config *cfg, double t_rk, double *U);
+
 /* Host-array state transfer in the reference layout.  gpm_get_state fills the
- * ghost cells with the boundary condition (Boundary_C, BC.f90:18-135) and, if
- * cellgpo is non-NULL, writes the order mask of the last RK stage. */
+ * ghost cells as the reference's array holds them: for copy boundaries by
+ * Boundary_C (BC.f90:18-135, 383-406); for formula boundaries and problem RT
+ * with the values the integrator's whole-array statements leave there (the
+ * values the next Setdt reads: RK combinations of the stages' Boundary_C
+ * fills and the gravity source), or, before the first step, the ghost cells
+ * passed to gpm_set_state.  If cellgpo is non-NULL it receives the order mask
+ * of the last RK stage. */
 int gpm_set_state(gpm_ctx *ctx, const double *U);
 int gpm_get_state(gpm_ctx *ctx, double *U, int32_t *cellgpo);
 
 /* Device-resident state transfer: `u_dev` points to nf*lf*4 doubles of device
- * memory holding the interior cells [n][l][k] (no ghosts). */
+ * memory holding the interior cells [n][l][k] (no ghosts).  For formula
+ * boundaries gpm_set_state_device fills the ghost cells by Boundary_C at the
+ * current time. */
 int gpm_set_state_device(gpm_ctx *ctx, const double *u_dev);
 int gpm_get_state_device(gpm_ctx *ctx, double *u_dev, int8_t *order_dev);
 
-/* Setdt (physics.f90:78-129) on the resident state: dt = CFL*min(...),
- * clipped to dtfinal. */
+/* Setdt (physics.f90:78-129) on the resident state, ghost cells included:
+ * dt = CFL*min(...), clipped to dtfinal. */
 int gpm_compute_dt(gpm_ctx *ctx, double dtfinal, double *dt);
 
-/* Drop-in for `subroutine time_stepping(Uin, Uout)` (time_stepper.f90:12-23,
- * SSP_RK3_ of mod_SSP_RK3.f90:12-39) with the reference's implicit globals made
- * explicit: t and dt are the module variables set by main.f90 and Setdt; the
- * order mask of the last stage is written to cellgpo (may be NULL) and the
- * counters to stats (may be NULL).  Uin's ghost cells are refilled as the
- * reference's Boundary_C(Uin) does; Uout's ghost cells are filled. */
+/* Drop-in for `subroutine time_stepping(Uin, Uout)` (time_stepper.f90:12-23:
+ * Forward_Euler, SSP_RK2_, SSP_RK3_ or SSP_RK4_ by the configured integrator)
+ * with the reference's implicit globals made explicit: t and dt are the module
+ * variables set by main.f90 and Setdt; the order mask of the last stage is
+ * written to cellgpo (may be NULL) and the counters to stats (may be NULL).
+ * Uin's ghost cells are refilled as the reference's Boundary_C(Uin) of the
+ * first stage does; Uout's ghost cells hold what the reference's Uout holds
+ * (see gpm_get_state). */
 int gpm_time_stepping(gpm_ctx *ctx, double *Uin, double *Uout, double t, double dt,
                       int32_t *cellgpo, gpm_stats *stats);
 
 /* Device-resident driver loop of main.f90:145-162 for nsteps steps:
- * dtfinal = tmax - t; Setdt; SSP-RK3 step; t = t + dt.  No host round trip per
- * step.  If dts is non-NULL it receives the nsteps time steps (host memory). */
+ * dtfinal = tmax - t; Setdt; one step of the integrator; t = t + dt.  No host
+ * round trip per step.  If dts is non-NULL it receives the nsteps time steps
+ * (host memory). */
 int gpm_run(gpm_ctx *ctx, int32_t nsteps, double tmax, double *dts);
 
 int gpm_get_stats(gpm_ctx *ctx, gpm_stats *stats);

@@ -143,7 +143,8 @@ def config(name):
 
 # configs for which the Fortran drop-in (reference modules + product
 # time_stepper over libgpmood) is also built
-DROPIN_CONFIGS = ["r2_n64_neu", "r3_n64_ref", "r3_n64_kh_uns"]
+DROPIN_CONFIGS = ["r2_n64_neu", "r3_n64_ref", "r3_n64_kh_uns", "r1_dmr_64x16", "r1_rt_16x48", "r1_m800_32_fe",
+                  "r2_n32_ref_rk4"]
 
 PARAMS_TEMPLATE = """module parameters
    use constants

@@ -1,7 +1,7 @@
 """Fortran drop-in: the reference's own modules and step loop (Setdt,
 Boundary_C, GP_presim_sphere, ...) linked with the product's time_stepper
-module (gp-mood_amd/fortran/time_stepper.f90), which runs the SSP-RK3 step on
-the GPU through libgpmood.  Built by oracle/ref/build_ref.py; its outputs must
+module (gp-mood_amd/fortran/time_stepper.f90), which runs the step of the
+configured integrator on the GPU through libgpmood.  Built by oracle/ref/build_ref.py; its outputs must
 equal the pure reference run stored in the golden fixtures, bit for bit."""
 import os
 import resource
@@ -16,7 +16,9 @@ from conftest import ROOT, golden, has_gpu
 pytestmark = pytest.mark.gpu
 
 CASES = [("r2_n64_neu", "step_r2_n64_neu_fq8.npz"), ("r3_n64_ref", "step_r3_n64_ref_bl6.npz"),
-         ("r3_n64_kh_uns", "step_r3_n64_kh_uns_fq15.npz")]
+         ("r3_n64_kh_uns", "step_r3_n64_kh_uns_fq15.npz"), ("r1_dmr_64x16", "step_r1_dmr_64x16.npz"),
+         ("r1_rt_16x48", "step_r1_rt_16x48_q35.npz"), ("r1_m800_32_fe", "step_r1_m800_32_fe_n33.npz"),
+         ("r2_n32_ref_rk4", "step_r2_n32_ref_rk4_bl30.npz")]
 STEP_TYPE = np.dtype([("dt", "<f8"), ("t", "<f8"), ("cnt", "<i4")])
 
 
@@ -37,6 +39,7 @@ def test_fortran_dropin_equals_reference(cfg, fixture):
         pytest.skip("drop-in binary not built (needs the reference sources at build time)")
     z = golden(fixture)
     n, nsteps = int(z["n"]), int(z["nsteps"])
+    nf = int(z["nf"]) if "nf" in z else n
     print(f"[case Fortran drop-in {cfg}, {nsteps} steps]", flush=True)
     with tempfile.TemporaryDirectory() as d:
         np.ascontiguousarray(z["U0"]).tofile(os.path.join(d, "ic.bin"))
@@ -44,8 +47,8 @@ def test_fortran_dropin_equals_reference(cfg, fixture):
                            preexec_fn=_stack, timeout=240)
         assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
         steps = np.fromfile(os.path.join(d, "steps.bin"), dtype=STEP_TYPE)
-        masks = np.fromfile(os.path.join(d, "masks.bin"), dtype=np.int8).reshape(-1, n, n)
-        U = np.fromfile(os.path.join(d, "U_final.bin"), dtype=np.float64).reshape(n + 8, n + 8, 4)
+        masks = np.fromfile(os.path.join(d, "masks.bin"), dtype=np.int8).reshape(-1, nf, n)
+        U = np.fromfile(os.path.join(d, "U_final.bin"), dtype=np.float64).reshape(nf + 8, n + 8, 4)
     assert np.array_equal(steps["dt"], z["dt"])
     assert np.array_equal(steps["cnt"], z["counts"])
     assert np.array_equal(masks, z["masks"])
