@@ -29,7 +29,9 @@ struct gpm_ctx {
     int N = 0;  // NX where only square grids are supported (learned predictor, adjoint)
     int integrator = 0, nstages = 0, gravity = 0;
     bool ring_mode = false;  // ghost cells are not all copies of interior cells: keep them in `ring`
-    Bnd B{};
+    Bnd B{}, Bh{};           // the boundary for kernels (states in st_dev) and for host code (st_host)
+    double4 st_host[kBndStates] = {};
+    double4 *st_dev = nullptr;
     double dx = 0, dy = 0;
     hipStream_t stream = nullptr;
     DevTables hT{};
@@ -156,35 +158,38 @@ void ring_to_ghosts(const double4 *ring, double *full, int NX, int NY)
     }
 }
 
-// The boundary descriptor of a configuration: fixed ghost states of the
-// formula rules by primitive_to_conservative (BC.f90:60-381, IC.f90:336-355).
-Bnd make_bnd(int NX, int NY, int bc, double dx, double dy)
+// The boundary descriptor of a configuration, with its fixed ghost states in
+// st[kBndStates]: the formula rules' states by primitive_to_conservative
+// (BC.f90:60-381, IC.f90:336-355).
+Bnd make_bnd(int NX, int NY, int bc, double dx, double dy, double4 *st)
 {
     Bnd B{};
     B.NX = NX; B.NY = NY; B.bc = bc; B.dx = dx; B.dy = dy;
     B.sqrt3 = std::sqrt(3.);
+    for (int k = 0; k < kBndStates; ++k) st[k] = make_double4(0, 0, 0, 0);
     switch (bc) {
     case kMach800:
-        B.st[0] = p2c(1.4, 0.0, 800.0, 1.0);
+        st[0] = p2c(1.4, 0.0, 800.0, 1.0);
         break;
     case kDoubleMach800:
-        B.st[0] = p2c(1.4, 0.0, 800.0, 1.0);
-        B.st[1] = p2c(1.4, 0.0, -800.0, 1.0);
+        st[0] = p2c(1.4, 0.0, 800.0, 1.0);
+        st[1] = p2c(1.4, 0.0, -800.0, 1.0);
         break;
     case kDMR:
-        B.st[0] = p2c(8., 7.1447096, -4.125, 116.5);
-        B.st[1] = p2c(1.4, 0.0, 0.0, 1.0);
+        st[0] = p2c(8., 7.1447096, -4.125, 116.5);
+        st[1] = p2c(1.4, 0.0, 0.0, 1.0);
         break;
     case kRMI:
-        B.st[0] = p2c(2.666666666666667, 2. * std::sqrt(1.4), 0., 4.5);
+        st[0] = p2c(2.666666666666667, 2. * std::sqrt(1.4), 0., 4.5);
         break;
     case kDirichlet:
-        B.st[0] = p2c(3.857143, 2.629369, 0., 10.33333);
-        for (int g = 1; g <= 4; ++g) B.st[g] = p2c(1. + 0.2 * std::sin(5 * (mesh_c(NX - 1 + g, dx) - 4.5)), 0., 0., 1.);
+        st[0] = p2c(3.857143, 2.629369, 0., 10.33333);
+        for (int g = 1; g <= 4; ++g) st[g] = p2c(1. + 0.2 * std::sin(5 * (mesh_c(NX - 1 + g, dx) - 4.5)), 0., 0., 1.);
         break;
     default:
         break;
     }
+    B.st = st;
     return B;
 }
 
@@ -385,7 +390,7 @@ int enqueue_stages(gpm_ctx *c)
             HIPCHK(c, hipEventRecord(e0, c->stream));
         }
         // a multiple of 8 workgroups, so the XCD-aware remap in k_stage is a bijection
-        HIPCHK(c, launch_stage(c->mord, c->ngp, c->cfg.num_flux, s, (c->ntiles + 7) / 8 * 8, c->stream));
+        HIPCHK(c, launch_stage(c->mord, c->ngp, c->cfg.num_flux, c->ring_mode, s, (c->ntiles + 7) / 8 * 8, c->stream));
         if (c->timing) HIPCHK(c, hipEventRecord(e1, c->stream));
 
         MoodArgs m{};
@@ -405,7 +410,7 @@ int enqueue_stages(gpm_ctx *c)
             c->ev_pairs.push_back({(int)i0, 1});
             HIPCHK(c, hipEventRecord(e0, c->stream));
         }
-        HIPCHK(c, hipLaunchCooperativeKernel(mood_kernel(c->mord, c->ngp, c->cfg.num_flux), dim3(c->mood_blocks),
+        HIPCHK(c, hipLaunchCooperativeKernel(mood_kernel(c->mord, c->ngp, c->cfg.num_flux, c->ring_mode), dim3(c->mood_blocks),
                                              dim3(kMoodThreads), args, 0, c->stream));
         if (c->timing) HIPCHK(c, hipEventRecord(e1, c->stream));
         if (c->rec_masks && c->rec_count < c->rec_cap) {
@@ -497,7 +502,8 @@ int gpm_create(const gpm_config *cfg, gpm_ctx **out)
     // dx = real(Lx_16/lf, 8) (global_variables.f90:19-21)
     c->dx = quad_ratio(cfg->lx, cfg->lf);
     c->dy = quad_ratio(cfg->ly, cfg->nf);
-    c->B = make_bnd(c->NX, c->NY, c->bc, c->dx, c->dy);
+    c->Bh = make_bnd(c->NX, c->NY, c->bc, c->dx, c->dy, c->st_host);
+    c->B = c->Bh;
     c->nstages = cfg->integrator == GPM_FE ? 1 : cfg->integrator == GPM_SSP_RK2 ? 2 : cfg->integrator == GPM_SSP_RK4 ? 5 : 3;
     const size_t n = (size_t)c->NX * c->NY;
     hipError_t e;
@@ -518,6 +524,7 @@ int gpm_create(const gpm_config *cfg, gpm_ctx **out)
              dalloc(&c->dF, n) == hipSuccess;
     if (ok && c->ring_mode)
         ok = dalloc(&c->ring, (size_t)ring_size(c->NX, c->NY)) == hipSuccess && dalloc(&c->band, n) == hipSuccess;
+    if (ok) ok = dalloc(&c->st_dev, kBndStates) == hipSuccess;
     if (!ok) {
         gpm_destroy(c);
         std::fprintf(stderr, "gpm_create: device allocation failed\n");
@@ -541,7 +548,8 @@ int gpm_create(const gpm_config *cfg, gpm_ctx **out)
     if (const char *env = std::getenv("GPM_MOOD_BLOCKS")) want = std::atoi(env) > 0 ? std::atoi(env) : ncu;
     {
         int per = 0;
-        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, mood_kernel(c->mord, c->ngp, c->cfg.num_flux), kMoodThreads, 0);
+        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, mood_kernel(c->mord, c->ngp, c->cfg.num_flux, c->ring_mode),
+                                                         kMoodThreads, 0);
         if (e != hipSuccess || per <= 0) per = 1;
         c->mood_blocks = want < per * ncu ? want : per * ncu;
     }
@@ -558,6 +566,8 @@ int gpm_create(const gpm_config *cfg, gpm_ctx **out)
     hipMemsetAsync(c->err, 0, sizeof(int), c->stream);
     int one = 1;
     hipMemcpyAsync(c->stamp, &one, sizeof(int), hipMemcpyHostToDevice, c->stream);
+    hipMemcpyAsync(c->st_dev, c->st_host, sizeof(c->st_host), hipMemcpyHostToDevice, c->stream);
+    c->B.st = c->st_dev;
     if ((e = hipStreamSynchronize(c->stream)) != hipSuccess) {
         gpm_destroy(c);
         std::fprintf(stderr, "gpm_create: %s\n", hipGetErrorString(e));
@@ -586,7 +596,7 @@ int gpm_destroy(gpm_ctx *c)
 {
     if (!c) return GPM_OK;
     if (c->stream) hipStreamSynchronize(c->stream);
-    void *bufs[] = {c->dT, c->dU, c->dA, c->dB, c->dC, c->dD, c->dE, c->dF, c->ring, c->band,
+    void *bufs[] = {c->dT, c->dU, c->dA, c->dB, c->dC, c->dD, c->dE, c->dF, c->ring, c->band, c->st_dev,
                     c->gpo, c->slistA, c->slistB, c->mark, c->counters,
                     c->stamp, c->err, c->solo_p, c->stats, c->tstate, c->block_min, c->dt_log, c->done, c->pred, c->nnlist,
                     c->nncount, c->adj_ub, c->adj_G, c->adj_A, c->adj_tmp, c->adj_ordx, c->adj_ordy};
@@ -618,7 +628,8 @@ int gpm_boundary_c(const gpm_config *cfg, double t_rk, double *U)
 {
     if (!cfg || !U || cfg->lf < 8 || cfg->nf < 8) return GPM_EINVAL;
     const double dx = quad_ratio(cfg->lx, cfg->lf), dy = quad_ratio(cfg->ly, cfg->nf);
-    const Bnd B = make_bnd(cfg->lf, cfg->nf, cfg->bc_type, dx, dy);
+    double4 st[kBndStates];
+    const Bnd B = make_bnd(cfg->lf, cfg->nf, cfg->bc_type, dx, dy, st);
     std::vector<double> in((size_t)cfg->lf * cfg->nf * 4);
     extract_interior(U, in.data(), cfg->lf, cfg->nf);
     fill_ghosted(in.data(), U, B, dmr_xmin_top(t_rk, B.sqrt3));
@@ -664,7 +675,7 @@ int gpm_get_state(gpm_ctx *c, double *U, int32_t *cellgpo)
             std::memcpy(U + ((size_t)(j + 4) * W + 4) * 4, in.data() + (size_t)j * c->NX * 4, sizeof(double) * c->NX * 4);
         ring_to_ghosts(ring.data(), U, c->NX, c->NY);
     } else {
-        fill_ghosted(in.data(), U, c->B, 0.0);
+        fill_ghosted(in.data(), U, c->Bh, 0.0);
     }
     if (cellgpo) {
         for (int jj = 0; jj < H; ++jj)
@@ -733,7 +744,7 @@ int gpm_time_stepping(gpm_ctx *c, double *Uin, double *Uout, double t, double dt
     {
         std::vector<double> in((size_t)c->NX * c->NY * 4);
         extract_interior(Uin, in.data(), c->NX, c->NY);
-        fill_ghosted(in.data(), Uin, c->B, dmr_xmin_top(t, c->B.sqrt3));
+        fill_ghosted(in.data(), Uin, c->Bh, dmr_xmin_top(t, c->Bh.sqrt3));
     }
     rc = gpm_set_time(c, t);
     if (rc) return rc;
@@ -957,7 +968,7 @@ int gpm_frozen_run(gpm_ctx *c, const int8_t *masks_dev, const double *dts, int32
             s.apriori = 1;
             s.frozen = 1;
             s.pred = masks_dev + ((size_t)st * 3 + k) * n;
-            HIPCHK(c, launch_stage(c->mord, c->ngp, c->cfg.num_flux, s, (c->ntiles + 7) / 8 * 8, c->stream));
+            HIPCHK(c, launch_stage(c->mord, c->ngp, c->cfg.num_flux, false, s, (c->ntiles + 7) / 8 * 8, c->stream));
         }
         std::swap(c->dU, c->dA);
     }

@@ -85,15 +85,22 @@ __device__ __forceinline__ int bc_map(int i, int N, int bc, bool &flip)
 __device__ __forceinline__ int bc_x(int bc) { return bc == kKH ? kPeriodic : bc; }
 __device__ __forceinline__ int bc_y(int bc) { return bc == kKH ? kNeumann : bc; }
 
-__device__ __forceinline__ double4 load_bc(const double4 *__restrict__ U, int N, int bc, int i, int j)
+// A stage input's cell (i, j) of an NX x NY grid under a copy boundary
+// (Neumann, Periodic, Reflective, KH_BC): branch-free index arithmetic.
+__device__ __forceinline__ double4 load_copy(const double4 *__restrict__ U, int NX, int NY, int bc, int i, int j)
 {
     bool fx, fy;
-    int si = bc_map(i, N, bc_x(bc), fx);
-    int sj = bc_map(j, N, bc_y(bc), fy);
-    double4 v = U[(size_t)sj * N + si];
+    int si = bc_map(i, NX, bc_x(bc), fx);
+    int sj = bc_map(j, NY, bc_y(bc), fy);
+    double4 v = U[(size_t)sj * NX + si];
     if (fx) v.y = -v.y;
     if (fy) v.z = -v.z;
     return v;
+}
+
+__device__ __forceinline__ double4 load_bc(const double4 *__restrict__ U, int N, int bc, int i, int j)
+{
+    return load_copy(U, N, N, bc, i, j);
 }
 
 // primitive_to_conservative, physics.f90:146-160
@@ -106,7 +113,9 @@ __host__ __device__ __forceinline__ double4 p2c(double r, double vx, double vy, 
 __host__ __device__ __forceinline__ double mesh_c(int i, double d) { return (double)(i + 1) * d - d / 2; }
 
 // The boundary of a stage input on an NX x NY grid (lf x nf): the BC_type and
-// the fixed ghost states of the formula rules, computed on the host with
+// a pointer (device memory in kernels, host memory on the host; kept out of the
+// kernel argument so that indexing it does not copy the argument to scratch)
+// to the fixed ghost states of the formula rules, computed on the host with
 // primitive_to_conservative (BC.f90:60-381):
 //   Mach800_BC        st[0] jet (1.4, 0, 800, 1)
 //   DoubleMach800_BC  st[0] bottom jet (1.4, 0, 800, 1), st[1] top jet (1.4, 0, -800, 1)
@@ -116,8 +125,9 @@ __host__ __device__ __forceinline__ double mesh_c(int i, double d) { return (dou
 struct Bnd {
     int NX, NY, bc;
     double dx, dy, sqrt3;
-    double4 st[5];
+    const double4 *st;  // [5]
 };
+constexpr int kBndStates = 5;
 
 __host__ __device__ __forceinline__ bool index_map_bc(int bc)
 {

@@ -5,22 +5,22 @@ namespace gpm {
 
 #define GPM_COMBOS(X) X(3, 2) X(5, 3) X(7, 4)
 
-hipError_t launch_stage(int mord, int ngp, int flux, const StageArgs &a, int nblocks, hipStream_t s)
+hipError_t launch_stage(int mord, int ngp, int flux, bool gen, const StageArgs &a, int nblocks, hipStream_t s)
 {
-#define X(M, G) if (mord == M && ngp == G) return launch_stage_mg<M, G>(flux, a, nblocks, s);
+#define X(M, G) if (mord == M && ngp == G) return launch_stage_mg<M, G>(flux, gen, a, nblocks, s);
     GPM_COMBOS(X)
 #undef X
     return hipErrorInvalidValue;
 }
 
-const void *mood_kernel(int mord, int ngp, int flux)
+const void *mood_kernel(int mord, int ngp, int flux, bool gen)
 {
-#define X(M, G) if (mord == M && ngp == G) return mood_kernel_mg<M, G>(flux);
+#define X(M, G) if (mord == M && ngp == G) return mood_kernel_mg<M, G>(flux, gen);
     GPM_COMBOS(X)
 #undef X
     return nullptr;
 }
 
-bool supported(int mord, int ngp) { return mood_kernel(mord, ngp, kHLLC) != nullptr; }
+bool supported(int mord, int ngp) { return mood_kernel(mord, ngp, kHLLC, false) != nullptr; }
 
 }  // namespace gpm

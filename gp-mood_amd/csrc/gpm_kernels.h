@@ -175,13 +175,15 @@ struct RingArgs {
 };
 
 template <int MORD, int NGP>
-hipError_t launch_stage_mg(int flux, const StageArgs &a, int nblocks, hipStream_t s);
+hipError_t launch_stage_mg(int flux, bool gen, const StageArgs &a, int nblocks, hipStream_t s);
 template <int MORD, int NGP>
-const void *mood_kernel_mg(int flux);
+const void *mood_kernel_mg(int flux, bool gen);
 
-// flux: numFlux id (kHLLC, kLLF, kHLL, kUnsplit)
-hipError_t launch_stage(int mord, int ngp, int flux, const StageArgs &a, int nblocks, hipStream_t s);
-const void *mood_kernel(int mord, int ngp, int flux);
+// flux: numFlux id (kHLLC, kLLF, kHLL, kUnsplit); gen: the general boundary
+// kernels (formula boundaries, or problem RT's source term) instead of the
+// copy-boundary ones
+hipError_t launch_stage(int mord, int ngp, int flux, bool gen, const StageArgs &a, int nblocks, hipStream_t s);
+const void *mood_kernel(int mord, int ngp, int flux, bool gen);
 bool supported(int mord, int ngp);
 hipError_t launch_dt(const DtArgs &a, int nblocks, hipStream_t s);
 hipError_t launch_ring(const RingArgs &a, hipStream_t s);

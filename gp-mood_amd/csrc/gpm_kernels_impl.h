@@ -100,16 +100,28 @@ __device__ __forceinline__ double stage_out1(int kind, double un, double uin, do
     }
 }
 
+// A stage input's cell (i, j) with the boundary rule: GEN selects the
+// general rule (formula boundaries) over the copy boundaries' index map.
+template <bool GEN>
+__device__ __forceinline__ double4 load_stage(const double4 *__restrict__ U, const Bnd &B, int i, int j,
+                                              double xmin_top)
+{
+    if (GEN) return load_bnd(U, B, i, j, xmin_top);
+    return load_copy(U, B.NX, B.NY, B.bc, i, j);
+}
+
 // Writes the output of interior cell gc: the band copy of the Forward_Euler
 // value (before the source term), the gravity source, the RK combination and,
 // for kCmbRK4d, FU3.  un: the step-start state of the cell (loaded by the
-// caller when C.kind needs it), uin: the stage input.
+// caller when C.kind needs it), uin: the stage input.  The band and the source
+// term exist only with GEN (formula boundaries or problem RT).
+template <bool GEN>
 __device__ __forceinline__ void write_stage_output(const StepCtl &C, double4 *__restrict__ Uout, size_t gc,
                                                    bool band, const double4 &un, const double4 &uin,
                                                    double cand[4], double dtg)
 {
-    if (band) C.band[gc] = make_double4(cand[0], cand[1], cand[2], cand[3]);
-    if (C.gravity) gravity_source(cand, dtg);
+    if (GEN && band) C.band[gc] = make_double4(cand[0], cand[1], cand[2], cand[3]);
+    if (GEN && C.gravity) gravity_source(cand, dtg);
     if (C.kind == kCmbCopy) {
         Uout[gc] = make_double4(cand[0], cand[1], cand[2], cand[3]);
         return;
@@ -183,7 +195,7 @@ __device__ __forceinline__ void mark_neighbourhood(int i, int j, int NX, int NY,
     }
 }
 
-template <int MORD, int NGP, int FLUX>
+template <int MORD, int NGP, int FLUX, bool GEN>
 __global__ __launch_bounds__(kStageThreads, GPM_STAGE_WAVES_PER_EU) void k_stage(StageArgs a)
 {
     constexpr int H = kHalo<MORD>;
@@ -201,7 +213,7 @@ __global__ __launch_bounds__(kStageThreads, GPM_STAGE_WAVES_PER_EU) void k_stage
 
     for (int idx = tid; idx < PW * PH; idx += kStageThreads) {
         const int ti = idx % PW, tj = idx / PW;
-        sU[idx] = load_bnd(a.Uin, a.B, x0 - H + ti, y0 - H + tj, xmin_top);
+        sU[idx] = load_stage<GEN>(a.Uin, a.B, x0 - H + ti, y0 - H + tj, xmin_top);
     }
     // a priori orders of the tile and its one-cell ring (ghost cells: Mord)
     constexpr int OW = kTX + 2;
@@ -264,7 +276,7 @@ __global__ __launch_bounds__(kStageThreads, GPM_STAGE_WAVES_PER_EU) void k_stage
             if (me > 1 && !a.frozen) dec = decide(P, T, acc, cx + H, cy + H, cand);
             const size_t gc = (size_t)gj * NX + gi;
             const double4 un = a.C.kind != kCmbCopy ? a.C.Un[gc] : u;
-            write_stage_output(a.C, a.Uout, gc, a.C.band && in_band(gi, gj, NX, NY), un, u, cand, dtg);
+            write_stage_output<GEN>(a.C, a.Uout, gc, GEN && a.C.band && in_band(gi, gj, NX, NY), un, u, cand, dtg);
             // a failing cell takes the cascade at once
             a.gpo[gc] = pack_order(dec ? lower_order(me, a.cascade) : me, me, 0);
             if (dec) mark_neighbourhood(gi, gj, NX, NY, stamp1, a.mark, a.slist1, a.cnt_s1);
@@ -324,7 +336,7 @@ __device__ __forceinline__ void mark_neighbourhood_sc(int i, int j, int NX, int 
 // depend on the staged neighbourhood (the face's two order words, the cell's
 // own order word and step-start state, the next trip's list entry) are
 // issued together with the neighbourhood loads.
-template <int MORD, int NGP, int FLUX>
+template <int MORD, int NGP, int FLUX, bool GEN>
 __global__ __launch_bounds__(kMoodThreads) void k_mood(MoodArgs a)
 {
     constexpr int NGPP = NGP <= 1 ? 1 : NGP <= 2 ? 2 : NGP <= 4 ? 4 : 8;
@@ -400,7 +412,7 @@ __global__ __launch_bounds__(kMoodThreads) void k_mood(MoodArgs a)
             if (it + 1 < niter && tn < nS) dnext = S[tn];
             __syncthreads();  // the previous trip's readers are done with nb
             for (int q = gl; q < BOX * BOX; q += G)
-                nb[cs][q] = load_bnd(a.Uin, a.B, i - HB + q % BOX, j - HB + q / BOX, xmin_top);
+                nb[cs][q] = load_stage<GEN>(a.Uin, a.B, i - HB + q % BOX, j - HB + q / BOX, xmin_top);
             __syncthreads();
             double F[4] = {0.0, 0.0, 0.0, 0.0};
             if (valid && jg < NGP)
@@ -428,7 +440,7 @@ __global__ __launch_bounds__(kMoodThreads) void k_mood(MoodArgs a)
                 apply_fluxes(uin, fyb, fxl, fxr, fyt, dtdx, dtdy, cand);
                 bool dec = false;
                 if (me > 1) dec = decide(P, T, lacc, HB, HB, cand);
-                write_stage_output(a.C, a.Uout, d, a.C.band && in_band(i, j, NX, NY), un, uin, cand, dtg);
+                write_stage_output<GEN>(a.C, a.Uout, d, GEN && a.C.band && in_band(i, j, NX, NY), un, uin, cand, dtg);
                 if (dec) {
                     M::store(&a.gpo[d], pack_order(lower_order(me, a.cascade), me, p));
                     M::add(cnt_fail, 1);
@@ -506,26 +518,38 @@ __global__ __launch_bounds__(kMoodThreads) void k_mood(MoodArgs a)
     }
 }
 
-// Launch entry points of one (Mord, ngp) combination, for every numFlux; each
+// Launch entry points of one (Mord, ngp) combination, for every numFlux and
+// both boundary classes (gen: formula boundaries or a source term); each
 // combination is compiled in a translation unit of its own.
 #define GPM_INSTANTIATE_COMBO(M, G)                                                                          \
-    template <>                                                                                               \
-    hipError_t launch_stage_mg<M, G>(int flux, const StageArgs &a, int nblocks, hipStream_t s)               \
+    template <int F>                                                                                          \
+    static void launch_stage_f##M##G(bool gen, const StageArgs &a, int nblocks, hipStream_t s)               \
     {                                                                                                         \
-        if (flux == kHLL) hipLaunchKernelGGL((k_stage<M, G, kHLL>), dim3(nblocks), dim3(kStageThreads), 0, s, a); \
-        else if (flux == kLLF) hipLaunchKernelGGL((k_stage<M, G, kLLF>), dim3(nblocks), dim3(kStageThreads), 0, s, a); \
-        else if (flux == kUnsplit)                                                                            \
-            hipLaunchKernelGGL((k_stage<M, G, kUnsplit>), dim3(nblocks), dim3(kStageThreads), 0, s, a);       \
-        else hipLaunchKernelGGL((k_stage<M, G, kHLLC>), dim3(nblocks), dim3(kStageThreads), 0, s, a);       \
+        if (gen) hipLaunchKernelGGL((k_stage<M, G, F, true>), dim3(nblocks), dim3(kStageThreads), 0, s, a);  \
+        else hipLaunchKernelGGL((k_stage<M, G, F, false>), dim3(nblocks), dim3(kStageThreads), 0, s, a);     \
+    }                                                                                                         \
+    template <int F>                                                                                          \
+    static const void *mood_kernel_f##M##G(bool gen)                                                          \
+    {                                                                                                         \
+        return gen ? reinterpret_cast<const void *>(&k_mood<M, G, F, true>)                                  \
+                   : reinterpret_cast<const void *>(&k_mood<M, G, F, false>);                                \
+    }                                                                                                         \
+    template <>                                                                                               \
+    hipError_t launch_stage_mg<M, G>(int flux, bool gen, const StageArgs &a, int nblocks, hipStream_t s)     \
+    {                                                                                                         \
+        if (flux == kHLL) launch_stage_f##M##G<kHLL>(gen, a, nblocks, s);                                    \
+        else if (flux == kLLF) launch_stage_f##M##G<kLLF>(gen, a, nblocks, s);                               \
+        else if (flux == kUnsplit) launch_stage_f##M##G<kUnsplit>(gen, a, nblocks, s);                       \
+        else launch_stage_f##M##G<kHLLC>(gen, a, nblocks, s);                                                \
         return hipGetLastError();                                                                             \
     }                                                                                                         \
     template <>                                                                                               \
-    const void *mood_kernel_mg<M, G>(int flux)                                                                \
+    const void *mood_kernel_mg<M, G>(int flux, bool gen)                                                      \
     {                                                                                                         \
-        if (flux == kHLL) return reinterpret_cast<const void *>(&k_mood<M, G, kHLL>);                        \
-        if (flux == kLLF) return reinterpret_cast<const void *>(&k_mood<M, G, kLLF>);                        \
-        if (flux == kUnsplit) return reinterpret_cast<const void *>(&k_mood<M, G, kUnsplit>);                \
-        return reinterpret_cast<const void *>(&k_mood<M, G, kHLLC>);                                         \
+        if (flux == kHLL) return mood_kernel_f##M##G<kHLL>(gen);                                             \
+        if (flux == kLLF) return mood_kernel_f##M##G<kLLF>(gen);                                             \
+        if (flux == kUnsplit) return mood_kernel_f##M##G<kUnsplit>(gen);                                     \
+        return mood_kernel_f##M##G<kHLLC>(gen);                                                              \
     }
 
 }  // namespace gpm
