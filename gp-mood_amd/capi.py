@@ -17,6 +17,7 @@ except Exception:  # pragma: no cover - torch is part of the image
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.environ.get("GPM_LIB_PATH") or os.path.join(HERE, "libgpmood.so")
 
+FOG = 1
 GP_MOOD = 2
 NN_GP_MOOD = 203
 FE, SSP_RK2, SSP_RK3, SSP_RK4 = 8, 9, 10, 12

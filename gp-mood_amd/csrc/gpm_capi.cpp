@@ -28,6 +28,7 @@ struct gpm_ctx {
     int NX = 0, NY = 0, mord = 0, ngp = 0, bc = 0, device = 0;
     int N = 0;  // NX where only square grids are supported (learned predictor, adjoint)
     int integrator = 0, nstages = 0, gravity = 0;
+    bool fog = false;        // method FOG: first-order Godunov, no detection
     bool ring_mode = false;  // ghost cells are not all copies of interior cells: keep them in `ring`
     Bnd B{}, Bh{};           // the boundary for kernels (states in st_dev) and for host code (st_host)
     double4 st_host[kBndStates] = {};
@@ -319,7 +320,7 @@ int plan_stages(gpm_ctx *c, StagePlan *P)
         st(n++, c->dA, c->dB, kCmbRK3b, 1);
         st(n++, c->dB, c->dA, kCmbRK3c, 2);
     }
-    if (c->ring_mode) P[n - 1].C.band = c->band;
+    if (c->ring_mode && !c->fog) P[n - 1].C.band = c->band;
     return n;
 }
 
@@ -343,7 +344,8 @@ int enqueue_ring(gpm_ctx *c)
 {
     RingArgs r{};
     r.A = c->dU; r.band = c->band; r.ring = c->ring; r.tstate = c->tstate; r.B = c->B; r.gravity = c->gravity;
-    r.mode = c->integrator == GPM_FE ? kRingFE : c->integrator == GPM_SSP_RK2 ? kRingRK2 : kRingRK3;
+    r.T = c->dT; r.flux = c->cfg.num_flux;
+    r.mode = c->fog ? kRingFOG : c->integrator == GPM_FE ? kRingFE : c->integrator == GPM_SSP_RK2 ? kRingRK2 : kRingRK3;
     HIPCHK(c, launch_ring(r, c->stream));
     return GPM_OK;
 }
@@ -389,9 +391,12 @@ int enqueue_stages(gpm_ctx *c)
             c->ev_pairs.push_back({(int)i0, 0});
             HIPCHK(c, hipEventRecord(e0, c->stream));
         }
-        // a multiple of 8 workgroups, so the XCD-aware remap in k_stage is a bijection
-        HIPCHK(c, launch_stage(c->mord, c->ngp, c->cfg.num_flux, c->ring_mode, s, (c->ntiles + 7) / 8 * 8, c->stream));
+        // a multiple of 8 workgroups, so the XCD-aware remap in k_stage is a bijection;
+        // FOG runs the first-order kernel (Mord 1 template, one Gauss point)
+        HIPCHK(c, launch_stage(c->fog ? 1 : c->mord, c->fog ? 1 : c->ngp, c->cfg.num_flux, c->ring_mode, s,
+                               (c->ntiles + 7) / 8 * 8, c->stream));
         if (c->timing) HIPCHK(c, hipEventRecord(e1, c->stream));
+        if (c->fog) continue;  // one pass, no detection (mod_FE.f90:160-162)
 
         MoodArgs m{};
         m.Uin = in; m.Uout = plan[k].out; m.C = plan[k].C; m.B = c->B; m.gpo = c->gpo;
@@ -469,6 +474,7 @@ int gpm_create(const gpm_config *cfg, gpm_ctx **out)
     c->bc = cfg->bc_type;
     c->integrator = cfg->integrator;
     c->gravity = cfg->problem == GPM_PROBLEM_RT;
+    c->fog = cfg->method == GPM_FOG;
     auto bad = [&](int code, const char *m) {
         std::fprintf(stderr, "gpm_create: %s\n", m);
         delete c;
@@ -485,10 +491,14 @@ int gpm_create(const gpm_config *cfg, gpm_ctx **out)
     if (cfg->num_flux != GPM_HLLC && cfg->num_flux != GPM_LLF && cfg->num_flux != GPM_HLL &&
         cfg->num_flux != GPM_UNSPLIT)
         return bad(GPM_EUNSUP, "numFlux must be HLLC, LLF, HLL or unsplit");
-    if (cfg->method != GPM_GP_MOOD && cfg->method != GPM_NN_GP_MOOD)
-        return bad(GPM_EUNSUP, "method must be GP_MOOD or NN_GP_MOOD");
+    if (cfg->method != GPM_GP_MOOD && cfg->method != GPM_NN_GP_MOOD && cfg->method != GPM_FOG)
+        return bad(GPM_EUNSUP, "method must be GP_MOOD, NN_GP_MOOD or FOG");
+    if (c->fog && (cfg->ngp != 1 || cfg->integrator != GPM_FE))
+        return bad(GPM_EUNSUP, "FOG needs ngp = 1 and integrator FE (the problems/*_FOG.f90 set-up)");
     if (!bc_ok) return bad(GPM_EUNSUP, "unknown BC_type");
-    c->ring_mode = !index_map_bc(cfg->bc_type) || c->gravity;
+    // FOG leaves the face loop's updates in the first ghost layer (no Boundary_C
+    // after the step), so its ghost cells are never copies of interior cells
+    c->ring_mode = !index_map_bc(cfg->bc_type) || c->gravity || c->fog;
     if (c->ring_mode && cfg->integrator == GPM_SSP_RK4)
         return bad(GPM_EUNSUP, "SSP_RK4 needs a copy boundary (Neumann, Periodic, Reflective, KH_BC) and no source term");
     const bool square_copy = cfg->lf == cfg->nf && index_map_bc(cfg->bc_type) && !c->gravity &&
@@ -497,7 +507,7 @@ int gpm_create(const gpm_config *cfg, gpm_ctx **out)
         return bad(GPM_EUNSUP, "NN_GP_MOOD needs lf = nf, a copy boundary, SSP_RK3 and no source term");
     if (cfg->cascade != GPM_CASCADE_REFERENCE && cfg->cascade != GPM_CASCADE_FULL)
         return bad(GPM_EINVAL, "unknown cascade");
-    if (!supported(cfg->mord, cfg->ngp)) return bad(GPM_EUNSUP, "(Mord, ngp) combination not compiled");
+    if (!c->fog && !supported(cfg->mord, cfg->ngp)) return bad(GPM_EUNSUP, "(Mord, ngp) combination not compiled");
     if ((long long)cfg->lf * cfg->nf > (1ll << 31) - 1) return bad(GPM_EUNSUP, "grid too large for int32 cell ids");
     // dx = real(Lx_16/lf, 8) (global_variables.f90:19-21)
     c->dx = quad_ratio(cfg->lx, cfg->lf);
@@ -548,8 +558,8 @@ int gpm_create(const gpm_config *cfg, gpm_ctx **out)
     if (const char *env = std::getenv("GPM_MOOD_BLOCKS")) want = std::atoi(env) > 0 ? std::atoi(env) : ncu;
     {
         int per = 0;
-        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, mood_kernel(c->mord, c->ngp, c->cfg.num_flux, c->ring_mode),
-                                                         kMoodThreads, 0);
+        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(
+            &per, mood_kernel(c->fog ? 1 : c->mord, c->fog ? 1 : c->ngp, c->cfg.num_flux, c->ring_mode), kMoodThreads, 0);
         if (e != hipSuccess || per <= 0) per = 1;
         c->mood_blocks = want < per * ncu ? want : per * ncu;
     }
@@ -681,7 +691,8 @@ int gpm_get_state(gpm_ctx *c, double *U, int32_t *cellgpo)
         for (int jj = 0; jj < H; ++jj)
             for (int ii = 0; ii < W; ++ii) {
                 const int i = ii - 4, j = jj - 4;
-                cellgpo[(size_t)jj * W + ii] = (i < 0 || i >= c->NX || j < 0 || j >= c->NY)
+                // FOG leaves CellGPO = Mord (mod_FE.f90:48)
+                cellgpo[(size_t)jj * W + ii] = (c->fog || i < 0 || i >= c->NX || j < 0 || j >= c->NY)
                                                    ? c->mord
                                                    : (int32_t)order_of_word(g[(size_t)j * c->NX + i]);
             }
@@ -707,7 +718,8 @@ int gpm_get_state_device(gpm_ctx *c, double *u_dev, int8_t *order_dev)
     if (!c->has_state) return fail(c, GPM_ESTATE, "no state set");
     const size_t n = (size_t)c->NX * c->NY;
     if (u_dev) HIPCHK(c, hipMemcpyAsync(u_dev, c->dU, n * 4 * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
-    if (order_dev) HIPCHK(c, launch_unpack_orders(c->gpo, order_dev, n, c->stream));
+    if (order_dev && c->fog) HIPCHK(c, hipMemsetAsync(order_dev, c->mord, n, c->stream));
+    else if (order_dev) HIPCHK(c, launch_unpack_orders(c->gpo, order_dev, n, c->stream));
     return GPM_OK;
 }
 

@@ -3,7 +3,8 @@
 
 namespace gpm {
 
-#define GPM_COMBOS(X) X(3, 2) X(5, 3) X(7, 4)
+// (1, 1) is the first-order Godunov method (FOG), the others GP-MOOD.
+#define GPM_COMBOS(X) X(1, 1) X(3, 2) X(5, 3) X(7, 4)
 
 hipError_t launch_stage(int mord, int ngp, int flux, bool gen, const StageArgs &a, int nblocks, hipStream_t s)
 {
@@ -21,6 +22,6 @@ const void *mood_kernel(int mord, int ngp, int flux, bool gen)
     return nullptr;
 }
 
-bool supported(int mord, int ngp) { return mood_kernel(mord, ngp, kHLLC, false) != nullptr; }
+bool supported(int mord, int ngp) { return mord > 1 && mood_kernel(mord, ngp, kHLLC, false) != nullptr; }
 
 }  // namespace gpm

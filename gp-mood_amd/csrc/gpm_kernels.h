@@ -163,6 +163,7 @@ enum {
     kRingFE,        // S(G_t0(B))
     kRingRK2,       // 0.5*(G_t0(Un) + S(G_t0(B)))
     kRingRK3,       // (1./3)*G_t0(Un) + (2./3)*S(G_{t0+dt/2}(B))
+    kRingFOG,       // S(G_t0(Un) + the face loop's updates of the first ghost layer), FE with FOG
 };
 
 struct RingArgs {
@@ -171,7 +172,8 @@ struct RingArgs {
     double4 *__restrict__ ring;
     const double *__restrict__ tstate;
     Bnd B;
-    int mode, gravity;
+    const DevTables *__restrict__ T;  // kRingFOG: gauss_weight(1, 1)
+    int mode, gravity, flux;
 };
 
 template <int MORD, int NGP>

@@ -168,6 +168,21 @@ __device__ __forceinline__ void apply_fluxes(const double4 &u, const double fyb[
     }
 }
 
+// Face flux of the first-order Godunov method (FOG_, mod_FOG.f90:11-25): the
+// face states are the two cells' values themselves (no reconstruction sum), one
+// Gauss point, L = 0 + F * gauss_weight(1, 1) (mod_FE.f90:80-98).
+template <int DIR, int FLUX>
+__device__ __forceinline__ void fog_flux(const DevTables *__restrict__ T, const double4 &a, const double4 &b,
+                                         double L[4])
+{
+    const double ua[4] = {a.x, a.y, a.z, a.w}, ub[4] = {b.x, b.y, b.z, b.w};
+    double F[4];
+    riemann<FLUX, DIR>(ua, ub, F);
+    const double w = T->gw[0];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) L[k] = 0.0 + F[k] * w;
+}
+
 // XCD-aware tile order: consecutive workgroups are dispatched round-robin over
 // the 8 XCDs; give each XCD a contiguous band of tiles so halo reads of
 // neighbouring tiles hit the same L2.  nb (the grid size) is a multiple of 8,
@@ -233,7 +248,9 @@ __global__ __launch_bounds__(kStageThreads, GPM_STAGE_WAVES_PER_EU) void k_stage
         double L[4];
         if (f < NFX) {
             const int fy = f / (kTX + 1), fx = f % (kTX + 1);
-            if (!a.apriori) {
+            if (MORD == 1) {  // FOG
+                fog_flux<0, FLUX>(T, acc(fx - 1 + H, fy + H), acc(fx + H, fy + H), L);
+            } else if (!a.apriori) {
                 face_flux<MORD, NGP, 0, FLUX>(T, acc, fx - 1 + H, fy + H, fx + H, fy + H, L);
             } else {
                 const int o = min(sO[(fy + 1) * OW + fx], sO[(fy + 1) * OW + fx + 1]);
@@ -243,7 +260,9 @@ __global__ __launch_bounds__(kStageThreads, GPM_STAGE_WAVES_PER_EU) void k_stage
         } else {
             const int g = f - NFX;
             const int gy = g / kTX, gx = g % kTX;
-            if (!a.apriori) {
+            if (MORD == 1) {
+                fog_flux<1, FLUX>(T, acc(gx + H, gy - 1 + H), acc(gx + H, gy + H), L);
+            } else if (!a.apriori) {
                 face_flux<MORD, NGP, 1, FLUX>(T, acc, gx + H, gy - 1 + H, gx + H, gy + H, L);
             } else {
                 const int o = min(sO[gy * OW + gx + 1], sO[(gy + 1) * OW + gx + 1]);

@@ -30,6 +30,7 @@ extern "C" {
 #endif
 
 /* Identifiers shared with the reference's constants.f90:16-34. */
+#define GPM_FOG 1
 #define GPM_GP_MOOD 2
 #define GPM_NN_GP_MOOD 203
 #define GPM_FE 8
@@ -69,7 +70,8 @@ typedef struct gpm_config {
     int32_t lf, nf;       /* interior cells in x and y (>= 8 each) */
     int32_t mord;         /* Mord: 1, 3, 5 or 7 */
     int32_t ngp;          /* Gauss points per face: 1..5 */
-    int32_t method;       /* GPM_GP_MOOD or GPM_NN_GP_MOOD (lf == nf, copy boundary, SSP_RK3) */
+    int32_t method;       /* GPM_GP_MOOD; GPM_NN_GP_MOOD (lf == nf, copy boundary, SSP_RK3);
+                             GPM_FOG, first-order Godunov (mod_FOG.f90; ngp = 1, integrator FE) */
     int32_t integrator;   /* GPM_FE, GPM_SSP_RK2, GPM_SSP_RK3 or GPM_SSP_RK4 (time_stepper.f90:17-21;
                              SSP_RK4 only with a copy boundary and no source term) */
     int32_t num_flux;     /* GPM_HLLC (mod_HLLC.f90), GPM_LLF (mod_LLF.f90), GPM_HLL (mod_HLL.f90)

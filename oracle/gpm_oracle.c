@@ -7,7 +7,7 @@
  *
  * It restates, operation for operation, the FP64 arithmetic of the reference
  * Fortran (slug-cfd/gp-mood, gp-mood-code/) for
- *   method = GP_MOOD,
+ *   method in {GP_MOOD, FOG (ngp = 1)},
  *   integrator in {FE, SSP_RK2, SSP_RK3, SSP_RK4},
  *   numFlux in {HLLC, LLF, HLL, unsplit},
  *   BC_type in {Neumann, Periodic, Dirichlet, Reflective, KH_BC, Mach800_BC,
@@ -44,12 +44,14 @@ enum {
     MACH800_BC = 201, DOUBLEMACH800_BC = 301, RMI_BC = 401, DMR_BC = 35, RT_BC = 402
 };
 enum { PROBLEM_RT = 51 };
+enum { METHOD_FOG = 1, METHOD_GP_MOOD = 2 };
 
 typedef struct {
     int lf, nf, mord, ngp, bc; /* bc: the reference's BC_type id */
     int flux;                  /* numFlux id */
     int integrator;            /* integrator id */
     int gravity;               /* problem == RT: source term after MOOD (mod_FE.f90:183-186) */
+    int method;                /* METHOD_GP_MOOD or METHOD_FOG */
     int dmp, u2, u2tol;        /* MOOD switches (problems/<name>.f90: DMP, U2, U2_tol) */
     int cascade;               /* 0: reference (>3 -> 3, 3 -> 1); 1: full (7->5->3->1) */
     double cfl, dx, dy;
@@ -110,6 +112,7 @@ orc_t *orc_create(int lf, int nf, int mord, int ngp, int bc, int dmp, int u2, in
     o->lf = lf; o->nf = nf; o->mord = mord; o->ngp = ngp; o->bc = bc;
     o->flux = HLLC;
     o->integrator = SSP_RK3;
+    o->method = METHOD_GP_MOOD;
     o->dmp = dmp; o->u2 = u2; o->u2tol = u2tol; o->cascade = cascade;
     o->cfl = cfl; o->dx = dx; o->dy = dy;
     o->sz = stcl_sz(mord);
@@ -648,6 +651,31 @@ static void face_flux(const orc_t *o, int dir, int l, int n, double *Lf)
    start of DETECTION, so their face-loop updates are not formed.  After MOOD:
    the gravity source of problem RT (:183-186) and, for SSP_RK4, Uout - Uin
    (:187), both over the whole array. */
+/* Face loop of mod_FE.f90:75-145 for FOG (FOG_, mod_FOG.f90:11-25: the face
+   states at Gauss point 1 are the cell values of cells 0..lf+1, 0..nf+1; ngp =
+   1): every face flagged, taken in the reference's n-outer / l-inner order, so
+   the ghost cells of the first layer get their updates too. */
+static void fog_face_loop(orc_t *o, const double *Uin, double *Uout, double dtdx, double dtdy)
+{
+    const double w = GW(o, 1, 0);
+    for (int n = 0; n <= o->nf; ++n)
+        for (int l = 0; l <= o->lf; ++l) {
+            double F[4];
+            riemann(o->flux, Uin + CELL(o, l, n) * 4, Uin + CELL(o, l + 1, n) * 4, 0, F);
+            for (int k = 0; k < 4; ++k) {
+                const double Lk = 0.0 + F[k] * w;
+                Uout[CELL(o, l, n) * 4 + k] = Uout[CELL(o, l, n) * 4 + k] - dtdx * Lk;
+                Uout[CELL(o, l + 1, n) * 4 + k] = Uout[CELL(o, l + 1, n) * 4 + k] + dtdx * Lk;
+            }
+            riemann(o->flux, Uin + CELL(o, l, n) * 4, Uin + CELL(o, l, n + 1) * 4, 1, F);
+            for (int k = 0; k < 4; ++k) {
+                const double Lk = 0.0 + F[k] * w;
+                Uout[CELL(o, l, n) * 4 + k] = Uout[CELL(o, l, n) * 4 + k] - dtdy * Lk;
+                Uout[CELL(o, l, n + 1) * 4 + k] = Uout[CELL(o, l, n + 1) * 4 + k] + dtdy * Lk;
+            }
+        }
+}
+
 static void orc_fe(orc_t *o, double *Uin, double *Uout, double dt, int first, int *npass)
 {
     const int lf = o->lf, nf = o->nf;
@@ -664,7 +692,13 @@ static void orc_fe(orc_t *o, double *Uin, double *Uout, double dt, int first, in
     memcpy(Uout, Uin, ncell * 4 * sizeof(double));
     const double dtdx = dt / o->dx, dtdy = dt / o->dy;
     int passes = 0;
-    for (;;) {
+    if (o->method == METHOD_FOG) {
+        /* one pass; MOOD_finished without DETECTION (mod_FE.f90:160-162), so
+           Uout's ghost cells keep Uin's boundary values plus the face updates */
+        fog_face_loop(o, Uin, Uout, dtdx, dtdy);
+        passes = 1;
+    }
+    for (; o->method != METHOD_FOG;) {
         ++passes;
         gp_mood_recons(o, Uin);
 #pragma omp parallel for schedule(static)
@@ -830,6 +864,7 @@ void orc_riemann(int flux, const double *ul, const double *ur, int dir, double *
 void orc_set_flux(orc_t *o, int flux) { o->flux = flux; }
 void orc_set_integrator(orc_t *o, int integrator) { o->integrator = integrator; }
 void orc_set_problem(orc_t *o, int problem) { o->gravity = problem == PROBLEM_RT; }
+void orc_set_method(orc_t *o, int method) { o->method = method; }
 void orc_set_t_rk(orc_t *o, double t_rk) { o->t_rk = t_rk; }
 
 /* Fixed a priori orders for every stage (NULL restores CellGPO = Mord). */

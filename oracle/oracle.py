@@ -16,6 +16,7 @@ BC_CODES = {"neumann": 17, "periodic": 18, "dirichlet": 19, "reflective": 20, "k
 FLUX_CODES = {"hllc": 100, "llf": 101, "hll": 102, "unsplit": 103}
 INTEGRATORS = {"fe": 8, "ssp_rk2": 9, "ssp_rk3": 10, "ssp_rk4": 12}
 PROBLEM_RT = 51
+METHODS = {"gp_mood": 2, "fog": 1}
 CASCADES = {"reference": 0, "full": 1}
 
 _lib = None
@@ -34,6 +35,7 @@ def lib():
         L.orc_set_flux.argtypes = [P, I]
         L.orc_set_integrator.argtypes = [P, I]
         L.orc_set_problem.argtypes = [P, I]
+        L.orc_set_method.argtypes = [P, I]
         L.orc_set_t_rk.argtypes = [P, D]
         L.orc_riemann.argtypes = [I, P, P, I, P]
         L.orc_bc.argtypes = [P, P]
@@ -64,7 +66,7 @@ class Oracle:
 
     def __init__(self, n, mord, ngp, bc, tables, dmp=True, u2=True, u2tol=True,
                  cascade="reference", cfl=0.8, flux="hllc", nf=None, lx=1.0, ly=1.0,
-                 integrator="ssp_rk3", problem=0, dx=None, dy=None):
+                 integrator="ssp_rk3", problem=0, dx=None, dy=None, method="gp_mood"):
         self.n = self.lf = n
         self.nf = n if nf is None else nf
         self.tables = {k: np.ascontiguousarray(v) for k, v in tables.items()}
@@ -80,6 +82,7 @@ class Oracle:
         lib().orc_set_flux(self._h, FLUX_CODES[flux])
         lib().orc_set_integrator(self._h, INTEGRATORS[integrator])
         lib().orc_set_problem(self._h, int(problem))
+        lib().orc_set_method(self._h, METHODS[method])
 
     def __del__(self):
         if getattr(self, "_h", None):

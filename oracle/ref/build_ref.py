@@ -106,6 +106,14 @@ CONFIGS = {
     "r1_dmr_64x16_rk2": (64, 3, 2, "dmr", True, True, True),
     "r1_m800_32_fe": (32, 3, 2, "mach800", True, True, True, "hll"),
     "r1_rt_16x48_rk2": (16, 3, 2, "rt", True, True, True, "unsplit"),
+    # first-order Godunov (method = FOG, ngp = 1, FE) as the problems/*_FOG.f90 files
+    "fog_n64_neu_uns": (64, 3, 1, "neumann", True, True, True, "unsplit"),
+    "fog_n64_kh_uns": (64, 3, 1, "kh", True, True, True, "unsplit"),
+    "fog_m800_32_hll": (32, 3, 1, "mach800", True, True, True, "hll"),
+    "fog_n64_per_hll": (64, 3, 1, "periodic", True, True, True, "hll"),
+    "fog_n32_ref_hllc": (32, 3, 1, "reflective", True, True, True),
+    "fog_dmr_64x16": (64, 3, 1, "dmr", True, True, True),
+    "fog_rt_16x48_uns": (16, 3, 1, "rt", True, True, True, "unsplit"),
 }
 
 # grid, domain, integrator and problem of the configurations that differ from
@@ -126,12 +134,19 @@ EXTRAS = {
     "r1_dmr_64x16_rk2": dict(nf=16, lx="4.", ly="1.", integrator="ssp_rk2"),
     "r1_m800_32_fe": dict(lx="1.5", ly="1.5", integrator="fe"),
     "r1_rt_16x48_rk2": dict(nf=48, lx="0.5", ly="1.5", problem="RT", integrator="ssp_rk2"),
+    "fog_n64_neu_uns": dict(integrator="fe", method="FOG"),
+    "fog_n64_kh_uns": dict(integrator="fe", method="FOG"),
+    "fog_m800_32_hll": dict(lx="1.5", ly="1.5", integrator="fe", method="FOG"),
+    "fog_n64_per_hll": dict(integrator="fe", method="FOG"),
+    "fog_n32_ref_hllc": dict(integrator="fe", method="FOG"),
+    "fog_dmr_64x16": dict(nf=16, lx="4.", ly="1.", integrator="fe", method="FOG"),
+    "fog_rt_16x48_uns": dict(nf=48, lx="0.5", ly="1.5", problem="RT", integrator="fe", method="FOG"),
 }
 
 
 def extras(name):
-    """nf, Lx, Ly (as Fortran literals), integrator and problem of a configuration."""
-    e = dict(nf=CONFIGS[name][0], lx="1.", ly="1.", integrator="ssp_rk3", problem="RP_2D_3")
+    """nf, Lx, Ly (as Fortran literals), integrator, problem and method of a configuration."""
+    e = dict(nf=CONFIGS[name][0], lx="1.", ly="1.", integrator="ssp_rk3", problem="RP_2D_3", method="GP_MOOD")
     e.update(EXTRAS.get(name, {}))
     return e
 
@@ -144,7 +159,7 @@ def config(name):
 # configs for which the Fortran drop-in (reference modules + product
 # time_stepper over libgpmood) is also built
 DROPIN_CONFIGS = ["r2_n64_neu", "r3_n64_ref", "r3_n64_kh_uns", "r1_dmr_64x16", "r1_rt_16x48", "r1_m800_32_fe",
-                  "r2_n32_ref_rk4"]
+                  "r2_n32_ref_rk4", "fog_n64_kh_uns", "fog_m800_32_hll"]
 
 PARAMS_TEMPLATE = """module parameters
    use constants
@@ -153,7 +168,7 @@ PARAMS_TEMPLATE = """module parameters
    integer , parameter :: integrator    = {integrator}
    logical , parameter :: restart = .false.
    character(100) :: restart_filename='.'
-   integer, parameter :: method   = GP_MOOD
+   integer, parameter :: method   = {method}
    real(16), parameter :: ell_o_dx = 12.0
    integer, parameter :: Mord= {mord}
    integer, parameter :: ngp = {ngp}
@@ -198,7 +213,8 @@ def build(name, dropin=False):
         f.write(PARAMS_TEMPLATE.format(mord=mord, ngp=ngp, dmp=fbool(dmp), u2=fbool(u2),
                                        u2tol=fbool(u2tol), n=n, bc=BC_NAMES[bc], flux=FLUX_NAMES[flux],
                                        nf=e["nf"], lx=e["lx"], ly=e["ly"], problem=e["problem"],
-                                       integrator=INTEGRATOR_NAMES[e["integrator"]], nn_l=min(32, n)))
+                                       integrator=INTEGRATOR_NAMES[e["integrator"]], nn_l=min(32, n),
+                                       method=e["method"]))
     flags = ["-fdefault-real-8", "-O3", "-ffp-contract=off", "-J", out, "-I", out]
     if n >= 1024:
         flags.append("-mcmodel=medium")

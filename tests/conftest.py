@@ -61,7 +61,8 @@ def case_options(z):
     return dict(nf=int(z["nf"]) if "nf" in z else n, lx=float(z["lx"]) if "lx" in z else 1.0,
                 ly=float(z["ly"]) if "ly" in z else 1.0, flux=str(z["flux"]) if "flux" in z else "hllc",
                 integrator=str(z["integrator"]) if "integrator" in z else "ssp_rk3",
-                problem=int(z["problem"]) if "problem" in z else 0)
+                problem=int(z["problem"]) if "problem" in z else 0,
+                method=str(z["method"]) if "method" in z else "gp_mood")
 
 
 @pytest.fixture(scope="session")

@@ -112,6 +112,14 @@ STEP_CASES = {
     "r1_rt_16x48_q35": ("r1_rt_16x48", "quadrants:0", 35, 20),
     "r2_rt_16x48_q36": ("r2_rt_16x48", "quadrants:0", 36, 20),
     "r1_rt_16x48_rk2_q37": ("r1_rt_16x48_rk2", "quadrants:0", 37, 20),
+    # first-order Godunov (FOG, FE)
+    "fog_n64_neu_uns_fq38": ("fog_n64_neu_uns", "four_quadrant", 38, 30),
+    "fog_n64_kh_uns_sh39": ("fog_n64_kh_uns", "shear_layer", 39, 30),
+    "fog_m800_32_hll_n40": ("fog_m800_32_hll", "mach800:0.02", 40, 30),
+    "fog_n64_per_hll_fq41": ("fog_n64_per_hll", "four_quadrant", 41, 30),
+    "fog_n32_ref_hllc_bl42": ("fog_n32_ref_hllc", "blast", 42, 30),
+    "fog_dmr_64x16": ("fog_dmr_64x16", "dmr:0", 0, 30),
+    "fog_rt_16x48_uns_q43": ("fog_rt_16x48_uns", "quadrants:0", 43, 30),
 }
 # long runs stored as checksums: name: (config, IC kind, seed, steps)
 HASH_CASES = {
@@ -140,7 +148,7 @@ def main(argv):
         np.savez_compressed(
             os.path.join(OUT, f"step_{name}.npz"), n=n, mord=mord, ngp=ngp, bc=bc, dmp=dmp, u2=u2, u2tol=u2tol, flux=flux,
             nf=e["nf"], lx=float(e["lx"]), ly=float(e["ly"]), integrator=e["integrator"],
-            problem=51 if e["problem"] == "RT" else 0,
+            problem=51 if e["problem"] == "RT" else 0, method=e["method"].lower(),
             kind=kind, seed=seed, nsteps=nsteps, ic_sha=sha(U0), U0=U0,
             dt=r["steps"]["dt"], t=r["steps"]["t"], counts=r["steps"]["cnt"], masks=r["masks"], U=r["U"],
             **extra, **{k: v for k, v in r["tables"].items()})

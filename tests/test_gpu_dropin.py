@@ -18,7 +18,8 @@ pytestmark = pytest.mark.gpu
 CASES = [("r2_n64_neu", "step_r2_n64_neu_fq8.npz"), ("r3_n64_ref", "step_r3_n64_ref_bl6.npz"),
          ("r3_n64_kh_uns", "step_r3_n64_kh_uns_fq15.npz"), ("r1_dmr_64x16", "step_r1_dmr_64x16.npz"),
          ("r1_rt_16x48", "step_r1_rt_16x48_q35.npz"), ("r1_m800_32_fe", "step_r1_m800_32_fe_n33.npz"),
-         ("r2_n32_ref_rk4", "step_r2_n32_ref_rk4_bl30.npz")]
+         ("r2_n32_ref_rk4", "step_r2_n32_ref_rk4_bl30.npz"), ("fog_n64_kh_uns", "step_fog_n64_kh_uns_sh39.npz"),
+         ("fog_m800_32_hll", "step_fog_m800_32_hll_n40.npz")]
 STEP_TYPE = np.dtype([("dt", "<f8"), ("t", "<f8"), ("cnt", "<i4")])
 
 

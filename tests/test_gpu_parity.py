@@ -21,7 +21,8 @@ def make_config(gpm, z):
     o = case_options(z)
     return gpm.Config(lf=int(z["n"]), nf=o["nf"], Mord=int(z["mord"]), ngp=int(z["ngp"]), BC_type=str(z["bc"]),
                       DMP=bool(z["dmp"]), U2=bool(z["u2"]), U2_tol=bool(z["u2tol"]), numFlux=o["flux"],
-                      integrator=o["integrator"], problem=o["problem"], Lx=o["lx"], Ly=o["ly"])
+                      integrator=o["integrator"], problem=o["problem"], Lx=o["lx"], Ly=o["ly"],
+                      method=gpm.capi.FOG if o["method"] == "fog" else gpm.capi.GP_MOOD)
 
 
 def make_solver(gpm, z, tables=True):
@@ -36,7 +37,7 @@ def oracle_first_step(z, U):
     dx, dy = (float(v) for v in z["dxdy"])
     orc = O.Oracle(int(z["n"]), int(z["mord"]), int(z["ngp"]), str(z["bc"]), tables_from_npz(z),
                    dmp=bool(z["dmp"]), u2=bool(z["u2"]), u2tol=bool(z["u2tol"]), flux=o["flux"], nf=o["nf"],
-                   integrator=o["integrator"], problem=o["problem"], dx=dx, dy=dy)
+                   integrator=o["integrator"], problem=o["problem"], dx=dx, dy=dy, method=o["method"])
     Uo = U.copy()
     return orc.step(Uo, float(z["dt"][0]), 0.0), Uo
 

@@ -23,7 +23,7 @@ def make_oracle(O, z):
     dx, dy = (float(v) for v in z["dxdy"])
     return O.Oracle(int(z["n"]), int(z["mord"]), int(z["ngp"]), str(z["bc"]), tables_from_npz(z),
                     dmp=bool(z["dmp"]), u2=bool(z["u2"]), u2tol=bool(z["u2tol"]), flux=o["flux"], nf=o["nf"],
-                    integrator=o["integrator"], problem=o["problem"], dx=dx, dy=dy)
+                    integrator=o["integrator"], problem=o["problem"], dx=dx, dy=dy, method=o["method"])
 
 
 @pytest.mark.parametrize("case", STEP_CASES)
