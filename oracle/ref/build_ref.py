@@ -116,6 +116,37 @@ CONFIGS = {
     "fog_rt_16x48_uns": (16, 3, 1, "rt", True, True, True, "unsplit"),
 }
 
+# The problem files' own configurations (problems/*.f90: grid, domain, method,
+# integrator, numFlux, BC_type, Mord, ngp, DMP/U2 switches), for checksum runs.
+PROBLEM_FILES = {
+    # name: (problem file, lf, nf, Lx, Ly, method, integrator, numFlux, BC, Mord, ngp, DMP, problem)
+    "pf_2drp3": ("2DRP3_GPMOOD.f90", 256, 256, "1.", "1.", "GP_MOOD", "ssp_rk3", "unsplit", "neumann", 3, 2, True, "RP_2D_3"),
+    "pf_2drp3_nodmp": ("2DRP3_GPMOOD_noDMP.f90", 256, 256, "1.", "1.", "GP_MOOD", "ssp_rk3", "unsplit", "neumann", 3, 2,
+                       False, "RP_2D_3"),
+    "pf_2drp12": ("2DRP12.f90", 256, 256, "1.", "1.", "GP_MOOD", "ssp_rk3", "hllc", "neumann", 3, 2, True, "RP_2D_12"),
+    "pf_kh": ("KH_GPMOOD.f90", 256, 256, "1.", "1.", "GP_MOOD", "ssp_rk3", "unsplit", "kh", 3, 2, True, "KH"),
+    "pf_mach800": ("Mach800_GPMOOD.f90", 256, 256, "1.5", "1.5", "GP_MOOD", "ssp_rk3", "hll", "mach800", 3, 2, True,
+                   "Mach800"),
+    "pf_mach800_nodmp": ("Mach800_GPMOOD_noDMP.f90", 256, 256, "1.5", "1.5", "GP_MOOD", "ssp_rk3", "hll", "mach800", 3,
+                         2, False, "Mach800"),
+    "pf_rt": ("RT.f90", 64, 192, "0.5", "1.5", "GP_MOOD", "ssp_rk3", "unsplit", "rt", 3, 2, True, "RT"),
+    "pf_dmr": ("dmr.f90", 512, 128, "4.", "1.", "GP_MOOD", "ssp_rk3", "hllc", "dmr", 3, 2, True, "DMR"),
+    "pf_explosion": ("explosion.f90", 256, 256, "1.5", "1.5", "GP_MOOD", "ssp_rk3", "hllc", "neumann", 3, 2, True,
+                     "explosion"),
+    "pf_implosion": ("implosion.f90", 256, 256, "0.3", "0.3", "GP_MOOD", "ssp_rk3", "hllc", "reflective", 3, 2, True,
+                     "implosion"),
+    "pf_shu_osher": ("shu_osher_GPMOOD.f90", 512, 512, "28.2842712474619", "28.2842712474619", "GP_MOOD", "ssp_rk3",
+                     "hll", "periodic", 3, 2, True, "Shu_Osher_rotated"),
+    "pf_2drp3_fog": ("2DRP3_FOG.f90", 256, 256, "1.", "1.", "FOG", "fe", "unsplit", "neumann", 3, 1, True, "RP_2D_3"),
+    "pf_kh_fog": ("KH_FOG.f90", 256, 256, "1.", "1.", "FOG", "fe", "unsplit", "kh", 3, 1, True, "KH"),
+    "pf_mach800_fog": ("Mach800_FOG.f90", 256, 256, "1.5", "1.5", "FOG", "fe", "hll", "mach800", 3, 1, True, "Mach800"),
+    "pf_shu_osher_fog": ("shu_osher_FOG.f90", 512, 512, "28.2842712474619", "28.2842712474619", "FOG", "fe", "hll",
+                         "periodic", 3, 1, True, "Shu_Osher_rotated"),
+    "pf_sedov_fog": ("sedov_FOG.f90", 256, 256, "1.", "1.", "FOG", "fe", "unsplit", "neumann", 3, 1, True, "sedov"),
+}
+for _k, (_f, _lf, _nf, _lx, _ly, _m, _i, _fl, _bc, _mo, _ng, _dmp, _pb) in PROBLEM_FILES.items():
+    CONFIGS[_k] = (_lf, _mo, _ng, _bc, _dmp, _dmp, _dmp, _fl)  # the noDMP files also switch U2, U2_tol off
+
 # grid, domain, integrator and problem of the configurations that differ from
 # a square unit domain, SSP_RK3 and a problem without source term
 EXTRAS = {
@@ -148,6 +179,9 @@ def extras(name):
     """nf, Lx, Ly (as Fortran literals), integrator, problem and method of a configuration."""
     e = dict(nf=CONFIGS[name][0], lx="1.", ly="1.", integrator="ssp_rk3", problem="RP_2D_3", method="GP_MOOD")
     e.update(EXTRAS.get(name, {}))
+    if name in PROBLEM_FILES:
+        _, _, nf, lx, ly, method, integ, _, _, _, _, _, problem = PROBLEM_FILES[name]
+        e.update(nf=nf, lx=lx, ly=ly, method=method, integrator=integ, problem=problem)
     return e
 
 
@@ -216,7 +250,7 @@ def build(name, dropin=False):
                                        integrator=INTEGRATOR_NAMES[e["integrator"]], nn_l=min(32, n),
                                        method=e["method"]))
     flags = ["-fdefault-real-8", "-O3", "-ffp-contract=off", "-J", out, "-I", out]
-    if n >= 1024:
+    if n * extras(name)["nf"] >= 1024 * 1024:
         flags.append("-mcmodel=medium")
     objs = []
     tmpdir = tempfile.mkdtemp(prefix="pp_", dir=out)
