@@ -40,6 +40,7 @@ struct gpm_ctx {
     double4 *dU = nullptr, *dA = nullptr, *dB = nullptr;
     double4 *dC = nullptr, *dD = nullptr, *dE = nullptr, *dF = nullptr;  // SSP_RK4 stage states
     double4 *ring = nullptr, *band = nullptr;
+    StepExtra *ext = nullptr;  // [kMaxStages] in device memory (plan_stages' per-stage extras)
     order_t *gpo = nullptr;
     int *slistA = nullptr, *slistB = nullptr, *mark = nullptr;
     int *counters = nullptr, *stamp = nullptr, *err = nullptr, *solo_p = nullptr;
@@ -285,12 +286,15 @@ struct StagePlan {
 
 // The stages of one step of the configured integrator (time_stepper.f90:17-21)
 // on the context's buffers; the step's result lands in dA (swapped with dU
-// afterwards).  Returns the stage count.
-int plan_stages(gpm_ctx *c, StagePlan *P)
+// afterwards).  Returns the stage count.  With `extra`, also the per-stage
+// StepExtra (buffers that do not change from step to step).
+int plan_stages(gpm_ctx *c, StagePlan *P, StepExtra *extra = nullptr)
 {
+    StepExtra ex[kMaxStages] = {};
     auto st = [&](int k, const double4 *in, double4 *out, int kind, int trk) {
         P[k] = StagePlan{in, out, StepCtl{}};
         P[k].C.Un = c->dU;
+        P[k].C.ext = c->ext ? c->ext + k : nullptr;
         P[k].C.kind = kind;
         P[k].C.trk = trk;
         P[k].C.gravity = c->gravity;
@@ -309,18 +313,23 @@ int plan_stages(gpm_ctx *c, StagePlan *P)
         st(n++, c->dB, c->dC, kCmbRK4b, 0);
         st(n++, c->dC, c->dD, kCmbRK4c, 0);
         st(n++, c->dD, c->dE, kCmbRK4d, 0);
-        P[n - 1].C.W = c->dF;
+        ex[n - 1].W = c->dF;
         st(n++, c->dE, c->dA, kCmbRK4e, 0);
-        P[n - 1].C.X1 = c->dC;
-        P[n - 1].C.X2 = c->dD;
-        P[n - 1].C.X3 = c->dF;
+        ex[n - 1].X1 = c->dC;
+        ex[n - 1].X2 = c->dD;
+        ex[n - 1].X3 = c->dF;
         break;
     default:  // SSP_RK3, mod_SSP_RK3.f90:12-39
         st(n++, c->dU, c->dA, kCmbCopy, 0);
         st(n++, c->dA, c->dB, kCmbRK3b, 1);
         st(n++, c->dB, c->dA, kCmbRK3c, 2);
     }
-    if (c->ring_mode && !c->fog) P[n - 1].C.band = c->band;
+    if (c->ring_mode && !c->fog) {
+        ex[n - 1].band = c->band;
+        P[n - 1].C.band = 1;
+    }
+    if (extra)
+        for (int k = 0; k < kMaxStages; ++k) extra[k] = ex[k];
     return n;
 }
 
@@ -534,7 +543,7 @@ int gpm_create(const gpm_config *cfg, gpm_ctx **out)
              dalloc(&c->dF, n) == hipSuccess;
     if (ok && c->ring_mode)
         ok = dalloc(&c->ring, (size_t)ring_size(c->NX, c->NY)) == hipSuccess && dalloc(&c->band, n) == hipSuccess;
-    if (ok) ok = dalloc(&c->st_dev, kBndStates) == hipSuccess;
+    if (ok) ok = dalloc(&c->st_dev, kBndStates) == hipSuccess && dalloc(&c->ext, kMaxStages) == hipSuccess;
     if (!ok) {
         gpm_destroy(c);
         std::fprintf(stderr, "gpm_create: device allocation failed\n");
@@ -578,6 +587,12 @@ int gpm_create(const gpm_config *cfg, gpm_ctx **out)
     hipMemcpyAsync(c->stamp, &one, sizeof(int), hipMemcpyHostToDevice, c->stream);
     hipMemcpyAsync(c->st_dev, c->st_host, sizeof(c->st_host), hipMemcpyHostToDevice, c->stream);
     c->B.st = c->st_dev;
+    StepExtra ext_host[kMaxStages];
+    {
+        StagePlan plan[kMaxStages];
+        plan_stages(c, plan, ext_host);
+    }
+    hipMemcpyAsync(c->ext, ext_host, sizeof(ext_host), hipMemcpyHostToDevice, c->stream);
     if ((e = hipStreamSynchronize(c->stream)) != hipSuccess) {
         gpm_destroy(c);
         std::fprintf(stderr, "gpm_create: %s\n", hipGetErrorString(e));
@@ -606,7 +621,7 @@ int gpm_destroy(gpm_ctx *c)
 {
     if (!c) return GPM_OK;
     if (c->stream) hipStreamSynchronize(c->stream);
-    void *bufs[] = {c->dT, c->dU, c->dA, c->dB, c->dC, c->dD, c->dE, c->dF, c->ring, c->band, c->st_dev,
+    void *bufs[] = {c->dT, c->dU, c->dA, c->dB, c->dC, c->dD, c->dE, c->dF, c->ring, c->band, c->st_dev, c->ext,
                     c->gpo, c->slistA, c->slistB, c->mark, c->counters,
                     c->stamp, c->err, c->solo_p, c->stats, c->tstate, c->block_min, c->dt_log, c->done, c->pred, c->nnlist,
                     c->nncount, c->adj_ub, c->adj_G, c->adj_A, c->adj_tmp, c->adj_ordx, c->adj_ordy};

@@ -60,15 +60,23 @@ enum {
     kCmbRK4e,      // f2*X1 + f3*X2 + ff3*X3 + f4*Uin + ff4*(cand - Uin), X = (U2, U3, FU3)
 };
 
+// Buffers a stage uses only with SSP_RK4 or the ghost ring.  They live in
+// device memory behind one pointer (they never change after set-up), which
+// keeps the kernel argument, and the scalar registers it occupies, small.
+struct StepExtra {
+    const double4 *X1, *X2, *X3;  // kCmbRK4e operands: U2, U3, FU3
+    double4 *W;                   // kCmbRK4d: FU3
+    double4 *band;                // if set: pre-source FE values of the cells within 4 of the boundary
+};
+
 // Per-stage control of the step shared by k_stage and k_mood.
 struct StepCtl {
-    const double4 *__restrict__ Un;               // step-start state
-    const double4 *__restrict__ X1, *__restrict__ X2, *__restrict__ X3;  // kCmbRK4e operands
-    double4 *__restrict__ W;                      // kCmbRK4d: FU3
-    double4 *__restrict__ band;                   // if set: pre-source FE values of the cells within 4 of the boundary
-    int kind;                                     // kCmb*
-    int gravity;                                  // problem RT: source term after MOOD (mod_FE.f90:183-186)
-    int trk;                                      // t_rk of the stage: t (0), t+dt (1), t+0.5*dt (2)
+    const double4 *__restrict__ Un;  // step-start state
+    const StepExtra *ext;            // this stage's StepExtra (device memory)
+    int kind;                        // kCmb*
+    int gravity;                     // problem RT: source term after MOOD (mod_FE.f90:183-186)
+    int trk;                         // t_rk of the stage: t (0), t+dt (1), t+0.5*dt (2)
+    int band;                        // ext->band is set (the last stage of a ghost-ring step)
 };
 
 struct StageArgs {

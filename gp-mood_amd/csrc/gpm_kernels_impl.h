@@ -77,16 +77,14 @@ constexpr double kRK4c1 = 0.391752226571890, kRK4a20 = 0.444370493651235, kRK4a2
                  kRK4c4 = 0.544974750228521, kRK4f2 = 0.517231671970585, kRK4f3 = 0.096059710526147,
                  kRK4f4 = 0.386708617503269, kRK4ff3 = 0.063692468666290, kRK4ff4 = 0.226007483236906;
 
-// Stage output of one component (kCmb* of StepCtl) from the step-start value
-// un, the stage input uin and the Forward_Euler value c (source applied); x1..x3
-// are the kCmbRK4e operands; *w receives FU3 for kCmbRK4d.
-__device__ __forceinline__ double stage_out1(int kind, double un, double uin, double c, double x1, double x2,
-                                             double x3, double *w)
+// SSP_RK4 stage outputs (kCmbRK4*) of one component from the step-start value
+// un, the stage input uin and the Forward_Euler value c; x1..x3 are the
+// kCmbRK4e operands; *w receives FU3 for kCmbRK4d.  Kept out of the common
+// path: SSP_RK4 is the only integrator that needs it.
+__device__ __forceinline__ double rk4_out1(int kind, double un, double uin, double c, double x1, double x2, double x3,
+                                           double *w)
 {
     switch (kind) {
-    case kCmbRK3b: return 0.75 * un + 0.25 * c;
-    case kCmbRK3c: return (1.0 / 3) * un + (2.0 / 3) * c;
-    case kCmbRK2b: return 0.5 * (un + c);
     case kCmbRK4a: return un + kRK4c1 * (c - un);
     case kCmbRK4b: return kRK4a20 * un + kRK4a21 * uin + kRK4c2 * (c - uin);
     case kCmbRK4c: return kRK4a30 * un + kRK4a32 * uin + kRK4c3 * (c - uin);
@@ -95,9 +93,62 @@ __device__ __forceinline__ double stage_out1(int kind, double un, double uin, do
         *w = f;
         return kRK4a40 * un + kRK4a43 * uin + kRK4c4 * f;
     }
-    case kCmbRK4e: return kRK4f2 * x1 + kRK4f3 * x2 + kRK4ff3 * x3 + kRK4f4 * uin + kRK4ff4 * (c - uin);
-    default: return c;
+    default: return kRK4f2 * x1 + kRK4f3 * x2 + kRK4ff3 * x3 + kRK4f4 * uin + kRK4ff4 * (c - uin);
     }
+}
+
+// The combination kind of a stage: CMB when fixed at compile time (the
+// SSP-RK3 stages of the copy-boundary kernels), else the run-time C.kind.
+constexpr int kCmbRuntime = -1;
+template <int CMB>
+__device__ __forceinline__ int comb_kind(const StepCtl &C)
+{
+    return CMB == kCmbRuntime ? C.kind : CMB;
+}
+
+// Writes the output of interior cell gc: the band copy of the Forward_Euler
+// value (before the source term), the gravity source, the RK combination and,
+// for kCmbRK4d, FU3.  un: the step-start state of the cell (loaded by the
+// caller when the kind needs it), uin: the stage input.  The band and the
+// source term exist only with GEN (formula boundaries or problem RT).
+template <bool GEN, int CMB>
+__device__ __forceinline__ void write_stage_output(const StepCtl &C, double4 *__restrict__ Uout, size_t gc,
+                                                   bool band, const double4 &un, const double4 &uin,
+                                                   double cand[4], double dtg)
+{
+    if (GEN && band) C.ext->band[gc] = make_double4(cand[0], cand[1], cand[2], cand[3]);
+    if (GEN && C.gravity) gravity_source(cand, dtg);
+    const double u[4] = {un.x, un.y, un.z, un.w};
+    double o[4];
+    const int kind = comb_kind<CMB>(C);
+    switch (kind) {
+    case kCmbCopy:
+        Uout[gc] = make_double4(cand[0], cand[1], cand[2], cand[3]);
+        return;
+    case kCmbRK3b:
+#pragma unroll
+        for (int k = 0; k < 4; ++k) o[k] = 0.75 * u[k] + 0.25 * cand[k];
+        break;
+    case kCmbRK3c:
+#pragma unroll
+        for (int k = 0; k < 4; ++k) o[k] = (1.0 / 3) * u[k] + (2.0 / 3) * cand[k];
+        break;
+    case kCmbRK2b:
+#pragma unroll
+        for (int k = 0; k < 4; ++k) o[k] = 0.5 * (u[k] + cand[k]);
+        break;
+    default: {
+        const double v[4] = {uin.x, uin.y, uin.z, uin.w};
+        double4 x1 = make_double4(0, 0, 0, 0), x2 = x1, x3 = x1;
+        if (kind == kCmbRK4e) { x1 = C.ext->X1[gc]; x2 = C.ext->X2[gc]; x3 = C.ext->X3[gc]; }
+        const double a1[4] = {x1.x, x1.y, x1.z, x1.w}, a2[4] = {x2.x, x2.y, x2.z, x2.w}, a3[4] = {x3.x, x3.y, x3.z, x3.w};
+        double w[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) o[k] = rk4_out1(kind, u[k], v[k], cand[k], a1[k], a2[k], a3[k], &w[k]);
+        if (kind == kCmbRK4d) C.ext->W[gc] = make_double4(w[0], w[1], w[2], w[3]);
+    }
+    }
+    Uout[gc] = make_double4(o[0], o[1], o[2], o[3]);
 }
 
 // A stage input's cell (i, j) with the boundary rule: GEN selects the
@@ -108,32 +159,6 @@ __device__ __forceinline__ double4 load_stage(const double4 *__restrict__ U, con
 {
     if (GEN) return load_bnd(U, B, i, j, xmin_top);
     return load_copy(U, B.NX, B.NY, B.bc, i, j);
-}
-
-// Writes the output of interior cell gc: the band copy of the Forward_Euler
-// value (before the source term), the gravity source, the RK combination and,
-// for kCmbRK4d, FU3.  un: the step-start state of the cell (loaded by the
-// caller when C.kind needs it), uin: the stage input.  The band and the source
-// term exist only with GEN (formula boundaries or problem RT).
-template <bool GEN>
-__device__ __forceinline__ void write_stage_output(const StepCtl &C, double4 *__restrict__ Uout, size_t gc,
-                                                   bool band, const double4 &un, const double4 &uin,
-                                                   double cand[4], double dtg)
-{
-    if (GEN && band) C.band[gc] = make_double4(cand[0], cand[1], cand[2], cand[3]);
-    if (GEN && C.gravity) gravity_source(cand, dtg);
-    if (C.kind == kCmbCopy) {
-        Uout[gc] = make_double4(cand[0], cand[1], cand[2], cand[3]);
-        return;
-    }
-    double4 x1 = make_double4(0, 0, 0, 0), x2 = x1, x3 = x1;
-    if (C.kind == kCmbRK4e) { x1 = C.X1[gc]; x2 = C.X2[gc]; x3 = C.X3[gc]; }
-    double4 w;
-    Uout[gc] = make_double4(stage_out1(C.kind, un.x, uin.x, cand[0], x1.x, x2.x, x3.x, &w.x),
-                            stage_out1(C.kind, un.y, uin.y, cand[1], x1.y, x2.y, x3.y, &w.y),
-                            stage_out1(C.kind, un.z, uin.z, cand[2], x1.z, x2.z, x3.z, &w.z),
-                            stage_out1(C.kind, un.w, uin.w, cand[3], x1.w, x2.w, x3.w, &w.w));
-    if (C.kind == kCmbRK4d) C.W[gc] = w;
 }
 
 // Append the cells of the lanes with `flag` set to list/cnt: one atomic per
@@ -210,7 +235,7 @@ __device__ __forceinline__ void mark_neighbourhood(int i, int j, int NX, int NY,
     }
 }
 
-template <int MORD, int NGP, int FLUX, bool GEN>
+template <int MORD, int NGP, int FLUX, bool GEN, int CMB>
 __global__ __launch_bounds__(kStageThreads, GPM_STAGE_WAVES_PER_EU) void k_stage(StageArgs a)
 {
     constexpr int H = kHalo<MORD>;
@@ -224,7 +249,7 @@ __global__ __launch_bounds__(kStageThreads, GPM_STAGE_WAVES_PER_EU) void k_stage
     if (tile < 0 || tile >= a.ntiles) return;
     const int x0 = (tile % a.tiles_x) * kTX, y0 = (tile / a.tiles_x) * kTY;
     const int tid = threadIdx.x;
-    const double xmin_top = stage_xmin_top(a.B, a.tstate, a.C.trk);
+    const double xmin_top = GEN ? stage_xmin_top(a.B, a.tstate, a.C.trk) : 0.0;
 
     for (int idx = tid; idx < PW * PH; idx += kStageThreads) {
         const int ti = idx % PW, tj = idx / PW;
@@ -274,7 +299,7 @@ __global__ __launch_bounds__(kStageThreads, GPM_STAGE_WAVES_PER_EU) void k_stage
     __syncthreads();
 
     const double dtdx = a.tstate[2], dtdy = a.tstate[3];
-    const double dtg = a.tstate[1] * kGravity;
+    const double dtg = GEN ? a.tstate[1] * kGravity : 0.0;
     const MoodParams P{a.dmp, a.u2, a.u2tol, a.dx, a.dy};
     const int stamp1 = *a.stamp + 1;
     for (int c = tid; c < kTX * kTY; c += kStageThreads) {
@@ -294,8 +319,9 @@ __global__ __launch_bounds__(kStageThreads, GPM_STAGE_WAVES_PER_EU) void k_stage
             const int me = a.apriori ? (int)sO[(cy + 1) * OW + cx + 1] : MORD;
             if (me > 1 && !a.frozen) dec = decide(P, T, acc, cx + H, cy + H, cand);
             const size_t gc = (size_t)gj * NX + gi;
-            const double4 un = a.C.kind != kCmbCopy ? a.C.Un[gc] : u;
-            write_stage_output<GEN>(a.C, a.Uout, gc, GEN && a.C.band && in_band(gi, gj, NX, NY), un, u, cand, dtg);
+            const double4 un = comb_kind<CMB>(a.C) != kCmbCopy ? a.C.Un[gc] : u;
+            write_stage_output<GEN, CMB>(a.C, a.Uout, gc, GEN && a.C.band && in_band(gi, gj, NX, NY), un, u, cand,
+                                         dtg);
             // a failing cell takes the cascade at once
             a.gpo[gc] = pack_order(dec ? lower_order(me, a.cascade) : me, me, 0);
             if (dec) mark_neighbourhood(gi, gj, NX, NY, stamp1, a.mark, a.slist1, a.cnt_s1);
@@ -374,8 +400,8 @@ __global__ __launch_bounds__(kMoodThreads) void k_mood(MoodArgs a)
     const DevTables *__restrict__ T = a.T;
     const MoodParams P{a.dmp, a.u2, a.u2tol, a.dx, a.dy};
     const double dtdx = a.tstate[2], dtdy = a.tstate[3];
-    const double dtg = a.tstate[1] * kGravity;
-    const double xmin_top = stage_xmin_top(a.B, a.tstate, a.C.trk);
+    const double dtg = GEN ? a.tstate[1] * kGravity : 0.0;
+    const double xmin_top = GEN ? stage_xmin_top(a.B, a.tstate, a.C.trk) : 0.0;
     // face f of a cell (i, j): 0 bottom y-face, 1 left x-face, 2 right x-face, 3 top y-face
     const int fdir = (f == 1 || f == 2) ? 0 : 1;
     const int fa = fdir == 0 ? kR : kT, fb = fdir == 0 ? kL : kB;
@@ -459,7 +485,8 @@ __global__ __launch_bounds__(kMoodThreads) void k_mood(MoodArgs a)
                 apply_fluxes(uin, fyb, fxl, fxr, fyt, dtdx, dtdy, cand);
                 bool dec = false;
                 if (me > 1) dec = decide(P, T, lacc, HB, HB, cand);
-                write_stage_output<GEN>(a.C, a.Uout, d, GEN && a.C.band && in_band(i, j, NX, NY), un, uin, cand, dtg);
+                write_stage_output<GEN, kCmbRuntime>(a.C, a.Uout, d, GEN && a.C.band && in_band(i, j, NX, NY), un, uin,
+                                                     cand, dtg);
                 if (dec) {
                     M::store(&a.gpo[d], pack_order(lower_order(me, a.cascade), me, p));
                     M::add(cnt_fail, 1);
@@ -538,14 +565,26 @@ __global__ __launch_bounds__(kMoodThreads) void k_mood(MoodArgs a)
 }
 
 // Launch entry points of one (Mord, ngp) combination, for every numFlux and
-// both boundary classes (gen: formula boundaries or a source term); each
-// combination is compiled in a translation unit of its own.
+// both boundary classes (gen: formula boundaries or a source term); the
+// copy-boundary stage kernels are also specialised on the SSP-RK3 combination
+// kinds.  Each combination is compiled in a translation unit of its own.
 #define GPM_INSTANTIATE_COMBO(M, G)                                                                          \
+    template <int F, int C>                                                                                   \
+    static void launch_stage_fc##M##G(bool gen, const StageArgs &a, int nblocks, hipStream_t s)              \
+    {                                                                                                         \
+        if (gen)                                                                                              \
+            hipLaunchKernelGGL((k_stage<M, G, F, true, kCmbRuntime>), dim3(nblocks), dim3(kStageThreads), 0, s, a); \
+        else hipLaunchKernelGGL((k_stage<M, G, F, false, C>), dim3(nblocks), dim3(kStageThreads), 0, s, a);  \
+    }                                                                                                         \
     template <int F>                                                                                          \
     static void launch_stage_f##M##G(bool gen, const StageArgs &a, int nblocks, hipStream_t s)               \
     {                                                                                                         \
-        if (gen) hipLaunchKernelGGL((k_stage<M, G, F, true>), dim3(nblocks), dim3(kStageThreads), 0, s, a);  \
-        else hipLaunchKernelGGL((k_stage<M, G, F, false>), dim3(nblocks), dim3(kStageThreads), 0, s, a);     \
+        switch (gen ? kCmbRuntime : a.C.kind) {                                                               \
+        case kCmbCopy: launch_stage_fc##M##G<F, kCmbCopy>(gen, a, nblocks, s); break;                          \
+        case kCmbRK3b: launch_stage_fc##M##G<F, kCmbRK3b>(gen, a, nblocks, s); break;                          \
+        case kCmbRK3c: launch_stage_fc##M##G<F, kCmbRK3c>(gen, a, nblocks, s); break;                          \
+        default: launch_stage_fc##M##G<F, kCmbRuntime>(gen, a, nblocks, s);                                   \
+        }                                                                                                     \
     }                                                                                                         \
     template <int F>                                                                                          \
     static const void *mood_kernel_f##M##G(bool gen)                                                          \
