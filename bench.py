@@ -14,8 +14,10 @@ roofline of the dominant kernel (the first-pass stage kernel, timed with HIP
 events on its own stream), a CPU baseline (the reference Fortran, serial,
 timed on a one-step sample of the same workload on this host), the plain vs
 NN-GP-MOOD comparison at 2048^2 GP-R2 with the committed predictor weights,
-the host-array drop-in rate (PCIe round trip per step included) and the cost
-of the 512^2 initial-state sensitivity run (frozen-mask adjoint).
+the host-array drop-in rate (PCIe round trip per step included), the
+throughput of the reference's other options (fluxes, integrators, formula
+boundaries, the RT source, FOG) and the cost of the 512^2 initial-state
+sensitivity run (frozen-mask adjoint).
 """
 import argparse
 import importlib.util
@@ -189,6 +191,59 @@ def large_run(gpm, steps, n=8192, seed=0):
     return out
 
 
+# The reference's other options at GPU-scale grids: label, Config fields,
+# initial condition (ic.problem kind) and RK stages per step.
+OPTION_RUNS = [
+    ("hll_r3_2048", dict(lf=2048, Mord=7, ngp=4, BC_type="reflective", numFlux="hll"), "blast", 3),
+    ("unsplit_r3_2048", dict(lf=2048, Mord=7, ngp=4, BC_type="reflective", numFlux="unsplit"), "blast", 3),
+    ("llf_r3_2048", dict(lf=2048, Mord=7, ngp=4, BC_type="reflective", numFlux="llf"), "blast", 3),
+    ("ssp_rk2_r3_2048", dict(lf=2048, Mord=7, ngp=4, BC_type="reflective", integrator="ssp_rk2"), "blast", 2),
+    ("ssp_rk4_r3_2048", dict(lf=2048, Mord=7, ngp=4, BC_type="reflective", integrator="ssp_rk4"), "blast", 5),
+    ("dmr_r1_4096x1024", dict(lf=4096, nf=1024, Mord=3, ngp=2, BC_type="dmr", Lx=4.0, Ly=1.0), "dmr", 3),
+    ("mach800_r1_2048", dict(lf=2048, Mord=3, ngp=2, BC_type="mach800", numFlux="hll", Lx=1.5, Ly=1.5), "mach800", 3),
+    ("rt_r1_1024x3072", dict(lf=1024, nf=3072, Mord=3, ngp=2, BC_type="rt", numFlux="unsplit", Lx=0.5, Ly=1.5,
+                             problem=51), "rayleigh_taylor", 3),
+    ("fog_2048", dict(lf=2048, Mord=3, ngp=1, BC_type="kh", numFlux="unsplit", integrator="fe", method=1),
+     "quadrants", 1),
+]
+
+
+def options_measure(gpm, steps, warmup=3):
+    """Throughput of the reference's other options (fluxes, integrators,
+    formula boundaries, the RT source, FOG) at GPU-scale grids: cell-updates/s
+    per RK stage over `steps` device-resident steps, and the first-pass
+    kernel's average launch."""
+    import torch
+    out = {}
+    for label, kw, kind, stages in OPTION_RUNS:
+        cfg = gpm.Config(**kw)
+        lf, nf = cfg.lf, cfg.nf or cfg.lf
+        if kind == "blast":
+            U0 = gpm.ic.blast(lf, 0)
+        else:
+            U0 = gpm.ic.problem(kind, lf, nf, cfg.Lx, cfg.Ly, seed=0, noise=0.0)
+        s = gpm.Solver(cfg)
+        s.set_state(U0)
+        s.set_time(0.0)
+        s.run(warmup)
+        torch.cuda.synchronize()
+        s.reset_stats()
+        s.set_timing(True)
+        t0 = time.perf_counter()
+        s.run(steps)
+        torch.cuda.synchronize()
+        el = time.perf_counter() - t0
+        kt = s.kernel_time()
+        st = s.stats()
+        out[label] = {"cell_updates_per_s": stages * lf * nf * steps / el, "ms_per_step": 1e3 * el / steps,
+                      "stage_kernel_avg_ms": kt["stage_ms"], "mood_kernel_avg_ms": kt["mood_ms"],
+                      "max_passes": st["max_passes"], "grid": [lf, nf], "stages_per_step": stages}
+        s.close()
+        print(f"[bench]   {label}: {out[label]['cell_updates_per_s'] / 1e9:.3f} G cell-updates/s per stage",
+              file=sys.stderr, flush=True)
+    return out
+
+
 def nn_comparison(gpm, n, steps):
     """Plain GP-MOOD vs NN-GP-MOOD (committed predictor weights) on the
     2048^2 GP-R2 four-quadrant workload: throughput, second-pass cells and the
@@ -227,6 +282,8 @@ def main():
     ap.add_argument("--dropin-steps", type=int, default=3, help="host-array drop-in steps timed (0: skip)")
     ap.add_argument("--large-steps", type=int, default=0,
                     help="steps of the 8192^2 GP-R3 periodic shear-layer run, plain and NN-GP-MOOD (0: skip)")
+    ap.add_argument("--options-steps", type=int, default=20,
+                    help="steps of each run of the other flux, integrator, boundary and method options (0: skip)")
     ap.add_argument("--adjoint-steps", type=int, default=200,
                     help="K of the 512^2 GP-R2 initial-state sensitivity run (0: skip)")
     ap.add_argument("--calib-copy-mib", type=int, default=0,
@@ -319,6 +376,10 @@ def main():
     large = None
     if args.large_steps > 0:
         large = large_run(gpm, args.large_steps)
+    opts = None
+    if args.options_steps > 0:
+        print(f"[bench] other options, {args.options_steps} steps each", file=sys.stderr, flush=True)
+        opts = options_measure(gpm, args.options_steps)
     adj = None
     if args.adjoint_steps > 0:
         print(f"[bench] adjoint, K={args.adjoint_steps} at 512^2 GP-R2", file=sys.stderr, flush=True)
@@ -339,7 +400,7 @@ def main():
                  "second_pass_cells_per_step": st["second_pass_cells"] / max(1, args.steps),
                  "mood_kernel_avg_ms": kt["mood_ms"]},
         "roofline": roofline, "cpu_baseline": cpu, "nn_mood": nn, "host_array_drop_in": dropin, "adjoint": adj,
-        "large_run": large,
+        "large_run": large, "options": opts,
     }
     print(json.dumps(out))
 
