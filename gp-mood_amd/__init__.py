@@ -14,7 +14,7 @@ from dataclasses import dataclass
 
 import numpy as np
 
-from . import capi, ic  # noqa: F401
+from . import capi, h5io, ic  # noqa: F401
 from .nn import MLP  # noqa: F401
 from .capi import gp_tables  # noqa: F401
 
@@ -248,4 +248,4 @@ class Solver:
                                          capi.ptr(dts), int(dts.size), ctypes.c_void_p(ubar_ptr)), "gpm_adjoint")
 
 
-from . import adjoint  # noqa: E402,F401
+from . import adjoint, driver  # noqa: E402,F401

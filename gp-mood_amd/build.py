@@ -19,6 +19,10 @@ ARCH = os.environ.get("GPM_OFFLOAD_ARCH", "gfx950")
 
 HIP_SOURCES = sorted(f for f in os.listdir(CSRC) if f.endswith(".hip")) + ["gpm_capi.cpp", "gpm_dispatch.cpp"]
 HOST_SOURCES = ["gp_tables.cpp"]
+IO_LIB = os.path.join(HERE, "libgpmood_io.so")
+# HDF5 C library of the reference's file formats (libgpmood_io.so): HDF5_DIR,
+# else the image's conda prefix
+HDF5_DIR = os.environ.get("HDF5_DIR", "/opt/conda")
 
 HIPCC = shutil.which("hipcc") or "/opt/rocm/bin/hipcc"
 HIP_FLAGS = [
@@ -61,6 +65,22 @@ def build(verbose=True, jobs=None, out=None, defines=(), combos=None):
     return lib
 
 
+def build_io(verbose=True):
+    """libgpmood_io.so: the reference's HDF5 formats (host code).  Skipped,
+    with a message, when the HDF5 C headers are absent."""
+    inc, lib = os.path.join(HDF5_DIR, "include"), os.path.join(HDF5_DIR, "lib")
+    if not os.path.exists(os.path.join(inc, "hdf5.h")):
+        if verbose:
+            print(f"build: {inc}/hdf5.h not found; libgpmood_io.so (HDF5 formats) not built", file=sys.stderr)
+        return None
+    tmp = IO_LIB + ".tmp"
+    run(["g++", "-O2", "-fPIC", "-std=c++17", "-ffp-contract=off", "-shared", "-I" + inc,
+         os.path.join(CSRC, "gpm_io.cpp"), "-o", tmp, "-L" + lib, "-lhdf5", "-Wl,-rpath," + lib])
+    os.replace(tmp, IO_LIB)
+    return IO_LIB
+
+
 if __name__ == "__main__":
     build()
+    build_io()
     sys.exit(0)

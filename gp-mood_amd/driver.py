@@ -1,0 +1,68 @@
+"""The reference's driver loop (main.f90:112-215) over a Solver, with its
+outputs in the reference's HDF5 formats (h5io): snapshots at IO_freqStep /
+IO_freqTime, the per-step diagnostic series, and restart from a snapshot.
+
+Each step is gpm_run for one step (dtfinal = tmax - t, Setdt, the step,
+t = t + dt, all on the device), followed by reading the step's counters,
+as main.f90:145-167 records them.
+"""
+import os
+
+import numpy as np
+
+from . import h5io
+
+RK_FACTOR = {"fe": 1, "ssp_rk2": 2, "ssp_rk3": 3, "ssp_rk4": 5}  # main.f90:102-110
+
+
+def run_problem(solver, problem, tmax, nmax, outdir=".", IO_freqStep=-100, IO_freqTime=-1.0, restart=None,
+                nn_filename="", write_diagnostic=True, log=None):
+    """Runs `solver` from its current state as main.f90 does: while t < tmax
+    and niter <= nmax.  Writes output_<...>_<niter+100000>.h5 snapshots when
+    niter is a multiple of IO_freqStep (> 0) or when t crosses a multiple of
+    IO_freqTime (> 0, main.f90:185-192), and the diagnostic file at the end.
+    With `restart` (a snapshot path) the state, t and niter come from the file
+    (reader.f90) and the series start at iter_0 = niter + 1.  Note: the
+    reference keeps the ghost cells of its initial condition across a restart
+    read (main.f90:124-136), which for formula boundaries the first Setdt
+    sees; here they come from Boundary_C of the restart state at its time.
+    Returns (t, niter, list of snapshot paths)."""
+    cfg = solver.config
+    nf = cfg.lf if cfg.nf is None else cfg.nf
+    m = h5io.meta(cfg.lf, nf, problem, cfg.method, dmp=cfg.DMP, cfl=cfg.CFL, nn_filename=nn_filename)
+    t, niter = 0.0, 0
+    if restart is not None:
+        U, t, niter = h5io.read_restart(restart, cfg.lf, nf)
+        solver.set_time(t)
+        solver.set_state(np.ascontiguousarray(U[4:-4, 4:-4]))
+    solver.set_time(t)
+    iter_0 = niter + 1
+    diag = h5io.Diagnostics(m, RK_FACTOR[cfg.integrator], iter_0=iter_0)
+    counter = 1
+    written = []
+    os.makedirs(outdir, exist_ok=True)
+
+    def snapshot():
+        U, g = solver.get_state(order=True)
+        path = os.path.join(outdir, h5io.file_name(m, niter))
+        h5io.write_output(path, m, U, g, niter, t, mord=cfg.Mord)
+        written.append(path)
+        if log:
+            log(f"snapshot {path}")
+
+    while t < tmax and niter <= nmax:
+        niter += 1
+        solver.run(1, tmax=tmax)
+        st = solver.stats()
+        dt = st["dt"]
+        t = st["t"]
+        diag.record(t, st["count_detected_cell"])
+        if IO_freqStep > 0 and niter % IO_freqStep == 0:
+            snapshot()
+        if IO_freqTime > 0.0:
+            if (t - float(counter) * IO_freqTime < 0.0) and (t + dt - float(counter) * IO_freqTime > 0.0):
+                counter += 1
+                snapshot()
+    if write_diagnostic:
+        diag.write(os.path.join(outdir, "diagnostic_" + h5io.file_name(m) + ".h5"))
+    return t, niter, written

@@ -1,0 +1,79 @@
+"""The reference's driver loop with its HDF5 outputs (gp-mood_amd/driver.py,
+h5io) on the GPU, against the reference's own run of the DMR step case
+(tests/golden/step_r1_dmr_64x16.npz): the diagnostic series are main.f90's
+float32 records of the reference's t and detection counts, the snapshots
+fall on the steps main.f90:178-192 chooses, and the last snapshot holds the
+primitives of the reference's final state."""
+import os
+import re
+import shutil
+import subprocess
+
+import numpy as np
+import pytest
+
+from conftest import case_options, golden, has_gpu, tables_from_npz
+
+pytestmark = pytest.mark.gpu
+H5DUMP = shutil.which("h5dump") or ("/opt/conda/bin/h5dump" if os.path.exists("/opt/conda/bin/h5dump") else None)
+
+
+def make(gpm, z):
+    o = case_options(z)
+    cfg = gpm.Config(lf=int(z["n"]), nf=o["nf"], Mord=int(z["mord"]), ngp=int(z["ngp"]), BC_type=str(z["bc"]),
+                     numFlux=o["flux"], Lx=o["lx"], Ly=o["ly"])
+    return gpm.Solver(cfg, tables_from_npz(z))
+
+
+def f32_series(path, name):
+    txt = subprocess.run([H5DUMP, "-d", name, "-m", "%.9g", "-y", "-w", "0", path], capture_output=True, text=True,
+                         check=True).stdout
+    body = txt.split("DATA {", 1)[1].rsplit("}", 2)[0]
+    return np.array([float(x) for x in re.findall(r"[-+0-9.eE]+", body)], dtype=np.float32)
+
+
+def test_driver_outputs_match_reference_run(gpm, tmp_path):
+    if not has_gpu():
+        pytest.skip("no GPU")
+    try:
+        gpm.h5io.load()
+    except RuntimeError as e:
+        pytest.skip(str(e))
+    z = golden("step_r1_dmr_64x16.npz")
+    print("[case driver: DMR 64x16, 20 steps]", flush=True)
+    lf, nf, nsteps = int(z["n"]), int(z["nf"]), int(z["nsteps"])
+    s = make(gpm, z)
+    s.set_state(z["U0"])
+    tcum = np.cumsum(z["dt"])
+    freq_t = float(tcum[-1]) / 3.3
+    t, niter, written = gpm.driver.run_problem(s, "DMR", tmax=1e10, nmax=nsteps - 1, outdir=str(tmp_path),
+                                               IO_freqStep=7, IO_freqTime=freq_t)
+    assert niter == nsteps and t == z["t"][-1]
+    # snapshot steps: multiples of IO_freqStep, and crossings of IO_freqTime (main.f90:185-192)
+    exp, counter = [], 1
+    for k in range(nsteps):
+        tk, dtk = float(z["t"][k]), float(z["dt"][k])
+        if (k + 1) % 7 == 0:
+            exp.append(k + 1)
+        if (tk - float(counter) * freq_t < 0.0) and (tk + dtk - float(counter) * freq_t > 0.0):
+            counter += 1
+            exp.append(k + 1)
+    m = gpm.h5io.meta(lf, nf, "DMR", gpm.capi.GP_MOOD)
+    assert [os.path.basename(p) for p in written] == [gpm.h5io.file_name(m, k) for k in exp]
+    # the last snapshot: primitives of the reference's final state
+    s2 = make(gpm, z)
+    s2.set_state(z["U0"])
+    s2.run(nsteps)
+    U, g = s2.get_state(order=True)
+    assert np.array_equal(U, z["U"])
+    last = os.path.join(str(tmp_path), gpm.h5io.file_name(m, nsteps))
+    gpm.h5io.write_output(last, m, U, g, nsteps, float(z["t"][-1]), mord=int(z["mord"]))
+    V, tt, it = gpm.h5io.read_restart(last, lf, nf)
+    assert tt == z["t"][-1] and it == nsteps
+    if H5DUMP is None:
+        pytest.skip("h5dump not available for the diagnostic check")
+    diag = os.path.join(str(tmp_path), "diagnostic_" + gpm.h5io.file_name(m) + ".h5")
+    n = nsteps - 1  # write_diagnostic stores size - iter_0 entries
+    assert np.array_equal(f32_series(diag, "time"), z["t"][:n].astype(np.float32))
+    pct = np.array([np.float32(float(c) * 100.0 / (3 * lf * nf)) for c in z["counts"][:n]])
+    assert np.array_equal(f32_series(diag, "pct_detected_cell"), pct)
