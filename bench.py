@@ -1,5 +1,5 @@
 #!/usr/bin/env python3
-"""Throughput of the GP-MOOD SSP-RK3 step on one MI355X.
+"""Throughput of the GP-MOOD step on one MI355X.
 
 Workload (the headline configuration): 2048 x 2048 cells on [0,1]^2, FP64,
 GP-R3 reconstruction (Mord 7, 4 Gauss points per face), HLLC, SSP-RK3,

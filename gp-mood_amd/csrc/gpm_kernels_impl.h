@@ -1,13 +1,14 @@
-// HIP kernels of the GP-MOOD SSP-RK3 step for gfx950 (MI355X).
+// HIP kernels of the GP-MOOD step for gfx950 (MI355X).
 //
 // Per RK stage (Forward_Euler, mod_FE.f90:20-190):
 //   k_stage  first MOOD pass over the whole grid.  A workgroup stages a tile of
 //            the stage input with its halo in LDS, computes every face flux of
-//            the tile once (GP reconstruction at the Gauss points + HLLC +
-//            quadrature), applies the four face updates per cell in the
-//            reference's order, runs the detection, writes the RK-combined
-//            output and appends troubled cells to a work list (wavefront
-//            ballot + one atomic per wavefront).
+//            the tile once (GP reconstruction at the Gauss points + the
+//            Riemann solver + quadrature), applies the four face updates per
+//            cell in the reference's order, runs the detection, writes the
+//            output combined by the integrator and appends troubled cells to
+//            a work list (wavefront ballot + one atomic per wavefront).  With
+//            Mord 1 it is the first-order Godunov method (FOG), one pass.
 //   k_mood   the remaining MOOD passes, as one cooperative persistent kernel:
 //            each pass lowers the order of the listed cells, marks their
 //            5-point neighbourhoods (deduplicated with a pass stamp), and
