@@ -30,6 +30,8 @@ import sys
 import tempfile
 import time
 
+import numpy as np
+
 ROOT = os.path.dirname(os.path.abspath(__file__))
 METRIC = "cell-updates/s per RK stage at 2048² FP64, plain vs NN-MOOD; % HBM roofline"
 HBM_PEAK = 8.0e12        # B/s, MI355X HBM3E
@@ -244,10 +246,11 @@ def options_measure(gpm, steps, warmup=3):
     return out
 
 
-def nn_comparison(gpm, n, steps):
+def nn_comparison(gpm, n, steps, extra_seeds=(1, 2, 3)):
     """Plain GP-MOOD vs NN-GP-MOOD (committed predictor weights) on the
-    2048^2 GP-R2 four-quadrant workload: throughput, second-pass cells and the
-    density L1 distance after `steps` steps; held-out precision/recall of the
+    2048^2 GP-R2 four-quadrant workload (seed 0): throughput, second-pass
+    cells and the density L1 distance after `steps` steps; the same for the
+    four-quadrant states of `extra_seeds`; held-out precision/recall of the
     predictor from its training report."""
     wdir = os.path.join(ROOT, "gp-mood_amd", "weights")
     wfile = os.path.join(wdir, "nn_r2.npz")
@@ -257,6 +260,21 @@ def nn_comparison(gpm, n, steps):
     m = gpm.MLP.load(wfile)
     out = nn_train.in_step(m, n=n, steps=steps, kind="four_quadrant", seed=0, bc="neumann",
                            log=lambda msg: print(msg, file=sys.stderr, flush=True))
+    # the spread over other seeded four-quadrant states (the gain is bounded by
+    # the share of the plain step spent in MOOD passes, which varies by state)
+    spread = {"0": {"speedup": out["speedup"], "density_l1_distance": out["density_l1_distance"],
+                    "plain_ms_per_step": out["plain"]["ms_per_step"]}}
+    for sd in extra_seeds:
+        o = nn_train.in_step(m, n=n, steps=steps, kind="four_quadrant", seed=sd, bc="neumann", log=lambda msg: None)
+        spread[str(sd)] = {"speedup": o["speedup"], "density_l1_distance": o["density_l1_distance"],
+                           "plain_ms_per_step": o["plain"]["ms_per_step"]}
+        print(f"[bench]   NN seed {sd}: {o['speedup']:.3f}x, L1 {o['density_l1_distance']:.2e}", file=sys.stderr,
+              flush=True)
+    out["seeds"] = spread
+    sp = [v["speedup"] for v in spread.values()]
+    out["speedup_geomean_over_seeds"] = float(np.exp(np.mean(np.log(sp))))
+    out["seeds_meeting_target"] = sum(1 for v in spread.values()
+                                      if v["speedup"] >= 1.5 and v["density_l1_distance"] <= 1e-3)
     rep = os.path.join(wdir, "nn_r2_report.json")
     if os.path.exists(rep):
         r = json.load(open(rep))
@@ -279,6 +297,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-steps", type=int, default=1)
     ap.add_argument("--nn-steps", type=int, default=100, help="steps of the plain vs NN-MOOD comparison (0: skip)")
+    ap.add_argument("--nn-seeds", default="1,2,3",
+                    help="other four-quadrant seeds the NN comparison also reports (comma list, may be empty)")
     ap.add_argument("--dropin-steps", type=int, default=3, help="host-array drop-in steps timed (0: skip)")
     ap.add_argument("--large-steps", type=int, default=0,
                     help="steps of the 8192^2 GP-R3 periodic shear-layer run, plain and NN-GP-MOOD (0: skip)")
@@ -372,7 +392,7 @@ def main():
     nn = None
     if args.nn_steps > 0:
         print(f"[bench] plain vs NN-GP-MOOD, {args.nn_steps} steps at {n}^2 GP-R2", file=sys.stderr, flush=True)
-        nn = nn_comparison(gpm, n, args.nn_steps)
+        nn = nn_comparison(gpm, n, args.nn_steps, [int(v) for v in args.nn_seeds.split(",") if v.strip()])
     large = None
     if args.large_steps > 0:
         large = large_run(gpm, args.large_steps)
