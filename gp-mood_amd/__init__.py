@@ -208,6 +208,42 @@ class Solver:
         arr = None if orders is None else np.ascontiguousarray(orders, dtype=np.int8)
         self._check(self.lib.gpm_set_apriori_orders(self._h, capi.ptr(arr)), "gpm_set_apriori_orders")
 
+    # -- the reference's own network (NN_GP_MOOD / NN_GP_MOOD_CC) -----------
+    def set_reference_nn(self, weight0, bias0, weight1, bias1, weight2, bias2):
+        """The reference network's parameters as matrices, row i = output i
+        (PyTorch's nn.Linear layout, neural_network/NN.py): weight0 (length, L),
+        bias0 (length,), weight1 (length, length), bias1 (length,), weight2
+        (2, length), bias2 (2,).  Steps after the tenth predict every cell's
+        order with it (mod_NN.f90:14-58)."""
+        w0 = np.asarray(weight0, dtype=np.float32)
+        length, L = w0.shape
+        fortran = [np.ascontiguousarray(np.asarray(a, dtype=np.float32).reshape(shape).T).ravel()
+                   for a, shape in ((w0, (length, L)), (bias0, (length, 1)), (weight1, (length, length)),
+                                    (bias1, (length, 1)), (weight2, (2, length)), (bias2, (2, 1)))]
+        self._check(self.lib.gpm_set_reference_nn(self._h, int(L), int(length), *[capi.ptr(a) for a in fortran]),
+                    "gpm_set_reference_nn")
+
+    def load_reference_nn(self, nn_filename, L, length):
+        """load_NN (mod_NN.f90:60-117): reads "<nn_filename>.txt"."""
+        self._check(self.lib.gpm_load_reference_nn(self._h, str(nn_filename).encode(), int(L), int(length)),
+                    "gpm_load_reference_nn")
+
+    def set_niter(self, niter):
+        """niter before the next step (main.f90:147 advances it at each step)."""
+        self._check(self.lib.gpm_set_niter(self._h, int(niter)), "gpm_set_niter")
+
+    def cellgpo_priori(self):
+        """CellGPO_priori (nf+8, lf+8) int32 after the last step."""
+        out = np.zeros((self.nf + 2 * NGC, self.lf + 2 * NGC), dtype=np.int32)
+        self._check(self.lib.gpm_get_cellgpo_priori(self._h, capi.ptr(out)), "gpm_get_cellgpo_priori")
+        return out
+
+    def nn_expf_bits(self, first, n, out_ptr):
+        """The network's FP32 exp of the floats with bit patterns first ..
+        first+n-1 into device memory out_ptr."""
+        self._check(self.lib.gpm_nn_expf_bits(self._h, int(first), int(n), ctypes.c_void_p(out_ptr)),
+                    "gpm_nn_expf_bits")
+
     def set_dump(self, x_ptr, y_ptr, capacity, count_ptr, keep_mord=0.05, seed=0, gated_only=True, stage_mask=1):
         """Training-data dump into device buffers (see include/gpmood.h)."""
         self._check(self.lib.gpm_set_dump(self._h, ctypes.c_void_p(x_ptr) if x_ptr else None,

@@ -20,6 +20,7 @@ LIB_PATH = os.environ.get("GPM_LIB_PATH") or os.path.join(HERE, "libgpmood.so")
 FOG = 1
 GP_MOOD = 2
 NN_GP_MOOD = 203
+NN_GP_MOOD_CC = 204
 FE, SSP_RK2, SSP_RK3, SSP_RK4 = 8, 9, 10, 12
 INTEGRATOR_IDS = {"fe": FE, "ssp_rk2": SSP_RK2, "ssp_rk3": SSP_RK3, "ssp_rk4": SSP_RK4}
 HLLC, LLF, HLL, UNSPLIT = 100, 101, 102, 103
@@ -41,6 +42,8 @@ EXPORTS = [
     "gpm_mlp_get_params", "gpm_mlp_get_grads", "gpm_mlp_forward_backward", "gpm_mlp_adamw_step",
     "gpm_mlp_train_step", "gpm_mlp_logits", "gpm_set_predictor", "gpm_set_dump", "gpm_features",
     "gpm_set_apriori_orders", "gpm_get_apriori_orders",
+    "gpm_set_reference_nn", "gpm_load_reference_nn", "gpm_set_niter", "gpm_get_cellgpo_priori", "gpm_nn_expf_bits",
+    "gpm_read_reference_nn", "gpm_nn_expf_bits_host",
     "gpm_record_masks", "gpm_recorded_steps", "gpm_frozen_run", "gpm_adjoint", "gpm_hllc_vjp",
 ]
 
@@ -61,6 +64,8 @@ class GpmStats(ctypes.Structure):
         ("steps", ctypes.c_int64), ("count_detected_cell", ctypes.c_int64), ("detected_total", ctypes.c_int64),
         ("passes", ctypes.c_int32 * 3), ("max_passes", ctypes.c_int32), ("second_pass_cells", ctypes.c_int64),
         ("a_priori_lowered", ctypes.c_int64), ("t", ctypes.c_double), ("dt", ctypes.c_double),
+        ("count_detected_cell_a_priori", ctypes.c_int64), ("nn_nan_passes", ctypes.c_int64),
+        ("nn_nan_passes_total", ctypes.c_int64), ("niter", ctypes.c_int64),
     ]
 
     def as_dict(self):
@@ -68,7 +73,8 @@ class GpmStats(ctypes.Structure):
             "steps": self.steps, "count_detected_cell": self.count_detected_cell,
             "detected_total": self.detected_total, "passes": list(self.passes), "max_passes": self.max_passes,
             "second_pass_cells": self.second_pass_cells, "a_priori_lowered": self.a_priori_lowered,
-            "t": self.t, "dt": self.dt,
+            "t": self.t, "dt": self.dt, "count_detected_cell_a_priori": self.count_detected_cell_a_priori,
+            "nn_nan_passes": self.nn_nan_passes, "nn_nan_passes_total": self.nn_nan_passes_total, "niter": self.niter,
         }
 
 
@@ -122,6 +128,13 @@ def load():
         "gpm_features": ([P, P, I, P], I),
         "gpm_set_apriori_orders": ([P, P], I),
         "gpm_get_apriori_orders": ([P, P], I),
+        "gpm_set_reference_nn": ([P, I, I, P, P, P, P, P, P], I),
+        "gpm_load_reference_nn": ([P, ctypes.c_char_p, I, I], I),
+        "gpm_set_niter": ([P, ctypes.c_int64], I),
+        "gpm_get_cellgpo_priori": ([P, P], I),
+        "gpm_nn_expf_bits": ([P, ctypes.c_uint32, ctypes.c_int64, P], I),
+        "gpm_read_reference_nn": ([ctypes.c_char_p, I, I, P, P, P, P, P, P], I),
+        "gpm_nn_expf_bits_host": ([ctypes.c_uint32, ctypes.c_int64, P], I),
         "gpm_record_masks": ([P, P, I], I),
         "gpm_recorded_steps": ([P, P], I),
         "gpm_frozen_run": ([P, P, P, I, P], I),

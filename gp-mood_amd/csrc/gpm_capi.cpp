@@ -57,6 +57,15 @@ struct gpm_ctx {
     std::vector<std::pair<int, int>> ev_pairs;  // (start index, kind 0 stage / 1 mood)
     size_t ev_used = 0;
     std::string msg;
+    // the reference's own network (mod_NN.f90) in device memory, Fortran layouts:
+    // weight0 | bias0 | weight1 | bias1 | weight2 | bias2
+    float *refnn_w = nullptr;
+    int refnn_L = 0, refnn_len = 0;
+    bool refnn_cc = false;           // method NN_GP_MOOD_CC: softmax and the blended reconstruction
+    int8_t *priori = nullptr;        // CellGPO_priori of the last stage [NY][NX]
+    bool priori_set = false;         // the last step ran the network (else CellGPO_priori = Mord)
+    float *rowv = nullptr, *vnn = nullptr;  // NN_GP_MOOD_CC: values of each row's last cell; values_NN
+    long long niter = 0;             // main.f90's niter: steps started so far
     // learned a priori predictor (not owned)
     gpm_mlp *nn = nullptr;
     int nn_gate = 1;
@@ -369,15 +378,33 @@ int enqueue_ring_fill(gpm_ctx *c)
     return GPM_OK;
 }
 
-// One step of the configured integrator after k_dt has set dt.
+// nsteps_with_no_NN (global_variables.f90:97): the reference network predicts
+// from step 11 on; earlier steps run plain GP-MOOD with the configured DMP.
+constexpr int kStepsWithNoNN = 10;
+
+// One step of the configured integrator after k_dt has set dt.  A step first
+// advances niter (main.f90:147).
 int enqueue_stages(gpm_ctx *c)
 {
     StagePlan plan[kMaxStages];
     const int nst = plan_stages(c, plan);
+    ++c->niter;
+    // the reference network's steps: a priori orders from the network and the
+    // PAD-only check NN_DETECTION (mod_FE.f90:51-56,151-158)
+    const bool refnn = c->refnn_w && c->niter > kStepsWithNoNN;
+    c->priori_set = refnn;
     for (int k = 0; k < nst; ++k) {
         const int stage = k + 1;
         const double4 *in = plan[k].in;
         int *cnt = c->counters + k * kCountersPerStage;
+        if (refnn) {
+            RefNnArgs r{};
+            r.U = in; r.w = c->refnn_w; r.pred = c->pred; r.priori = c->priori;
+            r.rowv = c->refnn_cc ? c->rowv : nullptr; r.stats = c->stats; r.B = c->B;
+            r.L = c->refnn_L; r.len = c->refnn_len; r.mord = c->mord; r.softmax = c->refnn_cc;
+            HIPCHK(c, launch_refnn(r, c->stream));
+            if (c->refnn_cc) HIPCHK(c, launch_refnn_values(c->vnn, c->rowv, c->NX, c->NY, c->refnn_L, true, c->stream));
+        }
         if (c->nn) {
             // a priori orders of this stage's input (compute_CellGPO_with_NN, mod_NN.f90:14-58)
             HIPCHK(c, hipMemsetAsync(c->nncount, 0, sizeof(int), c->stream));
@@ -390,8 +417,12 @@ int enqueue_stages(gpm_ctx *c)
         if (c->fixed_orders && !c->nn)
             HIPCHK(c, launch_orders_from_pred(c->pred, c->gpo, (size_t)c->NX * c->NY, c->stream));
         StageArgs s = stage_args(c, k, plan[k]);
-        s.apriori = c->nn != nullptr || c->fixed_orders;
+        s.apriori = c->nn != nullptr || c->fixed_orders || refnn;
         s.pred = c->pred;
+        if (refnn) {
+            s.dmp = 0;
+            if (c->refnn_cc) s.vnn = c->vnn;  // NN_GP_MOOD_CC_ replaces GP_MOOD_ (mod_Reconstruction.f90:26-32)
+        }
         hipEvent_t e0 = nullptr, e1 = nullptr;
         if (c->timing) {
             size_t i0 = c->ev_used;
@@ -416,6 +447,11 @@ int enqueue_stages(gpm_ctx *c)
         m.dmp = c->cfg.dmp; m.u2 = c->cfg.u2; m.u2tol = c->cfg.u2_tol;
         m.cascade = c->cfg.cascade; m.pmax = kMaxPasses; m.stage_idx = k;
         m.dx = c->dx; m.dy = c->dy;
+        if (refnn) {
+            m.dmp = 0;
+            m.nn_passes = 1;
+            if (c->refnn_cc) m.vnn = c->vnn;
+        }
         void *args[] = {&m};
         if (c->timing) {
             size_t i0 = c->ev_used;
@@ -500,8 +536,9 @@ int gpm_create(const gpm_config *cfg, gpm_ctx **out)
     if (cfg->num_flux != GPM_HLLC && cfg->num_flux != GPM_LLF && cfg->num_flux != GPM_HLL &&
         cfg->num_flux != GPM_UNSPLIT)
         return bad(GPM_EUNSUP, "numFlux must be HLLC, LLF, HLL or unsplit");
-    if (cfg->method != GPM_GP_MOOD && cfg->method != GPM_NN_GP_MOOD && cfg->method != GPM_FOG)
-        return bad(GPM_EUNSUP, "method must be GP_MOOD, NN_GP_MOOD or FOG");
+    const bool nn_method = cfg->method == GPM_NN_GP_MOOD || cfg->method == GPM_NN_GP_MOOD_CC;
+    if (cfg->method != GPM_GP_MOOD && !nn_method && cfg->method != GPM_FOG)
+        return bad(GPM_EUNSUP, "method must be GP_MOOD, NN_GP_MOOD, NN_GP_MOOD_CC or FOG");
     if (c->fog && (cfg->ngp != 1 || cfg->integrator != GPM_FE))
         return bad(GPM_EUNSUP, "FOG needs ngp = 1 and integrator FE (the problems/*_FOG.f90 set-up)");
     if (!bc_ok) return bad(GPM_EUNSUP, "unknown BC_type");
@@ -510,10 +547,12 @@ int gpm_create(const gpm_config *cfg, gpm_ctx **out)
     c->ring_mode = !index_map_bc(cfg->bc_type) || c->gravity || c->fog;
     if (c->ring_mode && cfg->integrator == GPM_SSP_RK4)
         return bad(GPM_EUNSUP, "SSP_RK4 needs a copy boundary (Neumann, Periodic, Reflective, KH_BC) and no source term");
-    const bool square_copy = cfg->lf == cfg->nf && index_map_bc(cfg->bc_type) && !c->gravity &&
-                             cfg->integrator == GPM_SSP_RK3;
-    if (cfg->method == GPM_NN_GP_MOOD && !square_copy)
-        return bad(GPM_EUNSUP, "NN_GP_MOOD needs lf = nf, a copy boundary, SSP_RK3 and no source term");
+    // the network reads the stage input's ghost cells as copies of interior cells
+    if (nn_method && (!index_map_bc(cfg->bc_type) || c->gravity))
+        return bad(GPM_EUNSUP, "NN_GP_MOOD needs a copy boundary (Neumann, Periodic, Reflective, KH_BC) and no source term");
+    if (cfg->method == GPM_NN_GP_MOOD_CC && cfg->mord != 3)
+        return bad(GPM_EUNSUP, "NN_GP_MOOD_CC needs Mord = 3 (the reference stops at other orders, "
+                               "mod_NN_GP_MOOD_CC.f90:57-59)");
     if (cfg->cascade != GPM_CASCADE_REFERENCE && cfg->cascade != GPM_CASCADE_FULL)
         return bad(GPM_EINVAL, "unknown cascade");
     if (!c->fog && !supported(cfg->mord, cfg->ngp)) return bad(GPM_EUNSUP, "(Mord, ngp) combination not compiled");
@@ -537,6 +576,7 @@ int gpm_create(const gpm_config *cfg, gpm_ctx **out)
               dalloc(&c->err, 1) == hipSuccess && dalloc(&c->solo_p, 1) == hipSuccess && dalloc(&c->stats, 1) == hipSuccess &&
               dalloc(&c->tstate, 8) == hipSuccess && dalloc(&c->done, 1) == hipSuccess &&
               dalloc(&c->pred, n) == hipSuccess && dalloc(&c->nnlist, n) == hipSuccess &&
+              dalloc(&c->priori, n) == hipSuccess &&
               dalloc(&c->nncount, 1) == hipSuccess;
     if (ok && c->integrator == GPM_SSP_RK4)
         ok = dalloc(&c->dC, n) == hipSuccess && dalloc(&c->dD, n) == hipSuccess && dalloc(&c->dE, n) == hipSuccess &&
@@ -624,7 +664,8 @@ int gpm_destroy(gpm_ctx *c)
     void *bufs[] = {c->dT, c->dU, c->dA, c->dB, c->dC, c->dD, c->dE, c->dF, c->ring, c->band, c->st_dev, c->ext,
                     c->gpo, c->slistA, c->slistB, c->mark, c->counters,
                     c->stamp, c->err, c->solo_p, c->stats, c->tstate, c->block_min, c->dt_log, c->done, c->pred, c->nnlist,
-                    c->nncount, c->adj_ub, c->adj_G, c->adj_A, c->adj_tmp, c->adj_ordx, c->adj_ordy};
+                    c->nncount, c->adj_ub, c->adj_G, c->adj_A, c->adj_tmp, c->adj_ordx, c->adj_ordy,
+                    c->priori, c->refnn_w, c->rowv, c->vnn};
     for (void *b : bufs)
         if (b) hipFree(b);
     for (hipEvent_t e : c->ev_pool) hipEventDestroy(e);
@@ -842,6 +883,10 @@ int gpm_get_stats(gpm_ctx *c, gpm_stats *st)
     st->a_priori_lowered = d.a_priori_lowered;
     st->t = ts[0];
     st->dt = ts[1];
+    st->count_detected_cell_a_priori = d.apriori_step;
+    st->nn_nan_passes = d.nn_passes_step;
+    st->nn_nan_passes_total = d.nn_passes_total;
+    st->niter = c->niter;
     return GPM_OK;
 }
 
@@ -890,8 +935,142 @@ int gpm_set_predictor(gpm_ctx *c, gpm_mlp *m, int32_t gate)
         if (mlp_stencil(m) != 2 * R + 1 || mlp_classes(m) != (c->mord + 1) / 2)
             return fail(c, GPM_EINVAL, "predictor stencil/classes do not match Mord");
     }
+    if (m && c->refnn_w) return fail(c, GPM_ESTATE, "the reference network is set; a context runs one predictor");
     c->nn = m;
     c->nn_gate = gate;
+    return GPM_OK;
+}
+
+int gpm_set_reference_nn(gpm_ctx *c, int32_t L, int32_t length, const float *weight0, const float *bias0,
+                         const float *weight1, const float *bias1, const float *weight2, const float *bias2)
+{
+    if (!c) return GPM_EINVAL;
+    if (c->cfg.method != GPM_NN_GP_MOOD && c->cfg.method != GPM_NN_GP_MOOD_CC)
+        return fail(c, GPM_ESTATE, "the reference network needs method NN_GP_MOOD or NN_GP_MOOD_CC");
+    if (c->nn) return fail(c, GPM_ESTATE, "a learned predictor is set; a context runs one predictor");
+    if (c->mord != 3 && c->mord != 5)
+        return fail(c, GPM_EUNSUP, "the reference network's stencil (szs_sphere(Mord+2)) is built for Mord 3 and 5");
+    const int S = c->mord == 3 ? 13 : 25;
+    if (!weight0 || !bias0 || !weight1 || !bias1 || !weight2 || !bias2) return fail(c, GPM_EINVAL, "null argument");
+    if (L < 4 * S || L > 256)
+        return fail(c, GPM_EINVAL, "L must hold the 4*szs_sphere(Mord+2) stencil values and be at most 256");
+    if (length < 1 || length > kRefNnMaxLen) return fail(c, GPM_EUNSUP, "length must be 1..64");
+    const bool cc = c->cfg.method == GPM_NN_GP_MOOD_CC;
+    // values_NN(L, n, :) must lie inside values_NN(-1:lf+2, -1:nf+2, 2) for every row
+    if (cc && (long long)L + 1 >= 3ll * (c->NX + 4))
+        return fail(c, GPM_EUNSUP, "NN_GP_MOOD_CC with L >= 3*(lf+4)-1: the reference writes past values_NN");
+    const size_t n0 = (size_t)length * L, n1 = (size_t)length * length, n2 = 2 * (size_t)length;
+    const size_t total = n0 + length + n1 + length + n2 + 2;
+    std::vector<float> h(total);
+    size_t o = 0;
+    auto put = [&](const float *src, size_t cnt) { std::memcpy(h.data() + o, src, cnt * sizeof(float)); o += cnt; };
+    put(weight0, n0); put(bias0, length); put(weight1, n1); put(bias1, length); put(weight2, n2); put(bias2, 2);
+    if (c->refnn_w) { HIPCHK(c, hipStreamSynchronize(c->stream)); hipFree(c->refnn_w); c->refnn_w = nullptr; }
+    HIPCHK(c, dalloc(&c->refnn_w, total));
+    HIPCHK(c, hipMemcpyAsync(c->refnn_w, h.data(), total * sizeof(float), hipMemcpyHostToDevice, c->stream));
+    if (cc && !c->vnn) {
+        HIPCHK(c, dalloc(&c->rowv, 2 * (size_t)c->NY));
+        HIPCHK(c, dalloc(&c->vnn, 2 * (size_t)(c->NX + 4) * (c->NY + 4)));
+    }
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    c->refnn_L = L;
+    c->refnn_len = length;
+    c->refnn_cc = cc;
+    return GPM_OK;
+}
+
+int gpm_read_reference_nn(const char *nn_filename, int32_t L, int32_t length, float *weight0, float *bias0,
+                          float *weight1, float *bias1, float *weight2, float *bias2)
+{
+    if (!nn_filename || !weight0 || !bias0 || !weight1 || !bias1 || !weight2 || !bias2) return GPM_EINVAL;
+    if (L < 1 || length < 1 || L > 4096 || length > 4096) return GPM_EINVAL;
+    // load_NN (mod_NN.f90:60-117): "<NN_filename>.txt", one number per record,
+    // weight0 row by row (j fastest: update_ij), bias0, weight1 row by row,
+    // bias1, weight2 row by row, bias2; records past up5 are not read
+    std::string path = std::string(nn_filename);
+    while (!path.empty() && path.back() == ' ') path.pop_back();
+    size_t b = 0;
+    while (b < path.size() && path[b] == ' ') ++b;
+    path = path.substr(b) + ".txt";
+    FILE *f = std::fopen(path.c_str(), "r");
+    if (!f) return GPM_EINVAL;
+    // (Fortran array, rows, columns) in reading order; element (i, j) at i + j*rows
+    struct Part { float *a; int rows, cols; } parts[6] = {{weight0, length, L}, {bias0, length, 1},
+                                                         {weight1, length, length}, {bias1, length, 1},
+                                                         {weight2, 2, length}, {bias2, 2, 1}};
+    char line[512];
+    for (const Part &p : parts)
+        for (int i = 0; i < p.rows; ++i)
+            for (int j = 0; j < p.cols; ++j) {
+                char *end = nullptr;
+                float v = 0.0f;
+                bool got = false;
+                while (!got && std::fgets(line, sizeof line, f)) {
+                    char *q = line;
+                    while (*q == ' ' || *q == '\t') ++q;
+                    if (*q == '\n' || *q == '\r' || *q == 0) continue;  // list-directed reads skip blank records
+                    v = std::strtof(q, &end);
+                    if (end == q) { std::fclose(f); return GPM_EINVAL; }
+                    got = true;
+                }
+                if (!got) { std::fclose(f); return GPM_EINVAL; }  // the file ends before all weights are read
+                p.a[i + (size_t)j * p.rows] = v;
+            }
+    std::fclose(f);
+    return GPM_OK;
+}
+
+int gpm_load_reference_nn(gpm_ctx *c, const char *nn_filename, int32_t L, int32_t length)
+{
+    if (!c || !nn_filename) return GPM_EINVAL;
+    if (L < 1 || length < 1 || L > 4096 || length > 4096) return fail(c, GPM_EINVAL, "invalid L or length");
+    std::vector<float> w0((size_t)length * L), b0(length), w1((size_t)length * length), b1(length),
+        w2(2 * (size_t)length), b2(2);
+    if (gpm_read_reference_nn(nn_filename, L, length, w0.data(), b0.data(), w1.data(), b1.data(), w2.data(),
+                              b2.data()))
+        return fail(c, GPM_EINVAL, std::string("cannot read ") + nn_filename + ".txt: missing file, a record "
+                                   "that is not a number, or fewer records than the network holds");
+    return gpm_set_reference_nn(c, L, length, w0.data(), b0.data(), w1.data(), b1.data(), w2.data(), b2.data());
+}
+
+int gpm_set_niter(gpm_ctx *c, int64_t niter)
+{
+    if (!c || niter < 0) return fail(c, GPM_EINVAL, "invalid argument");
+    c->niter = niter;
+    return GPM_OK;
+}
+
+int gpm_get_cellgpo_priori(gpm_ctx *c, int32_t *priori)
+{
+    if (!c || !priori) return fail(c, GPM_EINVAL, "null argument");
+    const size_t n = (size_t)c->NX * c->NY;
+    std::vector<int8_t> h(n, (int8_t)c->mord);
+    if (c->priori_set) {
+        HIPCHK(c, hipMemcpyAsync(h.data(), c->priori, n, hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+    }
+    const int W = c->NX + 8, H = c->NY + 8;
+    for (int jj = 0; jj < H; ++jj)
+        for (int ii = 0; ii < W; ++ii) {
+            const int i = ii - 4, j = jj - 4;
+            priori[(size_t)jj * W + ii] =
+                (i < 0 || i >= c->NX || j < 0 || j >= c->NY) ? c->mord : (int32_t)h[(size_t)j * c->NX + i];
+        }
+    return GPM_OK;
+}
+
+int gpm_nn_expf_bits_host(uint32_t first, int64_t n, float *y)
+{
+    if (!y || n < 0) return GPM_EINVAL;
+    nn_expf_bits_host(first, n, y);
+    return GPM_OK;
+}
+
+int gpm_nn_expf_bits(gpm_ctx *c, uint32_t first, int64_t n, float *y_dev)
+{
+    if (!c || !y_dev || n < 0) return fail(c, GPM_EINVAL, "invalid argument");
+    HIPCHK(c, launch_nn_expf_bits(first, n, y_dev, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
     return GPM_OK;
 }
 

@@ -551,9 +551,14 @@ __device__ __forceinline__ void face_flux_dyn(const DevTables *__restrict__ T, c
 // order (the same operations, in the same order, as recon2 for that order) and one
 // Riemann solve, so lanes of a wavefront that hold different orders run a
 // single code path instead of one inlined solver per order.
+//
+// With cc (NN_GP_MOOD_CC, Mord 3) the face states are those of face_flux_cc:
+// the cell values at order 1, the blend v1 * U(cell) + v2 * GP value at order 3.
 template <int MORD, int FLUX, class Acc>
 __device__ __forceinline__ void face_gp_flux_dyn(const DevTables *__restrict__ T, const Acc &acc, int ord, int fa,
-                                                 int fb, int dir, int j, int ai, int aj, int bi, int bj, double F[4])
+                                                 int fb, int dir, int j, int ai, int aj, int bi, int bj, double F[4],
+                                                 bool cc = false, double va1 = 0.0, double va2 = 1.0,
+                                                 double vb1 = 0.0, double vb2 = 1.0)
 {
     constexpr int SZM = stencil_sz(MORD);
     const int oi = (ord - 1) / 2, sz = stencil_sz(ord);
@@ -570,7 +575,72 @@ __device__ __forceinline__ void face_gp_flux_dyn(const DevTables *__restrict__ T
         ua[0] = ua[0] + za * qa.x; ua[1] = ua[1] + za * qa.y; ua[2] = ua[2] + za * qa.z; ua[3] = ua[3] + za * qa.w;
         ub[0] = ub[0] + zb * qb.x; ub[1] = ub[1] + zb * qb.y; ub[2] = ub[2] + zb * qb.z; ub[3] = ub[3] + zb * qb.w;
     }
+    if (MORD == 3 && cc) {
+        const double4 ca = acc(ai, aj), cb = acc(bi, bj);
+        const double qa[4] = {ca.x, ca.y, ca.z, ca.w}, qb[4] = {cb.x, cb.y, cb.z, cb.w};
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            ua[k] = ord == 1 ? qa[k] : va1 * qa[k] + va2 * ua[k];
+            ub[k] = ord == 1 ? qb[k] : vb1 * qb[k] + vb2 * ub[k];
+        }
+    }
     riemann_dir<FLUX>(dir, ua, ub, F);
+}
+
+// values_NN(l, n, 1:2) of cell (i, j) (0-based interior indices, l = i+1,
+// n = j+1) from its Fortran storage, dimension(-1:lf+2, -1:nf+2, 2); cells
+// outside the reconstructed range [-1, NX] x [-1, NY] read (0, 1).
+__device__ __forceinline__ void vnn_at(const float *__restrict__ vnn, int NX, int NY, int i, int j, double &v1,
+                                       double &v2)
+{
+    if (i < -1 || i > NX || j < -1 || j > NY) {
+        v1 = 0.0;
+        v2 = 1.0;
+        return;
+    }
+    const size_t W = (size_t)NX + 4, c = (size_t)(i + 2) + (size_t)(j + 2) * W;
+    v1 = (double)vnn[c];
+    v2 = (double)vnn[c + W * ((size_t)NY + 4)];
+}
+
+// Face flux of NN_GP_MOOD_CC (NN_GP_MOOD_CC_, mod_NN_GP_MOOD_CC.f90:57-68;
+// Mord 3): order 1 takes each cell's own value, order 3 the GP value blended
+// with it, v1 * U(cell) + v2 * dot_product(zT, q), with the cell's values_NN.
+template <int NGP, int DIR, int FLUX, class Acc>
+__device__ __forceinline__ void face_flux_cc(const DevTables *__restrict__ T, const Acc &acc, int ord, int ai, int aj,
+                                             int bi, int bj, double va1, double va2, double vb1, double vb2,
+                                             double L[4])
+{
+    const double4 ca = acc(ai, aj), cb = acc(bi, bj);
+    L[0] = 0.0; L[1] = 0.0; L[2] = 0.0; L[3] = 0.0;
+    if (ord == 3) {
+        double ua[NGP][4], ub[NGP][4];
+        recon2<3, NGP, DIR == 0 ? kR : kT, DIR == 0 ? kL : kB, GPM_RECON_UNROLL>(T, acc, ai, aj, bi, bj, ua, ub);
+        const double qa[4] = {ca.x, ca.y, ca.z, ca.w}, qb[4] = {cb.x, cb.y, cb.z, cb.w};
+#pragma unroll
+        for (int j = 0; j < NGP; ++j) {
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                ua[j][k] = va1 * qa[k] + va2 * ua[j][k];
+                ub[j][k] = vb1 * qb[k] + vb2 * ub[j][k];
+            }
+            double F[4];
+            riemann<FLUX, DIR>(ua[j], ub[j], F);
+            const double w = T->gw[j];
+#pragma unroll
+            for (int k = 0; k < 4; ++k) L[k] = L[k] + F[k] * w;
+        }
+    } else {
+        // every Gauss point carries the two cell values: one Riemann solve
+        double sa[4] = {ca.x, ca.y, ca.z, ca.w}, sb[4] = {cb.x, cb.y, cb.z, cb.w}, F[4];
+        riemann<FLUX, DIR>(sa, sb, F);
+#pragma unroll
+        for (int j = 0; j < NGP; ++j) {
+            const double w = T->gw[j];
+#pragma unroll
+            for (int k = 0; k < 4; ++k) L[k] = L[k] + F[k] * w;
+        }
+    }
 }
 
 struct MoodParams {

@@ -103,6 +103,8 @@ __global__ __launch_bounds__(kDtThreads) void k_dt(DtArgs a)
         if (a.dt_log) a.dt_log[a.step_idx] = dt;
         if (a.new_step) {
             a.stats->detected_step = 0;
+            a.stats->apriori_step = 0;
+            a.stats->nn_passes_step = 0;
             a.stats->steps += 1;
         }
         *a.done = 0u;

@@ -40,6 +40,11 @@ struct DevStats {
     long long a_priori_lowered;
     int passes[5];  // MOOD passes of each stage of the last step (up to 5 stages: SSP_RK4)
     int max_passes;
+    // the reference network (NN_GP_MOOD / NN_GP_MOOD_CC with its own weights)
+    long long apriori_step;    // count_detected_cell_a_priori of the last step
+    long long nn_passes_step;  // passes of the last step whose check lowered a cell
+                               // (increments of count_steps_NN_produced_NAN, mod_FE.f90:154-157)
+    long long nn_passes_total;
 };
 
 constexpr int kMaxStages = 5;
@@ -96,6 +101,7 @@ struct StageArgs {
     int apriori;                      // start from the a priori orders in pred (learned predictor)
     int frozen;                       // orders fixed to pred, no detection (frozen-mask step)
     const int8_t *__restrict__ pred;  // a priori orders (read only here; gpo is rewritten)
+    const float *__restrict__ vnn;    // NN_GP_MOOD_CC: values_NN for the blended reconstruction (Mord 3), or null
     double dx, dy;
     int tiles_x, ntiles;
 };
@@ -117,7 +123,9 @@ struct MoodArgs {
     Bnd B;
     StepCtl C;
     int dmp, u2, u2tol, cascade, pmax, stage_idx;
+    int nn_passes;  // count the passes that lowered a cell (a step of the reference network)
     int solo_max;   // lists of at most this many cells run on workgroup 0 alone (0: never)
+    const float *__restrict__ vnn;  // as StageArgs::vnn
     double dx, dy;
 };
 
@@ -198,6 +206,28 @@ bool supported(int mord, int ngp);
 hipError_t launch_dt(const DtArgs &a, int nblocks, hipStream_t s);
 hipError_t launch_ring(const RingArgs &a, hipStream_t s);
 hipError_t launch_unpack_orders(const order_t *g, int8_t *out, size_t n, hipStream_t s);
+
+// The reference's own network (mod_NN.f90): per-cell a priori orders of a
+// stage input with a copy boundary.
+struct RefNnArgs {
+    const double4 *__restrict__ U;  // stage input (interior)
+    const float *__restrict__ w;    // weight0 | bias0 | weight1 | bias1 | weight2 | bias2 (Fortran layouts)
+    int8_t *pred;                   // CellGPO after the prediction (1 or 3)
+    int8_t *priori;                 // CellGPO_priori (1, 3 or Mord)
+    float *rowv;                    // [NY][2]: values_NN of each row's last cell (NN_GP_MOOD_CC), or null
+    DevStats *stats;
+    Bnd B;
+    int L, len, mord, softmax;
+};
+constexpr int kRefNnMaxLen = 64;     // hidden units held in registers
+constexpr int kRefNnThreads = 64;    // one wavefront per workgroup
+hipError_t launch_refnn(const RefNnArgs &a, hipStream_t s);
+// values_NN(-1:lf+2, -1:nf+2, 2) := (0, 1), then each row's values written at
+// values_NN(L, n, :) in row order, as the reference indexes it (mod_NN.f90:41,44)
+hipError_t launch_refnn_values(float *vnn, const float *rowv, int NX, int NY, int L, bool scatter, hipStream_t s);
+// exp of the floats with bit patterns first .. first+n-1 by the network's expf
+hipError_t launch_nn_expf_bits(uint32_t first, long long n, float *out, hipStream_t s);
+void nn_expf_bits_host(uint32_t first, long long n, float *out);
 hipError_t launch_orders_from_pred(const int8_t *pred, order_t *g, size_t n, hipStream_t s);
 
 }  // namespace gpm

@@ -278,6 +278,12 @@ __global__ __launch_bounds__(kStageThreads, GPM_STAGE_WAVES_PER_EU) void k_stage
                 fog_flux<0, FLUX>(T, acc(fx - 1 + H, fy + H), acc(fx + H, fy + H), L);
             } else if (!a.apriori) {
                 face_flux<MORD, NGP, 0, FLUX>(T, acc, fx - 1 + H, fy + H, fx + H, fy + H, L);
+            } else if (MORD == 3 && a.vnn) {
+                const int o = min(sO[(fy + 1) * OW + fx], sO[(fy + 1) * OW + fx + 1]);
+                double va1, va2, vb1, vb2;
+                vnn_at(a.vnn, NX, NY, x0 + fx - 1, y0 + fy, va1, va2);
+                vnn_at(a.vnn, NX, NY, x0 + fx, y0 + fy, vb1, vb2);
+                face_flux_cc<NGP, 0, FLUX>(T, acc, o, fx - 1 + H, fy + H, fx + H, fy + H, va1, va2, vb1, vb2, L);
             } else {
                 const int o = min(sO[(fy + 1) * OW + fx], sO[(fy + 1) * OW + fx + 1]);
                 face_flux_dyn<MORD, NGP, 0, FLUX>(T, acc, o, fx - 1 + H, fy + H, fx + H, fy + H, L);
@@ -290,6 +296,12 @@ __global__ __launch_bounds__(kStageThreads, GPM_STAGE_WAVES_PER_EU) void k_stage
                 fog_flux<1, FLUX>(T, acc(gx + H, gy - 1 + H), acc(gx + H, gy + H), L);
             } else if (!a.apriori) {
                 face_flux<MORD, NGP, 1, FLUX>(T, acc, gx + H, gy - 1 + H, gx + H, gy + H, L);
+            } else if (MORD == 3 && a.vnn) {
+                const int o = min(sO[gy * OW + gx + 1], sO[(gy + 1) * OW + gx + 1]);
+                double va1, va2, vb1, vb2;
+                vnn_at(a.vnn, NX, NY, x0 + gx, y0 + gy - 1, va1, va2);
+                vnn_at(a.vnn, NX, NY, x0 + gx, y0 + gy, vb1, vb2);
+                face_flux_cc<NGP, 1, FLUX>(T, acc, o, gx + H, gy - 1 + H, gx + H, gy + H, va1, va2, vb1, vb2, L);
             } else {
                 const int o = min(sO[gy * OW + gx + 1], sO[(gy + 1) * OW + gx + 1]);
                 face_flux_dyn<MORD, NGP, 1, FLUX>(T, acc, o, gx + H, gy - 1 + H, gx + H, gy + H, L);
@@ -461,8 +473,16 @@ __global__ __launch_bounds__(kMoodThreads) void k_mood(MoodArgs a)
                 nb[cs][q] = load_stage<GEN>(a.Uin, a.B, i - HB + q % BOX, j - HB + q / BOX, xmin_top);
             __syncthreads();
             double F[4] = {0.0, 0.0, 0.0, 0.0};
-            if (valid && jg < NGP)
-                face_gp_flux_dyn<MORD, FLUX>(T, lacc, ord, fa, fb, fdir, jg, HB + oai, HB + oaj, HB + obi, HB + obj, F);
+            if (valid && jg < NGP) {
+                double va1 = 0.0, va2 = 1.0, vb1 = 0.0, vb2 = 1.0;
+                const bool cc = MORD == 3 && a.vnn != nullptr;
+                if (cc) {
+                    vnn_at(a.vnn, NX, NY, i + oai, j + oaj, va1, va2);
+                    vnn_at(a.vnn, NX, NY, i + obi, j + obj, vb1, vb2);
+                }
+                face_gp_flux_dyn<MORD, FLUX>(T, lacc, ord, fa, fb, fdir, jg, HB + oai, HB + oaj, HB + obi, HB + obj, F,
+                                             cc, va1, va2, vb1, vb2);
+            }
             // quadrature over the face's Gauss points, j = 1..ngp from 0
             double L[4] = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
@@ -562,6 +582,11 @@ __global__ __launch_bounds__(kMoodThreads) void k_mood(MoodArgs a)
         st->detected_step += det;
         st->detected_total += det;
         st->second_pass_cells += second;
+        // every pass but the last lowered at least one cell
+        if (a.nn_passes && p > 0) {
+            st->nn_passes_step += p - 1;
+            st->nn_passes_total += p - 1;
+        }
     }
 }
 

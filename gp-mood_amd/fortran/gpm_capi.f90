@@ -16,6 +16,7 @@ module gpm_capi
       integer(c_int32_t) :: passes(3), max_passes
       integer(c_int64_t) :: second_pass_cells, a_priori_lowered
       real(c_double) :: t, dt
+      integer(c_int64_t) :: count_detected_cell_a_priori, nn_nan_passes, nn_nan_passes_total, niter
    end type gpm_stats
 
    interface
@@ -81,6 +82,27 @@ module gpm_capi
          real(c_double), value :: tmax
          real(c_double), intent(out) :: dts(*)
       end function gpm_run
+
+      ! the reference's own network: the arrays load_NN filled (global_variables.f90:74-80)
+      integer(c_int) function gpm_set_reference_nn(ctx, L, length, weight0, bias0, weight1, bias1, weight2, bias2) &
+         bind(C, name='gpm_set_reference_nn')
+         import :: c_int, c_ptr, c_int32_t, c_float
+         type(c_ptr), value :: ctx
+         integer(c_int32_t), value :: L, length
+         real(c_float), intent(in) :: weight0(*), bias0(*), weight1(*), bias1(*), weight2(*), bias2(*)
+      end function gpm_set_reference_nn
+
+      integer(c_int) function gpm_set_niter(ctx, niter) bind(C, name='gpm_set_niter')
+         import :: c_int, c_ptr, c_int64_t
+         type(c_ptr), value :: ctx
+         integer(c_int64_t), value :: niter
+      end function gpm_set_niter
+
+      integer(c_int) function gpm_get_cellgpo_priori(ctx, priori) bind(C, name='gpm_get_cellgpo_priori')
+         import :: c_int, c_ptr, c_int32_t
+         type(c_ptr), value :: ctx
+         integer(c_int32_t), intent(out) :: priori(*)
+      end function gpm_get_cellgpo_priori
    end interface
 
 contains

@@ -5,7 +5,9 @@
 ! the configured integrator (FE, SSP_RK2, SSP_RK3 or SSP_RK4, time_stepper.f90:17-21)
 ! runs on the GPU through libgpmood; the GP tables
 ! are the ones GP_presim_sphere left in global_variables, so both codes use the
-! same weights.  Link with -lgpmood.
+! same weights.  With method NN_GP_MOOD or NN_GP_MOOD_CC the network is the one
+! load_NN read into global_variables (main.f90:116), and the step sets the
+! counters mod_FE.f90 keeps for it.  Link with -lgpmood.
 module time_stepper
    use iso_c_binding
    use parameters
@@ -42,6 +44,14 @@ contains
          stop 1
       end if
       allocate(gpm_order(lb:le, nb:ne))
+      if ((method == NN_GP_MOOD) .or. (method == NN_GP_MOOD_CC)) then
+         rc = gpm_set_reference_nn(gpm_handle, int(L, c_int32_t), int(length, c_int32_t), weight0, bias0, &
+            weight1, bias1, weight2, bias2)
+         if (rc /= 0) then
+            print *, 'gpm_set_reference_nn failed: ', trim(gpm_error_message(gpm_handle))
+            stop 1
+         end if
+      end if
    end subroutine gpm_open
 
    subroutine time_stepping(Uin, Uout)
@@ -52,6 +62,8 @@ contains
 
       if (.not. c_associated(gpm_handle)) call gpm_open()
       t_rk = t
+      ! main.f90:147 has advanced niter for this step; the library advances it itself
+      rc = gpm_set_niter(gpm_handle, int(niter - 1, c_int64_t))
       rc = gpm_time_stepping(gpm_handle, Uin, Uout, t, dt, gpm_order, st)
       if (rc /= 0) then
          print *, 'gpm_time_stepping failed: ', trim(gpm_error_message(gpm_handle))
@@ -61,8 +73,17 @@ contains
       if (integrator == SSP_RK3) t_rk = t + 0.5*dt
       CellGPO = gpm_order
       count_detected_cell = int(st%count_detected_cell)
-      count_detected_cell_a_posteriori = int(st%count_detected_cell)
-      count_detected_cell_a_priori = 0
+      count_detected_cell_a_priori = int(st%count_detected_cell_a_priori)
+      count_detected_cell_a_posteriori = count_detected_cell - count_detected_cell_a_priori
+      if ((method == NN_GP_MOOD) .or. (method == NN_GP_MOOD_CC)) then
+         ! CellGPO_priori, steps_NN_produced_NAN and count_steps_NN_produced_NAN (mod_FE.f90:49-56,154-157)
+         rc = gpm_get_cellgpo_priori(gpm_handle, gpm_order)
+         CellGPO_priori = gpm_order
+         if (st%nn_nan_passes > 0) then
+            steps_NN_produced_NAN(niter) = 1
+            count_steps_NN_produced_NAN = count_steps_NN_produced_NAN + int(st%nn_nan_passes)
+         end if
+      end if
    end subroutine time_stepping
 
 end module time_stepper

@@ -33,6 +33,7 @@ extern "C" {
 #define GPM_FOG 1
 #define GPM_GP_MOOD 2
 #define GPM_NN_GP_MOOD 203
+#define GPM_NN_GP_MOOD_CC 204
 #define GPM_FE 8
 #define GPM_SSP_RK2 9
 #define GPM_SSP_RK3 10
@@ -70,7 +71,9 @@ typedef struct gpm_config {
     int32_t lf, nf;       /* interior cells in x and y (>= 8 each) */
     int32_t mord;         /* Mord: 1, 3, 5 or 7 */
     int32_t ngp;          /* Gauss points per face: 1..5 */
-    int32_t method;       /* GPM_GP_MOOD; GPM_NN_GP_MOOD (lf == nf, copy boundary, SSP_RK3);
+    int32_t method;       /* GPM_GP_MOOD; GPM_NN_GP_MOOD, GPM_NN_GP_MOOD_CC (Mord 3) (copy boundary,
+                             no source term; a network from gpm_set_reference_nn, or for
+                             NN_GP_MOOD a learned predictor from gpm_set_predictor);
                              GPM_FOG, first-order Godunov (mod_FOG.f90; ngp = 1, integrator FE) */
     int32_t integrator;   /* GPM_FE, GPM_SSP_RK2, GPM_SSP_RK3 or GPM_SSP_RK4 (time_stepper.f90:17-21;
                              SSP_RK4 only with a copy boundary and no source term) */
@@ -99,6 +102,13 @@ typedef struct gpm_stats {
     int64_t a_priori_lowered;      /* cells started below Mord by the learned predictor */
     double t;                      /* simulation time */
     double dt;                     /* dt of the last step */
+    /* the reference network (gpm_set_reference_nn); count_detected_cell
+       includes its a priori detections, as the reference counts them */
+    int64_t count_detected_cell_a_priori; /* a priori detections of the last step */
+    int64_t nn_nan_passes;         /* passes of the last step whose check lowered a cell: the
+                                      increments of count_steps_NN_produced_NAN (mod_FE.f90:154-157) */
+    int64_t nn_nan_passes_total;   /* the same since the last reset */
+    int64_t niter;                 /* niter: steps started so far (gpm_set_niter) */
 } gpm_stats;
 
 typedef struct gpm_ctx gpm_ctx;
@@ -241,6 +251,47 @@ int gpm_features(gpm_ctx *ctx, const int32_t *cells_dev, int32_t n, float *x_dev
 int gpm_set_apriori_orders(gpm_ctx *ctx, const int8_t *orders);
 /* a priori orders of the last stage (int8, device) */
 int gpm_get_apriori_orders(gpm_ctx *ctx, int8_t *pred_dev);
+
+/* ------------------------------------------------------------------------
+ * The reference's own network (methods NN_GP_MOOD and NN_GP_MOOD_CC with a
+ * network trained by neural_network/train_function.py and written by
+ * save_NN_for_fortran.py).  From step nsteps_with_no_NN + 1 = 11 on
+ * (global_variables.f90:97; niter as main.f90:147 counts it), every RK stage
+ * predicts each interior cell's order from its R+1 stencil in FP32
+ * (compute_CellGPO_with_NN, mod_NN.f90:14-58) and checks only PAD afterwards
+ * (NN_DETECTION, mod_subroutine_NN_MOOD.f90:13-101); NN_GP_MOOD_CC also blends
+ * the order-3 reconstruction (mod_NN_GP_MOOD_CC.f90:11-85).  Steps 1-10 are
+ * plain GP-MOOD.  Mord 3 or 5; L >= 4*szs_sphere(Mord+2) (52 for Mord 3), L <=
+ * 256, length <= 64.
+ *
+ * gpm_set_reference_nn replaces the arrays load_NN fills (global_variables.f90:
+ * 74-80) and takes them in their Fortran layouts: weight0(length, L),
+ * bias0(length), weight1(length, length), bias1(length), weight2(2, length),
+ * bias2(2).  gpm_load_reference_nn is load_NN itself (mod_NN.f90:60-117): it
+ * reads "<nn_filename>.txt" (blanks trimmed), one number per record, each
+ * weight matrix row by row. */
+int gpm_set_reference_nn(gpm_ctx *ctx, int32_t L, int32_t length, const float *weight0, const float *bias0,
+                         const float *weight1, const float *bias1, const float *weight2, const float *bias2);
+int gpm_load_reference_nn(gpm_ctx *ctx, const char *nn_filename, int32_t L, int32_t length);
+/* The reading part of gpm_load_reference_nn alone (host only, no context):
+ * fills caller arrays in the Fortran layouts above.  GPM_EINVAL for a missing
+ * file, a record that is not a number or fewer records than the network holds
+ * (where the reference's read stops with an end-of-file error). */
+int gpm_read_reference_nn(const char *nn_filename, int32_t L, int32_t length, float *weight0, float *bias0,
+                          float *weight1, float *bias1, float *weight2, float *bias2);
+/* niter before the next step (main.f90:129,147): every step first advances it.
+ * The Fortran time_stepping drop-in sets niter - 1, main having advanced it. */
+int gpm_set_niter(gpm_ctx *ctx, int64_t niter);
+/* CellGPO_priori(lb:le, nb:ne) as int32 (global_variables.f90:59) after the
+ * last step: 1 for a priori detections, 3 for constant stencils, Mord
+ * elsewhere and on ghost cells (Mord everywhere after a step without the
+ * network). */
+int gpm_get_cellgpo_priori(gpm_ctx *ctx, int32_t *priori);
+/* The network's FP32 exp on the floats with bit patterns first .. first+n-1,
+ * into y_dev (device memory): the expf of the host C library, for tests. */
+int gpm_nn_expf_bits(gpm_ctx *ctx, uint32_t first, int64_t n, float *y_dev);
+/* the same function compiled for the host, into host memory y */
+int gpm_nn_expf_bits_host(uint32_t first, int64_t n, float *y);
 
 /* ------------------------------------------------------------------------
  * Differentiable form.  No counterpart in the reference: the adjoint of K

@@ -7,7 +7,8 @@
  *
  * It restates, operation for operation, the FP64 arithmetic of the reference
  * Fortran (slug-cfd/gp-mood, gp-mood-code/) for
- *   method in {GP_MOOD, FOG (ngp = 1)},
+ *   method in {GP_MOOD, FOG (ngp = 1), NN_GP_MOOD and NN_GP_MOOD_CC with the
+ *             reference's own network (mod_NN.f90, text weights)},
  *   integrator in {FE, SSP_RK2, SSP_RK3, SSP_RK4},
  *   numFlux in {HLLC, LLF, HLL, unsplit},
  *   BC_type in {Neumann, Periodic, Dirichlet, Reflective, KH_BC, Mach800_BC,
@@ -44,7 +45,9 @@ enum {
     MACH800_BC = 201, DOUBLEMACH800_BC = 301, RMI_BC = 401, DMR_BC = 35, RT_BC = 402
 };
 enum { PROBLEM_RT = 51 };
-enum { METHOD_FOG = 1, METHOD_GP_MOOD = 2 };
+enum { METHOD_FOG = 1, METHOD_GP_MOOD = 2, METHOD_NN_GP_MOOD = 203, METHOD_NN_GP_MOOD_CC = 204 };
+/* global_variables.f90:97 */
+enum { NSTEPS_WITH_NO_NN = 10 };
 
 typedef struct {
     int lf, nf, mord, ngp, bc; /* bc: the reference's BC_type id */
@@ -74,6 +77,18 @@ typedef struct {
     long count_detected;       /* count_detected_cell (global_variables.f90:63) */
     int8_t *init_orders;       /* a priori CellGPO of the interior [nf][lf] (NULL: Mord) */
     int passes[5];             /* MOOD passes per stage of the last step */
+    /* the reference's network (mod_NN.f90; global_variables.f90:74-80, Fortran
+       layouts: weight0(length, L), weight1(length, length), weight2(2, length)) */
+    int niter;                 /* main.f90:129,147: steps started so far */
+    int nn_L, nn_len;          /* L and length of the parameters module */
+    float *nn_w0, *nn_b0, *nn_w1, *nn_b1, *nn_w2, *nn_b2;
+    int err;                   /* a configuration the reference cannot run (it stops) */
+    int nn_active;             /* this stage predicts (niter > nsteps_with_no_NN) */
+    long count_apriori;        /* count_detected_cell_a_priori of the step */
+    long nn_nan_passes;        /* passes of the step with NN_DETECTION decreases
+                                  (count_steps_NN_produced_NAN increments, mod_FE.f90:154-157) */
+    int8_t *priori;            /* CellGPO_priori(1:lf, 1:nf) of the last stage */
+    float *values_nn;          /* values_NN(-1:lf+2, -1:nf+2, 2), global_variables.f90:99 */
 } orc_t;
 
 static const double GAMMA = 1.4;
@@ -135,6 +150,9 @@ orc_t *orc_create(int lf, int nf, int mord, int ngp, int bc, int dmp, int u2, in
     o->detfx = (uint8_t *)malloc((size_t)(lf + 3) * (nf + 2));
     o->detfy = (uint8_t *)malloc((size_t)(lf + 2) * (nf + 3));
     o->decrease = (uint8_t *)malloc((size_t)lf * nf);
+    o->priori = (int8_t *)malloc((size_t)lf * nf);
+    memset(o->priori, mord, (size_t)lf * nf);
+    o->values_nn = (float *)calloc((size_t)(lf + 4) * (nf + 4) * 2, sizeof(float));
     for (int k = 0; k < 7; ++k) o->S[k] = (double *)calloc(ncell * 4, sizeof(double));
     return o;
 }
@@ -145,6 +163,8 @@ void orc_destroy(orc_t *o)
     free(o->zT); free(o->d2x); free(o->d2y); free(o->Uh); free(o->Fx); free(o->Fy);
     free(o->gpo); free(o->detcell); free(o->detfx); free(o->detfy); free(o->decrease);
     for (int k = 0; k < 7; ++k) free(o->S[k]);
+    free(o->nn_w0); free(o->nn_b0); free(o->nn_w1); free(o->nn_b1); free(o->nn_w2); free(o->nn_b2);
+    free(o->priori); free(o->values_nn);
     free(o->init_orders); free(o);
 }
 
@@ -536,7 +556,8 @@ static int decide(const orc_t *o, const double *Uin, const double *Uout, int l, 
     double p = pressure(uo);
     if (p <= 0. || isnan(p)) return 1;
     int shock = shock_detector(o, Uin, l, n);
-    if (!(o->dmp && shock)) return 0;
+    /* NN_DETECTION (mod_subroutine_NN_MOOD.f90:13-101) is this test without DMP */
+    if (!(o->dmp && !o->nn_active && shock)) return 0;
 #define R(a, b) Uin[CELL(o, a, b) * 4]
     double UB = R(l, n - 1), UL = R(l - 1, n), UM = R(l, n), UR = R(l + 1, n), UT = R(l, n + 1);
     double mn = fmin2(fmin2(fmin2(fmin2(UM, UL), UB), UT), UR);
@@ -676,12 +697,185 @@ static void fog_face_loop(orc_t *o, const double *Uin, double *Uout, double dtdx
         }
 }
 
+
+/* ---- The reference's own network (NN_GP_MOOD, NN_GP_MOOD_CC) ---- */
+
+/* ixiy_sp1 (GP_init.f90:273-343): offsets of the R+1 sphere stencil, the first
+   szs_sphere(Mord+2) entries. */
+static const int SP1_X[25] = {0, -1, 0, 1, 0, -1, -1, 1, 1, 0, -2, 0, 2, -3, -2, -1, 0, 1, 2, 3, 2, 1, 0, -1, -2};
+static const int SP1_Y[25] = {0, 0, 1, 0, -1, -1, 1, 1, -1, -2, 0, 2, 0, 0, 1, 2, 3, 2, 1, 0, -1, -2, -3, -2, -1};
+
+/* Flat index of values_NN(l, n, c) (c = 1, 2) in its Fortran storage,
+   dimension(-1:lf+2, -1:nf+2, 2). */
+static inline size_t vnn_index(const orc_t *o, int l, int n, int c)
+{
+    return (size_t)(l + 1) + (size_t)(n + 1) * (o->lf + 4) + (size_t)(c - 1) * (o->lf + 4) * (o->nf + 4);
+}
+
+/* sigmoid (mod_NN.f90:119-131): one / (one + exp(-x)) in real(4); exp of a
+   real(4) argument is the C library's expf. */
+static inline float nn_sigmoid(float x) { return 1.0f / (1.0f + expf(-x)); }
+
+/* forward (mod_NN.f90:165-181): x0 = matmul(weight0, x) + bias0, sigmoid,
+   x1 = matmul(weight1, x0) + bias1, sigmoid, r = matmul(weight2, x1) + bias2,
+   softmax for NN_GP_MOOD_CC.  matmul accumulates each row in real(4) from 0,
+   in column order (the Fortran runtime's order, checked on amdflang). */
+static void nn_forward(const orc_t *o, const float *x, float *r)
+{
+    const int L = o->nn_L, len = o->nn_len;
+    float h0[512], h1[512];
+    for (int i = 0; i < len; ++i) {
+        float acc = 0.0f;
+        for (int k = 0; k < L; ++k) acc = acc + o->nn_w0[i + (size_t)k * len] * x[k];
+        h0[i] = nn_sigmoid(acc + o->nn_b0[i]);
+    }
+    for (int i = 0; i < len; ++i) {
+        float acc = 0.0f;
+        for (int k = 0; k < len; ++k) acc = acc + o->nn_w1[i + (size_t)k * len] * h0[k];
+        h1[i] = nn_sigmoid(acc + o->nn_b1[i]);
+    }
+    for (int c = 0; c < 2; ++c) {
+        float acc = 0.0f;
+        for (int k = 0; k < len; ++k) acc = acc + o->nn_w2[c + (size_t)k * 2] * h1[k];
+        r[c] = acc + o->nn_b2[c];
+    }
+    if (o->method == METHOD_NN_GP_MOOD_CC) { /* softmax, mod_NN.f90:133-147 */
+        r[0] = expf(r[0]);
+        r[1] = expf(r[1]);
+        const float den = r[0] + r[1];
+        r[0] = r[0] / den;
+        r[1] = r[1] / den;
+    }
+}
+
+/* compute_CellGPO_with_NN (mod_NN.f90:14-58) on the stage input as it is
+   before Boundary_C(Uin) (mod_FE.f90:51-56 precede :64): for every interior
+   cell, the R+1 stencil in real(4), variable fastest, then L - 4*sz entries of
+   -666 (format_input, mod_append_NN_dataset.f90:81-126); a stencil whose every
+   variable spans less than 1e-10 (maxval/minval skip NaN) is order 3, else
+   r(1) > r(2) gives order 1 (an a priori detection) and anything else order 3.
+   values_NN(l, n, :) is indexed with the parameter L, as written in the
+   reference (mod_NN.f90:41,44), so each row's writes land on values_NN(L, n, :):
+   its last cell's values. */
+static int nn_orders(orc_t *o, const double *Uin)
+{
+    const int lf = o->lf, nf = o->nf, L = o->nn_L;
+    const int S = stcl_sz(o->mord + 2);
+    const size_t vsz = (size_t)(lf + 4) * (nf + 4) * 2;
+    if (vnn_index(o, L, nf, 2) >= vsz) return -1;  /* the reference writes past values_NN */
+    for (int n = 1; n <= nf; ++n) {
+        float rlast[2] = {0.0f, 1.0f};
+        for (int l = 1; l <= lf; ++l) {
+            float x[1024];
+            for (int k = 0; k < L; ++k) x[k] = -666.0f;
+            for (int j = 0; j < S; ++j) {
+                const double *u = Uin + CELL(o, l + SP1_X[j], n + SP1_Y[j]) * 4;
+                for (int v = 0; v < 4; ++v) x[4 * j + v] = (float)u[v];
+            }
+            int cst = 1;
+            for (int v = 0; v < 4; ++v) {
+                float mx = -INFINITY, mn = INFINITY;
+                int any = 0;
+                for (int j = 0; j < S; ++j) {
+                    const float z = x[4 * j + v];
+                    if (isnan(z)) continue;
+                    any = 1;
+                    if (z > mx) mx = z;
+                    if (z < mn) mn = z;
+                }
+                if (!any) mx = mn = NAN;
+                if (!((double)(mx - mn) < 1e-10)) cst = 0;
+            }
+            int8_t *g = &o->gpo[CELL(o, l, n)];
+            int8_t *pr = &o->priori[(size_t)(n - 1) * lf + (l - 1)];
+            if (cst) {
+                *g = 3; *pr = 3;
+                rlast[0] = 0.0f; rlast[1] = 1.0f;
+            } else {
+                float r[2];
+                nn_forward(o, x, r);
+                rlast[0] = r[0]; rlast[1] = r[1];
+                if (r[0] > r[1]) {
+                    *g = 1; *pr = 1;
+                    o->count_detected++;
+                    o->count_apriori++;
+                } else {
+                    *g = 3;
+                }
+            }
+        }
+        o->values_nn[vnn_index(o, L, n, 1)] = rlast[0];
+        o->values_nn[vnn_index(o, L, n, 2)] = rlast[1];
+    }
+    return 0;
+}
+
+/* NN_GP_MOOD_CC_ (mod_NN_GP_MOOD_CC.f90:11-85): as GP_MOOD_, but order 1 takes
+   the cell value itself and order 3 is blended with the cell value by
+   values_NN(l, n, :).  Returns -1 for an order other than 1 and 3 (the
+   reference stops there). */
+static int nn_cc_recons(orc_t *o, const double *Uin)
+{
+    const int lf = o->lf, nf = o->nf, mord = o->mord, ngp = o->ngp, sz = o->sz;
+    int bad = 0;
+    for (int n = 0; n <= nf + 1; ++n) {
+        double q[4][25];
+        for (int l = 0; l <= lf + 1; ++l) {
+            int done = 0;
+            for (int i = iL; i <= iB; ++i) {
+                int rec = 0, ord = 0;
+                int me = o->gpo[CELL(o, l, n)];
+                if (i == iL && DFX(o, l - 1, n)) { ord = (int)fmin2(o->gpo[CELL(o, l - 1, n)], me); rec = 1; }
+                else if (i == iR && DFX(o, l, n)) { ord = (int)fmin2(o->gpo[CELL(o, l + 1, n)], me); rec = 1; }
+                else if (i == iB && DFY(o, l, n - 1)) { ord = (int)fmin2(o->gpo[CELL(o, l, n - 1)], me); rec = 1; }
+                else if (i == iT && DFY(o, l, n)) { ord = (int)fmin2(o->gpo[CELL(o, l, n + 1)], me); rec = 1; }
+                if (!rec) continue;
+                if (!done) {
+                    for (int s = 0; s < sz; ++s) {
+                        const double *u = Uin + CELL(o, l + IXIY(o, mord, s, 0), n + IXIY(o, mord, s, 1)) * 4;
+                        for (int k = 0; k < 4; ++k) q[k][s] = u[k];
+                    }
+                    done = 1;
+                }
+                const double *uc = Uin + CELL(o, l, n) * 4;
+                const double v1 = (double)o->values_nn[vnn_index(o, l, n, 1)];
+                const double v2 = (double)o->values_nn[vnn_index(o, l, n, 2)];
+                for (int j = 0; j < ngp; ++j)
+                    for (int k = 0; k < 4; ++k) {
+                        double uh;
+                        if (ord == 1) {
+                            uh = uc[k];
+                        } else if (ord == 3) {
+                            double acc = 0.0;
+                            for (int s = 0; s < stcl_sz(3); ++s) acc = acc + ZT(o, 3, s, i, j) * q[k][s];
+                            uh = v1 * uc[k] + v2 * acc;
+                        } else {
+                            bad = 1;
+                            uh = 0.0;
+                        }
+                        o->Uh[UHI(o, k, i, j, l, n)] = uh;
+                    }
+            }
+        }
+    }
+    return bad ? -1 : 0;
+}
+
 static void orc_fe(orc_t *o, double *Uin, double *Uout, double dt, int first, int *npass)
 {
     const int lf = o->lf, nf = o->nf;
     const size_t ncell = (size_t)o->W * o->H;
-    if (first) o->count_detected = 0;
+    const int nn_method = o->method == METHOD_NN_GP_MOOD || o->method == METHOD_NN_GP_MOOD_CC;
+    if (first) { o->count_detected = 0; o->count_apriori = 0; o->nn_nan_passes = 0; }
     for (size_t c = 0; c < ncell; ++c) { o->gpo[c] = (int8_t)o->mord; o->detcell[c] = 1; }
+    /* values_NN := (0, 1), CellGPO_priori := Mord (mod_FE.f90:45-49) */
+    for (size_t c = 0; c < (size_t)(lf + 4) * (nf + 4); ++c) {
+        o->values_nn[c] = 0.0f;
+        o->values_nn[c + (size_t)(lf + 4) * (nf + 4)] = 1.0f;
+    }
+    memset(o->priori, o->mord, (size_t)lf * nf);
+    o->nn_active = nn_method && o->niter > NSTEPS_WITH_NO_NN && o->nn_w0;
+    if (o->nn_active && nn_orders(o, Uin)) o->err = 1;
     /* a priori orders (compute_CellGPO_with_NN, mod_FE.f90:51-56): interior only */
     if (o->init_orders)
         for (int n = 1; n <= nf; ++n)
@@ -700,7 +894,11 @@ static void orc_fe(orc_t *o, double *Uin, double *Uout, double dt, int first, in
     }
     for (; o->method != METHOD_FOG;) {
         ++passes;
-        gp_mood_recons(o, Uin);
+        if (o->method == METHOD_NN_GP_MOOD_CC && o->niter > NSTEPS_WITH_NO_NN) {
+            if (nn_cc_recons(o, Uin)) o->err = 1;
+        } else {
+            gp_mood_recons(o, Uin);
+        }
 #pragma omp parallel for schedule(static)
         for (int n = 0; n <= nf; ++n)
             for (int l = 0; l <= lf; ++l) {
@@ -724,6 +922,7 @@ static void orc_fe(orc_t *o, double *Uin, double *Uout, double dt, int first, in
                 }
             }
         if (detection(o, Uin, Uout)) break;
+        if (o->nn_active) o->nn_nan_passes++;
     }
     if (o->gravity) {
         for (size_t c = 0; c < ncell; ++c) {
@@ -791,8 +990,22 @@ void orc_ssprk3(orc_t *o, double *U, double *Ur, double dt) { orc_step(o, U, Ur,
    the current t_rk).  Optional outputs: dts[nsteps], counts[nsteps],
    masks[nsteps][nf][lf] (CellGPO after the last stage of each step),
    passes[nsteps][3]. */
+int orc_run2(orc_t *o, double *U, int nsteps, double *t_io, double tmax, double *dts, long *counts,
+             int8_t *masks, int *passes, long *apriori, long *nan_passes, int8_t *priori);
+
 int orc_run(orc_t *o, double *U, int nsteps, double *t_io, double tmax,
             double *dts, long *counts, int8_t *masks, int *passes)
+{
+    return orc_run2(o, U, nsteps, t_io, tmax, dts, counts, masks, passes, NULL, NULL, NULL);
+}
+
+/* orc_run with the reference network's per-step counters: apriori[nsteps]
+   (count_detected_cell_a_priori), nan_passes[nsteps] (increments of
+   count_steps_NN_produced_NAN) and priori[nsteps][nf][lf] (CellGPO_priori).
+   Each step first advances niter (main.f90:147).  Returns -1 when the
+   reference would stop (o->err). */
+int orc_run2(orc_t *o, double *U, int nsteps, double *t_io, double tmax, double *dts, long *counts,
+             int8_t *masks, int *passes, long *apriori, long *nan_passes, int8_t *priori)
 {
     const int lf = o->lf, nf = o->nf;
     const size_t n4 = (size_t)o->W * o->H * 4;
@@ -800,8 +1013,12 @@ int orc_run(orc_t *o, double *U, int nsteps, double *t_io, double tmax,
     double t = t_io ? *t_io : 0.0;
     orc_bc(o, U);
     for (int s = 0; s < nsteps; ++s) {
+        o->niter++;
         double dt = orc_setdt(o, U, tmax - t);
         orc_step(o, U, Ur, dt, t);
+        if (apriori) apriori[s] = o->count_apriori;
+        if (nan_passes) nan_passes[s] = o->nn_nan_passes;
+        if (priori) memcpy(priori + (size_t)s * lf * nf, o->priori, (size_t)lf * nf);
         orc_bc(o, U);
         t = t + dt;
         memcpy(U, Ur, n4 * sizeof(double));
@@ -815,7 +1032,7 @@ int orc_run(orc_t *o, double *U, int nsteps, double *t_io, double tmax,
     }
     if (t_io) *t_io = t;
     free(Ur);
-    return 0;
+    return o->err ? -1 : 0;
 }
 
 /* Per-stage variant of SSP_RK3 for parity on intermediate data: runs one step
@@ -875,4 +1092,46 @@ void orc_set_initial_orders(orc_t *o, const int8_t *orders)
     if (!orders) return;
     o->init_orders = (int8_t *)malloc((size_t)o->lf * o->nf);
     memcpy(o->init_orders, orders, (size_t)o->lf * o->nf);
+}
+
+/* The reference's network (load_NN, mod_NN.f90:60-117; global_variables.f90:
+   74-80): L inputs, `length` hidden units, arrays in their Fortran layouts
+   (weight0(length, L), bias0(length), weight1(length, length), bias1(length),
+   weight2(2, length), bias2(2)).  Returns -1 for sizes the oracle does not hold. */
+int orc_set_refnn(orc_t *o, int L, int length, const float *w0, const float *b0, const float *w1,
+                  const float *b1, const float *w2, const float *b2)
+{
+    if (L < 4 * stcl_sz(o->mord + 2) || L > 1024 || length < 1 || length > 512 || o->mord + 2 > 7) return -1;
+    o->nn_L = L;
+    o->nn_len = length;
+    float **dst[6] = {&o->nn_w0, &o->nn_b0, &o->nn_w1, &o->nn_b1, &o->nn_w2, &o->nn_b2};
+    const float *src[6] = {w0, b0, w1, b1, w2, b2};
+    const size_t cnt[6] = {(size_t)length * L, (size_t)length, (size_t)length * length, (size_t)length,
+                           (size_t)2 * length, 2};
+    for (int k = 0; k < 6; ++k) {
+        free(*dst[k]);
+        *dst[k] = (float *)malloc(cnt[k] * sizeof(float));
+        memcpy(*dst[k], src[k], cnt[k] * sizeof(float));
+    }
+    return 0;
+}
+
+/* niter of main.f90 (steps started so far; the next orc_run step is niter+1). */
+void orc_set_niter(orc_t *o, int niter) { o->niter = niter; }
+int orc_get_niter(const orc_t *o) { return o->niter; }
+int orc_error(const orc_t *o) { return o->err; }
+
+/* The network alone on one input vector x[L] (forward, mod_NN.f90:165-181). */
+void orc_nn_forward(const orc_t *o, const float *x, float *r) { nn_forward(o, x, r); }
+
+/* The C library's expf on the floats with bit patterns first .. first+n-1. */
+void orc_expf_bits(uint32_t first, long n, float *out)
+{
+#pragma omp parallel for schedule(static)
+    for (long i = 0; i < n; ++i) {
+        const uint32_t b = first + (uint32_t)i;
+        float x;
+        memcpy(&x, &b, 4);
+        out[i] = expf(x);
+    }
 }

@@ -16,7 +16,7 @@ BC_CODES = {"neumann": 17, "periodic": 18, "dirichlet": 19, "reflective": 20, "k
 FLUX_CODES = {"hllc": 100, "llf": 101, "hll": 102, "unsplit": 103}
 INTEGRATORS = {"fe": 8, "ssp_rk2": 9, "ssp_rk3": 10, "ssp_rk4": 12}
 PROBLEM_RT = 51
-METHODS = {"gp_mood": 2, "fog": 1}
+METHODS = {"gp_mood": 2, "fog": 1, "nn_gp_mood": 203, "nn_gp_mood_cc": 204}
 CASCADES = {"reference": 0, "full": 1}
 
 _lib = None
@@ -49,6 +49,13 @@ def lib():
         L.orc_passes.argtypes = [P, P]
         L.orc_hllc.argtypes = [P, P, I, P]
         L.orc_set_initial_orders.argtypes = [P, P]
+        L.orc_run2.argtypes = [P, P, I, P, D, P, P, P, P, P, P, P]
+        L.orc_set_refnn.argtypes = [P, I, I, P, P, P, P, P, P]
+        L.orc_set_niter.argtypes = [P, I]
+        L.orc_get_niter.argtypes = [P]
+        L.orc_error.argtypes = [P]
+        L.orc_nn_forward.argtypes = [P, P, P]
+        L.orc_expf_bits.argtypes = [ctypes.c_uint32, ctypes.c_long, P]
         _lib = L
     return _lib
 
@@ -111,6 +118,47 @@ class Oracle:
                       _ptr(m), _ptr(passes))
         return dict(dt=dts, counts=counts, masks=m, passes=passes, t=tt.value)
 
+    def set_reference_nn(self, weight0, bias0, weight1, bias1, weight2, bias2):
+        """The reference network, matrices with row i = output i (weight0
+        (length, L), ..., weight2 (2, length)); stored in Fortran layouts."""
+        w0 = np.asarray(weight0, dtype=np.float32)
+        length, L = w0.shape
+        self._nn = [np.ascontiguousarray(np.asarray(a, dtype=np.float32).reshape(shape).T).ravel()
+                    for a, shape in ((w0, (length, L)), (bias0, (length, 1)), (weight1, (length, length)),
+                                     (bias1, (length, 1)), (weight2, (2, length)), (bias2, (2, 1)))]
+        self.nn_L = L
+        if lib().orc_set_refnn(self._h, L, length, *[_ptr(a) for a in self._nn]):
+            raise ValueError("network sizes not supported by the oracle")
+
+    def set_niter(self, niter):
+        lib().orc_set_niter(self._h, int(niter))
+
+    def nn_forward(self, x):
+        """forward (mod_NN.f90:165-181) of one input vector x[L]."""
+        x = np.ascontiguousarray(x, dtype=np.float32)
+        r = np.zeros(2, dtype=np.float32)
+        lib().orc_nn_forward(self._h, _ptr(x), _ptr(r))
+        return r
+
+    def run_nn(self, U, nsteps, t=0.0, tmax=1e10):
+        """run() with the reference network's per-step counters and
+        CellGPO_priori; each step advances niter first (main.f90:147)."""
+        assert U.flags.c_contiguous and U.dtype == np.float64
+        dts = np.zeros(nsteps)
+        counts = np.zeros(nsteps, dtype=np.int64)
+        apriori = np.zeros(nsteps, dtype=np.int64)
+        nan_passes = np.zeros(nsteps, dtype=np.int64)
+        passes = np.zeros((nsteps, 3), dtype=np.int32)
+        m = np.zeros((nsteps, self.nf, self.lf), dtype=np.int8)
+        pri = np.zeros((nsteps, self.nf, self.lf), dtype=np.int8)
+        tt = ctypes.c_double(t)
+        rc = lib().orc_run2(self._h, _ptr(U), nsteps, ctypes.byref(tt), tmax, _ptr(dts), _ptr(counts), _ptr(m),
+                            _ptr(passes), _ptr(apriori), _ptr(nan_passes), _ptr(pri))
+        if rc:
+            raise ValueError("the reference stops on this configuration")
+        return dict(dt=dts, counts=counts, masks=m, passes=passes, apriori=apriori, nan_passes=nan_passes,
+                    priori=pri, t=tt.value)
+
     def step(self, U, dt, t=0.0):
         """One time_stepping call from full array U at time t; returns Ur (U's
         ghosts are refilled as by the reference)."""
@@ -133,6 +181,13 @@ class Oracle:
         p = np.zeros(3, dtype=np.int32)
         lib().orc_passes(self._h, _ptr(p))
         return outs, masks, p
+
+
+def expf_bits(first, n):
+    """The C library's expf of the floats with bit patterns first .. first+n-1."""
+    out = np.empty(n, dtype=np.float32)
+    lib().orc_expf_bits(int(first), int(n), _ptr(out))
+    return out
 
 
 def riemann(flux, ul, ur, direction):

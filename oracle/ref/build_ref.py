@@ -114,6 +114,13 @@ CONFIGS = {
     "fog_n32_ref_hllc": (32, 3, 1, "reflective", True, True, True),
     "fog_dmr_64x16": (64, 3, 1, "dmr", True, True, True),
     "fog_rt_16x48_uns": (16, 3, 1, "rt", True, True, True, "unsplit"),
+    # the reference's own network (method NN_GP_MOOD / NN_GP_MOOD_CC; mod_NN.f90,
+    # weights read by load_NN from refnn.txt in the working directory)
+    "nn_r1_n64_neu": (64, 3, 2, "neumann", True, True, True),
+    "nn_r1_n64_per_l52": (64, 3, 2, "periodic", True, True, True, "unsplit"),
+    "nn_r1_64x48_kh_rk2": (64, 3, 2, "kh", True, True, True),
+    "nn_r2_n128_ref": (128, 5, 3, "reflective", True, True, True),
+    "nncc_r1_n64_neu": (64, 3, 2, "neumann", True, True, True),
 }
 
 # The problem files' own configurations (problems/*.f90: grid, domain, method,
@@ -172,6 +179,11 @@ EXTRAS = {
     "fog_n32_ref_hllc": dict(integrator="fe", method="FOG"),
     "fog_dmr_64x16": dict(nf=16, lx="4.", ly="1.", integrator="fe", method="FOG"),
     "fog_rt_16x48_uns": dict(nf=48, lx="0.5", ly="1.5", problem="RT", integrator="fe", method="FOG"),
+    "nn_r1_n64_neu": dict(method="NN_GP_MOOD", nn_l=57, nn_file="refnn"),
+    "nn_r1_n64_per_l52": dict(method="NN_GP_MOOD", nn_l=52, nn_file="refnn"),
+    "nn_r1_64x48_kh_rk2": dict(nf=48, method="NN_GP_MOOD", integrator="ssp_rk2", nn_l=57, nn_file="refnn"),
+    "nn_r2_n128_ref": dict(method="NN_GP_MOOD", nn_l=100, nn_len=48, nn_file="refnn"),
+    "nncc_r1_n64_neu": dict(method="NN_GP_MOOD_CC", nn_l=57, nn_file="refnn"),
 }
 
 
@@ -193,7 +205,7 @@ def config(name):
 # configs for which the Fortran drop-in (reference modules + product
 # time_stepper over libgpmood) is also built
 DROPIN_CONFIGS = ["r2_n64_neu", "r3_n64_ref", "r3_n64_kh_uns", "r1_dmr_64x16", "r1_rt_16x48", "r1_m800_32_fe",
-                  "r2_n32_ref_rk4", "fog_n64_kh_uns", "fog_m800_32_hll"]
+                  "r2_n32_ref_rk4", "fog_n64_kh_uns", "fog_m800_32_hll", "nn_r1_n64_neu", "nncc_r1_n64_neu"]
 
 PARAMS_TEMPLATE = """module parameters
    use constants
@@ -212,8 +224,8 @@ PARAMS_TEMPLATE = """module parameters
    logical :: write_NN_dataset=.false.
    integer, parameter :: dataset_size = 1
    integer, parameter :: L={nn_l}
-   integer, parameter :: length=60
-   character(100) :: NN_filename=''
+   integer, parameter :: length={nn_len}
+   character(100) :: NN_filename='{nn_file}'
    integer, parameter :: numFlux = {flux}
    integer, parameter :: IO_freqStep = -100
    real(PR), parameter:: IO_freqTime = -1.e-3
@@ -247,7 +259,8 @@ def build(name, dropin=False):
         f.write(PARAMS_TEMPLATE.format(mord=mord, ngp=ngp, dmp=fbool(dmp), u2=fbool(u2),
                                        u2tol=fbool(u2tol), n=n, bc=BC_NAMES[bc], flux=FLUX_NAMES[flux],
                                        nf=e["nf"], lx=e["lx"], ly=e["ly"], problem=e["problem"],
-                                       integrator=INTEGRATOR_NAMES[e["integrator"]], nn_l=min(32, n),
+                                       integrator=INTEGRATOR_NAMES[e["integrator"]], nn_l=e.get("nn_l", min(32, n)),
+                                       nn_len=e.get("nn_len", 60), nn_file=e.get("nn_file", ""),
                                        method=e["method"]))
     flags = ["-fdefault-real-8", "-O3", "-ffp-contract=off", "-J", out, "-I", out]
     if n * extras(name)["nf"] >= 1024 * 1024:

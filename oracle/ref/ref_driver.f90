@@ -13,6 +13,12 @@
 !                after every step (mode 0)
 !   U_final.bin  U(4,lb:le,nb:ne) after the last step
 !   U_stage.bin  (mode 1) U after each RK stage of the first step
+!   nn.bin       (methods NN_GP_MOOD, NN_GP_MOOD_CC) per step:
+!                count_detected_cell_a_priori, count_detected_cell_a_posteriori,
+!                count_steps_NN_produced_NAN (i32 each)
+!   priori.bin   (the same methods) CellGPO_priori(1:lf,1:nf) as int8 per step
+! The NN methods load their network with load_NN (mod_NN.f90:60-117) first, as
+! main.f90:116 does.
 !
 ! Usage: gpm_ref.exe <ic.bin> <nsteps> <outdir> <mode>
 !   mode 0: call time_stepping (time_stepper.f90:12-23) exactly as main.f90:155
@@ -28,10 +34,12 @@ program gpm_ref_driver
    use BC
    use mod_FE
    use time_stepper
+   use mod_NN
    implicit none
 
    character(len=1024) :: icfile, outdir, arg
-   integer :: nsteps, mode, s, uo, um, us, ust
+   integer :: nsteps, mode, s, uo, um, us, ust, unn, upr
+   logical :: nnm
    real(PR), allocatable :: Uint(:,:,:)
    real(PR), allocatable :: U1(:,:,:), U2s(:,:,:)
    real(PR) :: tic, tac
@@ -44,6 +52,8 @@ program gpm_ref_driver
 
    call init_mesh()
    call GP_presim_sphere()
+   nnm = (method == NN_GP_MOOD) .or. (method == NN_GP_MOOD_CC)
+   if (nnm) call load_NN()
 
    open(newunit=uo, file=trim(outdir)//'/tables.bin', access='stream', form='unformatted', status='replace')
    write(uo) int(Mord,4), int(ngp,4), int(sz_sphere,4), int(lf,4), int(nf,4), int(BC_type,4)
@@ -72,6 +82,10 @@ program gpm_ref_driver
    open(newunit=uo, file=trim(outdir)//'/steps.bin', access='stream', form='unformatted', status='replace')
    open(newunit=um, file=trim(outdir)//'/masks.bin', access='stream', form='unformatted', status='replace')
    if (mode == 1) open(newunit=ust, file=trim(outdir)//'/U_stage.bin', access='stream', form='unformatted', status='replace')
+   if (nnm) then
+      open(newunit=unn, file=trim(outdir)//'/nn.bin', access='stream', form='unformatted', status='replace')
+      open(newunit=upr, file=trim(outdir)//'/priori.bin', access='stream', form='unformatted', status='replace')
+   end if
 
    niter = 0
    t = 0.
@@ -105,11 +119,21 @@ program gpm_ref_driver
       t = t + dt
       U = Ur
       write(uo) dt, t, int(count_detected_cell,4)
+      if (nnm) then
+         write(unn) int(count_detected_cell_a_priori,4), int(count_detected_cell_a_posteriori,4), &
+            int(count_steps_NN_produced_NAN,4)
+         m8 = int(CellGPO_priori(1:lf,1:nf), 1)
+         write(upr) m8
+      end if
    end do
    call cpu_time(tac)
    close(uo)
    close(um)
    if (mode == 1) close(ust)
+   if (nnm) then
+      close(unn)
+      close(upr)
+   end if
 
    open(newunit=us, file=trim(outdir)//'/U_final.bin', access='stream', form='unformatted', status='replace')
    write(us) U

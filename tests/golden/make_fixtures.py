@@ -45,7 +45,9 @@ def read_tables(path):
     return out
 
 
-def run_ref(cfg, U0, nsteps, mode):
+def run_ref(cfg, U0, nsteps, mode, nn=None):
+    """Run the reference build of `cfg`; `nn`: the network (refnn_weights) that
+    load_NN reads from refnn.txt in the working directory."""
     n = build_ref.CONFIGS[cfg][0]
     nf = build_ref.extras(cfg)["nf"]
     exe = os.path.join(REF, cfg, "gpm_ref.exe")
@@ -53,15 +55,62 @@ def run_ref(cfg, U0, nsteps, mode):
         build_ref.build(cfg)
     with tempfile.TemporaryDirectory(dir=os.path.join(REF, cfg)) as d:
         U0.tofile(os.path.join(d, "ic.bin"))
+        if nn is not None:
+            write_refnn_text(os.path.join(d, "refnn.txt"), nn)
         subprocess.run(f"ulimit -s unlimited; {exe} {d}/ic.bin {nsteps} {d} {mode}", shell=True, check=True,
-                       stdout=subprocess.DEVNULL)
+                       stdout=subprocess.DEVNULL, cwd=d)
         res = {"tables": read_tables(os.path.join(d, "tables.bin")),
                "steps": np.fromfile(os.path.join(d, "steps.bin"), dtype=STEP_TYPE),
                "masks": np.fromfile(os.path.join(d, "masks.bin"), dtype=np.int8).reshape(-1, nf, n),
                "U": np.fromfile(os.path.join(d, "U_final.bin"), dtype=np.float64).reshape(nf + 8, n + 8, 4)}
         if mode == 1:
             res["U_stage"] = np.fromfile(os.path.join(d, "U_stage.bin"), dtype=np.float64).reshape(3, nf + 8, n + 8, 4)
+        if nn is not None:
+            res["nn"] = np.fromfile(os.path.join(d, "nn.bin"), dtype=np.int32).reshape(-1, 3)
+            res["priori"] = np.fromfile(os.path.join(d, "priori.bin"), dtype=np.int8).reshape(-1, nf, n)
     return res
+
+
+# ixiy_sp1 offsets (GP_init.f90:273-343), for the network's input stencil
+SP1 = [(0, 0), (-1, 0), (0, 1), (1, 0), (0, -1), (-1, -1), (-1, 1), (1, 1), (1, -1), (0, -2), (-2, 0), (0, 2), (2, 0),
+       (-3, 0), (-2, 1), (-1, 2), (0, 3), (1, 2), (2, 1), (3, 0), (2, -1), (1, -2), (0, -3), (-1, -2), (-2, -1)]
+
+
+def refnn_weights(L, length, mord, seed, U0):
+    """A seeded network in the reference's shapes (neural_network/NN.py:
+    Linear(L, length), Linear(length, length), Linear(length, 2), matrices
+    with row i = output i).  Not trained: the parity runs only need decisions
+    that depend on the stencil.  bias2 is set so that about a quarter of the
+    cells of U0 (interior, shape (nf, lf, 4)) predict order 1."""
+    rng = np.random.default_rng(seed)
+    S = 13 if mord == 3 else 25
+    w0 = rng.normal(0.0, 2.0 / np.sqrt(4 * S), (length, L))
+    w0[:, 4 * S:] = rng.normal(0.0, 1e-3, (length, L - 4 * S))
+    b0 = rng.normal(0.0, 0.5, length)
+    w1 = rng.normal(0.0, 2.0 / np.sqrt(length), (length, length))
+    b1 = rng.normal(0.0, 0.5, length)
+    w2 = rng.normal(0.0, 2.0 / np.sqrt(length), (2, length))
+    P = np.pad(U0, ((3, 3), (3, 3), (0, 0)), mode="edge").astype(np.float32)
+    nf, lf = U0.shape[:2]
+    X = np.full((nf * lf, L), -666.0, dtype=np.float64)
+    for j, (ox, oy) in enumerate(SP1[:S]):
+        X[:, 4 * j:4 * j + 4] = P[3 + oy:3 + oy + nf, 3 + ox:3 + ox + lf].reshape(-1, 4)
+    sig = lambda z: 1.0 / (1.0 + np.exp(-z))  # noqa: E731
+    r = sig(sig(X @ w0.T + b0) @ w1.T + b1) @ w2.T
+    d = r[:, 0] - r[:, 1]
+    b2 = np.array([-np.quantile(d, 0.75), 0.0])
+    return {k: np.asarray(v, dtype=np.float32) for k, v in
+            dict(weight0=w0, bias0=b0, weight1=w1, bias1=b1, weight2=w2, bias2=b2).items()}
+
+
+def write_refnn_text(path, w):
+    """The text format of save_NN_for_fortran.py:12-25: every parameter of the
+    network in PyTorch order (weight0, bias0, weight1, bias1, weight2, bias2),
+    each flattened row-major, one number per line."""
+    with open(path, "w") as f:
+        for k in ("weight0", "bias0", "weight1", "bias1", "weight2", "bias2"):
+            for v in np.asarray(w[k], dtype=np.float32).ravel():
+                f.write(repr(float(v)) + "\n")
 
 
 def sha(a):
@@ -121,6 +170,15 @@ STEP_CASES = {
     "fog_dmr_64x16": ("fog_dmr_64x16", "dmr:0", 0, 30),
     "fog_rt_16x48_uns_q43": ("fog_rt_16x48_uns", "quadrants:0", 43, 30),
 }
+# the reference's own network (method NN_GP_MOOD / NN_GP_MOOD_CC): name:
+# (config, IC kind, seed, steps); the network is refnn_weights(seed)
+NN_CASES = {
+    "nn_r1_n64_neu_fq44": ("nn_r1_n64_neu", "four_quadrant", 44, 30),
+    "nn_r1_n64_per_l52_fq45": ("nn_r1_n64_per_l52", "four_quadrant", 45, 30),
+    "nn_r1_64x48_kh_rk2_q46": ("nn_r1_64x48_kh_rk2", "quadrants:0.02", 46, 30),
+    "nn_r2_n128_ref_bl47": ("nn_r2_n128_ref", "blast", 47, 25),
+    "nncc_r1_n64_neu_fq48": ("nncc_r1_n64_neu", "four_quadrant", 48, 30),
+}
 # long runs stored as checksums: name: (config, IC kind, seed, steps)
 HASH_CASES = {
     "r2_n256_neu_fq0_300": ("r2_n256_neu", "four_quadrant", 0, 300),
@@ -174,6 +232,28 @@ def main(argv):
             dt=r["steps"]["dt"], t=r["steps"]["t"], counts=r["steps"]["cnt"], masks=r["masks"], U=r["U"],
             **extra, **{k: v for k, v in r["tables"].items()})
         print("wrote", name, "lowered", int((r["masks"] < mord).sum()))
+    for name, (cfg, kind, seed, nsteps) in NN_CASES.items():
+        if want and name not in want:
+            continue
+        n = build_ref.CONFIGS[cfg][0]
+        e = build_ref.extras(cfg)
+        if ":" in kind:
+            pk, noise = kind.split(":")
+            U0 = ic.problem(pk, n, e["nf"], float(e["lx"]), float(e["ly"]), seed=seed, noise=float(noise))
+        else:
+            U0 = ic.make(kind, n, seed)
+        _, mord, ngp, bc, dmp, u2, u2tol, flux = build_ref.config(cfg)
+        w = refnn_weights(e["nn_l"], e.get("nn_len", 60), mord, seed, U0)
+        r = run_ref(cfg, U0, nsteps, 0, nn=w)
+        np.savez_compressed(
+            os.path.join(OUT, f"refnn_{name}.npz"), n=n, mord=mord, ngp=ngp, bc=bc, dmp=dmp, u2=u2, u2tol=u2tol,
+            flux=flux, nf=e["nf"], lx=float(e["lx"]), ly=float(e["ly"]), integrator=e["integrator"], problem=0,
+            method=e["method"].lower(), L=e["nn_l"], length=e.get("nn_len", 60), kind=kind, seed=seed,
+            nsteps=nsteps, ic_sha=sha(U0), U0=U0, dt=r["steps"]["dt"], t=r["steps"]["t"], counts=r["steps"]["cnt"],
+            masks=r["masks"], U=r["U"], apriori=r["nn"][:, 0], aposteriori=r["nn"][:, 1], nan_cum=r["nn"][:, 2],
+            priori=r["priori"], **w, **{k: v for k, v in r["tables"].items()})
+        print("wrote", name, "a priori", r["nn"][:, 0].sum(), "a posteriori", r["nn"][:, 1].sum(),
+              "nan passes", r["nn"][-1, 2])
     for name, (cfg, kind, seed, nsteps) in HASH_CASES.items():
         if want and name not in want:
             continue
