@@ -207,6 +207,11 @@ OPTION_RUNS = [
                              problem=51), "rayleigh_taylor", 3),
     ("fog_2048", dict(lf=2048, Mord=3, ngp=1, BC_type="kh", numFlux="unsplit", integrator="fe", method=1),
      "quadrants", 1),
+    # the reference's own network (NN_GP_MOOD, a seeded network of the problem
+    # files' L = 57, length = 60; niter past nsteps_with_no_NN) and plain
+    # GP-MOOD on the same R1 workload
+    ("gp_mood_r1_2048_quadrants", dict(lf=2048, Mord=3, ngp=2), "quadrants", 3),
+    ("refnn_r1_2048_quadrants", dict(lf=2048, Mord=3, ngp=2, method=203), "quadrants", 3),
 ]
 
 
@@ -225,6 +230,9 @@ def options_measure(gpm, steps, warmup=3):
         else:
             U0 = gpm.ic.problem(kind, lf, nf, cfg.Lx, cfg.Ly, seed=0, noise=0.0)
         s = gpm.Solver(cfg)
+        if label.startswith("refnn"):
+            s.set_reference_nn(**gpm.refnn.seeded_network(57, 60, seed=0, U=U0))
+            s.set_niter(10)
         s.set_state(U0)
         s.set_time(0.0)
         s.run(warmup)
@@ -240,6 +248,8 @@ def options_measure(gpm, steps, warmup=3):
         out[label] = {"cell_updates_per_s": stages * lf * nf * steps / el, "ms_per_step": 1e3 * el / steps,
                       "stage_kernel_avg_ms": kt["stage_ms"], "mood_kernel_avg_ms": kt["mood_ms"],
                       "max_passes": st["max_passes"], "grid": [lf, nf], "stages_per_step": stages}
+        if label.startswith("refnn"):
+            out[label]["a_priori_fraction"] = st["count_detected_cell_a_priori"] / (stages * lf * nf)
         s.close()
         print(f"[bench]   {label}: {out[label]['cell_updates_per_s'] / 1e9:.3f} G cell-updates/s per stage",
               file=sys.stderr, flush=True)

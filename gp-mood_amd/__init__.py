@@ -284,4 +284,4 @@ class Solver:
                                          capi.ptr(dts), int(dts.size), ctypes.c_void_p(ubar_ptr)), "gpm_adjoint")
 
 
-from . import adjoint, driver  # noqa: E402,F401
+from . import adjoint, driver, refnn  # noqa: E402,F401

@@ -952,19 +952,28 @@ int gpm_set_reference_nn(gpm_ctx *c, int32_t L, int32_t length, const float *wei
         return fail(c, GPM_EUNSUP, "the reference network's stencil (szs_sphere(Mord+2)) is built for Mord 3 and 5");
     const int S = c->mord == 3 ? 13 : 25;
     if (!weight0 || !bias0 || !weight1 || !bias1 || !weight2 || !bias2) return fail(c, GPM_EINVAL, "null argument");
-    if (L < 4 * S || L > 256)
-        return fail(c, GPM_EINVAL, "L must hold the 4*szs_sphere(Mord+2) stencil values and be at most 256");
+    if (L < 4 * S || L > 128)
+        return fail(c, GPM_EINVAL, "L must hold the 4*szs_sphere(Mord+2) stencil values and be at most 128");
     if (length < 1 || length > kRefNnMaxLen) return fail(c, GPM_EUNSUP, "length must be 1..64");
     const bool cc = c->cfg.method == GPM_NN_GP_MOOD_CC;
     // values_NN(L, n, :) must lie inside values_NN(-1:lf+2, -1:nf+2, 2) for every row
     if (cc && (long long)L + 1 >= 3ll * (c->NX + 4))
         return fail(c, GPM_EUNSUP, "NN_GP_MOOD_CC with L >= 3*(lf+4)-1: the reference writes past values_NN");
-    const size_t n0 = (size_t)length * L, n1 = (size_t)length * length, n2 = 2 * (size_t)length;
-    const size_t total = n0 + length + n1 + length + n2 + 2;
-    std::vector<float> h(total);
-    size_t o = 0;
-    auto put = [&](const float *src, size_t cnt) { std::memcpy(h.data() + o, src, cnt * sizeof(float)); o += cnt; };
-    put(weight0, n0); put(bias0, length); put(weight1, n1); put(bias1, length); put(weight2, n2); put(bias2, 2);
+    // the kernel's layout: rows of weight0 and weight1 zero-padded to the
+    // kernel's widths, then the biases and weight2 as given
+    const int KP = refnn_input_pad(L), HP = kRefNnMaxLen;
+    const size_t total = (size_t)length * KP + length + (size_t)length * HP + length + 2 * (size_t)length + 2;
+    std::vector<float> h(total, 0.0f);
+    float *p0 = h.data(), *pb0 = p0 + (size_t)length * KP, *p1 = pb0 + length, *pb1 = p1 + (size_t)length * HP;
+    float *p2 = pb1 + length, *pb2 = p2 + 2 * (size_t)length;
+    for (int i = 0; i < length; ++i) {
+        for (int k = 0; k < L; ++k) p0[(size_t)i * KP + k] = weight0[i + (size_t)k * length];
+        for (int k = 0; k < length; ++k) p1[(size_t)i * HP + k] = weight1[i + (size_t)k * length];
+        pb0[i] = bias0[i];
+        pb1[i] = bias1[i];
+    }
+    std::memcpy(p2, weight2, 2 * (size_t)length * sizeof(float));
+    std::memcpy(pb2, bias2, 2 * sizeof(float));
     if (c->refnn_w) { HIPCHK(c, hipStreamSynchronize(c->stream)); hipFree(c->refnn_w); c->refnn_w = nullptr; }
     HIPCHK(c, dalloc(&c->refnn_w, total));
     HIPCHK(c, hipMemcpyAsync(c->refnn_w, h.data(), total * sizeof(float), hipMemcpyHostToDevice, c->stream));

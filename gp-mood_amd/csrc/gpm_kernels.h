@@ -211,7 +211,8 @@ hipError_t launch_unpack_orders(const order_t *g, int8_t *out, size_t n, hipStre
 // stage input with a copy boundary.
 struct RefNnArgs {
     const double4 *__restrict__ U;  // stage input (interior)
-    const float *__restrict__ w;    // weight0 | bias0 | weight1 | bias1 | weight2 | bias2 (Fortran layouts)
+    const float *__restrict__ w;    // w0p [len][KP] | bias0 | w1p [len][kRefNnMaxLen] | bias1 | weight2(2,len) | bias2
+                                    // (rows zero-padded; KP = refnn_input_pad(L))
     int8_t *pred;                   // CellGPO after the prediction (1 or 3)
     int8_t *priori;                 // CellGPO_priori (1, 3 or Mord)
     float *rowv;                    // [NY][2]: values_NN of each row's last cell (NN_GP_MOOD_CC), or null
@@ -222,6 +223,7 @@ struct RefNnArgs {
 constexpr int kRefNnMaxLen = 64;     // hidden units held in registers
 constexpr int kRefNnThreads = 64;    // one wavefront per workgroup
 hipError_t launch_refnn(const RefNnArgs &a, hipStream_t s);
+int refnn_input_pad(int L);  // padded input width of the network kernel (64 or 128)
 // values_NN(-1:lf+2, -1:nf+2, 2) := (0, 1), then each row's values written at
 // values_NN(L, n, :) in row order, as the reference indexes it (mod_NN.f90:41,44)
 hipError_t launch_refnn_values(float *vnn, const float *rowv, int NX, int NY, int L, bool scatter, hipStream_t s);

@@ -91,34 +91,31 @@ __device__ __forceinline__ float nn_sigmoid(float x) { return 1.0f / (1.0f + nn_
 __constant__ int kSp1X[25] = {0, -1, 0, 1, 0, -1, -1, 1, 1, 0, -2, 0, 2, -3, -2, -1, 0, 1, 2, 3, 2, 1, 0, -1, -2};
 __constant__ int kSp1Y[25] = {0, 0, 1, 0, -1, -1, 1, 1, -1, -2, 0, 2, 0, 0, 1, 2, 3, 2, 1, 0, -1, -2, -3, -2, -1};
 
-// One layer of `len` outputs over `nin` inputs held in LDS ([k][lane]):
-// acc[i] = sum_k W[i + k*len] * x[k], summed from 0 in k order.
-__device__ __forceinline__ void nn_layer(const float *__restrict__ W, const float *xs, int nin, int len,
-                                         float (&acc)[kRefNnMaxLen])
-{
-#pragma unroll
-    for (int i = 0; i < kRefNnMaxLen; ++i) acc[i] = 0.0f;
-    for (int k = 0; k < nin; ++k) {
-        const float xk = xs[k * kRefNnThreads + threadIdx.x];
-        const float *wk = W + (size_t)k * len;
-#pragma unroll
-        for (int i = 0; i < kRefNnMaxLen; ++i)
-            if (i < len) acc[i] = acc[i] + wk[i] * xk;
-    }
-}
-
-// One thread per interior cell, one wavefront per workgroup; LDS holds the
-// inputs (then the first hidden layer) as [k][lane].
-template <int S>
+// One thread per interior cell, one wavefront per workgroup.  The weights
+// are zero-padded to the compile-time widths KP (inputs) and kRefNnMaxLen
+// (hidden units) in the device copy: row i of layer 1 is w0p[i*KP .. i*KP+KP-1]
+// and of layer 2 w1p[i*HP .. ].  Each output sums its row from 0 in input
+// order, so the real terms come first and in the reference's order; the
+// padding terms add +0*0, which leaves every sum unchanged except that a
+// -0 sum becomes +0, and no output depends on the sign of a zero sum (the sum
+// only enters sigmoid(sum + bias), r(1) > r(2) and exp).  The inputs live in
+// registers; the weight rows stream through the scalar cache; the first hidden
+// layer goes through LDS once to be held in registers for layer 2; layer 3 is
+// summed as layer 2 produces its outputs, in the same order.
+template <int S, int KP>
 __global__ __launch_bounds__(kRefNnThreads) void k_refnn(RefNnArgs a)
 {
-    extern __shared__ float xs[];
+    constexpr int HP = kRefNnMaxLen;
+    __shared__ float hs[HP * kRefNnThreads];
     const int NX = a.B.NX, NY = a.B.NY;
     const int lane = threadIdx.x;
     const long long cell = (long long)blockIdx.x * kRefNnThreads + lane;
     const bool valid = cell < (long long)NX * NY;
     const int i = valid ? (int)(cell % NX) : 0, j = valid ? (int)(cell / NX) : 0;
-    // format_input: stencil values, -666 beyond; per-variable max/min skipping NaN
+    const int L = a.L, len = a.len;
+    // format_input: stencil values, -666 up to L, then the padding's zeros;
+    // per-variable max/min skipping NaN
+    float x[KP];
     float mx[4], mn[4];
     bool any[4];
 #pragma unroll
@@ -130,7 +127,7 @@ __global__ __launch_bounds__(kRefNnThreads) void k_refnn(RefNnArgs a)
         const float f[4] = {(float)u.x, (float)u.y, (float)u.z, (float)u.w};
 #pragma unroll
         for (int v = 0; v < 4; ++v) {
-            xs[(4 * s + v) * kRefNnThreads + lane] = f[v];
+            x[4 * s + v] = f[v];
             if (!__builtin_isnan(f[v])) {
                 any[v] = true;
                 if (f[v] > mx[v]) mx[v] = f[v];
@@ -138,34 +135,45 @@ __global__ __launch_bounds__(kRefNnThreads) void k_refnn(RefNnArgs a)
             }
         }
     }
-    for (int k = 4 * S; k < a.L; ++k) xs[k * kRefNnThreads + lane] = -666.0f;
+#pragma unroll
+    for (int k = 4 * S; k < KP; ++k) x[k] = k < L ? -666.0f : 0.0f;
     bool cst = true;
 #pragma unroll
     for (int v = 0; v < 4; ++v) {
         const float d = any[v] ? mx[v] - mn[v] : __builtin_nanf("");
         if (!((double)d < 1e-10)) cst = false;
     }
-    const int len = a.len, L = a.L;
-    const float *w0 = a.w, *b0 = w0 + (size_t)len * L, *w1 = b0 + len, *b1 = w1 + (size_t)len * len;
-    const float *w2 = b1 + len, *b2 = w2 + 2 * len;
-    // every lane of the wavefront runs the network (cst lanes discard it), so
-    // the loops stay uniform
-    float acc[kRefNnMaxLen];
-    nn_layer(w0, xs, L, len, acc);
-    __syncthreads();  // xs is rewritten with the first hidden layer
+    const float *__restrict__ w0 = a.w;                      // [len][KP]
+    const float *__restrict__ b0 = w0 + (size_t)len * KP;     // [len]
+    const float *__restrict__ w1 = b0 + len;                  // [len][HP]
+    const float *__restrict__ b1 = w1 + (size_t)len * HP;     // [len]
+    const float *__restrict__ w2 = b1 + len;                  // weight2(2, length), Fortran layout
+    const float *__restrict__ b2 = w2 + 2 * len;
+    // every lane runs the network (cst lanes discard it): uniform loops
+    for (int q = 0; q < HP; ++q) {
+        float h = 0.0f;
+        if (q < len) {
+            const float *__restrict__ w = w0 + (size_t)q * KP;
+            float acc = 0.0f;
 #pragma unroll
-    for (int q = 0; q < kRefNnMaxLen; ++q)
-        if (q < len) xs[q * kRefNnThreads + lane] = nn_sigmoid(acc[q] + b0[q]);
-    __syncthreads();
-    nn_layer(w1, xs, len, len, acc);
-    float r0 = 0.0f, r1 = 0.0f;
-#pragma unroll
-    for (int k = 0; k < kRefNnMaxLen; ++k)
-        if (k < len) {
-            const float h = nn_sigmoid(acc[k] + b1[k]);
-            r0 = r0 + w2[2 * k] * h;
-            r1 = r1 + w2[2 * k + 1] * h;
+            for (int k = 0; k < KP; ++k) acc = acc + w[k] * x[k];
+            h = nn_sigmoid(acc + b0[q]);
         }
+        hs[q * kRefNnThreads + lane] = h;
+    }
+    float h0[HP];
+#pragma unroll
+    for (int k = 0; k < HP; ++k) h0[k] = hs[k * kRefNnThreads + lane];
+    float r0 = 0.0f, r1 = 0.0f;
+    for (int q = 0; q < len; ++q) {
+        const float *__restrict__ w = w1 + (size_t)q * HP;
+        float acc = 0.0f;
+#pragma unroll
+        for (int k = 0; k < HP; ++k) acc = acc + w[k] * h0[k];
+        const float h = nn_sigmoid(acc + b1[q]);
+        r0 = r0 + w2[2 * q] * h;
+        r1 = r1 + w2[2 * q + 1] * h;
+    }
     r0 = r0 + b2[0];
     r1 = r1 + b2[1];
     if (a.softmax) {  // softmax, mod_NN.f90:133-147
@@ -204,13 +212,16 @@ __global__ __launch_bounds__(kRefNnThreads) void k_refnn(RefNnArgs a)
     }
 }
 
+int refnn_input_pad(int L) { return L <= 64 ? 64 : 128; }
+
 hipError_t launch_refnn(const RefNnArgs &a, hipStream_t s)
 {
     const long long n = (long long)a.B.NX * a.B.NY;
     const dim3 grid((unsigned)((n + kRefNnThreads - 1) / kRefNnThreads)), block(kRefNnThreads);
-    const size_t lds = (size_t)(a.L > a.len ? a.L : a.len) * kRefNnThreads * sizeof(float);
-    if (a.mord == 3) hipLaunchKernelGGL(k_refnn<13>, grid, block, lds, s, a);
-    else hipLaunchKernelGGL(k_refnn<25>, grid, block, lds, s, a);
+    const int kp = refnn_input_pad(a.L);
+    if (a.mord == 3 && kp == 64) hipLaunchKernelGGL((k_refnn<13, 64>), grid, block, 0, s, a);
+    else if (a.mord == 3) hipLaunchKernelGGL((k_refnn<13, 128>), grid, block, 0, s, a);
+    else hipLaunchKernelGGL((k_refnn<25, 128>), grid, block, 0, s, a);
     return hipGetLastError();
 }
 

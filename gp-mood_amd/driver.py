@@ -10,13 +10,13 @@ import os
 
 import numpy as np
 
-from . import h5io
+from . import capi, h5io
 
 RK_FACTOR = {"fe": 1, "ssp_rk2": 2, "ssp_rk3": 3, "ssp_rk4": 5}  # main.f90:102-110
 
 
 def run_problem(solver, problem, tmax, nmax, outdir=".", IO_freqStep=-100, IO_freqTime=-1.0, restart=None,
-                nn_filename="", write_diagnostic=True, log=None):
+                nn_filename="", write_diagnostic=True, log=None, nn_size=None):
     """Runs `solver` from its current state as main.f90 does: while t < tmax
     and niter <= nmax.  Writes output_<...>_<niter+100000>.h5 snapshots when
     niter is a multiple of IO_freqStep (> 0) or when t crosses a multiple of
@@ -26,6 +26,9 @@ def run_problem(solver, problem, tmax, nmax, outdir=".", IO_freqStep=-100, IO_fr
     reference keeps the ghost cells of its initial condition across a restart
     read (main.f90:124-136), which for formula boundaries the first Setdt
     sees; here they come from Boundary_C of the restart state at its time.
+    With method NN_GP_MOOD or NN_GP_MOOD_CC and nn_size = (L, length), the
+    network is read from "<nn_filename>.txt" first (load_NN, main.f90:116);
+    a network set on the solver beforehand is used otherwise.
     Returns (t, niter, list of snapshot paths)."""
     cfg = solver.config
     nf = cfg.lf if cfg.nf is None else cfg.nf
@@ -36,6 +39,9 @@ def run_problem(solver, problem, tmax, nmax, outdir=".", IO_freqStep=-100, IO_fr
         solver.set_time(t)
         solver.set_state(np.ascontiguousarray(U[4:-4, 4:-4]))
     solver.set_time(t)
+    solver.set_niter(niter)  # the reference network predicts from niter 11 on
+    if cfg.method in (capi.NN_GP_MOOD, capi.NN_GP_MOOD_CC) and nn_filename and nn_size:
+        solver.load_reference_nn(nn_filename, *nn_size)  # load_NN, main.f90:116
     iter_0 = niter + 1
     diag = h5io.Diagnostics(m, RK_FACTOR[cfg.integrator], iter_0=iter_0)
     counter = 1
@@ -45,7 +51,9 @@ def run_problem(solver, problem, tmax, nmax, outdir=".", IO_freqStep=-100, IO_fr
     def snapshot():
         U, g = solver.get_state(order=True)
         path = os.path.join(outdir, h5io.file_name(m, niter))
-        h5io.write_output(path, m, U, g, niter, t, mord=cfg.Mord)
+        # CellGPO_priori: the network's a priori orders (Mord without a network)
+        pri = solver.cellgpo_priori() if cfg.method in (capi.NN_GP_MOOD, capi.NN_GP_MOOD_CC) else None
+        h5io.write_output(path, m, U, g, niter, t, ordr_priori=pri, mord=cfg.Mord)
         written.append(path)
         if log:
             log(f"snapshot {path}")
@@ -56,7 +64,8 @@ def run_problem(solver, problem, tmax, nmax, outdir=".", IO_freqStep=-100, IO_fr
         st = solver.stats()
         dt = st["dt"]
         t = st["t"]
-        diag.record(t, st["count_detected_cell"])
+        diag.record(t, st["count_detected_cell"], st["count_detected_cell_a_priori"],
+                    nn_nan_passes=st["nn_nan_passes"])
         if IO_freqStep > 0 and niter % IO_freqStep == 0:
             snapshot()
         if IO_freqTime > 0.0:

@@ -117,11 +117,17 @@ class Diagnostics:
     def __init__(self, m, rk_factor, iter_0=1):
         self.m, self.rk, self.iter_0 = m, int(rk_factor), int(iter_0)
         self.time, self.pct, self.pct_priori, self.pct_post = [], [], [], []
+        self.nn_nan = []                 # steps_NN_produced_NAN of each step (0/1)
+        self.count_steps_nn_nan = 0      # count_steps_NN_produced_NAN
         self.niter = iter_0 - 1
 
-    def record(self, t, count, count_priori=0, count_posteriori=None):
+    def record(self, t, count, count_priori=0, count_posteriori=None, nn_nan_passes=0):
+        """One step's counters; nn_nan_passes: the passes whose NN_DETECTION
+        lowered a cell (mod_FE.f90:154-157)."""
         if count_posteriori is None:
             count_posteriori = count - count_priori
+        self.nn_nan.append(1 if nn_nan_passes > 0 else 0)
+        self.count_steps_nn_nan += int(nn_nan_passes)
         den = float(self.rk * self.m.nf * self.m.lf)
         self.niter += 1
         self.time.append(np.float32(t))
@@ -129,15 +135,17 @@ class Diagnostics:
         self.pct_priori.append(np.float32(float(count_priori) * 100.0 / den))
         self.pct_post.append(np.float32(float(count_posteriori) * 100.0 / den))
 
-    def write(self, path, count_steps_nn_nan=0):
+    def write(self, path, count_steps_nn_nan=None):
         """Writes the series as write_diagnostic does: the reference stores
         the first size - iter_0 of its recorded entries (output.f90:151-158),
         which is every step but the last."""
         n = max(0, len(self.time) - 1)
         arr = [np.ascontiguousarray(np.array(v[:n], dtype=np.float32)) for v in
                (self.time, self.pct, self.pct_priori, self.pct_post)]
+        nan = np.ascontiguousarray(np.array(self.nn_nan[:n], dtype=np.int32))
+        total = self.count_steps_nn_nan if count_steps_nn_nan is None else int(count_steps_nn_nan)
         _check(load().gpm_io_write_diagnostic(os.fsencode(path), ctypes.byref(self.m), n, *(_ptr(a) for a in arr),
-                                              None, None, int(count_steps_nn_nan), self.iter_0, self.niter),
+                                              _ptr(nan), None, int(total), self.iter_0, self.niter),
                "gpm_io_write_diagnostic")
 
 

@@ -262,7 +262,7 @@ int gpm_get_apriori_orders(gpm_ctx *ctx, int8_t *pred_dev);
  * (NN_DETECTION, mod_subroutine_NN_MOOD.f90:13-101); NN_GP_MOOD_CC also blends
  * the order-3 reconstruction (mod_NN_GP_MOOD_CC.f90:11-85).  Steps 1-10 are
  * plain GP-MOOD.  Mord 3 or 5; L >= 4*szs_sphere(Mord+2) (52 for Mord 3), L <=
- * 256, length <= 64.
+ * 128, length <= 64.
  *
  * gpm_set_reference_nn replaces the arrays load_NN fills (global_variables.f90:
  * 74-80) and takes them in their Fortran layouts: weight0(length, L),

@@ -77,3 +77,55 @@ def test_driver_outputs_match_reference_run(gpm, tmp_path):
     assert np.array_equal(f32_series(diag, "time"), z["t"][:n].astype(np.float32))
     pct = np.array([np.float32(float(c) * 100.0 / (3 * lf * nf)) for c in z["counts"][:n]])
     assert np.array_equal(f32_series(diag, "pct_detected_cell"), pct)
+
+
+def int_series(path, name):
+    txt = subprocess.run([H5DUMP, "-d", name, "-y", "-w", "0", path], capture_output=True, text=True,
+                         check=True).stdout
+    body = txt.split("DATA {", 1)[1].rsplit("}", 2)[0]
+    return np.array([int(x) for x in re.findall(r"-?[0-9]+", body)], dtype=np.int64)
+
+
+def test_driver_with_reference_network(gpm, tmp_path, monkeypatch):
+    """The driver loop with the reference's network read from its text file
+    (load_NN, main.f90:116): the diagnostic's a priori and a posteriori
+    series, steps_NN_produced_NAN and count_steps_NN_produced_NAN, and the last
+    snapshot's ordr and ordr_priori, against the reference's own run."""
+    if not has_gpu():
+        pytest.skip("no GPU")
+    try:
+        gpm.h5io.load()
+    except RuntimeError as e:
+        pytest.skip(str(e))
+    if H5DUMP is None:
+        pytest.skip("h5dump not available")
+    from test_refnn import write_text
+    z = golden("refnn_nn_r1_n64_neu_fq44.npz")
+    lf = nf = int(z["n"])
+    nsteps = int(z["nsteps"])
+    print(f"[case driver: reference network, {lf}^2, {nsteps} steps]", flush=True)
+    write_text(str(tmp_path / "refnn.txt"), z)
+    cfg = gpm.Config(lf=lf, Mord=int(z["mord"]), ngp=int(z["ngp"]), BC_type=str(z["bc"]),
+                     method=gpm.capi.NN_GP_MOOD)
+    s = gpm.Solver(cfg, tables_from_npz(z))
+    s.set_state(z["U0"])
+    monkeypatch.chdir(tmp_path)  # NN_filename is a name in the working directory, as in the reference
+    stem = "refnn"
+    t, niter, written = gpm.driver.run_problem(s, "2DRP3", tmax=1e10, nmax=nsteps - 1, outdir=str(tmp_path),
+                                               IO_freqStep=nsteps, nn_filename=stem, nn_size=(int(z["L"]),
+                                                                                              int(z["length"])))
+    assert niter == nsteps and t == z["t"][-1] and len(written) == 1
+    m = gpm.h5io.meta(lf, nf, "2DRP3", gpm.capi.NN_GP_MOOD, nn_filename=stem)
+    diag = os.path.join(str(tmp_path), "diagnostic_" + gpm.h5io.file_name(m) + ".h5")
+    n = nsteps - 1
+    den = 3 * lf * nf
+    pri = np.array([np.float32(float(c) * 100.0 / den) for c in z["apriori"][:n]])
+    post = np.array([np.float32(float(c) * 100.0 / den) for c in z["aposteriori"][:n]])
+    assert np.array_equal(f32_series(diag, "pct_detected_cell_a_priori"), pri)
+    assert np.array_equal(f32_series(diag, "pct_detected_cell_a_posteriori"), post)
+    nan_step = (np.diff(np.concatenate([[0], z["nan_cum"]])) > 0).astype(np.int64)
+    assert np.array_equal(int_series(diag, "steps_NN_produced_NAN"), nan_step[:n])
+    assert int_series(diag, "count_steps_NN_produced_NAN")[0] == int(z["nan_cum"][-1])
+    snap = written[0]
+    assert np.array_equal(int_series(snap, "ordr").reshape(nf, lf), z["masks"][-1])
+    assert np.array_equal(int_series(snap, "ordr_priori").reshape(nf, lf), z["priori"][-1])
