@@ -226,6 +226,22 @@ int upload_tables(gpm_ctx *c, const double *zT, const double *d2x, const double 
             h.d2y[k] = d2y[2 + mord * k];
         }
     for (int j = 0; j < ngp; ++j) h.gw[j] = gw[(ngp - 1) + 5 * j];
+    if (c->cfg.method == GPM_POL_MOOD) {
+        // POL_MOOD reconstructs order 3 with Pol_zT_o3 (pol_presim, GP_init.f90:359-411)
+        // over the same five stencil cells: o3_coef (constants.f90:59-61) placed
+        // per face and Gauss point as (M, L, T, R, B)
+        const double cf[5] = {15. / 18, 1. / 3, 1. / (4 * std::sqrt(3.)), -1. / (4 * std::sqrt(3.)), -1. / 6};
+        enum { M = 0, CN = 1, CT = 2, FT = 3, FN = 4 };  // kM, kCloseNrm, kCloseTrans, kFarTrans, kFarNrm
+        static const int tab[4][2][5] = {
+            {{M, CN, FT, FN, CT}, {M, CN, CT, FN, FT}},  // iL
+            {{M, CT, CN, FT, FN}, {M, FT, CN, CT, FN}},  // iT
+            {{M, FN, FT, CN, CT}, {M, FN, CT, CN, FT}},  // iR
+            {{M, CT, FN, FT, CN}, {M, FT, FN, CT, CN}},  // iB
+        };
+        for (int f = 0; f < 4; ++f)
+            for (int g = 0; g < 2; ++g)
+                for (int k = 0; k < 5; ++k) h.zT[1][f][k][g] = cf[tab[f][g][k]];
+    }
     HIPCHK(c, hipMemcpyAsync(c->dT, &h, sizeof(h), hipMemcpyHostToDevice, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
     return GPM_OK;
@@ -352,6 +368,7 @@ StageArgs stage_args(gpm_ctx *c, int k, const StagePlan &P)
     s.mark = c->mark; s.stamp = c->stamp; s.T = c->dT; s.tstate = c->tstate;
     s.dmp = c->cfg.dmp; s.u2 = c->cfg.u2; s.u2tol = c->cfg.u2_tol;
     s.cascade = c->cfg.cascade;
+    s.recon = c->cfg.method == GPM_POL_MOOD ? kReconPol : kReconGP;
     s.dx = c->dx; s.dy = c->dy; s.tiles_x = c->tiles_x; s.ntiles = c->ntiles;
     return s;
 }
@@ -421,7 +438,10 @@ int enqueue_stages(gpm_ctx *c)
         s.pred = c->pred;
         if (refnn) {
             s.dmp = 0;
-            if (c->refnn_cc) s.vnn = c->vnn;  // NN_GP_MOOD_CC_ replaces GP_MOOD_ (mod_Reconstruction.f90:26-32)
+            if (c->refnn_cc) {  // NN_GP_MOOD_CC_ replaces GP_MOOD_ (mod_Reconstruction.f90:26-32)
+                s.vnn = c->vnn;
+                s.recon = kReconCC;
+            }
         }
         hipEvent_t e0 = nullptr, e1 = nullptr;
         if (c->timing) {
@@ -447,10 +467,11 @@ int enqueue_stages(gpm_ctx *c)
         m.dmp = c->cfg.dmp; m.u2 = c->cfg.u2; m.u2tol = c->cfg.u2_tol;
         m.cascade = c->cfg.cascade; m.pmax = kMaxPasses; m.stage_idx = k;
         m.dx = c->dx; m.dy = c->dy;
+        m.recon = s.recon;
         if (refnn) {
             m.dmp = 0;
             m.nn_passes = 1;
-            if (c->refnn_cc) m.vnn = c->vnn;
+            if (c->refnn_cc) m.vnn = c->vnn;  // m.recon is kReconCC from s
         }
         void *args[] = {&m};
         if (c->timing) {
@@ -537,8 +558,12 @@ int gpm_create(const gpm_config *cfg, gpm_ctx **out)
         cfg->num_flux != GPM_UNSPLIT)
         return bad(GPM_EUNSUP, "numFlux must be HLLC, LLF, HLL or unsplit");
     const bool nn_method = cfg->method == GPM_NN_GP_MOOD || cfg->method == GPM_NN_GP_MOOD_CC;
-    if (cfg->method != GPM_GP_MOOD && !nn_method && cfg->method != GPM_FOG)
-        return bad(GPM_EUNSUP, "method must be GP_MOOD, NN_GP_MOOD, NN_GP_MOOD_CC or FOG");
+    if (cfg->method != GPM_GP_MOOD && !nn_method && cfg->method != GPM_FOG && cfg->method != GPM_POL_MOOD)
+        return bad(GPM_EUNSUP, "method must be GP_MOOD, NN_GP_MOOD, NN_GP_MOOD_CC, POL_MOOD or FOG");
+    // Pol_zT_o3 holds 2 Gauss points and POL_MOOD_ has no order above 3 (any
+    // other order takes the cell value, mod_pol_MOOD.f90:62-76)
+    if (cfg->method == GPM_POL_MOOD && (cfg->mord != 3 || cfg->ngp != 2))
+        return bad(GPM_EUNSUP, "POL_MOOD needs Mord = 3 and ngp = 2");
     if (c->fog && (cfg->ngp != 1 || cfg->integrator != GPM_FE))
         return bad(GPM_EUNSUP, "FOG needs ngp = 1 and integrator FE (the problems/*_FOG.f90 set-up)");
     if (!bc_ok) return bad(GPM_EUNSUP, "unknown BC_type");

@@ -34,6 +34,13 @@ __device__ constexpr int kSY[25] = {0, 0, 1, 0, -1, -1, 1, 1, -1, -2, 0, 2, 0,
 
 __host__ __device__ constexpr int stencil_sz(int ord) { return ord == 1 ? 1 : ord == 3 ? 5 : ord == 5 ? 13 : 25; }
 
+// Reconstruction of a method: GP_MOOD_ (and NN_GP_MOOD, whose reconstruction
+// is GP_MOOD_'s), POL_MOOD_ (mod_pol_MOOD.f90: order 3 from the polynomial
+// weights, which the host puts in zT's order-3 block, and the cell values at
+// any other order), or NN_GP_MOOD_CC_ (mod_NN_GP_MOOD_CC.f90: the cell values
+// at order 1, the blend of values_NN at order 3).
+enum { kReconGP = 0, kReconPol = 1, kReconCC = 2 };
+
 // GP tables on the device, laid out for scalar loads in the unrolled loops:
 // zT[(ord-1)/2][face][s][gp].
 struct DevTables {
@@ -552,12 +559,13 @@ __device__ __forceinline__ void face_flux_dyn(const DevTables *__restrict__ T, c
 // Riemann solve, so lanes of a wavefront that hold different orders run a
 // single code path instead of one inlined solver per order.
 //
-// With cc (NN_GP_MOOD_CC, Mord 3) the face states are those of face_flux_cc:
-// the cell values at order 1, the blend v1 * U(cell) + v2 * GP value at order 3.
+// With recon = kReconPol or kReconCC (Mord 3) the face states at order 1 are
+// the cell values themselves; with kReconCC those at order 3 are the blend
+// v1 * U(cell) + v2 * GP value (face_flux_cc).
 template <int MORD, int FLUX, class Acc>
 __device__ __forceinline__ void face_gp_flux_dyn(const DevTables *__restrict__ T, const Acc &acc, int ord, int fa,
                                                  int fb, int dir, int j, int ai, int aj, int bi, int bj, double F[4],
-                                                 bool cc = false, double va1 = 0.0, double va2 = 1.0,
+                                                 int recon = kReconGP, double va1 = 0.0, double va2 = 1.0,
                                                  double vb1 = 0.0, double vb2 = 1.0)
 {
     constexpr int SZM = stencil_sz(MORD);
@@ -575,13 +583,18 @@ __device__ __forceinline__ void face_gp_flux_dyn(const DevTables *__restrict__ T
         ua[0] = ua[0] + za * qa.x; ua[1] = ua[1] + za * qa.y; ua[2] = ua[2] + za * qa.z; ua[3] = ua[3] + za * qa.w;
         ub[0] = ub[0] + zb * qb.x; ub[1] = ub[1] + zb * qb.y; ub[2] = ub[2] + zb * qb.z; ub[3] = ub[3] + zb * qb.w;
     }
-    if (MORD == 3 && cc) {
+    if (MORD == 3 && recon != kReconGP) {
         const double4 ca = acc(ai, aj), cb = acc(bi, bj);
         const double qa[4] = {ca.x, ca.y, ca.z, ca.w}, qb[4] = {cb.x, cb.y, cb.z, cb.w};
+        if (ord == 1) {
 #pragma unroll
-        for (int k = 0; k < 4; ++k) {
-            ua[k] = ord == 1 ? qa[k] : va1 * qa[k] + va2 * ua[k];
-            ub[k] = ord == 1 ? qb[k] : vb1 * qb[k] + vb2 * ub[k];
+            for (int k = 0; k < 4; ++k) { ua[k] = qa[k]; ub[k] = qb[k]; }
+        } else if (recon == kReconCC) {
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                ua[k] = va1 * qa[k] + va2 * ua[k];
+                ub[k] = vb1 * qb[k] + vb2 * ub[k];
+            }
         }
     }
     riemann_dir<FLUX>(dir, ua, ub, F);
@@ -646,6 +659,7 @@ __device__ __forceinline__ void face_flux_cc(const DevTables *__restrict__ T, co
 struct MoodParams {
     int dmp, u2, u2tol;
     double dx, dy;
+    int recon;  // kRecon*: the U2 second derivatives of POL_MOOD are its own
 };
 
 // shock_detector, mod_subroutine_MOOD.f90:191-238: compression (divV <
@@ -694,22 +708,38 @@ __device__ __forceinline__ bool shock_detector(const MoodParams &P, const Acc &a
 // and U2 smooth-extrema relaxation.  `cand` is the cell's candidate Uout.
 // Kept out of line (it runs only for DMP failures); its arguments are passed by
 // value so that the call does not make every thread build a stack frame.
+//
+// pol: POL_MOOD's second derivatives, dot_product(D2O2, rho(l+i-1:l+i+1, n+j))
+// / dx**2 and the y form with D2O2 = (1, -2, 1) (mod_subroutine_MOOD.f90:86-87,
+// constants.f90:43), instead of the GP_d2x/y_sp weights.
 template <class Acc>
 __device__ __noinline__ bool u2_relax(int u2tol, double dx, double dy, const DevTables *__restrict__ T,
-                                      const Acc acc, int i, int j)
+                                      const Acc acc, int i, int j, bool pol)
 {
     double D2x[5], D2y[5];  // (-1,0), (0,0), (1,0), (0,-1), (0,1)
     const int oi[5] = {-1, 0, 1, 0, 0}, oj[5] = {0, 0, 0, -1, 1};
 #pragma unroll
     for (int t = 0; t < 5; ++t) {
-        double q[5];
-#pragma unroll
-        for (int k = 0; k < 5; ++k) q[k] = acc(i + kSX[k] + oi[t], j + kSY[k] + oj[t]).x;
+        const int ci = i + oi[t], cj = j + oj[t];
         double ax = 0.0, ay = 0.0;
+        if (pol) {
+            ax = ax + 1.0 * acc(ci - 1, cj).x;
+            ax = ax + -2.0 * acc(ci, cj).x;
+            ax = ax + 1.0 * acc(ci + 1, cj).x;
+            ay = ay + 1.0 * acc(ci, cj - 1).x;
+            ay = ay + -2.0 * acc(ci, cj).x;
+            ay = ay + 1.0 * acc(ci, cj + 1).x;
+            ax = ax / (dx * dx);
+            ay = ay / (dy * dy);
+        } else {
+            double q[5];
 #pragma unroll
-        for (int k = 0; k < 5; ++k) ax = ax + T->d2x[k] * q[k];
+            for (int k = 0; k < 5; ++k) q[k] = acc(ci + kSX[k], cj + kSY[k]).x;
 #pragma unroll
-        for (int k = 0; k < 5; ++k) ay = ay + T->d2y[k] * q[k];
+            for (int k = 0; k < 5; ++k) ax = ax + T->d2x[k] * q[k];
+#pragma unroll
+            for (int k = 0; k < 5; ++k) ay = ay + T->d2y[k] * q[k];
+        }
         D2x[t] = ax;
         D2y[t] = ay;
     }
@@ -742,7 +772,7 @@ __device__ __forceinline__ bool decide(const MoodParams &P, const DevTables *__r
     const double Mm = mx - mn;
     if (!(Mm > P.dx * P.dx * P.dx)) return false;
     if (!(cand[0] > mx || cand[0] < mn)) return false;
-    if (P.u2 && u2_relax(P.u2tol, P.dx, P.dy, T, acc, i, j)) return false;
+    if (P.u2 && u2_relax(P.u2tol, P.dx, P.dy, T, acc, i, j, P.recon == kReconPol)) return false;
     return true;
 }
 

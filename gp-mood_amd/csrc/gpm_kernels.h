@@ -102,6 +102,7 @@ struct StageArgs {
     int frozen;                       // orders fixed to pred, no detection (frozen-mask step)
     const int8_t *__restrict__ pred;  // a priori orders (read only here; gpo is rewritten)
     const float *__restrict__ vnn;    // NN_GP_MOOD_CC: values_NN for the blended reconstruction (Mord 3), or null
+    int recon;                        // kRecon*
     double dx, dy;
     int tiles_x, ntiles;
 };
@@ -126,6 +127,7 @@ struct MoodArgs {
     int nn_passes;  // count the passes that lowered a cell (a step of the reference network)
     int solo_max;   // lists of at most this many cells run on workgroup 0 alone (0: never)
     const float *__restrict__ vnn;  // as StageArgs::vnn
+    int recon;                      // kRecon*
     double dx, dy;
 };
 

@@ -278,7 +278,7 @@ __global__ __launch_bounds__(kStageThreads, GPM_STAGE_WAVES_PER_EU) void k_stage
                 fog_flux<0, FLUX>(T, acc(fx - 1 + H, fy + H), acc(fx + H, fy + H), L);
             } else if (!a.apriori) {
                 face_flux<MORD, NGP, 0, FLUX>(T, acc, fx - 1 + H, fy + H, fx + H, fy + H, L);
-            } else if (MORD == 3 && a.vnn) {
+            } else if (MORD == 3 && a.recon == kReconCC) {
                 const int o = min(sO[(fy + 1) * OW + fx], sO[(fy + 1) * OW + fx + 1]);
                 double va1, va2, vb1, vb2;
                 vnn_at(a.vnn, NX, NY, x0 + fx - 1, y0 + fy, va1, va2);
@@ -296,7 +296,7 @@ __global__ __launch_bounds__(kStageThreads, GPM_STAGE_WAVES_PER_EU) void k_stage
                 fog_flux<1, FLUX>(T, acc(gx + H, gy - 1 + H), acc(gx + H, gy + H), L);
             } else if (!a.apriori) {
                 face_flux<MORD, NGP, 1, FLUX>(T, acc, gx + H, gy - 1 + H, gx + H, gy + H, L);
-            } else if (MORD == 3 && a.vnn) {
+            } else if (MORD == 3 && a.recon == kReconCC) {
                 const int o = min(sO[gy * OW + gx + 1], sO[(gy + 1) * OW + gx + 1]);
                 double va1, va2, vb1, vb2;
                 vnn_at(a.vnn, NX, NY, x0 + gx, y0 + gy - 1, va1, va2);
@@ -313,7 +313,7 @@ __global__ __launch_bounds__(kStageThreads, GPM_STAGE_WAVES_PER_EU) void k_stage
 
     const double dtdx = a.tstate[2], dtdy = a.tstate[3];
     const double dtg = GEN ? a.tstate[1] * kGravity : 0.0;
-    const MoodParams P{a.dmp, a.u2, a.u2tol, a.dx, a.dy};
+    const MoodParams P{a.dmp, a.u2, a.u2tol, a.dx, a.dy, a.recon};
     const int stamp1 = *a.stamp + 1;
     for (int c = tid; c < kTX * kTY; c += kStageThreads) {
         const int cx = c % kTX, cy = c / kTX;
@@ -411,7 +411,7 @@ __global__ __launch_bounds__(kMoodThreads) void k_mood(MoodArgs a)
     const int f = gl / NGPP, jg = gl % NGPP;
     const int stamp_base = *a.stamp;
     const DevTables *__restrict__ T = a.T;
-    const MoodParams P{a.dmp, a.u2, a.u2tol, a.dx, a.dy};
+    const MoodParams P{a.dmp, a.u2, a.u2tol, a.dx, a.dy, a.recon};
     const double dtdx = a.tstate[2], dtdy = a.tstate[3];
     const double dtg = GEN ? a.tstate[1] * kGravity : 0.0;
     const double xmin_top = GEN ? stage_xmin_top(a.B, a.tstate, a.C.trk) : 0.0;
@@ -475,13 +475,13 @@ __global__ __launch_bounds__(kMoodThreads) void k_mood(MoodArgs a)
             double F[4] = {0.0, 0.0, 0.0, 0.0};
             if (valid && jg < NGP) {
                 double va1 = 0.0, va2 = 1.0, vb1 = 0.0, vb2 = 1.0;
-                const bool cc = MORD == 3 && a.vnn != nullptr;
-                if (cc) {
+                const int recon = MORD == 3 ? a.recon : kReconGP;
+                if (recon == kReconCC) {
                     vnn_at(a.vnn, NX, NY, i + oai, j + oaj, va1, va2);
                     vnn_at(a.vnn, NX, NY, i + obi, j + obj, vb1, vb2);
                 }
                 face_gp_flux_dyn<MORD, FLUX>(T, lacc, ord, fa, fb, fdir, jg, HB + oai, HB + oaj, HB + obi, HB + obj, F,
-                                             cc, va1, va2, vb1, vb2);
+                                             recon, va1, va2, vb1, vb2);
             }
             // quadrature over the face's Gauss points, j = 1..ngp from 0
             double L[4] = {0.0, 0.0, 0.0, 0.0};

@@ -8,7 +8,7 @@
  * It restates, operation for operation, the FP64 arithmetic of the reference
  * Fortran (slug-cfd/gp-mood, gp-mood-code/) for
  *   method in {GP_MOOD, FOG (ngp = 1), NN_GP_MOOD and NN_GP_MOOD_CC with the
- *             reference's own network (mod_NN.f90, text weights)},
+ *             reference's own network (mod_NN.f90, text weights), POL_MOOD},
  *   integrator in {FE, SSP_RK2, SSP_RK3, SSP_RK4},
  *   numFlux in {HLLC, LLF, HLL, unsplit},
  *   BC_type in {Neumann, Periodic, Dirichlet, Reflective, KH_BC, Mach800_BC,
@@ -45,7 +45,7 @@ enum {
     MACH800_BC = 201, DOUBLEMACH800_BC = 301, RMI_BC = 401, DMR_BC = 35, RT_BC = 402
 };
 enum { PROBLEM_RT = 51 };
-enum { METHOD_FOG = 1, METHOD_GP_MOOD = 2, METHOD_NN_GP_MOOD = 203, METHOD_NN_GP_MOOD_CC = 204 };
+enum { METHOD_FOG = 1, METHOD_GP_MOOD = 2, METHOD_POL_MOOD = 200, METHOD_NN_GP_MOOD = 203, METHOD_NN_GP_MOOD_CC = 204 };
 /* global_variables.f90:97 */
 enum { NSTEPS_WITH_NO_NN = 10 };
 
@@ -77,6 +77,7 @@ typedef struct {
     long count_detected;       /* count_detected_cell (global_variables.f90:63) */
     int8_t *init_orders;       /* a priori CellGPO of the interior [nf][lf] (NULL: Mord) */
     int passes[5];             /* MOOD passes per stage of the last step */
+    double polzt[5 * 4 * 2];   /* Pol_zT_o3(5, iL:iB, 2) (global_variables.f90:46), pol_presim */
     /* the reference's network (mod_NN.f90; global_variables.f90:74-80, Fortran
        layouts: weight0(length, L), weight1(length, length), weight2(2, length)) */
     int niter;                 /* main.f90:129,147: steps started so far */
@@ -119,6 +120,24 @@ static inline int stcl_sz(int ord) /* GP_init.f90:14-23 */
 
 enum { iL = 0, iT = 1, iR = 2, iB = 3 };
 
+/* pol_presim, GP_init.f90:359-411: the third-order polynomial face weights
+   Pol_zT_o3(s, face, gp) over the 5-point stencil (M, L, T, R, B), from
+   o3_coef (constants.f90:59-61), stored at s + 5*(face + 4*gp). */
+static void pol_presim(double *pz)
+{
+    const double c[5] = {15. / 18, 1. / 3, 1. / (4 * sqrt(3.)), -1. / (4 * sqrt(3.)), -1. / 6};
+    enum { M = 0, CN = 1, CT = 2, FT = 3, FN = 4 };  /* kM, kCloseNrm, kCloseTrans, kFarTrans, kFarNrm */
+    static const int tab[4][2][5] = {
+        /* iL */ {{M, CN, FT, FN, CT}, {M, CN, CT, FN, FT}},
+        /* iT */ {{M, CT, CN, FT, FN}, {M, FT, CN, CT, FN}},
+        /* iR */ {{M, FN, FT, CN, CT}, {M, FN, CT, CN, FT}},
+        /* iB */ {{M, CT, FN, FT, CN}, {M, FT, FN, CT, CN}},
+    };
+    for (int f = 0; f < 4; ++f)
+        for (int g = 0; g < 2; ++g)
+            for (int k = 0; k < 5; ++k) pz[k + 5 * (f + 4 * g)] = c[tab[f][g][k]];
+}
+
 orc_t *orc_create(int lf, int nf, int mord, int ngp, int bc, int dmp, int u2, int u2tol, int cascade,
                   double cfl, double dx, double dy, const double *zT, const double *d2x,
                   const double *d2y, const double *gw, const int *ixiy)
@@ -141,6 +160,7 @@ orc_t *orc_create(int lf, int nf, int mord, int ngp, int bc, int dmp, int u2, in
     memcpy(o->ixiy, ixiy, sizeof(int) * 7 * 25 * 2);
     o->W = lf + 2 * NGC;
     o->H = nf + 2 * NGC;
+    pol_presim(o->polzt);
     size_t ncell = (size_t)o->W * o->H;
     o->Uh = (double *)calloc((size_t)(lf + 2) * (nf + 2) * 16 * ngp, sizeof(double));
     o->Fx = (double *)calloc((size_t)(lf + 3) * (nf + 2) * 4, sizeof(double));
@@ -525,6 +545,46 @@ static void gp_mood_recons(orc_t *o, const double *Uin)
     }
 }
 
+/* POL_MOOD_, mod_pol_MOOD.f90:11-89: order 3 from Pol_zT_o3 over the first
+   five stencil cells, any other order the cell value itself. */
+static void pol_mood_recons(orc_t *o, const double *Uin)
+{
+    const int lf = o->lf, nf = o->nf, ngp = o->ngp;
+#pragma omp parallel for schedule(static)
+    for (int n = 0; n <= nf + 1; ++n) {
+        double q[4][5];
+        for (int l = 0; l <= lf + 1; ++l) {
+            int done = 0;
+            for (int i = iL; i <= iB; ++i) {
+                int rec = 0, ord = 0;
+                int me = o->gpo[CELL(o, l, n)];
+                if (i == iL && DFX(o, l - 1, n)) { ord = (int)fmin2(o->gpo[CELL(o, l - 1, n)], me); rec = 1; }
+                else if (i == iR && DFX(o, l, n)) { ord = (int)fmin2(o->gpo[CELL(o, l + 1, n)], me); rec = 1; }
+                else if (i == iB && DFY(o, l, n - 1)) { ord = (int)fmin2(o->gpo[CELL(o, l, n - 1)], me); rec = 1; }
+                else if (i == iT && DFY(o, l, n)) { ord = (int)fmin2(o->gpo[CELL(o, l, n + 1)], me); rec = 1; }
+                if (!rec) continue;
+                if (!done) {
+                    for (int s = 0; s < 5; ++s) {
+                        const double *u = Uin + CELL(o, l + IXIY(o, 3, s, 0), n + IXIY(o, 3, s, 1)) * 4;
+                        for (int k = 0; k < 4; ++k) q[k][s] = u[k];
+                    }
+                    done = 1;
+                }
+                const double *uc = Uin + CELL(o, l, n) * 4;
+                for (int j = 0; j < ngp; ++j)
+                    for (int k = 0; k < 4; ++k) {
+                        double uh = uc[k];
+                        if (ord == 3) {
+                            uh = 0.0;
+                            for (int s = 0; s < 5; ++s) uh = uh + o->polzt[s + 5 * (i + 4 * j)] * q[k][s];
+                        }
+                        o->Uh[UHI(o, k, i, j, l, n)] = uh;
+                    }
+            }
+        }
+    }
+}
+
 /* shock_detector, mod_subroutine_MOOD.f90:191-238 */
 static int shock_detector(const orc_t *o, const double *Uin, int l, int n)
 {
@@ -572,6 +632,20 @@ static int decide(const orc_t *o, const double *Uin, const double *Uout, int l, 
                 for (int j = -1; j <= 1; ++j)
                     for (int i = -1; i <= 1; ++i) {
                         if (!(i == 0 || j == 0)) continue;
+                        if (o->method == METHOD_POL_MOOD) {
+                            /* dot_product(D2O2, U(l+i-1:l+i+1, n+j)) / dx**2 and the y form,
+                               D2O2 = (1, -2, 1) (constants.f90:43) */
+                            double ax = 0.0, ay = 0.0;
+                            ax = ax + 1. * R(l + i - 1, n + j);
+                            ax = ax + -2. * R(l + i, n + j);
+                            ax = ax + 1. * R(l + i + 1, n + j);
+                            ay = ay + 1. * R(l + i, n + j - 1);
+                            ay = ay + -2. * R(l + i, n + j);
+                            ay = ay + 1. * R(l + i, n + j + 1);
+                            D2x[i + 1][j + 1] = ax / (o->dx * o->dx);
+                            D2y[i + 1][j + 1] = ay / (o->dy * o->dy);
+                            continue;
+                        }
                         double q[5];
                         for (int k = 0; k < 5; ++k) q[k] = R(l + IXIY(o, 3, k, 0) + i, n + IXIY(o, 3, k, 1) + j);
                         double ax = 0.0, ay = 0.0;
@@ -896,6 +970,8 @@ static void orc_fe(orc_t *o, double *Uin, double *Uout, double dt, int first, in
         ++passes;
         if (o->method == METHOD_NN_GP_MOOD_CC && o->niter > NSTEPS_WITH_NO_NN) {
             if (nn_cc_recons(o, Uin)) o->err = 1;
+        } else if (o->method == METHOD_POL_MOOD) {
+            pol_mood_recons(o, Uin);
         } else {
             gp_mood_recons(o, Uin);
         }
@@ -1135,3 +1211,6 @@ void orc_expf_bits(uint32_t first, long n, float *out)
         out[i] = expf(x);
     }
 }
+
+/* Pol_zT_o3 as pol_presim fills it, 40 doubles at s + 5*(face + 4*gp). */
+void orc_pol_table(const orc_t *o, double *out) { memcpy(out, o->polzt, sizeof(o->polzt)); }

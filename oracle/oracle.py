@@ -16,7 +16,7 @@ BC_CODES = {"neumann": 17, "periodic": 18, "dirichlet": 19, "reflective": 20, "k
 FLUX_CODES = {"hllc": 100, "llf": 101, "hll": 102, "unsplit": 103}
 INTEGRATORS = {"fe": 8, "ssp_rk2": 9, "ssp_rk3": 10, "ssp_rk4": 12}
 PROBLEM_RT = 51
-METHODS = {"gp_mood": 2, "fog": 1, "nn_gp_mood": 203, "nn_gp_mood_cc": 204}
+METHODS = {"gp_mood": 2, "fog": 1, "pol_mood": 200, "nn_gp_mood": 203, "nn_gp_mood_cc": 204}
 CASCADES = {"reference": 0, "full": 1}
 
 _lib = None
@@ -56,6 +56,7 @@ def lib():
         L.orc_error.argtypes = [P]
         L.orc_nn_forward.argtypes = [P, P, P]
         L.orc_expf_bits.argtypes = [ctypes.c_uint32, ctypes.c_long, P]
+        L.orc_pol_table.argtypes = [P, P]
         _lib = L
     return _lib
 
@@ -132,6 +133,12 @@ class Oracle:
 
     def set_niter(self, niter):
         lib().orc_set_niter(self._h, int(niter))
+
+    def pol_table(self):
+        """Pol_zT_o3(5, iL:iB, 2) of pol_presim, flattened column-major."""
+        out = np.zeros(40)
+        lib().orc_pol_table(self._h, _ptr(out))
+        return out
 
     def nn_forward(self, x):
         """forward (mod_NN.f90:165-181) of one input vector x[L]."""

@@ -121,6 +121,11 @@ CONFIGS = {
     "nn_r1_64x48_kh_rk2": (64, 3, 2, "kh", True, True, True),
     "nn_r2_n128_ref": (128, 5, 3, "reflective", True, True, True),
     "nncc_r1_n64_neu": (64, 3, 2, "neumann", True, True, True),
+    # polynomial MOOD (method = POL_MOOD, mod_pol_MOOD.f90; Mord 3, ngp 2)
+    "pol_r1_n64_neu": (64, 3, 2, "neumann", True, True, True),
+    "pol_r1_n32_ref_hll": (32, 3, 2, "reflective", True, True, True, "hll"),
+    "pol_r1_dmr_64x16": (64, 3, 2, "dmr", True, True, True),
+    "pol_r1_n64_per_nou2": (64, 3, 2, "periodic", True, False, False, "unsplit"),
 }
 
 # The problem files' own configurations (problems/*.f90: grid, domain, method,
@@ -184,6 +189,10 @@ EXTRAS = {
     "nn_r1_64x48_kh_rk2": dict(nf=48, method="NN_GP_MOOD", integrator="ssp_rk2", nn_l=57, nn_file="refnn"),
     "nn_r2_n128_ref": dict(method="NN_GP_MOOD", nn_l=100, nn_len=48, nn_file="refnn"),
     "nncc_r1_n64_neu": dict(method="NN_GP_MOOD_CC", nn_l=57, nn_file="refnn"),
+    "pol_r1_n64_neu": dict(method="POL_MOOD"),
+    "pol_r1_n32_ref_hll": dict(method="POL_MOOD", integrator="ssp_rk2"),
+    "pol_r1_dmr_64x16": dict(nf=16, lx="4.", ly="1.", method="POL_MOOD"),
+    "pol_r1_n64_per_nou2": dict(method="POL_MOOD"),
 }
 
 
@@ -205,7 +214,8 @@ def config(name):
 # configs for which the Fortran drop-in (reference modules + product
 # time_stepper over libgpmood) is also built
 DROPIN_CONFIGS = ["r2_n64_neu", "r3_n64_ref", "r3_n64_kh_uns", "r1_dmr_64x16", "r1_rt_16x48", "r1_m800_32_fe",
-                  "r2_n32_ref_rk4", "fog_n64_kh_uns", "fog_m800_32_hll", "nn_r1_n64_neu", "nncc_r1_n64_neu"]
+                  "r2_n32_ref_rk4", "fog_n64_kh_uns", "fog_m800_32_hll", "nn_r1_n64_neu", "nncc_r1_n64_neu",
+                  "pol_r1_n64_neu"]
 
 PARAMS_TEMPLATE = """module parameters
    use constants

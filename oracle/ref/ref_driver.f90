@@ -8,6 +8,7 @@
 ! main.f90:145-162 (dtfinal, Setdt, time_stepping, Boundary_C, t=t+dt, U=Ur) on a
 ! conserved initial state read from a raw binary file, and dumps:
 !   tables.bin   GP tables from GP_presim_sphere (GP_init.f90:26-357)
+!   pol.bin      Pol_zT_o3 from pol_presim (GP_init.f90:359-411)
 !   steps.bin    per step: dt (f64), t (f64), count_detected_cell (i32)
 !   masks.bin    CellGPO(1:lf,1:nf) as int8 after every RK stage (mode 1) or
 !                after every step (mode 0)
@@ -52,6 +53,7 @@ program gpm_ref_driver
 
    call init_mesh()
    call GP_presim_sphere()
+   call pol_presim()
    nnm = (method == NN_GP_MOOD) .or. (method == NN_GP_MOOD_CC)
    if (nnm) call load_NN()
 
@@ -63,6 +65,9 @@ program gpm_ref_driver
    write(uo) gauss_weight
    write(uo) int(ixiy_sp,4)
    write(uo) dx, dy
+   close(uo)
+   open(newunit=uo, file=trim(outdir)//'/pol.bin', access='stream', form='unformatted', status='replace')
+   write(uo) Pol_zT_o3
    close(uo)
 
    allocate(Uint(4,lf,nf))

@@ -60,6 +60,7 @@ def run_ref(cfg, U0, nsteps, mode, nn=None):
         subprocess.run(f"ulimit -s unlimited; {exe} {d}/ic.bin {nsteps} {d} {mode}", shell=True, check=True,
                        stdout=subprocess.DEVNULL, cwd=d)
         res = {"tables": read_tables(os.path.join(d, "tables.bin")),
+               "pol": np.fromfile(os.path.join(d, "pol.bin"), dtype=np.float64),
                "steps": np.fromfile(os.path.join(d, "steps.bin"), dtype=STEP_TYPE),
                "masks": np.fromfile(os.path.join(d, "masks.bin"), dtype=np.int8).reshape(-1, nf, n),
                "U": np.fromfile(os.path.join(d, "U_final.bin"), dtype=np.float64).reshape(nf + 8, n + 8, 4)}
@@ -169,6 +170,11 @@ STEP_CASES = {
     "fog_n32_ref_hllc_bl42": ("fog_n32_ref_hllc", "blast", 42, 30),
     "fog_dmr_64x16": ("fog_dmr_64x16", "dmr:0", 0, 30),
     "fog_rt_16x48_uns_q43": ("fog_rt_16x48_uns", "quadrants:0", 43, 30),
+    # polynomial MOOD
+    "pol_r1_n64_neu_fq49": ("pol_r1_n64_neu", "four_quadrant", 49, 25),
+    "pol_r1_n32_ref_hll_bl50": ("pol_r1_n32_ref_hll", "blast", 50, 20),
+    "pol_r1_dmr_64x16": ("pol_r1_dmr_64x16", "dmr:0", 0, 20),
+    "pol_r1_n64_per_nou2_fq51": ("pol_r1_n64_per_nou2", "four_quadrant", 51, 25),
 }
 # the reference's own network (method NN_GP_MOOD / NN_GP_MOOD_CC): name:
 # (config, IC kind, seed, steps); the network is refnn_weights(seed)
@@ -230,6 +236,7 @@ def main(argv):
             problem=51 if e["problem"] == "RT" else 0, method=e["method"].lower(),
             kind=kind, seed=seed, nsteps=nsteps, ic_sha=sha(U0), U0=U0,
             dt=r["steps"]["dt"], t=r["steps"]["t"], counts=r["steps"]["cnt"], masks=r["masks"], U=r["U"],
+            **({"pol": r["pol"]} if e["method"] == "POL_MOOD" else {}),
             **extra, **{k: v for k, v in r["tables"].items()})
         print("wrote", name, "lowered", int((r["masks"] < mord).sum()))
     for name, (cfg, kind, seed, nsteps) in NN_CASES.items():
