@@ -20,6 +20,8 @@ LIB_PATH = os.environ.get("GPM_LIB_PATH") or os.path.join(HERE, "libgpmood.so")
 FOG = 1
 GP_MOOD = 2
 POL_MOOD = 200
+UNLIM_POL = 201
+GP_SE = 202
 NN_GP_MOOD = 203
 NN_GP_MOOD_CC = 204
 FE, SSP_RK2, SSP_RK3, SSP_RK4 = 8, 9, 10, 12

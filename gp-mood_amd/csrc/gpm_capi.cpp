@@ -29,6 +29,7 @@ struct gpm_ctx {
     int N = 0;  // NX where only square grids are supported (learned predictor, adjoint)
     int integrator = 0, nstages = 0, gravity = 0;
     bool fog = false;        // method FOG: first-order Godunov, no detection
+    bool nodetect = false;   // FOG, GP_SE, Unlim_POL: one pass, no DETECTION (mod_FE.f90:160-162)
     bool ring_mode = false;  // ghost cells are not all copies of interior cells: keep them in `ring`
     Bnd B{}, Bh{};           // the boundary for kernels (states in st_dev) and for host code (st_host)
     double4 st_host[kBndStates] = {};
@@ -226,8 +227,8 @@ int upload_tables(gpm_ctx *c, const double *zT, const double *d2x, const double 
             h.d2y[k] = d2y[2 + mord * k];
         }
     for (int j = 0; j < ngp; ++j) h.gw[j] = gw[(ngp - 1) + 5 * j];
-    if (c->cfg.method == GPM_POL_MOOD) {
-        // POL_MOOD reconstructs order 3 with Pol_zT_o3 (pol_presim, GP_init.f90:359-411)
+    if (c->cfg.method == GPM_POL_MOOD || c->cfg.method == GPM_UNLIM_POL) {
+        // POL_MOOD and Unlim_POL reconstruct order 3 with Pol_zT_o3 (pol_presim, GP_init.f90:359-411)
         // over the same five stencil cells: o3_coef (constants.f90:59-61) placed
         // per face and Gauss point as (M, L, T, R, B)
         const double cf[5] = {15. / 18, 1. / 3, 1. / (4 * std::sqrt(3.)), -1. / (4 * std::sqrt(3.)), -1. / 6};
@@ -349,7 +350,7 @@ int plan_stages(gpm_ctx *c, StagePlan *P, StepExtra *extra = nullptr)
         st(n++, c->dA, c->dB, kCmbRK3b, 1);
         st(n++, c->dB, c->dA, kCmbRK3c, 2);
     }
-    if (c->ring_mode && !c->fog) {
+    if (c->ring_mode && !c->nodetect) {
         ex[n - 1].band = c->band;
         P[n - 1].C.band = 1;
     }
@@ -369,6 +370,8 @@ StageArgs stage_args(gpm_ctx *c, int k, const StagePlan &P)
     s.dmp = c->cfg.dmp; s.u2 = c->cfg.u2; s.u2tol = c->cfg.u2_tol;
     s.cascade = c->cfg.cascade;
     s.recon = c->cfg.method == GPM_POL_MOOD ? kReconPol : kReconGP;
+    // GP_SE and Unlim_POL: every face at order Mord, no detection
+    s.frozen = c->cfg.method == GPM_GP_SE || c->cfg.method == GPM_UNLIM_POL;
     s.dx = c->dx; s.dy = c->dy; s.tiles_x = c->tiles_x; s.ntiles = c->ntiles;
     return s;
 }
@@ -380,7 +383,18 @@ int enqueue_ring(gpm_ctx *c)
     RingArgs r{};
     r.A = c->dU; r.band = c->band; r.ring = c->ring; r.tstate = c->tstate; r.B = c->B; r.gravity = c->gravity;
     r.T = c->dT; r.flux = c->cfg.num_flux;
-    r.mode = c->fog ? kRingFOG : c->integrator == GPM_FE ? kRingFE : c->integrator == GPM_SSP_RK2 ? kRingRK2 : kRingRK3;
+    const int integ = c->integrator == GPM_FE ? kRingFE : c->integrator == GPM_SSP_RK2 ? kRingRK2 : kRingRK3;
+    if (c->nodetect) {
+        // the last stage's input: Un (FE), U1 (SSP_RK2) or U2 (SSP_RK3), still in its buffer
+        StagePlan plan[kMaxStages];
+        const int nst = plan_stages(c, plan);
+        r.mode = kRingFace;
+        r.integ = integ;
+        r.Bin = plan[nst - 1].in;
+        HIPCHK(c, launch_ring_face(c->fog ? 1 : c->mord, c->fog ? 1 : c->ngp, c->cfg.num_flux, r, c->stream));
+        return GPM_OK;
+    }
+    r.mode = integ;
     HIPCHK(c, launch_ring(r, c->stream));
     return GPM_OK;
 }
@@ -456,7 +470,7 @@ int enqueue_stages(gpm_ctx *c)
         HIPCHK(c, launch_stage(c->fog ? 1 : c->mord, c->fog ? 1 : c->ngp, c->cfg.num_flux, c->ring_mode, s,
                                (c->ntiles + 7) / 8 * 8, c->stream));
         if (c->timing) HIPCHK(c, hipEventRecord(e1, c->stream));
-        if (c->fog) continue;  // one pass, no detection (mod_FE.f90:160-162)
+        if (c->nodetect) continue;  // one pass, no detection (mod_FE.f90:160-162)
 
         MoodArgs m{};
         m.Uin = in; m.Uout = plan[k].out; m.C = plan[k].C; m.B = c->B; m.gpo = c->gpo;
@@ -541,6 +555,7 @@ int gpm_create(const gpm_config *cfg, gpm_ctx **out)
     c->integrator = cfg->integrator;
     c->gravity = cfg->problem == GPM_PROBLEM_RT;
     c->fog = cfg->method == GPM_FOG;
+    c->nodetect = c->fog || cfg->method == GPM_GP_SE || cfg->method == GPM_UNLIM_POL;
     auto bad = [&](int code, const char *m) {
         std::fprintf(stderr, "gpm_create: %s\n", m);
         delete c;
@@ -558,18 +573,20 @@ int gpm_create(const gpm_config *cfg, gpm_ctx **out)
         cfg->num_flux != GPM_UNSPLIT)
         return bad(GPM_EUNSUP, "numFlux must be HLLC, LLF, HLL or unsplit");
     const bool nn_method = cfg->method == GPM_NN_GP_MOOD || cfg->method == GPM_NN_GP_MOOD_CC;
-    if (cfg->method != GPM_GP_MOOD && !nn_method && cfg->method != GPM_FOG && cfg->method != GPM_POL_MOOD)
-        return bad(GPM_EUNSUP, "method must be GP_MOOD, NN_GP_MOOD, NN_GP_MOOD_CC, POL_MOOD or FOG");
-    // Pol_zT_o3 holds 2 Gauss points and POL_MOOD_ has no order above 3 (any
-    // other order takes the cell value, mod_pol_MOOD.f90:62-76)
-    if (cfg->method == GPM_POL_MOOD && (cfg->mord != 3 || cfg->ngp != 2))
-        return bad(GPM_EUNSUP, "POL_MOOD needs Mord = 3 and ngp = 2");
-    if (c->fog && (cfg->ngp != 1 || cfg->integrator != GPM_FE))
-        return bad(GPM_EUNSUP, "FOG needs ngp = 1 and integrator FE (the problems/*_FOG.f90 set-up)");
+    const bool pol = cfg->method == GPM_POL_MOOD || cfg->method == GPM_UNLIM_POL;
+    if (cfg->method != GPM_GP_MOOD && !nn_method && cfg->method != GPM_FOG && !pol && cfg->method != GPM_GP_SE)
+        return bad(GPM_EUNSUP, "method must be GP_MOOD, NN_GP_MOOD, NN_GP_MOOD_CC, POL_MOOD, Unlim_POL, GP_SE or FOG");
+    // Pol_zT_o3 holds 2 Gauss points and the polynomial methods have no order
+    // above 3 (POL_MOOD_ takes the cell value at any other order,
+    // mod_pol_MOOD.f90:62-76; Unlim_POL_ stops, mod_Unlim_Pol.f90:33-36)
+    if (pol && (cfg->mord != 3 || cfg->ngp != 2))
+        return bad(GPM_EUNSUP, "POL_MOOD and Unlim_POL need Mord = 3 and ngp = 2");
+    if (c->fog && cfg->ngp != 1)
+        return bad(GPM_EUNSUP, "FOG needs ngp = 1 (the problems/*_FOG.f90 set-up)");
     if (!bc_ok) return bad(GPM_EUNSUP, "unknown BC_type");
     // FOG leaves the face loop's updates in the first ghost layer (no Boundary_C
     // after the step), so its ghost cells are never copies of interior cells
-    c->ring_mode = !index_map_bc(cfg->bc_type) || c->gravity || c->fog;
+    c->ring_mode = !index_map_bc(cfg->bc_type) || c->gravity || c->nodetect;
     if (c->ring_mode && cfg->integrator == GPM_SSP_RK4)
         return bad(GPM_EUNSUP, "SSP_RK4 needs a copy boundary (Neumann, Periodic, Reflective, KH_BC) and no source term");
     // the network reads the stage input's ghost cells as copies of interior cells

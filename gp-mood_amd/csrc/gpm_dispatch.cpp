@@ -22,6 +22,14 @@ const void *mood_kernel(int mord, int ngp, int flux, bool gen)
     return nullptr;
 }
 
+hipError_t launch_ring_face(int mord, int ngp, int flux, const RingArgs &a, hipStream_t s)
+{
+#define X(M, G) if (mord == M && ngp == G) return launch_ring_face_mg<M, G>(flux, a, s);
+    GPM_COMBOS(X)
+#undef X
+    return hipErrorInvalidValue;
+}
+
 bool supported(int mord, int ngp) { return mord > 1 && mood_kernel(mord, ngp, kHLLC, false) != nullptr; }
 
 }  // namespace gpm

@@ -171,58 +171,10 @@ __global__ void k_ring(RingArgs a)
                                  (1.0 / 3) * u.z + (2.0 / 3) * c[2], (1.0 / 3) * u.w + (2.0 / 3) * c[3]);
 }
 
-// Ghost ring after a FOG Forward_Euler step (kRingFOG).  FOG ends its MOOD
-// loop without DETECTION (mod_FE.f90:160-162), so no Boundary_C refills Uout:
-// its ghost cells keep the first stage's Boundary_C(Uin) values, and the face
-// loop (mod_FE.f90:75-145, faces between cells -1..lf and -1..nf in 0-based
-// indices) has updated those of the first layer, in the same order as an
-// interior cell's updates: + bottom, + left, - right, - top face.
-template <int FLUX>
-__global__ void k_ring_fog(RingArgs a)
-{
-    const int NX = a.B.NX, NY = a.B.NY;
-    const int r = blockIdx.x * blockDim.x + threadIdx.x;
-    if (r >= ring_size(NX, NY)) return;
-    int i, j;
-    ring_cell(r, NX, NY, i, j);
-    const double *ts = a.tstate;
-    const double t0 = ts[4], dt = ts[1], dtdx = ts[2], dtdy = ts[3];
-    const double xt = dmr_xmin_top(t0, a.B.sqrt3);
-    const DevSrc src{a.A, NX};
-    const double4 u = bnd_value(a.B, src, i, j, xt);
-    double c[4] = {u.x, u.y, u.z, u.w};
-    if (i >= -1 && i <= NX && j >= -1 && j <= NY) {
-        // face (p, q) -> (p+1, q) or (p, q) -> (p, q+1) exists for p in [-1, NX-1], q in [-1, NY-1]
-        auto exists = [&](int p, int q) { return p >= -1 && p <= NX - 1 && q >= -1 && q <= NY - 1; };
-        double L[4];
-        if (exists(i, j - 1)) {
-            fog_flux<1, FLUX>(a.T, bnd_value(a.B, src, i, j - 1, xt), u, L);
-            for (int k = 0; k < 4; ++k) c[k] = c[k] + dtdy * L[k];
-        }
-        if (exists(i - 1, j)) {
-            fog_flux<0, FLUX>(a.T, bnd_value(a.B, src, i - 1, j, xt), u, L);
-            for (int k = 0; k < 4; ++k) c[k] = c[k] + dtdx * L[k];
-        }
-        if (exists(i, j)) {
-            fog_flux<0, FLUX>(a.T, u, bnd_value(a.B, src, i + 1, j, xt), L);
-            for (int k = 0; k < 4; ++k) c[k] = c[k] - dtdx * L[k];
-            fog_flux<1, FLUX>(a.T, u, bnd_value(a.B, src, i, j + 1, xt), L);
-            for (int k = 0; k < 4; ++k) c[k] = c[k] - dtdy * L[k];
-        }
-    }
-    if (a.gravity) gravity_source(c, dt * kGravity);
-    a.ring[r] = make_double4(c[0], c[1], c[2], c[3]);
-}
-
 hipError_t launch_ring(const RingArgs &a, hipStream_t s)
 {
     const int n = ring_size(a.B.NX, a.B.NY);
-    const dim3 grid((n + 255) / 256), block(256);
-    if (a.mode != kRingFOG) hipLaunchKernelGGL(k_ring, grid, block, 0, s, a);
-    else if (a.flux == kHLL) hipLaunchKernelGGL(k_ring_fog<kHLL>, grid, block, 0, s, a);
-    else if (a.flux == kLLF) hipLaunchKernelGGL(k_ring_fog<kLLF>, grid, block, 0, s, a);
-    else if (a.flux == kUnsplit) hipLaunchKernelGGL(k_ring_fog<kUnsplit>, grid, block, 0, s, a);
-    else hipLaunchKernelGGL(k_ring_fog<kHLLC>, grid, block, 0, s, a);
+    hipLaunchKernelGGL(k_ring, dim3((n + 255) / 256), dim3(256), 0, s, a);
     return hipGetLastError();
 }
 

@@ -181,17 +181,22 @@ enum {
     kRingFE,        // S(G_t0(B))
     kRingRK2,       // 0.5*(G_t0(Un) + S(G_t0(B)))
     kRingRK3,       // (1./3)*G_t0(Un) + (2./3)*S(G_{t0+dt/2}(B))
-    kRingFOG,       // S(G_t0(Un) + the face loop's updates of the first ghost layer), FE with FOG
+    kRingFace,      // methods without DETECTION (FOG, GP_SE, Unlim_POL): the last stage's
+                    // E = S(G_tb(B) + the face loop's updates of the first ghost layer), B the
+                    // last stage's input; then E (FE), 0.5*(G_t0(Un) + E) (SSP_RK2) or
+                    // (1./3)*G_t0(Un) + (2./3)*E (SSP_RK3, tb = t0 + dt/2)
 };
 
 struct RingArgs {
     const double4 *__restrict__ A;     // kRingFill: the state; else the step-start state Un
     const double4 *__restrict__ band;
+    const double4 *__restrict__ Bin;  // kRingFace: the last stage's input
     double4 *__restrict__ ring;
     const double *__restrict__ tstate;
     Bnd B;
-    const DevTables *__restrict__ T;  // kRingFOG: gauss_weight(1, 1)
+    const DevTables *__restrict__ T;  // kRingFace: the reconstruction tables and Gauss weights
     int mode, gravity, flux;
+    int integ;                        // kRingFace: kRingFE, kRingRK2 or kRingRK3 (the combination)
 };
 
 template <int MORD, int NGP>
@@ -207,6 +212,11 @@ const void *mood_kernel(int mord, int ngp, int flux, bool gen);
 bool supported(int mord, int ngp);
 hipError_t launch_dt(const DtArgs &a, int nblocks, hipStream_t s);
 hipError_t launch_ring(const RingArgs &a, hipStream_t s);
+// kRingFace for (mord, ngp): (1, 1) FOG's cell values, else the face states of
+// the order-mord reconstruction in T (GP weights, or the polynomial weights)
+hipError_t launch_ring_face(int mord, int ngp, int flux, const RingArgs &a, hipStream_t s);
+template <int MORD, int NGP>
+hipError_t launch_ring_face_mg(int flux, const RingArgs &a, hipStream_t s);
 hipError_t launch_unpack_orders(const order_t *g, int8_t *out, size_t n, hipStream_t s);
 
 // The reference's own network (mod_NN.f90): per-cell a priori orders of a

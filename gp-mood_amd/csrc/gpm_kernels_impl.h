@@ -594,6 +594,83 @@ __global__ __launch_bounds__(kMoodThreads) void k_mood(MoodArgs a)
 // both boundary classes (gen: formula boundaries or a source term); the
 // copy-boundary stage kernels are also specialised on the SSP-RK3 combination
 // kinds.  Each combination is compiled in a translation unit of its own.
+// The stage input through the ghost rule, for reconstructions at ghost cells.
+struct BndAcc {
+    Bnd B;
+    const double4 *__restrict__ U;
+    double xmin_top;
+    __device__ __forceinline__ double4 operator()(int i, int j) const
+    {
+        return bnd_value(B, DevSrc{U, B.NX}, i, j, xmin_top);
+    }
+};
+
+// Ghost ring after a step of a method that ends its MOOD loop without
+// DETECTION (FOG, GP_SE, Unlim_POL; mod_FE.f90:160-162).  No Boundary_C
+// refills the last stage's Uout: its ghost cells keep that stage's
+// Boundary_C(Uin) values, and the face loop (mod_FE.f90:75-145, faces between
+// cells -1..lf and -1..nf in 0-based indices, face states from the
+// reconstruction of cells -1..lf, -1..nf) has updated those of the first
+// layer, in the order of an interior cell's updates: + bottom, + left,
+// - right, - top face.  The integrator's combination with the step-start state
+// follows, as for the interior.  MORD 1: FOG's cell values as face states.
+template <int MORD, int NGP, int FLUX>
+__global__ __launch_bounds__(128) void k_ring_face(RingArgs a)
+{
+    const int NX = a.B.NX, NY = a.B.NY;
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= ring_size(NX, NY)) return;
+    int i, j;
+    ring_cell(r, NX, NY, i, j);
+    const double *ts = a.tstate;
+    const double t0 = ts[4], dt = ts[1], dtdx = ts[2], dtdy = ts[3];
+    const double tb = a.integ == kRingRK3 ? t0 + 0.5 * dt : t0;
+    const BndAcc acc{a.B, a.Bin, dmr_xmin_top(tb, a.B.sqrt3)};
+    const double4 u = acc(i, j);
+    double c[4] = {u.x, u.y, u.z, u.w};
+    if (i >= -1 && i <= NX && j >= -1 && j <= NY) {
+        // face (p, q) -> (p+1, q) or (p, q) -> (p, q+1) exists for p in [-1, NX-1], q in [-1, NY-1]
+        auto exists = [&](int p, int q) { return p >= -1 && p <= NX - 1 && q >= -1 && q <= NY - 1; };
+        auto flux = [&](int dir, int ai, int aj, int bi, int bj, double L[4]) {
+            if (MORD == 1) {
+                if (dir == 0) fog_flux<0, FLUX>(a.T, acc(ai, aj), acc(bi, bj), L);
+                else fog_flux<1, FLUX>(a.T, acc(ai, aj), acc(bi, bj), L);
+            } else if (dir == 0) {
+                face_flux<MORD, NGP, 0, FLUX>(a.T, acc, ai, aj, bi, bj, L);
+            } else {
+                face_flux<MORD, NGP, 1, FLUX>(a.T, acc, ai, aj, bi, bj, L);
+            }
+        };
+        double L[4];
+        if (exists(i, j - 1)) {
+            flux(1, i, j - 1, i, j, L);
+            for (int k = 0; k < 4; ++k) c[k] = c[k] + dtdy * L[k];
+        }
+        if (exists(i - 1, j)) {
+            flux(0, i - 1, j, i, j, L);
+            for (int k = 0; k < 4; ++k) c[k] = c[k] + dtdx * L[k];
+        }
+        if (exists(i, j)) {
+            flux(0, i, j, i + 1, j, L);
+            for (int k = 0; k < 4; ++k) c[k] = c[k] - dtdx * L[k];
+            flux(1, i, j, i, j + 1, L);
+            for (int k = 0; k < 4; ++k) c[k] = c[k] - dtdy * L[k];
+        }
+    }
+    if (a.gravity) gravity_source(c, dt * kGravity);
+    if (a.integ == kRingFE) {
+        a.ring[r] = make_double4(c[0], c[1], c[2], c[3]);
+        return;
+    }
+    // the step-start state's ghosts as Boundary_C left them in the first stage
+    const double4 un = bnd_value(a.B, DevSrc{a.A, NX}, i, j, dmr_xmin_top(t0, a.B.sqrt3));
+    if (a.integ == kRingRK2)
+        a.ring[r] = make_double4(0.5 * (un.x + c[0]), 0.5 * (un.y + c[1]), 0.5 * (un.z + c[2]), 0.5 * (un.w + c[3]));
+    else
+        a.ring[r] = make_double4((1.0 / 3) * un.x + (2.0 / 3) * c[0], (1.0 / 3) * un.y + (2.0 / 3) * c[1],
+                                 (1.0 / 3) * un.z + (2.0 / 3) * c[2], (1.0 / 3) * un.w + (2.0 / 3) * c[3]);
+}
+
 #define GPM_INSTANTIATE_COMBO(M, G)                                                                          \
     template <int F, int C>                                                                                   \
     static void launch_stage_fc##M##G(bool gen, const StageArgs &a, int nblocks, hipStream_t s)              \
@@ -634,6 +711,16 @@ __global__ __launch_bounds__(kMoodThreads) void k_mood(MoodArgs a)
         if (flux == kLLF) return mood_kernel_f##M##G<kLLF>(gen);                                             \
         if (flux == kUnsplit) return mood_kernel_f##M##G<kUnsplit>(gen);                                     \
         return mood_kernel_f##M##G<kHLLC>(gen);                                                              \
+    }                                                                                                         \
+    template <>                                                                                               \
+    hipError_t launch_ring_face_mg<M, G>(int flux, const RingArgs &a, hipStream_t s)                         \
+    {                                                                                                         \
+        const dim3 grid((ring_size(a.B.NX, a.B.NY) + 127) / 128), block(128);                                 \
+        if (flux == kHLL) hipLaunchKernelGGL((k_ring_face<M, G, kHLL>), grid, block, 0, s, a);               \
+        else if (flux == kLLF) hipLaunchKernelGGL((k_ring_face<M, G, kLLF>), grid, block, 0, s, a);          \
+        else if (flux == kUnsplit) hipLaunchKernelGGL((k_ring_face<M, G, kUnsplit>), grid, block, 0, s, a);  \
+        else hipLaunchKernelGGL((k_ring_face<M, G, kHLLC>), grid, block, 0, s, a);                           \
+        return hipGetLastError();                                                                             \
     }
 
 }  // namespace gpm

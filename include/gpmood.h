@@ -33,6 +33,8 @@ extern "C" {
 #define GPM_FOG 1
 #define GPM_GP_MOOD 2
 #define GPM_POL_MOOD 200
+#define GPM_UNLIM_POL 201
+#define GPM_GP_SE 202
 #define GPM_NN_GP_MOOD 203
 #define GPM_NN_GP_MOOD_CC 204
 #define GPM_FE 8
@@ -75,8 +77,12 @@ typedef struct gpm_config {
     int32_t method;       /* GPM_GP_MOOD; GPM_NN_GP_MOOD, GPM_NN_GP_MOOD_CC (Mord 3) (copy boundary,
                              no source term; a network from gpm_set_reference_nn, or for
                              NN_GP_MOOD a learned predictor from gpm_set_predictor);
-                             GPM_FOG, first-order Godunov (mod_FOG.f90; ngp = 1, integrator FE);
-                             GPM_POL_MOOD, polynomial MOOD (mod_pol_MOOD.f90; Mord 3, ngp 2) */
+                             GPM_FOG, first-order Godunov (mod_FOG.f90; ngp = 1);
+                             GPM_POL_MOOD, polynomial MOOD (mod_pol_MOOD.f90; Mord 3, ngp 2);
+                             GPM_GP_SE, unlimited GP (mod_GP_SE.f90), and GPM_UNLIM_POL,
+                             unlimited polynomials (mod_Unlim_Pol.f90; Mord 3, ngp 2): one
+                             pass, no detection.  FOG, GP_SE and Unlim_POL: FE, SSP_RK2 or
+                             SSP_RK3 */
     int32_t integrator;   /* GPM_FE, GPM_SSP_RK2, GPM_SSP_RK3 or GPM_SSP_RK4 (time_stepper.f90:17-21;
                              SSP_RK4 only with a copy boundary and no source term) */
     int32_t num_flux;     /* GPM_HLLC (mod_HLLC.f90), GPM_LLF (mod_LLF.f90), GPM_HLL (mod_HLL.f90)

@@ -8,7 +8,8 @@
  * It restates, operation for operation, the FP64 arithmetic of the reference
  * Fortran (slug-cfd/gp-mood, gp-mood-code/) for
  *   method in {GP_MOOD, FOG (ngp = 1), NN_GP_MOOD and NN_GP_MOOD_CC with the
- *             reference's own network (mod_NN.f90, text weights), POL_MOOD},
+ *             reference's own network (mod_NN.f90, text weights), POL_MOOD,
+ *             Unlim_POL and GP_SE (one pass, no detection)},
  *   integrator in {FE, SSP_RK2, SSP_RK3, SSP_RK4},
  *   numFlux in {HLLC, LLF, HLL, unsplit},
  *   BC_type in {Neumann, Periodic, Dirichlet, Reflective, KH_BC, Mach800_BC,
@@ -45,7 +46,10 @@ enum {
     MACH800_BC = 201, DOUBLEMACH800_BC = 301, RMI_BC = 401, DMR_BC = 35, RT_BC = 402
 };
 enum { PROBLEM_RT = 51 };
-enum { METHOD_FOG = 1, METHOD_GP_MOOD = 2, METHOD_POL_MOOD = 200, METHOD_NN_GP_MOOD = 203, METHOD_NN_GP_MOOD_CC = 204 };
+enum {
+    METHOD_FOG = 1, METHOD_GP_MOOD = 2, METHOD_POL_MOOD = 200, METHOD_UNLIM_POL = 201, METHOD_GP_SE = 202,
+    METHOD_NN_GP_MOOD = 203, METHOD_NN_GP_MOOD_CC = 204
+};
 /* global_variables.f90:97 */
 enum { NSTEPS_WITH_NO_NN = 10 };
 
@@ -935,6 +939,28 @@ static int nn_cc_recons(orc_t *o, const double *Uin)
     return bad ? -1 : 0;
 }
 
+/* The face loop of mod_FE.f90:75-145 with every face flagged, in the
+   reference's n-outer / l-inner order, from the face states in Uh: for the
+   methods that end the MOOD loop without DETECTION (GP_SE, Unlim_POL), whose
+   Uout keeps the updates of the first ghost layer. */
+static void uh_face_loop(orc_t *o, double *Uout, double dtdx, double dtdy)
+{
+    for (int n = 0; n <= o->nf; ++n)
+        for (int l = 0; l <= o->lf; ++l) {
+            double Lf[4];
+            face_flux(o, 0, l, n, Lf);
+            for (int k = 0; k < 4; ++k) {
+                Uout[CELL(o, l, n) * 4 + k] = Uout[CELL(o, l, n) * 4 + k] - dtdx * Lf[k];
+                Uout[CELL(o, l + 1, n) * 4 + k] = Uout[CELL(o, l + 1, n) * 4 + k] + dtdx * Lf[k];
+            }
+            face_flux(o, 1, l, n, Lf);
+            for (int k = 0; k < 4; ++k) {
+                Uout[CELL(o, l, n) * 4 + k] = Uout[CELL(o, l, n) * 4 + k] - dtdy * Lf[k];
+                Uout[CELL(o, l, n + 1) * 4 + k] = Uout[CELL(o, l, n + 1) * 4 + k] + dtdy * Lf[k];
+            }
+        }
+}
+
 static void orc_fe(orc_t *o, double *Uin, double *Uout, double dt, int first, int *npass)
 {
     const int lf = o->lf, nf = o->nf;
@@ -960,13 +986,21 @@ static void orc_fe(orc_t *o, double *Uin, double *Uout, double dt, int first, in
     memcpy(Uout, Uin, ncell * 4 * sizeof(double));
     const double dtdx = dt / o->dx, dtdy = dt / o->dy;
     int passes = 0;
+    const int nodetect = o->method == METHOD_FOG || o->method == METHOD_GP_SE || o->method == METHOD_UNLIM_POL;
     if (o->method == METHOD_FOG) {
         /* one pass; MOOD_finished without DETECTION (mod_FE.f90:160-162), so
            Uout's ghost cells keep Uin's boundary values plus the face updates */
         fog_face_loop(o, Uin, Uout, dtdx, dtdy);
         passes = 1;
+    } else if (nodetect) {
+        /* GP_SE_ (mod_GP_SE.f90:11-37: every face at order Mord) and Unlim_POL_
+           (mod_Unlim_Pol.f90:11-58: every face from Pol_zT_o3), one pass */
+        if (o->method == METHOD_GP_SE) gp_mood_recons(o, Uin);
+        else pol_mood_recons(o, Uin);
+        uh_face_loop(o, Uout, dtdx, dtdy);
+        passes = 1;
     }
-    for (; o->method != METHOD_FOG;) {
+    for (; !nodetect;) {
         ++passes;
         if (o->method == METHOD_NN_GP_MOOD_CC && o->niter > NSTEPS_WITH_NO_NN) {
             if (nn_cc_recons(o, Uin)) o->err = 1;

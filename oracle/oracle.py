@@ -16,7 +16,8 @@ BC_CODES = {"neumann": 17, "periodic": 18, "dirichlet": 19, "reflective": 20, "k
 FLUX_CODES = {"hllc": 100, "llf": 101, "hll": 102, "unsplit": 103}
 INTEGRATORS = {"fe": 8, "ssp_rk2": 9, "ssp_rk3": 10, "ssp_rk4": 12}
 PROBLEM_RT = 51
-METHODS = {"gp_mood": 2, "fog": 1, "pol_mood": 200, "nn_gp_mood": 203, "nn_gp_mood_cc": 204}
+METHODS = {"gp_mood": 2, "fog": 1, "pol_mood": 200, "unlim_pol": 201, "gp_se": 202, "nn_gp_mood": 203,
+           "nn_gp_mood_cc": 204}
 CASCADES = {"reference": 0, "full": 1}
 
 _lib = None

@@ -126,6 +126,15 @@ CONFIGS = {
     "pol_r1_n32_ref_hll": (32, 3, 2, "reflective", True, True, True, "hll"),
     "pol_r1_dmr_64x16": (64, 3, 2, "dmr", True, True, True),
     "pol_r1_n64_per_nou2": (64, 3, 2, "periodic", True, False, False, "unsplit"),
+    # one pass without detection: unlimited GP (GP_SE, mod_GP_SE.f90) and
+    # unlimited polynomials (Unlim_POL, mod_Unlim_Pol.f90; Mord 3, ngp 2)
+    "gpse_r2_n64_per": (64, 5, 3, "periodic", True, True, True),
+    "gpse_r3_n32_neu_fe": (32, 7, 4, "neumann", True, True, True, "unsplit"),
+    "gpse_r2_rt_16x48": (16, 5, 3, "rt", True, True, True, "unsplit"),
+    "unlim_r1_n64_kh_rk2": (64, 3, 2, "kh", True, True, True, "hll"),
+    # FOG with the multi-stage integrators
+    "fog_n64_per_rk3": (64, 3, 1, "periodic", True, True, True),
+    "fog_dmr_64x16_rk2": (64, 3, 1, "dmr", True, True, True, "hll"),
 }
 
 # The problem files' own configurations (problems/*.f90: grid, domain, method,
@@ -193,6 +202,12 @@ EXTRAS = {
     "pol_r1_n32_ref_hll": dict(method="POL_MOOD", integrator="ssp_rk2"),
     "pol_r1_dmr_64x16": dict(nf=16, lx="4.", ly="1.", method="POL_MOOD"),
     "pol_r1_n64_per_nou2": dict(method="POL_MOOD"),
+    "gpse_r2_n64_per": dict(method="GP_SE"),
+    "gpse_r3_n32_neu_fe": dict(method="GP_SE", integrator="fe"),
+    "gpse_r2_rt_16x48": dict(nf=48, lx="0.5", ly="1.5", problem="RT", method="GP_SE"),
+    "unlim_r1_n64_kh_rk2": dict(method="Unlim_POL", integrator="ssp_rk2"),
+    "fog_n64_per_rk3": dict(method="FOG"),
+    "fog_dmr_64x16_rk2": dict(nf=16, lx="4.", ly="1.", method="FOG", integrator="ssp_rk2"),
 }
 
 

@@ -175,6 +175,13 @@ STEP_CASES = {
     "pol_r1_n32_ref_hll_bl50": ("pol_r1_n32_ref_hll", "blast", 50, 20),
     "pol_r1_dmr_64x16": ("pol_r1_dmr_64x16", "dmr:0", 0, 20),
     "pol_r1_n64_per_nou2_fq51": ("pol_r1_n64_per_nou2", "four_quadrant", 51, 25),
+    # GP_SE and Unlim_POL (smooth states: no limiter)
+    "gpse_r2_n64_per_sh52": ("gpse_r2_n64_per", "shear_layer", 52, 20),
+    "gpse_r3_n32_neu_fe_sh53": ("gpse_r3_n32_neu_fe", "shear_layer", 53, 20),
+    "gpse_r2_rt_16x48_n54": ("gpse_r2_rt_16x48", "rayleigh_taylor:0.02", 54, 15),
+    "unlim_r1_n64_kh_rk2_sh55": ("unlim_r1_n64_kh_rk2", "shear_layer", 55, 20),
+    "fog_n64_per_rk3_fq56": ("fog_n64_per_rk3", "four_quadrant", 56, 25),
+    "fog_dmr_64x16_rk2": ("fog_dmr_64x16_rk2", "dmr:0", 0, 25),
 }
 # the reference's own network (method NN_GP_MOOD / NN_GP_MOOD_CC): name:
 # (config, IC kind, seed, steps); the network is refnn_weights(seed)
@@ -236,7 +243,7 @@ def main(argv):
             problem=51 if e["problem"] == "RT" else 0, method=e["method"].lower(),
             kind=kind, seed=seed, nsteps=nsteps, ic_sha=sha(U0), U0=U0,
             dt=r["steps"]["dt"], t=r["steps"]["t"], counts=r["steps"]["cnt"], masks=r["masks"], U=r["U"],
-            **({"pol": r["pol"]} if e["method"] == "POL_MOOD" else {}),
+            **({"pol": r["pol"]} if e["method"] in ("POL_MOOD", "Unlim_POL") else {}),
             **extra, **{k: v for k, v in r["tables"].items()})
         print("wrote", name, "lowered", int((r["masks"] < mord).sum()))
     for name, (cfg, kind, seed, nsteps) in NN_CASES.items():

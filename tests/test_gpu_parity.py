@@ -22,7 +22,8 @@ def make_config(gpm, z):
     return gpm.Config(lf=int(z["n"]), nf=o["nf"], Mord=int(z["mord"]), ngp=int(z["ngp"]), BC_type=str(z["bc"]),
                       DMP=bool(z["dmp"]), U2=bool(z["u2"]), U2_tol=bool(z["u2tol"]), numFlux=o["flux"],
                       integrator=o["integrator"], problem=o["problem"], Lx=o["lx"], Ly=o["ly"],
-                      method={"fog": gpm.capi.FOG, "pol_mood": gpm.capi.POL_MOOD}.get(o["method"], gpm.capi.GP_MOOD))
+                      method={"fog": gpm.capi.FOG, "pol_mood": gpm.capi.POL_MOOD, "gp_se": gpm.capi.GP_SE,
+                              "unlim_pol": gpm.capi.UNLIM_POL}.get(o["method"], gpm.capi.GP_MOOD))
 
 
 def make_solver(gpm, z, tables=True):
