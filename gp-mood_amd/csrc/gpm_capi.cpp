@@ -67,6 +67,9 @@ struct gpm_ctx {
     bool priori_set = false;         // the last step ran the network (else CellGPO_priori = Mord)
     float *rowv = nullptr, *vnn = nullptr;  // NN_GP_MOOD_CC: values of each row's last cell; values_NN
     long long niter = 0;             // main.f90's niter: steps started so far
+    // host-array exchange (allocated on first use): a ghosted state and a ghosted CellGPO
+    double4 *xfer = nullptr;
+    int32_t *xfer_gpo = nullptr;
     // learned a priori predictor (not owned)
     gpm_mlp *nn = nullptr;
     int nn_gate = 1;
@@ -115,60 +118,34 @@ hipError_t dalloc(T **p, size_t n)
     return hipMalloc(reinterpret_cast<void **>(p), n * sizeof(T) > 0 ? n * sizeof(T) : 1);
 }
 
-// Interior cells of a host state [NY][NX][4].
-struct HostSrc {
-    const double *in;
-    int NX;
+// Interior cells of a ghosted host state [NY+8][NX+8][4].
+struct GhostedSrc {
+    const double *U;
+    int W;  // NX + 8
     double4 operator()(int i, int j) const
     {
-        const double *p = in + ((size_t)j * NX + i) * 4;
+        const double *p = U + ((size_t)(j + 4) * W + (i + 4)) * 4;
         return make_double4(p[0], p[1], p[2], p[3]);
     }
 };
 
-// interior [NY][NX][4] -> ghosted [NY+8][NX+8][4] with Boundary_C at the time
-// whose DMR switch abscissa is xmin_top (the same rule the kernels apply)
-void fill_ghosted(const double *in, double *out, const Bnd &B, double xmin_top)
+// The ghost cells of a ghosted host state by Boundary_C applied to its own
+// interior, at the time whose DMR switch abscissa is xmin_top (the rule the
+// kernels apply; it reads interior cells only).
+void fill_ghosts(double *U, const Bnd &B, double xmin_top)
 {
-    const int W = B.NX + 8, H = B.NY + 8;
-    const HostSrc src{in, B.NX};
-    for (int jj = 0; jj < H; ++jj)
-        for (int ii = 0; ii < W; ++ii) {
-            const double4 v = bnd_value(B, src, ii - 4, jj - 4, xmin_top);
-            double *d = out + ((size_t)jj * W + ii) * 4;
-            d[0] = v.x; d[1] = v.y; d[2] = v.z; d[3] = v.w;
-        }
-}
-
-void extract_interior(const double *full, double *in, int NX, int NY)
-{
-    const int W = NX + 8;
-    for (int j = 0; j < NY; ++j)
-        std::memcpy(in + (size_t)j * NX * 4, full + ((size_t)(j + 4) * W + 4) * 4, sizeof(double) * NX * 4);
-}
-
-// ghosted [NY+8][NX+8][4] <-> ring (ring_cell order)
-void ghosts_to_ring(const double *full, double4 *ring, int NX, int NY)
-{
-    const int W = NX + 8;
+    const int NX = B.NX, NY = B.NY, W = NX + 8;
+    const GhostedSrc src{U, W};
     for (int r = 0; r < ring_size(NX, NY); ++r) {
         int i, j;
         ring_cell(r, NX, NY, i, j);
-        const double *p = full + ((size_t)(j + 4) * W + (i + 4)) * 4;
-        ring[r] = make_double4(p[0], p[1], p[2], p[3]);
+        const double4 v = bnd_value(B, src, i, j, xmin_top);
+        double *d = U + ((size_t)(j + 4) * W + (i + 4)) * 4;
+        d[0] = v.x; d[1] = v.y; d[2] = v.z; d[3] = v.w;
     }
 }
 
-void ring_to_ghosts(const double4 *ring, double *full, int NX, int NY)
-{
-    const int W = NX + 8;
-    for (int r = 0; r < ring_size(NX, NY); ++r) {
-        int i, j;
-        ring_cell(r, NX, NY, i, j);
-        double *p = full + ((size_t)(j + 4) * W + (i + 4)) * 4;
-        p[0] = ring[r].x; p[1] = ring[r].y; p[2] = ring[r].z; p[3] = ring[r].w;
-    }
-}
+
 
 // The boundary descriptor of a configuration, with its fixed ghost states in
 // st[kBndStates]: the formula rules' states by primitive_to_conservative
@@ -707,7 +684,7 @@ int gpm_destroy(gpm_ctx *c)
                     c->gpo, c->slistA, c->slistB, c->mark, c->counters,
                     c->stamp, c->err, c->solo_p, c->stats, c->tstate, c->block_min, c->dt_log, c->done, c->pred, c->nnlist,
                     c->nncount, c->adj_ub, c->adj_G, c->adj_A, c->adj_tmp, c->adj_ordx, c->adj_ordy,
-                    c->priori, c->refnn_w, c->rowv, c->vnn};
+                    c->priori, c->refnn_w, c->rowv, c->vnn, c->xfer, c->xfer_gpo};
     for (void *b : bufs)
         if (b) hipFree(b);
     for (hipEvent_t e : c->ev_pool) hipEventDestroy(e);
@@ -738,9 +715,7 @@ int gpm_boundary_c(const gpm_config *cfg, double t_rk, double *U)
     const double dx = quad_ratio(cfg->lx, cfg->lf), dy = quad_ratio(cfg->ly, cfg->nf);
     double4 st[kBndStates];
     const Bnd B = make_bnd(cfg->lf, cfg->nf, cfg->bc_type, dx, dy, st);
-    std::vector<double> in((size_t)cfg->lf * cfg->nf * 4);
-    extract_interior(U, in.data(), cfg->lf, cfg->nf);
-    fill_ghosted(in.data(), U, B, dmr_xmin_top(t_rk, B.sqrt3));
+    fill_ghosts(U, B, dmr_xmin_top(t_rk, B.sqrt3));
     return GPM_OK;
 }
 
@@ -748,17 +723,13 @@ int gpm_set_state(gpm_ctx *c, const double *U)
 {
     if (!c || !U) return fail(c, GPM_EINVAL, "null argument");
     const size_t n = (size_t)c->NX * c->NY;
-    std::vector<double> in(n * 4);
-    extract_interior(U, in.data(), c->NX, c->NY);
-    HIPCHK(c, hipMemcpyAsync(c->dU, in.data(), in.size() * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    // the whole ghosted array in one contiguous copy, split on the device; in
+    // ring mode the caller's ghost cells are what Setdt sees before the first step
+    const size_t full = (size_t)(c->NX + 8) * (c->NY + 8);
+    if (!c->xfer) HIPCHK(c, dalloc(&c->xfer, full));
+    HIPCHK(c, hipMemcpyAsync(c->xfer, U, full * sizeof(double4), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, launch_unpack_ghosted(c->xfer, c->dU, c->ring_mode ? c->ring : nullptr, c->NX, c->NY, c->stream));
     HIPCHK(c, hipMemsetD16Async(c->gpo, pack_order(c->mord, c->mord, 0), n, c->stream));
-    std::vector<double4> ring;
-    if (c->ring_mode) {
-        // the caller's ghost cells are what Setdt sees before the first step
-        ring.resize(ring_size(c->NX, c->NY));
-        ghosts_to_ring(U, ring.data(), c->NX, c->NY);
-        HIPCHK(c, hipMemcpyAsync(c->ring, ring.data(), ring.size() * sizeof(double4), hipMemcpyHostToDevice, c->stream));
-    }
     HIPCHK(c, hipStreamSynchronize(c->stream));
     c->has_state = true;
     return GPM_OK;
@@ -768,33 +739,19 @@ int gpm_get_state(gpm_ctx *c, double *U, int32_t *cellgpo)
 {
     if (!c || !U) return fail(c, GPM_EINVAL, "null argument");
     if (!c->has_state) return fail(c, GPM_ESTATE, "no state set");
-    const size_t n = (size_t)c->NX * c->NY;
-    std::vector<double> in(n * 4);
-    std::vector<order_t> g(n);
-    std::vector<double4> ring(c->ring_mode ? ring_size(c->NX, c->NY) : 0);
-    HIPCHK(c, hipMemcpyAsync(in.data(), c->dU, n * 4 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
-    HIPCHK(c, hipMemcpyAsync(g.data(), c->gpo, n * sizeof(order_t), hipMemcpyDeviceToHost, c->stream));
-    if (c->ring_mode)
-        HIPCHK(c, hipMemcpyAsync(ring.data(), c->ring, ring.size() * sizeof(double4), hipMemcpyDeviceToHost, c->stream));
-    HIPCHK(c, hipStreamSynchronize(c->stream));
-    const int W = c->NX + 8, H = c->NY + 8;
-    if (c->ring_mode) {
-        for (int j = 0; j < c->NY; ++j)
-            std::memcpy(U + ((size_t)(j + 4) * W + 4) * 4, in.data() + (size_t)j * c->NX * 4, sizeof(double) * c->NX * 4);
-        ring_to_ghosts(ring.data(), U, c->NX, c->NY);
-    } else {
-        fill_ghosted(in.data(), U, c->Bh, 0.0);
-    }
+    // ghosted on the device (ring, or the boundary rule for copy boundaries),
+    // then one contiguous copy each for the state and CellGPO
+    const size_t full = (size_t)(c->NX + 8) * (c->NY + 8);
+    if (!c->xfer) HIPCHK(c, dalloc(&c->xfer, full));
+    HIPCHK(c, launch_pack_ghosted(c->dU, c->ring_mode ? c->ring : nullptr, c->B, c->xfer, c->stream));
+    HIPCHK(c, hipMemcpyAsync(U, c->xfer, full * sizeof(double4), hipMemcpyDeviceToHost, c->stream));
     if (cellgpo) {
-        for (int jj = 0; jj < H; ++jj)
-            for (int ii = 0; ii < W; ++ii) {
-                const int i = ii - 4, j = jj - 4;
-                // FOG leaves CellGPO = Mord (mod_FE.f90:48)
-                cellgpo[(size_t)jj * W + ii] = (c->fog || i < 0 || i >= c->NX || j < 0 || j >= c->NY)
-                                                   ? c->mord
-                                                   : (int32_t)order_of_word(g[(size_t)j * c->NX + i]);
-            }
+        if (!c->xfer_gpo) HIPCHK(c, dalloc(&c->xfer_gpo, full));
+        // FOG leaves CellGPO = Mord (mod_FE.f90:48)
+        HIPCHK(c, launch_cellgpo_ghosted(c->gpo, c->xfer_gpo, c->NX, c->NY, c->mord, c->fog, c->stream));
+        HIPCHK(c, hipMemcpyAsync(cellgpo, c->xfer_gpo, full * sizeof(int32_t), hipMemcpyDeviceToHost, c->stream));
     }
+    HIPCHK(c, hipStreamSynchronize(c->stream));
     return GPM_OK;
 }
 
@@ -851,11 +808,7 @@ int gpm_time_stepping(gpm_ctx *c, double *Uin, double *Uout, double t, double dt
     if (rc) return rc;
     // Boundary_C(Uin) of the first Forward_Euler stage (mod_FE.f90:64, t_rk = t)
     // refills Uin's ghosts.
-    {
-        std::vector<double> in((size_t)c->NX * c->NY * 4);
-        extract_interior(Uin, in.data(), c->NX, c->NY);
-        fill_ghosted(in.data(), Uin, c->Bh, dmr_xmin_top(t, c->Bh.sqrt3));
-    }
+    fill_ghosts(Uin, c->Bh, dmr_xmin_top(t, c->Bh.sqrt3));
     rc = gpm_set_time(c, t);
     if (rc) return rc;
     rc = launch_dt_kernel(c, kDtGiven, 0, 0.0, 0.0, dt, 0, 0, false, 1);

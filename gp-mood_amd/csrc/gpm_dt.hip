@@ -178,6 +178,72 @@ hipError_t launch_ring(const RingArgs &a, hipStream_t s)
     return hipGetLastError();
 }
 
+// Ghosted state [NY+8][NX+8] of the interior U: ghost cells from the ring, or
+// by the boundary rule (copy boundaries).
+__global__ void k_pack_ghosted(const double4 *__restrict__ U, const double4 *__restrict__ ring, Bnd B,
+                               double4 *__restrict__ out)
+{
+    const int NX = B.NX, NY = B.NY, W = NX + 8;
+    const long long n = (long long)W * (NY + 8);
+    for (long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x; c < n;
+         c += (long long)gridDim.x * blockDim.x) {
+        const int i = (int)(c % W) - 4, j = (int)(c / W) - 4;
+        double4 v;
+        if (i >= 0 && i < NX && j >= 0 && j < NY) v = U[(size_t)j * NX + i];
+        else if (ring) v = ring[ring_index(i, j, NX, NY)];
+        else v = bnd_value(B, DevSrc{U, NX}, i, j, 0.0);
+        out[c] = v;
+    }
+}
+
+// Interior (and, with ring, the ghost cells) of a ghosted state.
+__global__ void k_unpack_ghosted(const double4 *__restrict__ in, double4 *__restrict__ U, double4 *__restrict__ ring,
+                                 int NX, int NY)
+{
+    const int W = NX + 8;
+    const long long n = (long long)W * (NY + 8);
+    for (long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x; c < n;
+         c += (long long)gridDim.x * blockDim.x) {
+        const int i = (int)(c % W) - 4, j = (int)(c / W) - 4;
+        if (i >= 0 && i < NX && j >= 0 && j < NY) U[(size_t)j * NX + i] = in[c];
+        else if (ring) ring[ring_index(i, j, NX, NY)] = in[c];
+    }
+}
+
+// CellGPO(lb:le, nb:ne) as int32 from the order words; ghost cells (and every
+// cell when `all_mord`) hold Mord.
+__global__ void k_cellgpo_ghosted(const order_t *__restrict__ g, int32_t *__restrict__ out, int NX, int NY, int mord,
+                                  int all_mord)
+{
+    const int W = NX + 8;
+    const long long n = (long long)W * (NY + 8);
+    for (long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x; c < n;
+         c += (long long)gridDim.x * blockDim.x) {
+        const int i = (int)(c % W) - 4, j = (int)(c / W) - 4;
+        const bool in = i >= 0 && i < NX && j >= 0 && j < NY;
+        out[c] = (!in || all_mord) ? mord : order_of_word(g[(size_t)j * NX + i]);
+    }
+}
+
+hipError_t launch_pack_ghosted(const double4 *U, const double4 *ring, const Bnd &B, double4 *out, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_pack_ghosted, dim3(2048), dim3(256), 0, s, U, ring, B, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_unpack_ghosted(const double4 *in, double4 *U, double4 *ring, int NX, int NY, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_unpack_ghosted, dim3(2048), dim3(256), 0, s, in, U, ring, NX, NY);
+    return hipGetLastError();
+}
+
+hipError_t launch_cellgpo_ghosted(const order_t *g, int32_t *out, int NX, int NY, int mord, int all_mord,
+                                  hipStream_t s)
+{
+    hipLaunchKernelGGL(k_cellgpo_ghosted, dim3(2048), dim3(256), 0, s, g, out, NX, NY, mord, all_mord);
+    return hipGetLastError();
+}
+
 hipError_t launch_dt(const DtArgs &a, int nblocks, hipStream_t s)
 {
     hipLaunchKernelGGL(k_dt, dim3(nblocks), dim3(kDtThreads), 0, s, a);

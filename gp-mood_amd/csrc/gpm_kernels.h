@@ -243,5 +243,10 @@ hipError_t launch_refnn_values(float *vnn, const float *rowv, int NX, int NY, in
 hipError_t launch_nn_expf_bits(uint32_t first, long long n, float *out, hipStream_t s);
 void nn_expf_bits_host(uint32_t first, long long n, float *out);
 hipError_t launch_orders_from_pred(const int8_t *pred, order_t *g, size_t n, hipStream_t s);
+// host-array exchange of a ghosted state [NY+8][NX+8] through one contiguous copy
+hipError_t launch_pack_ghosted(const double4 *U, const double4 *ring, const Bnd &B, double4 *out, hipStream_t s);
+hipError_t launch_unpack_ghosted(const double4 *in, double4 *U, double4 *ring, int NX, int NY, hipStream_t s);
+hipError_t launch_cellgpo_ghosted(const order_t *g, int32_t *out, int NX, int NY, int mord, int all_mord,
+                                  hipStream_t s);
 
 }  // namespace gpm
