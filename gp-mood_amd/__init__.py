@@ -228,6 +228,15 @@ class Solver:
         self._check(self.lib.gpm_load_reference_nn(self._h, str(nn_filename).encode(), int(L), int(length)),
                     "gpm_load_reference_nn")
 
+    def set_reference_dump(self, inputs_ptr, labels_ptr, cells_ptr, capacity, count_ptr, nr_ptr, seed=0, L=52):
+        """Training samples for the reference's trainer after every step's first
+        stage (append_to_NN_datatset; see include/gpmood.h).  Device pointers;
+        inputs_ptr = 0 switches it off."""
+        vp = lambda p: ctypes.c_void_p(p) if p else None  # noqa: E731
+        self._check(self.lib.gpm_set_reference_dump(self._h, vp(inputs_ptr), vp(labels_ptr), vp(cells_ptr),
+                                                    int(capacity), vp(count_ptr), vp(nr_ptr),
+                                                    int(seed) & ((1 << 64) - 1), int(L)), "gpm_set_reference_dump")
+
     def set_niter(self, niter):
         """niter before the next step (main.f90:147 advances it at each step)."""
         self._check(self.lib.gpm_set_niter(self._h, int(niter)), "gpm_set_niter")

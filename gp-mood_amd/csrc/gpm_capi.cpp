@@ -67,6 +67,9 @@ struct gpm_ctx {
     bool priori_set = false;         // the last step ran the network (else CellGPO_priori = Mord)
     float *rowv = nullptr, *vnn = nullptr;  // NN_GP_MOOD_CC: values of each row's last cell; values_NN
     long long niter = 0;             // main.f90's niter: steps started so far
+    // training samples in the reference network's format (not owned, but nr_start)
+    RefDumpArgs rdump{};
+    bool rdump_on = false;
     // host-array exchange (allocated on first use): a ghosted state and a ghosted CellGPO
     double4 *xfer = nullptr;
     int32_t *xfer_gpo = nullptr;
@@ -479,6 +482,12 @@ int enqueue_stages(gpm_ctx *c)
             const size_t n = (size_t)c->NX * c->NY;
             HIPCHK(c, launch_unpack_orders(c->gpo, c->rec_masks + ((size_t)c->rec_count * 3 + k) * n, n, c->stream));
         }
+        if (c->rdump_on && k == 0) {
+            // append_to_NN_datatset after the first stage (mod_FE.f90:176-181)
+            RefDumpArgs d = c->rdump;
+            d.U = in; d.gpo = c->gpo; d.tstate = c->tstate; d.B = c->B; d.step = c->step_counter;
+            HIPCHK(c, launch_refnn_dump(d, c->mord, c->stream));
+        }
         if (c->dump_x && (c->dump_stages & (1 << k))) {
             // (stencil of the stage input, final order of the stage) pairs
             const int S = 2 * ((c->mord - 1) / 2) + 1;
@@ -684,7 +693,7 @@ int gpm_destroy(gpm_ctx *c)
                     c->gpo, c->slistA, c->slistB, c->mark, c->counters,
                     c->stamp, c->err, c->solo_p, c->stats, c->tstate, c->block_min, c->dt_log, c->done, c->pred, c->nnlist,
                     c->nncount, c->adj_ub, c->adj_G, c->adj_A, c->adj_tmp, c->adj_ordx, c->adj_ordy,
-                    c->priori, c->refnn_w, c->rowv, c->vnn, c->xfer, c->xfer_gpo};
+                    c->priori, c->refnn_w, c->rowv, c->vnn, c->xfer, c->xfer_gpo, c->rdump.nr_start};
     for (void *b : bufs)
         if (b) hipFree(b);
     for (hipEvent_t e : c->ev_pool) hipEventDestroy(e);
@@ -1035,6 +1044,26 @@ int gpm_load_reference_nn(gpm_ctx *c, const char *nn_filename, int32_t L, int32_
         return fail(c, GPM_EINVAL, std::string("cannot read ") + nn_filename + ".txt: missing file, a record "
                                    "that is not a number, or fewer records than the network holds");
     return gpm_set_reference_nn(c, L, length, w0.data(), b0.data(), w1.data(), b1.data(), w2.data(), b2.data());
+}
+
+int gpm_set_reference_dump(gpm_ctx *c, float *inputs_dev, float *labels_dev, int64_t *cells_dev, int64_t capacity,
+                           uint64_t *count_dev, int64_t *nr_dev, uint64_t seed, int32_t L)
+{
+    if (!c) return GPM_EINVAL;
+    if (!inputs_dev) {
+        c->rdump_on = false;
+        return GPM_OK;
+    }
+    if (!labels_dev || !count_dev || !nr_dev || capacity < 0) return fail(c, GPM_EINVAL, "null argument");
+    if (c->mord != 3 && c->mord != 5)
+        return fail(c, GPM_EUNSUP, "the reference network's stencil (szs_sphere(Mord+2)) is built for Mord 3 and 5");
+    if (L < 4 * (c->mord == 3 ? 13 : 25)) return fail(c, GPM_EINVAL, "L must hold the 4*szs_sphere(Mord+2) stencil values");
+    if (!c->rdump.nr_start) HIPCHK(c, dalloc(&c->rdump.nr_start, 2));
+    c->rdump.inputs = inputs_dev; c->rdump.labels = labels_dev; c->rdump.cells = reinterpret_cast<long long *>(cells_dev);
+    c->rdump.capacity = capacity; c->rdump.count = reinterpret_cast<unsigned long long *>(count_dev);
+    c->rdump.nr = reinterpret_cast<long long *>(nr_dev); c->rdump.seed = seed; c->rdump.L = L;
+    c->rdump_on = true;
+    return GPM_OK;
 }
 
 int gpm_set_niter(gpm_ctx *c, int64_t niter)

@@ -236,6 +236,25 @@ constexpr int kRefNnMaxLen = 64;     // hidden units held in registers
 constexpr int kRefNnThreads = 64;    // one wavefront per workgroup
 hipError_t launch_refnn(const RefNnArgs &a, hipStream_t s);
 int refnn_input_pad(int L);  // padded input width of the network kernel (64 or 128)
+
+// Training samples in the reference network's format (append_to_NN_datatset).
+struct RefDumpArgs {
+    const double4 *__restrict__ U;  // the first stage's input
+    const order_t *__restrict__ gpo;
+    const double *__restrict__ tstate;
+    Bnd B;
+    float *inputs;                  // [capacity][L]
+    float *labels;                  // [capacity][2]
+    long long *cells;               // [capacity] cell ids, or null
+    long long capacity;
+    unsigned long long *count;
+    long long *nr;                  // NR0, NR1 (running)
+    long long *nr_start;            // NR0, NR1 at the start of the stage
+    unsigned long long seed;
+    long long step;
+    int L;
+};
+hipError_t launch_refnn_dump(const RefDumpArgs &a, int mord, hipStream_t s);
 // values_NN(-1:lf+2, -1:nf+2, 2) := (0, 1), then each row's values written at
 // values_NN(L, n, :) in row order, as the reference indexes it (mod_NN.f90:41,44)
 hipError_t launch_refnn_values(float *vnn, const float *rowv, int NX, int NY, int L, bool scatter, hipStream_t s);

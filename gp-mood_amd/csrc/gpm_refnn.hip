@@ -214,6 +214,84 @@ __global__ __launch_bounds__(kRefNnThreads) void k_refnn(RefNnArgs a)
 
 int refnn_input_pad(int L) { return L <= 64 ? 64 : 128; }
 
+// 64-bit mix of (seed, step, cell) to a uniform double in [0, 1)
+__device__ __forceinline__ double dump_uniform(unsigned long long seed, long long step, long long cell)
+{
+    unsigned long long z = seed ^ (0x9E3779B97F4A7C15ull * (unsigned long long)(step + 1)) ^
+                           (0xBF58476D1CE4E5B9ull * (unsigned long long)(cell + 1));
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    z ^= z >> 31;
+    return (double)(z >> 11) * 0x1.0p-53;
+}
+
+// append_to_NN_datatset (mod_append_NN_dataset.f90:11-79) after the first RK
+// stage: for each interior cell, the stage input's R+1 stencil through
+// format_input (the network's inputs) and the stage's final order as the label,
+// (0, 1) for order 3 and (1, 0) otherwise.  Constant stencils are skipped;
+// order-3 samples are kept with probability freq_R0 = NR0 / (NR0 + NR1) (0.5
+// before any sample), from the counts at the start of this stage.  Kept samples
+// go to row atomicAdd(count) while it is below capacity.
+template <int S>
+__global__ __launch_bounds__(256) void k_refnn_dump(RefDumpArgs a)
+{
+    const int NX = a.B.NX, NY = a.B.NY;
+    const long long cell = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (cell >= (long long)NX * NY) return;
+    const int i = (int)(cell % NX), j = (int)(cell / NX);
+    const double xt = dmr_xmin_top(a.tstate[4], a.B.sqrt3);
+    float x[4 * S];
+    bool cst = true;
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+        float mx = -__builtin_inff(), mn = __builtin_inff();
+        bool any = false;
+#pragma unroll
+        for (int s = 0; s < S; ++s) {
+            const double4 u = bnd_value(a.B, DevSrc{a.U, NX}, i + kSp1X[s], j + kSp1Y[s], xt);
+            const float f = (float)(v == 0 ? u.x : v == 1 ? u.y : v == 2 ? u.z : u.w);
+            x[4 * s + v] = f;
+            if (!__builtin_isnan(f)) {
+                any = true;
+                if (f > mx) mx = f;
+                if (f < mn) mn = f;
+            }
+        }
+        const float d = any ? mx - mn : __builtin_nanf("");
+        if (!((double)d < 1e-10)) cst = false;
+    }
+    if (cst) return;
+    const int ord = order_of_word(a.gpo[cell]);
+    if (ord == 3) {
+        const long long n0 = a.nr_start[0], n1 = a.nr_start[1];
+        const double freq = (n0 + n1) > 0 ? (double)n0 / (double)(n0 + n1) : 0.5;
+        if (dump_uniform(a.seed, a.step, cell) > freq) return;
+    }
+    const unsigned long long row = atomicAdd(a.count, 1ull);
+    if (row >= (unsigned long long)a.capacity) return;
+    float *dst = a.inputs + row * (size_t)a.L;
+    for (int k = 0; k < a.L; ++k) dst[k] = k < 4 * S ? x[k] : -666.0f;
+    a.labels[2 * row] = ord == 3 ? 0.0f : 1.0f;
+    a.labels[2 * row + 1] = ord == 3 ? 1.0f : 0.0f;
+    if (a.cells) a.cells[row] = (long long)cell;
+    atomicAdd(reinterpret_cast<unsigned long long *>(&a.nr[ord == 3 ? 1 : 0]), 1ull);
+}
+
+__global__ void k_refnn_dump_begin(const long long *nr, long long *nr_start)
+{
+    if (threadIdx.x < 2) nr_start[threadIdx.x] = nr[threadIdx.x];
+}
+
+hipError_t launch_refnn_dump(const RefDumpArgs &a, int mord, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_refnn_dump_begin, dim3(1), dim3(64), 0, s, a.nr, a.nr_start);
+    const long long n = (long long)a.B.NX * a.B.NY;
+    const dim3 grid((unsigned)((n + 255) / 256)), block(256);
+    if (mord == 3) hipLaunchKernelGGL(k_refnn_dump<13>, grid, block, 0, s, a);
+    else hipLaunchKernelGGL(k_refnn_dump<25>, grid, block, 0, s, a);
+    return hipGetLastError();
+}
+
 hipError_t launch_refnn(const RefNnArgs &a, hipStream_t s)
 {
     const long long n = (long long)a.B.NX * a.B.NY;

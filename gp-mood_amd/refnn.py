@@ -57,3 +57,28 @@ def seeded_network(L=57, length=60, seed=0, stencil=13, lowered_quantile=0.75, U
             r = sig(sig(x @ net["weight0"].T + net["bias0"]) @ net["weight1"].T + net["bias1"]) @ net["weight2"].T
         net["bias2"] = np.array([-np.quantile(r[:, 0] - r[:, 1], lowered_quantile), 0.0])
     return {k: np.asarray(v, np.float32) for k, v in net.items()}
+
+
+def collect_dataset(solver, nsteps, capacity, L=52, seed=0, tmax=1e10):
+    """Runs `nsteps` steps of `solver` and collects the reference trainer's
+    samples after each step's first stage (append_to_NN_datatset): returns
+    (inputs (size, L), labels (size, 2), cells (size,), NR0, NR1), float32
+    inputs and labels as gpm.h5io.write_nn_dataset writes them."""
+    import torch
+    dev = "cuda"
+    inputs = torch.zeros((capacity, L), dtype=torch.float32, device=dev)
+    labels = torch.zeros((capacity, 2), dtype=torch.float32, device=dev)
+    cells = torch.zeros(capacity, dtype=torch.int64, device=dev)
+    count = torch.zeros(1, dtype=torch.int64, device=dev)
+    nr = torch.zeros(2, dtype=torch.int64, device=dev)
+    torch.cuda.synchronize()
+    solver.set_reference_dump(inputs.data_ptr(), labels.data_ptr(), cells.data_ptr(), capacity, count.data_ptr(),
+                              nr.data_ptr(), seed, L)
+    try:
+        solver.run(nsteps, tmax=tmax)
+    finally:
+        solver.set_reference_dump(0, 0, 0, 0, 0, 0)
+    torch.cuda.synchronize()
+    n = min(int(count.item()), capacity)
+    n0, n1 = (int(v) for v in nr.cpu())
+    return (inputs[:n].cpu().numpy(), labels[:n].cpu().numpy(), cells[:n].cpu().numpy(), n0, n1)

@@ -287,6 +287,23 @@ int gpm_load_reference_nn(gpm_ctx *ctx, const char *nn_filename, int32_t L, int3
  * (where the reference's read stops with an end-of-file error). */
 int gpm_read_reference_nn(const char *nn_filename, int32_t L, int32_t length, float *weight0, float *bias0,
                           float *weight1, float *bias1, float *weight2, float *bias2);
+/* Training samples for the reference's trainer (append_to_NN_datatset,
+ * mod_append_NN_dataset.f90:11-79, called after the first RK stage of every
+ * step, mod_FE.f90:176-181): each interior cell's R+1 stencil of the stage
+ * input through format_input (inputs_dev, float [capacity][L], -666 past the
+ * stencil) and the stage's final order as the label (labels_dev, float
+ * [capacity][2]: (0, 1) for order 3, (1, 0) otherwise).  Constant stencils are
+ * skipped; order-3 samples are kept with probability NR0/(NR0+NR1) (0.5 at
+ * first) from the running counts nr_dev (int64 NR0, NR1) at the start of the
+ * stage, drawn from a counter-based generator of (seed, step, cell): the
+ * reference draws Fortran random_number per sample, so the selection is not
+ * the reference's, the samples' contents are.  Kept samples take row
+ * atomicAdd(*count_dev, 1) while it is below capacity; cells_dev (int64, may
+ * be NULL) receives each row's cell id j*lf + i.  The order of rows is not
+ * defined.  inputs_dev = NULL switches it off.  Write the file with
+ * gpm_io_write_nn_dataset (include/gpmood_io.h). */
+int gpm_set_reference_dump(gpm_ctx *ctx, float *inputs_dev, float *labels_dev, int64_t *cells_dev, int64_t capacity,
+                           uint64_t *count_dev, int64_t *nr_dev, uint64_t seed, int32_t L);
 /* niter before the next step (main.f90:129,147): every step first advances it.
  * The Fortran time_stepping drop-in sets niter - 1, main having advanced it. */
 int gpm_set_niter(gpm_ctx *ctx, int64_t niter);

@@ -1231,6 +1231,33 @@ void orc_set_niter(orc_t *o, int niter) { o->niter = niter; }
 int orc_get_niter(const orc_t *o) { return o->niter; }
 int orc_error(const orc_t *o) { return o->err; }
 
+/* format_input (mod_append_NN_dataset.f90:81-126) of cell (l, n) (1-based)
+   of the full array U: x[L]; returns cst. */
+int orc_nn_format_input(const orc_t *o, const double *U, int l, int n, int L, float *x)
+{
+    const int S = stcl_sz(o->mord + 2);
+    for (int k = 0; k < L; ++k) x[k] = -666.0f;
+    for (int j = 0; j < S; ++j) {
+        const double *u = U + CELL(o, l + SP1_X[j], n + SP1_Y[j]) * 4;
+        for (int v = 0; v < 4; ++v) x[4 * j + v] = (float)u[v];
+    }
+    int cst = 1;
+    for (int v = 0; v < 4; ++v) {
+        float mx = -INFINITY, mn = INFINITY;
+        int any = 0;
+        for (int j = 0; j < S; ++j) {
+            const float z = x[4 * j + v];
+            if (isnan(z)) continue;
+            any = 1;
+            if (z > mx) mx = z;
+            if (z < mn) mn = z;
+        }
+        if (!any) mx = mn = NAN;
+        if (!((double)(mx - mn) < 1e-10)) cst = 0;
+    }
+    return cst;
+}
+
 /* The network alone on one input vector x[L] (forward, mod_NN.f90:165-181). */
 void orc_nn_forward(const orc_t *o, const float *x, float *r) { nn_forward(o, x, r); }
 

@@ -58,6 +58,7 @@ def lib():
         L.orc_nn_forward.argtypes = [P, P, P]
         L.orc_expf_bits.argtypes = [ctypes.c_uint32, ctypes.c_long, P]
         L.orc_pol_table.argtypes = [P, P]
+        L.orc_nn_format_input.argtypes = [P, P, I, I, I, P]
         _lib = L
     return _lib
 
@@ -140,6 +141,12 @@ class Oracle:
         out = np.zeros(40)
         lib().orc_pol_table(self._h, _ptr(out))
         return out
+
+    def nn_format_input(self, U, l, n, L):
+        """format_input of cell (l, n) (1-based) of full array U: (x[L], cst)."""
+        x = np.zeros(L, dtype=np.float32)
+        cst = lib().orc_nn_format_input(self._h, _ptr(np.ascontiguousarray(U)), int(l), int(n), int(L), _ptr(x))
+        return x, bool(cst)
 
     def nn_forward(self, x):
         """forward (mod_NN.f90:165-181) of one input vector x[L]."""

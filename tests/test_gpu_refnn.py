@@ -122,3 +122,47 @@ def test_network_expf_every_float(gpm, orc):
         s.nn_expf_bits(int(u), 1, buf.data_ptr())
         mine, ref = buf[:1].cpu().numpy(), orc.expf_bits(int(u), 1)
         assert (np.isnan(mine[0]) and np.isnan(ref[0])) or mine.view(np.uint32)[0] == ref.view(np.uint32)[0]
+
+
+def test_reference_dataset_samples(gpm, orc, tmp_path):
+    """The reference trainer's samples after the first RK stage
+    (append_to_NN_datatset): each kept row holds format_input of its cell's
+    R+1 stencil of the stage input, bit for bit as the oracle forms it, and the
+    stage's final order as the label; every order-1 cell with a non-constant
+    stencil is kept; NR0 and NR1 count the kept labels; the rows write as the
+    reference's dataset file."""
+    from conftest import golden
+    z = golden("step_r1_n32_neu_fq5.npz")
+    n, L = int(z["n"]), 57
+    print(f"[case reference dataset: {n}^2 R1, one step]", flush=True)
+    cfg = gpm.Config(lf=n, Mord=3, ngp=2, BC_type=str(z["bc"]))
+    s = gpm.Solver(cfg, tables_from_npz(z))
+    s.set_state(z["U0"])
+    s.set_time(0.0)
+    X, Y, cells, n0, n1 = gpm.refnn.collect_dataset(s, 1, n * n, L=L, seed=7)
+    o = case_options(z)
+    dx, dy = (float(v) for v in z["dxdy"])
+    oc = orc.Oracle(n, 3, 2, str(z["bc"]), tables_from_npz(z), flux=o["flux"], dx=dx, dy=dy)
+    U = oc.full(z["U0"])
+    _, masks, _ = oc.step_stages(U.copy(), float(z["dt"][0]), 0.0)
+    m0 = masks[0]
+    assert np.array_equal(m0, z["stage_masks"][0])
+    assert len(cells) == n0 + n1 and len(set(cells.tolist())) == len(cells)
+    for row, cell in enumerate(cells):
+        j, i = divmod(int(cell), n)
+        x, cst = oc.nn_format_input(U, i + 1, j + 1, L)
+        assert not cst
+        assert np.array_equal(X[row].view(np.uint32), x.view(np.uint32)), f"cell {cell}"
+        exp = (0.0, 1.0) if m0[j, i] == 3 else (1.0, 0.0)
+        assert tuple(Y[row]) == exp
+    # every order-1 cell whose stencil is not constant is kept
+    low = {j * n + i for j, i in zip(*np.nonzero(m0 == 1)) if not oc.nn_format_input(U, i + 1, j + 1, L)[1]}
+    assert low and low <= set(cells.tolist())
+    assert n0 == len(low) and n1 == int((Y[:, 1] == 1.0).sum())
+    try:
+        gpm.h5io.load()
+    except RuntimeError:
+        return
+    m = gpm.h5io.meta(n, n, "2DRP3", gpm.capi.GP_MOOD)
+    gpm.h5io.write_nn_dataset(str(tmp_path / "ds.h5"), m, X, Y, nr0=n0, nr1=n1)
+    assert os.path.getsize(str(tmp_path / "ds.h5")) > X.nbytes
