@@ -626,8 +626,10 @@ int gpm_create(const gpm_config *cfg, gpm_ctx **out)
         gpm_destroy(c);
         return GPM_ENOMEM;
     }
-    c->tiles_x = (c->NX + kTX - 1) / kTX;
-    c->ntiles = c->tiles_x * ((c->NY + kTY - 1) / kTY);
+    // the tiles of the stage kernel that runs (FOG: the Mord-1 kernel)
+    const int tx = stage_tx(c->fog ? 1 : c->mord), ty = stage_ty(c->fog ? 1 : c->mord);
+    c->tiles_x = (c->NX + tx - 1) / tx;
+    c->ntiles = c->tiles_x * ((c->NY + ty - 1) / ty);
     // Cooperative grid of the MOOD kernel: all its workgroups must be resident.
     // The correction work is sparse and the per-pass grid barrier gets cheaper
     // with fewer workgroups: by default one per four CUs.  GPM_MOOD_BLOCKS overrides the count.

@@ -237,8 +237,9 @@ __device__ __forceinline__ void mark_neighbourhood(int i, int j, int NX, int NY,
 }
 
 template <int MORD, int NGP, int FLUX, bool GEN, int CMB>
-__global__ __launch_bounds__(kStageThreads, GPM_STAGE_WAVES_PER_EU) void k_stage(StageArgs a)
+__global__ __launch_bounds__(kStageThreads, stage_waves(MORD)) void k_stage(StageArgs a)
 {
+    constexpr int kTX = stage_tx(MORD), kTY = stage_ty(MORD);
     constexpr int H = kHalo<MORD>;
     constexpr int PW = kTX + 2 * H, PH = kTY + 2 * H;
     __shared__ double4 sU[PW * PH];
