@@ -106,6 +106,10 @@ CONFIGS = {
     "r1_dmr_64x16_rk2": (64, 3, 2, "dmr", True, True, True),
     "r1_m800_32_fe": (32, 3, 2, "mach800", True, True, True, "hll"),
     "r1_rt_16x48_rk2": (16, 3, 2, "rt", True, True, True, "unsplit"),
+    # non-square cells (dx /= dy): the GP tables (ell from min(dx, dy),
+    # GP_init.f90), Setdt's two directions and dt/dx, dt/dy of the update
+    "r2_32x48_neu_rect": (32, 5, 3, "neumann", True, True, True),
+    "r3_n32_ref_rect": (32, 7, 4, "reflective", True, True, True),
     # first-order Godunov (method = FOG, ngp = 1, FE) as the problems/*_FOG.f90 files
     "fog_n64_neu_uns": (64, 3, 1, "neumann", True, True, True, "unsplit"),
     "fog_n64_kh_uns": (64, 3, 1, "kh", True, True, True, "unsplit"),
@@ -186,6 +190,8 @@ EXTRAS = {
     "r1_dmr_64x16_rk2": dict(nf=16, lx="4.", ly="1.", integrator="ssp_rk2"),
     "r1_m800_32_fe": dict(lx="1.5", ly="1.5", integrator="fe"),
     "r1_rt_16x48_rk2": dict(nf=48, lx="0.5", ly="1.5", problem="RT", integrator="ssp_rk2"),
+    "r2_32x48_neu_rect": dict(nf=48, lx="1.", ly="1."),
+    "r3_n32_ref_rect": dict(lx="2.", ly="1."),
     "fog_n64_neu_uns": dict(integrator="fe", method="FOG"),
     "fog_n64_kh_uns": dict(integrator="fe", method="FOG"),
     "fog_m800_32_hll": dict(lx="1.5", ly="1.5", integrator="fe", method="FOG"),
@@ -230,7 +236,7 @@ def config(name):
 # time_stepper over libgpmood) is also built
 DROPIN_CONFIGS = ["r2_n64_neu", "r3_n64_ref", "r3_n64_kh_uns", "r1_dmr_64x16", "r1_rt_16x48", "r1_m800_32_fe",
                   "r2_n32_ref_rk4", "fog_n64_kh_uns", "fog_m800_32_hll", "nn_r1_n64_neu", "nncc_r1_n64_neu",
-                  "pol_r1_n64_neu"]
+                  "pol_r1_n64_neu", "r3_n32_ref_rect"]
 
 PARAMS_TEMPLATE = """module parameters
    use constants

@@ -162,6 +162,9 @@ STEP_CASES = {
     "r1_rt_16x48_q35": ("r1_rt_16x48", "quadrants:0", 35, 20),
     "r2_rt_16x48_q36": ("r2_rt_16x48", "quadrants:0", 36, 20),
     "r1_rt_16x48_rk2_q37": ("r1_rt_16x48_rk2", "quadrants:0", 37, 20),
+    # non-square cells: dx = 1/32, dy = 1/48 and dx = 1/16, dy = 1/32
+    "r2_32x48_neu_rect_q65": ("r2_32x48_neu_rect", "quadrants:0.02", 65, 20),
+    "r3_n32_ref_rect_q66": ("r3_n32_ref_rect", "quadrants:0.02", 66, 15),
     # first-order Godunov (FOG, FE)
     "fog_n64_neu_uns_fq38": ("fog_n64_neu_uns", "four_quadrant", 38, 30),
     "fog_n64_kh_uns_sh39": ("fog_n64_kh_uns", "shear_layer", 39, 30),
