@@ -95,7 +95,8 @@ class Oracle:
         lib().orc_set_method(self._h, METHODS[method])
 
     def __del__(self):
-        if getattr(self, "_h", None):
+        # at interpreter exit the module's globals may already be gone
+        if getattr(self, "_h", None) and callable(globals().get("lib")):
             lib().orc_destroy(self._h)
             self._h = None
 

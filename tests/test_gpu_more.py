@@ -54,6 +54,32 @@ def test_partial_tiles_vs_oracle(gpm, n, mord, ngp, bc, kind):
     assert_same(*both(gpm, n, mord, ngp, bc, gpm.ic.make(kind, n, n), 12))
 
 
+@pytest.mark.parametrize("lf,nf,mord,ngp,bc,flux", [
+    (8, 8, 7, 4, "reflective", "hllc"), (8, 8, 3, 2, "periodic", "hll"), (9, 8, 5, 3, "neumann", "hllc"),
+    (8, 21, 3, 2, "kh", "unsplit"), (23, 8, 7, 4, "periodic", "llf"), (17, 13, 5, 3, "reflective", "hllc")])
+def test_smallest_and_ragged_grids_vs_oracle(gpm, lf, nf, mord, ngp, bc, flux):
+    """Grids of 8 cells a side (the smallest the library accepts) and ragged
+    shapes below one stage tile, where the ghost rule, the halo and the MOOD
+    neighbourhoods reach across the whole grid."""
+    print(f"[case {lf}x{nf} R{(mord - 1) // 2} {bc} {flux}]", flush=True)
+    O = load_oracle()
+    lx, ly = 1.0, nf / lf
+    t = gpm.gp_tables(mord, ngp, lf, nf, lx=lx, ly=ly)
+    U0 = gpm.ic.problem("quadrants", lf, nf, lx, ly, seed=lf * nf, noise=0.05)
+    cfg = gpm.Config(lf=lf, nf=nf, Lx=lx, Ly=ly, Mord=mord, ngp=ngp, BC_type=bc, numFlux=flux)
+    s = gpm.Solver(cfg, t)
+    s.set_state(U0)
+    dts = s.run(15, return_dts=True)
+    U, g = s.get_state(order=True)
+    o = O.Oracle(lf, mord, ngp, bc, t, flux=flux, nf=nf, dx=lx / lf, dy=ly / nf)
+    Uo = o.full(U0)
+    r = o.run(Uo, 15, masks=True)
+    assert np.array_equal(dts, r["dt"]), "dt sequence"
+    assert np.array_equal(g[4:-4, 4:-4], r["masks"][-1])
+    assert np.array_equal(U.view(np.uint64), Uo.view(np.uint64)), f"max diff {np.abs(U - Uo).max()}"
+    assert s.stats()["detected_total"] == int(r["counts"].sum())
+
+
 @pytest.mark.parametrize("mord,ngp", [(7, 4), (5, 3)])
 def test_full_cascade_vs_oracle(gpm, mord, ngp):
     print(f"[case full cascade Mord {mord}]", flush=True)
