@@ -102,17 +102,22 @@ __constant__ int kSp1Y[25] = {0, 0, 1, 0, -1, -1, 1, 1, -1, -2, 0, 2, 0, 0, 1, 2
 // registers; the weight rows stream through the scalar cache; the first hidden
 // layer goes through LDS once to be held in registers for layer 2; layer 3 is
 // summed as layer 2 produces its outputs, in the same order.
-template <int S, int KP>
+// LX, LEN > 0: a build for one network shape (L = LX, length = LEN), whose
+// loops stop at the real inputs and hidden units and skip the padding.
+template <int S, int KP, int LX, int LEN>
 __global__ __launch_bounds__(kRefNnThreads) void k_refnn(RefNnArgs a)
 {
     constexpr int HP = kRefNnMaxLen;
+    static_assert(LX <= KP && LEN <= HP, "shape within the padded widths");
+    constexpr int KU = LX > 0 ? LX : KP;    // inputs summed per hidden unit
+    constexpr int HU = LEN > 0 ? LEN : HP;  // hidden units evaluated
     __shared__ float hs[HP * kRefNnThreads];
     const int NX = a.B.NX, NY = a.B.NY;
     const int lane = threadIdx.x;
     const long long cell = (long long)blockIdx.x * kRefNnThreads + lane;
     const bool valid = cell < (long long)NX * NY;
     const int i = valid ? (int)(cell % NX) : 0, j = valid ? (int)(cell / NX) : 0;
-    const int L = a.L, len = a.len;
+    const int L = LX > 0 ? LX : a.L, len = LEN > 0 ? LEN : a.len;
     // format_input: stencil values, -666 up to L, then the padding's zeros;
     // per-variable max/min skipping NaN
     float x[KP];
@@ -150,26 +155,26 @@ __global__ __launch_bounds__(kRefNnThreads) void k_refnn(RefNnArgs a)
     const float *__restrict__ w2 = b1 + len;                  // weight2(2, length), Fortran layout
     const float *__restrict__ b2 = w2 + 2 * len;
     // every lane runs the network (cst lanes discard it): uniform loops
-    for (int q = 0; q < HP; ++q) {
+    for (int q = 0; q < HU; ++q) {
         float h = 0.0f;
-        if (q < len) {
+        if (LEN > 0 || q < len) {
             const float *__restrict__ w = w0 + (size_t)q * KP;
             float acc = 0.0f;
 #pragma unroll
-            for (int k = 0; k < KP; ++k) acc = acc + w[k] * x[k];
+            for (int k = 0; k < KU; ++k) acc = acc + w[k] * x[k];
             h = nn_sigmoid(acc + b0[q]);
         }
         hs[q * kRefNnThreads + lane] = h;
     }
-    float h0[HP];
+    float h0[HU];
 #pragma unroll
-    for (int k = 0; k < HP; ++k) h0[k] = hs[k * kRefNnThreads + lane];
+    for (int k = 0; k < HU; ++k) h0[k] = hs[k * kRefNnThreads + lane];
     float r0 = 0.0f, r1 = 0.0f;
     for (int q = 0; q < len; ++q) {
         const float *__restrict__ w = w1 + (size_t)q * HP;
         float acc = 0.0f;
 #pragma unroll
-        for (int k = 0; k < HP; ++k) acc = acc + w[k] * h0[k];
+        for (int k = 0; k < HU; ++k) acc = acc + w[k] * h0[k];
         const float h = nn_sigmoid(acc + b1[q]);
         r0 = r0 + w2[2 * q] * h;
         r1 = r1 + w2[2 * q + 1] * h;
@@ -297,9 +302,12 @@ hipError_t launch_refnn(const RefNnArgs &a, hipStream_t s)
     const long long n = (long long)a.B.NX * a.B.NY;
     const dim3 grid((unsigned)((n + kRefNnThreads - 1) / kRefNnThreads)), block(kRefNnThreads);
     const int kp = refnn_input_pad(a.L);
-    if (a.mord == 3 && kp == 64) hipLaunchKernelGGL((k_refnn<13, 64>), grid, block, 0, s, a);
-    else if (a.mord == 3) hipLaunchKernelGGL((k_refnn<13, 128>), grid, block, 0, s, a);
-    else hipLaunchKernelGGL((k_refnn<25, 128>), grid, block, 0, s, a);
+    // the problem files' shapes (L 57, length 60; L 52 is the bare stencil) get builds of their own
+    if (a.mord == 3 && a.L == 57 && a.len == 60) hipLaunchKernelGGL((k_refnn<13, 64, 57, 60>), grid, block, 0, s, a);
+    else if (a.mord == 3 && a.L == 52 && a.len == 60) hipLaunchKernelGGL((k_refnn<13, 64, 52, 60>), grid, block, 0, s, a);
+    else if (a.mord == 3 && kp == 64) hipLaunchKernelGGL((k_refnn<13, 64, 0, 0>), grid, block, 0, s, a);
+    else if (a.mord == 3) hipLaunchKernelGGL((k_refnn<13, 128, 0, 0>), grid, block, 0, s, a);
+    else hipLaunchKernelGGL((k_refnn<25, 128, 0, 0>), grid, block, 0, s, a);
     return hipGetLastError();
 }
 
