@@ -40,7 +40,7 @@ struct gpm_ctx {
     DevTables *dT = nullptr;
     double4 *dU = nullptr, *dA = nullptr, *dB = nullptr;
     double4 *dC = nullptr, *dD = nullptr, *dE = nullptr, *dF = nullptr;  // SSP_RK4 stage states
-    double4 *ring = nullptr, *band = nullptr;
+    double4 *ring = nullptr, *band = nullptr, *band3 = nullptr;  // band3: SSP_RK4's fourth stage
     StepExtra *ext = nullptr;  // [kMaxStages] in device memory (plan_stages' per-stage extras)
     order_t *gpo = nullptr;
     int *slistA = nullptr, *slistB = nullptr, *mark = nullptr;
@@ -333,6 +333,10 @@ int plan_stages(gpm_ctx *c, StagePlan *P, StepExtra *extra = nullptr)
     if (c->ring_mode && !c->nodetect) {
         ex[n - 1].band = c->band;
         P[n - 1].C.band = 1;
+        if (c->integrator == GPM_SSP_RK4) {  // FU3's ghosts come from the fourth stage
+            ex[n - 2].band = c->band3;
+            P[n - 2].C.band = 1;
+        }
     }
     if (extra)
         for (int k = 0; k < kMaxStages; ++k) extra[k] = ex[k];
@@ -363,7 +367,10 @@ int enqueue_ring(gpm_ctx *c)
     RingArgs r{};
     r.A = c->dU; r.band = c->band; r.ring = c->ring; r.tstate = c->tstate; r.B = c->B; r.gravity = c->gravity;
     r.T = c->dT; r.flux = c->cfg.num_flux;
-    const int integ = c->integrator == GPM_FE ? kRingFE : c->integrator == GPM_SSP_RK2 ? kRingRK2 : kRingRK3;
+    const int integ = c->integrator == GPM_FE        ? kRingFE
+                      : c->integrator == GPM_SSP_RK2 ? kRingRK2
+                      : c->integrator == GPM_SSP_RK4 ? kRingRK4
+                                                     : kRingRK3;
     if (c->nodetect) {
         // the last stage's input: Un (FE), U1 (SSP_RK2) or U2 (SSP_RK3), still in its buffer
         StagePlan plan[kMaxStages];
@@ -375,6 +382,9 @@ int enqueue_ring(gpm_ctx *c)
         return GPM_OK;
     }
     r.mode = integ;
+    if (integ == kRingRK4) {  // U2, U3, U4 still in their buffers (plan_stages)
+        r.X2 = c->dC; r.X3 = c->dD; r.X4 = c->dE; r.band3 = c->band3;
+    }
     HIPCHK(c, launch_ring(r, c->stream));
     return GPM_OK;
 }
@@ -573,8 +583,8 @@ int gpm_create(const gpm_config *cfg, gpm_ctx **out)
     // FOG leaves the face loop's updates in the first ghost layer (no Boundary_C
     // after the step), so its ghost cells are never copies of interior cells
     c->ring_mode = !index_map_bc(cfg->bc_type) || c->gravity || c->nodetect;
-    if (c->ring_mode && cfg->integrator == GPM_SSP_RK4)
-        return bad(GPM_EUNSUP, "SSP_RK4 needs a copy boundary (Neumann, Periodic, Reflective, KH_BC) and no source term");
+    if (c->nodetect && cfg->integrator == GPM_SSP_RK4)
+        return bad(GPM_EUNSUP, "SSP_RK4 needs a method with DETECTION (not FOG, GP_SE or Unlim_POL)");
     // the network reads the stage input's ghost cells as copies of interior cells
     if (nn_method && (!index_map_bc(cfg->bc_type) || c->gravity))
         return bad(GPM_EUNSUP, "NN_GP_MOOD needs a copy boundary (Neumann, Periodic, Reflective, KH_BC) and no source term");
@@ -611,6 +621,7 @@ int gpm_create(const gpm_config *cfg, gpm_ctx **out)
              dalloc(&c->dF, n) == hipSuccess;
     if (ok && c->ring_mode)
         ok = dalloc(&c->ring, (size_t)ring_size(c->NX, c->NY)) == hipSuccess && dalloc(&c->band, n) == hipSuccess;
+    if (ok && c->ring_mode && c->integrator == GPM_SSP_RK4) ok = dalloc(&c->band3, n) == hipSuccess;
     if (ok) ok = dalloc(&c->st_dev, kBndStates) == hipSuccess && dalloc(&c->ext, kMaxStages) == hipSuccess;
     if (!ok) {
         gpm_destroy(c);
@@ -691,7 +702,7 @@ int gpm_destroy(gpm_ctx *c)
 {
     if (!c) return GPM_OK;
     if (c->stream) hipStreamSynchronize(c->stream);
-    void *bufs[] = {c->dT, c->dU, c->dA, c->dB, c->dC, c->dD, c->dE, c->dF, c->ring, c->band, c->st_dev, c->ext,
+    void *bufs[] = {c->dT, c->dU, c->dA, c->dB, c->dC, c->dD, c->dE, c->dF, c->ring, c->band, c->band3, c->st_dev, c->ext,
                     c->gpo, c->slistA, c->slistB, c->mark, c->counters,
                     c->stamp, c->err, c->solo_p, c->stats, c->tstate, c->block_min, c->dt_log, c->done, c->pred, c->nnlist,
                     c->nncount, c->adj_ub, c->adj_G, c->adj_A, c->adj_tmp, c->adj_ordx, c->adj_ordy,

@@ -49,6 +49,59 @@ struct DevTables {
     double gw[5];           // gauss_weight(ngp, 1:ngp)
 };
 
+// a / b correctly rounded (IEEE round to nearest even).  The compiler's FP64
+// division (v_div_scale, v_rcp, two Newton steps, v_div_fmas, v_div_fixup) is
+// faithful but not always correctly rounded: -0x1.6666666666663p+0 /
+// -0x1.ffffffffffffbp-1 gives 0x1.6666666666666p+0 instead of
+// 0x1.6666666666667p+0 (the exact quotient lies 0.49999999999999994 ulp from
+// the latter).  Here a third Newton step makes the reciprocal y of the scaled
+// divisor its correctly rounded value (Markstein: a faithful y and e = 1 - d y
+// computed exactly give RN(1/d) in y + e y, unless d's significand is all
+// ones), so the final fused step q + (n - d q) y is the correctly rounded
+// quotient (Markstein: y = RN(1/d) and a faithful q).  The scale and fix-up
+// instructions keep the compiler sequence's handling of extreme exponents,
+// zeros, infinities and NaN.  A divisor with an all-ones significand takes an
+// exact comparison of the remainders of q and its neighbour instead.
+__device__ __forceinline__ double ddiv(double a, double b)
+{
+    bool flag_d, flag_n;
+    const double d = __builtin_amdgcn_div_scale(a, b, false, &flag_d);  // scaled divisor
+    const double n = __builtin_amdgcn_div_scale(a, b, true, &flag_n);   // scaled dividend
+    double y = __builtin_amdgcn_rcp(d);
+#pragma unroll
+    for (int it = 0; it < 3; ++it) y = __builtin_fma(y, __builtin_fma(-d, y, 1.0), y);
+    const double m = n * y;
+    const double r = __builtin_fma(-d, m, n);
+    double q = __builtin_amdgcn_div_fixup(__builtin_amdgcn_div_fmas(r, y, m, flag_n), b, a);
+    const unsigned long long bb = (unsigned long long)__double_as_longlong(b);
+    if ((bb & 0xFFFFFFFFFFFFFull) == 0xFFFFFFFFFFFFFull) {
+        const double aq = fabs(q), ab = fabs(b);
+        if (aq >= 0x1p-900 && aq <= 0x1p900 && ab >= 0x1p-900 && ab <= 0x1p900) {
+            const double e = __builtin_fma(-q, b, a);  // exact remainder of the faithful q
+            if (e != 0.0) {
+                // neighbour of q towards the exact quotient q + e / b
+                const bool up = (e > 0.0) == (b > 0.0);
+                const long long qb = __double_as_longlong(q);
+                const double q2 = __longlong_as_double(qb + (((q > 0.0) == up) ? 1 : -1));
+                const double e2 = __builtin_fma(-q2, b, a);
+                if (fabs(e2) < fabs(e) || (fabs(e2) == fabs(e) && (__double_as_longlong(q2) & 1) == 0)) q = q2;
+            }
+        }
+    }
+    return q;
+}
+
+// The reference's FP64 a / b in code compiled for both sides: ddiv on the
+// device, the host's IEEE division on the host.
+__host__ __device__ __forceinline__ double dv(double a, double b)
+{
+#ifdef __HIP_DEVICE_COMPILE__
+    return ddiv(a, b);
+#else
+    return a / b;
+#endif
+}
+
 // Fortran MIN/MAX of the reference build: the second operand unless the first
 // compares strictly smaller (larger).
 __device__ __forceinline__ double fmin2(double a, double b) { return (a < b) ? a : b; }
@@ -57,7 +110,7 @@ __device__ __forceinline__ double fmax2(double a, double b) { return (a > b) ? a
 // pressure, physics.f90:132-143
 __host__ __device__ __forceinline__ double pressure(double r, double mx, double my, double E)
 {
-    return (kGamma - 1.0) * (E - (0.5 * (mx * mx + my * my)) / r);
+    return (kGamma - 1.0) * (E - dv(0.5 * (mx * mx + my * my), r));
 }
 
 // Conserved state of a cell as read from memory (rho, rho*u, rho*v, E).
@@ -113,7 +166,7 @@ __device__ __forceinline__ double4 load_bc(const double4 *__restrict__ U, int N,
 // primitive_to_conservative, physics.f90:146-160
 __host__ __device__ __forceinline__ double4 p2c(double r, double vx, double vy, double p)
 {
-    return make_double4(r, r * vx, r * vy, (p / (kGamma - 1.)) + 0.5 * r * (vx * vx + vy * vy));
+    return make_double4(r, r * vx, r * vy, dv(p, kGamma - 1.) + 0.5 * r * (vx * vx + vy * vy));
 }
 
 // Cell centre of 0-based cell i: mesh_x(l) = l*dx - dx/2 with l = i+1 (mesh_init.f90:14-20).
@@ -145,7 +198,7 @@ __host__ __device__ __forceinline__ bool index_map_bc(int bc)
 // pre-shock state at time t_rk (BC.f90:301).
 __host__ __device__ __forceinline__ double dmr_xmin_top(double t_rk, double sqrt3)
 {
-    return 1. / 6 + 10. * t_rk / (0.5 * sqrt3) + 1. / sqrt3;
+    return 1. / 6 + dv(10. * t_rk, 0.5 * sqrt3) + dv(1., sqrt3);
 }
 
 // Ghost column i (i < 0 or i >= NX) of interior row j: the x part of
@@ -222,7 +275,7 @@ __host__ __device__ __forceinline__ double4 bnd_value(const Bnd &B, const Src &s
     case kRT: {  // hydrostatic reflection in primitive variables
         const int k = g - 1;
         const double4 u = col(top ? NY - 1 - k : k);
-        const double vx = u.y / u.x, vy = u.z / u.x, p = pressure(u.x, u.y, u.z, u.w);
+        const double vx = dv(u.y, u.x), vy = dv(u.z, u.x), p = pressure(u.x, u.y, u.z, u.w);
         const double h = (double)(2 * k) * B.dy * kGravity * u.x;
         return p2c(u.x, vx, -vy, top ? p - h : p + h);
     }
@@ -253,12 +306,12 @@ __device__ __forceinline__ double4 load_bnd(const double4 *__restrict__ U, const
 template <int DIR>
 __device__ __forceinline__ bool hllc_adm(const double ul[4], const double ur[4], double F[4])
 {
-    const double vxl = ul[1] / ul[0], vyl = ul[2] / ul[0];
+    const double vxl = dv(ul[1], ul[0]), vyl = dv(ul[2], ul[0]);
     const double pl = pressure(ul[0], ul[1], ul[2], ul[3]);
-    const double cl = sqrt(kGamma * pl / ul[0]);
-    const double vxr = ur[1] / ur[0], vyr = ur[2] / ur[0];
+    const double cl = sqrt(dv(kGamma * pl, ul[0]));
+    const double vxr = dv(ur[1], ur[0]), vyr = dv(ur[2], ur[0]);
     const double pr = pressure(ur[0], ur[1], ur[2], ur[3]);
-    const double cr = sqrt(kGamma * pr / ur[0]);
+    const double cr = sqrt(dv(kGamma * pr, ur[0]));
     const bool adm = !(ul[0] <= 0. || pl <= 0. || isnan(ul[0]) || isnan(pl) ||
                        ur[0] <= 0. || pr <= 0. || isnan(ur[0]) || isnan(pr));
     if (!adm) return false;
@@ -278,21 +331,21 @@ __device__ __forceinline__ bool hllc_adm(const double ul[4], const double ur[4],
     const double numerL = SL - vnl, numerR = SR - vnr;
     double uStar = ul[0] * vnl * numerL - ur[0] * vnr * numerR;
     uStar = (pr - pl) + uStar;
-    uStar = uStar / (ul[0] * numerL - ur[0] * numerR);
+    uStar = dv(uStar, ul[0] * numerL - ur[0] * numerR);
     if (SL >= 0.) {
         F[0] = Fl[0]; F[1] = Fl[1]; F[2] = Fl[2]; F[3] = Fl[3];
     } else if (SL < 0. && uStar >= 0.) {
-        const double dStarL = ul[0] * numerL / (SL - uStar);
+        const double dStarL = dv(ul[0] * numerL, SL - uStar);
         double Us[4];
         Us[0] = 1.0; Us[md] = uStar; Us[om] = vtl;
-        Us[3] = ul[3] / ul[0] + (uStar - vnl) * (uStar + pl / (ul[0] * numerL));
+        Us[3] = dv(ul[3], ul[0]) + (uStar - vnl) * (uStar + dv(pl, ul[0] * numerL));
 #pragma unroll
         for (int k = 0; k < 4; ++k) F[k] = Fl[k] + SL * (Us[k] * dStarL - ul[k]);
     } else if (uStar < 0. && SR >= 0.) {
-        const double dStarR = ur[0] * numerR / (SR - uStar);
+        const double dStarR = dv(ur[0] * numerR, SR - uStar);
         double Us[4];
         Us[0] = 1.0; Us[md] = uStar; Us[om] = vtr;
-        Us[3] = ur[3] / ur[0] + (uStar - vnr) * (uStar + pr / (ur[0] * numerR));
+        Us[3] = dv(ur[3], ur[0]) + (uStar - vnr) * (uStar + dv(pr, ur[0] * numerR));
 #pragma unroll
         for (int k = 0; k < 4; ++k) F[k] = Fr[k] + SR * (Us[k] * dStarR - ur[k]);
     } else {
@@ -305,9 +358,9 @@ __device__ __forceinline__ bool hllc_adm(const double ul[4], const double ur[4],
 // x-velocity of the right state's c2p (+Inf for finite vx, NaN otherwise).
 __device__ __forceinline__ void hllc_inadmissible(const double ur[4], double F[4])
 {
-    const double vx = ur[1] / ur[0];
+    const double vx = dv(ur[1], ur[0]);
     const double z = vx - vx;
-    const double f = 1.0 / z;
+    const double f = dv(1.0, z);
     F[0] = f; F[1] = f; F[2] = f; F[3] = f;
 }
 
@@ -322,12 +375,12 @@ __device__ __forceinline__ void hllc(const double ul[4], const double ur[4], dou
 template <int DIR>
 __device__ __forceinline__ bool hll_adm(const double ul[4], const double ur[4], double F[4])
 {
-    const double vxl = ul[1] / ul[0], vyl = ul[2] / ul[0];
+    const double vxl = dv(ul[1], ul[0]), vyl = dv(ul[2], ul[0]);
     const double pl = pressure(ul[0], ul[1], ul[2], ul[3]);
-    const double cl = sqrt(kGamma * pl / ul[0]);
-    const double vxr = ur[1] / ur[0], vyr = ur[2] / ur[0];
+    const double cl = sqrt(dv(kGamma * pl, ul[0]));
+    const double vxr = dv(ur[1], ur[0]), vyr = dv(ur[2], ur[0]);
     const double pr = pressure(ur[0], ur[1], ur[2], ur[3]);
-    const double cr = sqrt(kGamma * pr / ur[0]);
+    const double cr = sqrt(dv(kGamma * pr, ur[0]));
     const bool adm = !(ul[0] <= 0. || pl <= 0. || isnan(ul[0]) || isnan(pl) ||
                        ur[0] <= 0. || pr <= 0. || isnan(ur[0]) || isnan(pr));
     if (!adm) return false;
@@ -350,7 +403,7 @@ __device__ __forceinline__ bool hll_adm(const double ul[4], const double ur[4], 
         for (int k = 0; k < 4; ++k) F[k] = Fr[k];
     } else {
 #pragma unroll
-        for (int k = 0; k < 4; ++k) F[k] = (SR * Fl[k] - SL * Fr[k] + SR * SL * (ur[k] - ul[k])) / (SR - SL);
+        for (int k = 0; k < 4; ++k) F[k] = dv(SR * Fl[k] - SL * Fr[k] + SR * SL * (ur[k] - ul[k]), SR - SL);
     }
     return true;
 }
@@ -360,12 +413,12 @@ __device__ __forceinline__ bool hll_adm(const double ul[4], const double ur[4], 
 template <int DIR>
 __device__ __forceinline__ bool llf_adm(const double ul[4], const double ur[4], double F[4])
 {
-    const double vxl = ul[1] / ul[0], vyl = ul[2] / ul[0];
+    const double vxl = dv(ul[1], ul[0]), vyl = dv(ul[2], ul[0]);
     const double pl = pressure(ul[0], ul[1], ul[2], ul[3]);
-    const double cl = sqrt(kGamma * pl / ul[0]);
-    const double vxr = ur[1] / ur[0], vyr = ur[2] / ur[0];
+    const double cl = sqrt(dv(kGamma * pl, ul[0]));
+    const double vxr = dv(ur[1], ur[0]), vyr = dv(ur[2], ur[0]);
     const double pr = pressure(ur[0], ur[1], ur[2], ur[3]);
-    const double cr = sqrt(kGamma * pr / ur[0]);
+    const double cr = sqrt(dv(kGamma * pr, ur[0]));
     const bool adm = !(ul[0] <= 0. || pl <= 0. || isnan(ul[0]) || isnan(pl) ||
                        ur[0] <= 0. || pr <= 0. || isnan(ur[0]) || isnan(pr));
     if (!adm) return false;
@@ -390,12 +443,12 @@ __device__ __forceinline__ bool llf_adm(const double ul[4], const double ur[4], 
 template <int DIR>
 __device__ __forceinline__ bool unsplit_adm(const double ul[4], const double ur[4], double F[4])
 {
-    const double vxl = ul[1] / ul[0], vyl = ul[2] / ul[0];
+    const double vxl = dv(ul[1], ul[0]), vyl = dv(ul[2], ul[0]);
     const double pl = pressure(ul[0], ul[1], ul[2], ul[3]);
-    const double cl = sqrt(kGamma * pl / ul[0]);
-    const double vxr = ur[1] / ur[0], vyr = ur[2] / ur[0];
+    const double cl = sqrt(dv(kGamma * pl, ul[0]));
+    const double vxr = dv(ur[1], ur[0]), vyr = dv(ur[2], ur[0]);
     const double pr = pressure(ur[0], ur[1], ur[2], ur[3]);
-    const double cr = sqrt(kGamma * pr / ur[0]);
+    const double cr = sqrt(dv(kGamma * pr, ur[0]));
     const bool adm = !(ul[0] <= 0. || pl <= 0. || isnan(ul[0]) || isnan(pl) ||
                        ur[0] <= 0. || pr <= 0. || isnan(ur[0]) || isnan(pr));
     if (!adm) return false;
@@ -403,7 +456,7 @@ __device__ __forceinline__ bool unsplit_adm(const double ul[4], const double ur[
     const double vnl = DIR == 0 ? vxl : vyl, vnr = DIR == 0 ? vxr : vyr;
     const double vtl = DIR == 0 ? vyl : vxl, vtr = DIR == 0 ? vyr : vxr;
     const double pistar = 0.5 * (pl + pr) - 0.5 * a * (vnr - vnl);
-    const double ustar = 0.5 * (vnl + vnr) - 0.5 * (pr - pl) / a;
+    const double ustar = 0.5 * (vnl + vnr) - dv(0.5 * (pr - pl), a);
     const double trans = -(0.5 * a * (vtr - vtl));
     const bool left = ustar > 0.;
     constexpr int md = DIR == 0 ? 1 : 2, om = DIR == 0 ? 2 : 1;
@@ -420,10 +473,10 @@ __device__ __forceinline__ bool unsplit_adm(const double ul[4], const double ur[
 // which ends up holding the right state's vy**2.
 __device__ __forceinline__ void unsplit_inadmissible(const double ur[4], double F[4])
 {
-    const double vy = ur[2] / ur[0];
+    const double vy = dv(ur[2], ur[0]);
     const double no = vy * vy;
     const double z = no - no;
-    const double f = 1.0 / z;
+    const double f = dv(1.0, z);
     F[0] = f; F[1] = f; F[2] = f; F[3] = f;
 }
 
@@ -669,25 +722,25 @@ struct MoodParams {
 // compression test, so the divisions of the others are skipped).
 __device__ __forceinline__ bool shock_mach(const Cell &c, double pc)
 {
-    const double Ca = 1.4 * pc / c.r;
-    double Mach = (c.mx * c.mx + c.my * c.my) / (c.r * c.r);
-    Mach = sqrt(Mach / Ca);
+    const double Ca = dv(1.4 * pc, c.r);
+    double Mach = dv(c.mx * c.mx + c.my * c.my, c.r * c.r);
+    Mach = sqrt(dv(Mach, Ca));
     return Mach > 0.2;
 }
 
 __device__ __forceinline__ bool shock_compression(const Cell &c, const Cell &e, const Cell &w, const Cell &nn,
                                                   const Cell &s, double dx, double dy)
 {
-    double divV = (e.mx - w.mx) / dx + (nn.my - s.my) / dy;
-    divV = 0.5 * divV / c.r;
-    divV = divV - 0.5 * (c.mx * (e.r - w.r) / dx + c.my * (nn.r - s.r) / dy) / (c.r * c.r);
+    double divV = dv(e.mx - w.mx, dx) + dv(nn.my - s.my, dy);
+    divV = dv(0.5 * divV, c.r);
+    divV = divV - dv(0.5 * (dv(c.mx * (e.r - w.r), dx) + dv(c.my * (nn.r - s.r), dy)), c.r * c.r);
     const double m = fmax2(dx, dy);
     return divV < -(m * m);
 }
 
 __device__ __forceinline__ bool shock_pressure_jump(double pL, double pR, double pB, double pT, double dx, double dy)
 {
-    const double gradP = 0.5 * (fabs(pR - pL) / fmin2(pL, pR) / dx + fabs(pT - pB) / fmin2(pB, pT) / dy);
+    const double gradP = 0.5 * (dv(dv(fabs(pR - pL), fmin2(pL, pR)), dx) + dv(dv(fabs(pT - pB), fmin2(pB, pT)), dy));
     return gradP > 5.0;
 }
 
@@ -729,8 +782,8 @@ __device__ __noinline__ bool u2_relax(int u2tol, double dx, double dy, const Dev
             ay = ay + 1.0 * acc(ci, cj - 1).x;
             ay = ay + -2.0 * acc(ci, cj).x;
             ay = ay + 1.0 * acc(ci, cj + 1).x;
-            ax = ax / (dx * dx);
-            ay = ay / (dy * dy);
+            ax = dv(ax, dx * dx);
+            ay = dv(ay, dy * dy);
         } else {
             double q[5];
 #pragma unroll
@@ -749,9 +802,9 @@ __device__ __noinline__ bool u2_relax(int u2tol, double dx, double dy, const Dev
     const double Ymax = fmax2(fmax2(fmax2(fmax2(D2y[0], D2y[1]), D2y[2]), D2y[3]), D2y[4]);
     double delta = 0.0;
     if (u2tol) delta = fmax2(dx, dy);
-    if ((Xmin * Xmax > -delta) && ((fmax2(fabs(Xmin), fabs(Xmax)) < delta) || (fabs(Xmin / Xmax) >= 0.5)))
+    if ((Xmin * Xmax > -delta) && ((fmax2(fabs(Xmin), fabs(Xmax)) < delta) || (fabs(dv(Xmin, Xmax)) >= 0.5)))
         return true;
-    if ((Ymin * Ymax > -delta) && ((fmax2(fabs(Ymin), fabs(Ymax)) < delta) || (fabs(Ymin / Ymax) >= 0.5)))
+    if ((Ymin * Ymax > -delta) && ((fmax2(fabs(Ymin), fabs(Ymax)) < delta) || (fabs(dv(Ymin, Ymax)) >= 0.5)))
         return true;
     return false;
 }

@@ -24,10 +24,10 @@ __device__ __forceinline__ double block_min(double m, double *red)
 
 __device__ __forceinline__ double dt_of_cell(const double4 &u, double dx, double dy)
 {
-    const double vx = u.y / u.x, vy = u.z / u.x;
+    const double vx = dv(u.y, u.x), vy = dv(u.z, u.x);
     const double pres = pressure(u.x, u.y, u.z, u.w);
-    const double cs = sqrt(kGamma * pres / u.x);
-    return 1.0 / (1.0 / (dx / (fabs(vx) + cs)) + 1.0 / (dy / (fabs(vy) + cs)));
+    const double cs = sqrt(dv(kGamma * pres, u.x));
+    return dv(1.0, dv(1.0, dv(dx, fabs(vx) + cs)) + dv(1.0, dv(dy, fabs(vy) + cs)));
 }
 
 // Setdt: per-cell 1/(1/(dx/(|vx|+c)) + 1/(dy/(|vy|+c))), minimum over the
@@ -96,8 +96,8 @@ __global__ __launch_bounds__(kDtThreads) void k_dt(DtArgs a)
             dt = a.dt_given;
         }
         ts[1] = dt;
-        ts[2] = dt / a.dx;
-        ts[3] = dt / a.dy;
+        ts[2] = dv(dt, a.dx);
+        ts[3] = dv(dt, a.dy);
         ts[4] = t;
         if (a.advance_t) ts[0] = t + dt;
         if (a.dt_log) a.dt_log[a.step_idx] = dt;
@@ -153,6 +153,26 @@ __global__ void k_ring(RingArgs a)
         return;
     }
     const double t0 = ts[4], dt = ts[1];
+    if (a.mode == kRingRK4) {  // mod_SSP_RK4.f90:34 over the ghost cells
+        const double xm = dmr_xmin_top(t0, sq);
+        const double4 q2 = bnd_value(a.B, DevSrc{a.X2, NX}, i, j, xm), q3 = bnd_value(a.B, DevSrc{a.X3, NX}, i, j, xm);
+        const double4 q4 = bnd_value(a.B, DevSrc{a.X4, NX}, i, j, xm);
+        const double4 e3 = bnd_value(a.B, DevSrc{a.band3, NX}, i, j, xm), e4 = bnd_value(a.B, DevSrc{a.band, NX}, i, j, xm);
+        double g3[4] = {e3.x, e3.y, e3.z, e3.w}, g4[4] = {e4.x, e4.y, e4.z, e4.w};
+        if (a.gravity) {
+            gravity_source(g3, dt * kGravity);
+            gravity_source(g4, dt * kGravity);
+        }
+        const double x2[4] = {q2.x, q2.y, q2.z, q2.w}, x3[4] = {q3.x, q3.y, q3.z, q3.w}, x4[4] = {q4.x, q4.y, q4.z, q4.w};
+        double o[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const double fu3 = g3[k] - x3[k], fu4 = g4[k] - x4[k];
+            o[k] = kRK4f2 * x2[k] + kRK4f3 * x3[k] + kRK4ff3 * fu3 + kRK4f4 * x4[k] + kRK4ff4 * fu4;
+        }
+        a.ring[r] = make_double4(o[0], o[1], o[2], o[3]);
+        return;
+    }
     // the last stage's Forward_Euler output: Boundary_C at its t_rk, then the source
     const double tb = a.mode == kRingRK3 ? t0 + 0.5 * dt : t0;
     const double4 b = bnd_value(a.B, DevSrc{a.band, NX}, i, j, dmr_xmin_top(tb, sq));

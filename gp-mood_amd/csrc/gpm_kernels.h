@@ -199,6 +199,10 @@ enum {
                     // E = S(G_tb(B) + the face loop's updates of the first ghost layer), B the
                     // last stage's input; then E (FE), 0.5*(G_t0(Un) + E) (SSP_RK2) or
                     // (1./3)*G_t0(Un) + (2./3)*E (SSP_RK3, tb = t0 + dt/2)
+    kRingRK4,       // SSP_RK4 (every stage at t_rk = t0; FE returns Uout - Uin, whose
+                    // Uin ghosts Boundary_C has refilled): f2*G(U2) + f3*G(U3)
+                    // + ff3*(S(G(B3)) - G(U3)) + f4*G(U4) + ff4*(S(G(B)) - G(U4)),
+                    // B3 the fourth stage's band
 };
 
 struct RingArgs {
@@ -211,6 +215,8 @@ struct RingArgs {
     const DevTables *__restrict__ T;  // kRingFace: the reconstruction tables and Gauss weights
     int mode, gravity, flux;
     int integ;                        // kRingFace: kRingFE, kRingRK2 or kRingRK3 (the combination)
+    const double4 *__restrict__ X2, *__restrict__ X3, *__restrict__ X4;  // kRingRK4: U2, U3, U4
+    const double4 *__restrict__ band3;                                   // kRingRK4: B3
 };
 
 template <int MORD, int NGP>

@@ -106,6 +106,10 @@ CONFIGS = {
     "r1_dmr_64x16_rk2": (64, 3, 2, "dmr", True, True, True),
     "r1_m800_32_fe": (32, 3, 2, "mach800", True, True, True, "hll"),
     "r1_rt_16x48_rk2": (16, 3, 2, "rt", True, True, True, "unsplit"),
+    # SSP-RK4 with formula boundaries and the RT source
+    "r1_dmr_64x16_rk4": (64, 3, 2, "dmr", True, True, True),
+    "r1_rt_16x48_rk4": (16, 3, 2, "rt", True, True, True, "unsplit"),
+    "r1_m800_32_rk4": (32, 3, 2, "mach800", True, True, True, "hll"),
     # non-square cells (dx /= dy): the GP tables (ell from min(dx, dy),
     # GP_init.f90), Setdt's two directions and dt/dx, dt/dy of the update
     "r2_32x48_neu_rect": (32, 5, 3, "neumann", True, True, True),
@@ -190,6 +194,9 @@ EXTRAS = {
     "r1_dmr_64x16_rk2": dict(nf=16, lx="4.", ly="1.", integrator="ssp_rk2"),
     "r1_m800_32_fe": dict(lx="1.5", ly="1.5", integrator="fe"),
     "r1_rt_16x48_rk2": dict(nf=48, lx="0.5", ly="1.5", problem="RT", integrator="ssp_rk2"),
+    "r1_dmr_64x16_rk4": dict(nf=16, lx="4.", ly="1.", integrator="ssp_rk4"),
+    "r1_rt_16x48_rk4": dict(nf=48, lx="0.5", ly="1.5", problem="RT", integrator="ssp_rk4"),
+    "r1_m800_32_rk4": dict(lx="1.5", ly="1.5", integrator="ssp_rk4"),
     "r2_32x48_neu_rect": dict(nf=48, lx="1.", ly="1."),
     "r3_n32_ref_rect": dict(lx="2.", ly="1."),
     "fog_n64_neu_uns": dict(integrator="fe", method="FOG"),
@@ -236,7 +243,7 @@ def config(name):
 # time_stepper over libgpmood) is also built
 DROPIN_CONFIGS = ["r2_n64_neu", "r3_n64_ref", "r3_n64_kh_uns", "r1_dmr_64x16", "r1_rt_16x48", "r1_m800_32_fe",
                   "r2_n32_ref_rk4", "fog_n64_kh_uns", "fog_m800_32_hll", "nn_r1_n64_neu", "nncc_r1_n64_neu",
-                  "pol_r1_n64_neu", "r3_n32_ref_rect"]
+                  "pol_r1_n64_neu", "r3_n32_ref_rect", "r1_rt_16x48_rk4"]
 
 PARAMS_TEMPLATE = """module parameters
    use constants

@@ -162,6 +162,11 @@ STEP_CASES = {
     "r1_rt_16x48_q35": ("r1_rt_16x48", "quadrants:0", 35, 20),
     "r2_rt_16x48_q36": ("r2_rt_16x48", "quadrants:0", 36, 20),
     "r1_rt_16x48_rk2_q37": ("r1_rt_16x48_rk2", "quadrants:0", 37, 20),
+    # SSP-RK4 with formula boundaries and the RT source
+    "r1_dmr_64x16_rk4": ("r1_dmr_64x16_rk4", "dmr:0", 0, 20),
+    "r1_rt_16x48_rk4_n67": ("r1_rt_16x48_rk4", "rayleigh_taylor:0.05", 67, 20),
+    "r1_m800_32_rk4_n68": ("r1_m800_32_rk4", "mach800:0.02", 68, 20),
+    "r1_rt_16x48_rk4_q69": ("r1_rt_16x48_rk4", "quadrants:0", 69, 20),
     # non-square cells: dx = 1/32, dy = 1/48 and dx = 1/16, dy = 1/32
     "r2_32x48_neu_rect_q65": ("r2_32x48_neu_rect", "quadrants:0.02", 65, 20),
     "r3_n32_ref_rect_q66": ("r3_n32_ref_rect", "quadrants:0.02", 66, 15),

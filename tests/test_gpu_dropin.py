@@ -24,7 +24,8 @@ CASES = [("r2_n64_neu", "step_r2_n64_neu_fq8.npz"), ("r3_n64_ref", "step_r3_n64_
          ("r2_n32_ref_rk4", "step_r2_n32_ref_rk4_bl30.npz"), ("fog_n64_kh_uns", "step_fog_n64_kh_uns_sh39.npz"),
          ("fog_m800_32_hll", "step_fog_m800_32_hll_n40.npz"),
          ("nn_r1_n64_neu", "refnn_nn_r1_n64_neu_fq44.npz"), ("nncc_r1_n64_neu", "refnn_nncc_r1_n64_neu_fq48.npz"),
-         ("pol_r1_n64_neu", "step_pol_r1_n64_neu_fq49.npz"), ("r3_n32_ref_rect", "step_r3_n32_ref_rect_q66.npz")]
+         ("pol_r1_n64_neu", "step_pol_r1_n64_neu_fq49.npz"), ("r3_n32_ref_rect", "step_r3_n32_ref_rect_q66.npz"),
+         ("r1_rt_16x48_rk4", "step_r1_rt_16x48_rk4_q69.npz")]
 STEP_TYPE = np.dtype([("dt", "<f8"), ("t", "<f8"), ("cnt", "<i4")])
 
 

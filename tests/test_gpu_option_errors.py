@@ -30,7 +30,7 @@ def create_rc(gpm, **kw):
     dict(lf=32, Mord=3, ngp=3, method=201),                              # Unlim_POL with 3 Gauss points
     dict(lf=32, Mord=3, ngp=2, method=1),                                # FOG with ngp 2
     dict(lf=32, Mord=5, ngp=3, method=202, integrator="ssp_rk4"),        # a method without detection, SSP-RK4
-    dict(lf=32, Mord=3, ngp=2, BC_type="dmr", integrator="ssp_rk4"),     # formula boundary, SSP-RK4
+    dict(lf=32, Mord=3, ngp=1, method=1, integrator="ssp_rk4", BC_type="dmr"),  # FOG, SSP-RK4
     dict(lf=64, Mord=5, ngp=3, method=204),                              # NN_GP_MOOD_CC above Mord 3
     dict(lf=64, Mord=3, ngp=2, method=203, BC_type="dirichlet"),         # the network with a formula boundary
     dict(lf=32, Mord=9, ngp=4),                                          # no such order
@@ -43,7 +43,9 @@ def test_unsupported_combinations(gpm, kw):
 def test_supported_combinations(gpm):
     for kw in (dict(lf=32, Mord=3, ngp=2, method=200), dict(lf=32, Mord=7, ngp=4, method=202, integrator="fe"),
                dict(lf=32, Mord=3, ngp=1, method=1, integrator="ssp_rk3"), dict(lf=64, Mord=3, ngp=2, method=204),
-               dict(lf=32, nf=48, Mord=5, ngp=3, method=203, integrator="ssp_rk2", BC_type="kh")):
+               dict(lf=32, nf=48, Mord=5, ngp=3, method=203, integrator="ssp_rk2", BC_type="kh"),
+               dict(lf=32, Mord=3, ngp=2, BC_type="dmr", integrator="ssp_rk4"),
+               dict(lf=32, Mord=5, ngp=3, BC_type="rt", problem=gpm.capi.PROBLEM_RT, integrator="ssp_rk4")):
         assert create_rc(gpm, **kw) == 0, kw
 
 
