@@ -84,7 +84,7 @@ typedef struct gpm_config {
                              pass, no detection.  FOG, GP_SE and Unlim_POL: FE, SSP_RK2 or
                              SSP_RK3 */
     int32_t integrator;   /* GPM_FE, GPM_SSP_RK2, GPM_SSP_RK3 or GPM_SSP_RK4 (time_stepper.f90:17-21;
-                             SSP_RK4 only with a copy boundary and no source term) */
+                             SSP_RK4 only with a method that has DETECTION) */
     int32_t num_flux;     /* GPM_HLLC (mod_HLLC.f90), GPM_LLF (mod_LLF.f90), GPM_HLL (mod_HLL.f90)
                              or GPM_UNSPLIT (mod_unsplit.f90) */
     int32_t bc_type;      /* copy boundaries GPM_NEUMANN, GPM_PERIODIC, GPM_REFLECTIVE, GPM_KH_BC;
