@@ -34,6 +34,15 @@ the reference's own Makefile (gp-mood-code/Makefile:8-58):
     one file is preprocessed to a temporary directory inside the (git-ignored)
     output directory, compiled from there, and the temporary file is deleted.
 
+The configurations marked ``cascade="full"`` build the reference with ONE line
+changed, for the order cascade 7 -> 5 -> 3 -> 1 of the configured workloads:
+the reference lowers a failing cell ``3 -> 1`` and ``>3 -> 3``
+(``mod_subroutine_MOOD.f90:159-163``); the patched build replaces the statement
+``CellGPO(l,n) = 3`` of the ``elseif (CellGPO(l,n) > 3)`` branch (line 162) by
+``CellGPO(l,n) = CellGPO(l,n) - 2``.  Every other line of DETECTION, and every
+other module, is the reference's own.  The patched copy is written to the same
+temporary directory and deleted after compiling.
+
 Usage: python oracle/ref/build_ref.py [config ...]   (default: all configs)
 """
 import os
@@ -143,6 +152,13 @@ CONFIGS = {
     # FOG with the multi-stage integrators
     "fog_n64_per_rk3": (64, 3, 1, "periodic", True, True, True),
     "fog_dmr_64x16_rk2": (64, 3, 1, "dmr", True, True, True, "hll"),
+    # the order cascade 7 -> 5 -> 3 -> 1 (one-line patched DETECTION, see the
+    # module docstring) at GP-R3 under the three copy boundaries
+    "r3_n64_ref_full": (64, 7, 4, "reflective", True, True, True),
+    "r3_n64_per_full": (64, 7, 4, "periodic", True, True, True),
+    "r3_n64_neu_full": (64, 7, 4, "neumann", True, True, True),
+    "r3_n32_ref_full": (32, 7, 4, "reflective", True, True, True),
+    "r3_n2048_ref_full": (2048, 7, 4, "reflective", True, True, True),
 }
 
 # The problem files' own configurations (problems/*.f90: grid, domain, method,
@@ -221,12 +237,35 @@ EXTRAS = {
     "unlim_r1_n64_kh_rk2": dict(method="Unlim_POL", integrator="ssp_rk2"),
     "fog_n64_per_rk3": dict(method="FOG"),
     "fog_dmr_64x16_rk2": dict(nf=16, lx="4.", ly="1.", method="FOG", integrator="ssp_rk2"),
+    "r3_n64_ref_full": dict(cascade="full"),
+    "r3_n64_per_full": dict(cascade="full"),
+    "r3_n64_neu_full": dict(cascade="full"),
+    "r3_n32_ref_full": dict(cascade="full"),
+    "r3_n2048_ref_full": dict(cascade="full"),
 }
+
+# DETECTION's cascade branch (mod_subroutine_MOOD.f90:161-162) and its
+# replacement in the cascade="full" builds
+CASCADE_LINE = ("               elseif (CellGPO(l,n) >  3)  then\n"
+                "                  CellGPO(l,n) = 3\n")
+CASCADE_FULL = ("               elseif (CellGPO(l,n) >  3)  then\n"
+                "                  CellGPO(l,n) = CellGPO(l,n) - 2\n")
+
+
+def patched_mood_source(src, dst):
+    """Write mod_subroutine_MOOD.f90 with the one-line cascade change to dst."""
+    text = open(src).read()
+    if text.count(CASCADE_LINE) != 1:
+        raise RuntimeError("cascade branch of mod_subroutine_MOOD.f90 not found exactly once")
+    with open(dst, "w") as f:
+        f.write(text.replace(CASCADE_LINE, CASCADE_FULL))
 
 
 def extras(name):
-    """nf, Lx, Ly (as Fortran literals), integrator, problem and method of a configuration."""
-    e = dict(nf=CONFIGS[name][0], lx="1.", ly="1.", integrator="ssp_rk3", problem="RP_2D_3", method="GP_MOOD")
+    """nf, Lx, Ly (as Fortran literals), integrator, problem, method and
+    cascade of a configuration."""
+    e = dict(nf=CONFIGS[name][0], lx="1.", ly="1.", integrator="ssp_rk3", problem="RP_2D_3", method="GP_MOOD",
+             cascade="reference")
     e.update(EXTRAS.get(name, {}))
     if name in PROBLEM_FILES:
         _, _, nf, lx, ly, method, integ, _, _, _, _, _, problem = PROBLEM_FILES[name]
@@ -321,6 +360,10 @@ def build(name, dropin=False):
             extra = []
             if mod == "mod_subroutine_MOOD":
                 extra = ["-cpp", "-DpR=pRgt", "-DpL=pLft"]
+                if e["cascade"] == "full":
+                    pp = os.path.join(tmpdir, "mod_subroutine_MOOD.f90")
+                    patched_mood_source(src, pp)
+                    src = pp
             if mod == "global_variables" and mord == 7:
                 pp = os.path.join(tmpdir, "global_variables_pp.f90")
                 with open(pp, "w") as f:

@@ -62,7 +62,8 @@ def case_options(z):
                 ly=float(z["ly"]) if "ly" in z else 1.0, flux=str(z["flux"]) if "flux" in z else "hllc",
                 integrator=str(z["integrator"]) if "integrator" in z else "ssp_rk3",
                 problem=int(z["problem"]) if "problem" in z else 0,
-                method=str(z["method"]) if "method" in z else "gp_mood")
+                method=str(z["method"]) if "method" in z else "gp_mood",
+                cascade=str(z["cascade"]) if "cascade" in z else "reference")
 
 
 @pytest.fixture(scope="session")

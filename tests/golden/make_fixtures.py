@@ -190,6 +190,12 @@ STEP_CASES = {
     "unlim_r1_n64_kh_rk2_sh55": ("unlim_r1_n64_kh_rk2", "shear_layer", 55, 20),
     "fog_n64_per_rk3_fq56": ("fog_n64_per_rk3", "four_quadrant", 56, 25),
     "fog_dmr_64x16_rk2": ("fog_dmr_64x16_rk2", "dmr:0", 0, 25),
+    # the order cascade 7 -> 5 -> 3 -> 1 (one-line patched reference build)
+    "r3_n64_ref_full_bl70": ("r3_n64_ref_full", "blast", 70, 20),
+    "r3_n64_per_full_bl71": ("r3_n64_per_full", "blast", 71, 20),
+    "r3_n64_neu_full_fq72": ("r3_n64_neu_full", "four_quadrant", 72, 20),
+    "r3_n32_ref_full_fq73": ("r3_n32_ref_full", "four_quadrant", 73, 20),
+    "r3_n64_per_full_fq74": ("r3_n64_per_full", "four_quadrant", 74, 15),
 }
 # the reference's own network (method NN_GP_MOOD / NN_GP_MOOD_CC): name:
 # (config, IC kind, seed, steps); the network is refnn_weights(seed)
@@ -248,7 +254,7 @@ def main(argv):
         np.savez_compressed(
             os.path.join(OUT, f"step_{name}.npz"), n=n, mord=mord, ngp=ngp, bc=bc, dmp=dmp, u2=u2, u2tol=u2tol, flux=flux,
             nf=e["nf"], lx=float(e["lx"]), ly=float(e["ly"]), integrator=e["integrator"],
-            problem=51 if e["problem"] == "RT" else 0, method=e["method"].lower(),
+            problem=51 if e["problem"] == "RT" else 0, method=e["method"].lower(), cascade=e["cascade"],
             kind=kind, seed=seed, nsteps=nsteps, ic_sha=sha(U0), U0=U0,
             dt=r["steps"]["dt"], t=r["steps"]["t"], counts=r["steps"]["cnt"], masks=r["masks"], U=r["U"],
             **({"pol": r["pol"]} if e["method"] in ("POL_MOOD", "Unlim_POL") else {}),
@@ -312,20 +318,22 @@ def main(argv):
 
 
 
-def long_case(n=2048, steps=500, seed=0, pin_steps=2, threads=8):
+def long_case(n=2048, steps=500, seed=0, pin_steps=2, threads=8, cascade="reference"):
     """Config-1 workload (2048^2, R3 = Mord 7/ngp 4, reflective, blast IC, 500
     steps).  The reference needs about 3 h for it on one core, so: (1) the
     reference and the C restatement (oracle/gpm_oracle.c) are run for
     `pin_steps` steps and must agree bit for bit; (2) the C restatement runs
-    the 500 steps; its dt sequence, counts and checksums are stored."""
+    the 500 steps; its dt sequence, counts and checksums are stored.  With
+    cascade="full" the reference is the one-line patched build of the order
+    cascade 7 -> 5 -> 3 -> 1 (oracle/ref/build_ref.py)."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O
     os.environ.setdefault("OMP_NUM_THREADS", str(threads))
-    cfg = "r3_n2048_ref"
+    cfg = "r3_n2048_ref" if cascade == "reference" else "r3_n2048_ref_full"
     U0 = ic.make("blast", n, seed)
     r = run_ref(cfg, U0, pin_steps, 0)
     t = r["tables"]
-    orc = O.Oracle(n, 7, 4, "reflective", {k: t[k] for k in ("zT", "d2x", "d2y", "gw", "ixiy")})
+    orc = O.Oracle(n, 7, 4, "reflective", {k: t[k] for k in ("zT", "d2x", "d2y", "gw", "ixiy")}, cascade=cascade)
     U = orc.full(U0)
     ro = orc.run(U, pin_steps, masks=True)
     assert np.array_equal(ro["dt"], r["steps"]["dt"]), "oracle dt differs from the reference"
@@ -337,9 +345,11 @@ def long_case(n=2048, steps=500, seed=0, pin_steps=2, threads=8):
     counts = np.concatenate([ro["counts"], ro2["counts"]])
     Ui = np.ascontiguousarray(U[4:-4, 4:-4])
     g = np.zeros((n, n), dtype=np.int8)
+    tag = "ref" if cascade == "reference" else "ref_full"
     np.savez_compressed(
-        os.path.join(OUT, f"hash_r3_n{n}_ref_bl{seed}_{steps}.npz"), n=n, mord=7, ngp=4, bc="reflective", dmp=True,
-        u2=True, u2tol=True, kind="blast", seed=seed, nsteps=steps, pin_steps=pin_steps, ic_sha=sha(U0), dt=dts,
+        os.path.join(OUT, f"hash_r3_n{n}_{tag}_bl{seed}_{steps}.npz"), n=n, mord=7, ngp=4, bc="reflective", dmp=True,
+        u2=True, u2tol=True, cascade=cascade, kind="blast", seed=seed, nsteps=steps, pin_steps=pin_steps,
+        ic_sha=sha(U0), dt=dts,
         counts=counts, U_sha=sha(Ui), U_sum=Ui.sum(axis=(0, 1)), U_abs_sum=np.abs(Ui).sum(axis=(0, 1)),
         U_pin_sha=sha(np.ascontiguousarray(r["U"][4:-4, 4:-4])), mask_pin=r["masks"][-1], dt_pin=r["steps"]["dt"],
         **{k: v for k, v in t.items()})
@@ -348,6 +358,6 @@ def long_case(n=2048, steps=500, seed=0, pin_steps=2, threads=8):
 
 if __name__ == "__main__":
     if sys.argv[1:2] == ["--long"]:
-        long_case()
+        long_case(cascade=sys.argv[2] if len(sys.argv) > 2 else "reference")
     else:
         main(sys.argv[1:])

@@ -21,7 +21,7 @@ def make_config(gpm, z):
     o = case_options(z)
     return gpm.Config(lf=int(z["n"]), nf=o["nf"], Mord=int(z["mord"]), ngp=int(z["ngp"]), BC_type=str(z["bc"]),
                       DMP=bool(z["dmp"]), U2=bool(z["u2"]), U2_tol=bool(z["u2tol"]), numFlux=o["flux"],
-                      integrator=o["integrator"], problem=o["problem"], Lx=o["lx"], Ly=o["ly"],
+                      integrator=o["integrator"], problem=o["problem"], Lx=o["lx"], Ly=o["ly"], cascade=o["cascade"],
                       method={"fog": gpm.capi.FOG, "pol_mood": gpm.capi.POL_MOOD, "gp_se": gpm.capi.GP_SE,
                               "unlim_pol": gpm.capi.UNLIM_POL}.get(o["method"], gpm.capi.GP_MOOD))
 
@@ -38,7 +38,8 @@ def oracle_first_step(z, U):
     dx, dy = (float(v) for v in z["dxdy"])
     orc = O.Oracle(int(z["n"]), int(z["mord"]), int(z["ngp"]), str(z["bc"]), tables_from_npz(z),
                    dmp=bool(z["dmp"]), u2=bool(z["u2"]), u2tol=bool(z["u2tol"]), flux=o["flux"], nf=o["nf"],
-                   integrator=o["integrator"], problem=o["problem"], dx=dx, dy=dy, method=o["method"])
+                   integrator=o["integrator"], problem=o["problem"], dx=dx, dy=dy, method=o["method"],
+                   cascade=o["cascade"])
     Uo = U.copy()
     return orc.step(Uo, float(z["dt"][0]), 0.0), Uo
 

@@ -23,7 +23,8 @@ def make_oracle(O, z):
     dx, dy = (float(v) for v in z["dxdy"])
     return O.Oracle(int(z["n"]), int(z["mord"]), int(z["ngp"]), str(z["bc"]), tables_from_npz(z),
                     dmp=bool(z["dmp"]), u2=bool(z["u2"]), u2tol=bool(z["u2tol"]), flux=o["flux"], nf=o["nf"],
-                    integrator=o["integrator"], problem=o["problem"], dx=dx, dy=dy, method=o["method"])
+                    integrator=o["integrator"], problem=o["problem"], dx=dx, dy=dy, method=o["method"],
+                    cascade=o["cascade"])
 
 
 @pytest.mark.parametrize("case", STEP_CASES)
@@ -83,3 +84,25 @@ def test_oracle_pinned_at_2048(orc):
     U = o.full(U0)
     r = o.run(U, 1)
     assert r["dt"][0] == z["dt_pin"][0] == z["dt"][0]
+
+
+FULL_CASCADE_CASES = [c for c in STEP_CASES if "_full_" in c]
+
+
+@pytest.mark.parametrize("case", FULL_CASCADE_CASES)
+def test_full_cascade_fixture_separates_the_cascades(orc, case):
+    """The full-cascade fixtures come from the reference build with DETECTION's
+    `>3 -> 3` lowering replaced by one order down (oracle/ref/build_ref.py).
+    They pin the oracle's cascade 7 -> 5 -> 3 -> 1 only if the reference
+    cascade 7 -> 3 -> 1 does NOT reproduce them: order 5 occurs in the masks,
+    and the oracle run with the reference cascade gives other masks."""
+    z = golden(f"step_{case}.npz")
+    assert str(z["cascade"]) == "full"
+    assert (z["masks"] == 5).any(), "no cell took the 7 -> 5 step"
+    o = make_oracle(orc, z)
+    o_ref = orc.Oracle(int(z["n"]), int(z["mord"]), int(z["ngp"]), str(z["bc"]), tables_from_npz(z),
+                       dmp=bool(z["dmp"]), u2=bool(z["u2"]), u2tol=bool(z["u2tol"]), cascade="reference")
+    U = o_ref.full(z["U0"])
+    r = o_ref.run(U, int(z["nsteps"]), masks=True)
+    assert not np.array_equal(r["masks"], z["masks"])
+    assert not np.array_equal(U, z["U"])
