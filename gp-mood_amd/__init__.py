@@ -195,6 +195,17 @@ class Solver:
     def stream(self):
         return self.lib.gpm_stream(self._h)
 
+    def set_mood_launch(self, cooperative=True):
+        """MOOD correction passes as one cooperative launch per stage (True)
+        or as per-pass launches (False); the results are the same."""
+        self._check(self.lib.gpm_set_mood_launch(self._h, int(cooperative)), "gpm_set_mood_launch")
+
+    def mood_launch(self):
+        """True while the MOOD passes run as a cooperative launch."""
+        v = ctypes.c_int32()
+        self._check(self.lib.gpm_get_mood_launch(self._h, ctypes.byref(v)), "gpm_get_mood_launch")
+        return bool(v.value)
+
     # -- learned a priori predictor (NN-GP-MOOD) ----------------------------
     def set_predictor(self, mlp, gate=True):
         """Attach an :class:`MLP` (None detaches): every RK stage starts from

@@ -40,7 +40,7 @@ EXPORTS = [
     "gpm_create", "gpm_destroy", "gpm_last_error", "gpm_set_gp_tables", "gpm_gp_tables", "gpm_boundary_c",
     "gpm_set_state", "gpm_get_state", "gpm_set_state_device", "gpm_get_state_device",
     "gpm_compute_dt", "gpm_time_stepping", "gpm_run", "gpm_get_stats", "gpm_reset_stats",
-    "gpm_set_time", "gpm_stream", "gpm_set_timing", "gpm_kernel_time",
+    "gpm_set_time", "gpm_stream", "gpm_set_timing", "gpm_kernel_time", "gpm_set_mood_launch", "gpm_get_mood_launch",
     "gpm_mlp_create", "gpm_mlp_destroy", "gpm_mlp_last_error", "gpm_mlp_num_params", "gpm_mlp_set_params",
     "gpm_mlp_get_params", "gpm_mlp_get_grads", "gpm_mlp_forward_backward", "gpm_mlp_adamw_step",
     "gpm_mlp_train_step", "gpm_mlp_logits", "gpm_set_predictor", "gpm_set_dump", "gpm_features",
@@ -113,6 +113,8 @@ def load():
         "gpm_set_time": ([P, D], I),
         "gpm_stream": ([P], P),
         "gpm_set_timing": ([P, I], I),
+        "gpm_set_mood_launch": ([P, I], I),
+        "gpm_get_mood_launch": ([P, ctypes.POINTER(I)], I),
         "gpm_kernel_time": ([P, ctypes.POINTER(D), ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(D),
                              ctypes.POINTER(ctypes.c_int64)], I),
         "gpm_mlp_create": ([I, I, I, ctypes.POINTER(P)], I),

@@ -44,12 +44,13 @@ struct gpm_ctx {
     StepExtra *ext = nullptr;  // [kMaxStages] in device memory (plan_stages' per-stage extras)
     order_t *gpo = nullptr;
     int *slistA = nullptr, *slistB = nullptr, *mark = nullptr;
-    int *counters = nullptr, *stamp = nullptr, *err = nullptr, *solo_p = nullptr;
+    int *counters = nullptr, *stamp = nullptr;
     DevStats *stats = nullptr;
     double *tstate = nullptr, *block_min = nullptr, *dt_log = nullptr;
     unsigned *done = nullptr;
     int dt_log_cap = 0;
     int tiles_x = 0, ntiles = 0, dt_blocks = 0, mood_blocks = 0, mood_solo = 0;
+    bool mood_coop = true;  // MOOD passes as one cooperative launch per stage (else per-pass launches)
     bool has_state = false;
     bool timing = false;
     double stage_ms = 0, mood_ms = 0;
@@ -349,7 +350,7 @@ StageArgs stage_args(gpm_ctx *c, int k, const StagePlan &P)
     int *cnt = c->counters + k * kCountersPerStage;
     StageArgs s{};
     s.Uin = P.in; s.Uout = P.out; s.C = P.C; s.B = c->B; s.gpo = c->gpo;
-    s.slist1 = c->slistB; s.cnt_s1 = cnt + kMaxPasses + 1 + 1; s.cnt_f0 = cnt;
+    s.slist1 = c->slistB; s.cnt_s1 = cnt + kCntS0 + 1; s.cnt_f0 = cnt + kCntFail0;
     s.mark = c->mark; s.stamp = c->stamp; s.T = c->dT; s.tstate = c->tstate;
     s.dmp = c->cfg.dmp; s.u2 = c->cfg.u2; s.u2tol = c->cfg.u2_tol;
     s.cascade = c->cfg.cascade;
@@ -396,6 +397,45 @@ int enqueue_ring_fill(gpm_ctx *c)
     RingArgs r{};
     r.A = c->dU; r.ring = c->ring; r.tstate = c->tstate; r.B = c->B; r.mode = kRingFill;
     HIPCHK(c, launch_ring(r, c->stream));
+    return GPM_OK;
+}
+
+// The MOOD correction passes of one stage.  Cooperative: one launch runs every
+// pass.  Otherwise (when asked for, or when the cooperative launch is
+// refused, e.g. while another process holds the device's cooperative-launch
+// resource) per-pass launches in growing batches, with a look at the stage's
+// pending flag between batches: a launch runs the current pass or one run of
+// workgroup-0 passes and completes at least one pass, and launches past the
+// last pass return at once.
+int enqueue_mood(gpm_ctx *c, MoodArgs &m)
+{
+    const int mord = c->mord, ngp = c->ngp, flux = c->cfg.num_flux;
+    if (c->mood_coop) {
+        void *args[] = {&m};
+        const hipError_t e = hipLaunchCooperativeKernel(mood_kernel(mord, ngp, flux, c->ring_mode, true),
+                                                        dim3(c->mood_blocks), dim3(kMoodThreads), args, 0, c->stream);
+        if (e == hipSuccess) return GPM_OK;
+        (void)hipGetLastError();
+        std::fprintf(stderr, "gpm: cooperative launch refused (%s); MOOD passes run as per-pass launches\n",
+                     hipGetErrorString(e));
+        c->mood_coop = false;
+    }
+    const void *k = mood_kernel(mord, ngp, flux, c->ring_mode, false);
+    void *args[] = {&m};
+    // a stage runs at most 3 x cells + 1 passes (every pass but the last lowers a cell)
+    const long long max_launches = 3ll * c->NX * c->NY + 2;
+    long long launched = 0;
+    for (int batch = 2;; batch = batch < 64 ? 2 * batch : 64) {
+        for (int b = 0; b < batch; ++b)
+            HIPCHK(c, hipLaunchKernel(k, dim3(c->mood_blocks), dim3(kMoodThreads), args, 0, c->stream));
+        launched += batch;
+        int pending = 0;
+        HIPCHK(c, hipMemcpyAsync(&pending, m.cnt + kCntPending, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        if (!pending) break;
+        if (launched > max_launches) return fail(c, GPM_EHIP, "MOOD passes did not end");
+    }
+    HIPCHK(c, launch_mood_finish(m, c->stream));
     return GPM_OK;
 }
 
@@ -465,11 +505,10 @@ int enqueue_stages(gpm_ctx *c)
         MoodArgs m{};
         m.Uin = in; m.Uout = plan[k].out; m.C = plan[k].C; m.B = c->B; m.gpo = c->gpo;
         m.slist[0] = c->slistA; m.slist[1] = c->slistB;
-        m.cnt_f = cnt; m.cnt_s = cnt + kMaxPasses + 1; m.mark = c->mark; m.stamp = c->stamp; m.err = c->err;
-        m.solo_p = c->solo_p; m.solo_max = c->mood_solo;
+        m.cnt = cnt; m.mark = c->mark; m.stamp = c->stamp; m.solo_max = c->mood_solo;
         m.stats = c->stats; m.T = c->dT; m.tstate = c->tstate;
         m.dmp = c->cfg.dmp; m.u2 = c->cfg.u2; m.u2tol = c->cfg.u2_tol;
-        m.cascade = c->cfg.cascade; m.pmax = kMaxPasses; m.stage_idx = k;
+        m.cascade = c->cfg.cascade; m.stage_idx = k;
         m.dx = c->dx; m.dy = c->dy;
         m.recon = s.recon;
         if (refnn) {
@@ -477,7 +516,6 @@ int enqueue_stages(gpm_ctx *c)
             m.nn_passes = 1;
             if (c->refnn_cc) m.vnn = c->vnn;  // m.recon is kReconCC from s
         }
-        void *args[] = {&m};
         if (c->timing) {
             size_t i0 = c->ev_used;
             e0 = next_event(c); e1 = next_event(c);
@@ -485,8 +523,8 @@ int enqueue_stages(gpm_ctx *c)
             c->ev_pairs.push_back({(int)i0, 1});
             HIPCHK(c, hipEventRecord(e0, c->stream));
         }
-        HIPCHK(c, hipLaunchCooperativeKernel(mood_kernel(c->mord, c->ngp, c->cfg.num_flux, c->ring_mode), dim3(c->mood_blocks),
-                                             dim3(kMoodThreads), args, 0, c->stream));
+        int rc = enqueue_mood(c, m);
+        if (rc) return rc;
         if (c->timing) HIPCHK(c, hipEventRecord(e1, c->stream));
         if (c->rec_masks && c->rec_count < c->rec_cap) {
             const size_t n = (size_t)c->NX * c->NY;
@@ -521,15 +559,6 @@ int enqueue_stages(gpm_ctx *c)
 bool square_copy_rk3(const gpm_ctx *c)
 {
     return c->NX == c->NY && index_map_bc(c->bc) && !c->gravity && c->integrator == GPM_SSP_RK3;
-}
-
-int check_device_error(gpm_ctx *c)
-{
-    int e = 0;
-    HIPCHK(c, hipMemcpyAsync(&e, c->err, sizeof(int), hipMemcpyDeviceToHost, c->stream));
-    HIPCHK(c, hipStreamSynchronize(c->stream));
-    if (e) return fail(c, GPM_EINVAL, "MOOD iteration exceeded the pass limit");
-    return GPM_OK;
 }
 
 }  // namespace
@@ -611,7 +640,7 @@ int gpm_create(const gpm_config *cfg, gpm_ctx **out)
               dalloc(&c->slistA, n) == hipSuccess && dalloc(&c->slistB, n) == hipSuccess &&
               dalloc(&c->mark, n) == hipSuccess &&
               dalloc(&c->counters, kMaxStages * kCountersPerStage) == hipSuccess && dalloc(&c->stamp, 1) == hipSuccess &&
-              dalloc(&c->err, 1) == hipSuccess && dalloc(&c->solo_p, 1) == hipSuccess && dalloc(&c->stats, 1) == hipSuccess &&
+              dalloc(&c->stats, 1) == hipSuccess &&
               dalloc(&c->tstate, 8) == hipSuccess && dalloc(&c->done, 1) == hipSuccess &&
               dalloc(&c->pred, n) == hipSuccess && dalloc(&c->nnlist, n) == hipSuccess &&
               dalloc(&c->priori, n) == hipSuccess &&
@@ -649,21 +678,23 @@ int gpm_create(const gpm_config *cfg, gpm_ctx **out)
     {
         int per = 0;
         e = hipOccupancyMaxActiveBlocksPerMultiprocessor(
-            &per, mood_kernel(c->fog ? 1 : c->mord, c->fog ? 1 : c->ngp, c->cfg.num_flux, c->ring_mode), kMoodThreads, 0);
+            &per, mood_kernel(c->fog ? 1 : c->mord, c->fog ? 1 : c->ngp, c->cfg.num_flux, c->ring_mode, true),
+            kMoodThreads, 0);
         if (e != hipSuccess || per <= 0) per = 1;
         c->mood_blocks = want < per * ncu ? want : per * ncu;
     }
+    // GPM_MOOD_COOP=0 selects the per-pass launches of the MOOD passes.
+    if (const char *env = std::getenv("GPM_MOOD_COOP")) c->mood_coop = std::atoi(env) != 0;
     // Passes over at most this many cells run on one workgroup (GPM_MOOD_SOLO overrides; 0 disables).
     c->mood_solo = 16;
     if (const char *env = std::getenv("GPM_MOOD_SOLO")) c->mood_solo = std::atoi(env) > 0 ? std::atoi(env) : 0;
     if (c->mood_solo > kSoloMax) c->mood_solo = kSoloMax;
     hipMemsetAsync(c->mark, 0, n * sizeof(int), c->stream);
-    hipMemsetD16Async(c->gpo, pack_order(c->mord, c->mord, 0), n, c->stream);
+    hipMemsetD32Async(c->gpo, pack_order(c->mord, c->mord, 0), n, c->stream);
     hipMemsetAsync(c->counters, 0, kMaxStages * kCountersPerStage * sizeof(int), c->stream);
     hipMemsetAsync(c->stats, 0, sizeof(DevStats), c->stream);
     hipMemsetAsync(c->tstate, 0, 8 * sizeof(double), c->stream);
     hipMemsetAsync(c->done, 0, sizeof(unsigned), c->stream);
-    hipMemsetAsync(c->err, 0, sizeof(int), c->stream);
     int one = 1;
     hipMemcpyAsync(c->stamp, &one, sizeof(int), hipMemcpyHostToDevice, c->stream);
     hipMemcpyAsync(c->st_dev, c->st_host, sizeof(c->st_host), hipMemcpyHostToDevice, c->stream);
@@ -704,7 +735,7 @@ int gpm_destroy(gpm_ctx *c)
     if (c->stream) hipStreamSynchronize(c->stream);
     void *bufs[] = {c->dT, c->dU, c->dA, c->dB, c->dC, c->dD, c->dE, c->dF, c->ring, c->band, c->band3, c->st_dev, c->ext,
                     c->gpo, c->slistA, c->slistB, c->mark, c->counters,
-                    c->stamp, c->err, c->solo_p, c->stats, c->tstate, c->block_min, c->dt_log, c->done, c->pred, c->nnlist,
+                    c->stamp, c->stats, c->tstate, c->block_min, c->dt_log, c->done, c->pred, c->nnlist,
                     c->nncount, c->adj_ub, c->adj_G, c->adj_A, c->adj_tmp, c->adj_ordx, c->adj_ordy,
                     c->priori, c->refnn_w, c->rowv, c->vnn, c->xfer, c->xfer_gpo, c->rdump.nr_start};
     for (void *b : bufs)
@@ -751,7 +782,7 @@ int gpm_set_state(gpm_ctx *c, const double *U)
     if (!c->xfer) HIPCHK(c, dalloc(&c->xfer, full));
     HIPCHK(c, hipMemcpyAsync(c->xfer, U, full * sizeof(double4), hipMemcpyHostToDevice, c->stream));
     HIPCHK(c, launch_unpack_ghosted(c->xfer, c->dU, c->ring_mode ? c->ring : nullptr, c->NX, c->NY, c->stream));
-    HIPCHK(c, hipMemsetD16Async(c->gpo, pack_order(c->mord, c->mord, 0), n, c->stream));
+    HIPCHK(c, hipMemsetD32Async(c->gpo, pack_order(c->mord, c->mord, 0), n, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
     c->has_state = true;
     return GPM_OK;
@@ -782,7 +813,7 @@ int gpm_set_state_device(gpm_ctx *c, const double *u_dev)
     if (!c || !u_dev) return fail(c, GPM_EINVAL, "null argument");
     const size_t n = (size_t)c->NX * c->NY;
     HIPCHK(c, hipMemcpyAsync(c->dU, u_dev, n * 4 * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
-    HIPCHK(c, hipMemsetD16Async(c->gpo, pack_order(c->mord, c->mord, 0), n, c->stream));
+    HIPCHK(c, hipMemsetD32Async(c->gpo, pack_order(c->mord, c->mord, 0), n, c->stream));
     int rc = enqueue_ring_fill(c);  // ghost cells by Boundary_C at the current time
     if (rc) return rc;
     c->has_state = true;
@@ -837,8 +868,6 @@ int gpm_time_stepping(gpm_ctx *c, double *Uin, double *Uout, double t, double dt
     if (rc) return rc;
     rc = enqueue_stages(c);
     if (rc) return rc;
-    rc = check_device_error(c);
-    if (rc) return rc;
     rc = collect_timing(c);
     if (rc) return rc;
     rc = gpm_get_state(c, Uout, cellgpo);
@@ -872,14 +901,12 @@ int gpm_run(gpm_ctx *c, int32_t nsteps, double tmax, double *dts)
             if (rc) return rc;
         }
     }
-    int rc = check_device_error(c);
+    int rc = collect_timing(c);
     if (rc) return rc;
-    rc = collect_timing(c);
-    if (rc) return rc;
-    if (dts && nsteps > 0) {
+    if (dts && nsteps > 0)
         HIPCHK(c, hipMemcpyAsync(dts, c->dt_log, nsteps * sizeof(double), hipMemcpyDeviceToHost, c->stream));
-        HIPCHK(c, hipStreamSynchronize(c->stream));
-    }
+    // synchronous: the steps are done when gpm_run returns
+    HIPCHK(c, hipStreamSynchronize(c->stream));
     return GPM_OK;
 }
 
@@ -920,6 +947,20 @@ int gpm_reset_stats(gpm_ctx *c)
 }
 
 void *gpm_stream(gpm_ctx *c) { return c ? (void *)c->stream : nullptr; }
+
+int gpm_set_mood_launch(gpm_ctx *c, int32_t cooperative)
+{
+    if (!c) return GPM_EINVAL;
+    c->mood_coop = cooperative != 0;
+    return GPM_OK;
+}
+
+int gpm_get_mood_launch(gpm_ctx *c, int32_t *cooperative)
+{
+    if (!c || !cooperative) return GPM_EINVAL;
+    *cooperative = c->mood_coop ? 1 : 0;
+    return GPM_OK;
+}
 
 int gpm_set_timing(gpm_ctx *c, int32_t enable)
 {

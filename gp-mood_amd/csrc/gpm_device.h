@@ -836,17 +836,17 @@ __device__ __forceinline__ int lower_order(int ord, int cascade)
     return ord > 1 ? ord - 2 : ord;
 }
 
-// Per-cell order word (16 bits): bits 0-1 the order index (CellGPO = 2*idx+1),
+// Per-cell order word (32 bits): bits 0-1 the order index (CellGPO = 2*idx+1),
 // bits 2-3 the index of the order the cell had before its last lowering, bits
-// 4-15 the MOOD pass in which the cascade last lowered it (0: the first pass,
+// 4-31 the MOOD pass in which the cascade last lowered it (0: the first pass,
 // or the a priori order).  A correction pass reads the orders its cells had
 // when the pass began: a cell lowered during the same pass p (version p) still
 // counts with its previous order, so decisions and lowering share one grid
 // barrier per pass.
-typedef uint16_t order_t;
+typedef uint32_t order_t;
 __host__ __device__ __forceinline__ order_t pack_order(int ord, int prev, int ver)
 {
-    return (order_t)((ver << 4) | (((prev - 1) >> 1) << 2) | ((ord - 1) >> 1));
+    return (order_t)(((unsigned)ver << 4) | (((prev - 1) >> 1) << 2) | ((ord - 1) >> 1));
 }
 __host__ __device__ __forceinline__ int order_of_word(order_t v) { return 2 * (v & 3) + 1; }
 __device__ __forceinline__ int order_at_pass(order_t v, int p)

@@ -14,9 +14,9 @@ hipError_t launch_stage(int mord, int ngp, int flux, bool gen, const StageArgs &
     return hipErrorInvalidValue;
 }
 
-const void *mood_kernel(int mord, int ngp, int flux, bool gen)
+const void *mood_kernel(int mord, int ngp, int flux, bool gen, bool cooperative)
 {
-#define X(M, G) if (mord == M && ngp == G) return mood_kernel_mg<M, G>(flux, gen);
+#define X(M, G) if (mord == M && ngp == G) return mood_kernel_mg<M, G>(flux, gen, cooperative);
     GPM_COMBOS(X)
 #undef X
     return nullptr;
@@ -30,6 +30,6 @@ hipError_t launch_ring_face(int mord, int ngp, int flux, const RingArgs &a, hipS
     return hipErrorInvalidValue;
 }
 
-bool supported(int mord, int ngp) { return mord > 1 && mood_kernel(mord, ngp, kHLLC, false) != nullptr; }
+bool supported(int mord, int ngp) { return mord > 1 && mood_kernel(mord, ngp, kHLLC, false, true) != nullptr; }
 
 }  // namespace gpm

@@ -112,6 +112,15 @@ __global__ __launch_bounds__(kDtThreads) void k_dt(DtArgs a)
 }
 
 
+// Statistics of a stage whose MOOD passes ran as per-pass launches.
+__global__ void k_mood_finish(MoodArgs a) { mood_finish(a, a.cnt[kCntPass] + 1); }
+
+hipError_t launch_mood_finish(const MoodArgs &a, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_mood_finish, dim3(1), dim3(1), 0, s, a);
+    return hipGetLastError();
+}
+
 // Order words -> plain CellGPO bytes.
 __global__ void k_unpack_orders(const order_t *__restrict__ g, int8_t *__restrict__ out, size_t n)
 {

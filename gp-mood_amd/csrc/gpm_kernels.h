@@ -43,8 +43,22 @@ constexpr int kSoloMax = 64;  // largest list a single workgroup takes over
 template <int MORD>
 constexpr int kHalo = ((MORD - 1) / 2 + 1) > 2 ? ((MORD - 1) / 2 + 1) : 2;
 
-constexpr int kMaxPasses = 4096;  // MOOD passes per stage before the run is declared failed
-constexpr int kCountersPerStage = 2 * kMaxPasses + 2;
+// Per-stage int counters of the MOOD passes (zeroed by k_dt at every step).
+// The recompute counts form a ring of three: pass p reads kCntS0 + p % 3, adds
+// the next pass's cells to kCntS0 + (p + 1) % 3 and clears kCntS0 + (p + 2) %
+// 3, whose last reader was pass p - 1; so a stage can run any number of
+// passes (at most 3 x cells + 1: every pass but the last lowers a cell).
+enum {
+    kCntFail0 = 0,    // troubled cells of the first pass (k_stage)
+    kCntDet = 1,      // troubled cells of the correction passes
+    kCntS0 = 2,       // kCntS0 .. kCntS0 + 2: the ring of recompute counts
+    kCntPass = 5,     // per-pass launches: the current pass minus one
+    kCntDone = 6,     // per-pass launches: workgroups done with the launch
+    kCntNext = 7,     // the pass that follows a launch or a run of workgroup-0 passes
+    kCntSecond = 8,   // cells recomputed in the correction passes
+    kCntPending = 9,  // per-pass launches: the current pass has cells to recompute
+    kCountersPerStage = 16
+};
 
 struct DevStats {
     long long steps;
@@ -126,18 +140,15 @@ struct MoodArgs {
     double4 *__restrict__ Uout;
     order_t *gpo;
     int *slist[2];  // recompute lists, pass p reads slist[p & 1]
-    int *cnt_f;     // [kMaxPasses+1] troubled-cell counts per pass (pass 0 = k_stage)
-    int *cnt_s;     // [kMaxPasses+1] recomputed-cell counts per pass
+    int *cnt;       // the stage's kCountersPerStage counters (kCnt*)
     int *mark;
     int *stamp;
-    int *err;
-    int *solo_p;    // first pass after a run of workgroup-0-only passes
     DevStats *stats;
     const DevTables *__restrict__ T;
     const double *__restrict__ tstate;
     Bnd B;
     StepCtl C;
-    int dmp, u2, u2tol, cascade, pmax, stage_idx;
+    int dmp, u2, u2tol, cascade, stage_idx;
     int nn_passes;  // count the passes that lowered a cell (a step of the reference network)
     int solo_max;   // lists of at most this many cells run on workgroup 0 alone (0: never)
     const float *__restrict__ vnn;  // as StageArgs::vnn
@@ -222,13 +233,20 @@ struct RingArgs {
 template <int MORD, int NGP>
 hipError_t launch_stage_mg(int flux, bool gen, const StageArgs &a, int nblocks, hipStream_t s);
 template <int MORD, int NGP>
-const void *mood_kernel_mg(int flux, bool gen);
+const void *mood_kernel_mg(int flux, bool gen, bool cooperative);
 
 // flux: numFlux id (kHLLC, kLLF, kHLL, kUnsplit); gen: the general boundary
 // kernels (formula boundaries, or problem RT's source term) instead of the
 // copy-boundary ones
 hipError_t launch_stage(int mord, int ngp, int flux, bool gen, const StageArgs &a, int nblocks, hipStream_t s);
-const void *mood_kernel(int mord, int ngp, int flux, bool gen);
+// The MOOD correction kernel: cooperative (every pass of a stage in one
+// launch, grid barriers between passes) or the per-launch form (one pass, or
+// one run of workgroup-0 passes, per launch; launches after the last pass
+// return at once).  Both take one MoodArgs.
+const void *mood_kernel(int mord, int ngp, int flux, bool gen, bool cooperative);
+// The stage's MOOD statistics after per-launch passes (the cooperative kernel
+// records them itself).
+hipError_t launch_mood_finish(const MoodArgs &a, hipStream_t s);
 bool supported(int mord, int ngp);
 hipError_t launch_dt(const DtArgs &a, int nblocks, hipStream_t s);
 hipError_t launch_ring(const RingArgs &a, hipStream_t s);

@@ -365,91 +365,138 @@ struct Scoped {
     static __device__ __forceinline__ int exch(int *p, int v) { return __hip_atomic_exchange(p, v, __ATOMIC_RELAXED, SC); }
 };
 
-// mark_neighbourhood at scope SC.
-template <int SC>
-__device__ __forceinline__ void mark_neighbourhood_sc(int i, int j, int NX, int NY, int stamp, int *mark, int *list,
-                                                      int *cnt)
-{
-    const int ni[5] = {i, i - 1, i, i + 1, i}, nj[5] = {j - 1, j, j, j, j + 1};
-    int old[5];
+// The next pass's list of a grid-wide pass: the 5-point neighbourhood of a
+// failing cell, each cell listed once (deduplicated by the pass stamp in the
+// global mark array; DetCell, mod_subroutine_MOOD.f90:165-169).
+struct GridMark {
+    int *mark, *list, *cnt;
+    int stamp;
+    __device__ __forceinline__ void operator()(int i, int j, int NX, int NY) const
+    {
+        constexpr int SC = __HIP_MEMORY_SCOPE_AGENT;
+        const int ni[5] = {i, i - 1, i, i + 1, i}, nj[5] = {j - 1, j, j, j, j + 1};
+        int old[5];
 #pragma unroll
-    for (int k = 0; k < 5; ++k) {
-        const bool in = !(ni[k] < 0 || ni[k] >= NX || nj[k] < 0 || nj[k] >= NY);
-        old[k] = in ? Scoped<SC>::exch(&mark[nj[k] * NX + ni[k]], stamp) : stamp;
+        for (int k = 0; k < 5; ++k) {
+            const bool in = !(ni[k] < 0 || ni[k] >= NX || nj[k] < 0 || nj[k] >= NY);
+            old[k] = in ? Scoped<SC>::exch(&mark[nj[k] * NX + ni[k]], stamp) : stamp;
+        }
+#pragma unroll
+        for (int k = 0; k < 5; ++k)
+            if (old[k] != stamp) list[Scoped<SC>::add(cnt, 1)] = nj[k] * NX + ni[k];
     }
-#pragma unroll
-    for (int k = 0; k < 5; ++k)
-        if (old[k] != stamp) list[Scoped<SC>::add(cnt, 1)] = nj[k] * NX + ni[k];
-}
+};
 
-// MOOD correction passes 1, 2, ... of one RK stage (mod_FE.f90:69-174), as a
-// cooperative persistent kernel with one grid barrier per pass.  Pass p
-// recomputes the cells of its list from scratch with the orders they had when
-// the pass began; a cell that fails takes the cascade at once (version p) and
-// flags its neighbourhood for pass p+1.  Each cell is handled by G lanes, one
-// per (face, Gauss point): the lanes reconstruct and solve their Riemann
-// problem, then cross-lane shuffles form the quadrature sum per face and the
-// four-face update in the reference's order on the cell's first lane.
+// The same for the passes workgroup 0 runs alone: the list lives in LDS and
+// an LDS hash set (open addressing, -1 = empty, cleared before each pass)
+// deduplicates it, so these passes make no global atomic round trips.
+constexpr int kSoloHash = 1024;  // slots; a solo pass lists at most 5 * kSoloMax cells
+struct SoloMark {
+    int *hash, *list, *cnt;
+    __device__ __forceinline__ void operator()(int i, int j, int NX, int NY) const
+    {
+        const int ni[5] = {i, i - 1, i, i + 1, i}, nj[5] = {j - 1, j, j, j, j + 1};
+#pragma unroll
+        for (int k = 0; k < 5; ++k) {
+            if (ni[k] < 0 || ni[k] >= NX || nj[k] < 0 || nj[k] >= NY) continue;
+            const int d = nj[k] * NX + ni[k];
+            unsigned h = ((unsigned)d * 0x9E3779B1u) >> 22;  // 10 bits: kSoloHash slots
+            for (;;) {
+                const int old = atomicCAS(&hash[h], -1, d);
+                if (old == -1) {
+                    list[atomicAdd(cnt, 1)] = d;
+                    break;
+                }
+                if (old == d) break;
+                h = (h + 1) & (kSoloHash - 1);
+            }
+        }
+    }
+};
+static_assert(5 * kSoloMax <= kSoloHash / 2, "the solo hash set must stay at most half full");
+
+// One MOOD correction pass (mod_FE.f90:69-174) over a list of cells, shared
+// by the cooperative kernel and the per-pass launches.  Pass p recomputes the
+// cells of its list from scratch with the orders they had when the pass
+// began; a cell that fails takes the cascade at once (version p) and flags its
+// neighbourhood for pass p+1.  Each cell is handled by G lanes, one per (face,
+// Gauss point): the lanes reconstruct and solve their Riemann problem, then
+// cross-lane shuffles form the quadrature sum per face and the four-face
+// update in the reference's order on the cell's first lane.
 //
 // A pass is a chain of dependent memory round trips, so the loads that do not
 // depend on the staged neighbourhood (the face's two order words, the cell's
 // own order word and step-start state, the next trip's list entry) are
 // issued together with the neighbourhood loads.
 template <int MORD, int NGP, int FLUX, bool GEN>
-__global__ __launch_bounds__(kMoodThreads) void k_mood(MoodArgs a)
-{
-    constexpr int NGPP = NGP <= 1 ? 1 : NGP <= 2 ? 2 : NGP <= 4 ? 4 : 8;
-    constexpr int G = 4 * NGPP;
+struct MoodPass {
+    static constexpr int NGPP = NGP <= 1 ? 1 : NGP <= 2 ? 2 : NGP <= 4 ? 4 : 8;
+    static constexpr int G = 4 * NGPP;
     static_assert(64 % G == 0, "lanes per cell must divide the wavefront");
-    constexpr int DEV = __HIP_MEMORY_SCOPE_AGENT, WG = __HIP_MEMORY_SCOPE_WORKGROUP;
-    cg::grid_group grid = cg::this_grid();
-    const int NX = a.B.NX, NY = a.B.NY;
-    const int ncell = NX * NY;
-    const int gtid = blockIdx.x * blockDim.x + threadIdx.x;
-    const int gsz = gridDim.x * blockDim.x;
-    const int lane = __lane_id();
-    const int gl = lane % G, gbase = lane - gl;
-    const int f = gl / NGPP, jg = gl % NGPP;
-    const int stamp_base = *a.stamp;
-    const DevTables *__restrict__ T = a.T;
-    const MoodParams P{a.dmp, a.u2, a.u2tol, a.dx, a.dy, a.recon};
-    const double dtdx = a.tstate[2], dtdy = a.tstate[3];
-    const double dtg = GEN ? a.tstate[1] * kGravity : 0.0;
-    const double xmin_top = GEN ? stage_xmin_top(a.B, a.tstate, a.C.trk) : 0.0;
-    // face f of a cell (i, j): 0 bottom y-face, 1 left x-face, 2 right x-face, 3 top y-face
-    const int fdir = (f == 1 || f == 2) ? 0 : 1;
-    const int fa = fdir == 0 ? kR : kT, fb = fdir == 0 ? kL : kB;
-    const int oai = f == 1 ? -1 : 0, oaj = f == 0 ? -1 : 0;
-    const int obi = f == 2 ? 1 : 0, obj = f == 3 ? 1 : 0;
     // Each cell's (2H+1)^2 neighbourhood of the stage input is staged in LDS by
     // its G lanes (all loads in flight at once); reconstruction and detection
     // then read LDS.  H covers the stencils of the cell and its four
     // neighbours and the detection's 2-cell reach.
-    constexpr int HB = kHalo<MORD>;
-    constexpr int BOX = 2 * HB + 1;
-    constexpr int CPB = kMoodThreads / G;  // cells per workgroup per trip
-    __shared__ double4 nb[CPB][BOX * BOX];
-    const int cs = threadIdx.x / G;
-    const LdsAcc lacc{&nb[cs][0], BOX};
-    // lists of the passes workgroup 0 runs alone (a failing cell adds at most
-    // five cells, so a list of at most kSoloMax cells yields at most 5x)
-    __shared__ int s_list[2][5 * kSoloMax];
-    __shared__ int s_cnt[2];
-    __shared__ int s_fail;
-    // One pass over the nS cells of list S, spread over the lanes
-    // [t0, t0 + nthr) of the grid (all workgroups) or of workgroup 0 alone;
-    // d0 is this lane's entry of the first trip (loaded with the count).
-    // Sn and cnt_next receive the next pass's list, cnt_fail counts the cells
-    // that fail.
-    auto run_pass = [&](auto sc, int p, int nS, int d0, int t0, int nthr, const int *S, int *Sn, int *cnt_next,
-                        int *cnt_fail) {
-        constexpr int SC = decltype(sc)::value;
+    static constexpr int HB = kHalo<MORD>;
+    static constexpr int BOX = 2 * HB + 1;
+    static constexpr int CPB = kMoodThreads / G;  // cells per workgroup per trip
+    struct Lds {
+        double4 nb[CPB][BOX * BOX];
+        // lists of the passes workgroup 0 runs alone (a failing cell adds at
+        // most five cells, so a list of at most kSoloMax cells yields at most 5x)
+        int list[2][5 * kSoloMax];
+        int hash[kSoloHash];
+        int cnt[2];
+        int fail;
+    };
+
+    const MoodArgs &a;
+    Lds &L;
+    int NX, NY, lane, gl, gbase, f, jg, fdir, fa, fb, oai, oaj, obi, obj, cs, stamp_base;
+    MoodParams P;
+    double dtdx, dtdy, dtg, xmin_top;
+
+    __device__ __forceinline__ MoodPass(const MoodArgs &args, Lds &lds) : a(args), L(lds)
+    {
+        NX = a.B.NX;
+        NY = a.B.NY;
+        lane = __lane_id();
+        gl = lane % G;
+        gbase = lane - gl;
+        f = gl / NGPP;
+        jg = gl % NGPP;
+        // face f of a cell (i, j): 0 bottom y-face, 1 left x-face, 2 right x-face, 3 top y-face
+        fdir = (f == 1 || f == 2) ? 0 : 1;
+        fa = fdir == 0 ? kR : kT;
+        fb = fdir == 0 ? kL : kB;
+        oai = f == 1 ? -1 : 0;
+        oaj = f == 0 ? -1 : 0;
+        obi = f == 2 ? 1 : 0;
+        obj = f == 3 ? 1 : 0;
+        cs = threadIdx.x / G;
+        stamp_base = *a.stamp;
+        P = MoodParams{a.dmp, a.u2, a.u2tol, a.dx, a.dy, a.recon};
+        dtdx = a.tstate[2];
+        dtdy = a.tstate[3];
+        dtg = GEN ? a.tstate[1] * kGravity : 0.0;
+        xmin_top = GEN ? stage_xmin_top(a.B, a.tstate, a.C.trk) : 0.0;
+    }
+
+    // One pass over the nS cells of list S, spread over the lanes [t0, t0 +
+    // nthr) of the grid (all workgroups) or of workgroup 0 alone; d0 is this
+    // lane's entry of the first trip (loaded with the count).  cnt_fail counts
+    // the cells that fail; mark lists their neighbourhoods for pass p+1.
+    template <int SC, class Mark>
+    __device__ __forceinline__ void run(int p, int nS, int d0, int t0, int nthr, const int *S, int *cnt_fail,
+                                        const Mark &mark)
+    {
         using M = Scoped<SC>;
+        const DevTables *__restrict__ T = a.T;
+        const LdsAcc lacc{&L.nb[cs][0], BOX};
         auto order_of = [&](int ii, int jj) -> int {
             if (ii < 0 || ii >= NX || jj < 0 || jj >= NY) return MORD;  // ghost cells keep Mord
             return order_at_pass(M::load(&a.gpo[(size_t)jj * NX + ii]), p);
         };
-        const int stamp_next = stamp_base + p + 1;
         const int per_iter = nthr / G;
         const int niter = (nS + per_iter - 1) / per_iter;
         int dnext = d0;
@@ -471,7 +518,7 @@ __global__ __launch_bounds__(kMoodThreads) void k_mood(MoodArgs a)
             if (it + 1 < niter && tn < nS) dnext = S[tn];
             __syncthreads();  // the previous trip's readers are done with nb
             for (int q = gl; q < BOX * BOX; q += G)
-                nb[cs][q] = load_stage<GEN>(a.Uin, a.B, i - HB + q % BOX, j - HB + q / BOX, xmin_top);
+                L.nb[cs][q] = load_stage<GEN>(a.Uin, a.B, i - HB + q % BOX, j - HB + q / BOX, xmin_top);
             __syncthreads();
             double F[4] = {0.0, 0.0, 0.0, 0.0};
             if (valid && jg < NGP) {
@@ -485,21 +532,21 @@ __global__ __launch_bounds__(kMoodThreads) void k_mood(MoodArgs a)
                                              recon, va1, va2, vb1, vb2);
             }
             // quadrature over the face's Gauss points, j = 1..ngp from 0
-            double L[4] = {0.0, 0.0, 0.0, 0.0};
+            double Lf[4] = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
             for (int jj = 0; jj < NGP; ++jj) {
                 const int src = gbase + f * NGPP + jj;
                 const double w = T->gw[jj];
 #pragma unroll
-                for (int k = 0; k < 4; ++k) L[k] = L[k] + __shfl(F[k], src) * w;
+                for (int k = 0; k < 4; ++k) Lf[k] = Lf[k] + __shfl(F[k], src) * w;
             }
             double fyb[4], fxl[4], fxr[4], fyt[4];
 #pragma unroll
             for (int k = 0; k < 4; ++k) {
-                fyb[k] = __shfl(L[k], gbase);
-                fxl[k] = __shfl(L[k], gbase + NGPP);
-                fxr[k] = __shfl(L[k], gbase + 2 * NGPP);
-                fyt[k] = __shfl(L[k], gbase + 3 * NGPP);
+                fyb[k] = __shfl(Lf[k], gbase);
+                fxl[k] = __shfl(Lf[k], gbase + NGPP);
+                fxr[k] = __shfl(Lf[k], gbase + 2 * NGPP);
+                fyt[k] = __shfl(Lf[k], gbase + 3 * NGPP);
             }
             if (valid && gl == 0) {
                 double cand[4];
@@ -512,83 +559,165 @@ __global__ __launch_bounds__(kMoodThreads) void k_mood(MoodArgs a)
                 if (dec) {
                     M::store(&a.gpo[d], pack_order(lower_order(me, a.cascade), me, p));
                     M::add(cnt_fail, 1);
-                    mark_neighbourhood_sc<SC>(i, j, NX, NY, stamp_next, a.mark, Sn, cnt_next);
+                    mark(i, j, NX, NY);
                 }
             }
         }
-    };
-    using Dev = std::integral_constant<int, DEV>;
-    using Wg = std::integral_constant<int, WG>;
+    }
+
+    // Short lists (the long tail of a failure chain): workgroup 0 runs the
+    // passes from p on alone while the list has 1..solo_max cells, with
+    // workgroup barriers, the lists, counts and deduplication in LDS and
+    // workgroup-scope memory operations, which cost far less than grid
+    // barriers and device-scope round trips.  Same passes, same order of
+    // effects: the results are those of the grid-wide path.  The current list
+    // goes back to global memory with its count; the stage's counters record
+    // the passes run.  Returns the first pass not run.
+    __device__ __forceinline__ int solo(int p, int nS)
+    {
+        constexpr int WG = __HIP_MEMORY_SCOPE_WORKGROUP;
+        const int tid = threadIdx.x;
+        for (int k = tid; k < nS; k += blockDim.x) L.list[p & 1][k] = a.slist[p & 1][k];
+        if (tid == 0) L.cnt[p & 1] = nS;
+        int q = p, second = 0, det = 0;
+        for (;;) {
+            __syncthreads();
+            const int m = L.cnt[q & 1];
+            if (m == 0 || m > a.solo_max) break;
+            for (int k = tid; k < kSoloHash; k += blockDim.x) L.hash[k] = -1;
+            if (tid == 0) {
+                L.cnt[(q + 1) & 1] = 0;
+                L.fail = 0;
+            }
+            __syncthreads();  // every lane has read the count; the set and the next count are clear
+            second += m;
+            const int e0 = tid / G < m ? L.list[q & 1][tid / G] : 0;
+            run<WG>(q, m, e0, tid, blockDim.x, L.list[q & 1], &L.fail,
+                    SoloMark{L.hash, L.list[(q + 1) & 1], &L.cnt[(q + 1) & 1]});
+            __syncthreads();
+            det += L.fail;
+            ++q;
+        }
+        // hand the current list back to the grid-wide path
+        const int m = L.cnt[q & 1];
+        for (int k = tid; k < m; k += blockDim.x) a.slist[q & 1][k] = L.list[q & 1][k];
+        if (tid == 0) {
+            a.cnt[kCntS0 + q % 3] = m;
+            a.cnt[kCntS0 + (q + 1) % 3] = 0;
+            a.cnt[kCntNext] = q;
+            a.cnt[kCntSecond] += second;
+            a.cnt[kCntDet] += det;
+        }
+        return q;
+    }
+
+    // Grid-wide pass p over the nS cells of slist[p & 1]: all lanes [0, gsz).
+    __device__ __forceinline__ void grid_pass(int p, int nS, int d0, int gtid, int gsz)
+    {
+        if (gtid == 0) {
+            a.cnt[kCntS0 + (p + 2) % 3] = 0;  // last read at the start of pass p - 1
+            a.cnt[kCntSecond] += nS;
+            a.cnt[kCntNext] = p + 1;
+        }
+        run<__HIP_MEMORY_SCOPE_AGENT>(p, nS, d0, gtid, gsz, a.slist[p & 1], &a.cnt[kCntDet],
+                                      GridMark{a.mark, a.slist[(p + 1) & 1], &a.cnt[kCntS0 + (p + 1) % 3],
+                                               stamp_base + p + 1});
+    }
+};
+
+// The stage's statistics once its MOOD loop ended at pass p (the first pass
+// with an empty list): passes, detections and recomputed cells, and the mark
+// stamp base of the next stage.
+__device__ __forceinline__ void mood_finish(const MoodArgs &a, int p)
+{
+    *a.stamp = *a.stamp + p + 2;
+    const long long det = (long long)a.cnt[kCntFail0] + a.cnt[kCntDet];
+    DevStats *st = a.stats;
+    st->passes[a.stage_idx] = p;
+    if (p > st->max_passes) st->max_passes = p;
+    st->detected_step += det;
+    st->detected_total += det;
+    st->second_pass_cells += a.cnt[kCntSecond];
+    // every pass but the last lowered at least one cell
+    if (a.nn_passes && p > 0) {
+        st->nn_passes_step += p - 1;
+        st->nn_passes_total += p - 1;
+    }
+}
+
+// MOOD correction passes 1, 2, ... of one RK stage as a cooperative
+// persistent kernel with one grid barrier per pass; the loop ends on the
+// device when a pass finds no cell to recompute.
+template <int MORD, int NGP, int FLUX, bool GEN>
+__global__ __launch_bounds__(kMoodThreads) void k_mood(MoodArgs a)
+{
+    using MP = MoodPass<MORD, NGP, FLUX, GEN>;
+    __shared__ typename MP::Lds lds;
+    MP mp(a, lds);
+    cg::grid_group grid = cg::this_grid();
+    const int ncell = a.B.NX * a.B.NY;
+    const int gtid = blockIdx.x * blockDim.x + threadIdx.x;
+    const int gsz = gridDim.x * blockDim.x;
     int p = 1;
-    long long second = 0;
     for (;;) {
         // the pass's count and this lane's first list entry, in one round trip
-        const int nS = load_relaxed(&a.cnt_s[p]);
-        const int d0 = gtid / G < ncell ? a.slist[p & 1][gtid / G] : 0;
+        const int nS = load_relaxed(&a.cnt[kCntS0 + p % 3]);
+        const int d0 = gtid / MP::G < ncell ? a.slist[p & 1][gtid / MP::G] : 0;
         if (nS == 0) break;
-        if (p + 1 >= a.pmax) {
-            if (gtid == 0) atomicOr(a.err, 1);
-            break;
-        }
         if (nS <= a.solo_max) {
-            // Short lists (the long tail of a failure chain): workgroup 0 runs
-            // the passes alone, with workgroup barriers, the lists and counts
-            // in LDS and workgroup-scope memory operations, which cost far
-            // less than grid barriers and device-scope round trips; the other
-            // workgroups wait at the next grid barrier.  Same passes, same
-            // order of effects: the results are those of the grid-wide path.
-            if (blockIdx.x == 0) {
-                for (int k = threadIdx.x; k < nS; k += blockDim.x) s_list[p & 1][k] = a.slist[p & 1][k];
-                if (threadIdx.x == 0) s_cnt[p & 1] = nS;
-                int q = p;
-                for (;;) {
-                    __syncthreads();
-                    const int m = s_cnt[q & 1];
-                    if (m == 0 || m > a.solo_max || q + 1 >= a.pmax) break;
-                    __syncthreads();  // every lane has read the count
-                    if (threadIdx.x == 0) { s_cnt[(q + 1) & 1] = 0; s_fail = 0; }
-                    __syncthreads();
-                    second += m;
-                    const int e0 = (int)threadIdx.x / G < m ? s_list[q & 1][threadIdx.x / G] : 0;
-                    run_pass(Wg{}, q, m, e0, threadIdx.x, blockDim.x, s_list[q & 1], s_list[(q + 1) & 1],
-                             &s_cnt[(q + 1) & 1], &s_fail);
-                    __syncthreads();
-                    if (threadIdx.x == 0) a.cnt_f[q] = s_fail;
-                    ++q;
-                }
-                // hand the current list back to the grid-wide path
-                const int m = s_cnt[q & 1];
-                for (int k = threadIdx.x; k < m; k += blockDim.x) a.slist[q & 1][k] = s_list[q & 1][k];
-                if (threadIdx.x == 0) {
-                    a.cnt_s[q] = m;
-                    __hip_atomic_store(a.solo_p, q, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                }
-            }
+            // workgroup 0 runs the short-list passes; the others wait here
+            if (blockIdx.x == 0) mp.solo(p, nS);
             grid.sync();
-            p = load_relaxed(a.solo_p);
+            p = load_relaxed(&a.cnt[kCntNext]);
             continue;
         }
-        second += nS;
-        run_pass(Dev{}, p, nS, d0, gtid, gsz, a.slist[p & 1], a.slist[(p + 1) & 1], &a.cnt_s[p + 1], &a.cnt_f[p]);
+        mp.grid_pass(p, nS, d0, gtid, gsz);
         grid.sync();
         ++p;
     }
-    if (gtid == 0) {
-        *a.stamp = stamp_base + p + 2;
-        long long det = 0;
-        for (int q = 0; q < p; ++q) det += a.cnt_f[q];
-        DevStats *st = a.stats;
-        st->passes[a.stage_idx] = p;
-        if (p > st->max_passes) st->max_passes = p;
-        st->detected_step += det;
-        st->detected_total += det;
-        st->second_pass_cells += second;
-        // every pass but the last lowered at least one cell
-        if (a.nn_passes && p > 0) {
-            st->nn_passes_step += p - 1;
-            st->nn_passes_total += p - 1;
-        }
+    if (gtid == 0) mood_finish(a, p);
+}
+
+// The same passes without a cooperative launch: each launch runs the current
+// pass (all workgroups) or one run of workgroup-0 passes, and its last
+// workgroup to finish publishes the next pass.  A launch whose pass has no
+// cell returns at once, so the host launches in batches and checks
+// kCntPending between them.
+template <int MORD, int NGP, int FLUX, bool GEN>
+__global__ __launch_bounds__(kMoodThreads) void k_mood_pass(MoodArgs a)
+{
+    using MP = MoodPass<MORD, NGP, FLUX, GEN>;
+    __shared__ typename MP::Lds lds;
+    __shared__ int s_p, s_n;
+    __shared__ bool s_last;
+    if (threadIdx.x == 0) {
+        s_p = load_relaxed(&a.cnt[kCntPass]) + 1;
+        s_n = load_relaxed(&a.cnt[kCntS0 + s_p % 3]);
     }
+    __syncthreads();
+    const int p = s_p, nS = s_n;
+    if (nS == 0) return;
+    MP mp(a, lds);
+    const int ncell = a.B.NX * a.B.NY;
+    const int gtid = blockIdx.x * blockDim.x + threadIdx.x;
+    if (nS <= a.solo_max) {
+        if (blockIdx.x == 0) mp.solo(p, nS);
+    } else {
+        const int d0 = gtid / MP::G < ncell ? a.slist[p & 1][gtid / MP::G] : 0;
+        mp.grid_pass(p, nS, d0, gtid, gridDim.x * blockDim.x);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        __threadfence();
+        s_last = atomicAdd(&a.cnt[kCntDone], 1) == (int)gridDim.x - 1;
+    }
+    __syncthreads();
+    if (!s_last || threadIdx.x != 0) return;
+    __threadfence();
+    const int q = load_relaxed(&a.cnt[kCntNext]);
+    a.cnt[kCntPass] = q - 1;
+    a.cnt[kCntPending] = load_relaxed(&a.cnt[kCntS0 + q % 3]) > 0;
+    a.cnt[kCntDone] = 0;
 }
 
 // Launch entry points of one (Mord, ngp) combination, for every numFlux and
@@ -691,10 +820,13 @@ __global__ __launch_bounds__(128) void k_ring_face(RingArgs a)
         }                                                                                                     \
     }                                                                                                         \
     template <int F>                                                                                          \
-    static const void *mood_kernel_f##M##G(bool gen)                                                          \
+    static const void *mood_kernel_f##M##G(bool gen, bool coop)                                               \
     {                                                                                                         \
-        return gen ? reinterpret_cast<const void *>(&k_mood<M, G, F, true>)                                  \
-                   : reinterpret_cast<const void *>(&k_mood<M, G, F, false>);                                \
+        if (coop)                                                                                             \
+            return gen ? reinterpret_cast<const void *>(&k_mood<M, G, F, true>)                              \
+                       : reinterpret_cast<const void *>(&k_mood<M, G, F, false>);                            \
+        return gen ? reinterpret_cast<const void *>(&k_mood_pass<M, G, F, true>)                             \
+                   : reinterpret_cast<const void *>(&k_mood_pass<M, G, F, false>);                           \
     }                                                                                                         \
     template <>                                                                                               \
     hipError_t launch_stage_mg<M, G>(int flux, bool gen, const StageArgs &a, int nblocks, hipStream_t s)     \
@@ -706,12 +838,12 @@ __global__ __launch_bounds__(128) void k_ring_face(RingArgs a)
         return hipGetLastError();                                                                             \
     }                                                                                                         \
     template <>                                                                                               \
-    const void *mood_kernel_mg<M, G>(int flux, bool gen)                                                      \
+    const void *mood_kernel_mg<M, G>(int flux, bool gen, bool coop)                                           \
     {                                                                                                         \
-        if (flux == kHLL) return mood_kernel_f##M##G<kHLL>(gen);                                             \
-        if (flux == kLLF) return mood_kernel_f##M##G<kLLF>(gen);                                             \
-        if (flux == kUnsplit) return mood_kernel_f##M##G<kUnsplit>(gen);                                     \
-        return mood_kernel_f##M##G<kHLLC>(gen);                                                              \
+        if (flux == kHLL) return mood_kernel_f##M##G<kHLL>(gen, coop);                                       \
+        if (flux == kLLF) return mood_kernel_f##M##G<kLLF>(gen, coop);                                       \
+        if (flux == kUnsplit) return mood_kernel_f##M##G<kUnsplit>(gen, coop);                               \
+        return mood_kernel_f##M##G<kHLLC>(gen, coop);                                                        \
     }                                                                                                         \
     template <>                                                                                               \
     hipError_t launch_ring_face_mg<M, G>(int flux, const RingArgs &a, hipStream_t s)                         \

@@ -184,8 +184,10 @@ int gpm_time_stepping(gpm_ctx *ctx, double *Uin, double *Uout, double t, double 
 
 /* Device-resident driver loop of main.f90:145-162 for nsteps steps:
  * dtfinal = tmax - t; Setdt; one step of the integrator; t = t + dt.  No host
- * round trip per step.  If dts is non-NULL it receives the nsteps time steps
- * (host memory). */
+ * round trip per step (with cooperative MOOD launches, gpm_set_mood_launch).
+ * Returns when the steps are done.  If dts is non-NULL it receives the
+ * nsteps time steps (host memory).  A MOOD stage may run any number of
+ * passes. */
 int gpm_run(gpm_ctx *ctx, int32_t nsteps, double tmax, double *dts);
 
 int gpm_get_stats(gpm_ctx *ctx, gpm_stats *stats);
@@ -194,6 +196,19 @@ int gpm_set_time(gpm_ctx *ctx, double t);
 
 /* The HIP stream the context enqueues on (a hipStream_t, as void*). */
 void *gpm_stream(gpm_ctx *ctx);
+
+/* How the MOOD correction passes of a stage (the MOOD loop of mod_FE.f90:69-174
+ * after its first pass) are launched.  cooperative = 1 (the default): one
+ * cooperative launch per stage runs every pass on the device, with grid
+ * barriers between passes.  cooperative = 0: per-pass launches in growing
+ * batches, with the stage's pending flag read back between batches (one
+ * host synchronisation per batch).  The results are the same bit for bit.
+ * The library switches to per-pass launches by itself when the device
+ * refuses a cooperative launch (one process at a time holds that resource),
+ * and the environment variable GPM_MOOD_COOP=0 selects them at gpm_create.
+ * No counterpart in the reference (its MOOD loop is a host loop). */
+int gpm_set_mood_launch(gpm_ctx *ctx, int32_t cooperative);
+int gpm_get_mood_launch(gpm_ctx *ctx, int32_t *cooperative);
 
 /* Launch shape and timing of the dominant kernel, for roofline reporting:
  * average duration in ms of the first-pass stage kernel over the launches

@@ -57,3 +57,43 @@ def test_8192_shear_layer_plain_and_nn(gpm):
     print(f"  density L1 NN vs plain {l1:.2e}", flush=True)
     assert l1 < 1e-3
     m.close()
+
+
+def test_8192_four_quadrant_r2_long_chains(gpm):
+    """8192^2 GP-R2 four-quadrant problem (outflow): MOOD failure chains run
+    thousands of passes in a stage, past any fixed pass budget.  Three steps
+    with the cooperative MOOD kernel and with per-pass launches give the same
+    bits; the state stays finite with positive density and pressure; the
+    largest pass count of a stage is reported."""
+    if not has_gpu():
+        pytest.skip("no GPU")
+    import torch
+
+    n, steps = 8192, 3
+    print(f"[case 8192^2 GP-R2 four-quadrant, {steps} steps, both MOOD launch forms]", flush=True)
+    U0 = torch.from_numpy(gpm.ic.make("four_quadrant", n, 0)).cuda()
+    out = {}
+    for coop in (True, False):
+        s = gpm.Solver(gpm.Config(lf=n, Mord=5, ngp=3, BC_type="neumann"))
+        s.set_mood_launch(coop)
+        torch.cuda.synchronize()
+        s.set_state_device(U0.data_ptr())
+        dts = s.run(steps, return_dts=True)
+        U = torch.empty_like(U0)
+        s.get_state_device(U.data_ptr())
+        torch.cuda.synchronize()
+        st = s.stats()
+        print(f"  cooperative={coop}: max_passes {st['max_passes']}, detected {st['detected_total']}, "
+              f"second-pass cells {st['second_pass_cells']}", flush=True)
+        assert s.mood_launch() == coop
+        out[coop] = (dts, U, st)
+        s.close()
+    (d1, U1, s1), (d2, U2, s2) = out[True], out[False]
+    assert np.array_equal(d1, d2)
+    assert bool(torch.equal(U1, U2))
+    assert s1["detected_total"] == s2["detected_total"] and s1["max_passes"] == s2["max_passes"]
+    assert bool(torch.isfinite(U1).all())
+    r = U1[..., 0]
+    p = 0.4 * (U1[..., 3] - 0.5 * (U1[..., 1] ** 2 + U1[..., 2] ** 2) / r)
+    assert float(r.min()) > 0 and float(p.min()) > 0
+    assert s1["max_passes"] > 100

@@ -95,6 +95,28 @@ def test_run_matches_reference(gpm, case):
     assert st["detected_total"] == int(np.sum(z["counts"]))
 
 
+@pytest.mark.parametrize("case", STEP_CASES)
+def test_run_per_pass_launches(gpm, case):
+    """The MOOD correction passes as per-pass launches instead of one
+    cooperative launch per stage (the path taken when the device refuses a
+    cooperative launch): the same dt sequence, counts, mask and state as the
+    reference run."""
+    z = golden(f"step_{case}.npz")
+    nsteps = int(z["nsteps"])
+    print(f"[case {case}] {nsteps} steps via gpm_run, per-pass MOOD launches", flush=True)
+    s = make_solver(gpm, z)
+    s.set_mood_launch(False)
+    s.set_state(z["U0"])
+    s.set_time(0.0)
+    dts = s.run(nsteps, return_dts=True)
+    U, g = s.get_state(order=True)
+    assert not s.mood_launch()
+    assert np.array_equal(dts, z["dt"]), f"dt differs at steps {np.nonzero(dts != z['dt'])[0][:5]}"
+    assert np.array_equal(g[4:-4, 4:-4], z["masks"][-1])
+    assert np.array_equal(U, z["U"]), f"max |diff| {np.abs(U - z['U']).max()}"
+    assert s.stats()["detected_total"] == int(np.sum(z["counts"]))
+
+
 def test_config0_256_300_steps_checksum(gpm):
     """Config-0 workload (256^2, R2/ngp3, Neumann, 300 steps, seeded
     four-quadrant IC): final state and mask checksums equal the reference's."""
