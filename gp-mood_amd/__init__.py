@@ -73,6 +73,22 @@ def boundary_c(config, U, t_rk=0.0):
     return U
 
 
+def run_noalloc(config, tables, u_dev_ptr, nsteps, t=0.0, tmax=1e10, stream=None):
+    """gpm_run_noalloc: nsteps steps of the driver loop on the device state at
+    u_dev_ptr (interior cells, nf*lf*4 float64) without a context and without
+    device allocation.  Returns the final time."""
+    arr = {k: np.ascontiguousarray(tables[k], dtype=np.int32 if k == "ixiy" else np.float64) for k in
+           ("zT", "d2x", "d2y", "gw", "ixiy")}
+    c = config.to_c()
+    t_end = ctypes.c_double()
+    rc = capi.load().gpm_run_noalloc(ctypes.byref(c), *(capi.ptr(arr[k]) for k in ("zT", "d2x", "d2y", "gw", "ixiy")),
+                                     ctypes.c_void_p(u_dev_ptr), int(nsteps), float(t), float(tmax),
+                                     ctypes.byref(t_end), ctypes.c_void_p(stream) if stream else None)
+    if rc != 0:
+        raise GpmError(f"gpm_run_noalloc failed ({rc})")
+    return t_end.value
+
+
 class Solver:
     """One solver context on the current HIP device."""
 

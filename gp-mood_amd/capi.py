@@ -39,7 +39,7 @@ CASCADE_IDS = {"reference": 0, "full": 1}
 EXPORTS = [
     "gpm_create", "gpm_destroy", "gpm_last_error", "gpm_set_gp_tables", "gpm_gp_tables", "gpm_boundary_c",
     "gpm_set_state", "gpm_get_state", "gpm_set_state_device", "gpm_get_state_device",
-    "gpm_compute_dt", "gpm_time_stepping", "gpm_run", "gpm_get_stats", "gpm_reset_stats",
+    "gpm_compute_dt", "gpm_time_stepping", "gpm_run", "gpm_run_noalloc", "gpm_get_stats", "gpm_reset_stats",
     "gpm_set_time", "gpm_stream", "gpm_set_timing", "gpm_kernel_time", "gpm_set_mood_launch", "gpm_get_mood_launch",
     "gpm_mlp_create", "gpm_mlp_destroy", "gpm_mlp_last_error", "gpm_mlp_num_params", "gpm_mlp_set_params",
     "gpm_mlp_get_params", "gpm_mlp_get_grads", "gpm_mlp_forward_backward", "gpm_mlp_adamw_step",
@@ -108,6 +108,7 @@ def load():
         "gpm_compute_dt": ([P, D, ctypes.POINTER(D)], I),
         "gpm_time_stepping": ([P, P, P, D, D, P, ctypes.POINTER(GpmStats)], I),
         "gpm_run": ([P, I, D, P], I),
+        "gpm_run_noalloc": ([ctypes.POINTER(GpmConfig), P, P, P, P, P, P, I, D, D, ctypes.POINTER(D), P], I),
         "gpm_get_stats": ([P, ctypes.POINTER(GpmStats)], I),
         "gpm_reset_stats": ([P], I),
         "gpm_set_time": ([P, D], I),

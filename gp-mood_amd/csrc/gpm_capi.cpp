@@ -12,6 +12,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -559,6 +560,58 @@ int enqueue_stages(gpm_ctx *c)
 bool square_copy_rk3(const gpm_ctx *c)
 {
     return c->NX == c->NY && index_map_bc(c->bc) && !c->gravity && c->integrator == GPM_SSP_RK3;
+}
+
+// ---- allocation-free runs (gpm_run_noalloc) ---------------------------------
+// Every device buffer of such a run lives in `scratch`, one static array per
+// GPU, laid out as below; its runs are limited to kNaMaxCells cells.
+constexpr size_t kNaMaxCells = (size_t)1 << 20;
+constexpr int kNaMaxDtBlocks = 1024;
+constexpr size_t na_align(size_t x) { return (x + 255) / 256 * 256; }
+constexpr size_t kNaTables = 0;
+constexpr size_t kNaCounters = na_align(kNaTables + sizeof(DevTables));
+constexpr size_t kNaStats = na_align(kNaCounters + sizeof(int) * kMaxStages * kCountersPerStage);
+constexpr size_t kNaTstate = na_align(kNaStats + sizeof(DevStats));
+constexpr size_t kNaBlockMin = na_align(kNaTstate + 8 * sizeof(double));
+constexpr size_t kNaDone = na_align(kNaBlockMin + kNaMaxDtBlocks * sizeof(double));
+constexpr size_t kNaStamp = na_align(kNaDone + sizeof(unsigned));
+constexpr size_t kNaA = na_align(kNaStamp + sizeof(int));
+constexpr size_t kNaB = kNaA + kNaMaxCells * sizeof(double4);
+constexpr size_t kNaGpo = kNaB + kNaMaxCells * sizeof(double4);
+constexpr size_t kNaListA = kNaGpo + kNaMaxCells * sizeof(order_t);
+constexpr size_t kNaListB = kNaListA + kNaMaxCells * sizeof(int);
+constexpr size_t kNaMark = kNaListB + kNaMaxCells * sizeof(int);
+constexpr size_t kNaBytes = kNaMark + kNaMaxCells * sizeof(int);
+
+}  // namespace
+
+// The scratch memory of gpm_run_noalloc (kNaBytes, about 80 MiB).  In host code
+// this name is not a device address: host code uses the address that
+// hipGetSymbolAddress returns for the current device (na_scratch).
+static __device__ __align__(256) unsigned char scratch[kNaBytes];
+
+namespace {
+
+constexpr int kNaMaxDevices = 64;
+std::mutex g_na_mutex;
+unsigned char *g_na_ptr[kNaMaxDevices] = {};
+
+// Device address of `scratch` on device `dev` (the current device), looked up
+// once per device.
+int na_scratch(int dev, unsigned char **out)
+{
+    std::lock_guard<std::mutex> lock(g_na_mutex);
+    if (!g_na_ptr[dev]) {
+        void *p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p) {
+            std::fprintf(stderr, "gpm_run_noalloc: hipGetSymbolAddress: %s\n", hipGetErrorString(e));
+            return GPM_EHIP;
+        }
+        g_na_ptr[dev] = static_cast<unsigned char *>(p);
+    }
+    *out = g_na_ptr[dev];
+    return GPM_OK;
 }
 
 }  // namespace
@@ -1325,6 +1378,101 @@ int gpm_adjoint(gpm_ctx *c, const double *stage_states_dev, const int8_t *masks_
     if (cur != reinterpret_cast<double4 *>(ubar_dev))
         HIPCHK(c, hipMemcpyAsync(ubar_dev, cur, n * sizeof(double4), hipMemcpyDeviceToDevice, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
+    return GPM_OK;
+}
+
+int gpm_run_noalloc(const gpm_config *cfg, const double *zT_sp, const double *gp_d2x_sp, const double *gp_d2y_sp,
+                    const double *gauss_weight, const int32_t *ixiy_sp, double *u_dev, int32_t nsteps, double t,
+                    double tmax, double *t_end, void *stream)
+{
+    auto bad = [](int code, const char *m) {
+        std::fprintf(stderr, "gpm_run_noalloc: %s\n", m);
+        return code;
+    };
+    if (!cfg || !zT_sp || !gp_d2x_sp || !gp_d2y_sp || !gauss_weight || !ixiy_sp || !u_dev || nsteps < 0)
+        return bad(GPM_EINVAL, "null argument or negative nsteps");
+    if (cfg->lf < 8 || cfg->nf < 8) return bad(GPM_EINVAL, "grids need lf, nf >= 8");
+    if ((long long)cfg->lf * cfg->nf > (long long)kNaMaxCells)
+        return bad(GPM_EINVAL, "lf * nf exceeds the 1048576 cells of the scratch buffer");
+    if (cfg->method != GPM_GP_MOOD) return bad(GPM_EUNSUP, "method must be GP_MOOD");
+    if (!index_map_bc(cfg->bc_type) || cfg->problem == GPM_PROBLEM_RT)
+        return bad(GPM_EUNSUP, "needs a copy boundary (Neumann, Periodic, Reflective, KH_BC) and no source term");
+    if (cfg->integrator != GPM_FE && cfg->integrator != GPM_SSP_RK2 && cfg->integrator != GPM_SSP_RK3)
+        return bad(GPM_EUNSUP, "integrator must be FE, SSP_RK2 or SSP_RK3");
+    if (cfg->num_flux != GPM_HLLC && cfg->num_flux != GPM_LLF && cfg->num_flux != GPM_HLL &&
+        cfg->num_flux != GPM_UNSPLIT)
+        return bad(GPM_EUNSUP, "numFlux must be HLLC, LLF, HLL or unsplit");
+    if (cfg->cascade != GPM_CASCADE_REFERENCE && cfg->cascade != GPM_CASCADE_FULL)
+        return bad(GPM_EINVAL, "unknown cascade");
+    if (!supported(cfg->mord, cfg->ngp)) return bad(GPM_EUNSUP, "(Mord, ngp) combination not compiled");
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kNaMaxDevices) return bad(GPM_EHIP, "no usable device");
+    unsigned char *base = nullptr;
+    if (int rc = na_scratch(dev, &base)) return rc;
+
+    // A context on the stack whose buffers are carved from the scratch memory.
+    gpm_ctx c{};
+    c.cfg = *cfg;
+    c.NX = cfg->lf; c.NY = cfg->nf; c.N = cfg->lf;
+    c.mord = cfg->mord; c.ngp = cfg->ngp; c.bc = cfg->bc_type; c.integrator = cfg->integrator;
+    c.device = dev;
+    c.nstages = cfg->integrator == GPM_FE ? 1 : cfg->integrator == GPM_SSP_RK2 ? 2 : 3;
+    c.dx = quad_ratio(cfg->lx, cfg->lf);
+    c.dy = quad_ratio(cfg->ly, cfg->nf);
+    c.Bh = make_bnd(c.NX, c.NY, c.bc, c.dx, c.dy, c.st_host);
+    c.B = c.Bh;
+    c.B.st = nullptr;  // copy boundaries read no fixed ghost state
+    c.stream = static_cast<hipStream_t>(stream);
+    c.dT = reinterpret_cast<DevTables *>(base + kNaTables);
+    c.counters = reinterpret_cast<int *>(base + kNaCounters);
+    c.stats = reinterpret_cast<DevStats *>(base + kNaStats);
+    c.tstate = reinterpret_cast<double *>(base + kNaTstate);
+    c.block_min = reinterpret_cast<double *>(base + kNaBlockMin);
+    c.done = reinterpret_cast<unsigned *>(base + kNaDone);
+    c.stamp = reinterpret_cast<int *>(base + kNaStamp);
+    c.dA = reinterpret_cast<double4 *>(base + kNaA);
+    c.dB = reinterpret_cast<double4 *>(base + kNaB);
+    c.gpo = reinterpret_cast<order_t *>(base + kNaGpo);
+    c.slistA = reinterpret_cast<int *>(base + kNaListA);
+    c.slistB = reinterpret_cast<int *>(base + kNaListB);
+    c.mark = reinterpret_cast<int *>(base + kNaMark);
+    c.dU = reinterpret_cast<double4 *>(u_dev);
+    int ncu = 0;
+    hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
+    if (ncu <= 0) ncu = 256;
+    c.dt_blocks = ncu < kNaMaxDtBlocks ? ncu : kNaMaxDtBlocks;
+    c.tiles_x = (c.NX + stage_tx(c.mord) - 1) / stage_tx(c.mord);
+    c.ntiles = c.tiles_x * ((c.NY + stage_ty(c.mord) - 1) / stage_ty(c.mord));
+    {
+        int per = 0;
+        const hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(
+            &per, mood_kernel(c.mord, c.ngp, cfg->num_flux, false, true), kMoodThreads, 0);
+        if (e != hipSuccess || per <= 0) per = 1;
+        const int want = ncu / 4 > 0 ? ncu / 4 : 1;
+        c.mood_blocks = want < per * ncu ? want : per * ncu;
+    }
+    c.mood_solo = 16;
+    if (const char *env = std::getenv("GPM_MOOD_COOP")) c.mood_coop = std::atoi(env) != 0;
+    c.has_state = true;
+
+    int rc = upload_tables(&c, zT_sp, gp_d2x_sp, gp_d2y_sp, gauss_weight, ixiy_sp);
+    if (rc) return bad(rc, c.msg.c_str());
+    const size_t n = (size_t)c.NX * c.NY;
+    if (hipError_t e = launch_run_reset(c.mark, c.gpo, n, pack_order(c.mord, c.mord, 0), c.counters,
+                                        kMaxStages * kCountersPerStage, c.stats, c.tstate, t, c.done, c.stamp, c.stream))
+        return bad(GPM_EHIP, hipGetErrorString(e));
+    for (int st = 0; st < nsteps; ++st) {
+        rc = launch_dt_kernel(&c, kDtCompute, 1, tmax, 0.0, 0.0, 1, st, false, 1);
+        if (!rc) rc = enqueue_stages(&c);
+        if (rc) return bad(rc, c.msg.c_str());
+    }
+    // the last step's output lies in u_dev or in a scratch state buffer
+    if (c.dU != reinterpret_cast<double4 *>(u_dev) &&
+        hipMemcpyAsync(u_dev, c.dU, n * sizeof(double4), hipMemcpyDeviceToDevice, c.stream) != hipSuccess)
+        return bad(GPM_EHIP, "copy of the final state");
+    if (t_end && hipMemcpyAsync(t_end, c.tstate, sizeof(double), hipMemcpyDeviceToHost, c.stream) != hipSuccess)
+        return bad(GPM_EHIP, "copy of the final time");
+    if (hipStreamSynchronize(c.stream) != hipSuccess) return bad(GPM_EHIP, "stream synchronisation");
     return GPM_OK;
 }
 

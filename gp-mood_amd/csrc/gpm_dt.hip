@@ -121,6 +121,39 @@ hipError_t launch_mood_finish(const MoodArgs &a, hipStream_t s)
     return hipGetLastError();
 }
 
+// Start of a run on buffers that keep an earlier run's contents (the
+// allocation-free entry point): marks 0, orders Mord, counters, statistics
+// and the completion counter 0, the mark stamp 1, time t.
+__global__ void k_run_reset(int *__restrict__ mark, order_t *__restrict__ gpo, size_t n, order_t g0,
+                            int *__restrict__ counters, int ncounters, DevStats *__restrict__ stats,
+                            double *__restrict__ tstate, double t, unsigned *__restrict__ done, int *__restrict__ stamp)
+{
+    const size_t gs = (size_t)gridDim.x * blockDim.x;
+    for (size_t c = (size_t)blockIdx.x * blockDim.x + threadIdx.x; c < n; c += gs) {
+        mark[c] = 0;
+        gpo[c] = g0;
+    }
+    if (blockIdx.x == 0) {
+        for (int k = threadIdx.x; k < ncounters; k += blockDim.x) counters[k] = 0;
+        if (threadIdx.x == 0) {
+            *stats = DevStats{};
+            for (int k = 0; k < 8; ++k) tstate[k] = 0.0;
+            tstate[0] = t;
+            tstate[4] = t;
+            *done = 0u;
+            *stamp = 1;
+        }
+    }
+}
+
+hipError_t launch_run_reset(int *mark, order_t *gpo, size_t n, order_t g0, int *counters, int ncounters,
+                            DevStats *stats, double *tstate, double t, unsigned *done, int *stamp, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_run_reset, dim3(512), dim3(256), 0, s, mark, gpo, n, g0, counters, ncounters, stats, tstate, t,
+                       done, stamp);
+    return hipGetLastError();
+}
+
 // Order words -> plain CellGPO bytes.
 __global__ void k_unpack_orders(const order_t *__restrict__ g, int8_t *__restrict__ out, size_t n)
 {

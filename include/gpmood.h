@@ -190,6 +190,29 @@ int gpm_time_stepping(gpm_ctx *ctx, double *Uin, double *Uout, double t, double 
  * passes. */
 int gpm_run(gpm_ctx *ctx, int32_t nsteps, double tmax, double *dts);
 
+/* gpm_run without a context and without any device allocation.  No
+ * counterpart in the reference: it is for callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc).  One call is gpm_create + gpm_set_gp_tables (the tables
+ * in the layouts of gpm_set_gp_tables, host memory) + gpm_set_state_device +
+ * gpm_set_time(t) + gpm_run(nsteps, tmax) + gpm_get_state_device: u_dev
+ * (device memory, nf*lf*4 doubles, interior cells [n][l][k]) holds the state
+ * on entry and the state after nsteps steps on return; t_end (host, may be
+ * NULL) receives the final time.  The results equal gpm_run's bit for bit.
+ * Work is enqueued on `stream` (a hipStream_t; NULL for the default stream)
+ * and the call returns when it is done.
+ * Every buffer of the run lives in one static device array of the library
+ * (about 80 MiB, held on each GPU the library runs on until the process
+ * ends), so: lf * nf <= 1048576 cells (GPM_EINVAL beyond); method GP_MOOD
+ * with a copy boundary (Neumann, Periodic, Reflective, KH_BC), no source
+ * term, integrator FE, SSP_RK2 or SSP_RK3 (GPM_EUNSUP otherwise).  Calls on
+ * one GPU share that array: calls that overlap on one GPU (from several host
+ * threads or streams) race.  Errors are reported on stderr (no context). */
+int gpm_run_noalloc(const gpm_config *cfg, const double *zT_sp, const double *gp_d2x_sp,
+                    const double *gp_d2y_sp, const double *gauss_weight, const int32_t *ixiy_sp,
+                    double *u_dev, int32_t nsteps, double t, double tmax, double *t_end,
+                    void *stream);
+
 int gpm_get_stats(gpm_ctx *ctx, gpm_stats *stats);
 int gpm_reset_stats(gpm_ctx *ctx);
 int gpm_set_time(gpm_ctx *ctx, double t);

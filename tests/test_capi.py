@@ -4,6 +4,8 @@ import ctypes
 import os
 import re
 
+import numpy as np
+
 from conftest import ROOT, load_package
 
 
@@ -64,3 +66,16 @@ def test_product_path_has_no_oracle_dependency():
             if f.endswith((".py", ".hip", ".cpp", ".h", ".f90")):
                 text = open(os.path.join(dirpath, f)).read()
                 assert "oracle" not in text.lower() or f == "build.py", f
+
+
+def test_run_noalloc_rejects_grids_beyond_its_scratch(gpm):
+    """gpm_run_noalloc refuses a grid larger than its static scratch memory
+    (2^20 cells) with GPM_EINVAL before any device call (runs without a GPU)."""
+    import ctypes
+    L = gpm.capi.load()
+    c = gpm.Config(lf=2048, Mord=5, ngp=3).to_c()
+    t = {k: np.zeros(4000, dtype=np.int32 if k == "ixiy" else np.float64) for k in ("zT", "d2x", "d2y", "gw", "ixiy")}
+    u = np.zeros(16)
+    rc = L.gpm_run_noalloc(ctypes.byref(c), *(gpm.capi.ptr(t[k]) for k in ("zT", "d2x", "d2y", "gw", "ixiy")),
+                           gpm.capi.ptr(u), 1, 0.0, 1.0, None, None)
+    assert rc == -1
