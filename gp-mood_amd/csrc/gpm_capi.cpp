@@ -44,6 +44,7 @@ struct gpm_ctx {
     double4 *ring = nullptr, *band = nullptr, *band3 = nullptr;  // band3: SSP_RK4's fourth stage
     StepExtra *ext = nullptr;  // [kMaxStages] in device memory (plan_stages' per-stage extras)
     order_t *gpo = nullptr;
+    uint8_t *dflag = nullptr;  // the first pass's record of the stage-input parts of each cell's decision
     int *slistA = nullptr, *slistB = nullptr, *mark = nullptr;
     int *counters = nullptr, *stamp = nullptr;
     DevStats *stats = nullptr;
@@ -350,7 +351,7 @@ StageArgs stage_args(gpm_ctx *c, int k, const StagePlan &P)
 {
     int *cnt = c->counters + k * kCountersPerStage;
     StageArgs s{};
-    s.Uin = P.in; s.Uout = P.out; s.C = P.C; s.B = c->B; s.gpo = c->gpo;
+    s.Uin = P.in; s.Uout = P.out; s.C = P.C; s.B = c->B; s.gpo = c->gpo; s.dflag = c->dflag;
     s.slist1 = c->slistB; s.cnt_s1 = cnt + kCntS0 + 1; s.cnt_f0 = cnt + kCntFail0;
     s.mark = c->mark; s.stamp = c->stamp; s.T = c->dT; s.tstate = c->tstate;
     s.dmp = c->cfg.dmp; s.u2 = c->cfg.u2; s.u2tol = c->cfg.u2_tol;
@@ -504,7 +505,7 @@ int enqueue_stages(gpm_ctx *c)
         if (c->nodetect) continue;  // one pass, no detection (mod_FE.f90:160-162)
 
         MoodArgs m{};
-        m.Uin = in; m.Uout = plan[k].out; m.C = plan[k].C; m.B = c->B; m.gpo = c->gpo;
+        m.Uin = in; m.Uout = plan[k].out; m.C = plan[k].C; m.B = c->B; m.gpo = c->gpo; m.dflag = c->dflag;
         m.slist[0] = c->slistA; m.slist[1] = c->slistB;
         m.cnt = cnt; m.mark = c->mark; m.stamp = c->stamp; m.solo_max = c->mood_solo;
         m.stats = c->stats; m.T = c->dT; m.tstate = c->tstate;
@@ -581,7 +582,8 @@ constexpr size_t kNaGpo = kNaB + kNaMaxCells * sizeof(double4);
 constexpr size_t kNaListA = kNaGpo + kNaMaxCells * sizeof(order_t);
 constexpr size_t kNaListB = kNaListA + kNaMaxCells * sizeof(int);
 constexpr size_t kNaMark = kNaListB + kNaMaxCells * sizeof(int);
-constexpr size_t kNaBytes = kNaMark + kNaMaxCells * sizeof(int);
+constexpr size_t kNaFlag = kNaMark + kNaMaxCells * sizeof(int);
+constexpr size_t kNaBytes = kNaFlag + kNaMaxCells;
 
 }  // namespace
 
@@ -689,7 +691,7 @@ int gpm_create(const gpm_config *cfg, gpm_ctx **out)
     if ((e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking)) != hipSuccess)
         return bad(GPM_EHIP, hipGetErrorString(e));
     bool ok = dalloc(&c->dT, 1) == hipSuccess && dalloc(&c->dU, n) == hipSuccess && dalloc(&c->dA, n) == hipSuccess &&
-              dalloc(&c->dB, n) == hipSuccess && dalloc(&c->gpo, n) == hipSuccess &&
+              dalloc(&c->dB, n) == hipSuccess && dalloc(&c->gpo, n) == hipSuccess && dalloc(&c->dflag, n) == hipSuccess &&
               dalloc(&c->slistA, n) == hipSuccess && dalloc(&c->slistB, n) == hipSuccess &&
               dalloc(&c->mark, n) == hipSuccess &&
               dalloc(&c->counters, kMaxStages * kCountersPerStage) == hipSuccess && dalloc(&c->stamp, 1) == hipSuccess &&
@@ -724,9 +726,10 @@ int gpm_create(const gpm_config *cfg, gpm_ctx **out)
     c->tiles_x = (c->NX + tx - 1) / tx;
     c->ntiles = c->tiles_x * ((c->NY + ty - 1) / ty);
     // Cooperative grid of the MOOD kernel: all its workgroups must be resident.
-    // The correction work is sparse and the per-pass grid barrier gets cheaper
-    // with fewer workgroups: by default one per four CUs.  GPM_MOOD_BLOCKS overrides the count.
-    int want = ncu / 4 > 0 ? ncu / 4 : 1;
+    // The per-pass grid barrier gets cheaper with fewer workgroups, while the
+    // first correction passes (thousands of cells) want more: one per two CUs
+    // (128 on MI355X) measured best.  GPM_MOOD_BLOCKS overrides the count.
+    int want = ncu / 2 > 0 ? ncu / 2 : 1;
     if (const char *env = std::getenv("GPM_MOOD_BLOCKS")) want = std::atoi(env) > 0 ? std::atoi(env) : ncu;
     {
         int per = 0;
@@ -787,7 +790,7 @@ int gpm_destroy(gpm_ctx *c)
     if (!c) return GPM_OK;
     if (c->stream) hipStreamSynchronize(c->stream);
     void *bufs[] = {c->dT, c->dU, c->dA, c->dB, c->dC, c->dD, c->dE, c->dF, c->ring, c->band, c->band3, c->st_dev, c->ext,
-                    c->gpo, c->slistA, c->slistB, c->mark, c->counters,
+                    c->gpo, c->dflag, c->slistA, c->slistB, c->mark, c->counters,
                     c->stamp, c->stats, c->tstate, c->block_min, c->dt_log, c->done, c->pred, c->nnlist,
                     c->nncount, c->adj_ub, c->adj_G, c->adj_A, c->adj_tmp, c->adj_ordx, c->adj_ordy,
                     c->priori, c->refnn_w, c->rowv, c->vnn, c->xfer, c->xfer_gpo, c->rdump.nr_start};
@@ -1436,6 +1439,7 @@ int gpm_run_noalloc(const gpm_config *cfg, const double *zT_sp, const double *gp
     c.slistA = reinterpret_cast<int *>(base + kNaListA);
     c.slistB = reinterpret_cast<int *>(base + kNaListB);
     c.mark = reinterpret_cast<int *>(base + kNaMark);
+    c.dflag = base + kNaFlag;
     c.dU = reinterpret_cast<double4 *>(u_dev);
     int ncu = 0;
     hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
@@ -1448,7 +1452,7 @@ int gpm_run_noalloc(const gpm_config *cfg, const double *zT_sp, const double *gp
         const hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(
             &per, mood_kernel(c.mord, c.ngp, cfg->num_flux, false, true), kMoodThreads, 0);
         if (e != hipSuccess || per <= 0) per = 1;
-        const int want = ncu / 4 > 0 ? ncu / 4 : 1;
+        const int want = ncu / 2 > 0 ? ncu / 2 : 1;
         c.mood_blocks = want < per * ncu ? want : per * ncu;
     }
     c.mood_solo = 16;

@@ -829,6 +829,56 @@ __device__ __forceinline__ bool decide(const MoodParams &P, const DevTables *__r
     return true;
 }
 
+// The parts of the decision that depend on the stage input alone (the shock
+// detector, and the U2 relaxation of a shocked cell whose 5-point density
+// range exceeds the plateau gap) do not change during a stage's MOOD passes.
+// The first pass records them per cell (decide_record), and the correction
+// passes finish the decision from the record with the cell's new candidate
+// (decide_with_flags); both return decide's boolean.
+enum { kFlagShock = 1, kFlagU2 = 2 };
+
+template <class Acc>
+__device__ __forceinline__ bool decide_record(const MoodParams &P, const DevTables *__restrict__ T, const Acc &acc,
+                                              int i, int j, const double cand[4], uint8_t &flags)
+{
+    bool pad = cand[0] <= 0. || isnan(cand[0]);
+    if (!pad) {
+        const double p = pressure(cand[0], cand[1], cand[2], cand[3]);
+        pad = p <= 0. || isnan(p);
+    }
+    flags = 0;
+    if (!P.dmp) return pad;
+    if (!shock_detector(P, acc, i, j)) return pad;
+    flags = kFlagShock;
+    const double UB = acc(i, j - 1).x, UL = acc(i - 1, j).x, UM = acc(i, j).x;
+    const double UR = acc(i + 1, j).x, UT = acc(i, j + 1).x;
+    const double mn = fmin2(fmin2(fmin2(fmin2(UM, UL), UB), UT), UR);
+    const double mx = fmax2(fmax2(fmax2(fmax2(UM, UL), UB), UT), UR);
+    if (!(mx - mn > P.dx * P.dx * P.dx)) return pad;
+    const bool relax = P.u2 && u2_relax(P.u2tol, P.dx, P.dy, T, acc, i, j, P.recon == kReconPol);
+    if (relax) flags |= kFlagU2;
+    if (pad) return true;
+    if (!(cand[0] > mx || cand[0] < mn)) return false;
+    return !relax;
+}
+
+template <class Acc>
+__device__ __forceinline__ bool decide_with_flags(const MoodParams &P, const Acc &acc, int i, int j,
+                                                  const double cand[4], uint8_t flags)
+{
+    if (cand[0] <= 0. || isnan(cand[0])) return true;
+    const double p = pressure(cand[0], cand[1], cand[2], cand[3]);
+    if (p <= 0. || isnan(p)) return true;
+    if (!P.dmp || !(flags & kFlagShock)) return false;
+    const double UB = acc(i, j - 1).x, UL = acc(i - 1, j).x, UM = acc(i, j).x;
+    const double UR = acc(i + 1, j).x, UT = acc(i, j + 1).x;
+    const double mn = fmin2(fmin2(fmin2(fmin2(UM, UL), UB), UT), UR);
+    const double mx = fmax2(fmax2(fmax2(fmax2(UM, UL), UB), UT), UR);
+    if (!(mx - mn > P.dx * P.dx * P.dx)) return false;
+    if (!(cand[0] > mx || cand[0] < mn)) return false;
+    return !(flags & kFlagU2);
+}
+
 // Cascade of a failing cell (mod_subroutine_MOOD.f90:159-163 or the full one).
 __device__ __forceinline__ int lower_order(int ord, int cascade)
 {

@@ -116,6 +116,7 @@ struct StageArgs {
     const double4 *__restrict__ Uin;  // stage input (interior, NX*NY)
     double4 *__restrict__ Uout;       // combined stage output
     order_t *__restrict__ gpo;        // packed CellGPO (interior)
+    uint8_t *__restrict__ dflag;      // per cell: the stage-input parts of the decision (kFlag*)
     int *__restrict__ slist1;         // cells to recompute in the first correction pass
     int *cnt_s1;                      // their count
     int *cnt_f0;                      // troubled cells of the first pass
@@ -139,6 +140,7 @@ struct MoodArgs {
     const double4 *__restrict__ Uin;
     double4 *__restrict__ Uout;
     order_t *gpo;
+    const uint8_t *__restrict__ dflag;  // the first pass's record of the stage-input parts of the decision
     int *slist[2];  // recompute lists, pass p reads slist[p & 1]
     int *cnt;       // the stage's kCountersPerStage counters (kCnt*)
     int *mark;

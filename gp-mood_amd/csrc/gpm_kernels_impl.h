@@ -331,8 +331,12 @@ __global__ __launch_bounds__(kStageThreads, stage_waves(MORD)) void k_stage(Stag
             apply_fluxes(u, fyb, fxl, fxr, fyt, dtdx, dtdy, cand);
             // CellGPO := Mord (mod_FE.f90:48), or the a priori order
             const int me = a.apriori ? (int)sO[(cy + 1) * OW + cx + 1] : MORD;
-            if (me > 1 && !a.frozen) dec = decide(P, T, acc, cx + H, cy + H, cand);
             const size_t gc = (size_t)gj * NX + gi;
+            if (me > 1 && !a.frozen) {
+                uint8_t fl;
+                dec = decide_record(P, T, acc, cx + H, cy + H, cand, fl);
+                a.dflag[gc] = fl;
+            }
             const double4 un = comb_kind<CMB>(a.C) != kCmbCopy ? a.C.Un[gc] : u;
             write_stage_output<GEN, CMB>(a.C, a.Uout, gc, GEN && a.C.band && in_band(gi, gj, NX, NY), un, u, cand,
                                          dtg);
@@ -343,6 +347,16 @@ __global__ __launch_bounds__(kStageThreads, stage_waves(MORD)) void k_stage(Stag
         const unsigned long long m = __ballot(dec);
         if (__lane_id() == 0 && m) atomicAdd(a.cnt_f0, (int)__popcll(m));
     }
+}
+
+// A workgroup barrier that orders LDS accesses only: global loads still in
+// flight stay in flight (a full __syncthreads waits for them), so a barrier
+// that guards LDS buffers does not serialise unrelated global latencies.
+__device__ __forceinline__ void lds_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+    __builtin_amdgcn_s_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
 }
 
 __device__ __forceinline__ int load_relaxed(const int *p)
@@ -440,7 +454,16 @@ struct MoodPass {
     static constexpr int HB = kHalo<MORD>;
     static constexpr int BOX = 2 * HB + 1;
     static constexpr int CPB = kMoodThreads / G;  // cells per workgroup per trip
+    // The passes workgroup 0 runs alone keep a direct-mapped cache of the
+    // stage input's interior cells (kCacheW x kCacheW slots, cell (i, j) in
+    // slot (i mod kCacheW, j mod kCacheW)): the stage input does not change
+    // during the passes, and a failure chain moves by one cell per pass, so
+    // most of a pass's neighbourhoods were staged by the passes before it.
+    static constexpr int kCacheW = 32;
     struct Lds {
+        DevTables T;  // the GP tables (the reconstruction's weights are read per lane)
+        double4 cache[kCacheW * kCacheW];
+        int tag[kCacheW * kCacheW];  // cell id held by a slot, -1 empty
         double4 nb[CPB][BOX * BOX];
         // lists of the passes workgroup 0 runs alone (a failing cell adds at
         // most five cells, so a list of at most kSoloMax cells yields at most 5x)
@@ -482,16 +505,28 @@ struct MoodPass {
         xmin_top = GEN ? stage_xmin_top(a.B, a.tstate, a.C.trk) : 0.0;
     }
 
+    // Copy the GP tables to LDS and empty the cache (all threads of the
+    // workgroup, at the start of a launch).
+    __device__ __forceinline__ void init_lds()
+    {
+        const double *src = reinterpret_cast<const double *>(a.T);
+        double *dst = reinterpret_cast<double *>(&L.T);
+        for (int k = threadIdx.x; k < (int)(sizeof(DevTables) / sizeof(double)); k += blockDim.x) dst[k] = src[k];
+        for (int k = threadIdx.x; k < kCacheW * kCacheW; k += blockDim.x) L.tag[k] = -1;
+        __syncthreads();
+    }
+
     // One pass over the nS cells of list S, spread over the lanes [t0, t0 +
     // nthr) of the grid (all workgroups) or of workgroup 0 alone; d0 is this
     // lane's entry of the first trip (loaded with the count).  cnt_fail counts
     // the cells that fail; mark lists their neighbourhoods for pass p+1.
-    template <int SC, class Mark>
+    // CACHE: stage through the LDS cache (workgroup 0 alone).
+    template <int SC, bool CACHE, class Mark>
     __device__ __forceinline__ void run(int p, int nS, int d0, int t0, int nthr, const int *S, int *cnt_fail,
                                         const Mark &mark)
     {
         using M = Scoped<SC>;
-        const DevTables *__restrict__ T = a.T;
+        const DevTables *__restrict__ T = &L.T;
         const LdsAcc lacc{&L.nb[cs][0], BOX};
         auto order_of = [&](int ii, int jj) -> int {
             if (ii < 0 || ii >= NX || jj < 0 || jj >= NY) return MORD;  // ghost cells keep Mord
@@ -516,10 +551,29 @@ struct MoodPass {
             }
             const int tn = t + per_iter;
             if (it + 1 < niter && tn < nS) dnext = S[tn];
-            __syncthreads();  // the previous trip's readers are done with nb
-            for (int q = gl; q < BOX * BOX; q += G)
-                L.nb[cs][q] = load_stage<GEN>(a.Uin, a.B, i - HB + q % BOX, j - HB + q / BOX, xmin_top);
-            __syncthreads();
+            uint8_t fl = 0;
+            if (valid && gl == 0 && me > 1) fl = a.dflag[d];
+            lds_sync();  // the previous trip's readers are done with nb
+            for (int q = gl; q < BOX * BOX; q += G) {
+                const int ii = i - HB + q % BOX, jj = j - HB + q / BOX;
+                if (CACHE && ii >= 0 && ii < NX && jj >= 0 && jj < NY) {
+                    const int slot = (ii & (kCacheW - 1)) | ((jj & (kCacheW - 1)) * kCacheW);
+                    L.nb[cs][q] = L.tag[slot] == jj * NX + ii ? L.cache[slot] : a.Uin[(size_t)jj * NX + ii];
+                } else {
+                    L.nb[cs][q] = load_stage<GEN>(a.Uin, a.B, ii, jj, xmin_top);
+                }
+            }
+            lds_sync();
+            if (CACHE) {
+                // claim the slots of the staged interior cells; one writer wins a
+                // contended slot and fills it after the next barrier
+                for (int q = gl; q < BOX * BOX; q += G) {
+                    const int ii = i - HB + q % BOX, jj = j - HB + q / BOX;
+                    if (ii < 0 || ii >= NX || jj < 0 || jj >= NY) continue;
+                    const int slot = (ii & (kCacheW - 1)) | ((jj & (kCacheW - 1)) * kCacheW);
+                    if (L.tag[slot] != jj * NX + ii) L.tag[slot] = jj * NX + ii;
+                }
+            }
             double F[4] = {0.0, 0.0, 0.0, 0.0};
             if (valid && jg < NGP) {
                 double va1 = 0.0, va2 = 1.0, vb1 = 0.0, vb2 = 1.0;
@@ -553,13 +607,24 @@ struct MoodPass {
                 const double4 uin = lacc(HB, HB);
                 apply_fluxes(uin, fyb, fxl, fxr, fyt, dtdx, dtdy, cand);
                 bool dec = false;
-                if (me > 1) dec = decide(P, T, lacc, HB, HB, cand);
+                if (me > 1) dec = decide_with_flags(P, lacc, HB, HB, cand, fl);
                 write_stage_output<GEN, kCmbRuntime>(a.C, a.Uout, d, GEN && a.C.band && in_band(i, j, NX, NY), un, uin,
                                                      cand, dtg);
                 if (dec) {
                     M::store(&a.gpo[d], pack_order(lower_order(me, a.cascade), me, p));
                     M::add(cnt_fail, 1);
                     mark(i, j, NX, NY);
+                }
+            }
+            if (CACHE) {
+                // fill the slots this lane's cells won (a slot holding the same
+                // cell already is rewritten with the same value)
+                lds_sync();
+                for (int q = gl; q < BOX * BOX; q += G) {
+                    const int ii = i - HB + q % BOX, jj = j - HB + q / BOX;
+                    if (ii < 0 || ii >= NX || jj < 0 || jj >= NY) continue;
+                    const int slot = (ii & (kCacheW - 1)) | ((jj & (kCacheW - 1)) * kCacheW);
+                    if (L.tag[slot] == jj * NX + ii) L.cache[slot] = L.nb[cs][q];
                 }
             }
         }
@@ -581,21 +646,22 @@ struct MoodPass {
         if (tid == 0) L.cnt[p & 1] = nS;
         int q = p, second = 0, det = 0;
         for (;;) {
+            // the previous pass's order words and outputs are visible to every
+            // wavefront, its list and counts are final
             __syncthreads();
+            if (q > p) det += L.fail;
             const int m = L.cnt[q & 1];
             if (m == 0 || m > a.solo_max) break;
             for (int k = tid; k < kSoloHash; k += blockDim.x) L.hash[k] = -1;
+            lds_sync();  // every lane has read the count and the failures
             if (tid == 0) {
                 L.cnt[(q + 1) & 1] = 0;
                 L.fail = 0;
             }
-            __syncthreads();  // every lane has read the count; the set and the next count are clear
             second += m;
             const int e0 = tid / G < m ? L.list[q & 1][tid / G] : 0;
-            run<WG>(q, m, e0, tid, blockDim.x, L.list[q & 1], &L.fail,
-                    SoloMark{L.hash, L.list[(q + 1) & 1], &L.cnt[(q + 1) & 1]});
-            __syncthreads();
-            det += L.fail;
+            run<WG, true>(q, m, e0, tid, blockDim.x, L.list[q & 1], &L.fail,
+                          SoloMark{L.hash, L.list[(q + 1) & 1], &L.cnt[(q + 1) & 1]});
             ++q;
         }
         // hand the current list back to the grid-wide path
@@ -619,8 +685,8 @@ struct MoodPass {
             a.cnt[kCntSecond] += nS;
             a.cnt[kCntNext] = p + 1;
         }
-        run<__HIP_MEMORY_SCOPE_AGENT>(p, nS, d0, gtid, gsz, a.slist[p & 1], &a.cnt[kCntDet],
-                                      GridMark{a.mark, a.slist[(p + 1) & 1], &a.cnt[kCntS0 + (p + 1) % 3],
+        run<__HIP_MEMORY_SCOPE_AGENT, false>(p, nS, d0, gtid, gsz, a.slist[p & 1], &a.cnt[kCntDet],
+                                             GridMark{a.mark, a.slist[(p + 1) & 1], &a.cnt[kCntS0 + (p + 1) % 3],
                                                stamp_base + p + 1});
     }
 };
@@ -654,6 +720,7 @@ __global__ __launch_bounds__(kMoodThreads) void k_mood(MoodArgs a)
     using MP = MoodPass<MORD, NGP, FLUX, GEN>;
     __shared__ typename MP::Lds lds;
     MP mp(a, lds);
+    mp.init_lds();
     cg::grid_group grid = cg::this_grid();
     const int ncell = a.B.NX * a.B.NY;
     const int gtid = blockIdx.x * blockDim.x + threadIdx.x;
@@ -698,6 +765,7 @@ __global__ __launch_bounds__(kMoodThreads) void k_mood_pass(MoodArgs a)
     const int p = s_p, nS = s_n;
     if (nS == 0) return;
     MP mp(a, lds);
+    mp.init_lds();
     const int ncell = a.B.NX * a.B.NY;
     const int gtid = blockIdx.x * blockDim.x + threadIdx.x;
     if (nS <= a.solo_max) {

@@ -196,21 +196,25 @@ def test_full_size_conservation(gpm):
     assert np.all(np.abs(tot1 - tot0) <= 1e-13 * np.abs(U0).sum(axis=(0, 1)))
 
 
-def _long_fixture():
-    paths = glob.glob(os.path.join(GOLDEN, "hash_r3_n2048_ref_bl*_*.npz"))
+def _long_fixture(cascade):
+    tag = "ref" if cascade == "reference" else "ref_full"
+    paths = glob.glob(os.path.join(GOLDEN, f"hash_r3_n2048_{tag}_bl*_*.npz"))
     if not paths:
         pytest.skip("2048^2 fixture not generated")
     return np.load(paths[0], allow_pickle=False)
 
 
-def test_config1_2048_first_steps_vs_reference(gpm):
+@pytest.mark.parametrize("cascade", ["reference", "full"])
+def test_config1_2048_first_steps_vs_reference(gpm, cascade):
     """Config-1 workload at full size: the first steps equal the reference's
-    (dt, order mask and state checksum pinned from the reference run)."""
-    z = _long_fixture()
+    (dt, order mask and state checksum pinned from the reference run; for the
+    cascade 7 -> 5 -> 3 -> 1 from the one-line patched reference build)."""
+    z = _long_fixture(cascade)
     n = int(z["n"])
+    print(f"[case config1 2048^2 first steps, cascade {cascade}]", flush=True)
     U0 = gpm.ic.make(str(z["kind"]), n, int(z["seed"]))
     assert hashlib.sha256(U0.tobytes()).hexdigest() == str(z["ic_sha"])
-    s = gpm.Solver(gpm.Config(lf=n, Mord=7, ngp=4, BC_type="reflective"))
+    s = gpm.Solver(gpm.Config(lf=n, Mord=7, ngp=4, BC_type="reflective", cascade=cascade))
     s.set_state(U0)
     dts = s.run(int(z["pin_steps"]), return_dts=True)
     U, g = s.get_state(order=True)
@@ -219,14 +223,17 @@ def test_config1_2048_first_steps_vs_reference(gpm):
     assert hashlib.sha256(np.ascontiguousarray(U[4:-4, 4:-4]).tobytes()).hexdigest() == str(z["U_pin_sha"])
 
 
-def test_config1_2048_500_steps_vs_oracle(gpm):
+@pytest.mark.parametrize("cascade", ["reference", "full"])
+def test_config1_2048_500_steps_vs_oracle(gpm, cascade):
     """The full config-1 run (2048^2, R3, reflective blast, 500 steps) against
-    the oracle's run (the oracle agrees with the reference bit for bit on the
+    the oracle's run (the oracle agrees with the reference, or with the
+    patched reference for the cascade 7 -> 5 -> 3 -> 1, bit for bit on the
     first steps at this size): dt sequence, detection counts and checksum."""
-    z = _long_fixture()
+    z = _long_fixture(cascade)
     n = int(z["n"])
+    print(f"[case config1 2048^2 x {int(z['nsteps'])} steps, cascade {cascade}]", flush=True)
     U0 = gpm.ic.make(str(z["kind"]), n, int(z["seed"]))
-    s = gpm.Solver(gpm.Config(lf=n, Mord=7, ngp=4, BC_type="reflective"))
+    s = gpm.Solver(gpm.Config(lf=n, Mord=7, ngp=4, BC_type="reflective", cascade=cascade))
     s.set_state(U0)
     dts = s.run(int(z["nsteps"]), return_dts=True)
     U = s.get_state()

@@ -68,41 +68,24 @@ def test_oracle_config0_300_steps(orc):
     assert hashlib.sha256(np.ascontiguousarray(U[4:-4, 4:-4]).tobytes()).hexdigest() == str(z["U_sha"])
 
 
-def test_oracle_pinned_at_2048(orc):
-    """The oracle agreed with the reference bit for bit over the first steps of
-    the 2048^2 R3 blast workload when the long fixture was written; the stored
-    pin data must still be reproduced (first step only, to keep this fast)."""
-    paths = glob.glob(os.path.join(GOLDEN, "hash_r3_n2048_ref_bl*_*.npz"))
-    if not paths:
-        pytest.skip("long fixture not generated")
-    z = np.load(paths[0], allow_pickle=False)
+LONG_CASES = sorted(os.path.basename(p) for p in glob.glob(os.path.join(GOLDEN, "hash_r3_n2048_ref*_bl*_*.npz")))
+
+
+@pytest.mark.parametrize("name", LONG_CASES)
+def test_oracle_pinned_at_2048(orc, name):
+    """The oracle agreed with the reference (or, for the cascade 7 -> 5 -> 3 ->
+    1, with the one-line patched reference) bit for bit over the first steps of
+    the 2048^2 R3 blast workload when the long fixture was written; the
+    stored pin data (dt, last order mask, state checksum) must still be
+    reproduced."""
+    z = np.load(os.path.join(GOLDEN, name), allow_pickle=False)
     from conftest import load_package
     gpm = load_package()
     U0 = gpm.ic.make(str(z["kind"]), int(z["n"]), int(z["seed"]))
     assert hashlib.sha256(U0.tobytes()).hexdigest() == str(z["ic_sha"])
     o = make_oracle(orc, z)
     U = o.full(U0)
-    r = o.run(U, 1)
-    assert r["dt"][0] == z["dt_pin"][0] == z["dt"][0]
-
-
-FULL_CASCADE_CASES = [c for c in STEP_CASES if "_full_" in c]
-
-
-@pytest.mark.parametrize("case", FULL_CASCADE_CASES)
-def test_full_cascade_fixture_separates_the_cascades(orc, case):
-    """The full-cascade fixtures come from the reference build with DETECTION's
-    `>3 -> 3` lowering replaced by one order down (oracle/ref/build_ref.py).
-    They pin the oracle's cascade 7 -> 5 -> 3 -> 1 only if the reference
-    cascade 7 -> 3 -> 1 does NOT reproduce them: order 5 occurs in the masks,
-    and the oracle run with the reference cascade gives other masks."""
-    z = golden(f"step_{case}.npz")
-    assert str(z["cascade"]) == "full"
-    assert (z["masks"] == 5).any(), "no cell took the 7 -> 5 step"
-    o = make_oracle(orc, z)
-    o_ref = orc.Oracle(int(z["n"]), int(z["mord"]), int(z["ngp"]), str(z["bc"]), tables_from_npz(z),
-                       dmp=bool(z["dmp"]), u2=bool(z["u2"]), u2tol=bool(z["u2tol"]), cascade="reference")
-    U = o_ref.full(z["U0"])
-    r = o_ref.run(U, int(z["nsteps"]), masks=True)
-    assert not np.array_equal(r["masks"], z["masks"])
-    assert not np.array_equal(U, z["U"])
+    r = o.run(U, int(z["pin_steps"]), masks=True)
+    assert np.array_equal(r["dt"], z["dt_pin"])
+    assert np.array_equal(r["masks"][-1], z["mask_pin"])
+    assert hashlib.sha256(np.ascontiguousarray(U[4:-4, 4:-4]).tobytes()).hexdigest() == str(z["U_pin_sha"])
