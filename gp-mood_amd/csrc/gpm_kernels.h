@@ -35,7 +35,10 @@ __host__ __device__ constexpr int stage_tx(int mord) { return mord == 3 ? GPM_TX
 __host__ __device__ constexpr int stage_ty(int mord) { return mord == 3 ? GPM_TY3 : GPM_TY; }
 constexpr int stage_waves(int mord) { return mord == 3 ? GPM_STAGE_WAVES_PER_EU3 : GPM_STAGE_WAVES_PER_EU; }
 constexpr int kStageThreads = 256;
-constexpr int kMoodThreads = 256;
+#ifndef GPM_MOOD_THREADS
+#define GPM_MOOD_THREADS 256
+#endif
+constexpr int kMoodThreads = GPM_MOOD_THREADS;
 constexpr int kSoloMax = 64;  // largest list a single workgroup takes over
 // Halo width: the stencil radius R = (Mord-1)/2 plus one cell for the face
 // neighbour, and at least 2 for the detection (U2 reads a cell's 5-point

@@ -454,16 +454,8 @@ struct MoodPass {
     static constexpr int HB = kHalo<MORD>;
     static constexpr int BOX = 2 * HB + 1;
     static constexpr int CPB = kMoodThreads / G;  // cells per workgroup per trip
-    // The passes workgroup 0 runs alone keep a direct-mapped cache of the
-    // stage input's interior cells (kCacheW x kCacheW slots, cell (i, j) in
-    // slot (i mod kCacheW, j mod kCacheW)): the stage input does not change
-    // during the passes, and a failure chain moves by one cell per pass, so
-    // most of a pass's neighbourhoods were staged by the passes before it.
-    static constexpr int kCacheW = 32;
     struct Lds {
         DevTables T;  // the GP tables (the reconstruction's weights are read per lane)
-        double4 cache[kCacheW * kCacheW];
-        int tag[kCacheW * kCacheW];  // cell id held by a slot, -1 empty
         double4 nb[CPB][BOX * BOX];
         // lists of the passes workgroup 0 runs alone (a failing cell adds at
         // most five cells, so a list of at most kSoloMax cells yields at most 5x)
@@ -505,14 +497,13 @@ struct MoodPass {
         xmin_top = GEN ? stage_xmin_top(a.B, a.tstate, a.C.trk) : 0.0;
     }
 
-    // Copy the GP tables to LDS and empty the cache (all threads of the
-    // workgroup, at the start of a launch).
+    // Copy the GP tables to LDS (all threads of the workgroup, at the start
+    // of a launch).
     __device__ __forceinline__ void init_lds()
     {
         const double *src = reinterpret_cast<const double *>(a.T);
         double *dst = reinterpret_cast<double *>(&L.T);
         for (int k = threadIdx.x; k < (int)(sizeof(DevTables) / sizeof(double)); k += blockDim.x) dst[k] = src[k];
-        for (int k = threadIdx.x; k < kCacheW * kCacheW; k += blockDim.x) L.tag[k] = -1;
         __syncthreads();
     }
 
@@ -520,8 +511,7 @@ struct MoodPass {
     // nthr) of the grid (all workgroups) or of workgroup 0 alone; d0 is this
     // lane's entry of the first trip (loaded with the count).  cnt_fail counts
     // the cells that fail; mark lists their neighbourhoods for pass p+1.
-    // CACHE: stage through the LDS cache (workgroup 0 alone).
-    template <int SC, bool CACHE, class Mark>
+    template <int SC, class Mark>
     __device__ __forceinline__ void run(int p, int nS, int d0, int t0, int nthr, const int *S, int *cnt_fail,
                                         const Mark &mark)
     {
@@ -554,26 +544,9 @@ struct MoodPass {
             uint8_t fl = 0;
             if (valid && gl == 0 && me > 1) fl = a.dflag[d];
             lds_sync();  // the previous trip's readers are done with nb
-            for (int q = gl; q < BOX * BOX; q += G) {
-                const int ii = i - HB + q % BOX, jj = j - HB + q / BOX;
-                if (CACHE && ii >= 0 && ii < NX && jj >= 0 && jj < NY) {
-                    const int slot = (ii & (kCacheW - 1)) | ((jj & (kCacheW - 1)) * kCacheW);
-                    L.nb[cs][q] = L.tag[slot] == jj * NX + ii ? L.cache[slot] : a.Uin[(size_t)jj * NX + ii];
-                } else {
-                    L.nb[cs][q] = load_stage<GEN>(a.Uin, a.B, ii, jj, xmin_top);
-                }
-            }
+            for (int q = gl; q < BOX * BOX; q += G)
+                L.nb[cs][q] = load_stage<GEN>(a.Uin, a.B, i - HB + q % BOX, j - HB + q / BOX, xmin_top);
             lds_sync();
-            if (CACHE) {
-                // claim the slots of the staged interior cells; one writer wins a
-                // contended slot and fills it after the next barrier
-                for (int q = gl; q < BOX * BOX; q += G) {
-                    const int ii = i - HB + q % BOX, jj = j - HB + q / BOX;
-                    if (ii < 0 || ii >= NX || jj < 0 || jj >= NY) continue;
-                    const int slot = (ii & (kCacheW - 1)) | ((jj & (kCacheW - 1)) * kCacheW);
-                    if (L.tag[slot] != jj * NX + ii) L.tag[slot] = jj * NX + ii;
-                }
-            }
             double F[4] = {0.0, 0.0, 0.0, 0.0};
             if (valid && jg < NGP) {
                 double va1 = 0.0, va2 = 1.0, vb1 = 0.0, vb2 = 1.0;
@@ -616,17 +589,6 @@ struct MoodPass {
                     mark(i, j, NX, NY);
                 }
             }
-            if (CACHE) {
-                // fill the slots this lane's cells won (a slot holding the same
-                // cell already is rewritten with the same value)
-                lds_sync();
-                for (int q = gl; q < BOX * BOX; q += G) {
-                    const int ii = i - HB + q % BOX, jj = j - HB + q / BOX;
-                    if (ii < 0 || ii >= NX || jj < 0 || jj >= NY) continue;
-                    const int slot = (ii & (kCacheW - 1)) | ((jj & (kCacheW - 1)) * kCacheW);
-                    if (L.tag[slot] == jj * NX + ii) L.cache[slot] = L.nb[cs][q];
-                }
-            }
         }
     }
 
@@ -660,7 +622,7 @@ struct MoodPass {
             }
             second += m;
             const int e0 = tid / G < m ? L.list[q & 1][tid / G] : 0;
-            run<WG, true>(q, m, e0, tid, blockDim.x, L.list[q & 1], &L.fail,
+            run<WG>(q, m, e0, tid, blockDim.x, L.list[q & 1], &L.fail,
                           SoloMark{L.hash, L.list[(q + 1) & 1], &L.cnt[(q + 1) & 1]});
             ++q;
         }
@@ -685,7 +647,7 @@ struct MoodPass {
             a.cnt[kCntSecond] += nS;
             a.cnt[kCntNext] = p + 1;
         }
-        run<__HIP_MEMORY_SCOPE_AGENT, false>(p, nS, d0, gtid, gsz, a.slist[p & 1], &a.cnt[kCntDet],
+        run<__HIP_MEMORY_SCOPE_AGENT>(p, nS, d0, gtid, gsz, a.slist[p & 1], &a.cnt[kCntDet],
                                              GridMark{a.mark, a.slist[(p + 1) & 1], &a.cnt[kCntS0 + (p + 1) % 3],
                                                stamp_base + p + 1});
     }
