@@ -3,7 +3,9 @@
 
 Workload (the headline configuration): 2048 x 2048 cells on [0,1]^2, FP64,
 GP-R3 reconstruction (Mord 7, 4 Gauss points per face), HLLC, SSP-RK3,
-CFL 0.8, reflecting walls, MOOD with PAD + DMP + U2 and the reference cascade;
+CFL 0.8, reflecting walls, MOOD with PAD + DMP + U2 and the order cascade
+GP-R3 -> GP-R2 -> GP-R1 -> first order (the reference's own cascade, 7 -> 3 ->
+1, is timed beside it);
 seeded blast initial condition (rho = 1 +/- 1% white noise, u = v = 0,
 p = 1e-5, total energy 1.0 added as a Gaussian of width 2 dx at a seeded centre
 in [0.3, 0.7]^2).  The state stays resident in HBM; a step is dt reduction +
@@ -65,11 +67,12 @@ def _raise_stack():
         pass
 
 
-def cpu_baseline_reference(U0, n, steps=1):
-    """The reference Fortran (built from its sources by oracle/ref/build_ref.py)
-    on the same initial state: serial, `steps` steps, its own cpu_time around
-    the step loop."""
-    exe = os.path.join(ROOT, "oracle", "_ref", f"r3_n{n}_ref", "gpm_ref.exe")
+def cpu_baseline_reference(U0, n, steps=1, cascade="full"):
+    """The reference Fortran (built from its sources by oracle/ref/build_ref.py;
+    for the cascade 7 -> 5 -> 3 -> 1 the build with DETECTION's one-line cascade
+    change) on the same initial state: serial, `steps` steps, its own cpu_time
+    around the step loop."""
+    exe = os.path.join(ROOT, "oracle", "_ref", f"r3_n{n}_ref" + ("_full" if cascade == "full" else ""), "gpm_ref.exe")
     if not os.path.exists(exe):
         return None
     with tempfile.TemporaryDirectory(dir=os.path.dirname(exe)) as d:
@@ -85,18 +88,18 @@ def cpu_baseline_reference(U0, n, steps=1):
             return None
         return {"value": float(m.group(2)), "unit": "cell-updates/s per RK stage", "cores": 1,
                 "kind": "reference", "seconds": float(m.group(1)),
-                "sample": f"{steps} SSP-RK3 step(s) of the same {n}^2 R3 blast workload from the same initial "
-                          f"state, reference Fortran (amdflang -O3, serial) on this host"}
+                "sample": f"{steps} SSP-RK3 step(s) of the same {n}^2 R3 blast workload ({cascade} cascade) from "
+                          f"the same initial state, reference Fortran (amdflang -O3, serial) on this host"}
 
 
-def cpu_baseline_port(U0, n, steps=1):
+def cpu_baseline_port(U0, n, steps=1, cascade="full"):
     """Fallback: the C restatement of the reference (oracle/gpm_oracle.c), one thread."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     try:
         import oracle as O
         gpm = sys.modules["gpmood_amd"]
         t = gpm.gp_tables(7, 4, n)
-        o = O.Oracle(n, 7, 4, "reflective", t)
+        o = O.Oracle(n, 7, 4, "reflective", t, cascade=cascade)
         U = o.full(U0)
         t0 = time.perf_counter()
         o.run(U, steps)
@@ -304,6 +307,30 @@ def nn_comparison(gpm, n, steps, extra_seeds=(1, 2, 3)):
     return out
 
 
+def plain_run(gpm, n, steps, warmup, seed, cascade):
+    """The headline workload (2048^2 GP-R3 reflective blast) with the given
+    order cascade: `warmup` untimed steps, then `steps` timed device-resident
+    steps with HIP-event timing of the kernels."""
+    import torch
+    U0 = gpm.ic.blast(n, seed)
+    cfg = gpm.Config(lf=n, Mord=7, ngp=4, BC_type="reflective", CFL=0.8, cascade=cascade)
+    s = gpm.Solver(cfg)
+    s.set_state(U0)
+    s.set_time(0.0)
+    print(f"[bench] {cascade} cascade: warmup {warmup} steps at {n}^2", file=sys.stderr, flush=True)
+    s.run(warmup)
+    torch.cuda.synchronize()
+    s.reset_stats()
+    s.set_timing(True)
+    print(f"[bench] {cascade} cascade: timing {steps} steps", file=sys.stderr, flush=True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    s.run(steps)
+    torch.cuda.synchronize()
+    el = time.perf_counter() - t0
+    return s, U0, el, s.kernel_time(), s.stats()
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -317,7 +344,7 @@ def main():
     ap.add_argument("--nn-seeds", default="1,2,3",
                     help="other four-quadrant seeds the NN comparison also reports (comma list, may be empty)")
     ap.add_argument("--dropin-steps", type=int, default=3, help="host-array drop-in steps timed (0: skip)")
-    ap.add_argument("--large-steps", type=int, default=0,
+    ap.add_argument("--large-steps", type=int, default=20,
                     help="steps of the 8192^2 GP-R3 periodic shear-layer run, plain and NN-GP-MOOD (0: skip)")
     ap.add_argument("--options-steps", type=int, default=20,
                     help="steps of each run of the other flux, integrator, boundary and method options (0: skip)")
@@ -335,24 +362,7 @@ def main():
     import torch
     gpm = load_package()
     n, mord, ngp = args.n, 7, 4
-    U0 = gpm.ic.blast(n, args.seed)
-    cfg = gpm.Config(lf=n, Mord=mord, ngp=ngp, BC_type="reflective", CFL=0.8)
-    s = gpm.Solver(cfg)
-    s.set_state(U0)
-    s.set_time(0.0)
-    print(f"[bench] warmup {args.warmup} steps at {n}^2", file=sys.stderr, flush=True)
-    s.run(args.warmup)
-    torch.cuda.synchronize()
-    s.reset_stats()
-    s.set_timing(True)
-    print(f"[bench] timing {args.steps} steps", file=sys.stderr, flush=True)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    s.run(args.steps)
-    torch.cuda.synchronize()
-    el = time.perf_counter() - t0
-    kt = s.kernel_time()
-    st = s.stats()
+    s, U0, el, kt, st = plain_run(gpm, n, args.steps, args.warmup, args.seed, "full")
     if args.calib_copy_mib > 0:
         nbytes = args.calib_copy_mib << 20
         src = torch.ones(nbytes // 8, dtype=torch.float64, device="cuda")
@@ -360,6 +370,15 @@ def main():
         for _ in range(3):
             dst.copy_(src)
         torch.cuda.synchronize()
+    # the reference's own cascade (7 -> 3 -> 1) on the same workload
+    s_ref, _, el_ref, kt_ref, st_ref = plain_run(gpm, n, args.steps, args.warmup, args.seed, "reference")
+    s_ref.close()
+    ref_cascade = {"value": 3.0 * n * n * args.steps / el_ref, "ms_per_step": 1e3 * el_ref / args.steps,
+                   "stage_kernel_avg_ms": kt_ref["stage_ms"], "mood_kernel_avg_ms": kt_ref["mood_ms"],
+                   "max_passes": st_ref["max_passes"],
+                   "detected_per_step": st_ref["detected_total"] / max(1, args.steps),
+                   "workload": f"the same {n}x{n} GP-R3 blast run with the reference's cascade 7 -> 3 -> 1 "
+                               "(mod_subroutine_MOOD.f90:159-163)"}
 
     value = 3.0 * n * n * args.steps / el
     fl = flops_per_cell_stage(mord, ngp)
@@ -369,16 +388,21 @@ def main():
     fp64_ach = flops_launch / avg_s if avg_s > 0 else 0.0
     hbm_ach = bytes_launch / avg_s if avg_s > 0 else 0.0
     attainable = min(HBM_PEAK * fl / BYTES_PER_CELL_STAGE, FP64_PEAK)
-    traffic = None
+    # HBM bytes per launch need hardware counters (rocprofv3 --pmc), which a
+    # program cannot read about itself: the value is the one stored from the
+    # profiling run of this kernel on this workload
+    traffic, traffic_source = None, None
     tf = os.path.join(ROOT, "profiles", "traffic_k_stage_r3_2048.json")
     if os.path.exists(tf):
         try:
             traffic = json.load(open(tf)).get("hbm_bytes_per_launch")
+            traffic_source = ("stored: profiles/traffic_k_stage_r3_2048.json (rocprofv3 --pmc FETCH_SIZE and "
+                              "WRITE_SIZE of this kernel on this workload, scaled by a copy-kernel calibration)")
         except Exception:
             traffic = None
     roofline = {
-        "bound": "mfma", "achieved": fp64_ach / 1e12, "peak": FP64_PEAK / 1e12, "unit": "TFLOP/s",
-        "frac": fp64_ach / FP64_PEAK, "traffic": traffic,
+        "bound": "fp64-valu", "achieved": fp64_ach / 1e12, "peak": FP64_PEAK / 1e12, "unit": "TFLOP/s",
+        "frac": fp64_ach / FP64_PEAK, "traffic": traffic, "traffic_source": traffic_source,
         "kernel": "k_stage<7,4,*> (first MOOD pass: GP-R3 reconstruction + HLLC + update + detection)",
         "avg_launch_ms": kt["stage_ms"], "launches": kt["stage_launches"],
         "flop_per_cell_stage": fl, "bytes_per_cell_stage": BYTES_PER_CELL_STAGE,
@@ -429,15 +453,17 @@ def main():
     cpu = None
     if not args.no_cpu_baseline:
         print("[bench] CPU baseline", file=sys.stderr, flush=True)
-        cpu = cpu_baseline_reference(U0, n, args.cpu_steps) or cpu_baseline_port(U0, n, args.cpu_steps)
+        cpu = (cpu_baseline_reference(U0, n, args.cpu_steps, "full") or
+               cpu_baseline_reference(U0, n, args.cpu_steps, "reference") or cpu_baseline_port(U0, n, args.cpu_steps))
     out = {
         "metric": METRIC, "value": value, "unit": "cell-updates/s per RK stage", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": 1e3 * el / args.steps,
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
         "data": f"synthetic: seeded blast initial condition (seed {args.seed})",
-        "config": {"workload": f"plain GP-MOOD step {n}x{n}, GP-R3 (Mord 7, ngp 4), HLLC, SSP-RK3, CFL 0.8, "
-                               "reflective walls, blast IC", "n": n, "mord": mord, "ngp": ngp,
-                   "cascade": "reference", "parallelism": "single GPU"},
+        "config": {"workload": f"plain GP-MOOD step {n}x{n}, GP-R3 (Mord 7, ngp 4), cascade R3->R2->R1->first "
+                               "order, HLLC, SSP-RK3, CFL 0.8, reflective walls, blast IC", "n": n, "mord": mord,
+                   "ngp": ngp, "cascade": "full", "parallelism": "single GPU"},
+        "reference_cascade": ref_cascade,
         "mood": {"max_passes": st["max_passes"], "detected_per_step": st["detected_total"] / max(1, args.steps),
                  "second_pass_cells_per_step": st["second_pass_cells"] / max(1, args.steps),
                  "mood_kernel_avg_ms": kt["mood_ms"]},
