@@ -73,6 +73,15 @@ struct gpm_ctx {
     // training samples in the reference network's format (not owned, but nr_start)
     RefDumpArgs rdump{};
     bool rdump_on = false;
+    // the reference's own dataset arrays (gpm_set_reference_dataset and
+    // gpm_append_reference_dataset): the device rows of the last step (owned)
+    struct HostDataset {
+        int32_t L = 0;
+        float *d_in = nullptr, *d_lab = nullptr;
+        long long *d_cells = nullptr, *d_nr = nullptr;
+        unsigned long long *d_count = nullptr;
+    } hds;
+    bool hds_on = false;
     // host-array exchange (allocated on first use): a ghosted state and a ghosted CellGPO
     double4 *xfer = nullptr;
     int32_t *xfer_gpo = nullptr;
@@ -796,6 +805,9 @@ int gpm_destroy(gpm_ctx *c)
                     c->priori, c->refnn_w, c->rowv, c->vnn, c->xfer, c->xfer_gpo, c->rdump.nr_start};
     for (void *b : bufs)
         if (b) hipFree(b);
+    void *hbufs[] = {c->hds.d_in, c->hds.d_lab, c->hds.d_cells, c->hds.d_nr, c->hds.d_count};
+    for (void *b : hbufs)
+        if (b) hipFree(b);
     for (hipEvent_t e : c->ev_pool) hipEventDestroy(e);
     if (c->stream) hipStreamDestroy(c->stream);
     delete c;
@@ -913,6 +925,8 @@ int gpm_time_stepping(gpm_ctx *c, double *Uin, double *Uout, double t, double dt
                       gpm_stats *stats)
 {
     if (!c || !Uin || !Uout) return fail(c, GPM_EINVAL, "null argument");
+    if (c->hds_on)  // this step's rows start at 0
+        HIPCHK(c, hipMemsetAsync(c->hds.d_count, 0, sizeof(unsigned long long), c->stream));
     int rc = gpm_set_state(c, Uin);
     if (rc) return rc;
     // Boundary_C(Uin) of the first Forward_Euler stage (mod_FE.f90:64, t_rk = t)
@@ -941,6 +955,7 @@ int gpm_run(gpm_ctx *c, int32_t nsteps, double tmax, double *dts)
 {
     if (!c || nsteps < 0) return fail(c, GPM_EINVAL, "invalid argument");
     if (!c->has_state) return fail(c, GPM_ESTATE, "no state set");
+    if (c->hds_on) return fail(c, GPM_ESTATE, "the reference dataset arrays are filled step by step (gpm_time_stepping)");
     if (dts && nsteps > c->dt_log_cap) {
         if (c->dt_log) hipFree(c->dt_log);
         c->dt_log = nullptr;
@@ -1173,6 +1188,88 @@ int gpm_set_reference_dump(gpm_ctx *c, float *inputs_dev, float *labels_dev, int
     c->rdump.capacity = capacity; c->rdump.count = reinterpret_cast<unsigned long long *>(count_dev);
     c->rdump.nr = reinterpret_cast<long long *>(nr_dev); c->rdump.seed = seed; c->rdump.L = L;
     c->rdump_on = true;
+    return GPM_OK;
+}
+
+int gpm_set_reference_dataset(gpm_ctx *c, int32_t L, int32_t nr0, int32_t nr1)
+{
+    if (!c) return GPM_EINVAL;
+    auto &h = c->hds;
+    if (L <= 0) {
+        c->hds_on = false;
+        c->rdump_on = false;
+        return GPM_OK;
+    }
+    if (nr0 < 0 || nr1 < 0) return fail(c, GPM_EINVAL, "NR0 and NR1 must not be negative");
+    const size_t n = (size_t)c->NX * c->NY;
+    if (!h.d_in || h.L != L) {
+        if (h.d_in) { HIPCHK(c, hipStreamSynchronize(c->stream)); hipFree(h.d_in); h.d_in = nullptr; }
+        HIPCHK(c, dalloc(&h.d_in, n * (size_t)L));
+    }
+    if (!h.d_lab) HIPCHK(c, dalloc(&h.d_lab, 2 * n));
+    if (!h.d_cells) HIPCHK(c, dalloc(&h.d_cells, n));
+    if (!h.d_nr) HIPCHK(c, dalloc(&h.d_nr, 2));
+    if (!h.d_count) HIPCHK(c, dalloc(&h.d_count, 1));
+    h.L = L;
+    int rc = gpm_set_reference_dump(c, h.d_in, h.d_lab, reinterpret_cast<int64_t *>(h.d_cells), (int64_t)n,
+                                    reinterpret_cast<uint64_t *>(h.d_count), reinterpret_cast<int64_t *>(h.d_nr),
+                                    0x5EED0D5Eull, L);
+    if (rc) return rc;
+    const long long nr[2] = {nr0, nr1};
+    HIPCHK(c, hipMemsetAsync(h.d_count, 0, sizeof(unsigned long long), c->stream));
+    HIPCHK(c, hipMemcpyAsync(h.d_nr, nr, sizeof(nr), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    c->hds_on = true;
+    return GPM_OK;
+}
+
+int gpm_append_reference_dataset(gpm_ctx *c, float *inputs, float *labels, int32_t dataset_size, int32_t *index,
+                                 int32_t *n_overwrite, int32_t *nr0, int32_t *nr1)
+{
+    if (!c) return GPM_EINVAL;
+    if (!c->hds_on) return fail(c, GPM_ESTATE, "gpm_set_reference_dataset has not switched the samples on");
+    if (!inputs || !labels || !index || !n_overwrite || !nr0 || !nr1 || dataset_size < 1)
+        return fail(c, GPM_EINVAL, "null argument or dataset_size < 1");
+    if (*index < 1 || *index > dataset_size) return fail(c, GPM_EINVAL, "index must be in 1..dataset_size");
+    auto &h = c->hds;
+    unsigned long long count = 0;
+    HIPCHK(c, hipMemcpyAsync(&count, h.d_count, sizeof(count), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    const size_t n = (size_t)c->NX * c->NY;
+    if (count > n) count = n;
+    std::vector<float> in(count * (size_t)h.L), lab(count * 2);
+    std::vector<long long> cells(count);
+    if (count) {
+        HIPCHK(c, hipMemcpyAsync(in.data(), h.d_in, in.size() * sizeof(float), hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(c, hipMemcpyAsync(lab.data(), h.d_lab, lab.size() * sizeof(float), hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(c, hipMemcpyAsync(cells.data(), h.d_cells, cells.size() * sizeof(long long), hipMemcpyDeviceToHost,
+                                 c->stream));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+    }
+    // append_to_NN_datatset's loop order (n outer, l inner): rows by cell id
+    std::vector<long long> row_of(n, -1);
+    for (size_t r = 0; r < count; ++r) row_of[(size_t)cells[r]] = (long long)r;
+    for (size_t cell = 0; cell < n; ++cell) {
+        const long long r = row_of[cell];
+        if (r < 0) continue;
+        const int64_t i = *index - 1;  // inputs(index, 1:L), labels(index, 1:2), column-major
+        for (int k = 0; k < h.L; ++k) inputs[i + (int64_t)k * dataset_size] = in[(size_t)r * h.L + k];
+        labels[i] = lab[2 * (size_t)r];
+        labels[i + dataset_size] = lab[2 * (size_t)r + 1];
+        if (lab[2 * (size_t)r + 1] == 1.0f) ++*nr1;
+        else ++*nr0;
+        if (*index == dataset_size) {
+            *index = 1;
+            ++*n_overwrite;
+        } else {
+            ++*index;
+        }
+    }
+    // the next step's order-3 keep probability starts from these counts
+    const long long nr[2] = {*nr0, *nr1};
+    HIPCHK(c, hipMemsetAsync(h.d_count, 0, sizeof(unsigned long long), c->stream));
+    HIPCHK(c, hipMemcpyAsync(h.d_nr, nr, sizeof(nr), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
     return GPM_OK;
 }
 

@@ -21,6 +21,13 @@ namespace gpm {
 // float in [-0x1.9fe368p6, 0x1.62e42ep6] (the whole range where neither
 // overflow nor underflow applies).
 //
+// Upstream: ARM optimized-routines, math/expf.c with its table in
+// math/exp2f_data.c (licence: MIT OR Apache-2.0 WITH LLVM-exception), which
+// glibc carries as sysdeps/ieee754/flt-32/e_expf.c and e_exp2f_data.c
+// (LGPL-2.1-or-later).  This is a restatement of the published method: the
+// table entries below are the values 2^(i/32) rounded to double, and the
+// cubic's coefficients are the method's; no upstream source is copied.
+//
 // kExp2Tab[i] = bits of 2^(i/32) (rounded to double) minus i << 47, so that
 // adding k << 47 puts k/32's integer part into the exponent.
 constexpr unsigned long long kExp2Tab[32] = {

@@ -103,6 +103,21 @@ module gpm_capi
          type(c_ptr), value :: ctx
          integer(c_int32_t), intent(out) :: priori(*)
       end function gpm_get_cellgpo_priori
+
+      integer(c_int) function gpm_set_reference_dataset(ctx, L, nr0, nr1) bind(C, name='gpm_set_reference_dataset')
+         import :: c_int, c_ptr, c_int32_t
+         type(c_ptr), value :: ctx
+         integer(c_int32_t), value :: L, nr0, nr1
+      end function gpm_set_reference_dataset
+
+      integer(c_int) function gpm_append_reference_dataset(ctx, inputs, labels, dataset_size, index, n_overwrite, &
+         nr0, nr1) bind(C, name='gpm_append_reference_dataset')
+         import :: c_int, c_ptr, c_int32_t, c_float
+         type(c_ptr), value :: ctx
+         real(c_float), intent(inout) :: inputs(*), labels(*)
+         integer(c_int32_t), value :: dataset_size
+         integer(c_int32_t), intent(inout) :: index, n_overwrite, nr0, nr1
+      end function gpm_append_reference_dataset
    end interface
 
 contains

@@ -7,7 +7,11 @@
 ! are the ones GP_presim_sphere left in global_variables, so both codes use the
 ! same weights.  With method NN_GP_MOOD or NN_GP_MOOD_CC the network is the one
 ! load_NN read into global_variables (main.f90:116), and the step sets the
-! counters mod_FE.f90 keeps for it.  Link with -lgpmood.
+! counters mod_FE.f90 keeps for it.  With write_NN_dataset the first stage's
+! samples go into the dataset arrays inputs / labels with index, n_overwrite,
+! NR0, NR1 and freq_R0 as append_to_NN_datatset leaves them
+! (mod_FE.f90:176-181); which order-3 samples are kept is drawn by the library.
+! Link with -lgpmood.
 module time_stepper
    use iso_c_binding
    use parameters
@@ -44,6 +48,13 @@ contains
          stop 1
       end if
       allocate(gpm_order(lb:le, nb:ne))
+      if (write_NN_dataset) then
+         rc = gpm_set_reference_dataset(gpm_handle, int(L, c_int32_t), int(NR0, c_int32_t), int(NR1, c_int32_t))
+         if (rc /= 0) then
+            print *, 'gpm_set_reference_dataset failed: ', trim(gpm_error_message(gpm_handle))
+            stop 1
+         end if
+      end if
       if ((method == NN_GP_MOOD) .or. (method == NN_GP_MOOD_CC)) then
          rc = gpm_set_reference_nn(gpm_handle, int(L, c_int32_t), int(length, c_int32_t), weight0, bias0, &
             weight1, bias1, weight2, bias2)
@@ -68,6 +79,17 @@ contains
       if (rc /= 0) then
          print *, 'gpm_time_stepping failed: ', trim(gpm_error_message(gpm_handle))
          stop 1
+      end if
+      if (write_NN_dataset) then
+         ! append_to_NN_datatset after the first stage of this step (mod_FE.f90:178-181)
+         steps_NN_sample(niter) = 1
+         rc = gpm_append_reference_dataset(gpm_handle, inputs, labels, int(dataset_size, c_int32_t), index, &
+            n_overwrite, NR0, NR1)
+         if (rc /= 0) then
+            print *, 'gpm_append_reference_dataset failed: ', trim(gpm_error_message(gpm_handle))
+            stop 1
+         end if
+         freq_R0 = NR0*one/(NR0+NR1)
       end if
       ! t_rk as the integrator leaves it (mod_SSP_RK3.f90:31 sets the last stage's)
       if (integrator == SSP_RK3) t_rk = t + 0.5*dt

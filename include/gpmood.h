@@ -342,6 +342,24 @@ int gpm_read_reference_nn(const char *nn_filename, int32_t L, int32_t length, fl
  * gpm_io_write_nn_dataset (include/gpmood_io.h). */
 int gpm_set_reference_dump(gpm_ctx *ctx, float *inputs_dev, float *labels_dev, int64_t *cells_dev, int64_t capacity,
                            uint64_t *count_dev, int64_t *nr_dev, uint64_t seed, int32_t L);
+/* The same samples for the reference's own dataset arrays, as
+ * append_to_NN_datatset writes them with write_NN_dataset = .true.
+ * (mod_append_NN_dataset.f90:11-79, global_variables.f90:117-120), for the
+ * host-array drop-in.  gpm_set_reference_dataset switches them on (L > 0, the
+ * input width) with the running counts NR0, NR1 (L = 0 switches them off);
+ * every gpm_time_stepping call then keeps its first stage's rows on the
+ * device, and gpm_append_reference_dataset, called after the step, appends
+ * them to inputs(dataset_size, L) and labels(dataset_size, 2) (real(4),
+ * Fortran layouts): in cell order (n outer, l inner) at row *index (1-based),
+ * wrapping to 1 with *n_overwrite + 1 after dataset_size rows; *nr0 / *nr1
+ * count the rows labelled (1,0) / (0,1) and set the next step's order-3 keep
+ * probability NR0/(NR0+NR1).  Which order-3 rows are kept is drawn as in
+ * gpm_set_reference_dump (not the reference's random_number); every other
+ * row is the reference's.  The next step discards rows not appended.
+ * gpm_run refuses to run while the samples are on (GPM_ESTATE). */
+int gpm_set_reference_dataset(gpm_ctx *ctx, int32_t L, int32_t nr0, int32_t nr1);
+int gpm_append_reference_dataset(gpm_ctx *ctx, float *inputs, float *labels, int32_t dataset_size,
+                                 int32_t *index, int32_t *n_overwrite, int32_t *nr0, int32_t *nr1);
 /* niter before the next step (main.f90:129,147): every step first advances it.
  * The Fortran time_stepping drop-in sets niter - 1, main having advanced it. */
 int gpm_set_niter(gpm_ctx *ctx, int64_t niter);

@@ -159,6 +159,9 @@ CONFIGS = {
     "r3_n64_neu_full": (64, 7, 4, "neumann", True, True, True),
     "r3_n32_ref_full": (32, 7, 4, "reflective", True, True, True),
     "r3_n2048_ref_full": (2048, 7, 4, "reflective", True, True, True),
+    # write_NN_dataset = .true.: the training samples of append_to_NN_datatset
+    # (mod_append_NN_dataset.f90:11-79), dumped by the harness driver
+    "dset_r1_n64_neu": (64, 3, 2, "neumann", True, True, True),
 }
 
 # The problem files' own configurations (problems/*.f90: grid, domain, method,
@@ -242,6 +245,7 @@ EXTRAS = {
     "r3_n64_neu_full": dict(cascade="full"),
     "r3_n32_ref_full": dict(cascade="full"),
     "r3_n2048_ref_full": dict(cascade="full"),
+    "dset_r1_n64_neu": dict(write_nn=True, dataset_size=60000, nn_l=52),
 }
 
 # DETECTION's cascade branch (mod_subroutine_MOOD.f90:161-162) and its
@@ -282,7 +286,7 @@ def config(name):
 # time_stepper over libgpmood) is also built
 DROPIN_CONFIGS = ["r2_n64_neu", "r3_n64_ref", "r3_n64_kh_uns", "r1_dmr_64x16", "r1_rt_16x48", "r1_m800_32_fe",
                   "r2_n32_ref_rk4", "fog_n64_kh_uns", "fog_m800_32_hll", "nn_r1_n64_neu", "nncc_r1_n64_neu",
-                  "pol_r1_n64_neu", "r3_n32_ref_rect", "r1_rt_16x48_rk4"]
+                  "pol_r1_n64_neu", "r3_n32_ref_rect", "r1_rt_16x48_rk4", "dset_r1_n64_neu"]
 
 PARAMS_TEMPLATE = """module parameters
    use constants
@@ -298,8 +302,8 @@ PARAMS_TEMPLATE = """module parameters
    logical , parameter :: DMP        = {dmp}
    logical , parameter :: U2         = {u2}
    logical , parameter :: U2_tol     = {u2tol}
-   logical :: write_NN_dataset=.false.
-   integer, parameter :: dataset_size = 1
+   logical :: write_NN_dataset={write_nn}
+   integer, parameter :: dataset_size = {dataset_size}
    integer, parameter :: L={nn_l}
    integer, parameter :: length={nn_len}
    character(100) :: NN_filename='{nn_file}'
@@ -338,7 +342,8 @@ def build(name, dropin=False):
                                        nf=e["nf"], lx=e["lx"], ly=e["ly"], problem=e["problem"],
                                        integrator=INTEGRATOR_NAMES[e["integrator"]], nn_l=e.get("nn_l", min(32, n)),
                                        nn_len=e.get("nn_len", 60), nn_file=e.get("nn_file", ""),
-                                       method=e["method"]))
+                                       method=e["method"], write_nn=fbool(e.get("write_nn", False)),
+                                       dataset_size=e.get("dataset_size", 1)))
     flags = ["-fdefault-real-8", "-O3", "-ffp-contract=off", "-J", out, "-I", out]
     if n * extras(name)["nf"] >= 1024 * 1024:
         flags.append("-mcmodel=medium")

@@ -18,6 +18,9 @@
 !                count_detected_cell_a_priori, count_detected_cell_a_posteriori,
 !                count_steps_NN_produced_NAN (i32 each)
 !   priori.bin   (the same methods) CellGPO_priori(1:lf,1:nf) as int8 per step
+!   dataset.bin  (write_NN_dataset) index, n_overwrite, NR0, NR1, dataset_size, L
+!                (i32 each), inputs(dataset_size, L), labels(dataset_size, 2)
+!                (f32, Fortran order), steps_NN_sample(1:nsteps) (i32)
 ! The NN methods load their network with load_NN (mod_NN.f90:60-117) first, as
 ! main.f90:116 does.
 !
@@ -143,6 +146,16 @@ program gpm_ref_driver
    open(newunit=us, file=trim(outdir)//'/U_final.bin', access='stream', form='unformatted', status='replace')
    write(us) U
    close(us)
+
+   if (write_NN_dataset) then
+      ! the training-sample arrays of append_to_NN_datatset and their counters
+      open(newunit=us, file=trim(outdir)//'/dataset.bin', access='stream', form='unformatted', status='replace')
+      write(us) int(index,4), int(n_overwrite,4), int(NR0,4), int(NR1,4), int(dataset_size,4), int(L,4)
+      write(us) inputs
+      write(us) labels
+      write(us) int(steps_NN_sample(1:nsteps),4)
+      close(us)
+   end if
 
    write(*,'(A,I0,A,I0,A,I0,A,ES12.5,A,ES12.5)') 'N=', lf, ' steps=', nsteps, ' mode=', mode, &
       ' cpu_s=', tac-tic, ' cell_stage_per_s=', 3.0d0*real(lf,8)*real(nf,8)*real(nsteps,8)/max(tac-tic,1.0d-9)

@@ -45,6 +45,22 @@ def read_tables(path):
     return out
 
 
+def read_dataset(path, nsteps):
+    """dataset.bin of the harness driver: the counters index, n_overwrite, NR0,
+    NR1 and the arrays inputs(dataset_size, L), labels(dataset_size, 2) of
+    append_to_NN_datatset, as rows; steps_NN_sample of the run's steps."""
+    b = open(path, "rb").read()
+    index, n_over, nr0, nr1, size, L = (int(v) for v in np.frombuffer(b[:24], dtype=np.int32))
+    off = 24
+    inputs = np.frombuffer(b[off:off + 4 * size * L], dtype=np.float32).reshape(L, size).T
+    off += 4 * size * L
+    labels = np.frombuffer(b[off:off + 8 * size], dtype=np.float32).reshape(2, size).T
+    off += 8 * size
+    steps = np.frombuffer(b[off:off + 4 * nsteps], dtype=np.int32)
+    return dict(index=index, n_overwrite=n_over, nr0=nr0, nr1=nr1, size=size, L=L, inputs=inputs.copy(),
+                labels=labels.copy(), steps_NN_sample=steps.copy())
+
+
 def run_ref(cfg, U0, nsteps, mode, nn=None):
     """Run the reference build of `cfg`; `nn`: the network (refnn_weights) that
     load_NN reads from refnn.txt in the working directory."""
@@ -66,6 +82,8 @@ def run_ref(cfg, U0, nsteps, mode, nn=None):
                "U": np.fromfile(os.path.join(d, "U_final.bin"), dtype=np.float64).reshape(nf + 8, n + 8, 4)}
         if mode == 1:
             res["U_stage"] = np.fromfile(os.path.join(d, "U_stage.bin"), dtype=np.float64).reshape(3, nf + 8, n + 8, 4)
+        if os.path.exists(os.path.join(d, "dataset.bin")):
+            res["dataset"] = read_dataset(os.path.join(d, "dataset.bin"), nsteps)
         if nn is not None:
             res["nn"] = np.fromfile(os.path.join(d, "nn.bin"), dtype=np.int32).reshape(-1, 3)
             res["priori"] = np.fromfile(os.path.join(d, "priori.bin"), dtype=np.int8).reshape(-1, nf, n)
@@ -206,6 +224,13 @@ NN_CASES = {
     "nn_r2_n128_ref_bl47": ("nn_r2_n128_ref", "blast", 47, 25),
     "nncc_r1_n64_neu_fq48": ("nncc_r1_n64_neu", "four_quadrant", 48, 30),
 }
+# write_NN_dataset = .true.: name: (config, IC kind, seed, steps).  The
+# reference keeps every order-1 sample and a random_number-drawn share of the
+# order-3 ones, so the fixture stores the order-1 rows in the order written,
+# the counters and the order-3 count.
+DATASET_CASES = {
+    "dset_r1_n64_neu_fq75": ("dset_r1_n64_neu", "four_quadrant", 75, 8),
+}
 # long runs stored as checksums: name: (config, IC kind, seed, steps)
 HASH_CASES = {
     "r2_n256_neu_fq0_300": ("r2_n256_neu", "four_quadrant", 0, 300),
@@ -282,6 +307,25 @@ def main(argv):
             priori=r["priori"], **w, **{k: v for k, v in r["tables"].items()})
         print("wrote", name, "a priori", r["nn"][:, 0].sum(), "a posteriori", r["nn"][:, 1].sum(),
               "nan passes", r["nn"][-1, 2])
+    for name, (cfg, kind, seed, nsteps) in DATASET_CASES.items():
+        if want and name not in want:
+            continue
+        n = build_ref.CONFIGS[cfg][0]
+        U0 = ic.make(kind, n, seed)
+        r = run_ref(cfg, U0, nsteps, 0)
+        ds = r["dataset"]
+        rows = ds["nr0"] + ds["nr1"]
+        assert ds["n_overwrite"] == 0 and ds["index"] == rows + 1, "dataset fixture must not wrap"
+        lab = ds["labels"][:rows]
+        r1 = lab[:, 0] == 1.0
+        _, mord, ngp, bc, dmp, u2, u2tol, flux = build_ref.config(cfg)
+        np.savez_compressed(
+            os.path.join(OUT, f"{name}.npz"), n=n, mord=mord, ngp=ngp, bc=bc, dmp=dmp, u2=u2, u2tol=u2tol, flux=flux,
+            kind=kind, seed=seed, nsteps=nsteps, ic_sha=sha(U0), U0=U0, dt=r["steps"]["dt"], counts=r["steps"]["cnt"],
+            masks=r["masks"], U=r["U"], L=ds["L"], dataset_size=ds["size"], nr0=ds["nr0"], nr1=ds["nr1"],
+            index=ds["index"], n_overwrite=ds["n_overwrite"], steps_nn_sample=ds["steps_NN_sample"],
+            order1_rows=ds["inputs"][:rows][r1], **{k: v for k, v in r["tables"].items()})
+        print("wrote", name, "rows", rows, "order-1", int(r1.sum()))
     for name, (cfg, kind, seed, nsteps) in HASH_CASES.items():
         if want and name not in want:
             continue

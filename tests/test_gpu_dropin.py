@@ -72,3 +72,43 @@ def test_fortran_dropin_equals_reference(cfg, fixture):
     assert np.array_equal(steps["cnt"], z["counts"])
     assert np.array_equal(masks, z["masks"])
     assert np.array_equal(U, z["U"])
+
+
+def test_fortran_dropin_writes_the_reference_dataset():
+    """write_NN_dataset = .true. through the drop-in: after each step's first
+    stage the training samples land in the reference's arrays inputs / labels
+    with index, n_overwrite, NR0 and NR1 as append_to_NN_datatset leaves them
+    (mod_append_NN_dataset.f90:11-79).  The reference keeps every order-1
+    sample, so those rows and their order must equal the reference run's; the
+    order-3 samples are a random share in both (the reference draws Fortran
+    random_number, the library a seeded generator)."""
+    if not has_gpu():
+        pytest.skip("no GPU")
+    exe = os.path.join(ROOT, "oracle", "_ref", "dset_r1_n64_neu_dropin", "gpm_dropin.exe")
+    if not os.path.exists(exe):
+        pytest.skip("drop-in binary not built (needs the reference sources at build time)")
+    z = golden("dset_r1_n64_neu_fq75.npz")
+    n, nsteps = int(z["n"]), int(z["nsteps"])
+    print(f"[case Fortran drop-in dataset, {nsteps} steps]", flush=True)
+    with tempfile.TemporaryDirectory() as d:
+        np.ascontiguousarray(z["U0"]).tofile(os.path.join(d, "ic.bin"))
+        r = subprocess.run([exe, os.path.join(d, "ic.bin"), str(nsteps), d, "0"], capture_output=True, text=True,
+                           preexec_fn=_stack, timeout=240, cwd=d)
+        assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
+        steps = np.fromfile(os.path.join(d, "steps.bin"), dtype=STEP_TYPE)
+        U = np.fromfile(os.path.join(d, "U_final.bin"), dtype=np.float64).reshape(n + 8, n + 8, 4)
+        b = open(os.path.join(d, "dataset.bin"), "rb").read()
+    index, n_over, nr0, nr1, size, L = (int(v) for v in np.frombuffer(b[:24], dtype=np.int32))
+    inputs = np.frombuffer(b[24:24 + 4 * size * L], dtype=np.float32).reshape(L, size).T
+    labels = np.frombuffer(b[24 + 4 * size * L:24 + 4 * size * L + 8 * size], dtype=np.float32).reshape(2, size).T
+    sample_steps = np.frombuffer(b[24 + 4 * size * L + 8 * size:], dtype=np.int32)
+    assert np.array_equal(steps["dt"], z["dt"]) and np.array_equal(U, z["U"])
+    assert (size, L) == (int(z["dataset_size"]), int(z["L"]))
+    rows = nr0 + nr1
+    assert n_over == 0 and index == rows + 1
+    assert nr0 == int(z["nr0"]) and nr1 > 0
+    assert np.all(labels[rows:] == -666) and np.all(inputs[rows:] == -666)
+    lab = labels[:rows]
+    assert np.all((lab[:, 0] + lab[:, 1]) == 1.0)
+    assert np.array_equal(inputs[:rows][lab[:, 0] == 1.0], z["order1_rows"])
+    assert np.array_equal(sample_steps, np.ones(nsteps, dtype=np.int32))

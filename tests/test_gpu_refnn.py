@@ -166,3 +166,49 @@ def test_reference_dataset_samples(gpm, orc, tmp_path):
     m = gpm.h5io.meta(n, n, "2DRP3", gpm.capi.GP_MOOD)
     gpm.h5io.write_nn_dataset(str(tmp_path / "ds.h5"), m, X, Y, nr0=n0, nr1=n1)
     assert os.path.getsize(str(tmp_path / "ds.h5")) > X.nbytes
+
+
+def test_reference_dataset_arrays_wrap_as_the_reference(gpm):
+    """gpm_set_reference_dataset / gpm_append_reference_dataset with a small
+    dataset_size: the arrays act as the reference's ring (index wraps to 1,
+    n_overwrite counts the wraps, mod_append_NN_dataset.f90:64-71), and hold
+    the last dataset_size rows of an unwrapped run with the same draws."""
+    import ctypes
+    if not has_gpu():
+        pytest.skip("no GPU")
+    n, L, steps, small = 64, 52, 6, 100
+    print(f"[case reference dataset ring, {n}^2 R1, {steps} steps, dataset_size {small}]", flush=True)
+    U0 = gpm.ic.make("four_quadrant", n, 76)
+    lib = gpm.capi.load()
+    runs = {}
+    for size in (100000, small):
+        s = gpm.Solver(gpm.Config(lf=n, Mord=3, ngp=2, BC_type="neumann"))
+        U = s.full_shape()
+        Uf = np.zeros(U)
+        Uf[4:-4, 4:-4] = U0
+        gpm.boundary_c(s.config, Uf, 0.0)
+        s.set_state(Uf)
+        inputs = np.full((L, size), -666.0, dtype=np.float32)   # Fortran inputs(size, L), column-major
+        labels = np.full((2, size), -666.0, dtype=np.float32)
+        cnt = {k: ctypes.c_int32(v) for k, v in (("index", 1), ("n_over", 0), ("nr0", 0), ("nr1", 0))}
+        assert lib.gpm_set_reference_dataset(s._h, L, 0, 0) == 0
+        t = 0.0
+        for _ in range(steps):
+            dt = s.setdt()
+            Uf, _, _ = s.time_stepping(Uf, t, dt)
+            t += dt
+            assert lib.gpm_append_reference_dataset(s._h, gpm.capi.ptr(inputs), gpm.capi.ptr(labels), size,
+                                                    ctypes.byref(cnt["index"]), ctypes.byref(cnt["n_over"]),
+                                                    ctypes.byref(cnt["nr0"]), ctypes.byref(cnt["nr1"])) == 0
+        runs[size] = (inputs.T.copy(), labels.T.copy(), {k: v.value for k, v in cnt.items()})
+        s.close()
+    big_in, big_lab, cb = runs[100000]
+    ring_in, ring_lab, cr = runs[small]
+    total = cb["nr0"] + cb["nr1"]
+    assert total > 3 * small and cb["n_over"] == 0 and cb["index"] == total + 1
+    assert (cr["nr0"], cr["nr1"]) == (cb["nr0"], cb["nr1"])
+    assert cr["n_over"] == total // small and cr["index"] == total % small + 1
+    # row r goes to slot r mod size: slot k holds the last such row before `total`
+    for k in range(small):
+        r = k + small * ((total - 1 - k) // small)
+        assert np.array_equal(ring_in[k], big_in[r]) and np.array_equal(ring_lab[k], big_lab[r])
