@@ -1577,6 +1577,13 @@ int gpm_run_noalloc(const gpm_config *cfg, const double *zT_sp, const double *gp
     return GPM_OK;
 }
 
+int gpm_ddiv(int64_t n, const double *a_dev, const double *b_dev, double *q_dev)
+{
+    if (n < 0 || (n > 0 && (!a_dev || !b_dev || !q_dev))) return GPM_EINVAL;
+    if (launch_ddiv(a_dev, b_dev, q_dev, n, nullptr) != hipSuccess) return GPM_EHIP;
+    return hipDeviceSynchronize() == hipSuccess ? GPM_OK : GPM_EHIP;
+}
+
 int gpm_hllc_vjp(int32_t dir, int32_t n, const double *ul, const double *ur, const double *fbar, double *ulbar,
                  double *urbar)
 {

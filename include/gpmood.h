@@ -373,6 +373,10 @@ int gpm_get_cellgpo_priori(gpm_ctx *ctx, int32_t *priori);
 int gpm_nn_expf_bits(gpm_ctx *ctx, uint32_t first, int64_t n, float *y_dev);
 /* the same function compiled for the host, into host memory y */
 int gpm_nn_expf_bits_host(uint32_t first, int64_t n, float *y);
+/* The step's FP64 division a / b (correctly rounded, as the reference
+ * build's IEEE division) on n pairs in device memory, for tests.  No
+ * counterpart in the reference.  Synchronous. */
+int gpm_ddiv(int64_t n, const double *a_dev, const double *b_dev, double *q_dev);
 
 /* ------------------------------------------------------------------------
  * Differentiable form.  No counterpart in the reference: the adjoint of K

@@ -1,0 +1,65 @@
+"""The step's FP64 division (ddiv, gp-mood_amd/csrc/gpm_device.h) against the
+host's IEEE division, bit for bit.  The compiler's own FP64 division for
+gfx950 is faithful but not correctly rounded (the documented counterexample
+below), and the reference's results depend on every quotient of the HLLC
+solver, so every division of the step goes through ddiv."""
+import numpy as np
+import pytest
+
+from conftest import has_gpu
+
+pytestmark = pytest.mark.gpu
+
+
+def _cases():
+    rng = np.random.default_rng(2024)
+    A, B = [], []
+    # the documented counterexample of the compiler's division
+    A.append(np.array([-float.fromhex("0x1.6666666666663p+0")]))
+    B.append(np.array([-float.fromhex("0x1.ffffffffffffbp-1")]))
+    # divisors a few ulps from +-1 (the counterexample's family: near-uniform
+    # HLLC star states give S / (S - u*) with S - u* close to S)
+    a = 1.0 + rng.random(1 << 14)
+    for k in range(1, 65):
+        for b in (1.0 - k * 2.0 ** -53, 1.0 + k * 2.0 ** -52):
+            for sgn in (1.0, -1.0):
+                A.append(a * (1.0 if k % 2 else -1.0))
+                B.append(np.full_like(a, sgn * b))
+    # all-ones divisor significands across the exponent range, inside and
+    # outside the 2^+-900 guard of the exact fallback
+    for e in (-1022, -1000, -901, -900, -899, -100, -1, 0, 1, 100, 899, 900, 901, 1000, 1023):
+        b = (2.0 - 2.0 ** -52) * 2.0 ** e
+        aa = rng.random(4096) * 2.0 ** rng.integers(-40, 40, 4096)
+        A.append(aa)
+        B.append(np.full_like(aa, b))
+        A.append(-aa)
+        B.append(np.full_like(aa, -b))
+    # random bit patterns: every exponent, denormals, infinities and NaN included
+    bits = rng.integers(0, 2 ** 63 - 1, size=(2, 1 << 20), dtype=np.int64) * rng.choice([1, -1], (2, 1 << 20))
+    A.append(bits[0].view(np.float64))
+    B.append(bits[1].view(np.float64))
+    # special values against each other
+    sp = np.array([0.0, -0.0, 5e-324, -5e-324, 2.2250738585072009e-308, 2.2250738585072014e-308,
+                   1.7976931348623157e308, -1.7976931348623157e308, np.inf, -np.inf, np.nan, 1.0, -3.0, 0.1])
+    A.append(np.repeat(sp, sp.size))
+    B.append(np.tile(sp, sp.size))
+    return np.concatenate(A), np.concatenate(B)
+
+
+def test_ddiv_equals_host_division(gpm):
+    if not has_gpu():
+        pytest.skip("no GPU")
+    import torch
+    a, b = _cases()
+    print(f"[case ddiv vs host a/b, {a.size} pairs]", flush=True)
+    with np.errstate(all="ignore"):
+        ref = a / b
+    da, db = torch.from_numpy(a).cuda(), torch.from_numpy(b).cuda()
+    dq = torch.empty_like(da)
+    torch.cuda.synchronize()
+    assert gpm.capi.load().gpm_ddiv(a.size, da.data_ptr(), db.data_ptr(), dq.data_ptr()) == 0
+    q = dq.cpu().numpy()
+    nan = np.isnan(ref)
+    assert np.array_equal(np.isnan(q), nan)
+    bad = np.nonzero(q[~nan].view(np.int64) != ref[~nan].view(np.int64))[0]
+    assert bad.size == 0, f"{bad.size} quotients differ, e.g. {a[~nan][bad[:3]]} / {b[~nan][bad[:3]]}"
