@@ -153,6 +153,15 @@ class Solver:
         self._check(self.lib.gpm_get_state(self._h, capi.ptr(U), capi.ptr(g)), "gpm_get_state")
         return (U, g) if order else U
 
+    def set_dt_ghosts(self, U):
+        """Ghost cells of the full array U (nf+8, lf+8, 4) that the next Setdt
+        reads once in place of the resident state's (the reference's restart:
+        main.f90:124-151)."""
+        U = np.ascontiguousarray(U, dtype=np.float64)
+        if U.shape != self.full_shape():
+            raise ValueError(f"state shape {U.shape} != {self.full_shape()}")
+        self._check(self.lib.gpm_set_dt_ghosts(self._h, capi.ptr(U)), "gpm_set_dt_ghosts")
+
     def set_state_device(self, u_dev_ptr):
         self._check(self.lib.gpm_set_state_device(self._h, ctypes.c_void_p(u_dev_ptr)), "gpm_set_state_device")
 

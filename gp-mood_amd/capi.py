@@ -38,7 +38,7 @@ CASCADE_IDS = {"reference": 0, "full": 1}
 # Every symbol include/gpmood.h declares.
 EXPORTS = [
     "gpm_create", "gpm_destroy", "gpm_last_error", "gpm_set_gp_tables", "gpm_gp_tables", "gpm_boundary_c",
-    "gpm_set_state", "gpm_get_state", "gpm_set_state_device", "gpm_get_state_device",
+    "gpm_set_state", "gpm_get_state", "gpm_set_dt_ghosts", "gpm_set_state_device", "gpm_get_state_device",
     "gpm_compute_dt", "gpm_time_stepping", "gpm_run", "gpm_run_noalloc", "gpm_get_stats", "gpm_reset_stats",
     "gpm_set_time", "gpm_stream", "gpm_set_timing", "gpm_kernel_time", "gpm_set_mood_launch", "gpm_get_mood_launch",
     "gpm_mlp_create", "gpm_mlp_destroy", "gpm_mlp_last_error", "gpm_mlp_num_params", "gpm_mlp_set_params",
@@ -104,6 +104,7 @@ def load():
         "gpm_boundary_c": ([ctypes.POINTER(GpmConfig), D, P], I),
         "gpm_set_state": ([P, P], I),
         "gpm_get_state": ([P, P, P], I),
+        "gpm_set_dt_ghosts": ([P, P], I),
         "gpm_set_state_device": ([P, P], I),
         "gpm_get_state_device": ([P, P, P], I),
         "gpm_compute_dt": ([P, D, ctypes.POINTER(D)], I),

@@ -42,6 +42,8 @@ struct gpm_ctx {
     double4 *dU = nullptr, *dA = nullptr, *dB = nullptr;
     double4 *dC = nullptr, *dD = nullptr, *dE = nullptr, *dF = nullptr;  // SSP_RK4 stage states
     double4 *ring = nullptr, *band = nullptr, *band3 = nullptr;  // band3: SSP_RK4's fourth stage
+    double4 *dt_ring = nullptr;  // copy boundaries: ghost cells the next Setdt reads once (gpm_set_dt_ghosts)
+    bool dt_ring_once = false;
     StepExtra *ext = nullptr;  // [kMaxStages] in device memory (plan_stages' per-stage extras)
     order_t *gpo = nullptr;
     uint8_t *dflag = nullptr;  // the first pass's record of the stage-input parts of each cell's decision
@@ -270,7 +272,8 @@ int launch_dt_kernel(gpm_ctx *c, int mode, int use_tmax, double tmax, double dtf
 {
     DtArgs a{};
     a.U = c->dU;
-    a.ring = c->ring_mode ? c->ring : nullptr;
+    a.ring = c->ring_mode ? c->ring : (c->dt_ring_once && mode == kDtCompute ? c->dt_ring : nullptr);
+    if (mode == kDtCompute) c->dt_ring_once = false;
     a.block_min = c->block_min;
     a.done = c->done;
     a.tstate = c->tstate;
@@ -798,7 +801,8 @@ int gpm_destroy(gpm_ctx *c)
 {
     if (!c) return GPM_OK;
     if (c->stream) hipStreamSynchronize(c->stream);
-    void *bufs[] = {c->dT, c->dU, c->dA, c->dB, c->dC, c->dD, c->dE, c->dF, c->ring, c->band, c->band3, c->st_dev, c->ext,
+    void *bufs[] = {c->dT, c->dU, c->dA, c->dB, c->dC, c->dD, c->dE, c->dF, c->ring, c->dt_ring, c->band, c->band3,
+                    c->st_dev, c->ext,
                     c->gpo, c->dflag, c->slistA, c->slistB, c->mark, c->counters,
                     c->stamp, c->stats, c->tstate, c->block_min, c->dt_log, c->done, c->pred, c->nnlist,
                     c->nncount, c->adj_ub, c->adj_G, c->adj_A, c->adj_tmp, c->adj_ordx, c->adj_ordy,
@@ -853,6 +857,25 @@ int gpm_set_state(gpm_ctx *c, const double *U)
     HIPCHK(c, hipMemsetD32Async(c->gpo, pack_order(c->mord, c->mord, 0), n, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
     c->has_state = true;
+    c->dt_ring_once = false;
+    return GPM_OK;
+}
+
+int gpm_set_dt_ghosts(gpm_ctx *c, const double *U)
+{
+    if (!c || !U) return fail(c, GPM_EINVAL, "null argument");
+    if (!c->has_state) return fail(c, GPM_ESTATE, "no state set");
+    const size_t full = (size_t)(c->NX + 8) * (c->NY + 8);
+    if (!c->xfer) HIPCHK(c, dalloc(&c->xfer, full));
+    double4 *ring = c->ring;
+    if (!c->ring_mode) {
+        if (!c->dt_ring) HIPCHK(c, dalloc(&c->dt_ring, (size_t)ring_size(c->NX, c->NY)));
+        ring = c->dt_ring;
+    }
+    HIPCHK(c, hipMemcpyAsync(c->xfer, U, full * sizeof(double4), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, launch_unpack_ghosted(c->xfer, nullptr, ring, c->NX, c->NY, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    if (!c->ring_mode) c->dt_ring_once = true;
     return GPM_OK;
 }
 
@@ -885,6 +908,7 @@ int gpm_set_state_device(gpm_ctx *c, const double *u_dev)
     int rc = enqueue_ring_fill(c);  // ghost cells by Boundary_C at the current time
     if (rc) return rc;
     c->has_state = true;
+    c->dt_ring_once = false;
     return GPM_OK;
 }
 

@@ -281,8 +281,11 @@ __global__ void k_unpack_ghosted(const double4 *__restrict__ in, double4 *__rest
     for (long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x; c < n;
          c += (long long)gridDim.x * blockDim.x) {
         const int i = (int)(c % W) - 4, j = (int)(c / W) - 4;
-        if (i >= 0 && i < NX && j >= 0 && j < NY) U[(size_t)j * NX + i] = in[c];
-        else if (ring) ring[ring_index(i, j, NX, NY)] = in[c];
+        if (i >= 0 && i < NX && j >= 0 && j < NY) {
+            if (U) U[(size_t)j * NX + i] = in[c];
+        } else if (ring) {
+            ring[ring_index(i, j, NX, NY)] = in[c];
+        }
     }
 }
 

@@ -160,6 +160,18 @@ int gpm_boundary_c(const gpm_config *cfg, double t_rk, double *U);
 int gpm_set_state(gpm_ctx *ctx, const double *U);
 int gpm_get_state(gpm_ctx *ctx, double *U, int32_t *cellgpo);
 
+/* Ghost cells that the next Setdt takes its minimum over (physics.f90:98-99
+ * reads lb:le x nb:ne), once, in place of the resident state's own: the
+ * ghost cells of U (host array in the layout above; its interior is
+ * ignored).  For the reference's restart (main.f90:124-151): InitialCdt and
+ * Boundary_C fill U, read() (reader.f90:12-120) overwrites the interior only,
+ * and the first Setdt sees the initial condition's ghost cells.  Call after
+ * the state is set.  For formula boundaries and problem RT these become the
+ * ghost values the library keeps anyway (see gpm_get_state); for copy
+ * boundaries they serve the next Setdt only (the stage's Boundary_C refills
+ * every ghost cell before it is read again). */
+int gpm_set_dt_ghosts(gpm_ctx *ctx, const double *U);
+
 /* Device-resident state transfer: `u_dev` points to nf*lf*4 doubles of device
  * memory holding the interior cells [n][l][k] (no ghosts).  For formula
  * boundaries gpm_set_state_device fills the ghost cells by Boundary_C at the

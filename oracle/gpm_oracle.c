@@ -61,6 +61,7 @@ typedef struct {
     int method;                /* METHOD_GP_MOOD or METHOD_FOG */
     int dmp, u2, u2tol;        /* MOOD switches (problems/<name>.f90: DMP, U2, U2_tol) */
     int cascade;               /* 0: reference (>3 -> 3, 3 -> 1); 1: full (7->5->3->1) */
+    int skip_initial_bc;       /* the next run starts from U's ghost cells as given (a restart) */
     double cfl, dx, dy;
     double t_rk;               /* global_variables.f90:33, time seen by the DMR boundary */
     /* Tables in the reference's Fortran layouts (global_variables.f90:37-38,53,61):
@@ -1121,7 +1122,10 @@ int orc_run2(orc_t *o, double *U, int nsteps, double *t_io, double tmax, double 
     const size_t n4 = (size_t)o->W * o->H * 4;
     double *Ur = (double *)calloc(n4, sizeof(double));
     double t = t_io ? *t_io : 0.0;
-    orc_bc(o, U);
+    /* Boundary_C(U) of main.f90:125; a restart reads its state after it
+       (main.f90:132-136) and keeps the ghost cells it left */
+    if (!o->skip_initial_bc) orc_bc(o, U);
+    o->skip_initial_bc = 0;
     for (int s = 0; s < nsteps; ++s) {
         o->niter++;
         double dt = orc_setdt(o, U, tmax - t);
@@ -1225,6 +1229,10 @@ int orc_set_refnn(orc_t *o, int L, int length, const float *w0, const float *b0,
     }
     return 0;
 }
+
+/* The next run keeps U's ghost cells as given: the restart of
+   main.f90:124-151, where read() replaces the interior after Boundary_C. */
+void orc_skip_initial_bc(orc_t *o) { o->skip_initial_bc = 1; }
 
 /* niter of main.f90 (steps started so far; the next orc_run step is niter+1). */
 void orc_set_niter(orc_t *o, int niter) { o->niter = niter; }

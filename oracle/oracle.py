@@ -53,6 +53,7 @@ def lib():
         L.orc_run2.argtypes = [P, P, I, P, D, P, P, P, P, P, P, P]
         L.orc_set_refnn.argtypes = [P, I, I, P, P, P, P, P, P]
         L.orc_set_niter.argtypes = [P, I]
+        L.orc_skip_initial_bc.argtypes = [P]
         L.orc_get_niter.argtypes = [P]
         L.orc_error.argtypes = [P]
         L.orc_nn_forward.argtypes = [P, P, P]
@@ -110,9 +111,13 @@ class Oracle:
     def set_t_rk(self, t_rk):
         lib().orc_set_t_rk(self._h, float(t_rk))
 
-    def run(self, U, nsteps, t=0.0, tmax=1e10, masks=False):
-        """Advance full array U (modified in place) by nsteps steps."""
+    def run(self, U, nsteps, t=0.0, tmax=1e10, masks=False, initial_bc=True):
+        """Advance full array U (modified in place) by nsteps steps.  With
+        initial_bc=False the first Setdt reads U's ghost cells as given (the
+        reference's restart: main.f90:124-151) instead of Boundary_C's."""
         assert U.flags.c_contiguous and U.dtype == np.float64
+        if not initial_bc:
+            lib().orc_skip_initial_bc(self._h)
         dts = np.zeros(nsteps)
         counts = np.zeros(nsteps, dtype=np.int64)
         passes = np.zeros((nsteps, 3), dtype=np.int32)

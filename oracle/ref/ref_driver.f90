@@ -24,7 +24,11 @@
 ! The NN methods load their network with load_NN (mod_NN.f90:60-117) first, as
 ! main.f90:116 does.
 !
-! Usage: gpm_ref.exe <ic.bin> <nsteps> <outdir> <mode>
+! Usage: gpm_ref.exe <ic.bin> <nsteps> <outdir> <mode> [restart.bin]
+!   restart.bin (optional): the restart of main.f90:124-136 -- after the
+!           initial condition and Boundary_C(U), the interior of U is
+!           overwritten from this file (conserved, as read() leaves it) with
+!           no refill of the ghost cells, before the step loop
 !   mode 0: call time_stepping (time_stepper.f90:12-23) exactly as main.f90:155
 !   mode 1: the three Forward_Euler stages of SSP_RK3_ (mod_SSP_RK3.f90:12-39)
 !           written out here so that the order mask can be dumped per stage
@@ -79,8 +83,15 @@ program gpm_ref_driver
    close(uo)
    U = 0.
    U(:,1:lf,1:nf) = Uint
-   deallocate(Uint)
    call Boundary_C(U)
+   if (command_argument_count() >= 5) then
+      call get_command_argument(5, arg)
+      open(newunit=uo, file=trim(arg), access='stream', form='unformatted', status='old')
+      read(uo) Uint
+      close(uo)
+      U(:,1:lf,1:nf) = Uint
+   end if
+   deallocate(Uint)
 
    allocate(m8(lf,nf))
    if (mode == 1) then

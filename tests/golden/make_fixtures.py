@@ -61,9 +61,10 @@ def read_dataset(path, nsteps):
                 labels=labels.copy(), steps_NN_sample=steps.copy())
 
 
-def run_ref(cfg, U0, nsteps, mode, nn=None):
+def run_ref(cfg, U0, nsteps, mode, nn=None, restart=None):
     """Run the reference build of `cfg`; `nn`: the network (refnn_weights) that
-    load_NN reads from refnn.txt in the working directory."""
+    load_NN reads from refnn.txt in the working directory; `restart`: an
+    interior state read over U0's after its Boundary_C (main.f90:124-136)."""
     n = build_ref.CONFIGS[cfg][0]
     nf = build_ref.extras(cfg)["nf"]
     exe = os.path.join(REF, cfg, "gpm_ref.exe")
@@ -73,7 +74,11 @@ def run_ref(cfg, U0, nsteps, mode, nn=None):
         U0.tofile(os.path.join(d, "ic.bin"))
         if nn is not None:
             write_refnn_text(os.path.join(d, "refnn.txt"), nn)
-        subprocess.run(f"ulimit -s unlimited; {exe} {d}/ic.bin {nsteps} {d} {mode}", shell=True, check=True,
+        extra = ""
+        if restart is not None:
+            restart.tofile(os.path.join(d, "restart.bin"))
+            extra = f" {d}/restart.bin"
+        subprocess.run(f"ulimit -s unlimited; {exe} {d}/ic.bin {nsteps} {d} {mode}{extra}", shell=True, check=True,
                        stdout=subprocess.DEVNULL, cwd=d)
         res = {"tables": read_tables(os.path.join(d, "tables.bin")),
                "pol": np.fromfile(os.path.join(d, "pol.bin"), dtype=np.float64),
@@ -231,6 +236,14 @@ NN_CASES = {
 DATASET_CASES = {
     "dset_r1_n64_neu_fq75": ("dset_r1_n64_neu", "four_quadrant", 75, 8),
 }
+# restarts (main.f90:124-151): name: (config, initial condition (kind, seed),
+# restart state (kind, seed), steps).  The restart state replaces the
+# interior after the initial condition's Boundary_C; the first Setdt reads
+# the initial condition's ghost cells.
+RESTART_CASES = {
+    "restart_r2_n64_neu_fq77": ("r2_n64_neu", ("four_quadrant", 77), ("four_quadrant", 78), 10),
+    "restart_r1_dmr_64x16": ("r1_dmr_64x16", ("mach800:0", 0), ("dmr:0.05", 79), 10),
+}
 # long runs stored as checksums: name: (config, IC kind, seed, steps)
 HASH_CASES = {
     "r2_n256_neu_fq0_300": ("r2_n256_neu", "four_quadrant", 0, 300),
@@ -326,6 +339,26 @@ def main(argv):
             index=ds["index"], n_overwrite=ds["n_overwrite"], steps_nn_sample=ds["steps_NN_sample"],
             order1_rows=ds["inputs"][:rows][r1], **{k: v for k, v in r["tables"].items()})
         print("wrote", name, "rows", rows, "order-1", int(r1.sum()))
+    for name, (cfg, (k0, s0), (k1, s1), nsteps) in RESTART_CASES.items():
+        if want and name not in want:
+            continue
+        n = build_ref.CONFIGS[cfg][0]
+        e = build_ref.extras(cfg)
+
+        def state(kind, seed):
+            if ":" in kind:
+                pk, noise = kind.split(":")
+                return ic.problem(pk, n, e["nf"], float(e["lx"]), float(e["ly"]), seed=seed, noise=float(noise))
+            return ic.make(kind, n, seed)
+        U0, R = state(k0, s0), state(k1, s1)
+        r = run_ref(cfg, U0, nsteps, 0, restart=R)
+        _, mord, ngp, bc, dmp, u2, u2tol, flux = build_ref.config(cfg)
+        np.savez_compressed(
+            os.path.join(OUT, f"{name}.npz"), n=n, mord=mord, ngp=ngp, bc=bc, dmp=dmp, u2=u2, u2tol=u2tol, flux=flux,
+            nf=e["nf"], lx=float(e["lx"]), ly=float(e["ly"]), integrator=e["integrator"], problem=0,
+            method=e["method"].lower(), nsteps=nsteps, U0=U0, U_restart=R, dt=r["steps"]["dt"], t=r["steps"]["t"],
+            counts=r["steps"]["cnt"], masks=r["masks"], U=r["U"], **{k: v for k, v in r["tables"].items()})
+        print("wrote", name, "first dt", r["steps"]["dt"][0])
     for name, (cfg, kind, seed, nsteps) in HASH_CASES.items():
         if want and name not in want:
             continue

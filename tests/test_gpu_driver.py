@@ -129,3 +129,69 @@ def test_driver_with_reference_network(gpm, tmp_path, monkeypatch):
     snap = written[0]
     assert np.array_equal(int_series(snap, "ordr").reshape(nf, lf), z["masks"][-1])
     assert np.array_equal(int_series(snap, "ordr_priori").reshape(nf, lf), z["priori"][-1])
+
+
+@pytest.mark.parametrize("case", ["restart_r2_n64_neu_fq77", "restart_r1_dmr_64x16"])
+def test_restart_first_setdt_reads_initial_ghosts(gpm, case):
+    """main.f90:124-151: after InitialCdt and Boundary_C, read() replaces the
+    interior only, so the first Setdt after a restart sees the initial
+    condition's ghost cells.  gpm_set_dt_ghosts gives them to the solver:
+    dt sequence, last mask and state equal the reference's restart run
+    (a copy boundary and the DMR formula boundary)."""
+    if not has_gpu():
+        pytest.skip("no GPU")
+    z = golden(f"{case}.npz")
+    print(f"[case restart {case}]", flush=True)
+    o = case_options(z)
+    cfg = gpm.Config(lf=int(z["n"]), nf=o["nf"], Mord=int(z["mord"]), ngp=int(z["ngp"]), BC_type=str(z["bc"]),
+                     numFlux=o["flux"], Lx=o["lx"], Ly=o["ly"])
+    s = gpm.Solver(cfg, tables_from_npz(z))
+    Uic = np.zeros(s.full_shape())
+    Uic[4:-4, 4:-4] = z["U0"]
+    gpm.boundary_c(cfg, Uic, 0.0)
+    s.set_state(Uic)
+    s.set_time(0.0)
+    s.set_state(np.ascontiguousarray(z["U_restart"]))
+    s.set_dt_ghosts(Uic)
+    dts = s.run(int(z["nsteps"]), return_dts=True)
+    U, g = s.get_state(order=True)
+    assert np.array_equal(dts, z["dt"])
+    assert np.array_equal(g[4:-4, 4:-4], z["masks"][-1])
+    assert np.array_equal(U, z["U"])
+
+
+def test_driver_restart_matches_oracle_restart(gpm, tmp_path):
+    """driver.run_problem(restart=...) from a snapshot file: the solver's state
+    before the call plays the initial condition (main.f90:124-125), the file's
+    primitives replace the interior (reader.f90), and the run equals the
+    oracle's from that array with the initial condition's ghost cells kept."""
+    if not has_gpu():
+        pytest.skip("no GPU")
+    try:
+        gpm.h5io.load()
+    except RuntimeError as e:
+        pytest.skip(str(e))
+    from conftest import load_oracle
+    O = load_oracle()
+    z = golden("restart_r2_n64_neu_fq77.npz")
+    print("[case driver restart from a snapshot, 64^2 R2]", flush=True)
+    n, nsteps = int(z["n"]), 6
+    cfg = gpm.Config(lf=n, Mord=5, ngp=3, BC_type="neumann")
+    m = gpm.h5io.meta(n, n, "2DRP3", gpm.capi.GP_MOOD)
+    Ur = np.zeros((n + 8, n + 8, 4))
+    Ur[4:-4, 4:-4] = z["U_restart"]
+    snap = str(tmp_path / "restart.h5")
+    gpm.h5io.write_output(snap, m, Ur, np.full((n + 8, n + 8), 5, dtype=np.int32), 40, 0.25, mord=5)
+    s = gpm.Solver(cfg, tables_from_npz(z))
+    s.set_state(z["U0"])
+    t, niter, _ = gpm.driver.run_problem(s, "2DRP3", tmax=1e10, nmax=40 + nsteps - 1, outdir=str(tmp_path / "out"),
+                                         restart=snap, write_diagnostic=False)
+    assert niter == 40 + nsteps
+    U = s.get_state()
+    U_read, t0, it0 = gpm.h5io.read_restart(snap, n, n)
+    o = O.Oracle(n, 5, 3, "neumann", tables_from_npz(z))
+    Uo = o.full(z["U0"])
+    Uo[4:-4, 4:-4] = U_read[4:-4, 4:-4]
+    r = o.run(Uo, nsteps, t=t0, initial_bc=False)
+    assert t == r["t"]
+    assert np.array_equal(U, Uo)
