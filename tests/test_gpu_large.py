@@ -1,6 +1,7 @@
 """The large configuration at its full size: 8192^2 FP64, GP-R3 with the full
-cascade, periodic shear layer (2.1 GB per state), plain GP-MOOD and
-NN-GP-MOOD with the committed GP-R3 predictor.  Size-independent properties:
+cascade, periodic shear layer (2.1 GB per state; seed 2000, which no training
+run of the predictor generated), plain GP-MOOD and NN-GP-MOOD with the
+committed GP-R3 predictor.  Size-independent properties:
 the totals of mass, momentum and energy stay constant up to the reference's
 seam effect (ghost cells keep order Mord, so the two evaluations of a seam
 face differ once MOOD lowers a cell next to the seam; see
@@ -23,8 +24,8 @@ def test_8192_shear_layer_plain_and_nn(gpm):
     import torch
 
     n, steps = 8192, 10
-    print(f"[case 8192^2 GP-R3 periodic shear layer, {steps} steps]", flush=True)
-    U0 = torch.from_numpy(gpm.ic.make("shear_layer", n, 0)).cuda()
+    print(f"[case 8192^2 GP-R3 periodic shear layer seed 2000, {steps} steps]", flush=True)
+    U0 = torch.from_numpy(gpm.ic.make("shear_layer", n, 2000)).cuda()
     tot0 = U0.sum(dim=(0, 1)).cpu().numpy()
     m = gpm.MLP.load(os.path.join(ROOT, "gp-mood_amd", "weights", "nn_r3.npz"))
     rho = {}
