@@ -316,6 +316,46 @@ def nn_comparison(gpm, n, steps, seeds=(2000, 2001, 2002, 2003)):
     return out
 
 
+BF16_PEAK = 2.5e15        # FLOP/s, MI355X dense BF16 matrix
+
+
+def mlp_train_measure(gpm, steps, batch=8192, warmup=5):
+    """AdamW training steps of the order predictor (forward, cross-entropy,
+    backward on the matrix cores, AdamW) at batch 8192 on resident random
+    features, for the GP-R2 (100 -> 3 classes) and GP-R3 (196 -> 4) networks.
+    Loss is not read back inside the timed loop.  FLOPs are the algorithmic
+    ones of the unpadded layers: forward 2*MACs, backward weights 2*MACs,
+    backward data 2*MACs of all layers but the first."""
+    import torch
+    out = {}
+    for tag, in_f, C in (("r2", 100, 3), ("r3", 196, 4)):
+        dims = (in_f, 128, 128, 64, C)
+        macs = [a * b for a, b in zip(dims[:-1], dims[1:])]
+        flop_sample = 2 * (2 * sum(macs) + sum(macs[1:]))
+        m = gpm.MLP(in_f, C, max_batch=batch, seed=1)
+        g = torch.Generator(device="cuda").manual_seed(7)
+        x = torch.rand((batch, in_f), generator=g, device="cuda", dtype=torch.float32) * 2 - 1
+        y = torch.randint(0, C, (batch,), generator=g, device="cuda", dtype=torch.int32)
+        for _ in range(warmup):
+            m.train_step(x, y, report=False)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            m.train_step(x, y, report=False)
+        torch.cuda.synchronize()
+        el = time.perf_counter() - t0
+        loss, _ = m.train_step(x, y)
+        m.close()
+        fl = flop_sample * batch * steps / el
+        out[tag] = {"value": batch * steps / el, "unit": "samples/s", "ms_per_step": 1e3 * el / steps,
+                    "batch": batch, "layers": list(dims), "flop_per_sample": flop_sample,
+                    "achieved_tflops": fl / 1e12, "bf16_peak_tflops": BF16_PEAK / 1e12, "frac": fl / BF16_PEAK,
+                    "final_loss": loss}
+    out["note"] = ("each step is 32 dependent launches (forward 4, loss 1, backward 15, AdamW 8, BF16 copies 4) "
+                   "of a few microseconds over a few MB: launch- and latency-bound, not matrix-core-bound")
+    return out
+
+
 def plain_run(gpm, n, steps, warmup, seed, cascade):
     """The headline workload (2048^2 GP-R3 reflective blast) with the given
     order cascade: `warmup` untimed steps, then `steps` timed device-resident
@@ -360,6 +400,8 @@ def main():
                     help="steps of each run of the other flux, integrator, boundary and method options (0: skip)")
     ap.add_argument("--adjoint-steps", type=int, default=200,
                     help="K of the 512^2 GP-R2 initial-state sensitivity run (0: skip)")
+    ap.add_argument("--mlp-train-steps", type=int, default=200,
+                    help="AdamW steps of the order predictor at batch 8192, GP-R2 and GP-R3 networks (0: skip)")
     ap.add_argument("--calib-copy-mib", type=int, default=0,
                     help="after the run, copy a buffer of this size with a plain device copy kernel "
                          "(reference point for scaling hardware byte counters)")
@@ -460,6 +502,10 @@ def main():
     if args.adjoint_steps > 0:
         print(f"[bench] adjoint, K={args.adjoint_steps} at 512^2 GP-R2", file=sys.stderr, flush=True)
         adj = adjoint_measure(gpm, args.adjoint_steps)
+    mlp = None
+    if args.mlp_train_steps > 0:
+        print(f"[bench] predictor training, {args.mlp_train_steps} steps", file=sys.stderr, flush=True)
+        mlp = mlp_train_measure(gpm, args.mlp_train_steps)
     cpu = None
     if not args.no_cpu_baseline:
         print("[bench] CPU baseline", file=sys.stderr, flush=True)
@@ -478,7 +524,7 @@ def main():
                  "second_pass_cells_per_step": st["second_pass_cells"] / max(1, args.steps),
                  "mood_kernel_avg_ms": kt["mood_ms"]},
         "roofline": roofline, "cpu_baseline": cpu, "nn_mood": nn, "host_array_drop_in": dropin, "adjoint": adj,
-        "large_run": large, "options": opts,
+        "large_run": large, "options": opts, "predictor_training": mlp,
     }
     print(json.dumps(out))
 

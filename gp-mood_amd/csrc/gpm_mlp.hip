@@ -1,17 +1,20 @@
 // Learned a priori order predictor: an MLP  in -> 128 -> 128 -> 64 -> C
 // (ReLU, BF16 weights and activations, FP32 accumulation) with hand-written
-// forward, backward and AdamW kernels on the gfx950 matrix cores
-// (v_mfma_f32_16x16x16_bf16), and the feature extraction shared by the
-// solver's data dump and its fused inference.
+// forward, backward and AdamW kernels on the gfx950 matrix cores, and the
+// feature extraction shared by the solver's data dump and its fused inference.
 //
-// Matrix-core fragments (16 x 16 x 16, wavefront of 64 lanes, lane l):
-//   A (16 x 16, rows x reduction): lane l holds A[l % 16][4*(l/16) + 0..3]
-//   B (16 x 16, reduction x cols): lane l holds B[4*(l/16) + 0..3][l % 16]
+// Matrix-core fragments (wavefront of 64 lanes, lane l), gfx950's
+// v_mfma_f32_16x16x32_bf16 for every reduction of a multiple of 32:
+//   A (16 x 32, rows x reduction): lane l holds A[l % 16][8*(l/16) + 0..7]
+//   B (32 x 16, reduction x cols): lane l holds B[8*(l/16) + 0..7][l % 16]
 //   D (16 x 16): lane l holds D[4*(l/16) + r][l % 16], r = 0..3
+// and v_mfma_f32_16x16x16_bf16 (4 bf16 per lane, 4*(l/16) + 0..3) for the one
+// reduction of 16 (the last layer's backward, over the 16 padded classes).
 // Weights are stored [out][in] (PyTorch Linear order), so a B fragment of
-// X W^T is four consecutive bf16 of one weight row.
+// X W^T is consecutive bf16 of one weight row.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdint>
 #include <cstring>
@@ -25,6 +28,7 @@
 namespace gpm {
 
 typedef short bf16x4 __attribute__((ext_vector_type(4)));
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
 __device__ __forceinline__ uint16_t f2bf(float f)
@@ -50,6 +54,17 @@ __device__ __forceinline__ f32x4 mfma16(bf16x4 a, bf16x4 b, f32x4 c)
     return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, c, 0, 0, 0);
 }
 
+// eight consecutive bf16 (16-byte aligned) as an x32 fragment
+__device__ __forceinline__ bf16x8 load_bf8(const uint16_t *p)
+{
+    return __builtin_bit_cast(bf16x8, *reinterpret_cast<const uint4 *>(p));
+}
+
+__device__ __forceinline__ f32x4 mfma32(bf16x8 a, bf16x8 b, f32x4 c)
+{
+    return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
+}
+
 // One wavefront: Y[16 x NP] = act(X[16 x KP] W[NP x KP]^T + b).  X and W are
 // bf16 with row strides ldx and KP; rows of W past n_valid and bias entries
 // past n_valid read as zero.  Y goes to bf16 (ldy) or, with OUTF, to fp32.
@@ -61,13 +76,14 @@ __device__ __forceinline__ void dense16(const uint16_t *X, int ldx, const uint16
 #pragma unroll
     for (int t = 0; t < NP / 16; ++t) acc[t] = f32x4{0.f, 0.f, 0.f, 0.f};
     const int r16 = lane & 15, q = lane >> 4;
+    static_assert(KP % 32 == 0, "the x32 fragments tile the reduction");
 #pragma unroll 2
-    for (int ks = 0; ks < KP / 16; ++ks) {
-        const bf16x4 a = load_bf4(X + r16 * ldx + ks * 16 + 4 * q);
+    for (int ks = 0; ks < KP / 32; ++ks) {
+        const bf16x8 a = load_bf8(X + r16 * ldx + ks * 32 + 8 * q);
 #pragma unroll
         for (int t = 0; t < NP / 16; ++t) {
-            const bf16x4 w = load_bf4(W + (size_t)(t * 16 + r16) * KP + ks * 16 + 4 * q);
-            acc[t] = mfma16(a, w, acc[t]);
+            const bf16x8 w = load_bf8(W + (size_t)(t * 16 + r16) * KP + ks * 32 + 8 * q);
+            acc[t] = mfma32(a, w, acc[t]);
         }
     }
 #pragma unroll
@@ -225,11 +241,22 @@ __global__ __launch_bounds__(256) void k_bwd_data(const uint16_t *__restrict__ d
     for (int t = 0; t < KO / 16; ++t) acc[t] = f32x4{0.f, 0.f, 0.f, 0.f};
     const int r16 = lane & 15, q = lane >> 4;
     const uint16_t *Yr = dY + (size_t)row0 * NI;
+    if constexpr (NI % 32 == 0) {
 #pragma unroll
-    for (int ks = 0; ks < NI / 16; ++ks) {
-        const bf16x4 a = load_bf4(Yr + r16 * NI + ks * 16 + 4 * q);
+        for (int ks = 0; ks < NI / 32; ++ks) {
+            const bf16x8 a = load_bf8(Yr + r16 * NI + ks * 32 + 8 * q);
 #pragma unroll
-        for (int t = 0; t < KO / 16; ++t) acc[t] = mfma16(a, load_bf4(WT + (size_t)(t * 16 + r16) * NI + ks * 16 + 4 * q), acc[t]);
+            for (int t = 0; t < KO / 16; ++t)
+                acc[t] = mfma32(a, load_bf8(WT + (size_t)(t * 16 + r16) * NI + ks * 32 + 8 * q), acc[t]);
+        }
+    } else {
+#pragma unroll
+        for (int ks = 0; ks < NI / 16; ++ks) {
+            const bf16x4 a = load_bf4(Yr + r16 * NI + ks * 16 + 4 * q);
+#pragma unroll
+            for (int t = 0; t < KO / 16; ++t)
+                acc[t] = mfma16(a, load_bf4(WT + (size_t)(t * 16 + r16) * NI + ks * 16 + 4 * q), acc[t]);
+        }
     }
 #pragma unroll
     for (int t = 0; t < KO / 16; ++t) {
@@ -274,15 +301,16 @@ __global__ __launch_bounds__(256) void k_bwd_weight(const uint16_t *__restrict__
     for (int tile = wv; tile < NT; tile += 4) {
         const int tn = tile / TK, tk = tile % TK;
         f32x4 acc = f32x4{0.f, 0.f, 0.f, 0.f};
+        static_assert(CH % 32 == 0, "the x32 fragments tile the batch chunk");
 #pragma unroll
-        for (int ms = 0; ms < CH / 16; ++ms) {
-            bf16x4 a, bb;
+        for (int ms = 0; ms < CH / 32; ++ms) {
+            uint16_t a[8], bb[8];
 #pragma unroll
-            for (int c = 0; c < 4; ++c) {
-                a[c] = (short)yT[tn * 16 + r16][ms * 16 + 4 * q + c];
-                bb[c] = (short)xT[tk * 16 + r16][ms * 16 + 4 * q + c];
+            for (int c = 0; c < 8; ++c) {
+                a[c] = yT[tn * 16 + r16][ms * 32 + 8 * q + c];
+                bb[c] = xT[tk * 16 + r16][ms * 32 + 8 * q + c];
             }
-            acc = mfma16(a, bb, acc);
+            acc = mfma32(__builtin_bit_cast(bf16x8, a), __builtin_bit_cast(bf16x8, bb), acc);
         }
         float *P = part + (size_t)blockIdx.x * NO * KP;
 #pragma unroll
@@ -295,32 +323,54 @@ __global__ __launch_bounds__(256) void k_bwd_weight(const uint16_t *__restrict__
     }
 }
 
-// Deterministic sum of the per-workgroup partials (fixed block order).
-__global__ void k_reduce_parts(const float *__restrict__ part, int nblk, int len, float *__restrict__ out)
+// Deterministic sum of the per-workgroup partials: 64 outputs per workgroup,
+// wavefront g sums partials g, g+4, g+8, ... of its output in order, and the
+// four sums are combined in a fixed order.
+__global__ __launch_bounds__(256) void k_reduce_parts(const float *__restrict__ part, int nblk, int len,
+                                                      float *__restrict__ out)
 {
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < len; i += gridDim.x * blockDim.x) {
-        float s = 0.f;
-        for (int b = 0; b < nblk; ++b) s += part[(size_t)b * len + i];
-        out[i] = s;
-    }
+    __shared__ float red[4][64];
+    const int c = threadIdx.x & 63, g = threadIdx.x >> 6;
+    const int i = blockIdx.x * 64 + c;
+    float s = 0.f;
+    if (i < len)
+        for (int b = g; b < nblk; b += 4) s += part[(size_t)b * len + i];
+    red[g][c] = s;
+    __syncthreads();
+    if (g == 0 && i < len) out[i] = (red[0][c] + red[1][c]) + (red[2][c] + red[3][c]);
 }
 
 // AdamW (decoupled weight decay, bias-corrected moments) on the FP32 master
-// parameters, as torch.optim.AdamW computes it; refreshes the bf16 copies of
-// the weights (and their transposes) used by the kernels.
-__global__ void k_adamw(float *__restrict__ p, const float *__restrict__ g, float *__restrict__ m, float *__restrict__ v,
-                        int n, float lr, float b1, float b2, float eps, float wd, float bc1, float bc2sqrt)
+// parameters, as torch.optim.AdamW computes it, over all eight tensors of the
+// network in one launch; each weight's bf16 copies (W [NO][KP] and its
+// transpose) used by the kernels are rewritten from the new value.
+struct AdamSet {
+    float *p[8], *g[8], *m[8], *v[8];
+    uint16_t *W[8], *WT[8];   // null for the biases
+    int off[9];               // tensor t has off[t+1] - off[t] elements
+    int kp[8], no[8];
+};
+
+__global__ __launch_bounds__(256) void k_adamw(const AdamSet a, float lr, float b1, float b2, float eps, float wd,
+                                               float bc1, float bc2sqrt)
 {
+    const int t = blockIdx.y;  // one tensor per grid row: uniform index into the argument arrays
+    const int n = a.off[t + 1] - a.off[t];
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-        float pi = p[i] * (1.0f - lr * wd);
-        const float gi = g[i];
-        const float mi = b1 * m[i] + (1.0f - b1) * gi;
-        const float vi = b2 * v[i] + (1.0f - b2) * gi * gi;
-        m[i] = mi;
-        v[i] = vi;
+        float pi = a.p[t][i] * (1.0f - lr * wd);
+        const float gi = a.g[t][i];
+        const float mi = b1 * a.m[t][i] + (1.0f - b1) * gi;
+        const float vi = b2 * a.v[t][i] + (1.0f - b2) * gi * gi;
+        a.m[t][i] = mi;
+        a.v[t][i] = vi;
         const float denom = sqrtf(vi) / bc2sqrt + eps;
         pi = pi - (lr / bc1) * (mi / denom);
-        p[i] = pi;
+        a.p[t][i] = pi;
+        if (a.W[t]) {
+            const uint16_t h = f2bf(pi);
+            a.W[t][i] = h;
+            a.WT[t][(size_t)(i % a.kp[t]) * a.no[t] + i / a.kp[t]] = h;
+        }
     }
 }
 
@@ -564,10 +614,10 @@ hipError_t mlp_launch_predict(const double4 *U, int N, int bc, const int *list, 
                               hipStream_t s)
 {
     if (stencil == 5)
-        hipLaunchKernelGGL((k_mlp_predict<5, 112>), dim3(kPredictBlocks), dim3(256), 0, s, U, N, bc, list, count, w, C, mord, gpo,
+        hipLaunchKernelGGL((k_mlp_predict<5, 128>), dim3(kPredictBlocks), dim3(256), 0, s, U, N, bc, list, count, w, C, mord, gpo,
                            pred, lowered);
     else if (stencil == 7)
-        hipLaunchKernelGGL((k_mlp_predict<7, 208>), dim3(kPredictBlocks), dim3(256), 0, s, U, N, bc, list, count, w, C, mord, gpo,
+        hipLaunchKernelGGL((k_mlp_predict<7, 224>), dim3(kPredictBlocks), dim3(256), 0, s, U, N, bc, list, count, w, C, mord, gpo,
                            pred, lowered);
     else
         return hipErrorInvalidValue;
@@ -628,7 +678,7 @@ int hchk(gpm_mlp *m, hipError_t e, const char *what)
 }
 #define MCHK(m, expr) do { int r_ = hchk((m), (expr), #expr); if (r_) return r_; } while (0)
 
-int mlp_in_kp(int in) { return (in + 15) / 16 * 16; }
+int mlp_in_kp(int in) { return (in + 31) / 32 * 32; }  // x32 fragments tile the input
 
 template <class T>
 hipError_t zalloc(T **p, size_t n)
@@ -691,7 +741,7 @@ int gpm_mlp_create(int32_t in_features, int32_t n_classes, int32_t max_batch, gp
         if (l < 3 && e == hipSuccess) e = zalloc(&m->H[l], B * m->NO[l]);
     }
     if (e == hipSuccess) e = zalloc(&m->logits, B * 16);
-    if (e == hipSuccess) e = zalloc(&m->part, (B / 64) * (size_t)128 * 208);
+    if (e == hipSuccess) e = zalloc(&m->part, (B / 64) * (size_t)128 * 224);
     if (e == hipSuccess) e = zalloc(&m->partb, (B / 64) * (size_t)128);
     if (e == hipSuccess) e = zalloc(&m->lossp, 2 * (B / 256 + 1));
     if (e != hipSuccess) {
@@ -803,29 +853,29 @@ int gpm_mlp_forward_backward(gpm_mlp *m, const float *x, const int32_t *y, int32
     if (!m || !x || !y || batch < 1 || batch > m->max_batch) return -1;
     const int B = batch, nb64 = (B + 63) / 64, nb256 = (B + 255) / 256;
     hipStream_t s = m->stream;
-    if (m->kp == 112) hipLaunchKernelGGL(k_fwd_in<112>, dim3(nb64), dim3(256), 0, s, x, m->in, B, m->W[0], m->pb[0], m->H[0]);
-    else hipLaunchKernelGGL(k_fwd_in<208>, dim3(nb64), dim3(256), 0, s, x, m->in, B, m->W[0], m->pb[0], m->H[0]);
+    if (m->kp == 128) hipLaunchKernelGGL(k_fwd_in<128>, dim3(nb64), dim3(256), 0, s, x, m->in, B, m->W[0], m->pb[0], m->H[0]);
+    else hipLaunchKernelGGL(k_fwd_in<224>, dim3(nb64), dim3(256), 0, s, x, m->in, B, m->W[0], m->pb[0], m->H[0]);
     hipLaunchKernelGGL((k_fwd_hidden<128, 128, true, false>), dim3(nb64), dim3(256), 0, s, m->H[0], B, m->W[1], m->pb[1], (void *)m->H[1]);
     hipLaunchKernelGGL((k_fwd_hidden<128, 64, true, false>), dim3(nb64), dim3(256), 0, s, m->H[1], B, m->W[2], m->pb[2], (void *)m->H[2]);
     hipLaunchKernelGGL((k_fwd_hidden<64, 16, false, true>), dim3(nb64), dim3(256), 0, s, m->H[2], B, m->W[3], m->pb[3], (void *)m->logits);
     hipLaunchKernelGGL(k_softmax_ce, dim3(nb256), dim3(256), 0, s, m->logits, y, B, m->C, m->dH[3], m->lossp);
     // layer 3: dW3 = dlogits^T H2, dH2 = dlogits W3 * (H2 > 0)
     hipLaunchKernelGGL((k_bwd_weight<16, 64, false>), dim3(nb64), dim3(256), 0, s, m->dH[3], (const void *)m->H[2], 64, B, m->part, m->partb);
-    hipLaunchKernelGGL(k_reduce_parts, dim3(64), dim3(256), 0, s, m->part, nb64, 16 * 64, m->g[3]);
-    hipLaunchKernelGGL(k_reduce_parts, dim3(1), dim3(256), 0, s, m->partb, nb64, 16, m->gb[3]);
+    hipLaunchKernelGGL(k_reduce_parts, dim3((16 * 64 + 63) / 64), dim3(256), 0, s, m->part, nb64, 16 * 64, m->g[3]);
+    hipLaunchKernelGGL(k_reduce_parts, dim3((16 + 63) / 64), dim3(256), 0, s, m->partb, nb64, 16, m->gb[3]);
     hipLaunchKernelGGL((k_bwd_data<16, 64>), dim3(nb64), dim3(256), 0, s, m->dH[3], B, m->WT[3], m->H[2], m->dH[2]);
     hipLaunchKernelGGL((k_bwd_weight<64, 128, false>), dim3(nb64), dim3(256), 0, s, m->dH[2], (const void *)m->H[1], 128, B, m->part, m->partb);
-    hipLaunchKernelGGL(k_reduce_parts, dim3(64), dim3(256), 0, s, m->part, nb64, 64 * 128, m->g[2]);
-    hipLaunchKernelGGL(k_reduce_parts, dim3(1), dim3(256), 0, s, m->partb, nb64, 64, m->gb[2]);
+    hipLaunchKernelGGL(k_reduce_parts, dim3((64 * 128 + 63) / 64), dim3(256), 0, s, m->part, nb64, 64 * 128, m->g[2]);
+    hipLaunchKernelGGL(k_reduce_parts, dim3((64 + 63) / 64), dim3(256), 0, s, m->partb, nb64, 64, m->gb[2]);
     hipLaunchKernelGGL((k_bwd_data<64, 128>), dim3(nb64), dim3(256), 0, s, m->dH[2], B, m->WT[2], m->H[1], m->dH[1]);
     hipLaunchKernelGGL((k_bwd_weight<128, 128, false>), dim3(nb64), dim3(256), 0, s, m->dH[1], (const void *)m->H[0], 128, B, m->part, m->partb);
-    hipLaunchKernelGGL(k_reduce_parts, dim3(64), dim3(256), 0, s, m->part, nb64, 128 * 128, m->g[1]);
-    hipLaunchKernelGGL(k_reduce_parts, dim3(1), dim3(256), 0, s, m->partb, nb64, 128, m->gb[1]);
+    hipLaunchKernelGGL(k_reduce_parts, dim3((128 * 128 + 63) / 64), dim3(256), 0, s, m->part, nb64, 128 * 128, m->g[1]);
+    hipLaunchKernelGGL(k_reduce_parts, dim3((128 + 63) / 64), dim3(256), 0, s, m->partb, nb64, 128, m->gb[1]);
     hipLaunchKernelGGL((k_bwd_data<128, 128>), dim3(nb64), dim3(256), 0, s, m->dH[1], B, m->WT[1], m->H[0], m->dH[0]);
-    if (m->kp == 112) hipLaunchKernelGGL((k_bwd_weight<128, 112, true>), dim3(nb64), dim3(256), 0, s, m->dH[0], (const void *)x, m->in, B, m->part, m->partb);
-    else hipLaunchKernelGGL((k_bwd_weight<128, 208, true>), dim3(nb64), dim3(256), 0, s, m->dH[0], (const void *)x, m->in, B, m->part, m->partb);
-    hipLaunchKernelGGL(k_reduce_parts, dim3(64), dim3(256), 0, s, m->part, nb64, 128 * m->kp, m->g[0]);
-    hipLaunchKernelGGL(k_reduce_parts, dim3(1), dim3(256), 0, s, m->partb, nb64, 128, m->gb[0]);
+    if (m->kp == 128) hipLaunchKernelGGL((k_bwd_weight<128, 128, true>), dim3(nb64), dim3(256), 0, s, m->dH[0], (const void *)x, m->in, B, m->part, m->partb);
+    else hipLaunchKernelGGL((k_bwd_weight<128, 224, true>), dim3(nb64), dim3(256), 0, s, m->dH[0], (const void *)x, m->in, B, m->part, m->partb);
+    hipLaunchKernelGGL(k_reduce_parts, dim3((128 * m->kp + 63) / 64), dim3(256), 0, s, m->part, nb64, 128 * m->kp, m->g[0]);
+    hipLaunchKernelGGL(k_reduce_parts, dim3((128 + 63) / 64), dim3(256), 0, s, m->partb, nb64, 128, m->gb[0]);
     MCHK(m, hipGetLastError());
     if (loss || accuracy) {
         std::vector<float> lp(2 * nb256);
@@ -845,15 +895,23 @@ int gpm_mlp_adamw_step(gpm_mlp *m, double lr, double beta1, double beta2, double
     m->step += 1;
     const float bc1 = (float)(1.0 - std::pow(beta1, (double)m->step));
     const float bc2s = (float)std::sqrt(1.0 - std::pow(beta2, (double)m->step));
+    AdamSet a{};
+    a.off[0] = 0;
     for (int l = 0; l < 4; ++l) {
-        const int nw = m->NO[l] * m->KPL[l];
-        hipLaunchKernelGGL(k_adamw, dim3(64), dim3(256), 0, m->stream, m->p[l], m->g[l], m->m[l], m->v[l], nw, (float)lr,
-                           (float)beta1, (float)beta2, (float)eps, (float)weight_decay, bc1, bc2s);
-        hipLaunchKernelGGL(k_adamw, dim3(1), dim3(256), 0, m->stream, m->pb[l], m->gb[l], m->mb[l], m->vb[l], m->NO[l],
-                           (float)lr, (float)beta1, (float)beta2, (float)eps, (float)weight_decay, bc1, bc2s);
+        const int t = 2 * l, tb = 2 * l + 1;
+        a.p[t] = m->p[l], a.g[t] = m->g[l], a.m[t] = m->m[l], a.v[t] = m->v[l];
+        a.W[t] = m->W[l], a.WT[t] = m->WT[l], a.kp[t] = m->KPL[l], a.no[t] = m->NO[l];
+        a.off[t + 1] = a.off[t] + m->NO[l] * m->KPL[l];
+        a.p[tb] = m->pb[l], a.g[tb] = m->gb[l], a.m[tb] = m->mb[l], a.v[tb] = m->vb[l];
+        a.W[tb] = nullptr, a.WT[tb] = nullptr, a.kp[tb] = 1, a.no[tb] = m->NO[l];
+        a.off[tb + 1] = a.off[tb] + m->NO[l];
     }
+    int most = 0;
+    for (int t = 0; t < 8; ++t) most = std::max(most, a.off[t + 1] - a.off[t]);
+    hipLaunchKernelGGL(k_adamw, dim3((most + 255) / 256, 8), dim3(256), 0, m->stream, a, (float)lr, (float)beta1,
+                       (float)beta2, (float)eps, (float)weight_decay, bc1, bc2s);
     MCHK(m, hipGetLastError());
-    return refresh_bf16(m);
+    return 0;
 }
 
 int gpm_mlp_train_step(gpm_mlp *m, const float *x, const int32_t *y, int32_t batch, double lr, double weight_decay,
@@ -873,8 +931,8 @@ int gpm_mlp_logits(gpm_mlp *m, const float *x, int32_t n, float *logits)
         const int nb64 = (B + 63) / 64;
         hipStream_t s = m->stream;
         const float *xb = x + (size_t)off * m->in;
-        if (m->kp == 112) hipLaunchKernelGGL(k_fwd_in<112>, dim3(nb64), dim3(256), 0, s, xb, m->in, B, m->W[0], m->pb[0], m->H[0]);
-        else hipLaunchKernelGGL(k_fwd_in<208>, dim3(nb64), dim3(256), 0, s, xb, m->in, B, m->W[0], m->pb[0], m->H[0]);
+        if (m->kp == 128) hipLaunchKernelGGL(k_fwd_in<128>, dim3(nb64), dim3(256), 0, s, xb, m->in, B, m->W[0], m->pb[0], m->H[0]);
+        else hipLaunchKernelGGL(k_fwd_in<224>, dim3(nb64), dim3(256), 0, s, xb, m->in, B, m->W[0], m->pb[0], m->H[0]);
         hipLaunchKernelGGL((k_fwd_hidden<128, 128, true, false>), dim3(nb64), dim3(256), 0, s, m->H[0], B, m->W[1], m->pb[1], (void *)m->H[1]);
         hipLaunchKernelGGL((k_fwd_hidden<128, 64, true, false>), dim3(nb64), dim3(256), 0, s, m->H[1], B, m->W[2], m->pb[2], (void *)m->H[2]);
         hipLaunchKernelGGL((k_fwd_hidden<64, 16, false, true>), dim3(nb64), dim3(256), 0, s, m->H[2], B, m->W[3], m->pb[3], (void *)m->logits);

@@ -100,7 +100,14 @@ class MLP:
         self._check(self.lib.gpm_mlp_adamw_step(self._h, lr, betas[0], betas[1], eps, weight_decay),
                     "gpm_mlp_adamw_step")
 
-    def train_step(self, x, y, lr=1e-3, weight_decay=1e-2):
+    def train_step(self, x, y, lr=1e-3, weight_decay=1e-2, report=True):
+        """One AdamW step on a batch.  With report=False the loss and accuracy
+        are not read back, so the call only enqueues work (returns None, None)."""
+        if not report:
+            self._check(self.lib.gpm_mlp_train_step(self._h, ctypes.c_void_p(x.data_ptr()),
+                                                    ctypes.c_void_p(y.data_ptr()), int(x.shape[0]), lr,
+                                                    weight_decay, None, None), "gpm_mlp_train_step")
+            return None, None
         loss, acc = ctypes.c_double(), ctypes.c_double()
         self._check(self.lib.gpm_mlp_train_step(self._h, ctypes.c_void_p(x.data_ptr()), ctypes.c_void_p(y.data_ptr()),
                                                 int(x.shape[0]), lr, weight_decay, ctypes.byref(loss),

@@ -277,7 +277,9 @@ int64_t gpm_mlp_num_params(const gpm_mlp *m);
 int gpm_mlp_set_params(gpm_mlp *m, const float *params);
 int gpm_mlp_get_params(gpm_mlp *m, float *params);
 int gpm_mlp_get_grads(gpm_mlp *m, float *grads);
-/* forward + mean cross-entropy + backward; gradients stay on the device */
+/* forward + mean cross-entropy + backward; gradients stay on the device.
+ * loss / accuracy may be NULL: then nothing is read back and the call only
+ * enqueues work on the network's stream (no synchronisation). */
 int gpm_mlp_forward_backward(gpm_mlp *m, const float *x, const int32_t *y, int32_t batch, double *loss,
                              double *accuracy);
 /* AdamW on the FP32 master weights (torch.optim.AdamW semantics) */

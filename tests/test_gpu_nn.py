@@ -1,5 +1,6 @@
 """The learned predictor on the GPU: matrix-core forward/backward/AdamW
-against the CPU restatement (oracle/mlp_oracle.py) and PyTorch autograd, the
+against the CPU restatement (oracle/mlp_oracle.py) and PyTorch FP64 autograd
+with the same BF16 rounding points (tests/bf16_torch.py), the
 a priori order path of the step against the oracle, and the dump."""
 import os
 import sys
@@ -44,41 +45,34 @@ def test_logits_vs_oracle(gpm, in_f, C):
     assert np.mean(np.argmax(z_gpu, 1) == np.argmax(z_ref, 1)) > 0.99
 
 
-@pytest.mark.parametrize("in_f,C,B", [(100, 3, 8192), (196, 4, 1000)])
-def test_gradients_vs_oracle_and_autograd(gpm, in_f, C, B):
+@pytest.mark.parametrize("in_f,C,B", [(100, 3, 8192), (196, 4, 1000), (100, 3, 512)])
+def test_gradients_per_tensor(gpm, in_f, C, B):
+    """Loss and every weight and bias gradient of the matrix-core backward,
+    per tensor, against the oracle and against PyTorch FP64 autograd rounding
+    to BF16 at the same points.  The matrix cores accumulate in FP32 in their
+    own order, so a BF16-rounded activation or gradient can land one ulp
+    (2^-8 relative) away; the bounds leave room for a few such flips."""
     import torch
+    import bf16_torch as T
     M = mlp_oracle()
     rng = np.random.default_rng(2)
     m = gpm.MLP(in_f, C, max_batch=B, seed=5)
     x = rng.uniform(-1, 1, size=(B, in_f)).astype(np.float32)
     y = rng.integers(0, C, size=B).astype(np.int32)
     loss, acc = m.forward_backward(torch.from_numpy(x).cuda(), torch.from_numpy(y).cuda())
-    g = m.get_grads()
+    g = T.split_flat(m.get_grads(), in_f, C)
     p = m.get_params()
     loss_r, acc_r, g_r = M.forward_backward(p, in_f, C, x, y)
-    assert abs(loss - loss_r) < 1e-4 * max(1.0, abs(loss_r))
-    off = 0
-    for shape in gpm.nn.param_shapes(in_f, C):
-        n = int(np.prod(shape))
-        assert rel(g[off:off + n], g_r[off:off + n]) < 2e-2, shape
-        off += n
-    # against PyTorch FP32 autograd (no BF16 rounding): same direction
-    layers, off = [], 0
-    dims = (in_f, 128, 128, 64, C)
-    for a, b in zip(dims[:-1], dims[1:]):
-        lin = torch.nn.Linear(a, b)
-        with torch.no_grad():
-            lin.weight.copy_(torch.from_numpy(p[off:off + a * b].reshape(b, a)))
-            off += a * b
-            lin.bias.copy_(torch.from_numpy(p[off:off + b]))
-            off += b
-        layers += [lin, torch.nn.ReLU()]
-    net = torch.nn.Sequential(*layers[:-1])
-    lt = torch.nn.functional.cross_entropy(net(torch.from_numpy(x)), torch.from_numpy(y).long())
-    lt.backward()
-    gt = np.concatenate([t.grad.numpy().ravel() for t in net.parameters()])
-    cos = float(np.dot(g, gt) / (np.linalg.norm(g) * np.linalg.norm(gt)))
-    assert cos > 0.995 and abs(loss - lt.item()) < 2e-2
+    _, loss_t, g_t = T.forward_backward(p, in_f, C, x, y)
+    assert abs(loss - loss_r) < 1e-5 * abs(loss_r) and abs(loss - loss_t) < 1e-5 * abs(loss_t)
+    errs = {}
+    for li, ((gw, gb), (ow, ob), (tw, tb)) in enumerate(zip(g, T.split_flat(g_r, in_f, C), g_t)):
+        errs[f"W{li}"] = (rel(gw, ow), rel(gw, tw))
+        errs[f"b{li}"] = (rel(gb, ob), rel(gb, tb))
+    print(f"in={in_f} C={C} B={B} per-tensor (vs oracle, vs torch):",
+          {k: (f"{a:.1e}", f"{b:.1e}") for k, (a, b) in errs.items()})
+    bad = {k: v for k, v in errs.items() if not max(v) < 2e-3}
+    assert not bad, errs
 
 
 def test_adamw_vs_torch(gpm):
