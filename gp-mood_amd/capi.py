@@ -46,7 +46,7 @@ EXPORTS = [
     "gpm_mlp_train_step", "gpm_mlp_logits", "gpm_set_predictor", "gpm_set_dump", "gpm_features",
     "gpm_set_apriori_orders", "gpm_get_apriori_orders",
     "gpm_set_reference_nn", "gpm_load_reference_nn", "gpm_set_niter", "gpm_get_cellgpo_priori", "gpm_nn_expf_bits",
-    "gpm_read_reference_nn", "gpm_nn_expf_bits_host", "gpm_ddiv", "gpm_set_reference_dump", "gpm_set_reference_dataset",
+    "gpm_read_reference_nn", "gpm_nn_expf_bits_host", "gpm_ddiv", "gpm_qdiv", "gpm_set_reference_dump", "gpm_set_reference_dataset",
     "gpm_append_reference_dataset",
     "gpm_record_masks", "gpm_recorded_steps", "gpm_frozen_run", "gpm_adjoint", "gpm_hllc_vjp",
 ]
@@ -144,6 +144,7 @@ def load():
         "gpm_read_reference_nn": ([ctypes.c_char_p, I, I, P, P, P, P, P, P], I),
         "gpm_nn_expf_bits_host": ([ctypes.c_uint32, ctypes.c_int64, P], I),
         "gpm_ddiv": ([ctypes.c_int64, P, P, P], I),
+        "gpm_qdiv": ([ctypes.c_int64, P, P, P], I),
         "gpm_set_reference_dump": ([P, P, P, P, ctypes.c_int64, P, P, ctypes.c_uint64, I], I),
         "gpm_set_reference_dataset": ([P, I, I, I], I),
         "gpm_append_reference_dataset": ([P, P, P, I, P, P, P, P], I),

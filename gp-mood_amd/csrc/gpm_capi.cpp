@@ -1604,7 +1604,14 @@ int gpm_run_noalloc(const gpm_config *cfg, const double *zT_sp, const double *gp
 int gpm_ddiv(int64_t n, const double *a_dev, const double *b_dev, double *q_dev)
 {
     if (n < 0 || (n > 0 && (!a_dev || !b_dev || !q_dev))) return GPM_EINVAL;
-    if (launch_ddiv(a_dev, b_dev, q_dev, n, nullptr) != hipSuccess) return GPM_EHIP;
+    if (launch_ddiv(a_dev, b_dev, q_dev, n, false, nullptr) != hipSuccess) return GPM_EHIP;
+    return hipDeviceSynchronize() == hipSuccess ? GPM_OK : GPM_EHIP;
+}
+
+int gpm_qdiv(int64_t n, const double *a_dev, const double *b_dev, double *q_dev)
+{
+    if (n < 0 || (n > 0 && (!a_dev || !b_dev || !q_dev))) return GPM_EINVAL;
+    if (launch_ddiv(a_dev, b_dev, q_dev, n, true, nullptr) != hipSuccess) return GPM_EHIP;
     return hipDeviceSynchronize() == hipSuccess ? GPM_OK : GPM_EHIP;
 }
 

@@ -91,6 +91,70 @@ __device__ __forceinline__ double ddiv(double a, double b)
     return q;
 }
 
+// A divisor shared by several quotients (the density of one Riemann state
+// divides its momenta, kinetic energy, pressure and total energy).  When
+// div_scale leaves both operands alone (no scaling, flag clear), ddiv's
+// reciprocal chain -- v_rcp and the three Newton steps -- depends on the
+// divisor only, so it is computed once here and qdiv replays the rest of
+// ddiv's sequence (m = n y, r = n - d m, fmas, fixup) with the same operands:
+// the same instructions on the same values, hence the same bits.
+//
+// div_scale scales only near the ends of the range: an exponent difference
+// of 768 or more, a denormal divisor, reciprocal or quotient, a numerator
+// exponent of 53 or less (biased).  With the divisor in [2^-300, 2^300) and
+// the numerator's frexp exponent in [-600, 400] (|a| in [2^-601, 2^400)) none
+// of these can happen.  Numerators that are zero, infinite or NaN (frexp
+// exponent 0) take the shared form too: div_fixup returns their quotient
+// (signed zero, signed infinity, quiet NaN) whatever the approximation.
+// Every other numerator, and a divisor outside the range or with an all-ones
+// significand (ddiv's exact remainder comparison), goes through ddiv itself.
+struct Recip {
+    double b, y;
+    bool fast;
+};
+
+// biased frexp exponent of a numerator: at most 1000 when a may share
+__device__ __forceinline__ unsigned qdiv_span(double a)
+{
+    return (unsigned)(__builtin_amdgcn_frexp_exp(a) + 600);
+}
+
+__host__ __device__ __forceinline__ Recip recip(double b)
+{
+#ifdef __HIP_DEVICE_COMPILE__
+    double y = __builtin_amdgcn_rcp(b);
+#pragma unroll
+    for (int it = 0; it < 3; ++it) y = __builtin_fma(y, __builtin_fma(-b, y, 1.0), y);
+    const unsigned long long bb = (unsigned long long)__double_as_longlong(b);
+    const unsigned e = (unsigned)(bb >> 52) & 0x7FFu;  // biased exponent: normal b in [2^-300, 2^300)
+    return Recip{b, y, e - 723u < 600u && (bb & 0xFFFFFFFFFFFFFull) != 0xFFFFFFFFFFFFFull};
+#else
+    return Recip{b, 0.0, false};
+#endif
+}
+
+// Whether a / R.b may take the shared form.
+__device__ __forceinline__ bool qdiv_ok(double a, const Recip &R) { return R.fast && qdiv_span(a) <= 1000u; }
+
+// a / R.b in the shared form, for a with qdiv_ok(a, R): ddiv's sequence after
+// the reciprocal, unscaled.
+__device__ __forceinline__ double qdiv_shared(double a, const Recip &R)
+{
+    const double m = a * R.y;
+    const double r = __builtin_fma(-R.b, m, a);
+    return __builtin_amdgcn_div_fixup(__builtin_amdgcn_div_fmas(r, R.y, m, false), R.b, a);
+}
+
+// a / R.b correctly rounded, bit-identical to ddiv(a, R.b).
+__host__ __device__ __forceinline__ double qdiv(double a, const Recip &R)
+{
+#ifdef __HIP_DEVICE_COMPILE__
+    return qdiv_ok(a, R) ? qdiv_shared(a, R) : ddiv(a, R.b);
+#else
+    return a / R.b;
+#endif
+}
+
 // The reference's FP64 a / b in code compiled for both sides: ddiv on the
 // device, the host's IEEE division on the host.
 __host__ __device__ __forceinline__ double dv(double a, double b)
@@ -299,6 +363,36 @@ __device__ __forceinline__ double4 load_bnd(const double4 *__restrict__ U, const
     return bnd_value(B, DevSrc{U, B.NX}, i, j, xmin_top);
 }
 
+// c2p_local (physics.f90:51-75) of one Riemann state: velocities, pressure
+// (physics.f90:132-143) and sound speed, the four divisions by the density
+// sharing its reciprocal.
+struct Prim {
+    double vx, vy, p, c;
+    Recip r;
+};
+
+__device__ __forceinline__ Prim c2p_shared(const double u[4])
+{
+    Prim s;
+    s.r = recip(u[0]);
+    const double ke = 0.5 * (u[1] * u[1] + u[2] * u[2]);
+    // the shared form unconditionally, then one check for all four quotients
+    // and one (rarely taken) branch to the full divisions
+    s.vx = qdiv_shared(u[1], s.r);
+    s.vy = qdiv_shared(u[2], s.r);
+    s.p = (kGamma - 1.0) * (u[3] - qdiv_shared(ke, s.r));
+    double cq = qdiv_shared(kGamma * s.p, s.r);
+    const unsigned span = max(max(qdiv_span(u[1]), qdiv_span(u[2])), max(qdiv_span(ke), qdiv_span(kGamma * s.p)));
+    if (!s.r.fast || span > 1000u) {
+        s.vx = ddiv(u[1], u[0]);
+        s.vy = ddiv(u[2], u[0]);
+        s.p = (kGamma - 1.0) * (u[3] - ddiv(ke, u[0]));
+        cq = ddiv(kGamma * s.p, u[0]);
+    }
+    s.c = sqrt(cq);
+    return s;
+}
+
 // HLLC flux (mod_HLLC.f90:11-169) with c2p_local and Flux (physics.f90:9-75).
 // DIR 0: x-face (normal momentum mx), DIR 1: y-face.
 // hllc_adm returns false, leaving F untouched, when either state is
@@ -306,12 +400,9 @@ __device__ __forceinline__ double4 load_bnd(const double4 *__restrict__ U, const
 template <int DIR>
 __device__ __forceinline__ bool hllc_adm(const double ul[4], const double ur[4], double F[4])
 {
-    const double vxl = dv(ul[1], ul[0]), vyl = dv(ul[2], ul[0]);
-    const double pl = pressure(ul[0], ul[1], ul[2], ul[3]);
-    const double cl = sqrt(dv(kGamma * pl, ul[0]));
-    const double vxr = dv(ur[1], ur[0]), vyr = dv(ur[2], ur[0]);
-    const double pr = pressure(ur[0], ur[1], ur[2], ur[3]);
-    const double cr = sqrt(dv(kGamma * pr, ur[0]));
+    const Prim L = c2p_shared(ul), R = c2p_shared(ur);
+    const double vxl = L.vx, vyl = L.vy, pl = L.p, cl = L.c;
+    const double vxr = R.vx, vyr = R.vy, pr = R.p, cr = R.c;
     const bool adm = !(ul[0] <= 0. || pl <= 0. || isnan(ul[0]) || isnan(pl) ||
                        ur[0] <= 0. || pr <= 0. || isnan(ur[0]) || isnan(pr));
     if (!adm) return false;
@@ -338,14 +429,14 @@ __device__ __forceinline__ bool hllc_adm(const double ul[4], const double ur[4],
         const double dStarL = dv(ul[0] * numerL, SL - uStar);
         double Us[4];
         Us[0] = 1.0; Us[md] = uStar; Us[om] = vtl;
-        Us[3] = dv(ul[3], ul[0]) + (uStar - vnl) * (uStar + dv(pl, ul[0] * numerL));
+        Us[3] = qdiv(ul[3], L.r) + (uStar - vnl) * (uStar + dv(pl, ul[0] * numerL));
 #pragma unroll
         for (int k = 0; k < 4; ++k) F[k] = Fl[k] + SL * (Us[k] * dStarL - ul[k]);
     } else if (uStar < 0. && SR >= 0.) {
         const double dStarR = dv(ur[0] * numerR, SR - uStar);
         double Us[4];
         Us[0] = 1.0; Us[md] = uStar; Us[om] = vtr;
-        Us[3] = dv(ur[3], ur[0]) + (uStar - vnr) * (uStar + dv(pr, ur[0] * numerR));
+        Us[3] = qdiv(ur[3], R.r) + (uStar - vnr) * (uStar + dv(pr, ur[0] * numerR));
 #pragma unroll
         for (int k = 0; k < 4; ++k) F[k] = Fr[k] + SR * (Us[k] * dStarR - ur[k]);
     } else {
@@ -375,12 +466,9 @@ __device__ __forceinline__ void hllc(const double ul[4], const double ur[4], dou
 template <int DIR>
 __device__ __forceinline__ bool hll_adm(const double ul[4], const double ur[4], double F[4])
 {
-    const double vxl = dv(ul[1], ul[0]), vyl = dv(ul[2], ul[0]);
-    const double pl = pressure(ul[0], ul[1], ul[2], ul[3]);
-    const double cl = sqrt(dv(kGamma * pl, ul[0]));
-    const double vxr = dv(ur[1], ur[0]), vyr = dv(ur[2], ur[0]);
-    const double pr = pressure(ur[0], ur[1], ur[2], ur[3]);
-    const double cr = sqrt(dv(kGamma * pr, ur[0]));
+    const Prim L = c2p_shared(ul), R = c2p_shared(ur);
+    const double vxl = L.vx, vyl = L.vy, pl = L.p, cl = L.c;
+    const double vxr = R.vx, vyr = R.vy, pr = R.p, cr = R.c;
     const bool adm = !(ul[0] <= 0. || pl <= 0. || isnan(ul[0]) || isnan(pl) ||
                        ur[0] <= 0. || pr <= 0. || isnan(ur[0]) || isnan(pr));
     if (!adm) return false;
@@ -413,12 +501,9 @@ __device__ __forceinline__ bool hll_adm(const double ul[4], const double ur[4], 
 template <int DIR>
 __device__ __forceinline__ bool llf_adm(const double ul[4], const double ur[4], double F[4])
 {
-    const double vxl = dv(ul[1], ul[0]), vyl = dv(ul[2], ul[0]);
-    const double pl = pressure(ul[0], ul[1], ul[2], ul[3]);
-    const double cl = sqrt(dv(kGamma * pl, ul[0]));
-    const double vxr = dv(ur[1], ur[0]), vyr = dv(ur[2], ur[0]);
-    const double pr = pressure(ur[0], ur[1], ur[2], ur[3]);
-    const double cr = sqrt(dv(kGamma * pr, ur[0]));
+    const Prim L = c2p_shared(ul), R = c2p_shared(ur);
+    const double vxl = L.vx, vyl = L.vy, pl = L.p, cl = L.c;
+    const double vxr = R.vx, vyr = R.vy, pr = R.p, cr = R.c;
     const bool adm = !(ul[0] <= 0. || pl <= 0. || isnan(ul[0]) || isnan(pl) ||
                        ur[0] <= 0. || pr <= 0. || isnan(ur[0]) || isnan(pr));
     if (!adm) return false;
@@ -443,12 +528,9 @@ __device__ __forceinline__ bool llf_adm(const double ul[4], const double ur[4], 
 template <int DIR>
 __device__ __forceinline__ bool unsplit_adm(const double ul[4], const double ur[4], double F[4])
 {
-    const double vxl = dv(ul[1], ul[0]), vyl = dv(ul[2], ul[0]);
-    const double pl = pressure(ul[0], ul[1], ul[2], ul[3]);
-    const double cl = sqrt(dv(kGamma * pl, ul[0]));
-    const double vxr = dv(ur[1], ur[0]), vyr = dv(ur[2], ur[0]);
-    const double pr = pressure(ur[0], ur[1], ur[2], ur[3]);
-    const double cr = sqrt(dv(kGamma * pr, ur[0]));
+    const Prim L = c2p_shared(ul), R = c2p_shared(ur);
+    const double vxl = L.vx, vyl = L.vy, pl = L.p, cl = L.c;
+    const double vxr = R.vx, vyr = R.vy, pr = R.p, cr = R.c;
     const bool adm = !(ul[0] <= 0. || pl <= 0. || isnan(ul[0]) || isnan(pl) ||
                        ur[0] <= 0. || pr <= 0. || isnan(ur[0]) || isnan(pr));
     if (!adm) return false;

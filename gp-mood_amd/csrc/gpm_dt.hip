@@ -112,17 +112,19 @@ __global__ __launch_bounds__(kDtThreads) void k_dt(DtArgs a)
 }
 
 
-// The step's FP64 division (ddiv) on n operand pairs, for tests against the
-// host's IEEE division.
-__global__ void k_ddiv(const double *__restrict__ a, const double *__restrict__ b, double *__restrict__ q, long long n)
+// The step's FP64 division on n operand pairs, for tests against the host's
+// IEEE division: ddiv itself (shared = false) or the shared-reciprocal form
+// qdiv(a, recip(b)) the Riemann solvers use.
+__global__ void k_ddiv(const double *__restrict__ a, const double *__restrict__ b, double *__restrict__ q, long long n,
+                       bool shared)
 {
     for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
-        q[i] = ddiv(a[i], b[i]);
+        q[i] = shared ? qdiv(a[i], recip(b[i])) : ddiv(a[i], b[i]);
 }
 
-hipError_t launch_ddiv(const double *a, const double *b, double *q, long long n, hipStream_t s)
+hipError_t launch_ddiv(const double *a, const double *b, double *q, long long n, bool shared, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_ddiv, dim3(1024), dim3(256), 0, s, a, b, q, n);
+    hipLaunchKernelGGL(k_ddiv, dim3(1024), dim3(256), 0, s, a, b, q, n, shared);
     return hipGetLastError();
 }
 

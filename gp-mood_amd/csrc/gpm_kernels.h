@@ -261,7 +261,7 @@ hipError_t launch_ring_face(int mord, int ngp, int flux, const RingArgs &a, hipS
 template <int MORD, int NGP>
 hipError_t launch_ring_face_mg(int flux, const RingArgs &a, hipStream_t s);
 hipError_t launch_unpack_orders(const order_t *g, int8_t *out, size_t n, hipStream_t s);
-hipError_t launch_ddiv(const double *a, const double *b, double *q, long long n, hipStream_t s);
+hipError_t launch_ddiv(const double *a, const double *b, double *q, long long n, bool shared, hipStream_t s);
 hipError_t launch_run_reset(int *mark, order_t *gpo, size_t n, order_t g0, int *counters, int ncounters,
                             DevStats *stats, double *tstate, double t, unsigned *done, int *stamp, hipStream_t s);
 

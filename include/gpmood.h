@@ -391,6 +391,11 @@ int gpm_nn_expf_bits_host(uint32_t first, int64_t n, float *y);
  * build's IEEE division) on n pairs in device memory, for tests.  No
  * counterpart in the reference.  Synchronous. */
 int gpm_ddiv(int64_t n, const double *a_dev, const double *b_dev, double *q_dev);
+/* The same quotients through the form the Riemann solvers use for the
+ * divisions by one state's density: the divisor's reciprocal computed once
+ * and shared (range-guarded, falling back to the full division).  No
+ * counterpart in the reference.  Synchronous. */
+int gpm_qdiv(int64_t n, const double *a_dev, const double *b_dev, double *q_dev);
 
 /* ------------------------------------------------------------------------
  * Differentiable form.  No counterpart in the reference: the adjoint of K

@@ -38,6 +38,20 @@ def _cases():
     bits = rng.integers(0, 2 ** 63 - 1, size=(2, 1 << 20), dtype=np.int64) * rng.choice([1, -1], (2, 1 << 20))
     A.append(bits[0].view(np.float64))
     B.append(bits[1].view(np.float64))
+    # operands at and across the edges of the shared-reciprocal form's guard
+    # (divisor in [2^-300, 2^300), numerator in [2^-601, 2^400) or zero,
+    # infinite or NaN), zeros of both signs among them
+    def edges(lo, hi):
+        e = np.array([lo / 2, lo * (1 - 2.0 ** -53), lo, lo * 2, hi / 2, hi * (1 - 2.0 ** -53), hi, hi * 2])
+        return np.concatenate([np.outer(e, mant).ravel(), np.outer(e, 1.0 / mant).ravel()])
+    mant = 1.0 + rng.random(32)
+    na = np.concatenate([edges(2.0 ** -601, 2.0 ** 400), edges(2.0 ** -300, 2.0 ** 300), mant,
+                         [0.0, np.inf, np.nan, 5e-324, 2.2250738585072014e-308]])
+    nb = np.concatenate([edges(2.0 ** -300, 2.0 ** 300), mant, [(2.0 - 2.0 ** -52) * 0.5]])
+    for sa in (1.0, -1.0):
+        for sb in (1.0, -1.0):
+            A.append(np.repeat(sa * na, nb.size))
+            B.append(np.tile(sb * nb, na.size))
     # special values against each other
     sp = np.array([0.0, -0.0, 5e-324, -5e-324, 2.2250738585072009e-308, 2.2250738585072014e-308,
                    1.7976931348623157e308, -1.7976931348623157e308, np.inf, -np.inf, np.nan, 1.0, -3.0, 0.1])
@@ -46,18 +60,22 @@ def _cases():
     return np.concatenate(A), np.concatenate(B)
 
 
-def test_ddiv_equals_host_division(gpm):
+@pytest.mark.parametrize("form", ["gpm_ddiv", "gpm_qdiv"])
+def test_ddiv_equals_host_division(gpm, form):
+    """Both the division and its shared-reciprocal form (the one the Riemann
+    solvers use for the divisions by a state's density) give the host's
+    quotient bit for bit, on both sides of the shared form's range guard."""
     if not has_gpu():
         pytest.skip("no GPU")
     import torch
     a, b = _cases()
-    print(f"[case ddiv vs host a/b, {a.size} pairs]", flush=True)
+    print(f"[case {form} vs host a/b, {a.size} pairs]", flush=True)
     with np.errstate(all="ignore"):
         ref = a / b
     da, db = torch.from_numpy(a).cuda(), torch.from_numpy(b).cuda()
     dq = torch.empty_like(da)
     torch.cuda.synchronize()
-    assert gpm.capi.load().gpm_ddiv(a.size, da.data_ptr(), db.data_ptr(), dq.data_ptr()) == 0
+    assert getattr(gpm.capi.load(), form)(a.size, da.data_ptr(), db.data_ptr(), dq.data_ptr()) == 0
     q = dq.cpu().numpy()
     nan = np.isnan(ref)
     assert np.array_equal(np.isnan(q), nan)
