@@ -55,6 +55,8 @@ struct gpm_ctx {
     int dt_log_cap = 0;
     int tiles_x = 0, ntiles = 0, dt_blocks = 0, mood_blocks = 0, mood_solo = 0;
     bool mood_coop = true;  // MOOD passes as one cooperative launch per stage (else per-pass launches)
+    bool mood_fc = true;    // workgroup 0's MOOD passes reuse face fluxes and compute likely next ones ahead
+    FluxTable gfc{};        // device-wide face-flux cache that the other workgroups fill meanwhile
     bool has_state = false;
     bool timing = false;
     double stage_ms = 0, mood_ms = 0;
@@ -519,7 +521,8 @@ int enqueue_stages(gpm_ctx *c)
         MoodArgs m{};
         m.Uin = in; m.Uout = plan[k].out; m.C = plan[k].C; m.B = c->B; m.gpo = c->gpo; m.dflag = c->dflag;
         m.slist[0] = c->slistA; m.slist[1] = c->slistB;
-        m.cnt = cnt; m.mark = c->mark; m.stamp = c->stamp; m.solo_max = c->mood_solo;
+        m.cnt = cnt; m.mark = c->mark; m.stamp = c->stamp; m.solo_max = c->mood_solo; m.fcache = c->mood_fc;
+        m.gfc = c->gfc;
         m.stats = c->stats; m.T = c->dT; m.tstate = c->tstate;
         m.dmp = c->cfg.dmp; m.u2 = c->cfg.u2; m.u2tol = c->cfg.u2_tol;
         m.cascade = c->cfg.cascade; m.stage_idx = k;
@@ -719,6 +722,29 @@ int gpm_create(const gpm_config *cfg, gpm_ctx **out)
         ok = dalloc(&c->ring, (size_t)ring_size(c->NX, c->NY)) == hipSuccess && dalloc(&c->band, n) == hipSuccess;
     if (ok && c->ring_mode && c->integrator == GPM_SSP_RK4) ok = dalloc(&c->band3, n) == hipSuccess;
     if (ok) ok = dalloc(&c->st_dev, kBndStates) == hipSuccess && dalloc(&c->ext, kMaxStages) == hipSuccess;
+    // The device-wide face-flux cache of the MOOD passes: a quarter slot per
+    // cell (powers of two from 2^12 to 2^24 slots, 44 B each; 46 MB at 2048^2),
+    // and a 4-byte claim word per cell.
+    // GPM_MOOD_HELP=0 leaves it out (same results).
+    bool help = !c->nodetect && c->mord > 1;
+    if (const char *env = std::getenv("GPM_MOOD_HELP")) help = help && std::atoi(env) != 0;
+    if (ok && help) {
+        int bits = 12;
+        while (bits < 24 && ((size_t)1 << bits) < n / 4) ++bits;
+        const size_t slots = (size_t)1 << bits;
+        ok = dalloc(&c->gfc.kw, slots) == hipSuccess && dalloc(&c->gfc.ready, slots) == hipSuccess &&
+             dalloc(&c->gfc.data, slots) == hipSuccess && dalloc(&c->gfc.beacon, 2 + kSoloMax) == hipSuccess &&
+             dalloc(&c->gfc.done, n) == hipSuccess;
+        if (ok) {
+            c->gfc.bits = bits;
+            c->gfc.radius = 3;
+            if (const char *env = std::getenv("GPM_MOOD_HELP_R")) c->gfc.radius = std::atoi(env);
+            ok = hipMemsetAsync(c->gfc.kw, 0, slots * sizeof(*c->gfc.kw), c->stream) == hipSuccess &&
+                 hipMemsetAsync(c->gfc.ready, 0, slots * sizeof(*c->gfc.ready), c->stream) == hipSuccess &&
+                 hipMemsetAsync(c->gfc.beacon, 0, (2 + kSoloMax) * sizeof(int), c->stream) == hipSuccess &&
+                 hipMemsetAsync(c->gfc.done, 0, n * sizeof(unsigned), c->stream) == hipSuccess;
+        }
+    }
     if (!ok) {
         gpm_destroy(c);
         std::fprintf(stderr, "gpm_create: device allocation failed\n");
@@ -757,6 +783,8 @@ int gpm_create(const gpm_config *cfg, gpm_ctx **out)
     c->mood_solo = 16;
     if (const char *env = std::getenv("GPM_MOOD_SOLO")) c->mood_solo = std::atoi(env) > 0 ? std::atoi(env) : 0;
     if (c->mood_solo > kSoloMax) c->mood_solo = kSoloMax;
+    // GPM_MOOD_FC=0 switches the face-flux cache of workgroup 0's passes off (same results).
+    if (const char *env = std::getenv("GPM_MOOD_FC")) c->mood_fc = std::atoi(env) != 0;
     hipMemsetAsync(c->mark, 0, n * sizeof(int), c->stream);
     hipMemsetD32Async(c->gpo, pack_order(c->mord, c->mord, 0), n, c->stream);
     hipMemsetAsync(c->counters, 0, kMaxStages * kCountersPerStage * sizeof(int), c->stream);
@@ -806,7 +834,8 @@ int gpm_destroy(gpm_ctx *c)
                     c->gpo, c->dflag, c->slistA, c->slistB, c->mark, c->counters,
                     c->stamp, c->stats, c->tstate, c->block_min, c->dt_log, c->done, c->pred, c->nnlist,
                     c->nncount, c->adj_ub, c->adj_G, c->adj_A, c->adj_tmp, c->adj_ordx, c->adj_ordy,
-                    c->priori, c->refnn_w, c->rowv, c->vnn, c->xfer, c->xfer_gpo, c->rdump.nr_start};
+                    c->priori, c->refnn_w, c->rowv, c->vnn, c->xfer, c->xfer_gpo, c->rdump.nr_start,
+                    c->gfc.kw, c->gfc.ready, c->gfc.data, c->gfc.beacon, c->gfc.done};
     for (void *b : bufs)
         if (b) hipFree(b);
     void *hbufs[] = {c->hds.d_in, c->hds.d_lab, c->hds.d_cells, c->hds.d_nr, c->hds.d_count};

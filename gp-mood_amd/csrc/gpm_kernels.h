@@ -36,10 +36,10 @@ __host__ __device__ constexpr int stage_ty(int mord) { return mord == 3 ? GPM_TY
 constexpr int stage_waves(int mord) { return mord == 3 ? GPM_STAGE_WAVES_PER_EU3 : GPM_STAGE_WAVES_PER_EU; }
 constexpr int kStageThreads = 256;
 #ifndef GPM_MOOD_THREADS
-#define GPM_MOOD_THREADS 256
+#define GPM_MOOD_THREADS 512
 #endif
 constexpr int kMoodThreads = GPM_MOOD_THREADS;
-constexpr int kSoloMax = 64;  // largest list a single workgroup takes over
+constexpr int kSoloMax = 32;  // largest list a single workgroup takes over
 // Halo width: the stencil radius R = (Mord-1)/2 plus one cell for the face
 // neighbour, and at least 2 for the detection (U2 reads a cell's 5-point
 // neighbours' 5-point stencils).
@@ -60,6 +60,7 @@ enum {
     kCntNext = 7,     // the pass that follows a launch or a run of workgroup-0 passes
     kCntSecond = 8,   // cells recomputed in the correction passes
     kCntPending = 9,  // per-pass launches: the current pass has cells to recompute
+    kCntStop = 10,    // the first pass of workgroup 0's last run of passes, once that run has ended
     kCountersPerStage = 16
 };
 
@@ -139,6 +140,20 @@ struct StageArgs {
     int tiles_x, ntiles;
 };
 
+// Device-wide face-flux cache of a stage (open-addressing hash table): slot
+// h holds key word kw = (stage tag << 32) | face-and-order key, the flux, and
+// `ready` = the stage tag once the flux is written.  Entries of earlier
+// stages are free slots; nothing is cleared between stages.
+struct FluxTable {
+    unsigned long long *kw;
+    unsigned *ready;
+    double4 *data;
+    int bits;        // log2 of the slot count; 0: no table
+    int radius;      // the helpers take the cells within this distance of workgroup 0's list
+    int *beacon;     // [2 + kSoloMax]: workgroup 0's current pass, list length and list
+    unsigned *done;  // per cell: the stage tag once a workgroup has taken the cell
+};
+
 struct MoodArgs {
     const double4 *__restrict__ Uin;
     double4 *__restrict__ Uout;
@@ -156,6 +171,8 @@ struct MoodArgs {
     int dmp, u2, u2tol, cascade, stage_idx;
     int nn_passes;  // count the passes that lowered a cell (a step of the reference network)
     int solo_max;   // lists of at most this many cells run on workgroup 0 alone (0: never)
+    int fcache;     // workgroup 0's passes reuse face fluxes and compute likely next ones ahead
+    FluxTable gfc;  // while workgroup 0 runs passes alone, the other workgroups fill it (cooperative kernel)
     const float *__restrict__ vnn;  // as StageArgs::vnn
     int recon;                      // kRecon*
     double dx, dy;

@@ -402,32 +402,84 @@ struct GridMark {
 };
 
 // The same for the passes workgroup 0 runs alone: the list lives in LDS and
-// an LDS hash set (open addressing, -1 = empty, cleared before each pass)
-// deduplicates it, so these passes make no global atomic round trips.
-constexpr int kSoloHash = 1024;  // slots; a solo pass lists at most 5 * kSoloMax cells
-struct SoloMark {
-    int *hash, *list, *cnt;
-    __device__ __forceinline__ void operator()(int i, int j, int NX, int NY) const
+// an LDS hash set (open addressing, -1 = empty) deduplicates it, so these
+// passes make no global atomic round trips.
+constexpr int kSoloHashBits = 9;
+constexpr int kSoloHash = 1 << kSoloHashBits;  // slots; a solo pass lists at most 5 * kSoloMax cells
+__device__ __forceinline__ bool lds_set_insert(int *hash, int bits, int key)
+{
+    unsigned h = ((unsigned)key * 0x9E3779B1u) >> (32 - bits);
+    for (;;) {
+        const int old = atomicCAS(&hash[h], -1, key);
+        if (old == -1) return true;
+        if (old == key) return false;
+        h = (h + 1) & ((1u << bits) - 1u);
+    }
+}
+__device__ __forceinline__ bool lds_set_has(const int *hash, int bits, int key)
+{
+    unsigned h = ((unsigned)key * 0x9E3779B1u) >> (32 - bits);
+    for (;;) {
+        const int v = hash[h];
+        if (v == key) return true;
+        if (v == -1) return false;
+        h = (h + 1) & ((1u << bits) - 1u);
+    }
+}
+static_assert(5 * kSoloMax <= kSoloHash / 2, "the solo hash set must stay at most half full");
+
+// Face-flux cache of the passes workgroup 0 runs alone.  A face's flux at a
+// given order (the quadrature sum of its Gauss-point Riemann fluxes) depends
+// only on the stage input and that order, so within a stage it can be reused
+// bit for bit.  Entries live in a direct-mapped window of kFcW x kFcW cells
+// (a cell owns its left x-face and bottom y-face; two slots per face, by the
+// parity of the order index, so a face's current order and the next lower
+// one coexist).  A slot's 64-bit tag holds the face-and-order key and a
+// version: writers claim a slot by compare-and-swap (version + 1, busy), write
+// the flux, and publish (version + 2, key); readers compare the tag before and
+// after reading the flux, so a torn read is a miss, never a wrong value.
+constexpr int kFcW = 16;
+constexpr int kFcSlots = kFcW * kFcW * 4;
+constexpr unsigned kFcEmpty = 0xFFFFFFFFu, kFcBusy = 0xFFFFFFFEu;
+__device__ __forceinline__ int fc_slot(int dir, int fi, int fj, int oi)
+{
+    return ((((fj & (kFcW - 1)) * kFcW + (fi & (kFcW - 1))) * 2 + dir) * 2) + (oi & 1);
+}
+// x-face (fi, fj) lies between cells (fi - 1, fj) and (fi, fj), fi in [0, NX];
+// y-face (fi, fj) between (fi, fj - 1) and (fi, fj), fj in [0, NY]
+__device__ __forceinline__ unsigned fc_key(int dir, int fi, int fj, int oi, int NX, int NY)
+{
+    const unsigned id = dir == 0 ? (unsigned)fj * (unsigned)(NX + 1) + (unsigned)fi
+                                 : (unsigned)(NX + 1) * (unsigned)NY + (unsigned)fj * (unsigned)NX + (unsigned)fi;
+    return id * 4u + (unsigned)oi;
+}
+// whether every face key of an NX x NY grid stays below kFcBusy
+__host__ __device__ __forceinline__ bool fc_fits(int NX, int NY)
+{
+    const unsigned long long faces = (unsigned long long)(NX + 1) * NY + (unsigned long long)NX * (NY + 1);
+    return 4ull * faces < (unsigned long long)kFcBusy;
+}
+
+// A stage input's 5-point neighbourhood held in registers, as an accessor in
+// the staged-box coordinates of the detection (centre at (h, h)).
+struct FiveAcc {
+    double4 c, l, r, b, t;
+    int h;
+    __device__ __forceinline__ double4 operator()(int i, int j) const
     {
-        const int ni[5] = {i, i - 1, i, i + 1, i}, nj[5] = {j - 1, j, j, j, j + 1};
-#pragma unroll
-        for (int k = 0; k < 5; ++k) {
-            if (ni[k] < 0 || ni[k] >= NX || nj[k] < 0 || nj[k] >= NY) continue;
-            const int d = nj[k] * NX + ni[k];
-            unsigned h = ((unsigned)d * 0x9E3779B1u) >> 22;  // 10 bits: kSoloHash slots
-            for (;;) {
-                const int old = atomicCAS(&hash[h], -1, d);
-                if (old == -1) {
-                    list[atomicAdd(cnt, 1)] = d;
-                    break;
-                }
-                if (old == d) break;
-                h = (h + 1) & (kSoloHash - 1);
-            }
-        }
+        const int di = i - h, dj = j - h;
+        return di < 0 ? l : di > 0 ? r : dj < 0 ? b : dj > 0 ? t : c;
     }
 };
-static_assert(5 * kSoloMax <= kSoloHash / 2, "the solo hash set must stay at most half full");
+
+// LDS writes of a wavefront's lanes visible to its other lanes (no workgroup
+// barrier: the lanes that stage a cell's neighbourhood are the lanes that read it).
+__device__ __forceinline__ void wave_lds_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
 
 // One MOOD correction pass (mod_FE.f90:69-174) over a list of cells, shared
 // by the cooperative kernel and the per-pass launches.  Pass p recomputes the
@@ -454,20 +506,37 @@ struct MoodPass {
     static constexpr int HB = kHalo<MORD>;
     static constexpr int BOX = 2 * HB + 1;
     static constexpr int CPB = kMoodThreads / G;  // cells per workgroup per trip
+    // speculative jobs of one workgroup-0 pass: for each of the first 16 listed
+    // cells, the cell at its next lower order and its four neighbours
+    static constexpr int FCN = MORD == 1 ? 1 : kFcSlots;
+    static constexpr int PW = 64 / G;  // jobs per wavefront
     struct Lds {
         DevTables T;  // the GP tables (the reconstruction's weights are read per lane)
         double4 nb[CPB][BOX * BOX];
+        // face-flux cache of the workgroup-0 passes (none at first order, where
+        // a pass never runs: FOG has no detection)
+        unsigned long long ctag[FCN];
+        double4 cflux[FCN];
         // lists of the passes workgroup 0 runs alone (a failing cell adds at
-        // most five cells, so a list of at most kSoloMax cells yields at most 5x)
+        // most five cells, so a list of at most kSoloMax cells yields at most
+        // 5x).  List q + 1 is built during pass q in hash[(q + 1) % 3]; pass q
+        // tells list q's cells by hash[q % 3] and clears hash[(q + 2) % 3] and
+        // cnt[(q + 2) % 3], so a pass needs no barrier but the one before it.
         int list[2][5 * kSoloMax];
-        int hash[kSoloHash];
-        int cnt[2];
-        int fail;
+        int hash[3][kSoloHash];
+        int cnt[3];   // list q has cnt[q % 3] cells
+        int fail[3];  // failures of pass q in fail[q % 3]
+        // the other workgroups' precompute (cooperative kernel): workgroup
+        // 0's pass and list as last read, cells taken this round
+        int hseq, hm, hn, hstop;
+        int hlist[16];
     };
 
     const MoodArgs &a;
     Lds &L;
     int NX, NY, lane, gl, gbase, f, jg, fdir, fa, fb, oai, oaj, obi, obj, cs, stamp_base;
+    bool fc_on, g_on;
+    unsigned gst;  // this stage's tag in the device-wide cache
     MoodParams P;
     double dtdx, dtdy, dtg, xmin_top;
 
@@ -490,6 +559,9 @@ struct MoodPass {
         obj = f == 3 ? 1 : 0;
         cs = threadIdx.x / G;
         stamp_base = *a.stamp;
+        fc_on = MORD > 1 && a.fcache && fc_fits(NX, NY);
+        g_on = fc_on && a.gfc.bits > 0;
+        gst = (unsigned)stamp_base + 1u;
         P = MoodParams{a.dmp, a.u2, a.u2tol, a.dx, a.dy, a.recon};
         dtdx = a.tstate[2];
         dtdy = a.tstate[3];
@@ -504,6 +576,7 @@ struct MoodPass {
         const double *src = reinterpret_cast<const double *>(a.T);
         double *dst = reinterpret_cast<double *>(&L.T);
         for (int k = threadIdx.x; k < (int)(sizeof(DevTables) / sizeof(double)); k += blockDim.x) dst[k] = src[k];
+        for (int k = threadIdx.x; k < FCN; k += blockDim.x) L.ctag[k] = kFcEmpty;
         __syncthreads();
     }
 
@@ -592,44 +665,395 @@ struct MoodPass {
         }
     }
 
+    // Face-flux cache access (workgroup 0's passes).
+    __device__ __forceinline__ bool fc_get(unsigned key, int slot, double4 &F)
+    {
+        constexpr int WG = __HIP_MEMORY_SCOPE_WORKGROUP;
+        unsigned long long *tp = &L.ctag[slot];
+        const unsigned long long t1 = __hip_atomic_load(tp, __ATOMIC_RELAXED, WG);
+        if ((unsigned)t1 != key) return false;
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+        F = L.cflux[slot];
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+        return __hip_atomic_load(tp, __ATOMIC_RELAXED, WG) == t1;
+    }
+    __device__ __forceinline__ void fc_put(unsigned key, int slot, const double4 &F)
+    {
+        constexpr int WG = __HIP_MEMORY_SCOPE_WORKGROUP;
+        unsigned long long *tp = &L.ctag[slot];
+        unsigned long long t = __hip_atomic_load(tp, __ATOMIC_RELAXED, WG);
+        if ((unsigned)t == key || (unsigned)t == kFcBusy) return;
+        const unsigned long long ver = t >> 32;
+        if (!__hip_atomic_compare_exchange_strong(tp, &t, ((ver + 1) << 32) | kFcBusy, __ATOMIC_RELAXED,
+                                                  __ATOMIC_RELAXED, WG))
+            return;
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+        L.cflux[slot] = F;
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+        __hip_atomic_store(tp, ((ver + 2) << 32) | key, __ATOMIC_RELAXED, WG);
+    }
+
+    // Device-wide cache access.  Probe sequences are short and bounded; a
+    // full or crowded table only costs misses.  A writer claims a free slot
+    // (an empty one or one of an earlier stage) by compare-and-swap of its
+    // key word, writes the flux and then `ready`, with a device-scope release
+    // between; a reader that finds the key and `ready` reads the flux after
+    // them.  The flux of a face at an order is the same bits whoever computes it.
+    static constexpr int kGProbe = 8;
+    __device__ __forceinline__ unsigned g_hash(unsigned key) const
+    {
+        return (key * 0x9E3779B1u) >> (32 - a.gfc.bits);
+    }
+    __device__ __forceinline__ bool gfc_get(unsigned key, double4 &F) const
+    {
+        constexpr int AG = __HIP_MEMORY_SCOPE_AGENT;
+        const unsigned long long want = ((unsigned long long)gst << 32) | key;
+        const unsigned mask = (1u << a.gfc.bits) - 1u;
+        unsigned h = g_hash(key);
+        for (int pr = 0; pr < kGProbe; ++pr) {
+            const unsigned long long kw = __hip_atomic_load(&a.gfc.kw[h], __ATOMIC_RELAXED, AG);
+            const unsigned r = __hip_atomic_load(&a.gfc.ready[h], __ATOMIC_RELAXED, AG);
+            if (kw == want) {
+                if (r != gst) return false;
+                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+                const double *src = reinterpret_cast<const double *>(&a.gfc.data[h]);
+                F.x = __hip_atomic_load(src + 0, __ATOMIC_RELAXED, AG);
+                F.y = __hip_atomic_load(src + 1, __ATOMIC_RELAXED, AG);
+                F.z = __hip_atomic_load(src + 2, __ATOMIC_RELAXED, AG);
+                F.w = __hip_atomic_load(src + 3, __ATOMIC_RELAXED, AG);
+                return true;
+            }
+            if ((unsigned)(kw >> 32) != gst) return false;  // the first free slot ends the sequence
+            h = (h + 1) & mask;
+        }
+        return false;
+    }
+    __device__ __forceinline__ void gfc_put(unsigned key, const double4 &F) const
+    {
+        constexpr int AG = __HIP_MEMORY_SCOPE_AGENT;
+        const unsigned long long want = ((unsigned long long)gst << 32) | key;
+        const unsigned mask = (1u << a.gfc.bits) - 1u;
+        unsigned h = g_hash(key);
+        for (int pr = 0; pr < kGProbe; ++pr) {
+            unsigned long long kw = __hip_atomic_load(&a.gfc.kw[h], __ATOMIC_RELAXED, AG);
+            if (kw == want) return;
+            if ((unsigned)(kw >> 32) != gst &&
+                __hip_atomic_compare_exchange_strong(&a.gfc.kw[h], &kw, want, __ATOMIC_RELAXED, __ATOMIC_RELAXED, AG)) {
+                double *dst = reinterpret_cast<double *>(&a.gfc.data[h]);
+                __hip_atomic_store(dst + 0, F.x, __ATOMIC_RELAXED, AG);
+                __hip_atomic_store(dst + 1, F.y, __ATOMIC_RELAXED, AG);
+                __hip_atomic_store(dst + 2, F.z, __ATOMIC_RELAXED, AG);
+                __hip_atomic_store(dst + 3, F.w, __ATOMIC_RELAXED, AG);
+                __hip_atomic_store(&a.gfc.ready[h], gst, __ATOMIC_RELEASE, AG);
+                return;
+            }
+            if (kw == want) return;  // claimed for this key meanwhile
+            h = (h + 1) & mask;
+        }
+    }
+
+    // Speculative job number t (this lane's group's) of workgroup-0 pass q,
+    // listed by the whole wavefront from list q: for each of its first 16
+    // cells the cell at its next lower order (its faces if it fails now),
+    // then each neighbour not in the list at the current orders (the faces of
+    // the cells that enter list q + 1 if it fails).  Every wavefront that
+    // runs speculative jobs lists them itself, in the same order.  Which jobs
+    // run changes which fluxes are cached, never a result.  Returns false
+    // when there is no job t.
+    __device__ __forceinline__ bool spec_job(int q, int m, int t, int &d, int &var)
+    {
+        const int k = lane >> 2, dd = lane & 3;
+        bool el = false, ec = false;
+        int vl = 0, vc = 0;
+        if (k < m) {
+            const int c = L.list[q & 1][k];
+            const int ci = c % NX, cj = c / NX;
+            el = dd == 0;
+            vl = 2 * c + 1;
+            const int ni = ci + (dd == 1 ? -1 : dd == 2 ? 1 : 0), nj = cj + (dd == 0 ? -1 : dd == 3 ? 1 : 0);
+            if (ni >= 0 && ni < NX && nj >= 0 && nj < NY) {
+                const int n = nj * NX + ni;
+                ec = !lds_set_has(L.hash[q % 3], kSoloHashBits, n);
+                vc = 2 * n;
+            }
+        }
+        const unsigned long long bl = __ballot(el), bc = __ballot(ec);
+        const unsigned long long lt = (1ull << lane) - 1ull;
+        const int nl = __popcll(bl), il = __popcll(bl & lt), ic = nl + __popcll(bc & lt);
+        int v = -1;
+#pragma unroll
+        for (int g = 0; g < PW; ++g) {
+            const int tg = __shfl(t, g * G);
+            const unsigned long long sl = __ballot(el && il == tg), sc = __ballot(ec && ic == tg);
+            const int src = sl ? __ffsll((long long)sl) - 1 : sc ? __ffsll((long long)sc) - 1 : 0;
+            const int val = __shfl(sl ? vl : vc, src);
+            if (lane / G == g && (sl | sc)) v = val;
+        }
+        if (v < 0) return false;
+        d = v >> 1;
+        var = v & 1;
+        return true;
+    }
+
+    // One job of a workgroup-0 pass q on this lane's group of G lanes:
+    // kind 1 recomputes listed cell d (the pass's work, as in run()), kind 2
+    // computes the face fluxes of cell d at the current orders (var 0) or with
+    // the cell at its next lower order (var 1) into the cache, kind 3 (the
+    // other workgroups) computes all four face fluxes of cell d at order
+    // 2 var + 1 into the device-wide cache, kind 0 idles.  Each face flux
+    // comes from a cache (workgroup 0: its LDS cache, then the device-wide
+    // one) or is computed and cached; the staged neighbourhood is written and
+    // read by the group's own lanes, so jobs of different wavefronts never
+    // wait for each other.
+    __device__ __forceinline__ void job(int q, int kind, int d, int var)
+    {
+        using M = Scoped<__HIP_MEMORY_SCOPE_WORKGROUP>;
+        const DevTables *__restrict__ T = &L.T;
+        const bool valid = kind != 0;
+        const int i = d % NX, j = d / NX;
+        auto order_of = [&](int ii, int jj) -> int {
+            if (ii < 0 || ii >= NX || jj < 0 || jj >= NY) return MORD;  // ghost cells keep Mord
+            return order_at_pass(M::load(&a.gpo[(size_t)jj * NX + ii]), q);
+        };
+        // this lane's face f: the neighbour across it, the face's coordinates
+        const int nbi = i + (f == 1 ? -1 : f == 2 ? 1 : 0), nbj = j + (f == 0 ? -1 : f == 3 ? 1 : 0);
+        const int fi = i + (f == 2 ? 1 : 0), fj = j + (f == 3 ? 1 : 0);
+        int me = 1, ord = 1;
+        if (kind == 1 || kind == 2) {
+            me = order_of(i, j);
+            ord = min(var ? lower_order(me, a.cascade) : me, order_of(nbi, nbj));
+        } else if (kind == 3) {
+            ord = 2 * var + 1;
+        }
+        uint8_t fl = 0;
+        const double4 zero = make_double4(0.0, 0.0, 0.0, 0.0);
+        double4 un = zero, c0 = zero, cl = zero, cr = zero, cb = zero, ct = zero;
+        if (kind == 1 && gl == 0) {
+            if (a.C.kind != kCmbCopy) un = a.C.Un[d];
+            fl = a.dflag[d];
+            c0 = load_stage<GEN>(a.Uin, a.B, i, j, xmin_top);
+            cl = load_stage<GEN>(a.Uin, a.B, i - 1, j, xmin_top);
+            cr = load_stage<GEN>(a.Uin, a.B, i + 1, j, xmin_top);
+            cb = load_stage<GEN>(a.Uin, a.B, i, j - 1, xmin_top);
+            ct = load_stage<GEN>(a.Uin, a.B, i, j + 1, xmin_top);
+        }
+        const int oi = (ord - 1) >> 1;
+        const unsigned key = fc_on ? fc_key(fdir, fi, fj, oi, NX, NY) : 0u;
+        const int slot = fc_slot(fdir, fi, fj, oi);
+        double4 Lc = zero;
+        bool hit = false;
+        if (valid && fc_on && jg == 0) {
+            if (kind != 3) hit = fc_get(key, slot, Lc);
+            if (!hit && g_on) {
+                hit = gfc_get(key, Lc);
+                if (hit && kind != 3) fc_put(key, slot, Lc);
+            }
+        }
+        // the group's four face lanes are gbase + f * NGPP
+        const unsigned long long hm = __ballot(hit);
+        const unsigned long long lead = (1ull | (1ull << NGPP) | (1ull << (2 * NGPP)) | (1ull << (3 * NGPP))) << gbase;
+        const bool allhit = (hm & lead) == lead;
+        const bool fhit = (hm >> (gbase + f * NGPP)) & 1ull;
+        double Lf[4] = {Lc.x, Lc.y, Lc.z, Lc.w};
+        // a speculative job only copies what the device-wide cache holds into
+        // the LDS one; with no device-wide cache it computes the faces itself
+        if (valid && !allhit && !(kind == 2 && g_on)) {
+            for (int qq = gl; qq < BOX * BOX; qq += G)
+                L.nb[cs][qq] = load_stage<GEN>(a.Uin, a.B, i - HB + qq % BOX, j - HB + qq / BOX, xmin_top);
+            wave_lds_sync();
+            const LdsAcc lacc{&L.nb[cs][0], BOX};
+            double F[4] = {0.0, 0.0, 0.0, 0.0};
+            if (!fhit && jg < NGP) {
+                double va1 = 0.0, va2 = 1.0, vb1 = 0.0, vb2 = 1.0;
+                const int recon = MORD == 3 ? a.recon : kReconGP;
+                if (recon == kReconCC) {
+                    vnn_at(a.vnn, NX, NY, i + oai, j + oaj, va1, va2);
+                    vnn_at(a.vnn, NX, NY, i + obi, j + obj, vb1, vb2);
+                }
+                face_gp_flux_dyn<MORD, FLUX>(T, lacc, ord, fa, fb, fdir, jg, HB + oai, HB + oaj, HB + obi, HB + obj, F,
+                                             recon, va1, va2, vb1, vb2);
+            }
+            // quadrature over the face's Gauss points, j = 1..ngp from 0
+            double S[4] = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+            for (int jj = 0; jj < NGP; ++jj) {
+                const int src = gbase + f * NGPP + jj;
+                const double w = T->gw[jj];
+#pragma unroll
+                for (int k = 0; k < 4; ++k) S[k] = S[k] + __shfl(F[k], src) * w;
+            }
+            if (!fhit) {
+#pragma unroll
+                for (int k = 0; k < 4; ++k) Lf[k] = S[k];
+                if (jg == 0 && fc_on) {
+                    if (kind == 3) gfc_put(key, make_double4(S[0], S[1], S[2], S[3]));
+                    else fc_put(key, slot, make_double4(S[0], S[1], S[2], S[3]));
+                }
+            }
+        }
+        double fyb[4], fxl[4], fxr[4], fyt[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            fyb[k] = __shfl(Lf[k], gbase);
+            fxl[k] = __shfl(Lf[k], gbase + NGPP);
+            fxr[k] = __shfl(Lf[k], gbase + 2 * NGPP);
+            fyt[k] = __shfl(Lf[k], gbase + 3 * NGPP);
+        }
+        bool dec = false;
+        if (kind == 1 && gl == 0) {
+            double cand[4];
+            apply_fluxes(c0, fyb, fxl, fxr, fyt, dtdx, dtdy, cand);
+            if (me > 1) dec = decide_with_flags(P, FiveAcc{c0, cl, cr, cb, ct, HB}, HB, HB, cand, fl);
+            write_stage_output<GEN, kCmbRuntime>(a.C, a.Uout, d, GEN && a.C.band && in_band(i, j, NX, NY), un, c0,
+                                                 cand, dtg);
+            if (dec) {
+                M::store(&a.gpo[d], pack_order(lower_order(me, a.cascade), me, q));
+                __hip_atomic_fetch_add(&L.fail[q % 3], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+            }
+        }
+        // a failing cell's 5-point neighbourhood enters list q + 1, one
+        // neighbour per lane (lanes 0..4 of the group)
+        if (__shfl(dec ? 1 : 0, gbase) && gl < 5) {
+            const int ni = i + (gl == 1 ? -1 : gl == 3 ? 1 : 0), nj = j + (gl == 0 ? -1 : gl == 4 ? 1 : 0);
+            if (ni >= 0 && ni < NX && nj >= 0 && nj < NY) {
+                const int dn = nj * NX + ni;
+                if (lds_set_insert(L.hash[(q + 1) % 3], kSoloHashBits, dn))
+                    L.list[(q + 1) & 1][atomicAdd(&L.cnt[(q + 1) % 3], 1)] = dn;
+            }
+        }
+    }
+
+    // The other workgroups of the cooperative kernel while workgroup 0 runs
+    // the passes from p alone: they fill the device-wide cache ahead of it.
+    // Workgroup 0 publishes each pass's list (the first 16 cells); a failure
+    // chain moves by one cell per pass, so the helpers take the cells within
+    // gfc.radius (Chebyshev distance) of the listed ones (each cell once per
+    // stage, claimed in `done`) and compute their four face fluxes at every
+    // order.  They stop when workgroup 0 has ended this run of passes.
+    __device__ __forceinline__ void helper(int p)
+    {
+        if (!g_on || gridDim.x < 2) return;
+        constexpr int AG = __HIP_MEMORY_SCOPE_AGENT;
+        constexpr int K = (MORD + 1) / 2;  // orders 1, 3, ..., Mord
+        const int hr = a.gfc.radius < 0 ? 0 : a.gfc.radius > 8 ? 8 : a.gfc.radius;
+        const int WIN = 2 * hr + 1, NWIN = WIN * WIN;
+        const int mcap = 3 * kSoloHash / NWIN;  // listed cells whose windows fit the solo tables
+        const int tid = threadIdx.x;
+        const int nh = (int)gridDim.x - 1, hw = (int)blockIdx.x - 1;
+        const int slots = (int)blockDim.x / G;
+        int *cells = &L.hash[0][0];  // the solo tables are unused outside workgroup 0
+        int last = -1;
+        for (;;) {
+            if (tid == 0) {
+                L.hstop = load_relaxed(&a.cnt[kCntStop]) == p;
+                L.hseq = __hip_atomic_load(&a.gfc.beacon[0], __ATOMIC_RELAXED, AG);
+                const int m = __hip_atomic_load(&a.gfc.beacon[1], __ATOMIC_RELAXED, AG);
+                L.hm = m < 0 ? 0 : m > 16 ? 16 : m;
+                L.hn = 0;
+            }
+            if (tid < 16) L.hlist[tid] = __hip_atomic_load(&a.gfc.beacon[2 + tid], __ATOMIC_RELAXED, AG);
+            __syncthreads();
+            const bool stop = L.hstop, fresh = L.hseq != last;
+            const int m = L.hm < mcap ? L.hm : mcap;
+            last = L.hseq;
+            if (stop) break;
+            if (fresh) {
+                // this workgroup's share of the window cells, each taken once per stage
+                for (int k = tid; k < m * NWIN; k += blockDim.x) {
+                    if (k % nh != hw) continue;
+                    const int c0 = L.hlist[k / NWIN], o = k % NWIN;
+                    if (c0 < 0 || c0 >= NX * NY) continue;
+                    const int ci = c0 % NX + o % WIN - hr, cj = c0 / NX + o / WIN - hr;
+                    if (ci < 0 || ci >= NX || cj < 0 || cj >= NY) continue;
+                    const int c = cj * NX + ci;
+                    if (__hip_atomic_exchange(&a.gfc.done[c], gst, __ATOMIC_RELAXED, AG) != gst)
+                        cells[atomicAdd(&L.hn, 1)] = c;
+                }
+            }
+            __syncthreads();
+            const int nj = fresh ? L.hn * K : 0;
+            for (int b = 0; b < nj; b += slots) {
+                const int e = b + cs;
+                int kind = 0, c = 0, var = 0;
+                if (e < nj) {
+                    kind = 3;
+                    c = cells[e / K];
+                    var = e % K;
+                }
+                job(p, kind, c, var);
+            }
+            if (!fresh) __builtin_amdgcn_s_sleep(8);
+            __syncthreads();  // every lane is done with the round's LDS fields
+        }
+    }
+
     // Short lists (the long tail of a failure chain): workgroup 0 runs the
     // passes from p on alone while the list has 1..solo_max cells, with
     // workgroup barriers, the lists, counts and deduplication in LDS and
     // workgroup-scope memory operations, which cost far less than grid
     // barriers and device-scope round trips.  Same passes, same order of
-    // effects: the results are those of the grid-wide path.  The current list
-    // goes back to global memory with its count; the stage's counters record
-    // the passes run.  Returns the first pass not run.
+    // effects: the results are those of the grid-wide path.  Each pass runs
+    // the listed cells' jobs and, on the wavefronts the list leaves free, the
+    // speculative jobs that fill the face-flux cache for the next pass.  The
+    // current list goes back to global memory with its count; the stage's
+    // counters record the passes run.  Returns the first pass not run.
     __device__ __forceinline__ int solo(int p, int nS)
     {
-        constexpr int WG = __HIP_MEMORY_SCOPE_WORKGROUP;
         const int tid = threadIdx.x;
         for (int k = tid; k < nS; k += blockDim.x) L.list[p & 1][k] = a.slist[p & 1][k];
-        if (tid == 0) L.cnt[p & 1] = nS;
+        for (int k = tid; k < 3 * kSoloHash; k += blockDim.x) L.hash[k / kSoloHash][k % kSoloHash] = -1;
+        if (tid < 3) {
+            L.cnt[tid] = tid == p % 3 ? nS : 0;
+            L.fail[tid] = 0;
+        }
+        const int slots = (int)blockDim.x / G;
         int q = p, second = 0, det = 0;
         for (;;) {
             // the previous pass's order words and outputs are visible to every
             // wavefront, its list and counts are final
             __syncthreads();
-            if (q > p) det += L.fail;
-            const int m = L.cnt[q & 1];
-            if (m == 0 || m > a.solo_max) break;
-            for (int k = tid; k < kSoloHash; k += blockDim.x) L.hash[k] = -1;
-            lds_sync();  // every lane has read the count and the failures
-            if (tid == 0) {
-                L.cnt[(q + 1) & 1] = 0;
-                L.fail = 0;
+            const int m = L.cnt[q % 3];
+            if (tid == 0 && q > p) {
+                det += L.fail[(q - 1) % 3];
+                L.fail[(q - 1) % 3] = 0;  // next written in pass q + 2
             }
+            if (m == 0 || m > a.solo_max) break;
             second += m;
-            const int e0 = tid / G < m ? L.list[q & 1][tid / G] : 0;
-            run<WG>(q, m, e0, tid, blockDim.x, L.list[q & 1], &L.fail,
-                          SoloMark{L.hash, L.list[(q + 1) & 1], &L.cnt[(q + 1) & 1]});
+            // the tables of pass q + 1 (no reader or writer in this pass)
+            if (tid == 0) L.cnt[(q + 2) % 3] = 0;
+            for (int k = tid; k < kSoloHash; k += blockDim.x) L.hash[(q + 2) % 3][k] = -1;
+            // the pass and its list for the other workgroups (read as hints only)
+            if (g_on && tid < 16) {
+                constexpr int AG = __HIP_MEMORY_SCOPE_AGENT;
+                if (tid < m) __hip_atomic_store(&a.gfc.beacon[2 + tid], L.list[q & 1][tid], __ATOMIC_RELAXED, AG);
+                if (tid == 0) {
+                    __hip_atomic_store(&a.gfc.beacon[1], m, __ATOMIC_RELAXED, AG);
+                    __hip_atomic_store(&a.gfc.beacon[0], q, __ATOMIC_RELAXED, AG);
+                }
+            }
+            // listed cells first, speculative jobs from the next wavefront on
+            const int e1 = m < slots ? m : slots;
+            const int ew = (e1 + PW - 1) / PW * PW;
+            const int niter = (m + slots - 1) / slots;
+            for (int it = 0; it < niter; ++it) {
+                const int e = it * slots + cs;
+                int kind = 0, d = 0, var = 0;
+                if (e < m) {
+                    kind = 1;
+                    d = L.list[q & 1][e];
+                } else if (it == 0 && fc_on && m <= slots && cs >= ew) {
+                    // whole wavefronts from here on
+                    if (spec_job(q, m, cs - ew, d, var)) kind = 2;
+                }
+                job(q, kind, d, var);
+            }
             ++q;
         }
         // hand the current list back to the grid-wide path
-        const int m = L.cnt[q & 1];
+        const int m = L.cnt[q % 3];
         for (int k = tid; k < m; k += blockDim.x) a.slist[q & 1][k] = L.list[q & 1][k];
         if (tid == 0) {
+            __hip_atomic_store(&a.cnt[kCntStop], p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             a.cnt[kCntS0 + q % 3] = m;
             a.cnt[kCntS0 + (q + 1) % 3] = 0;
             a.cnt[kCntNext] = q;
@@ -694,8 +1118,10 @@ __global__ __launch_bounds__(kMoodThreads) void k_mood(MoodArgs a)
         const int d0 = gtid / MP::G < ncell ? a.slist[p & 1][gtid / MP::G] : 0;
         if (nS == 0) break;
         if (nS <= a.solo_max) {
-            // workgroup 0 runs the short-list passes; the others wait here
+            // workgroup 0 runs the short-list passes; the others fill the
+            // device-wide face-flux cache meanwhile, then wait here
             if (blockIdx.x == 0) mp.solo(p, nS);
+            else mp.helper(p);
             grid.sync();
             p = load_relaxed(&a.cnt[kCntNext]);
             continue;
@@ -727,6 +1153,7 @@ __global__ __launch_bounds__(kMoodThreads) void k_mood_pass(MoodArgs a)
     const int p = s_p, nS = s_n;
     if (nS == 0) return;
     MP mp(a, lds);
+    mp.g_on = false;  // no other workgroups fill the device-wide cache here
     mp.init_lds();
     const int ncell = a.B.NX * a.B.NY;
     const int gtid = blockIdx.x * blockDim.x + threadIdx.x;

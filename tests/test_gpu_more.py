@@ -240,3 +240,50 @@ def test_config1_2048_500_steps_vs_oracle(gpm, cascade):
     assert np.array_equal(dts, z["dt"])
     assert s.stats()["detected_total"] == int(np.sum(z["counts"]))
     assert hashlib.sha256(np.ascontiguousarray(U[4:-4, 4:-4]).tobytes()).hexdigest() == str(z["U_sha"])
+
+
+@pytest.mark.parametrize("mord,ngp,cascade,bc", [(5, 3, "reference", "neumann"), (7, 4, "full", "periodic")])
+def test_face_flux_cache_changes_no_bit(gpm, mord, ngp, cascade, bc):
+    """Workgroup 0's MOOD passes reuse cached face fluxes, compute the likely
+    next ones ahead, and read the device-wide cache that the other
+    workgroups of the cooperative kernel fill meanwhile (switched off at
+    gpm_create by GPM_MOOD_FC=0 and GPM_MOOD_HELP=0).  1024^2 four-quadrant
+    problems, whose failure chains run long runs of small passes on
+    workgroup 0, give the same dt sequence, counts, order mask and state in
+    every combination and in both MOOD launch forms."""
+    n, steps = 1024, 12
+    print(f"[case face-flux caches on/off, {n}^2 R{(mord - 1) // 2} {cascade} cascade {bc}, {steps} steps]", flush=True)
+    U0 = gpm.ic.four_quadrant(n, 0)
+    t = gpm.gp_tables(mord, ngp, n)
+    out = {}
+    variants = {"both": {"GPM_MOOD_FC": "1", "GPM_MOOD_HELP": "1"}, "lds": {"GPM_MOOD_FC": "1", "GPM_MOOD_HELP": "0"},
+                "none": {"GPM_MOOD_FC": "0", "GPM_MOOD_HELP": "0"}}
+    for name, env in variants.items():
+        for coop in (True, False):
+            old = {k: os.environ.get(k) for k in env}
+            os.environ.update(env)
+            try:
+                s = gpm.Solver(gpm.Config(lf=n, Mord=mord, ngp=ngp, BC_type=bc, cascade=cascade), t)
+            finally:
+                for k, v in old.items():
+                    if v is None:
+                        del os.environ[k]
+                    else:
+                        os.environ[k] = v
+            s.set_mood_launch(coop)
+            s.set_state(U0)
+            dts = s.run(steps, return_dts=True)
+            U, g = s.get_state(order=True)
+            st = s.stats()
+            s.close()
+            out[(name, coop)] = (dts, U, g, st)
+            print(f"  caches={name} cooperative={coop}: max_passes {st['max_passes']}, "
+                  f"second-pass cells {st['second_pass_cells']}", flush=True)
+    ref = out[("none", True)]
+    assert ref[3]["max_passes"] > 20
+    for key, (dts, U, g, st) in out.items():
+        assert np.array_equal(dts, ref[0]), key
+        assert np.array_equal(U, ref[1]), key
+        assert np.array_equal(g, ref[2]), key
+        for k in ("detected_total", "second_pass_cells", "max_passes"):
+            assert st[k] == ref[3][k], (key, k)
