@@ -765,9 +765,11 @@ int gpm_create(const gpm_config *cfg, gpm_ctx **out)
     c->ntiles = c->tiles_x * ((c->NY + ty - 1) / ty);
     // Cooperative grid of the MOOD kernel: all its workgroups must be resident.
     // The per-pass grid barrier gets cheaper with fewer workgroups, while the
-    // first correction passes (thousands of cells) want more: one per two CUs
-    // (128 on MI355X) measured best.  GPM_MOOD_BLOCKS overrides the count.
-    int want = ncu / 2 > 0 ? ncu / 2 : 1;
+    // first correction passes (thousands of cells) want more lanes, and the
+    // helpers of workgroup 0's passes need a few dozen: one workgroup of 512
+    // threads per four CUs (64 on MI355X) measured best.  GPM_MOOD_BLOCKS
+    // overrides the count.
+    int want = ncu / 4 > 0 ? ncu / 4 : 1;
     if (const char *env = std::getenv("GPM_MOOD_BLOCKS")) want = std::atoi(env) > 0 ? std::atoi(env) : ncu;
     {
         int per = 0;
@@ -780,7 +782,7 @@ int gpm_create(const gpm_config *cfg, gpm_ctx **out)
     // GPM_MOOD_COOP=0 selects the per-pass launches of the MOOD passes.
     if (const char *env = std::getenv("GPM_MOOD_COOP")) c->mood_coop = std::atoi(env) != 0;
     // Passes over at most this many cells run on one workgroup (GPM_MOOD_SOLO overrides; 0 disables).
-    c->mood_solo = 16;
+    c->mood_solo = kSoloMax;
     if (const char *env = std::getenv("GPM_MOOD_SOLO")) c->mood_solo = std::atoi(env) > 0 ? std::atoi(env) : 0;
     if (c->mood_solo > kSoloMax) c->mood_solo = kSoloMax;
     // GPM_MOOD_FC=0 switches the face-flux cache of workgroup 0's passes off (same results).
@@ -1602,10 +1604,10 @@ int gpm_run_noalloc(const gpm_config *cfg, const double *zT_sp, const double *gp
         const hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(
             &per, mood_kernel(c.mord, c.ngp, cfg->num_flux, false, true), kMoodThreads, 0);
         if (e != hipSuccess || per <= 0) per = 1;
-        const int want = ncu / 2 > 0 ? ncu / 2 : 1;
+        const int want = ncu / 4 > 0 ? ncu / 4 : 1;
         c.mood_blocks = want < per * ncu ? want : per * ncu;
     }
-    c.mood_solo = 16;
+    c.mood_solo = kSoloMax;
     if (const char *env = std::getenv("GPM_MOOD_COOP")) c.mood_coop = std::atoi(env) != 0;
     c.has_state = true;
 
