@@ -530,6 +530,9 @@ struct MoodPass {
         // 0's pass and list as last read, cells taken this round
         int hseq, hm, hn, hstop;
         int hlist[16];
+        // workgroup 0's passes: the listed cells' wavefronts' barrier, and
+        // what they tell the speculative wavefronts
+        int ebar_cnt, ebar_gen, spass, sdone, qend;
     };
 
     const MoodArgs &a;
@@ -616,10 +619,12 @@ struct MoodPass {
             if (it + 1 < niter && tn < nS) dnext = S[tn];
             uint8_t fl = 0;
             if (valid && gl == 0 && me > 1) fl = a.dflag[d];
-            lds_sync();  // the previous trip's readers are done with nb
+            // a group's staged neighbourhood is written and read by its own
+            // lanes (one wavefront): no workgroup barrier around it
+            wave_lds_sync();  // the previous trip's readers are done with nb
             for (int q = gl; q < BOX * BOX; q += G)
                 L.nb[cs][q] = load_stage<GEN>(a.Uin, a.B, i - HB + q % BOX, j - HB + q / BOX, xmin_top);
-            lds_sync();
+            wave_lds_sync();
             double F[4] = {0.0, 0.0, 0.0, 0.0};
             if (valid && jg < NGP) {
                 double va1 = 0.0, va2 = 1.0, vb1 = 0.0, vb2 = 1.0;
@@ -765,8 +770,8 @@ struct MoodPass {
         const int k = lane >> 2, dd = lane & 3;
         bool el = false, ec = false;
         int vl = 0, vc = 0;
-        if (k < m) {
-            const int c = L.list[q & 1][k];
+        const int c = k < m ? L.list[q & 1][k] : -1;
+        if (c >= 0 && c < NX * NY) {
             const int ci = c % NX, cj = c / NX;
             el = dd == 0;
             vl = 2 * c + 1;
@@ -988,68 +993,127 @@ struct MoodPass {
     }
 
     // Short lists (the long tail of a failure chain): workgroup 0 runs the
-    // passes from p on alone while the list has 1..solo_max cells, with
-    // workgroup barriers, the lists, counts and deduplication in LDS and
-    // workgroup-scope memory operations, which cost far less than grid
-    // barriers and device-scope round trips.  Same passes, same order of
-    // effects: the results are those of the grid-wide path.  Each pass runs
-    // the listed cells' jobs and, on the wavefronts the list leaves free, the
-    // speculative jobs that fill the face-flux cache for the next pass.  The
-    // current list goes back to global memory with its count; the stage's
-    // counters record the passes run.  Returns the first pass not run.
+    // passes from p on alone while the list has 1..solo_max cells, with the
+    // lists, counts and deduplication in LDS and workgroup-scope memory
+    // operations, which cost far less than grid barriers and device-scope
+    // round trips.  Same passes, same order of effects: the results are
+    // those of the grid-wide path.  Its first kEWaves wavefronts run the
+    // listed cells, pass after pass, separated by their own barrier; the
+    // other wavefronts run the speculative jobs of the latest pass those
+    // publish, filling the LDS face-flux cache for the passes that follow,
+    // and never hold a pass up.  The current list goes back to global
+    // memory with its count; the stage's counters record the passes run.
+    // Returns the first pass not run.
+    // Barrier of the kEWaves wavefronts that run the listed cells of
+    // workgroup 0's passes (the speculative wavefronts do not wait for the
+    // passes): an arrival count and a generation word in LDS, with
+    // workgroup-scope release and acquire around it, as a workgroup barrier
+    // has.  The wait is bounded; false means it timed out.
+    static constexpr int kEWaves = 4;
+    static_assert(kMoodThreads >= 64 * kEWaves, "the listed cells' wavefronts are part of the workgroup");
+    __device__ __forceinline__ bool e_barrier(int &gen)
+    {
+        constexpr int WG = __HIP_MEMORY_SCOPE_WORKGROUP;
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+        int ok = 1;
+        if (lane == 0) {
+            const int g = gen;
+            if (__hip_atomic_fetch_add(&L.ebar_cnt, 1, __ATOMIC_RELAXED, WG) == kEWaves - 1) {
+                __hip_atomic_store(&L.ebar_cnt, 0, __ATOMIC_RELAXED, WG);
+                __hip_atomic_store(&L.ebar_gen, g + 1, __ATOMIC_RELEASE, WG);
+            } else {
+                for (int spin = 0; __hip_atomic_load(&L.ebar_gen, __ATOMIC_ACQUIRE, WG) == g; ++spin) {
+                    if (spin > (1 << 22)) {
+                        ok = 0;
+                        break;
+                    }
+                    __builtin_amdgcn_s_sleep(1);
+                }
+            }
+        }
+        ++gen;
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+        return __shfl(ok, 0) != 0;
+    }
+
     __device__ __forceinline__ int solo(int p, int nS)
     {
-        const int tid = threadIdx.x;
+        constexpr int WG = __HIP_MEMORY_SCOPE_WORKGROUP;
+        const int tid = threadIdx.x, wave = tid >> 6;
+        constexpr int ES = kEWaves * PW;  // job slots of the listed cells' wavefronts
         for (int k = tid; k < nS; k += blockDim.x) L.list[p & 1][k] = a.slist[p & 1][k];
         for (int k = tid; k < 3 * kSoloHash; k += blockDim.x) L.hash[k / kSoloHash][k % kSoloHash] = -1;
         if (tid < 3) {
             L.cnt[tid] = tid == p % 3 ? nS : 0;
             L.fail[tid] = 0;
         }
-        const int slots = (int)blockDim.x / G;
-        int q = p, second = 0, det = 0;
-        for (;;) {
-            // the previous pass's order words and outputs are visible to every
-            // wavefront, its list and counts are final
-            __syncthreads();
-            const int m = L.cnt[q % 3];
-            if (tid == 0 && q > p) {
-                det += L.fail[(q - 1) % 3];
-                L.fail[(q - 1) % 3] = 0;  // next written in pass q + 2
-            }
-            if (m == 0 || m > a.solo_max) break;
-            second += m;
-            // the tables of pass q + 1 (no reader or writer in this pass)
-            if (tid == 0) L.cnt[(q + 2) % 3] = 0;
-            for (int k = tid; k < kSoloHash; k += blockDim.x) L.hash[(q + 2) % 3][k] = -1;
-            // the pass and its list for the other workgroups (read as hints only)
-            if (g_on && tid < 16) {
-                constexpr int AG = __HIP_MEMORY_SCOPE_AGENT;
-                if (tid < m) __hip_atomic_store(&a.gfc.beacon[2 + tid], L.list[q & 1][tid], __ATOMIC_RELAXED, AG);
-                if (tid == 0) {
-                    __hip_atomic_store(&a.gfc.beacon[1], m, __ATOMIC_RELAXED, AG);
-                    __hip_atomic_store(&a.gfc.beacon[0], q, __ATOMIC_RELAXED, AG);
-                }
-            }
-            // listed cells first, speculative jobs from the next wavefront on
-            const int e1 = m < slots ? m : slots;
-            const int ew = (e1 + PW - 1) / PW * PW;
-            const int niter = (m + slots - 1) / slots;
-            for (int it = 0; it < niter; ++it) {
-                const int e = it * slots + cs;
-                int kind = 0, d = 0, var = 0;
-                if (e < m) {
-                    kind = 1;
-                    d = L.list[q & 1][e];
-                } else if (it == 0 && fc_on && m <= slots && cs >= ew) {
-                    // whole wavefronts from here on
-                    if (spec_job(q, m, cs - ew, d, var)) kind = 2;
-                }
-                job(q, kind, d, var);
-            }
-            ++q;
+        if (tid == 0) {
+            L.ebar_cnt = 0;
+            L.ebar_gen = 0;
+            L.spass = -1;
+            L.sdone = 0;
         }
+        __syncthreads();
+        int second = 0, det = 0;
+        if (wave < kEWaves) {
+            int q = p, gen = 0;
+            for (;;) {
+                // the previous pass's order words and outputs are visible to
+                // these wavefronts, its list and counts are final
+                if (q > p && !e_barrier(gen)) break;
+                const int m = L.cnt[q % 3];
+                if (tid == 0 && q > p) {
+                    det += L.fail[(q - 1) % 3];
+                    L.fail[(q - 1) % 3] = 0;  // next written in pass q + 2
+                }
+                if (m == 0 || m > a.solo_max) break;
+                second += m;
+                // the tables of pass q + 1 (no reader or writer in this pass)
+                if (tid == 0) L.cnt[(q + 2) % 3] = 0;
+                for (int k = tid; k < kSoloHash; k += 64 * kEWaves) L.hash[(q + 2) % 3][k] = -1;
+                if (g_on && tid < 16) {
+                    // the pass and its list for the other workgroups (read as hints only)
+                    constexpr int AG = __HIP_MEMORY_SCOPE_AGENT;
+                    if (tid < m) __hip_atomic_store(&a.gfc.beacon[2 + tid], L.list[q & 1][tid], __ATOMIC_RELAXED, AG);
+                    if (tid == 0) {
+                        __hip_atomic_store(&a.gfc.beacon[1], m, __ATOMIC_RELAXED, AG);
+                        __hip_atomic_store(&a.gfc.beacon[0], q, __ATOMIC_RELAXED, AG);
+                    }
+                }
+                if (tid == 0) __hip_atomic_store(&L.spass, q, __ATOMIC_RELEASE, WG);
+                const int niter = (m + ES - 1) / ES;
+                for (int it = 0; it < niter; ++it) {
+                    const int e = it * ES + cs;
+                    job(q, e < m ? 1 : 0, e < m ? L.list[q & 1][e] : 0, 0);
+                }
+                ++q;
+            }
+            if (tid == 0) {
+                L.qend = q;
+                __hip_atomic_store(&L.sdone, 1, __ATOMIC_RELEASE, WG);
+            }
+        } else if (fc_on) {
+            // speculative wavefronts: the jobs of the latest pass published,
+            // until the passes end (they never hold a pass up)
+            int last = -1;
+            for (;;) {
+                if (__hip_atomic_load(&L.sdone, __ATOMIC_ACQUIRE, WG)) break;
+                const int cur = __hip_atomic_load(&L.spass, __ATOMIC_ACQUIRE, WG);
+                if (cur < 0 || cur == last) {
+                    __builtin_amdgcn_s_sleep(2);
+                    continue;
+                }
+                last = cur;
+                int mm = L.cnt[cur % 3];
+                mm = mm < 0 ? 0 : mm > 16 ? 16 : mm;
+                int d = 0, var = 0;
+                const bool has = spec_job(cur, mm, cs - ES, d, var);
+                job(cur, has ? 2 : 0, d, var);
+            }
+        }
+        __syncthreads();
         // hand the current list back to the grid-wide path
+        const int q = L.qend;
         const int m = L.cnt[q % 3];
         for (int k = tid; k < m; k += blockDim.x) a.slist[q & 1][k] = L.list[q & 1][k];
         if (tid == 0) {
