@@ -176,6 +176,34 @@ __device__ __forceinline__ void wave_append(bool flag, int value, int *__restric
     if (flag) list[base + __popcll(m & ((1ull << lane) - 1ull))] = value;
 }
 
+// Append up to K values per lane (value k where add[k]) to list/cnt: one
+// atomic per wavefront for all of them.  Many cells of a pass can fail at
+// once, and one atomic per entry on the single count word serialises in L2.
+template <int K>
+__device__ __forceinline__ void wave_append_n(const bool (&add)[K], const int (&val)[K], int *__restrict__ list,
+                                              int *cnt)
+{
+    unsigned long long m[K];
+    int tot = 0;
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+        m[k] = __ballot(add[k]);
+        tot += __popcll(m[k]);
+    }
+    if (tot == 0) return;
+    const int lane = __lane_id();
+    const int leader = __ffsll((long long)__ballot(1)) - 1;
+    int base = 0;
+    if (lane == leader) base = atomicAdd(cnt, tot);
+    base = __shfl(base, leader);
+    const unsigned long long lt = (1ull << lane) - 1ull;
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+        if (add[k]) list[base + __popcll(m[k] & lt)] = val[k];
+        base += __popcll(m[k]);
+    }
+}
+
 // Cell update of the face loop (mod_FE.f90:100-101, 134-135) in the order the
 // reference's n-outer, l-inner loop applies it to an interior cell.
 __device__ __forceinline__ void apply_fluxes(const double4 &u, const double fyb[4], const double fxl[4],
@@ -223,17 +251,20 @@ __device__ __forceinline__ int xcd_tile(int b, int nb)
 
 // Flag the 5-point neighbourhood of a failing cell for recomputation in the
 // next pass (DetCell, mod_subroutine_MOOD.f90:165-169): each cell enters the
-// next pass's list once, deduplicated by the pass stamp.
-__device__ __forceinline__ void mark_neighbourhood(int i, int j, int NX, int NY, int stamp, int *mark, int *list,
-                                                   int *cnt)
+// next pass's list once, deduplicated by the pass stamp.  Called by every
+// lane of the wavefront (dec: this lane's cell fails).
+__device__ __forceinline__ void mark_neighbourhood(bool dec, int i, int j, int NX, int NY, int stamp, int *mark,
+                                                   int *list, int *cnt)
 {
     const int ni[5] = {i, i - 1, i, i + 1, i}, nj[5] = {j - 1, j, j, j, j + 1};
+    bool add[5];
+    int d[5];
 #pragma unroll
     for (int k = 0; k < 5; ++k) {
-        if (ni[k] < 0 || ni[k] >= NX || nj[k] < 0 || nj[k] >= NY) continue;
-        const int d = nj[k] * NX + ni[k];
-        if (atomicExch(&mark[d], stamp) != stamp) list[atomicAdd(cnt, 1)] = d;
+        d[k] = nj[k] * NX + ni[k];
+        add[k] = dec && !(ni[k] < 0 || ni[k] >= NX || nj[k] < 0 || nj[k] >= NY) && atomicExch(&mark[d[k]], stamp) != stamp;
     }
+    wave_append_n<5>(add, d, list, cnt);
 }
 
 template <int MORD, int NGP, int FLUX, bool GEN, int CMB>
@@ -342,8 +373,8 @@ __global__ __launch_bounds__(kStageThreads, stage_waves(MORD)) void k_stage(Stag
                                          dtg);
             // a failing cell takes the cascade at once
             a.gpo[gc] = pack_order(dec ? lower_order(me, a.cascade) : me, me, 0);
-            if (dec) mark_neighbourhood(gi, gj, NX, NY, stamp1, a.mark, a.slist1, a.cnt_s1);
         }
+        mark_neighbourhood(dec, gi, gj, NX, NY, stamp1, a.mark, a.slist1, a.cnt_s1);
         const unsigned long long m = __ballot(dec);
         if (__lane_id() == 0 && m) atomicAdd(a.cnt_f0, (int)__popcll(m));
     }
@@ -381,23 +412,21 @@ struct Scoped {
 
 // The next pass's list of a grid-wide pass: the 5-point neighbourhood of a
 // failing cell, each cell listed once (deduplicated by the pass stamp in the
-// global mark array; DetCell, mod_subroutine_MOOD.f90:165-169).
+// global mark array; DetCell, mod_subroutine_MOOD.f90:165-169).  Called by
+// every lane of the wavefront: `on` asks for neighbour k (0 bottom, 1 left,
+// 2 the cell, 3 right, 4 top) of cell (i, j); the new entries go to the list
+// with one atomic per wavefront.
 struct GridMark {
     int *mark, *list, *cnt;
     int stamp;
-    __device__ __forceinline__ void operator()(int i, int j, int NX, int NY) const
+    __device__ __forceinline__ void operator()(bool on, int i, int j, int k, int NX, int NY) const
     {
-        constexpr int SC = __HIP_MEMORY_SCOPE_AGENT;
-        const int ni[5] = {i, i - 1, i, i + 1, i}, nj[5] = {j - 1, j, j, j, j + 1};
-        int old[5];
-#pragma unroll
-        for (int k = 0; k < 5; ++k) {
-            const bool in = !(ni[k] < 0 || ni[k] >= NX || nj[k] < 0 || nj[k] >= NY);
-            old[k] = in ? Scoped<SC>::exch(&mark[nj[k] * NX + ni[k]], stamp) : stamp;
-        }
-#pragma unroll
-        for (int k = 0; k < 5; ++k)
-            if (old[k] != stamp) list[Scoped<SC>::add(cnt, 1)] = nj[k] * NX + ni[k];
+        const int ni = i + (k == 1 ? -1 : k == 3 ? 1 : 0), nj = j + (k == 0 ? -1 : k == 4 ? 1 : 0);
+        bool add[1] = {false};
+        int d[1] = {nj * NX + ni};
+        if (on && !(ni < 0 || ni >= NX || nj < 0 || nj >= NY))
+            add[0] = Scoped<__HIP_MEMORY_SCOPE_AGENT>::exch(&mark[d[0]], stamp) != stamp;
+        wave_append_n<1>(add, d, list, cnt);
     }
 };
 
@@ -653,20 +682,23 @@ struct MoodPass {
                 fxr[k] = __shfl(Lf[k], gbase + 2 * NGPP);
                 fyt[k] = __shfl(Lf[k], gbase + 3 * NGPP);
             }
+            bool dec = false;
             if (valid && gl == 0) {
                 double cand[4];
                 const double4 uin = lacc(HB, HB);
                 apply_fluxes(uin, fyb, fxl, fxr, fyt, dtdx, dtdy, cand);
-                bool dec = false;
                 if (me > 1) dec = decide_with_flags(P, lacc, HB, HB, cand, fl);
                 write_stage_output<GEN, kCmbRuntime>(a.C, a.Uout, d, GEN && a.C.band && in_band(i, j, NX, NY), un, uin,
                                                      cand, dtg);
-                if (dec) {
-                    M::store(&a.gpo[d], pack_order(lower_order(me, a.cascade), me, p));
-                    M::add(cnt_fail, 1);
-                    mark(i, j, NX, NY);
-                }
+                if (dec) M::store(&a.gpo[d], pack_order(lower_order(me, a.cascade), me, p));
             }
+            // the failures, one count per wavefront; a failing cell's 5-point
+            // neighbourhood enters list p + 1, one neighbour per lane of its group
+            const unsigned long long fm = __ballot(dec);
+            if (fm && lane == __ffsll((long long)fm) - 1) M::add(cnt_fail, (int)__popcll(fm));
+            const bool decg = __shfl(dec ? 1 : 0, gbase) != 0;
+#pragma unroll
+            for (int k0 = 0; k0 < 5; k0 += G) mark(decg && k0 + gl < 5, i, j, k0 + gl, NX, NY);
         }
     }
 
