@@ -56,6 +56,10 @@ struct gpm_ctx {
     int tiles_x = 0, ntiles = 0, dt_blocks = 0, mood_blocks = 0, mood_solo = 0;
     bool mood_coop = true;  // MOOD passes as one cooperative launch per stage (else per-pass launches)
     bool mood_fc = true;    // workgroup 0's MOOD passes reuse face fluxes and compute likely next ones ahead
+    // wide launches of long-list passes ahead of the cooperative kernel: launches per stage, grid, shortest list
+    int mood_wide = 0, mood_wide_blocks = 0, mood_wide_min = 0;
+    bool mood_wide_on = true;  // the last call's first lists were long (or there was no call yet)
+    int long_list_h = 0;       // host copy of DevStats::long_list
     FluxTable gfc{};        // device-wide face-flux cache that the other workgroups fill meanwhile
     bool has_state = false;
     bool timing = false;
@@ -416,6 +420,51 @@ int enqueue_ring_fill(gpm_ctx *c)
     return GPM_OK;
 }
 
+// Wide launches of the MOOD passes over long lists (k_mood_wide), ahead of the
+// cooperative kernel of each stage: as many workgroups as the GPU holds at
+// once, for lists longer than one trip of the cooperative grid.
+// GPM_MOOD_WIDE sets the launches per stage (0 switches them off),
+// GPM_MOOD_WIDE_MIN the shortest list they take.
+void setup_mood_wide(gpm_ctx *c, int ncu, int flux)
+{
+    c->mood_wide = 0;
+    // FOG runs no MOOD pass (its stage kernel is the first-order one)
+    const void *k = c->fog || c->nodetect ? nullptr : mood_wide_kernel(c->mord, c->ngp, flux, c->ring_mode);
+    if (!k) return;
+    int per = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k, kWideThreads, 0) != hipSuccess) {
+        (void)hipGetLastError();
+        per = 1;
+    }
+    if (per <= 0) per = 1;
+    c->mood_wide_blocks = per * ncu;
+    const int ngpp = c->ngp <= 1 ? 1 : c->ngp <= 2 ? 2 : c->ngp <= 4 ? 4 : 8;
+    c->mood_wide_min = c->mood_blocks * (kMoodThreads / (4 * ngpp));
+    c->mood_wide = 2;
+    if (const char *env = std::getenv("GPM_MOOD_WIDE")) c->mood_wide = std::atoi(env) > 0 ? std::atoi(env) : 0;
+    if (const char *env = std::getenv("GPM_MOOD_WIDE_MIN")) c->mood_wide_min = std::atoi(env) > 0 ? std::atoi(env) : 0;
+}
+
+// An idle wide launch still costs a few microseconds per stage, so a context
+// enqueues them only while its lists are long: the device records the longest
+// first or second correction list (DevStats::long_list), a call that ends
+// synchronised reads it (mood_wide_read before the synchronisation,
+// mood_wide_decide after it), and the next call follows it.  Results do not
+// depend on the launch plan.
+int mood_wide_read(gpm_ctx *c)
+{
+    if (c->mood_wide <= 0 || !c->mood_coop) return GPM_OK;
+    HIPCHK(c, hipMemcpyAsync(&c->long_list_h, &c->stats->long_list, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    return GPM_OK;
+}
+int mood_wide_decide(gpm_ctx *c)
+{
+    if (c->mood_wide <= 0 || !c->mood_coop) return GPM_OK;
+    c->mood_wide_on = c->long_list_h > c->mood_wide_min;
+    HIPCHK(c, hipMemsetAsync(&c->stats->long_list, 0, sizeof(int), c->stream));
+    return GPM_OK;
+}
+
 // The MOOD correction passes of one stage.  Cooperative: one launch runs every
 // pass.  Otherwise (when asked for, or when the cooperative launch is
 // refused, e.g. while another process holds the device's cooperative-launch
@@ -428,6 +477,14 @@ int enqueue_mood(gpm_ctx *c, MoodArgs &m)
     const int mord = c->mord, ngp = c->ngp, flux = c->cfg.num_flux;
     if (c->mood_coop) {
         void *args[] = {&m};
+        if (c->mood_wide > 0 && c->mood_wide_on) {
+            // the first passes over long lists in ordinary launches of the whole
+            // GPU; each returns at once when its pass lists few cells
+            m.wide_min = c->mood_wide_min;
+            const void *wk = mood_wide_kernel(mord, ngp, flux, c->ring_mode);
+            for (int w = 0; w < c->mood_wide; ++w)
+                HIPCHK(c, hipLaunchKernel(wk, dim3(c->mood_wide_blocks), dim3(kWideThreads), args, 0, c->stream));
+        }
         const hipError_t e = hipLaunchCooperativeKernel(mood_kernel(mord, ngp, flux, c->ring_mode, true),
                                                         dim3(c->mood_blocks), dim3(kMoodThreads), args, 0, c->stream);
         if (e == hipSuccess) return GPM_OK;
@@ -779,6 +836,7 @@ int gpm_create(const gpm_config *cfg, gpm_ctx **out)
         if (e != hipSuccess || per <= 0) per = 1;
         c->mood_blocks = want < per * ncu ? want : per * ncu;
     }
+    setup_mood_wide(c, ncu, c->cfg.num_flux);
     // GPM_MOOD_COOP=0 selects the per-pass launches of the MOOD passes.
     if (const char *env = std::getenv("GPM_MOOD_COOP")) c->mood_coop = std::atoi(env) != 0;
     // Passes over at most this many cells run on one workgroup (GPM_MOOD_SOLO overrides; 0 disables).
@@ -995,7 +1053,11 @@ int gpm_time_stepping(gpm_ctx *c, double *Uin, double *Uout, double t, double dt
     if (rc) return rc;
     rc = collect_timing(c);
     if (rc) return rc;
+    rc = mood_wide_read(c);
+    if (rc) return rc;
     rc = gpm_get_state(c, Uout, cellgpo);
+    if (rc) return rc;
+    rc = mood_wide_decide(c);
     if (rc) return rc;
     if (stats) {
         rc = gpm_get_stats(c, stats);
@@ -1031,9 +1093,11 @@ int gpm_run(gpm_ctx *c, int32_t nsteps, double tmax, double *dts)
     if (rc) return rc;
     if (dts && nsteps > 0)
         HIPCHK(c, hipMemcpyAsync(dts, c->dt_log, nsteps * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    rc = mood_wide_read(c);
+    if (rc) return rc;
     // synchronous: the steps are done when gpm_run returns
     HIPCHK(c, hipStreamSynchronize(c->stream));
-    return GPM_OK;
+    return mood_wide_decide(c);
 }
 
 int gpm_get_stats(gpm_ctx *c, gpm_stats *st)
@@ -1608,6 +1672,7 @@ int gpm_run_noalloc(const gpm_config *cfg, const double *zT_sp, const double *gp
         c.mood_blocks = want < per * ncu ? want : per * ncu;
     }
     c.mood_solo = kSoloMax;
+    setup_mood_wide(&c, ncu, cfg->num_flux);
     if (const char *env = std::getenv("GPM_MOOD_COOP")) c.mood_coop = std::atoi(env) != 0;
     c.has_state = true;
 

@@ -22,6 +22,14 @@ const void *mood_kernel(int mord, int ngp, int flux, bool gen, bool cooperative)
     return nullptr;
 }
 
+const void *mood_wide_kernel(int mord, int ngp, int flux, bool gen)
+{
+#define X(M, G) if (mord == M && ngp == G) return mood_wide_kernel_mg<M, G>(flux, gen);
+    GPM_COMBOS(X)
+#undef X
+    return nullptr;
+}
+
 hipError_t launch_ring_face(int mord, int ngp, int flux, const RingArgs &a, hipStream_t s)
 {
 #define X(M, G) if (mord == M && ngp == G) return launch_ring_face_mg<M, G>(flux, a, s);

@@ -39,6 +39,8 @@ constexpr int kStageThreads = 256;
 #define GPM_MOOD_THREADS 512
 #endif
 constexpr int kMoodThreads = GPM_MOOD_THREADS;
+// workgroup size of the wide launch of long-list MOOD passes (k_mood_wide)
+constexpr int kWideThreads = 256;
 constexpr int kSoloMax = 32;  // largest list a single workgroup takes over
 // Halo width: the stencil radius R = (Mord-1)/2 plus one cell for the face
 // neighbour, and at least 2 for the detection (U2 reads a cell's 5-point
@@ -77,6 +79,9 @@ struct DevStats {
     long long nn_passes_step;  // passes of the last step whose check lowered a cell
                                // (increments of count_steps_NN_produced_NAN, mod_FE.f90:154-157)
     long long nn_passes_total;
+    // the longest list of a stage's first two correction passes since the
+    // host last read it (the lists the wide launches take)
+    int long_list;
 };
 
 constexpr int kMaxStages = 5;
@@ -171,6 +176,7 @@ struct MoodArgs {
     int dmp, u2, u2tol, cascade, stage_idx;
     int nn_passes;  // count the passes that lowered a cell (a step of the reference network)
     int solo_max;   // lists of at most this many cells run on workgroup 0 alone (0: never)
+    int wide_min;   // a wide launch runs the current pass only when it lists more cells than this
     int fcache;     // workgroup 0's passes reuse face fluxes and compute likely next ones ahead
     FluxTable gfc;  // while workgroup 0 runs passes alone, the other workgroups fill it (cooperative kernel)
     const float *__restrict__ vnn;  // as StageArgs::vnn
@@ -256,6 +262,8 @@ template <int MORD, int NGP>
 hipError_t launch_stage_mg(int flux, bool gen, const StageArgs &a, int nblocks, hipStream_t s);
 template <int MORD, int NGP>
 const void *mood_kernel_mg(int flux, bool gen, bool cooperative);
+template <int MORD, int NGP>
+const void *mood_wide_kernel_mg(int flux, bool gen);
 
 // flux: numFlux id (kHLLC, kLLF, kHLL, kUnsplit); gen: the general boundary
 // kernels (formula boundaries, or problem RT's source term) instead of the
@@ -266,6 +274,8 @@ hipError_t launch_stage(int mord, int ngp, int flux, bool gen, const StageArgs &
 // one run of workgroup-0 passes, per launch; launches after the last pass
 // return at once).  Both take one MoodArgs.
 const void *mood_kernel(int mord, int ngp, int flux, bool gen, bool cooperative);
+// the wide launch of long-list MOOD passes (k_mood_wide)
+const void *mood_wide_kernel(int mord, int ngp, int flux, bool gen);
 // The stage's MOOD statistics after per-launch passes (the cooperative kernel
 // records them itself).
 hipError_t launch_mood_finish(const MoodArgs &a, hipStream_t s);

@@ -523,7 +523,11 @@ __device__ __forceinline__ void wave_lds_sync()
 // depend on the staged neighbourhood (the face's two order words, the cell's
 // own order word and step-start state, the next trip's list entry) are
 // issued together with the neighbourhood loads.
-template <int MORD, int NGP, int FLUX, bool GEN>
+//
+// WIDE selects the form of the wide launch (k_mood_wide): grid-wide passes
+// only, in workgroups of kWideThreads threads whose LDS holds the tables and
+// the staged neighbourhoods alone, so that several workgroups share a CU.
+template <int MORD, int NGP, int FLUX, bool GEN, bool WIDE = false>
 struct MoodPass {
     static constexpr int NGPP = NGP <= 1 ? 1 : NGP <= 2 ? 2 : NGP <= 4 ? 4 : 8;
     static constexpr int G = 4 * NGPP;
@@ -534,12 +538,17 @@ struct MoodPass {
     // neighbours and the detection's 2-cell reach.
     static constexpr int HB = kHalo<MORD>;
     static constexpr int BOX = 2 * HB + 1;
-    static constexpr int CPB = kMoodThreads / G;  // cells per workgroup per trip
+    static constexpr int NTH = WIDE ? kWideThreads : kMoodThreads;
+    static constexpr int CPB = NTH / G;  // cells per workgroup per trip
     // speculative jobs of one workgroup-0 pass: for each of the first 16 listed
     // cells, the cell at its next lower order and its four neighbours
     static constexpr int FCN = MORD == 1 ? 1 : kFcSlots;
     static constexpr int PW = 64 / G;  // jobs per wavefront
-    struct Lds {
+    struct LdsWide {
+        DevTables T;
+        double4 nb[CPB][BOX * BOX];
+    };
+    struct LdsFull {
         DevTables T;  // the GP tables (the reconstruction's weights are read per lane)
         double4 nb[CPB][BOX * BOX];
         // face-flux cache of the workgroup-0 passes (none at first order, where
@@ -563,6 +572,7 @@ struct MoodPass {
         // what they tell the speculative wavefronts
         int ebar_cnt, ebar_gen, spass, sdone, qend;
     };
+    using Lds = typename std::conditional<WIDE, LdsWide, LdsFull>::type;
 
     const MoodArgs &a;
     Lds &L;
@@ -608,7 +618,8 @@ struct MoodPass {
         const double *src = reinterpret_cast<const double *>(a.T);
         double *dst = reinterpret_cast<double *>(&L.T);
         for (int k = threadIdx.x; k < (int)(sizeof(DevTables) / sizeof(double)); k += blockDim.x) dst[k] = src[k];
-        for (int k = threadIdx.x; k < FCN; k += blockDim.x) L.ctag[k] = kFcEmpty;
+        if constexpr (!WIDE)
+            for (int k = threadIdx.x; k < FCN; k += blockDim.x) L.ctag[k] = kFcEmpty;
         __syncthreads();
     }
 
@@ -1193,8 +1204,9 @@ __device__ __forceinline__ void mood_finish(const MoodArgs &a, int p)
     }
 }
 
-// MOOD correction passes 1, 2, ... of one RK stage as a cooperative
-// persistent kernel with one grid barrier per pass; the loop ends on the
+// MOOD correction passes of one RK stage as a cooperative persistent kernel
+// with one grid barrier per pass, from the first pass the wide launches
+// before it left (kCntPass + 1; pass 1 when none ran); the loop ends on the
 // device when a pass finds no cell to recompute.
 template <int MORD, int NGP, int FLUX, bool GEN>
 __global__ __launch_bounds__(kMoodThreads) void k_mood(MoodArgs a)
@@ -1207,12 +1219,13 @@ __global__ __launch_bounds__(kMoodThreads) void k_mood(MoodArgs a)
     const int ncell = a.B.NX * a.B.NY;
     const int gtid = blockIdx.x * blockDim.x + threadIdx.x;
     const int gsz = gridDim.x * blockDim.x;
-    int p = 1;
+    int p = load_relaxed(&a.cnt[kCntPass]) + 1;
     for (;;) {
         // the pass's count and this lane's first list entry, in one round trip
         const int nS = load_relaxed(&a.cnt[kCntS0 + p % 3]);
         const int d0 = gtid / MP::G < ncell ? a.slist[p & 1][gtid / MP::G] : 0;
         if (nS == 0) break;
+        if (gtid == 0 && p <= 2) atomicMax(&a.stats->long_list, nS);
         if (nS <= a.solo_max) {
             // workgroup 0 runs the short-list passes; the others fill the
             // device-wide face-flux cache meanwhile, then wait here
@@ -1256,6 +1269,52 @@ __global__ __launch_bounds__(kMoodThreads) void k_mood_pass(MoodArgs a)
     if (nS <= a.solo_max) {
         if (blockIdx.x == 0) mp.solo(p, nS);
     } else {
+        const int d0 = gtid / MP::G < ncell ? a.slist[p & 1][gtid / MP::G] : 0;
+        mp.grid_pass(p, nS, d0, gtid, gridDim.x * blockDim.x);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        __threadfence();
+        s_last = atomicAdd(&a.cnt[kCntDone], 1) == (int)gridDim.x - 1;
+    }
+    __syncthreads();
+    if (!s_last || threadIdx.x != 0) return;
+    __threadfence();
+    const int q = load_relaxed(&a.cnt[kCntNext]);
+    a.cnt[kCntPass] = q - 1;
+    a.cnt[kCntPending] = load_relaxed(&a.cnt[kCntS0 + q % 3]) > 0;
+    a.cnt[kCntDone] = 0;
+}
+
+// A grid-wide pass over a long list in an ordinary launch of many small
+// workgroups.  The cooperative kernel's grid is kept small, because its
+// per-pass grid barrier costs more with every workgroup, so it covers a list
+// of thousands of cells in many trips, one after another.  Launched before
+// it: the launch runs the current pass (kCntPass + 1) only when that pass
+// lists more than wide_min cells, and then publishes the next pass as the
+// per-pass launches do; otherwise every workgroup returns at once (all read
+// the same counters, which no workgroup of the launch changes).
+template <int MORD, int NGP, int FLUX, bool GEN>
+__global__ __launch_bounds__(kWideThreads) void k_mood_wide(MoodArgs a)
+{
+    using MP = MoodPass<MORD, NGP, FLUX, GEN, true>;
+    __shared__ typename MP::Lds lds;
+    __shared__ int s_p, s_n;
+    __shared__ bool s_last;
+    if (threadIdx.x == 0) {
+        s_p = load_relaxed(&a.cnt[kCntPass]) + 1;
+        s_n = load_relaxed(&a.cnt[kCntS0 + s_p % 3]);
+    }
+    __syncthreads();
+    const int p = s_p, nS = s_n;
+    if (blockIdx.x == 0 && threadIdx.x == 0 && p <= 2) atomicMax(&a.stats->long_list, nS);
+    if (nS <= a.wide_min) return;
+    // a workgroup whose first trip has no cell has none in any trip
+    if ((int)blockIdx.x * MP::CPB < nS) {
+        MP mp(a, lds);
+        mp.init_lds();
+        const int ncell = a.B.NX * a.B.NY;
+        const int gtid = blockIdx.x * blockDim.x + threadIdx.x;
         const int d0 = gtid / MP::G < ncell ? a.slist[p & 1][gtid / MP::G] : 0;
         mp.grid_pass(p, nS, d0, gtid, gridDim.x * blockDim.x);
     }
@@ -1380,6 +1439,20 @@ __global__ __launch_bounds__(128) void k_ring_face(RingArgs a)
                        : reinterpret_cast<const void *>(&k_mood<M, G, F, false>);                            \
         return gen ? reinterpret_cast<const void *>(&k_mood_pass<M, G, F, true>)                             \
                    : reinterpret_cast<const void *>(&k_mood_pass<M, G, F, false>);                           \
+    }                                                                                                         \
+    template <int F>                                                                                          \
+    static const void *mood_wide_f##M##G(bool gen)                                                            \
+    {                                                                                                         \
+        return gen ? reinterpret_cast<const void *>(&k_mood_wide<M, G, F, true>)                             \
+                   : reinterpret_cast<const void *>(&k_mood_wide<M, G, F, false>);                           \
+    }                                                                                                         \
+    template <>                                                                                               \
+    const void *mood_wide_kernel_mg<M, G>(int flux, bool gen)                                                 \
+    {                                                                                                         \
+        if (flux == kHLL) return mood_wide_f##M##G<kHLL>(gen);                                               \
+        if (flux == kLLF) return mood_wide_f##M##G<kLLF>(gen);                                               \
+        if (flux == kUnsplit) return mood_wide_f##M##G<kUnsplit>(gen);                                       \
+        return mood_wide_f##M##G<kHLLC>(gen);                                                                \
     }                                                                                                         \
     template <>                                                                                               \
     hipError_t launch_stage_mg<M, G>(int flux, bool gen, const StageArgs &a, int nblocks, hipStream_t s)     \

@@ -62,10 +62,13 @@ def test_8192_shear_layer_plain_and_nn(gpm):
 
 def test_8192_four_quadrant_r2_long_chains(gpm):
     """8192^2 GP-R2 four-quadrant problem (outflow): MOOD failure chains run
-    thousands of passes in a stage, past any fixed pass budget.  Three steps
-    with the cooperative MOOD kernel and with per-pass launches give the same
-    bits; the state stays finite with positive density and pressure; the
-    largest pass count of a stage is reported."""
+    thousands of passes in a stage, past any fixed pass budget, and the first
+    correction lists hold hundreds of thousands of cells.  Three steps with
+    the cooperative MOOD kernel (its first long-list passes in wide launches,
+    the default), with the cooperative kernel alone (GPM_MOOD_WIDE=0) and
+    with per-pass launches give the same bits; the state stays finite with
+    positive density and pressure; the largest pass count of a stage is
+    reported."""
     if not has_gpu():
         pytest.skip("no GPU")
     import torch
@@ -74,8 +77,16 @@ def test_8192_four_quadrant_r2_long_chains(gpm):
     print(f"[case 8192^2 GP-R2 four-quadrant, {steps} steps, both MOOD launch forms]", flush=True)
     U0 = torch.from_numpy(gpm.ic.make("four_quadrant", n, 0)).cuda()
     out = {}
-    for coop in (True, False):
-        s = gpm.Solver(gpm.Config(lf=n, Mord=5, ngp=3, BC_type="neumann"))
+    for name, coop, wide in (("cooperative", True, "2"), ("cooperative alone", True, "0"), ("per-pass", False, "2")):
+        old = os.environ.get("GPM_MOOD_WIDE")
+        os.environ["GPM_MOOD_WIDE"] = wide
+        try:
+            s = gpm.Solver(gpm.Config(lf=n, Mord=5, ngp=3, BC_type="neumann"))
+        finally:
+            if old is None:
+                del os.environ["GPM_MOOD_WIDE"]
+            else:
+                os.environ["GPM_MOOD_WIDE"] = old
         s.set_mood_launch(coop)
         torch.cuda.synchronize()
         s.set_state_device(U0.data_ptr())
@@ -84,15 +95,18 @@ def test_8192_four_quadrant_r2_long_chains(gpm):
         s.get_state_device(U.data_ptr())
         torch.cuda.synchronize()
         st = s.stats()
-        print(f"  cooperative={coop}: max_passes {st['max_passes']}, detected {st['detected_total']}, "
+        print(f"  {name}: max_passes {st['max_passes']}, detected {st['detected_total']}, "
               f"second-pass cells {st['second_pass_cells']}", flush=True)
         assert s.mood_launch() == coop
-        out[coop] = (dts, U, st)
+        out[name] = (dts, U, st)
         s.close()
-    (d1, U1, s1), (d2, U2, s2) = out[True], out[False]
-    assert np.array_equal(d1, d2)
-    assert bool(torch.equal(U1, U2))
-    assert s1["detected_total"] == s2["detected_total"] and s1["max_passes"] == s2["max_passes"]
+    d1, U1, s1 = out["cooperative"]
+    for name in ("cooperative alone", "per-pass"):
+        d2, U2, s2 = out[name]
+        assert np.array_equal(d1, d2), name
+        assert bool(torch.equal(U1, U2)), name
+        for k in ("detected_total", "second_pass_cells", "max_passes"):
+            assert s1[k] == s2[k], (name, k)
     assert bool(torch.isfinite(U1).all())
     r = U1[..., 0]
     p = 0.4 * (U1[..., 3] - 0.5 * (U1[..., 1] ** 2 + U1[..., 2] ** 2) / r)

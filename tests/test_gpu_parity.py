@@ -117,6 +117,31 @@ def test_run_per_pass_launches(gpm, case):
     assert s.stats()["detected_total"] == int(np.sum(z["counts"]))
 
 
+@pytest.mark.parametrize("case", STEP_CASES)
+def test_run_wide_launches(gpm, case, monkeypatch):
+    """Every stage's first two MOOD correction passes through the wide
+    launches (GPM_MOOD_WIDE_MIN=1: any list of more than one cell; by default
+    only lists longer than one trip of the cooperative grid go there), the
+    rest in the cooperative kernel: the same dt sequence, counts, mask and
+    state as the reference run."""
+    z = golden(f"step_{case}.npz")
+    nsteps = int(z["nsteps"])
+    print(f"[case {case}] {nsteps} steps via gpm_run, wide MOOD launches for every list", flush=True)
+    monkeypatch.setenv("GPM_MOOD_WIDE", "2")
+    monkeypatch.setenv("GPM_MOOD_WIDE_MIN", "1")
+    s = make_solver(gpm, z)
+    s.set_state(z["U0"])
+    s.set_time(0.0)
+    dts = s.run(nsteps, return_dts=True)
+    U, g = s.get_state(order=True)
+    assert np.array_equal(dts, z["dt"]), f"dt differs at steps {np.nonzero(dts != z['dt'])[0][:5]}"
+    assert np.array_equal(g[4:-4, 4:-4], z["masks"][-1])
+    assert np.array_equal(U, z["U"]), f"max |diff| {np.abs(U - z['U']).max()}"
+    st = s.stats()
+    assert st["steps"] == nsteps
+    assert st["detected_total"] == int(np.sum(z["counts"]))
+
+
 def test_config0_256_300_steps_checksum(gpm):
     """Config-0 workload (256^2, R2/ngp3, Neumann, 300 steps, seeded
     four-quadrant IC): final state and mask checksums equal the reference's."""
