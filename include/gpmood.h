@@ -241,6 +241,12 @@ void *gpm_stream(gpm_ctx *ctx);
  * The library switches to per-pass launches by itself when the device
  * refuses a cooperative launch (one process at a time holds that resource),
  * and the environment variable GPM_MOOD_COOP=0 selects them at gpm_create.
+ * With cooperative launches, a stage's first passes over long lists (more
+ * cells than one trip of the cooperative grid) run in two ordinary launches
+ * of the whole GPU ahead of the cooperative one, while the context's lists
+ * are long: each gpm_run or gpm_time_stepping call sets the next call's
+ * plan from the longest first list it saw (GPM_MOOD_WIDE=0 at gpm_create
+ * switches them off).  The plan changes no result.
  * No counterpart in the reference (its MOOD loop is a host loop). */
 int gpm_set_mood_launch(gpm_ctx *ctx, int32_t cooperative);
 int gpm_get_mood_launch(gpm_ctx *ctx, int32_t *cooperative);

@@ -229,7 +229,7 @@ def test_trained_predictor_gains_over_plain(gpm):
     fewer cells go to a second MOOD pass and fewer cells fail the
     a posteriori check, and the density stays within 1e-3 (relative L1) of
     plain GP-MOOD.  The step times are printed, not asserted: on such states
-    the gate and the network cost about what the saved MOOD passes do."""
+    the gate and the network cost more than the MOOD passes they save."""
     import time
 
     import torch
