@@ -207,8 +207,11 @@ class Solver:
     def reset_stats(self):
         self._check(self.lib.gpm_reset_stats(self._h), "gpm_reset_stats")
 
-    def set_timing(self, on=True):
-        self._check(self.lib.gpm_set_timing(self._h, int(on)), "gpm_set_timing")
+    def set_timing(self, on=True, every=1):
+        """HIP-event timing of the stage and MOOD launches (kernel_time) of
+        every step, or of every `every`-th step (a sample, with less event
+        overhead on the stream)."""
+        self._check(self.lib.gpm_set_timing(self._h, int(every) if on else 0), "gpm_set_timing")
 
     def kernel_time(self):
         a, b = ctypes.c_double(), ctypes.c_double()

@@ -63,6 +63,8 @@ struct gpm_ctx {
     FluxTable gfc{};        // device-wide face-flux cache that the other workgroups fill meanwhile
     bool has_state = false;
     bool timing = false;
+    int timing_every = 1;     // HIP events around the kernels of every timing_every-th step
+    long long timing_step = 0;
     double stage_ms = 0, mood_ms = 0;
     long long stage_launches = 0, mood_launches = 0;
     std::vector<hipEvent_t> ev_pool;
@@ -523,6 +525,8 @@ int enqueue_stages(gpm_ctx *c)
     StagePlan plan[kMaxStages];
     const int nst = plan_stages(c, plan);
     ++c->niter;
+    // the steps timed with HIP events (every step, or a sample)
+    const bool timed = c->timing && c->timing_step++ % c->timing_every == 0;
     // the reference network's steps: a priori orders from the network and the
     // PAD-only check NN_DETECTION (mod_FE.f90:51-56,151-158)
     const bool refnn = c->refnn_w && c->niter > kStepsWithNoNN;
@@ -561,7 +565,7 @@ int enqueue_stages(gpm_ctx *c)
             }
         }
         hipEvent_t e0 = nullptr, e1 = nullptr;
-        if (c->timing) {
+        if (timed) {
             size_t i0 = c->ev_used;
             e0 = next_event(c); e1 = next_event(c);
             if (!e0 || !e1) return fail(c, GPM_EHIP, "hipEventCreate failed");
@@ -572,7 +576,7 @@ int enqueue_stages(gpm_ctx *c)
         // FOG runs the first-order kernel (Mord 1 template, one Gauss point)
         HIPCHK(c, launch_stage(c->fog ? 1 : c->mord, c->fog ? 1 : c->ngp, c->cfg.num_flux, c->ring_mode, s,
                                (c->ntiles + 7) / 8 * 8, c->stream));
-        if (c->timing) HIPCHK(c, hipEventRecord(e1, c->stream));
+        if (timed) HIPCHK(c, hipEventRecord(e1, c->stream));
         if (c->nodetect) continue;  // one pass, no detection (mod_FE.f90:160-162)
 
         MoodArgs m{};
@@ -590,7 +594,7 @@ int enqueue_stages(gpm_ctx *c)
             m.nn_passes = 1;
             if (c->refnn_cc) m.vnn = c->vnn;  // m.recon is kReconCC from s
         }
-        if (c->timing) {
+        if (timed) {
             size_t i0 = c->ev_used;
             e0 = next_event(c); e1 = next_event(c);
             if (!e0 || !e1) return fail(c, GPM_EHIP, "hipEventCreate failed");
@@ -599,7 +603,7 @@ int enqueue_stages(gpm_ctx *c)
         }
         int rc = enqueue_mood(c, m);
         if (rc) return rc;
-        if (c->timing) HIPCHK(c, hipEventRecord(e1, c->stream));
+        if (timed) HIPCHK(c, hipEventRecord(e1, c->stream));
         if (c->rec_masks && c->rec_count < c->rec_cap) {
             const size_t n = (size_t)c->NX * c->NY;
             HIPCHK(c, launch_unpack_orders(c->gpo, c->rec_masks + ((size_t)c->rec_count * 3 + k) * n, n, c->stream));
@@ -1154,9 +1158,11 @@ int gpm_get_mood_launch(gpm_ctx *c, int32_t *cooperative)
 
 int gpm_set_timing(gpm_ctx *c, int32_t enable)
 {
-    if (!c) return GPM_EINVAL;
+    if (!c || enable < 0) return GPM_EINVAL;
     int rc = collect_timing(c);
     c->timing = enable != 0;
+    c->timing_every = enable > 1 ? enable : 1;
+    c->timing_step = 0;
     return rc;
 }
 
@@ -1672,7 +1678,9 @@ int gpm_run_noalloc(const gpm_config *cfg, const double *zT_sp, const double *gp
         c.mood_blocks = want < per * ncu ? want : per * ncu;
     }
     c.mood_solo = kSoloMax;
-    setup_mood_wide(&c, ncu, cfg->num_flux);
+    // no earlier call to choose the wide launches' plan from: the
+    // cooperative kernel alone, as for short lists
+    c.mood_wide = 0;
     if (const char *env = std::getenv("GPM_MOOD_COOP")) c.mood_coop = std::atoi(env) != 0;
     c.has_state = true;
 

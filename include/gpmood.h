@@ -253,7 +253,10 @@ int gpm_get_mood_launch(gpm_ctx *ctx, int32_t *cooperative);
 
 /* Launch shape and timing of the dominant kernel, for roofline reporting:
  * average duration in ms of the first-pass stage kernel over the launches
- * since the last gpm_reset_stats (0 if timing is off), and launch count. */
+ * since the last gpm_reset_stats (0 if timing is off), and launch count.
+ * gpm_set_timing: 0 off; 1 HIP events around the stage and MOOD launches of
+ * every step; k > 1 around those of every k-th step from now on (a sample:
+ * each event pair costs the stream a few microseconds). */
 int gpm_set_timing(gpm_ctx *ctx, int32_t enable);
 int gpm_kernel_time(gpm_ctx *ctx, double *avg_stage_ms, int64_t *stage_launches,
                     double *avg_mood_ms, int64_t *mood_launches);

@@ -287,3 +287,27 @@ def test_face_flux_cache_changes_no_bit(gpm, mord, ngp, cascade, bc):
         assert np.array_equal(g, ref[2]), key
         for k in ("detected_total", "second_pass_cells", "max_passes"):
             assert st[k] == ref[3][k], (key, k)
+
+
+def test_kernel_timing_every_kth_step(gpm):
+    """gpm_set_timing: events around the stage and MOOD launches of every step
+    (1) or of every k-th step (k), none when off; the averages are positive
+    and the results do not depend on the timing."""
+    n = 128
+    print(f"[case kernel timing {n}^2]", flush=True)
+    U0 = gpm.ic.four_quadrant(n, 4)
+    out = {}
+    for every in (0, 1, 3):
+        s = gpm.Solver(gpm.Config(lf=n, Mord=5, ngp=3))
+        s.set_state(U0)
+        s.set_timing(every > 0, every=max(every, 1))
+        dts = s.run(7, return_dts=True)
+        kt = s.kernel_time()
+        out[every] = (dts, s.get_state(), kt)
+        s.close()
+    assert out[0][2]["stage_launches"] == 0 and out[0][2]["mood_launches"] == 0
+    assert out[1][2]["stage_launches"] == 21 and out[1][2]["mood_launches"] == 21
+    assert out[3][2]["stage_launches"] == 9 and out[3][2]["mood_launches"] == 9  # steps 0, 3, 6
+    for k in (1, 3):
+        assert out[k][2]["stage_ms"] > 0 and out[k][2]["mood_ms"] > 0
+        assert np.array_equal(out[k][0], out[0][0]) and np.array_equal(out[k][1], out[0][1])
