@@ -580,22 +580,24 @@ __device__ __forceinline__ void riemann(const double ul[4], const double ur[4], 
 // Run-time direction: a y-face solve is the x-solve on momentum-swapped states
 // with the flux's momenta swapped back (exact for all four solvers: their
 // y-direction formulas are the x-direction ones with the roles of the
-// momenta exchanged, operation for operation).
+// momenta exchanged, operation for operation).  One x-solve serves both
+// directions, with the swaps as selects: the lanes of a MOOD wavefront hold
+// x- and y-faces together, and a branch per direction made every wavefront
+// run the solver twice.  For an x-face the selects pass the states through,
+// so the operations are those of riemann<FLUX, 0>.
 template <int FLUX>
 __device__ __forceinline__ void riemann_dir(int dir, const double ul[4], const double ur[4], double F[4])
 {
-    if (dir == 0) {
-        riemann<FLUX, 0>(ul, ur, F);
-        return;
-    }
-    const double sl[4] = {ul[0], ul[2], ul[1], ul[3]}, sr[4] = {ur[0], ur[2], ur[1], ur[3]};
+    const bool y = dir != 0;
+    const double sl[4] = {ul[0], y ? ul[2] : ul[1], y ? ul[1] : ul[2], ul[3]};
+    const double sr[4] = {ur[0], y ? ur[2] : ur[1], y ? ur[1] : ur[2], ur[3]};
     double G[4];
     const bool ok = FLUX == kHLL       ? hll_adm<0>(sl, sr, G)
                     : FLUX == kLLF     ? llf_adm<0>(sl, sr, G)
                     : FLUX == kUnsplit ? unsplit_adm<0>(sl, sr, G)
                                        : hllc_adm<0>(sl, sr, G);
     if (ok) {
-        F[0] = G[0]; F[1] = G[2]; F[2] = G[1]; F[3] = G[3];
+        F[0] = G[0]; F[1] = y ? G[2] : G[1]; F[2] = y ? G[1] : G[2]; F[3] = G[3];
     } else if (FLUX == kUnsplit) {
         unsplit_inadmissible(ur, F);
     } else {
