@@ -544,9 +544,14 @@ struct MoodPass {
     // cells, the cell at its next lower order and its four neighbours
     static constexpr int FCN = MORD == 1 ? 1 : kFcSlots;
     static constexpr int PW = 64 / G;  // jobs per wavefront
+    static constexpr int KK = (5 + G - 1) / G;  // neighbours of a failing cell per lane
     struct LdsWide {
         DevTables T;
         double4 nb[CPB][BOX * BOX];
+        // a trip's new list entries and failures, gathered over the workgroup
+        // (double-buffered by trip parity) before one device atomic each
+        int wbuf[2][5 * CPB];
+        int wcnt[2], wfail[2], wbase;
     };
     struct LdsFull {
         DevTables T;  // the GP tables (the reconstruction's weights are read per lane)
@@ -618,8 +623,12 @@ struct MoodPass {
         const double *src = reinterpret_cast<const double *>(a.T);
         double *dst = reinterpret_cast<double *>(&L.T);
         for (int k = threadIdx.x; k < (int)(sizeof(DevTables) / sizeof(double)); k += blockDim.x) dst[k] = src[k];
-        if constexpr (!WIDE)
+        if constexpr (!WIDE) {
             for (int k = threadIdx.x; k < FCN; k += blockDim.x) L.ctag[k] = kFcEmpty;
+        } else if (threadIdx.x < 2) {
+            L.wcnt[threadIdx.x] = 0;
+            L.wfail[threadIdx.x] = 0;
+        }
         __syncthreads();
     }
 
@@ -706,11 +715,57 @@ struct MoodPass {
             // the failures, one count per wavefront; a failing cell's 5-point
             // neighbourhood enters list p + 1, one neighbour per lane of its group
             const unsigned long long fm = __ballot(dec);
-            if (fm && lane == __ffsll((long long)fm) - 1) M::add(cnt_fail, (int)__popcll(fm));
             const bool decg = __shfl(dec ? 1 : 0, gbase) != 0;
+            if constexpr (WIDE) {
+                wide_append(it & 1, decg, fm, i, j, cnt_fail, mark);
+            } else {
+                if (fm && lane == __ffsll((long long)fm) - 1) M::add(cnt_fail, (int)__popcll(fm));
 #pragma unroll
-            for (int k0 = 0; k0 < 5; k0 += G) mark(decg && k0 + gl < 5, i, j, k0 + gl, NX, NY);
+                for (int k0 = 0; k0 < 5; k0 += G) mark(decg && k0 + gl < 5, i, j, k0 + gl, NX, NY);
+            }
         }
+    }
+
+    // The wide launch's list appends and failure count for one trip: every
+    // wavefront of a wide launch appends at once, and one device atomic per
+    // wavefront on the single count word still serialises (thousands of
+    // wavefronts), so the workgroup gathers its trip's entries and failures
+    // in LDS first, and one thread reserves the workgroup's range.  Called by
+    // every thread of the workgroup once per trip; b is the trip's parity
+    // (the LDS buffers of trip t + 1 are not those trip t copies out, and the
+    // counts of trip t + 1 are cleared while trip t holds its two barriers).
+    __device__ __forceinline__ void wide_append(int b, bool decg, unsigned long long fm, int i, int j,
+                                                int *cnt_fail, const GridMark &mk)
+    {
+        constexpr int WG = __HIP_MEMORY_SCOPE_WORKGROUP;
+        const unsigned long long lt = (1ull << lane) - 1ull;
+        const int leader = __ffsll((long long)__ballot(1)) - 1;
+#pragma unroll
+        for (int kk = 0; kk < KK; ++kk) {
+            const int k = kk * G + gl;  // neighbour k of the group's cell: 0 bottom, 1 left, 2 the cell, 3 right, 4 top
+            const int ni = i + (k == 1 ? -1 : k == 3 ? 1 : 0), nj = j + (k == 0 ? -1 : k == 4 ? 1 : 0);
+            const int d = nj * NX + ni;
+            bool add = false;
+            if (decg && k < 5 && !(ni < 0 || ni >= NX || nj < 0 || nj >= NY))
+                add = Scoped<__HIP_MEMORY_SCOPE_AGENT>::exch(&mk.mark[d], mk.stamp) != mk.stamp;
+            const unsigned long long m = __ballot(add);
+            int off = 0;
+            if (lane == leader && m) off = __hip_atomic_fetch_add(&L.wcnt[b], (int)__popcll(m), __ATOMIC_RELAXED, WG);
+            off = __shfl(off, leader);
+            if (add) L.wbuf[b][off + __popcll(m & lt)] = d;
+        }
+        if (lane == leader && fm) __hip_atomic_fetch_add(&L.wfail[b], (int)__popcll(fm), __ATOMIC_RELAXED, WG);
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            const int n = L.wcnt[b], f = L.wfail[b];
+            L.wbase = n ? atomicAdd(mk.cnt, n) : 0;
+            if (f) Scoped<__HIP_MEMORY_SCOPE_AGENT>::add(cnt_fail, f);
+            L.wcnt[b ^ 1] = 0;
+            L.wfail[b ^ 1] = 0;
+        }
+        __syncthreads();
+        const int n = L.wcnt[b], base = L.wbase;
+        for (int q = threadIdx.x; q < n; q += NTH) mk.list[base + q] = L.wbuf[b][q];
     }
 
     // Face-flux cache access (workgroup 0's passes).
