@@ -584,6 +584,10 @@ struct MoodPass {
     int NX, NY, lane, gl, gbase, f, jg, fdir, fa, fb, oai, oaj, obi, obj, cs, stamp_base;
     bool fc_on, g_on;
     unsigned gst;  // this stage's tag in the device-wide cache
+    // trips of grid-wide passes run by this workgroup since the launch began:
+    // their parity picks wide_append's LDS buffers, alternating across passes
+    // as well as within one
+    int trip = 0;
     MoodParams P;
     double dtdx, dtdy, dtg, xmin_top;
 
@@ -717,7 +721,7 @@ struct MoodPass {
             const unsigned long long fm = __ballot(dec);
             const bool decg = __shfl(dec ? 1 : 0, gbase) != 0;
             if constexpr (WIDE) {
-                wide_append(it & 1, decg, fm, i, j, cnt_fail, mark);
+                wide_append(trip++ & 1, decg, fm, i, j, cnt_fail, mark);
             } else {
                 if (fm && lane == __ffsll((long long)fm) - 1) M::add(cnt_fail, (int)__popcll(fm));
 #pragma unroll
@@ -731,9 +735,12 @@ struct MoodPass {
     // wavefront on the single count word still serialises (thousands of
     // wavefronts), so the workgroup gathers its trip's entries and failures
     // in LDS first, and one thread reserves the workgroup's range.  Called by
-    // every thread of the workgroup once per trip; b is the trip's parity
-    // (the LDS buffers of trip t + 1 are not those trip t copies out, and the
-    // counts of trip t + 1 are cleared while trip t holds its two barriers).
+    // every thread of the workgroup once per trip; b is the parity of the
+    // workgroup's trip count since the launch (init_lds clears both buffers'
+    // counts; trip t clears those of trip t + 1 between its two barriers, and
+    // trip t + 1 does not write the buffer trip t copies out).  The cooperative
+    // kernel's short, latency-bound passes keep one atomic per wavefront: the
+    // two barriers cost them about what the atomics would.
     __device__ __forceinline__ void wide_append(int b, bool decg, unsigned long long fm, int i, int j,
                                                 int *cnt_fail, const GridMark &mk)
     {
