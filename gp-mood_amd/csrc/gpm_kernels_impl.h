@@ -432,28 +432,33 @@ struct GridMark {
 
 // The same for the passes workgroup 0 runs alone: the list lives in LDS and
 // an LDS hash set (open addressing, -1 = empty) deduplicates it, so these
-// passes make no global atomic round trips.
+// passes make no global atomic round trips.  The set is at most half full
+// (static_assert below), so a probe meets an empty slot within the table; the
+// probe loops still stop after one sweep of the table, so that damaged LDS
+// contents give wrong bits, which the parity tests catch, rather than a hang.
 constexpr int kSoloHashBits = 9;
 constexpr int kSoloHash = 1 << kSoloHashBits;  // slots; a solo pass lists at most 5 * kSoloMax cells
 __device__ __forceinline__ bool lds_set_insert(int *hash, int bits, int key)
 {
     unsigned h = ((unsigned)key * 0x9E3779B1u) >> (32 - bits);
-    for (;;) {
+    for (int probe = 0; probe < (1 << bits); ++probe) {
         const int old = atomicCAS(&hash[h], -1, key);
         if (old == -1) return true;
         if (old == key) return false;
         h = (h + 1) & ((1u << bits) - 1u);
     }
+    return false;
 }
 __device__ __forceinline__ bool lds_set_has(const int *hash, int bits, int key)
 {
     unsigned h = ((unsigned)key * 0x9E3779B1u) >> (32 - bits);
-    for (;;) {
+    for (int probe = 0; probe < (1 << bits); ++probe) {
         const int v = hash[h];
         if (v == key) return true;
         if (v == -1) return false;
         h = (h + 1) & ((1u << bits) - 1u);
     }
+    return false;
 }
 static_assert(5 * kSoloMax <= kSoloHash / 2, "the solo hash set must stay at most half full");
 
