@@ -605,6 +605,22 @@ __device__ __forceinline__ void riemann_dir(int dir, const double ul[4], const d
     }
 }
 
+// A stencil accessor anchored at cell (i, j): q(dx, dy) = acc(i + dx, j + dy).
+// Accessors over an LDS tile overload anchor() with a shifted pointer, so a
+// stencil point costs one address add per side (the uniform offset comes
+// from the scalar unit) instead of a row product per point.
+template <class Acc>
+struct Anchored {
+    Acc acc;
+    int i, j;
+    __device__ __forceinline__ double4 operator()(int dx, int dy) const { return acc(i + dx, j + dy); }
+};
+template <class Acc>
+__device__ __forceinline__ Anchored<Acc> anchor(const Acc &acc, int i, int j)
+{
+    return Anchored<Acc>{acc, i, j};
+}
+
 // GP reconstruction of both sides of a face at all NGP Gauss points
 // (mod_GP_MOOD.f90:46-65): ua[j][k] = sum_s zT(ORD, s, FA, j) * q_A(k, s) and
 // likewise ub for cell B, each summed from s = 1 in stencil order starting at 0
@@ -621,11 +637,13 @@ __device__ __forceinline__ void recon2(const DevTables *__restrict__ T, const Ac
     for (int j = 0; j < NGP; ++j)
 #pragma unroll
         for (int k = 0; k < 4; ++k) { ua[j][k] = 0.0; ub[j][k] = 0.0; }
+    const auto A = anchor(acc, ai, aj);
+    const auto B = anchor(acc, bi, bj);
 #pragma unroll UNR
     for (int s = 0; s < SZ; ++s) {
         const int dx = kSX[s], dy = kSY[s];
-        const double4 qa = acc(ai + dx, aj + dy);
-        const double4 qb = acc(bi + dx, bj + dy);
+        const double4 qa = A(dx, dy);
+        const double4 qb = B(dx, dy);
 #pragma unroll
         for (int j = 0; j < NGP; ++j) {
             const double za = T->zT[OI][FA][s][j];
@@ -710,12 +728,14 @@ __device__ __forceinline__ void face_gp_flux_dyn(const DevTables *__restrict__ T
     const double *__restrict__ za_row = &T->zT[oi][fa][0][j];
     const double *__restrict__ zb_row = &T->zT[oi][fb][0][j];
     double ua[4] = {0.0, 0.0, 0.0, 0.0}, ub[4] = {0.0, 0.0, 0.0, 0.0};
+    const auto A = anchor(acc, ai, aj);
+    const auto B = anchor(acc, bi, bj);
 #pragma unroll 5
     for (int s = 0; s < SZM; ++s) {
         if (s >= sz) break;
         const int dx = kSX[s], dy = kSY[s];
-        const double4 qa = acc(ai + dx, aj + dy);
-        const double4 qb = acc(bi + dx, bj + dy);
+        const double4 qa = A(dx, dy);
+        const double4 qb = B(dx, dy);
         const double za = za_row[s * 5], zb = zb_row[s * 5];
         ua[0] = ua[0] + za * qa.x; ua[1] = ua[1] + za * qa.y; ua[2] = ua[2] + za * qa.z; ua[3] = ua[3] + za * qa.w;
         ub[0] = ub[0] + zb * qb.x; ub[1] = ub[1] + zb * qb.y; ub[2] = ub[2] + zb * qb.z; ub[3] = ub[3] + zb * qb.w;

@@ -41,6 +41,8 @@ struct LdsAcc {
     int pitch;
     __device__ __forceinline__ double4 operator()(int i, int j) const { return s[j * pitch + i]; }
 };
+// the tile anchored at (i, j): the same accessor on a shifted pointer
+__device__ __forceinline__ LdsAcc anchor(const LdsAcc &a, int i, int j) { return LdsAcc{a.s + (j * a.pitch + i), a.pitch}; }
 
 // Time t_rk of a stage (time_stepper.f90:17, mod_SSP_RK3.f90:22-31) from the
 // step-start time tstate[4] and dt tstate[1].
