@@ -59,7 +59,15 @@ struct gpm_ctx {
     // wide launches of long-list passes ahead of the cooperative kernel: launches per stage, grid, shortest list
     int mood_wide = 0, mood_wide_blocks = 0, mood_wide_min = 0;
     bool mood_wide_on = true;  // the last call's first lists were long (or there was no call yet)
-    int long_list_h = 0;       // host copy of DevStats::long_list
+    // the persistent MOOD kernel as an ordinary launch with its own rendezvous
+    // and grid barrier (else a cooperative launch); its rendezvous polls, and
+    // the test switch that sends every pass to one workgroup
+    bool mood_ord = false;
+    int mood_rv_spins = 1 << 14;
+    int mood_bar_spins = 1 << 21;  // about a second of polls; a pass takes milliseconds at most
+    bool mood_rv_alone = false;
+    bool mood_alone_told = false;
+    int mood_chk[3] = {0, 0, 0};  // host copy of DevStats::long_list, mood_alone, mood_stall
     FluxTable gfc{};        // device-wide face-flux cache that the other workgroups fill meanwhile
     bool has_state = false;
     bool timing = false;
@@ -447,36 +455,68 @@ void setup_mood_wide(gpm_ctx *c, int ncu, int flux)
     if (const char *env = std::getenv("GPM_MOOD_WIDE_MIN")) c->mood_wide_min = std::atoi(env) > 0 ? std::atoi(env) : 0;
 }
 
+// GPM_MOOD_ORD=1: the persistent MOOD kernel as an ordinary launch (0: a
+// cooperative launch); GPM_MOOD_RV_SPINS: its rendezvous polls;
+// GPM_MOOD_RV_ALONE=1: one workgroup runs every stage's passes (tests);
+// GPM_MOOD_BAR_SPINS: the polls of its grid-barrier waits (tests of a stall).
+void mood_ord_env(gpm_ctx *c)
+{
+    if (const char *env = std::getenv("GPM_MOOD_BAR_SPINS")) c->mood_bar_spins = std::atoi(env) > 0 ? std::atoi(env) : 0;
+    if (const char *env = std::getenv("GPM_MOOD_ORD")) c->mood_ord = std::atoi(env) != 0;
+    if (const char *env = std::getenv("GPM_MOOD_RV_SPINS")) c->mood_rv_spins = std::atoi(env) > 0 ? std::atoi(env) : 0;
+    if (const char *env = std::getenv("GPM_MOOD_RV_ALONE")) c->mood_rv_alone = std::atoi(env) != 0;
+}
+
 // An idle wide launch still costs a few microseconds per stage, so a context
 // enqueues them only while its lists are long: the device records the longest
 // first or second correction list (DevStats::long_list), a call that ends
 // synchronised reads it (mood_wide_read before the synchronisation,
 // mood_wide_decide after it), and the next call follows it.  Results do not
-// depend on the launch plan.
+// depend on the launch plan.  The same read brings the ordinary persistent
+// launches' reports: stages run by one workgroup (said once on stderr) and
+// grid-barrier waits that reached their bound (the call fails).
+static_assert(offsetof(DevStats, mood_alone) == offsetof(DevStats, long_list) + sizeof(int) &&
+                  offsetof(DevStats, mood_stall) == offsetof(DevStats, long_list) + 2 * sizeof(int),
+              "the MOOD launch reports are read as one block");
 int mood_wide_read(gpm_ctx *c)
 {
-    if (c->mood_wide <= 0 || !c->mood_coop) return GPM_OK;
-    HIPCHK(c, hipMemcpyAsync(&c->long_list_h, &c->stats->long_list, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    if (!c->mood_coop) return GPM_OK;
+    HIPCHK(c, hipMemcpyAsync(c->mood_chk, &c->stats->long_list, sizeof(c->mood_chk), hipMemcpyDeviceToHost, c->stream));
     return GPM_OK;
 }
 int mood_wide_decide(gpm_ctx *c)
 {
-    if (c->mood_wide <= 0 || !c->mood_coop) return GPM_OK;
-    c->mood_wide_on = c->long_list_h > c->mood_wide_min;
-    HIPCHK(c, hipMemsetAsync(&c->stats->long_list, 0, sizeof(int), c->stream));
+    if (!c->mood_coop) return GPM_OK;
+    if (c->mood_wide > 0) c->mood_wide_on = c->mood_chk[0] > c->mood_wide_min;
+    HIPCHK(c, hipMemsetAsync(&c->stats->long_list, 0, sizeof(c->mood_chk), c->stream));
+    if (c->mood_chk[2] > 0) return fail(c, GPM_EHIP, "a grid barrier of the MOOD passes did not complete");
+    if (c->mood_chk[1] > 0 && !c->mood_alone_told) {
+        c->mood_alone_told = true;
+        std::fprintf(stderr,
+                     "gpm: the MOOD passes of %d stage(s) ran on one workgroup (the workgroups of the persistent "
+                     "launch were not resident together)\n",
+                     c->mood_chk[1]);
+    }
     return GPM_OK;
 }
 
-// The MOOD correction passes of one stage.  Cooperative: one launch runs every
-// pass.  Otherwise (when asked for, or when the cooperative launch is
-// refused, e.g. while another process holds the device's cooperative-launch
-// resource) per-pass launches in growing batches, with a look at the stage's
-// pending flag between batches: a launch runs the current pass or one run of
-// workgroup-0 passes and completes at least one pass, and launches past the
-// last pass return at once.
+// The MOOD correction passes of one stage.  Persistent (the default): one
+// launch of k_mood runs every pass, with grid barriers between passes; it is
+// a cooperative launch, whose barrier is the hardware's, or an ordinary
+// launch whose workgroups meet at a rendezvous first (when the device refuses
+// a cooperative launch, e.g. while another process holds that resource, or
+// when GPM_MOOD_ORD=1 asks for it).  Otherwise (when asked for) per-pass
+// launches in growing batches, with a look at the stage's pending flag
+// between batches: a launch runs the current pass or one run of workgroup-0
+// passes and completes at least one pass, and launches past the last pass
+// return at once.
 int enqueue_mood(gpm_ctx *c, MoodArgs &m)
 {
     const int mord = c->mord, ngp = c->ngp, flux = c->cfg.num_flux;
+    m.sync = kMoodCoop;
+    m.rv_spins = c->mood_rv_spins;
+    m.rv_alone = c->mood_rv_alone ? 1 : 0;
+    m.bar_spins = c->mood_bar_spins;
     if (c->mood_coop) {
         void *args[] = {&m};
         if (c->mood_wide > 0 && c->mood_wide_on) {
@@ -487,13 +527,19 @@ int enqueue_mood(gpm_ctx *c, MoodArgs &m)
             for (int w = 0; w < c->mood_wide; ++w)
                 HIPCHK(c, hipLaunchKernel(wk, dim3(c->mood_wide_blocks), dim3(kWideThreads), args, 0, c->stream));
         }
-        const hipError_t e = hipLaunchCooperativeKernel(mood_kernel(mord, ngp, flux, c->ring_mode, true),
-                                                        dim3(c->mood_blocks), dim3(kMoodThreads), args, 0, c->stream);
-        if (e == hipSuccess) return GPM_OK;
-        (void)hipGetLastError();
-        std::fprintf(stderr, "gpm: cooperative launch refused (%s); MOOD passes run as per-pass launches\n",
-                     hipGetErrorString(e));
-        c->mood_coop = false;
+        const void *k = mood_kernel(mord, ngp, flux, c->ring_mode, true);
+        if (!c->mood_ord) {
+            const hipError_t e =
+                hipLaunchCooperativeKernel(k, dim3(c->mood_blocks), dim3(kMoodThreads), args, 0, c->stream);
+            if (e == hipSuccess) return GPM_OK;
+            (void)hipGetLastError();
+            std::fprintf(stderr, "gpm: cooperative launch refused (%s); the MOOD kernel runs as an ordinary launch\n",
+                         hipGetErrorString(e));
+            c->mood_ord = true;
+        }
+        m.sync = kMoodOrd;
+        HIPCHK(c, hipLaunchKernel(k, dim3(c->mood_blocks), dim3(kMoodThreads), args, 0, c->stream));
+        return GPM_OK;
     }
     const void *k = mood_kernel(mord, ngp, flux, c->ring_mode, false);
     void *args[] = {&m};
@@ -843,6 +889,7 @@ int gpm_create(const gpm_config *cfg, gpm_ctx **out)
     setup_mood_wide(c, ncu, c->cfg.num_flux);
     // GPM_MOOD_COOP=0 selects the per-pass launches of the MOOD passes.
     if (const char *env = std::getenv("GPM_MOOD_COOP")) c->mood_coop = std::atoi(env) != 0;
+    mood_ord_env(c);
     // Passes over at most this many cells run on one workgroup (GPM_MOOD_SOLO overrides; 0 disables).
     c->mood_solo = kSoloMax;
     if (const char *env = std::getenv("GPM_MOOD_SOLO")) c->mood_solo = std::atoi(env) > 0 ? std::atoi(env) : 0;
@@ -1682,6 +1729,7 @@ int gpm_run_noalloc(const gpm_config *cfg, const double *zT_sp, const double *gp
     // cooperative kernel alone, as for short lists
     c.mood_wide = 0;
     if (const char *env = std::getenv("GPM_MOOD_COOP")) c.mood_coop = std::atoi(env) != 0;
+    mood_ord_env(&c);
     c.has_state = true;
 
     int rc = upload_tables(&c, zT_sp, gp_d2x_sp, gp_d2y_sp, gauss_weight, ixiy_sp);
@@ -1701,7 +1749,11 @@ int gpm_run_noalloc(const gpm_config *cfg, const double *zT_sp, const double *gp
         return bad(GPM_EHIP, "copy of the final state");
     if (t_end && hipMemcpyAsync(t_end, c.tstate, sizeof(double), hipMemcpyDeviceToHost, c.stream) != hipSuccess)
         return bad(GPM_EHIP, "copy of the final time");
+    // the MOOD launch reports (an ordinary persistent launch's stalls)
+    rc = mood_wide_read(&c);
+    if (rc) return bad(rc, c.msg.c_str());
     if (hipStreamSynchronize(c.stream) != hipSuccess) return bad(GPM_EHIP, "stream synchronisation");
+    if (c.mood_chk[2] > 0) return bad(GPM_EHIP, "a grid barrier of the MOOD passes did not complete");
     return GPM_OK;
 }
 

@@ -63,7 +63,17 @@ enum {
     kCntSecond = 8,   // cells recomputed in the correction passes
     kCntPending = 9,  // per-pass launches: the current pass has cells to recompute
     kCntStop = 10,    // the first pass of workgroup 0's last run of passes, once that run has ended
+    kCntArrive = 11,  // ordinary persistent launch: workgroups arrived (| kArriveAbort: one runs alone)
+    kCntBar = 12,     // ordinary persistent launch: grid-barrier arrivals, gridDim.x per barrier
     kCountersPerStage = 16
+};
+constexpr int kArriveAbort = 1 << 30;
+constexpr int kBarStalled = 1 << 29;  // kCntBar from here on: a grid-barrier wait stalled, the launch ends
+
+// How k_mood's workgroups wait for each other between passes.
+enum {
+    kMoodCoop = 0,  // cooperative launch: cooperative-groups grid barrier
+    kMoodOrd = 1,   // ordinary launch: rendezvous, then a grid barrier on kCntBar
 };
 
 struct DevStats {
@@ -82,6 +92,12 @@ struct DevStats {
     // the longest list of a stage's first two correction passes since the
     // host last read it (the lists the wide launches take)
     int long_list;
+    // ordinary persistent MOOD launches since the host last read them: stages
+    // whose passes ran on one workgroup (the others were not resident
+    // together), and grid-barrier waits that reached their bound (the stage's
+    // results are then invalid and the call fails)
+    int mood_alone;
+    int mood_stall;
 };
 
 constexpr int kMaxStages = 5;
@@ -178,7 +194,11 @@ struct MoodArgs {
     int solo_max;   // lists of at most this many cells run on workgroup 0 alone (0: never)
     int wide_min;   // a wide launch runs the current pass only when it lists more cells than this
     int fcache;     // workgroup 0's passes reuse face fluxes and compute likely next ones ahead
-    FluxTable gfc;  // while workgroup 0 runs passes alone, the other workgroups fill it (cooperative kernel)
+    int sync;       // kMoodCoop or kMoodOrd: how k_mood was launched
+    int rv_spins;   // kMoodOrd: polls of the rendezvous before one workgroup takes every pass alone
+    int rv_alone;   // kMoodOrd: the first workgroup to arrive takes every pass alone (tests)
+    int bar_spins;  // kMoodOrd: polls of a grid-barrier wait before it gives up (a stall)
+    FluxTable gfc;  // while workgroup 0 runs passes alone, the other workgroups fill it (persistent kernel)
     const float *__restrict__ vnn;  // as StageArgs::vnn
     int recon;                      // kRecon*
     double dx, dy;

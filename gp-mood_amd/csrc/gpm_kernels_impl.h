@@ -1062,7 +1062,10 @@ struct MoodPass {
         int last = -1;
         for (;;) {
             if (tid == 0) {
-                L.hstop = load_relaxed(&a.cnt[kCntStop]) == p;
+                // workgroup 0 ended this run of passes, or (ordinary launch) a
+                // grid barrier stalled and workgroup 0 may have left
+                L.hstop = load_relaxed(&a.cnt[kCntStop]) == p ||
+                          (a.sync == kMoodOrd && load_relaxed(&a.cnt[kCntBar]) >= kBarStalled);
                 L.hseq = __hip_atomic_load(&a.gfc.beacon[0], __ATOMIC_RELAXED, AG);
                 const int m = __hip_atomic_load(&a.gfc.beacon[1], __ATOMIC_RELAXED, AG);
                 L.hm = m < 0 ? 0 : m > 16 ? 16 : m;
@@ -1273,21 +1276,125 @@ __device__ __forceinline__ void mood_finish(const MoodArgs &a, int p)
     }
 }
 
-// MOOD correction passes of one RK stage as a cooperative persistent kernel
-// with one grid barrier per pass, from the first pass the wide launches
-// before it left (kCntPass + 1; pass 1 when none ran); the loop ends on the
-// device when a pass finds no cell to recompute.
+// Rendezvous of an ordinary persistent launch (thread 0 of each workgroup),
+// before any workgroup touches the stage's data.  Returns 0 when every
+// workgroup of the grid has arrived: all are running, and a running
+// workgroup stays resident until it returns, so grid barriers cannot wait on
+// one that has not started.  When the others have not all arrived after
+// rv_spins polls (the device is shared with other work), this workgroup
+// sets kArriveAbort and returns 2: it takes every pass alone; the others
+// return 1 and leave.  Arrivals and the abort are compare-and-swaps on one
+// word, and an arrival is refused once the abort is set, so all workgroups
+// agree: the count reaches gridDim.x only without an abort.
+__device__ __forceinline__ int mood_rendezvous(const MoodArgs &a)
+{
+    constexpr int AG = __HIP_MEMORY_SCOPE_AGENT;
+    int *w = &a.cnt[kCntArrive];
+    const int n = (int)gridDim.x;
+    int v = load_relaxed(w);
+    for (;;) {
+        if (v & kArriveAbort) return 1;
+        if (__hip_atomic_compare_exchange_strong(w, &v, v + 1, __ATOMIC_RELAXED, __ATOMIC_RELAXED, AG)) break;
+    }
+    if (a.rv_alone) return v == 0 ? 2 : 1;  // the first to arrive takes the passes alone
+    for (int s = 0;; ++s) {
+        v = load_relaxed(w);
+        if (v & kArriveAbort) return 1;
+        if (v >= n) return 0;
+        if (s >= a.rv_spins) {
+            if (__hip_atomic_compare_exchange_strong(w, &v, v | kArriveAbort, __ATOMIC_RELAXED, __ATOMIC_RELAXED, AG))
+                return 2;
+            continue;  // the word changed: an arrival or another abort
+        }
+        __builtin_amdgcn_s_sleep(2);
+    }
+}
+
+// Grid barrier of an ordinary persistent launch after its rendezvous: each
+// workgroup adds one arrival to kCntBar and waits for gridDim.x arrivals per
+// barrier so far; agent-scope fences on every lane publish the workgroup's
+// writes before it and make the others' visible after it.  A wait that
+// reaches its bound (a.bar_spins polls) counts a stall and lifts the count to
+// kBarStalled, past every later target; a workgroup that finds the count
+// there after its wait returns false and leaves the kernel without starting
+// another pass (the helpers of workgroup 0's passes stop on it too), so no
+// wait blocks on a workgroup that has left, every pass a workgroup starts
+// follows a completed barrier, and the host reports the stall.  ok is a
+// workgroup-shared flag.
+__device__ __forceinline__ bool ord_grid_sync(const MoodArgs &a, int &nbar, int *ok)
+{
+    constexpr int AG = __HIP_MEMORY_SCOPE_AGENT;
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int *b = &a.cnt[kCntBar];
+        __hip_atomic_fetch_add(b, 1, __ATOMIC_RELAXED, AG);
+        const int target = (nbar + 1) * (int)gridDim.x;
+        int v;
+        for (int s = 0; (v = load_relaxed(b)) < target; ++s) {
+            if (s >= a.bar_spins) {
+                atomicAdd(&a.stats->mood_stall, 1);
+                __hip_atomic_fetch_max(b, kBarStalled, __ATOMIC_RELAXED, AG);
+                v = kBarStalled;
+                break;
+            }
+            __builtin_amdgcn_s_sleep(1);
+        }
+        *ok = v < kBarStalled;
+    }
+    ++nbar;
+    __syncthreads();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    return *ok != 0;
+}
+
+// The same between the passes of a workgroup that runs them alone.
+__device__ __forceinline__ void alone_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+    __syncthreads();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+}
+
+// MOOD correction passes of one RK stage as a persistent kernel with one
+// grid barrier per pass, from the first pass the wide launches before it
+// left (kCntPass + 1; pass 1 when none ran); the loop ends on the device when
+// a pass finds no cell to recompute.  a.sync says how it was launched:
+// cooperatively (cooperative-groups barrier), or as an ordinary launch that
+// first meets at mood_rendezvous and then uses ord_grid_sync; an ordinary
+// launch avoids the idle time the runtime puts around every cooperative one.
 template <int MORD, int NGP, int FLUX, bool GEN>
 __global__ __launch_bounds__(kMoodThreads) void k_mood(MoodArgs a)
 {
     using MP = MoodPass<MORD, NGP, FLUX, GEN>;
     __shared__ typename MP::Lds lds;
+    __shared__ int s_rv, s_ok;
     MP mp(a, lds);
+    const bool ord = a.sync == kMoodOrd;
+    bool alone = false;
+    if (ord) {
+        if (threadIdx.x == 0) s_rv = mood_rendezvous(a);
+        __syncthreads();
+        if (s_rv == 1) return;
+        alone = s_rv == 2;
+        if (alone) {
+            mp.g_on = false;  // no helper workgroups
+            if (threadIdx.x == 0) atomicAdd(&a.stats->mood_alone, 1);
+        }
+    }
     mp.init_lds();
     cg::grid_group grid = cg::this_grid();
     const int ncell = a.B.NX * a.B.NY;
-    const int gtid = blockIdx.x * blockDim.x + threadIdx.x;
-    const int gsz = gridDim.x * blockDim.x;
+    const int gtid = alone ? (int)threadIdx.x : (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    const int gsz = alone ? (int)blockDim.x : (int)(gridDim.x * blockDim.x);
+    int nbar = 0;
+    // false: an ordinary launch's barrier stalled, and the workgroup leaves
+    const auto sync = [&]() -> bool {
+        if (alone) alone_sync();
+        else if (ord) return ord_grid_sync(a, nbar, &s_ok);
+        else grid.sync();
+        return true;
+    };
     int p = load_relaxed(&a.cnt[kCntPass]) + 1;
     for (;;) {
         // the pass's count and this lane's first list entry, in one round trip
@@ -1298,14 +1405,14 @@ __global__ __launch_bounds__(kMoodThreads) void k_mood(MoodArgs a)
         if (nS <= a.solo_max) {
             // workgroup 0 runs the short-list passes; the others fill the
             // device-wide face-flux cache meanwhile, then wait here
-            if (blockIdx.x == 0) mp.solo(p, nS);
+            if (alone || blockIdx.x == 0) mp.solo(p, nS);
             else mp.helper(p);
-            grid.sync();
+            if (!sync()) return;
             p = load_relaxed(&a.cnt[kCntNext]);
             continue;
         }
         mp.grid_pass(p, nS, d0, gtid, gsz);
-        grid.sync();
+        if (!sync()) return;
         ++p;
     }
     if (gtid == 0) mood_finish(a, p);

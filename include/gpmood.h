@@ -234,13 +234,18 @@ void *gpm_stream(gpm_ctx *ctx);
 
 /* How the MOOD correction passes of a stage (the MOOD loop of mod_FE.f90:69-174
  * after its first pass) are launched.  cooperative = 1 (the default): one
- * cooperative launch per stage runs every pass on the device, with grid
- * barriers between passes.  cooperative = 0: per-pass launches in growing
- * batches, with the stage's pending flag read back between batches (one
- * host synchronisation per batch).  The results are the same bit for bit.
- * The library switches to per-pass launches by itself when the device
- * refuses a cooperative launch (one process at a time holds that resource),
- * and the environment variable GPM_MOOD_COOP=0 selects them at gpm_create.
+ * persistent launch per stage runs every pass on the device, with grid
+ * barriers between passes.  It is a cooperative launch (the hardware's grid
+ * barrier); when the device refuses one (one process at a time holds that
+ * resource) the library switches by itself to an ordinary launch of the same
+ * kernel, whose workgroups first meet at a bounded rendezvous (if they are
+ * not all resident together, one workgroup runs every pass alone) and then
+ * use grid barriers of their own (GPM_MOOD_ORD=1 at gpm_create asks for it;
+ * a barrier wait that reaches its bound fails the call with GPM_EHIP).
+ * cooperative = 0: per-pass launches in growing batches, with the stage's
+ * pending flag read back between batches (one host synchronisation per
+ * batch); the environment variable GPM_MOOD_COOP=0 selects them at
+ * gpm_create.  The results are the same bit for bit.
  * With cooperative launches, a stage's first passes over long lists (more
  * cells than one trip of the cooperative grid) run in two ordinary launches
  * of the whole GPU ahead of the cooperative one, while the context's lists

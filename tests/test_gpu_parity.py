@@ -98,9 +98,8 @@ def test_run_matches_reference(gpm, case):
 @pytest.mark.parametrize("case", STEP_CASES)
 def test_run_per_pass_launches(gpm, case):
     """The MOOD correction passes as per-pass launches instead of one
-    cooperative launch per stage (the path taken when the device refuses a
-    cooperative launch): the same dt sequence, counts, mask and state as the
-    reference run."""
+    persistent launch per stage (set_mood_launch(False)): the same dt
+    sequence, counts, mask and state as the reference run."""
     z = golden(f"step_{case}.npz")
     nsteps = int(z["nsteps"])
     print(f"[case {case}] {nsteps} steps via gpm_run, per-pass MOOD launches", flush=True)
@@ -140,6 +139,62 @@ def test_run_wide_launches(gpm, case, monkeypatch):
     st = s.stats()
     assert st["steps"] == nsteps
     assert st["detected_total"] == int(np.sum(z["counts"]))
+
+
+@pytest.mark.parametrize("alone", [False, True], ids=["grid", "alone"])
+@pytest.mark.parametrize("case", STEP_CASES)
+def test_run_ordinary_persistent_launch(gpm, case, alone, monkeypatch, capfd):
+    """The persistent MOOD kernel as an ordinary launch (GPM_MOOD_ORD=1, the
+    path taken when the device refuses a cooperative launch): its workgroups
+    meet at a rendezvous and then use grid barriers of their own.  With
+    GPM_MOOD_RV_ALONE=1 the first workgroup to arrive runs every pass alone
+    and the others leave (the path taken when they are not all resident
+    together).  The same dt sequence, counts, mask and state as the
+    reference run."""
+    z = golden(f"step_{case}.npz")
+    nsteps = int(z["nsteps"])
+    print(f"[case {case}] {nsteps} steps via gpm_run, ordinary persistent MOOD launch"
+          f"{', one workgroup' if alone else ''}", flush=True)
+    monkeypatch.setenv("GPM_MOOD_ORD", "1")
+    monkeypatch.setenv("GPM_MOOD_RV_ALONE", "1" if alone else "0")
+    s = make_solver(gpm, z)
+    s.set_state(z["U0"])
+    s.set_time(0.0)
+    dts = s.run(nsteps, return_dts=True)
+    U, g = s.get_state(order=True)
+    assert np.array_equal(dts, z["dt"]), f"dt differs at steps {np.nonzero(dts != z['dt'])[0][:5]}"
+    assert np.array_equal(g[4:-4, 4:-4], z["masks"][-1])
+    assert np.array_equal(U, z["U"]), f"max |diff| {np.abs(U - z['U']).max()}"
+    st = s.stats()
+    assert st["detected_total"] == int(np.sum(z["counts"]))
+    # the persistent kernel ran (every stage with detection launches it) and,
+    # when asked, on one workgroup
+    said = "ran on one workgroup" in capfd.readouterr().err
+    assert said == (alone and st["max_passes"] > 0)
+
+
+def test_ordinary_launch_stall_is_reported(gpm, monkeypatch):
+    """A grid-barrier wait of the ordinary persistent launch that reaches its
+    bound (forced from the host: GPM_MOOD_BAR_SPINS=0, no poll before giving
+    up) ends the launch and fails the call with an error instead of waiting
+    on; a solver created afterwards without the switch runs the case exactly."""
+    z = golden("step_r3_n64_ref_bl6.npz")
+    print("[case r3_n64_ref_bl6] 3 steps, ordinary persistent MOOD launch, forced barrier stall", flush=True)
+    monkeypatch.setenv("GPM_MOOD_ORD", "1")
+    monkeypatch.setenv("GPM_MOOD_BAR_SPINS", "0")
+    s = make_solver(gpm, z)
+    s.set_state(z["U0"])
+    s.set_time(0.0)
+    with pytest.raises(gpm.GpmError, match="grid barrier"):
+        s.run(3)
+    s.close()
+    monkeypatch.delenv("GPM_MOOD_BAR_SPINS")
+    s = make_solver(gpm, z)
+    s.set_state(z["U0"])
+    s.set_time(0.0)
+    dts = s.run(int(z["nsteps"]), return_dts=True)
+    assert np.array_equal(dts, z["dt"])
+    assert np.array_equal(s.get_state(), z["U"])
 
 
 def test_config0_256_300_steps_checksum(gpm):
