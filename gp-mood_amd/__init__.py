@@ -224,12 +224,13 @@ class Solver:
         return self.lib.gpm_stream(self._h)
 
     def set_mood_launch(self, cooperative=True):
-        """MOOD correction passes as one cooperative launch per stage (True)
-        or as per-pass launches (False); the results are the same."""
+        """MOOD correction passes as one persistent launch per stage (True;
+        an ordinary launch, or a cooperative one with GPM_MOOD_ORD=0 at
+        creation) or as per-pass launches (False); the results are the same."""
         self._check(self.lib.gpm_set_mood_launch(self._h, int(cooperative)), "gpm_set_mood_launch")
 
     def mood_launch(self):
-        """True while the MOOD passes run as a cooperative launch."""
+        """True while the MOOD passes run as one persistent launch per stage."""
         v = ctypes.c_int32()
         self._check(self.lib.gpm_get_mood_launch(self._h, ctypes.byref(v)), "gpm_get_mood_launch")
         return bool(v.value)

@@ -2,7 +2,7 @@
 //
 // Owns the device buffers of one solver context and enqueues the step on the
 // context's HIP stream: k_dt, then per RK stage k_stage (first MOOD pass) and
-// the cooperative k_mood (remaining passes), then, for boundaries whose ghost
+// the persistent k_mood (remaining passes), then, for boundaries whose ghost
 // cells are not copies of interior cells, k_ring (the ghost values the next
 // Setdt sees).  Host-array entry points convert between the reference's
 // ghosted layout U(4, lb:le, nb:ne) and the device's interior-only layout.
@@ -54,15 +54,15 @@ struct gpm_ctx {
     unsigned *done = nullptr;
     int dt_log_cap = 0;
     int tiles_x = 0, ntiles = 0, dt_blocks = 0, mood_blocks = 0, mood_solo = 0;
-    bool mood_coop = true;  // MOOD passes as one cooperative launch per stage (else per-pass launches)
+    bool mood_coop = true;  // MOOD passes as one persistent launch per stage (else per-pass launches)
     bool mood_fc = true;    // workgroup 0's MOOD passes reuse face fluxes and compute likely next ones ahead
-    // wide launches of long-list passes ahead of the cooperative kernel: launches per stage, grid, shortest list
+    // wide launches of long-list passes ahead of the persistent kernel: launches per stage, grid, shortest list
     int mood_wide = 0, mood_wide_blocks = 0, mood_wide_min = 0;
     bool mood_wide_on = true;  // the last call's first lists were long (or there was no call yet)
     // the persistent MOOD kernel as an ordinary launch with its own rendezvous
     // and grid barrier (else a cooperative launch); its rendezvous polls, and
     // the test switch that sends every pass to one workgroup
-    bool mood_ord = false;
+    bool mood_ord = true;
     int mood_rv_spins = 1 << 14;
     int mood_bar_spins = 1 << 21;  // about a second of polls; a pass takes milliseconds at most
     bool mood_rv_alone = false;
@@ -431,8 +431,8 @@ int enqueue_ring_fill(gpm_ctx *c)
 }
 
 // Wide launches of the MOOD passes over long lists (k_mood_wide), ahead of the
-// cooperative kernel of each stage: as many workgroups as the GPU holds at
-// once, for lists longer than one trip of the cooperative grid.
+// persistent kernel of each stage: as many workgroups as the GPU holds at
+// once, for lists longer than one trip of the persistent grid.
 // GPM_MOOD_WIDE sets the launches per stage (0 switches them off),
 // GPM_MOOD_WIDE_MIN the shortest list they take.
 void setup_mood_wide(gpm_ctx *c, int ncu, int flux)
@@ -455,8 +455,8 @@ void setup_mood_wide(gpm_ctx *c, int ncu, int flux)
     if (const char *env = std::getenv("GPM_MOOD_WIDE_MIN")) c->mood_wide_min = std::atoi(env) > 0 ? std::atoi(env) : 0;
 }
 
-// GPM_MOOD_ORD=1: the persistent MOOD kernel as an ordinary launch (0: a
-// cooperative launch); GPM_MOOD_RV_SPINS: its rendezvous polls;
+// GPM_MOOD_ORD=0: the persistent MOOD kernel as a cooperative launch (1, the
+// default: an ordinary launch); GPM_MOOD_RV_SPINS: its rendezvous polls;
 // GPM_MOOD_RV_ALONE=1: one workgroup runs every stage's passes (tests);
 // GPM_MOOD_BAR_SPINS: the polls of its grid-barrier waits (tests of a stall).
 void mood_ord_env(gpm_ctx *c)
@@ -502,10 +502,10 @@ int mood_wide_decide(gpm_ctx *c)
 
 // The MOOD correction passes of one stage.  Persistent (the default): one
 // launch of k_mood runs every pass, with grid barriers between passes; it is
-// a cooperative launch, whose barrier is the hardware's, or an ordinary
-// launch whose workgroups meet at a rendezvous first (when the device refuses
-// a cooperative launch, e.g. while another process holds that resource, or
-// when GPM_MOOD_ORD=1 asks for it).  Otherwise (when asked for) per-pass
+// an ordinary launch whose workgroups meet at a rendezvous first (the
+// default), or a cooperative launch, whose barrier is the hardware's
+// (GPM_MOOD_ORD=0; when the device refuses it, e.g. while another process
+// holds that resource, the ordinary launch takes over).  Otherwise (when asked for) per-pass
 // launches in growing batches, with a look at the stage's pending flag
 // between batches: a launch runs the current pass or one run of workgroup-0
 // passes and completes at least one pass, and launches past the last pass
@@ -1726,7 +1726,7 @@ int gpm_run_noalloc(const gpm_config *cfg, const double *zT_sp, const double *gp
     }
     c.mood_solo = kSoloMax;
     // no earlier call to choose the wide launches' plan from: the
-    // cooperative kernel alone, as for short lists
+    // persistent kernel alone, as for short lists
     c.mood_wide = 0;
     if (const char *env = std::getenv("GPM_MOOD_COOP")) c.mood_coop = std::atoi(env) != 0;
     mood_ord_env(&c);

@@ -63,17 +63,20 @@ enum {
     kCntSecond = 8,   // cells recomputed in the correction passes
     kCntPending = 9,  // per-pass launches: the current pass has cells to recompute
     kCntStop = 10,    // the first pass of workgroup 0's last run of passes, once that run has ended
-    kCntArrive = 11,  // ordinary persistent launch: workgroups arrived (| kArriveAbort: one runs alone)
-    kCntBar = 12,     // ordinary persistent launch: grid-barrier arrivals, gridDim.x per barrier
-    kCountersPerStage = 16
+    // ordinary persistent launch, each on a cache line of its own (the
+    // workgroups poll them while the passes update the counters above):
+    kCntArrive = 32,  // workgroups arrived (| kArriveAbort: one runs alone)
+    kCntBar = 64,     // grid-barrier arrivals, gridDim.x per barrier
+    kCntRel = 96,     // grid barriers released (the last arrival publishes each)
+    kCountersPerStage = 128
 };
 constexpr int kArriveAbort = 1 << 30;
-constexpr int kBarStalled = 1 << 29;  // kCntBar from here on: a grid-barrier wait stalled, the launch ends
+constexpr int kBarStalled = 1 << 29;  // kCntRel from here on: a grid-barrier wait stalled, the launch ends
 
 // How k_mood's workgroups wait for each other between passes.
 enum {
     kMoodCoop = 0,  // cooperative launch: cooperative-groups grid barrier
-    kMoodOrd = 1,   // ordinary launch: rendezvous, then a grid barrier on kCntBar
+    kMoodOrd = 1,   // ordinary launch: rendezvous, then grid barriers on kCntBar / kCntRel
 };
 
 struct DevStats {

@@ -9,7 +9,7 @@
 //            output combined by the integrator and appends troubled cells to
 //            a work list (wavefront ballot + one atomic per wavefront).  With
 //            Mord 1 it is the first-order Godunov method (FOG), one pass.
-//   k_mood   the remaining MOOD passes, as one cooperative persistent kernel:
+//   k_mood   the remaining MOOD passes, as one persistent kernel:
 //            each pass lowers the order of the listed cells, marks their
 //            5-point neighbourhoods (deduplicated with a pass stamp), and
 //            recomputes exactly those cells from scratch with the current
@@ -518,7 +518,7 @@ __device__ __forceinline__ void wave_lds_sync()
 }
 
 // One MOOD correction pass (mod_FE.f90:69-174) over a list of cells, shared
-// by the cooperative kernel and the per-pass launches.  Pass p recomputes the
+// by the persistent kernel and the per-pass launches.  Pass p recomputes the
 // cells of its list from scratch with the orders they had when the pass
 // began; a cell that fails takes the cascade at once (version p) and flags its
 // neighbourhood for pass p+1.  Each cell is handled by G lanes, one per (face,
@@ -576,7 +576,7 @@ struct MoodPass {
         int hash[3][kSoloHash];
         int cnt[3];   // list q has cnt[q % 3] cells
         int fail[3];  // failures of pass q in fail[q % 3]
-        // the other workgroups' precompute (cooperative kernel): workgroup
+        // the other workgroups' precompute (persistent kernel): workgroup
         // 0's pass and list as last read, cells taken this round
         int hseq, hm, hn, hstop;
         int hlist[16];
@@ -745,7 +745,7 @@ struct MoodPass {
     // every thread of the workgroup once per trip; b is the parity of the
     // workgroup's trip count since the launch (init_lds clears both buffers'
     // counts; trip t clears those of trip t + 1 between its two barriers, and
-    // trip t + 1 does not write the buffer trip t copies out).  The cooperative
+    // trip t + 1 does not write the buffer trip t copies out).  The persistent
     // kernel's short, latency-bound passes keep one atomic per wavefront: the
     // two barriers cost them about what the atomics would.
     __device__ __forceinline__ void wide_append(int b, bool decg, unsigned long long fm, int i, int j,
@@ -1040,7 +1040,7 @@ struct MoodPass {
         }
     }
 
-    // The other workgroups of the cooperative kernel while workgroup 0 runs
+    // The other workgroups of the persistent kernel while workgroup 0 runs
     // the passes from p alone: they fill the device-wide cache ahead of it.
     // Workgroup 0 publishes each pass's list (the first 16 cells); a failure
     // chain moves by one cell per pass, so the helpers take the cells within
@@ -1065,7 +1065,7 @@ struct MoodPass {
                 // workgroup 0 ended this run of passes, or (ordinary launch) a
                 // grid barrier stalled and workgroup 0 may have left
                 L.hstop = load_relaxed(&a.cnt[kCntStop]) == p ||
-                          (a.sync == kMoodOrd && load_relaxed(&a.cnt[kCntBar]) >= kBarStalled);
+                          (a.sync == kMoodOrd && load_relaxed(&a.cnt[kCntRel]) >= kBarStalled);
                 L.hseq = __hip_atomic_load(&a.gfc.beacon[0], __ATOMIC_RELAXED, AG);
                 const int m = __hip_atomic_load(&a.gfc.beacon[1], __ATOMIC_RELAXED, AG);
                 L.hm = m < 0 ? 0 : m > 16 ? 16 : m;
@@ -1283,19 +1283,18 @@ __device__ __forceinline__ void mood_finish(const MoodArgs &a, int p)
 // one that has not started.  When the others have not all arrived after
 // rv_spins polls (the device is shared with other work), this workgroup
 // sets kArriveAbort and returns 2: it takes every pass alone; the others
-// return 1 and leave.  Arrivals and the abort are compare-and-swaps on one
-// word, and an arrival is refused once the abort is set, so all workgroups
-// agree: the count reaches gridDim.x only without an abort.
+// return 1 and leave.  Arrivals are atomic adds on one word (no retries when
+// all workgroups arrive at once); the abort is a compare-and-swap that sets
+// the bit only while the count is below gridDim.x, and every reader checks
+// the bit first, so all workgroups agree: either all see gridDim.x arrivals
+// without the bit, or all but the one that set it see the bit.
 __device__ __forceinline__ int mood_rendezvous(const MoodArgs &a)
 {
     constexpr int AG = __HIP_MEMORY_SCOPE_AGENT;
     int *w = &a.cnt[kCntArrive];
     const int n = (int)gridDim.x;
-    int v = load_relaxed(w);
-    for (;;) {
-        if (v & kArriveAbort) return 1;
-        if (__hip_atomic_compare_exchange_strong(w, &v, v + 1, __ATOMIC_RELAXED, __ATOMIC_RELAXED, AG)) break;
-    }
+    int v = __hip_atomic_fetch_add(w, 1, __ATOMIC_RELAXED, AG);
+    if (v & kArriveAbort) return 1;
     if (a.rv_alone) return v == 0 ? 2 : 1;  // the first to arrive takes the passes alone
     for (int s = 0;; ++s) {
         v = load_relaxed(w);
@@ -1311,34 +1310,42 @@ __device__ __forceinline__ int mood_rendezvous(const MoodArgs &a)
 }
 
 // Grid barrier of an ordinary persistent launch after its rendezvous: each
-// workgroup adds one arrival to kCntBar and waits for gridDim.x arrivals per
-// barrier so far; agent-scope fences on every lane publish the workgroup's
+// workgroup adds one arrival to kCntBar (gridDim.x per barrier); the last to
+// arrive publishes the barrier's number in kCntRel, on a cache line of its
+// own, which the others poll (so that their polls do not queue behind the
+// arrivals).  Agent-scope fences on every lane publish the workgroup's
 // writes before it and make the others' visible after it.  A wait that
-// reaches its bound (a.bar_spins polls) counts a stall and lifts the count to
-// kBarStalled, past every later target; a workgroup that finds the count
-// there after its wait returns false and leaves the kernel without starting
-// another pass (the helpers of workgroup 0's passes stop on it too), so no
-// wait blocks on a workgroup that has left, every pass a workgroup starts
-// follows a completed barrier, and the host reports the stall.  ok is a
-// workgroup-shared flag.
+// reaches its bound (a.bar_spins polls) counts a stall and lifts kCntRel to
+// kBarStalled (an atomic maximum, which no later release lowers); a
+// workgroup that finds it there returns false and leaves the kernel without
+// starting another pass (the helpers of workgroup 0's passes stop on it too),
+// so no wait blocks on a workgroup that has left, every pass a workgroup
+// starts follows a completed barrier, and the host reports the stall.  ok is
+// a workgroup-shared flag.
 __device__ __forceinline__ bool ord_grid_sync(const MoodArgs &a, int &nbar, int *ok)
 {
     constexpr int AG = __HIP_MEMORY_SCOPE_AGENT;
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
     __syncthreads();
     if (threadIdx.x == 0) {
-        int *b = &a.cnt[kCntBar];
-        __hip_atomic_fetch_add(b, 1, __ATOMIC_RELAXED, AG);
-        const int target = (nbar + 1) * (int)gridDim.x;
+        int *rel = &a.cnt[kCntRel];
+        const int gen = nbar + 1;
+        const int old = __hip_atomic_fetch_add(&a.cnt[kCntBar], 1, __ATOMIC_RELAXED, AG);
         int v;
-        for (int s = 0; (v = load_relaxed(b)) < target; ++s) {
-            if (s >= a.bar_spins) {
-                atomicAdd(&a.stats->mood_stall, 1);
-                __hip_atomic_fetch_max(b, kBarStalled, __ATOMIC_RELAXED, AG);
-                v = kBarStalled;
-                break;
+        if (old == gen * (int)gridDim.x - 1) {
+            // the last arrival: every workgroup's writes are published
+            v = __hip_atomic_fetch_max(rel, gen, __ATOMIC_RELEASE, AG);
+            if (v < kBarStalled) v = gen;
+        } else {
+            for (int s = 0; (v = load_relaxed(rel)) < gen; ++s) {
+                if (s >= a.bar_spins) {
+                    atomicAdd(&a.stats->mood_stall, 1);
+                    __hip_atomic_fetch_max(rel, kBarStalled, __ATOMIC_RELAXED, AG);
+                    v = kBarStalled;
+                    break;
+                }
+                __builtin_amdgcn_s_sleep(1);
             }
-            __builtin_amdgcn_s_sleep(1);
         }
         *ok = v < kBarStalled;
     }
@@ -1359,10 +1366,11 @@ __device__ __forceinline__ void alone_sync()
 // MOOD correction passes of one RK stage as a persistent kernel with one
 // grid barrier per pass, from the first pass the wide launches before it
 // left (kCntPass + 1; pass 1 when none ran); the loop ends on the device when
-// a pass finds no cell to recompute.  a.sync says how it was launched:
-// cooperatively (cooperative-groups barrier), or as an ordinary launch that
-// first meets at mood_rendezvous and then uses ord_grid_sync; an ordinary
-// launch avoids the idle time the runtime puts around every cooperative one.
+// a pass finds no cell to recompute.  a.sync says how it was launched: as an
+// ordinary launch that first meets at mood_rendezvous and then uses
+// ord_grid_sync (the default: it avoids the idle time the runtime puts
+// around every cooperative launch, and the one-process resource those
+// need), or cooperatively (cooperative-groups barrier).
 template <int MORD, int NGP, int FLUX, bool GEN>
 __global__ __launch_bounds__(kMoodThreads) void k_mood(MoodArgs a)
 {
@@ -1418,7 +1426,7 @@ __global__ __launch_bounds__(kMoodThreads) void k_mood(MoodArgs a)
     if (gtid == 0) mood_finish(a, p);
 }
 
-// The same passes without a cooperative launch: each launch runs the current
+// The same passes without a persistent launch: each launch runs the current
 // pass (all workgroups) or one run of workgroup-0 passes, and its last
 // workgroup to finish publishes the next pass.  A launch whose pass has no
 // cell returns at once, so the host launches in batches and checks
@@ -1463,7 +1471,7 @@ __global__ __launch_bounds__(kMoodThreads) void k_mood_pass(MoodArgs a)
 }
 
 // A grid-wide pass over a long list in an ordinary launch of many small
-// workgroups.  The cooperative kernel's grid is kept small, because its
+// workgroups.  The persistent kernel's grid is kept small, because its
 // per-pass grid barrier costs more with every workgroup, so it covers a list
 // of thousands of cells in many trips, one after another.  Launched before
 // it: the launch runs the current pass (kCntPass + 1) only when that pass

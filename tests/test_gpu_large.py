@@ -64,9 +64,10 @@ def test_8192_four_quadrant_r2_long_chains(gpm):
     """8192^2 GP-R2 four-quadrant problem (outflow): MOOD failure chains run
     thousands of passes in a stage, past any fixed pass budget, and the first
     correction lists hold hundreds of thousands of cells.  Three steps with
-    the cooperative MOOD kernel (its first long-list passes in wide launches,
-    the default), with the cooperative kernel alone (GPM_MOOD_WIDE=0) and
-    with per-pass launches give the same bits; the state stays finite with
+    the persistent MOOD kernel as an ordinary launch (its first long-list
+    passes in wide launches: the default), as a cooperative launch
+    (GPM_MOOD_ORD=0), without the wide launches (GPM_MOOD_WIDE=0) and with
+    per-pass launches give the same bits; the state stays finite with
     positive density and pressure; the largest pass count of a stage is
     reported."""
     if not has_gpu():
@@ -74,19 +75,23 @@ def test_8192_four_quadrant_r2_long_chains(gpm):
     import torch
 
     n, steps = 8192, 3
-    print(f"[case 8192^2 GP-R2 four-quadrant, {steps} steps, both MOOD launch forms]", flush=True)
+    print(f"[case 8192^2 GP-R2 four-quadrant, {steps} steps, four MOOD launch forms]", flush=True)
     U0 = torch.from_numpy(gpm.ic.make("four_quadrant", n, 0)).cuda()
     out = {}
-    for name, coop, wide in (("cooperative", True, "2"), ("cooperative alone", True, "0"), ("per-pass", False, "2")):
-        old = os.environ.get("GPM_MOOD_WIDE")
+    forms = (("ordinary", True, "2", "1"), ("cooperative", True, "2", "0"), ("persistent alone", True, "0", "1"),
+             ("per-pass", False, "2", "1"))
+    for name, coop, wide, ordl in forms:
+        saved = {k: os.environ.get(k) for k in ("GPM_MOOD_WIDE", "GPM_MOOD_ORD")}
         os.environ["GPM_MOOD_WIDE"] = wide
+        os.environ["GPM_MOOD_ORD"] = ordl
         try:
             s = gpm.Solver(gpm.Config(lf=n, Mord=5, ngp=3, BC_type="neumann"))
         finally:
-            if old is None:
-                del os.environ["GPM_MOOD_WIDE"]
-            else:
-                os.environ["GPM_MOOD_WIDE"] = old
+            for k, v in saved.items():
+                if v is None:
+                    del os.environ[k]
+                else:
+                    os.environ[k] = v
         s.set_mood_launch(coop)
         torch.cuda.synchronize()
         s.set_state_device(U0.data_ptr())
@@ -100,8 +105,8 @@ def test_8192_four_quadrant_r2_long_chains(gpm):
         assert s.mood_launch() == coop
         out[name] = (dts, U, st)
         s.close()
-    d1, U1, s1 = out["cooperative"]
-    for name in ("cooperative alone", "per-pass"):
+    d1, U1, s1 = out["ordinary"]
+    for name in ("cooperative", "persistent alone", "per-pass"):
         d2, U2, s2 = out[name]
         assert np.array_equal(d1, d2), name
         assert bool(torch.equal(U1, U2)), name

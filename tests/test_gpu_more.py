@@ -246,7 +246,7 @@ def test_config1_2048_500_steps_vs_oracle(gpm, cascade):
 def test_face_flux_cache_changes_no_bit(gpm, mord, ngp, cascade, bc):
     """Workgroup 0's MOOD passes reuse cached face fluxes, compute the likely
     next ones ahead, and read the device-wide cache that the other
-    workgroups of the cooperative kernel fill meanwhile (switched off at
+    workgroups of the persistent kernel fill meanwhile (switched off at
     gpm_create by GPM_MOOD_FC=0 and GPM_MOOD_HELP=0).  1024^2 four-quadrant
     problems, whose failure chains run long runs of small passes on
     workgroup 0, give the same dt sequence, counts, order mask and state in

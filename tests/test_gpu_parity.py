@@ -120,8 +120,8 @@ def test_run_per_pass_launches(gpm, case):
 def test_run_wide_launches(gpm, case, monkeypatch):
     """Every stage's first two MOOD correction passes through the wide
     launches (GPM_MOOD_WIDE_MIN=1: any list of more than one cell; by default
-    only lists longer than one trip of the cooperative grid go there), the
-    rest in the cooperative kernel: the same dt sequence, counts, mask and
+    only lists longer than one trip of the persistent grid go there), the
+    rest in the persistent kernel: the same dt sequence, counts, mask and
     state as the reference run."""
     z = golden(f"step_{case}.npz")
     nsteps = int(z["nsteps"])
@@ -141,22 +141,22 @@ def test_run_wide_launches(gpm, case, monkeypatch):
     assert st["detected_total"] == int(np.sum(z["counts"]))
 
 
-@pytest.mark.parametrize("alone", [False, True], ids=["grid", "alone"])
+@pytest.mark.parametrize("form", ["cooperative", "ordinary", "alone"])
 @pytest.mark.parametrize("case", STEP_CASES)
-def test_run_ordinary_persistent_launch(gpm, case, alone, monkeypatch, capfd):
-    """The persistent MOOD kernel as an ordinary launch (GPM_MOOD_ORD=1, the
-    path taken when the device refuses a cooperative launch): its workgroups
-    meet at a rendezvous and then use grid barriers of their own.  With
-    GPM_MOOD_RV_ALONE=1 the first workgroup to arrive runs every pass alone
-    and the others leave (the path taken when they are not all resident
-    together).  The same dt sequence, counts, mask and state as the
+def test_run_persistent_launch_forms(gpm, case, form, monkeypatch, capfd):
+    """The persistent MOOD kernel in each of its launch forms: an ordinary
+    launch whose workgroups meet at a rendezvous and then use grid barriers
+    of their own (the default, GPM_MOOD_ORD=1); a cooperative launch with the
+    hardware's grid barrier (GPM_MOOD_ORD=0); and the ordinary launch with
+    GPM_MOOD_RV_ALONE=1, where the first workgroup to arrive runs every pass
+    alone and the others leave (the path taken when they are not all
+    resident together).  The same dt sequence, counts, mask and state as the
     reference run."""
     z = golden(f"step_{case}.npz")
     nsteps = int(z["nsteps"])
-    print(f"[case {case}] {nsteps} steps via gpm_run, ordinary persistent MOOD launch"
-          f"{', one workgroup' if alone else ''}", flush=True)
-    monkeypatch.setenv("GPM_MOOD_ORD", "1")
-    monkeypatch.setenv("GPM_MOOD_RV_ALONE", "1" if alone else "0")
+    print(f"[case {case}] {nsteps} steps via gpm_run, {form} persistent MOOD launch", flush=True)
+    monkeypatch.setenv("GPM_MOOD_ORD", "0" if form == "cooperative" else "1")
+    monkeypatch.setenv("GPM_MOOD_RV_ALONE", "1" if form == "alone" else "0")
     s = make_solver(gpm, z)
     s.set_state(z["U0"])
     s.set_time(0.0)
@@ -170,7 +170,7 @@ def test_run_ordinary_persistent_launch(gpm, case, alone, monkeypatch, capfd):
     # the persistent kernel ran (every stage with detection launches it) and,
     # when asked, on one workgroup
     said = "ran on one workgroup" in capfd.readouterr().err
-    assert said == (alone and st["max_passes"] > 0)
+    assert said == (form == "alone" and st["max_passes"] > 0)
 
 
 def test_ordinary_launch_stall_is_reported(gpm, monkeypatch):
