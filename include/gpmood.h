@@ -247,9 +247,9 @@ void *gpm_stream(gpm_ctx *ctx);
  * pending flag read back between batches (one host synchronisation per
  * batch); the environment variable GPM_MOOD_COOP=0 selects them at
  * gpm_create.  The results are the same bit for bit.
- * With cooperative launches, a stage's first passes over long lists (more
- * cells than one trip of the cooperative grid) run in two ordinary launches
- * of the whole GPU ahead of the cooperative one, while the context's lists
+ * With persistent launches, a stage's first passes over long lists (more
+ * cells than one trip of the persistent grid) run in two ordinary launches
+ * of the whole GPU ahead of the persistent one, while the context's lists
  * are long: each gpm_run or gpm_time_stepping call sets the next call's
  * plan from the longest first list it saw (GPM_MOOD_WIDE=0 at gpm_create
  * switches them off).  The plan changes no result.
