@@ -62,7 +62,7 @@ struct gpm_ctx {
     // the persistent MOOD kernel as an ordinary launch with its own rendezvous
     // and grid barrier (else a cooperative launch); its rendezvous polls, and
     // the test switch that sends every pass to one workgroup
-    bool mood_ord = true;
+    bool mood_ord = false;
     int mood_rv_spins = 1 << 14;
     int mood_bar_spins = 1 << 21;  // about a second of polls; a pass takes milliseconds at most
     bool mood_rv_alone = false;
@@ -455,8 +455,8 @@ void setup_mood_wide(gpm_ctx *c, int ncu, int flux)
     if (const char *env = std::getenv("GPM_MOOD_WIDE_MIN")) c->mood_wide_min = std::atoi(env) > 0 ? std::atoi(env) : 0;
 }
 
-// GPM_MOOD_ORD=0: the persistent MOOD kernel as a cooperative launch (1, the
-// default: an ordinary launch); GPM_MOOD_RV_SPINS: its rendezvous polls;
+// GPM_MOOD_ORD=1: the persistent MOOD kernel as an ordinary launch (0, the
+// default: a cooperative launch); GPM_MOOD_RV_SPINS: its rendezvous polls;
 // GPM_MOOD_RV_ALONE=1: one workgroup runs every stage's passes (tests);
 // GPM_MOOD_BAR_SPINS: the polls of its grid-barrier waits (tests of a stall).
 void mood_ord_env(gpm_ctx *c)
@@ -502,10 +502,10 @@ int mood_wide_decide(gpm_ctx *c)
 
 // The MOOD correction passes of one stage.  Persistent (the default): one
 // launch of k_mood runs every pass, with grid barriers between passes; it is
-// an ordinary launch whose workgroups meet at a rendezvous first (the
-// default), or a cooperative launch, whose barrier is the hardware's
-// (GPM_MOOD_ORD=0; when the device refuses it, e.g. while another process
-// holds that resource, the ordinary launch takes over).  Otherwise (when asked for) per-pass
+// a cooperative launch, whose barrier is the hardware's (the default), or an
+// ordinary launch whose workgroups meet at a rendezvous first (GPM_MOOD_ORD=1,
+// and when the device refuses a cooperative launch, e.g. while another
+// process holds that resource).  Otherwise (when asked for) per-pass
 // launches in growing batches, with a look at the stage's pending flag
 // between batches: a launch runs the current pass or one run of workgroup-0
 // passes and completes at least one pass, and launches past the last pass

@@ -1381,11 +1381,11 @@ __device__ __forceinline__ void alone_sync()
 // MOOD correction passes of one RK stage as a persistent kernel with one
 // grid barrier per pass, from the first pass the wide launches before it
 // left (kCntPass + 1; pass 1 when none ran); the loop ends on the device when
-// a pass finds no cell to recompute.  a.sync says how it was launched: as an
-// ordinary launch that first meets at mood_rendezvous and then uses
-// ord_grid_sync (the default: it avoids the idle time the runtime puts
-// around every cooperative launch, and the one-process resource those
-// need), or cooperatively (cooperative-groups barrier).
+// a pass finds no cell to recompute.  a.sync says how it was launched:
+// cooperatively (cooperative-groups barrier, the default), or as an ordinary
+// launch that first meets at mood_rendezvous and then uses ord_grid_sync (it
+// avoids the idle time the runtime puts around every cooperative launch, and
+// the one-process resource those need).
 template <int MORD, int NGP, int FLUX, bool GEN>
 __global__ __launch_bounds__(kMoodThreads) void k_mood(MoodArgs a)
 {

@@ -235,14 +235,13 @@ void *gpm_stream(gpm_ctx *ctx);
 /* How the MOOD correction passes of a stage (the MOOD loop of mod_FE.f90:69-174
  * after its first pass) are launched.  cooperative = 1 (the default): one
  * persistent launch per stage runs every pass on the device, with grid
- * barriers between passes.  By default it is an ordinary launch whose
- * workgroups first meet at a bounded rendezvous (if they are not all
- * resident together, one workgroup runs every pass alone) and then use grid
- * barriers of their own; a barrier wait that reaches its bound (about a
- * second) fails the call with GPM_EHIP.  GPM_MOOD_ORD=0 at gpm_create makes
- * it a cooperative launch (the hardware's grid barrier; when the device
- * refuses one, because one process at a time holds that resource, the
- * library switches to the ordinary launch by itself).
+ * barriers between passes.  By default it is a cooperative launch (the
+ * hardware's grid barrier).  When the device refuses one (one process at a
+ * time holds that resource), or with GPM_MOOD_ORD=1 at gpm_create, it is an
+ * ordinary launch whose workgroups first meet at a bounded rendezvous (if
+ * they are not all resident together, one workgroup runs every pass alone)
+ * and then use grid barriers of their own; a barrier wait that reaches its
+ * bound (about a second) fails the call with GPM_EHIP.
  * cooperative = 0: per-pass launches in growing batches, with the stage's
  * pending flag read back between batches (one host synchronisation per
  * batch); the environment variable GPM_MOOD_COOP=0 selects them at

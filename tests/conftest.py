@@ -18,11 +18,11 @@ def pytest_configure(config):
 def pytest_collection_modifyitems(session, config, items):
     """Run the Fortran drop-in cases first.  They start the drop-in program
     as a child process.  The GPU grants the resource of cooperative launches
-    to one process at a time, and a test process that has made one (the
-    tests of GPM_MOOD_ORD=0) keeps it until it exits; a child asked for
-    cooperative launches would then fall back to an ordinary launch (same
-    results, covered by test_run_persistent_launch_forms).  Running the
-    drop-ins first keeps them on the path that a drop-in run alone takes."""
+    to one process at a time, and a test process that has made one keeps it
+    until it exits; the child's MOOD kernel would then fall back to an
+    ordinary launch (same results, covered by
+    test_run_persistent_launch_forms).  Running the drop-ins first keeps them
+    on the cooperative path that a drop-in run alone takes."""
     items.sort(key=lambda it: 0 if "test_gpu_dropin" in it.nodeid else 1)
 
 

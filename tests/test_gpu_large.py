@@ -64,9 +64,9 @@ def test_8192_four_quadrant_r2_long_chains(gpm):
     """8192^2 GP-R2 four-quadrant problem (outflow): MOOD failure chains run
     thousands of passes in a stage, past any fixed pass budget, and the first
     correction lists hold hundreds of thousands of cells.  Three steps with
-    the persistent MOOD kernel as an ordinary launch (its first long-list
-    passes in wide launches: the default), as a cooperative launch
-    (GPM_MOOD_ORD=0), without the wide launches (GPM_MOOD_WIDE=0) and with
+    the persistent MOOD kernel as an ordinary launch (GPM_MOOD_ORD=1, its
+    first long-list passes in wide launches), as a cooperative launch (the
+    default), without the wide launches (GPM_MOOD_WIDE=0) and with
     per-pass launches give the same bits; the state stays finite with
     positive density and pressure; the largest pass count of a stage is
     reported."""

@@ -144,10 +144,11 @@ def test_run_wide_launches(gpm, case, monkeypatch):
 @pytest.mark.parametrize("form", ["cooperative", "ordinary", "alone"])
 @pytest.mark.parametrize("case", STEP_CASES)
 def test_run_persistent_launch_forms(gpm, case, form, monkeypatch, capfd):
-    """The persistent MOOD kernel in each of its launch forms: an ordinary
-    launch whose workgroups meet at a rendezvous and then use grid barriers
-    of their own (the default, GPM_MOOD_ORD=1); a cooperative launch with the
-    hardware's grid barrier (GPM_MOOD_ORD=0); and the ordinary launch with
+    """The persistent MOOD kernel in each of its launch forms: a cooperative
+    launch with the hardware's grid barrier (the default, GPM_MOOD_ORD=0); an
+    ordinary launch whose workgroups meet at a rendezvous and then use grid
+    barriers of their own (GPM_MOOD_ORD=1, the form taken when the device
+    refuses a cooperative launch); and the ordinary launch with
     GPM_MOOD_RV_ALONE=1, where the first workgroup to arrive runs every pass
     alone and the others leave (the path taken when they are not all
     resident together).  The same dt sequence, counts, mask and state as the
