@@ -1313,14 +1313,13 @@ __device__ __forceinline__ int mood_rendezvous(const MoodArgs &a)
 // workgroup adds one arrival to kCntBar (gridDim.x per barrier); the last to
 // arrive publishes the barrier's number in kCntRel, on a cache line of its
 // own, which the others poll (so that their polls do not queue behind the
-// arrivals).  Every wavefront first waits until its own stores and atomics
-// have reached L2 (s_waitcnt: a workgroup-scope release emits no wait on
-// this target); then one thread writes the XCD's L2 back (agent-scope
-// release) for the whole workgroup before arriving, and after the release
-// invalidates the CU's caches (agent-scope acquire) and waits for that
-// before the workgroup reads on.  One wavefront's cache operations instead
-// of every wavefront's: MOOD 0.165 -> 0.139 ms per stage on the 2048^2
-// blast, with the same bits.  A wait that
+// arrivals).  Every wavefront makes its own agent-scope release (the XCD's
+// L2 written back after its stores) before the arrival and its own acquire
+// (its caches invalidated) after the release: with the write-back made by
+// one thread for the workgroup, after every wavefront's s_waitcnt, a 2048^2
+// run with the workgroup-0 face-flux cache off read stale counters and
+// stalled, so a wavefront's completed stores are not taken to be covered by
+// another wavefront's write-back.  A wait that
 // reaches its bound (a.bar_spins polls) counts a stall and lifts kCntRel to
 // kBarStalled (an atomic maximum, which no later release lowers); a
 // workgroup that finds it there returns false and leaves the kernel without
@@ -1331,10 +1330,9 @@ __device__ __forceinline__ int mood_rendezvous(const MoodArgs &a)
 __device__ __forceinline__ bool ord_grid_sync(const MoodArgs &a, int &nbar, int *ok)
 {
     constexpr int AG = __HIP_MEMORY_SCOPE_AGENT;
-    __builtin_amdgcn_s_waitcnt(0);  // this wavefront's stores have reached L2
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
     __syncthreads();
     if (threadIdx.x == 0) {
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");  // the L2 written back, for the workgroup
         int *rel = &a.cnt[kCntRel];
         const int gen = nbar + 1;
         const int old = __hip_atomic_fetch_add(&a.cnt[kCntBar], 1, __ATOMIC_RELAXED, AG);
@@ -1355,11 +1353,10 @@ __device__ __forceinline__ bool ord_grid_sync(const MoodArgs &a, int &nbar, int 
             }
         }
         *ok = v < kBarStalled;
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");  // the CU's caches invalidated, for the workgroup
-        __builtin_amdgcn_s_waitcnt(0);                      // before any wavefront reads
     }
     ++nbar;
     __syncthreads();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
     return *ok != 0;
 }
 
@@ -1368,14 +1365,9 @@ __device__ __forceinline__ bool ord_grid_sync(const MoodArgs &a, int &nbar, int 
 // wavefronts wrote, through the CU's caches).
 __device__ __forceinline__ void alone_sync()
 {
-    __builtin_amdgcn_s_waitcnt(0);
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
     __syncthreads();
-    if (threadIdx.x == 0) {
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-        __builtin_amdgcn_s_waitcnt(0);
-    }
-    __syncthreads();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
 }
 
 // MOOD correction passes of one RK stage as a persistent kernel with one
